@@ -1,0 +1,166 @@
+/* gsj.h — C ABI of the MI355X spatial join engine (libgsj.so).
+ *
+ * This is the boundary a host program binds to (the gpujoind daemon in this repository, or any FFI). Every
+ * entry point replaces one engine entry point of the reference (landstats-us/postgis-cuda), whose engine is a
+ * set of C++ templates in engine/engine.cuh and engine/buffer.cuh; the reference file:line is given per function.
+ * All geometry is FP64. Plain pointers and sizes only; no device pointers cross this boundary (callbacks receive
+ * host buffers that stay valid until the callback returns).
+ *
+ * Errors: every int-returning function returns GSJ_OK (0) or a negative GSJ_E_* code and sets a thread-local
+ * message readable with gsj_last_error(). Nothing in the library calls exit() (the reference exits on any CUDA
+ * error or overflow, engine/common.h:13-14, engine/engine.cuh:43, 927).
+ *
+ * Threading: one context owns one HIP stream on one device; calls on one context are serialised internally (the
+ * reference serialises all GPU work under one mutex, daemon/gpujoind.cu:35). Layers belong to the context that
+ * created them.
+ */
+#ifndef GSJ_H
+#define GSJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define GSJ_OK 0
+#define GSJ_E_ARG -1      /* invalid argument (null, size out of range, wrong layer kind) */
+#define GSJ_E_HIP -2      /* HIP runtime error (message carries hipGetErrorString) */
+#define GSJ_E_NOMEM -3    /* device or host allocation failed */
+#define GSJ_E_LIMIT -4    /* a documented limit was exceeded (geometries >= 2^31, rings >= 2^32, ...) */
+#define GSJ_E_CANCEL -5   /* the cancel flag was raised; partial results were delivered */
+#define GSJ_E_IO -6       /* file could not be read or written, or has a bad header */
+#define GSJ_E_INTERNAL -7
+
+/* message of the last failing call on this thread ("" if none) */
+const char* gsj_last_error(void);
+/* library version string */
+const char* gsj_version(void);
+
+/* ------------------------------------------------------------------ context */
+typedef struct gsj_ctx gsj_ctx;
+typedef struct gsj_layer gsj_layer;
+
+/* Create a context on HIP device `device` (its own stream, default priority). */
+int gsj_ctx_create(int device, gsj_ctx** out);
+void gsj_ctx_destroy(gsj_ctx* ctx);
+/* JSON description of the device and memory (reference op_status, daemon/gpujoind.cu:496-504); the returned
+ * string is owned by the context and valid until the next call on it */
+const char* gsj_ctx_status_json(gsj_ctx* ctx);
+/* Release the context's cached scratch buffers (they are grow-only and reused across calls). */
+int gsj_ctx_trim(gsj_ctx* ctx);
+
+/* ------------------------------------------------------------------ layers
+ * Polygon layer from the flat .gset arrays (engine/gset.h:5-11): gro[n+1] geometry -> first ring, rvo[nr+1] ring
+ * -> first vertex (rings closed, shells CCW, holes CW), xy[2 nv]. Optional parent[n] (piece -> parent geometry,
+ * non-decreasing) and syn[nv] (edge v -> v+1 was created by subdivision) describe a subdivided layer.
+ * Replaces upload_set (engine/engine.cuh:42-89): per-geometry origin, exact origin-relative coordinates,
+ * outward-rounded float boxes, real boundary length per parent and the synthetic-edge bitset are computed on the
+ * device. Limits: n < 2^31, nr < 2^32 (engine/engine.cuh:43). */
+int gsj_layer_create_polys(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr,
+                           uint64_t nv, const uint32_t* parent, const uint8_t* syn, gsj_layer** out);
+/* Point layer (engine/engine.cuh:1068-1076, upload_points): xy[2n]. */
+int gsj_layer_create_points(gsj_ctx* ctx, const double* xy, uint64_t n, gsj_layer** out);
+void gsj_layer_free(gsj_layer* layer);
+/* Build (or rebuild) the layer's LBVH (engine/engine.cuh:197-228, build_bvh): 64-bit Morton keys, on-device
+ * radix sort, Karras hierarchy, atomic bottom-up refit, nodes stored in depth-first order with skip links. */
+int gsj_bvh_build(gsj_layer* layer);
+
+typedef struct {
+  uint64_t n, nr, nv;      /* geometries (pieces), rings, vertices */
+  uint32_t nparent;        /* distinct parent geometries (n if not subdivided) */
+  int is_points;           /* 1 for a point layer */
+  int has_bvh;
+  uint64_t device_bytes;   /* layer + BVH */
+} gsj_layer_info;
+int gsj_layer_info_get(const gsj_layer* layer, gsj_layer_info* info);
+/* Copy the prepared device arrays back (for verification): origin[2n], ext[2n], bb[4n] (float), q[2nv]
+ * (origin-relative xy), blen[nparent]. Any pointer may be NULL. */
+int gsj_layer_export(const gsj_layer* layer, double* origin, double* ext, float* bb, double* q, double* blen);
+
+/* ------------------------------------------------------------------ joins */
+typedef struct {
+  int want_area;           /* intersection area per row (engine/engine.cuh:517 RefineParams::want_area) */
+  int want_relate;         /* DE-9IM code per row (RefineParams::want_relate) */
+  int upper;               /* self-join: each unordered pair once, b >= a (JoinOpts::upper, engine.cuh:810) */
+  int snap_half_ulp;       /* RefineParams::snap_half_ulp */
+  uint32_t batch_q;        /* probe geometries per traversal batch; 0 = default 4M (JoinOpts::batch_q) */
+  uint64_t max_pairs;      /* candidate pairs per refine launch; 0 = default 128M (JoinOpts::max_pairs) */
+  int no_agg;              /* debug: report piece pairs instead of parent pairs (JoinOpts::no_agg) */
+  const volatile int* cancel;   /* polled between batches (JoinOpts::cancel); may be NULL */
+} gsj_join_opts;
+
+typedef struct {
+  double traverse_ms, refine_ms, compact_ms, copy_ms;   /* JoinStats, engine/engine.cuh:814 */
+  uint64_t candidates;     /* candidate pairs (refined pairs) */
+  uint64_t hits;           /* result rows delivered (parent pairs) */
+  uint64_t heavy;          /* pairs whose local edge lists exceeded on-chip capacity */
+  uint64_t batches;
+} gsj_join_stats;
+
+/* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;
+ * code[i] (DE-9IM, 2 bits per cell II IB IE BI BB BE EI EB EE, 0=F 1='0' 2='1' 3='2') or NULL. Buffers are valid
+ * until the callback returns. Return nonzero to cancel. */
+typedef int (*gsj_rows_cb)(void* user, const uint32_t* pairs, const double* area, const uint32_t* code, size_t n);
+
+/* ST_Intersects [+ ST_Area(ST_Intersection)] [+ ST_Relate] of probe layer P against build layer R (R needs a BVH).
+ * Replaces join_stream (engine/engine.cuh:895-1061). Rows are distinct parent pairs. */
+int gsj_join(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, const gsj_join_opts* opts, gsj_rows_cb cb, void* user,
+             gsj_join_stats* stats);
+
+/* Points in polygons, boundary counts (replaces join_points_stream, engine/engine.cuh:1112-1178). Callback rows:
+ * pairs (point index, parent polygon), where[i] = 1 interior, 2 boundary. */
+typedef int (*gsj_points_cb)(void* user, const uint32_t* pairs, const uint8_t* where, size_t n);
+int gsj_points_in_polygons(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, const gsj_join_opts* opts, gsj_points_cb cb,
+                           void* user, gsj_join_stats* stats);
+
+/* ST_ClusterIntersecting over the parents of P (replaces cluster_intersecting, engine/engine.cuh:1216-1234):
+ * root[g] = smallest parent index of g's connected component, for g in [0, nparent). P needs a BVH. */
+int gsj_cluster(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, uint32_t* root, uint64_t nroot, gsj_join_stats* stats);
+
+/* ------------------------------------------------------------------ DE-9IM helpers (engine/engine.cuh:871-888) */
+void gsj_de9im_str(uint32_t code, char out[9]);
+uint32_t gsj_de9im_transpose(uint32_t code);
+/* pattern of 9 chars from T F * 0 1 2 (case-insensitive); returns 1 on match */
+int gsj_de9im_match(uint32_t code, const char* pattern);
+#define GSJ_DE9IM_PT_INTERIOR (1u | 3u << 12 | 2u << 14 | 3u << 16)   /* 0FFFFF212, engine/engine.cuh:337 */
+#define GSJ_DE9IM_PT_BOUNDARY (1u << 2 | 3u << 12 | 2u << 14 | 3u << 16) /* F0FFFF212, engine/engine.cuh:338 */
+
+/* ------------------------------------------------------------------ host-side layer preparation
+ * Flat polygon set owned by the library (the reference HostSet, engine/gset.h:5-11). */
+typedef struct gsj_hostset gsj_hostset;
+void gsj_hostset_free(gsj_hostset* s);
+int gsj_hostset_sizes(const gsj_hostset* s, uint64_t* n, uint64_t* nr, uint64_t* nv);
+/* pointers into the set (valid until gsj_hostset_free); parent/syn are NULL unless the set is subdivided */
+int gsj_hostset_arrays(const gsj_hostset* s, const uint64_t** gro, const uint64_t** rvo, const double** xy, const uint32_t** parent,
+                       const uint8_t** syn);
+
+/* .gset / .pset files (engine/gset.h:13-45; layout scripts/dump_gset.py:5-12, scripts/dump_pset.py:3) */
+int gsj_gset_read(const char* path, gsj_hostset** out);
+int gsj_gset_write(const char* path, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr, uint64_t nv);
+int gsj_pset_read(const char* path, double** xy, uint64_t* n);   /* *xy freed with gsj_free */
+int gsj_pset_write(const char* path, const double* xy, uint64_t n);
+void gsj_free(void* p);
+
+/* ST_Subdivide analogue (engine/subdivide.h:71-129): pieces of at most maxv stored vertices (depth <= 24) */
+int gsj_subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv, gsj_hostset** out);
+/* exact dedup of identical geometries (engine/dedup.h:51-124, first-occurrence order): distinct set plus
+ * member lists moff[un+1], midx[n] (caller-allocated, n entries suffice for both) */
+int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct, uint64_t* moff,
+              uint32_t* midx, uint64_t* n_distinct);
+
+/* Deterministic synthetic layers (counter-based RNG, identical on every run):
+ *   kind "parcels_grid": config-1 parcels (one 4..12-vertex star polygon per cell of an nx x ny grid)
+ *   kind "regions_grid": config-1 regions (kx x ky tiles, `edge_verts` vertices per side, shared jittered borders)
+ *   kind "parcels_mesh": config-2 parcels (jittered lattice, lognormal vertex counts, shared / overlapping edges)
+ * `spec` is "key=value;key=value" (see DESIGN notes in gen.cpp for keys); returns the polygon set. */
+int gsj_generate(const char* kind, const char* spec, gsj_hostset** out);
+/* config-3 points: "n=..;seed=..;x0=..;y0=..;w=..;h=..;clusters=..;sigma_m=..;frac_clustered=.." */
+int gsj_generate_points(const char* spec, double** xy, uint64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSJ_H */

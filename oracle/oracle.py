@@ -1,0 +1,233 @@
+"""ctypes wrapper of the oracle (test infrastructure only): the CPU restatement of the reference engine that the HIP
+engine is checked against. Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE], check=True, capture_output=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    d, u64, u32, i32, vp = C.c_double, C.c_uint64, C.c_uint32, C.c_int, C.c_void_p
+    dp, u64p, u32p, u8p, fp = C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.POINTER(C.c_float)
+    sig = {
+        "orc_orient": (d, [d] * 6), "orc_left_of": (i32, [d] * 6), "orc_seg_isect": (i32, [d] * 8),
+        "orc_f_down": (C.c_float, [d]), "orc_f_up": (C.c_float, [d]), "orc_de9im_code": (u32, [dp, u32, d, d]),
+        "orc_set_threads": (None, [i32]),
+        "orc_layer_polys": (vp, [u64p, u64p, dp, u64, u64, u64, u32p, u8p]), "orc_layer_points": (vp, [dp, u64]),
+        "orc_layer_free": (None, [vp]), "orc_layer_export": (None, [vp, dp, dp, fp, dp, dp]), "orc_layer_nparent": (u32, [vp]),
+        "orc_result_new": (vp, []), "orc_result_free": (None, [vp]), "orc_result_rows": (u64, [vp]), "orc_result_candidates": (u64, [vp]),
+        "orc_result_copy": (None, [vp, u32p, dp, u32p, u8p]), "orc_result_nroot": (u64, [vp]), "orc_result_roots": (None, [vp, u32p]),
+        "orc_candidates": (u64, [vp, vp, i32, u32p, u64]),
+        "orc_refine": (None, [vp, vp, u32p, u64, i32, i32, i32, u8p, dp, dp, u32p]),
+        "orc_join": (None, [vp, vp, i32, i32, i32, i32, vp]),
+        "orc_pip_pair": (C.c_uint8, [vp, vp, u32, u32]), "orc_points_join": (None, [vp, vp, vp]), "orc_cluster": (None, [vp, vp]),
+        "orc_subdivide": (vp, [u64p, u64p, dp, u64, i32]), "orc_flat_sizes": (None, [vp, u64p, u64p, u64p]),
+        "orc_flat_copy": (None, [vp, u64p, u64p, dp, u8p, u32p]), "orc_flat_free": (None, [vp]),
+        "orc_dedup": (vp, [u64p, u64p, dp, u64]), "orc_dedup_n": (u64, [vp]), "orc_dedup_copy": (None, [vp, u32p, u64p, u32p]),
+        "orc_dedup_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    _lib = lib
+    return lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def set_threads(n: int):
+    load().orc_set_threads(int(n))
+
+
+def orient(ax, ay, bx, by, cx, cy):
+    return load().orc_orient(ax, ay, bx, by, cx, cy)
+
+
+def left_of(ax, ay, bx, by, px, py):
+    return bool(load().orc_left_of(ax, ay, bx, by, px, py))
+
+
+def seg_isect(*a):
+    return bool(load().orc_seg_isect(*a))
+
+
+def f_down(v):
+    return load().orc_f_down(v)
+
+
+def f_up(v):
+    return load().orc_f_up(v)
+
+
+def de9im_code(ev, bits, la, lb):
+    e = np.ascontiguousarray(ev, dtype=np.float64)
+    return int(load().orc_de9im_code(_p(e, C.c_double), bits, la, lb))
+
+
+class Layer:
+    def __init__(self, h, n, points=False):
+        self.h, self.n, self.points = h, n, points
+        self._keep = None
+
+    @classmethod
+    def polys(cls, hs):
+        gro = np.ascontiguousarray(hs.gro, np.uint64)
+        rvo = np.ascontiguousarray(hs.rvo, np.uint64)
+        xy = np.ascontiguousarray(hs.xy, np.float64).reshape(-1, 2)
+        par = None if hs.parent is None else np.ascontiguousarray(hs.parent, np.uint32)
+        syn = None if hs.syn is None else np.ascontiguousarray(hs.syn, np.uint8)
+        h = load().orc_layer_polys(_p(gro, C.c_uint64), _p(rvo, C.c_uint64), _p(xy, C.c_double), len(gro) - 1, len(rvo) - 1, len(xy),
+                                   _p(par, C.c_uint32), _p(syn, C.c_uint8))
+        L = cls(h, len(gro) - 1)
+        L.nv = len(xy)
+        return L
+
+    @classmethod
+    def points_layer(cls, xy):
+        xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+        return cls(load().orc_layer_points(_p(xy, C.c_double), len(xy)), len(xy), points=True)
+
+    @property
+    def nparent(self):
+        return int(load().orc_layer_nparent(self.h))
+
+    def export(self):
+        if self.points:
+            bb = np.zeros((self.n, 4), np.float32)
+            load().orc_layer_export(self.h, None, None, _p(bb, C.c_float), None, None)
+            return {"bb": bb}
+        origin, ext, bb = np.zeros((self.n, 2)), np.zeros((self.n, 2)), np.zeros((self.n, 4), np.float32)
+        q, blen = np.zeros((self.nv, 2)), np.zeros(self.nparent)
+        load().orc_layer_export(self.h, _p(origin, C.c_double), _p(ext, C.c_double), _p(bb, C.c_float), _p(q, C.c_double), _p(blen, C.c_double))
+        return {"origin": origin, "ext": ext, "bb": bb, "q": q, "blen": blen}
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().orc_layer_free(self.h)
+        except Exception:
+            pass
+
+
+def candidates(P: Layer, R: Layer, upper=False):
+    lib = load()
+    n = lib.orc_candidates(P.h, R.h, int(upper), None, 0)
+    out = np.zeros((max(n, 1), 2), np.uint32)
+    lib.orc_candidates(P.h, R.h, int(upper), _p(out, C.c_uint32), n)
+    return out[:n]
+
+
+def refine(P: Layer, R: Layer, pairs, want_area=True, want_relate=False, snap_half_ulp=False):
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    n = len(pairs)
+    isect, area, ev, evb = np.zeros(n, np.uint8), np.zeros(n), np.zeros((n, 4)), np.zeros(n, np.uint32)
+    load().orc_refine(P.h, R.h, _p(pairs, C.c_uint32), n, int(want_area), int(want_relate), int(snap_half_ulp), _p(isect, C.c_uint8),
+                      _p(area, C.c_double), _p(ev, C.c_double), _p(evb, C.c_uint32))
+    return isect, area, ev, evb
+
+
+def _result(r, area=False, code=False, where=False):
+    lib = load()
+    n = lib.orc_result_rows(r)
+    pairs = np.zeros((max(n, 1), 2), np.uint32)
+    a = np.zeros(max(n, 1)) if area else None
+    c = np.zeros(max(n, 1), np.uint32) if code else None
+    w = np.zeros(max(n, 1), np.uint8) if where else None
+    lib.orc_result_copy(r, _p(pairs, C.c_uint32), _p(a, C.c_double), _p(c, C.c_uint32), _p(w, C.c_uint8))
+    out = {"pairs": pairs[:n], "candidates": int(lib.orc_result_candidates(r))}
+    if area:
+        out["area"] = a[:n]
+    if code:
+        out["code"] = c[:n]
+    if where:
+        out["where"] = w[:n]
+    return out
+
+
+def join(P: Layer, R: Layer, want_area=True, want_relate=False, upper=False, no_agg=False):
+    lib = load()
+    r = lib.orc_result_new()
+    try:
+        lib.orc_join(P.h, R.h, int(want_area), int(want_relate), int(upper), int(no_agg), r)
+        return _result(r, area=want_area, code=want_relate)
+    finally:
+        lib.orc_result_free(r)
+
+
+def points_join(pts: Layer, R: Layer):
+    lib = load()
+    r = lib.orc_result_new()
+    try:
+        lib.orc_points_join(pts.h, R.h, r)
+        return _result(r, where=True)
+    finally:
+        lib.orc_result_free(r)
+
+
+def pip_pair(pts: Layer, R: Layer, i: int, g: int) -> int:
+    return int(load().orc_pip_pair(pts.h, R.h, i, g))
+
+
+def cluster(P: Layer):
+    lib = load()
+    r = lib.orc_result_new()
+    try:
+        lib.orc_cluster(P.h, r)
+        n = lib.orc_result_nroot(r)
+        root = np.zeros(max(n, 1), np.uint32)
+        lib.orc_result_roots(r, _p(root, C.c_uint32))
+        out = _result(r)
+        out["root"] = root[:n]
+        return out
+    finally:
+        lib.orc_result_free(r)
+
+
+def subdivide(hs, maxv):
+    lib = load()
+    gro, rvo = np.ascontiguousarray(hs.gro, np.uint64), np.ascontiguousarray(hs.rvo, np.uint64)
+    xy = np.ascontiguousarray(hs.xy, np.float64).reshape(-1, 2)
+    h = lib.orc_subdivide(_p(gro, C.c_uint64), _p(rvo, C.c_uint64), _p(xy, C.c_double), len(gro) - 1, int(maxv))
+    try:
+        n, nr, nv = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        lib.orc_flat_sizes(h, C.byref(n), C.byref(nr), C.byref(nv))
+        og, orv = np.zeros(n.value + 1, np.uint64), np.zeros(nr.value + 1, np.uint64)
+        oxy, osyn, opar = np.zeros((max(nv.value, 1), 2)), np.zeros(max(nv.value, 1), np.uint8), np.zeros(max(n.value, 1), np.uint32)
+        lib.orc_flat_copy(h, _p(og, C.c_uint64), _p(orv, C.c_uint64), _p(oxy, C.c_double), _p(osyn, C.c_uint8), _p(opar, C.c_uint32))
+        return og, orv, oxy[: nv.value], opar[: n.value], osyn[: nv.value]
+    finally:
+        lib.orc_flat_free(h)
+
+
+def dedup(hs):
+    lib = load()
+    gro, rvo = np.ascontiguousarray(hs.gro, np.uint64), np.ascontiguousarray(hs.rvo, np.uint64)
+    xy = np.ascontiguousarray(hs.xy, np.float64).reshape(-1, 2)
+    n = len(gro) - 1
+    h = lib.orc_dedup(_p(gro, C.c_uint64), _p(rvo, C.c_uint64), _p(xy, C.c_double), n)
+    try:
+        un = lib.orc_dedup_n(h)
+        first, moff, midx = np.zeros(max(un, 1), np.uint32), np.zeros(un + 1, np.uint64), np.zeros(max(n, 1), np.uint32)
+        lib.orc_dedup_copy(h, _p(first, C.c_uint32), _p(moff, C.c_uint64), _p(midx, C.c_uint32))
+        return first[:un], moff, midx[:n]
+    finally:
+        lib.orc_dedup_free(h)

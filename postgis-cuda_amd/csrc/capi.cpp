@@ -1,0 +1,367 @@
+// extern "C" boundary of libgsj (declared in include/gsj.h). Every entry point converts exceptions into GSJ_E_*
+// codes plus a thread-local message; nothing here exits the process.
+#include "engine.h"
+#include "hostprep.h"
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <sstream>
+
+struct gsj_ctx { gsj::Context c; };
+struct gsj_layer { gsj::Layer L; };
+struct gsj_hostset { gsj::HostSet s; };
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return GSJ_OK;
+  } catch (const gsj::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return GSJ_E_NOMEM;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return GSJ_E_ARG;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return GSJ_E_IO;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return GSJ_E_INTERNAL;
+  }
+}
+
+void set_device(const gsj::Context& c) { GSJ_HIP(hipSetDevice(c.device)); }
+
+}  // namespace
+
+namespace gsj {
+Context::~Context() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+}  // namespace gsj
+
+extern "C" {
+
+const char* gsj_last_error(void) { return g_last_error.c_str(); }
+const char* gsj_version(void) { return "gsj 0.1 (gfx950)"; }
+
+int gsj_ctx_create(int device, gsj_ctx** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(out, GSJ_E_ARG, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    GSJ_HIP(hipGetDeviceCount(&ndev));
+    GSJ_REQUIRE(device >= 0 && device < ndev, GSJ_E_ARG, "no such HIP device: " + std::to_string(device));
+    GSJ_HIP(hipSetDevice(device));
+    auto* ctx = new gsj_ctx();
+    ctx->c.device = device;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) == hipSuccess) ctx->c.cu_count = p.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      throw gsj::Error(GSJ_E_HIP, "hipStreamCreate failed");
+    }
+    *out = ctx;
+  });
+}
+
+void gsj_ctx_destroy(gsj_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  delete ctx;
+}
+
+const char* gsj_ctx_status_json(gsj_ctx* ctx) {
+  if (!ctx) return "{}";
+  std::lock_guard<std::mutex> lk(ctx->c.mu);
+  int rc = guarded([&] {
+    set_device(ctx->c);
+    hipDeviceProp_t p;
+    GSJ_HIP(hipGetDeviceProperties(&p, ctx->c.device));
+    size_t fr = 0, tot = 0;
+    GSJ_HIP(hipMemGetInfo(&fr, &tot));
+    std::ostringstream o;
+    o << "{\"device\":\"" << p.name << "\",\"arch\":\"" << p.gcnArchName << "\",\"cu\":" << p.multiProcessorCount
+      << ",\"mem_total_mb\":" << (double)tot / 1048576.0 << ",\"mem_free_mb\":" << (double)fr / 1048576.0 << "}";
+    ctx->c.status = o.str();
+  });
+  if (rc != GSJ_OK) ctx->c.status = "{}";
+  return ctx->c.status.c_str();
+}
+
+int gsj_ctx_trim(gsj_ctx* ctx) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx, GSJ_E_ARG, "null context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    GSJ_HIP(hipStreamSynchronize(ctx->c.stream));
+    gsj::Context& c = ctx->c;
+    for (gsj::DevBuf* b : {&c.s_counts, &c.s_qoffs, &c.s_offs, &c.s_pairs_a, &c.s_pairs_b, &c.s_flag, &c.s_area, &c.s_ev, &c.s_evb, &c.s_tmp,
+                           &c.s_tmp2, &c.s_cls, &c.s_out_a, &c.s_out_b, &c.s_out_area, &c.s_out_code, &c.s_out_flag, &c.s_keys, &c.s_keys2,
+                           &c.s_vals, &c.s_vals2, &c.s_seg, &c.s_heavy, &c.s_heavy_edges, &c.s_scal, &c.s_qbb, &c.s_uf, &c.s_root, &c.s_misc})
+      b->release();
+  });
+}
+
+int gsj_layer_create_polys(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr, uint64_t nv,
+                           const uint32_t* parent, const uint8_t* syn, gsj_layer** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && out, GSJ_E_ARG, "null context or output pointer");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    auto* l = new gsj_layer();
+    l->L.ctx = &ctx->c;
+    try {
+      gsj::layer_upload_polys(l->L, gro, rvo, xy, n, nr, nv, parent, syn);
+    } catch (...) {
+      delete l;
+      throw;
+    }
+    *out = l;
+  });
+}
+
+int gsj_layer_create_points(gsj_ctx* ctx, const double* xy, uint64_t n, gsj_layer** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && out, GSJ_E_ARG, "null context or output pointer");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    auto* l = new gsj_layer();
+    l->L.ctx = &ctx->c;
+    try {
+      gsj::layer_upload_points(l->L, xy, n);
+    } catch (...) {
+      delete l;
+      throw;
+    }
+    *out = l;
+  });
+}
+
+void gsj_layer_free(gsj_layer* layer) {
+  if (!layer) return;
+  if (layer->L.ctx) {
+    std::lock_guard<std::mutex> lk(layer->L.ctx->mu);
+    (void)hipSetDevice(layer->L.ctx->device);
+    (void)hipStreamSynchronize(layer->L.ctx->stream);
+    delete layer;
+    return;
+  }
+  delete layer;
+}
+
+int gsj_bvh_build(gsj_layer* layer) {
+  return guarded([&] {
+    GSJ_REQUIRE(layer, GSJ_E_ARG, "null layer");
+    std::lock_guard<std::mutex> lk(layer->L.ctx->mu);
+    set_device(*layer->L.ctx);
+    gsj::bvh_build(layer->L);
+  });
+}
+
+int gsj_layer_info_get(const gsj_layer* layer, gsj_layer_info* info) {
+  return guarded([&] {
+    GSJ_REQUIRE(layer && info, GSJ_E_ARG, "null layer or info");
+    const gsj::Layer& L = layer->L;
+    info->n = L.n; info->nr = L.nr; info->nv = L.nv; info->nparent = L.nparent;
+    info->is_points = L.points ? 1 : 0; info->has_bvh = L.has_bvh ? 1 : 0;
+    info->device_bytes = L.mem_bytes + L.bvh_bytes;
+  });
+}
+
+int gsj_layer_export(const gsj_layer* layer, double* origin, double* ext, float* bb, double* q, double* blen) {
+  return guarded([&] {
+    GSJ_REQUIRE(layer, GSJ_E_ARG, "null layer");
+    const gsj::Layer& L = layer->L;
+    std::lock_guard<std::mutex> lk(L.ctx->mu);
+    set_device(*L.ctx);
+    hipStream_t s = L.ctx->stream;
+    if (L.points) {
+      if (bb && L.n) GSJ_HIP(hipMemcpyAsync(bb, L.pts.bb, L.n * 16, hipMemcpyDeviceToHost, s));
+    } else {
+      if (origin && L.n) GSJ_HIP(hipMemcpyAsync(origin, L.poly.origin, L.n * 16, hipMemcpyDeviceToHost, s));
+      if (ext && L.n) GSJ_HIP(hipMemcpyAsync(ext, L.poly.ext, L.n * 16, hipMemcpyDeviceToHost, s));
+      if (bb && L.n) GSJ_HIP(hipMemcpyAsync(bb, L.poly.bb, L.n * 16, hipMemcpyDeviceToHost, s));
+      if (q && L.nv) GSJ_HIP(hipMemcpyAsync(q, L.poly.xy, L.nv * 16, hipMemcpyDeviceToHost, s));
+      if (blen && L.nparent) GSJ_HIP(hipMemcpyAsync(blen, L.poly.blen, (size_t)L.nparent * 8, hipMemcpyDeviceToHost, s));
+    }
+    GSJ_HIP(hipStreamSynchronize(s));
+  });
+}
+
+static gsj_join_opts defaults(const gsj_join_opts* o) {
+  gsj_join_opts d{};
+  if (o) d = *o;
+  return d;
+}
+
+int gsj_join(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, const gsj_join_opts* opts, gsj_rows_cb cb, void* user, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && P && R, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(P->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::join_run(ctx->c, P->L, R->L, defaults(opts), cb, user, stats, nullptr);
+  });
+}
+
+int gsj_points_in_polygons(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, const gsj_join_opts* opts, gsj_points_cb cb, void* user,
+                           gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && pts && R, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(pts->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::points_join_run(ctx->c, pts->L, R->L, defaults(opts), cb, user, stats);
+  });
+}
+
+int gsj_cluster(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, uint32_t* root, uint64_t nroot, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && P, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(P->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::cluster_run(ctx->c, P->L, defaults(opts), root, nroot, stats);
+  });
+}
+
+void gsj_de9im_str(uint32_t code, char out[9]) {
+  for (int k = 0; k < 9; k++) out[k] = "F012"[(code >> (2 * k)) & 3];
+}
+uint32_t gsj_de9im_transpose(uint32_t c) {
+  uint32_t t = 0;
+  for (int r = 0; r < 3; r++)
+    for (int q = 0; q < 3; q++) t |= ((c >> (2 * (3 * r + q))) & 3u) << (2 * (3 * q + r));
+  return t;
+}
+int gsj_de9im_match(uint32_t code, const char* pat) {
+  if (!pat) return 0;
+  for (int k = 0; k < 9; k++) {
+    const char p = pat[k];
+    const uint32_t v = (code >> (2 * k)) & 3u;
+    if (p == '*') continue;
+    if (p == 'T' || p == 't') { if (v == 0) return 0; continue; }
+    if (p == 'F' || p == 'f') { if (v != 0) return 0; continue; }
+    if (p >= '0' && p <= '2') { if (v != (uint32_t)(p - '0') + 1) return 0; continue; }
+    return 0;
+  }
+  return 1;
+}
+
+// ---- host sets
+void gsj_hostset_free(gsj_hostset* s) { delete s; }
+int gsj_hostset_sizes(const gsj_hostset* s, uint64_t* n, uint64_t* nr, uint64_t* nv) {
+  return guarded([&] {
+    GSJ_REQUIRE(s, GSJ_E_ARG, "null set");
+    if (n) *n = s->s.n();
+    if (nr) *nr = s->s.nr();
+    if (nv) *nv = s->s.nv();
+  });
+}
+int gsj_hostset_arrays(const gsj_hostset* s, const uint64_t** gro, const uint64_t** rvo, const double** xy, const uint32_t** parent, const uint8_t** syn) {
+  return guarded([&] {
+    GSJ_REQUIRE(s, GSJ_E_ARG, "null set");
+    if (gro) *gro = s->s.gro.data();
+    if (rvo) *rvo = s->s.rvo.data();
+    if (xy) *xy = s->s.xy.data();
+    if (parent) *parent = s->s.has_parent ? s->s.parent.data() : nullptr;
+    if (syn) *syn = s->s.has_parent ? s->s.syn.data() : nullptr;
+  });
+}
+
+int gsj_gset_read(const char* path, gsj_hostset** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(path && out, GSJ_E_ARG, "null path or output");
+    try {
+      auto* h = new gsj_hostset();
+      try { h->s = gsj::read_gset(path); } catch (...) { delete h; throw; }
+      *out = h;
+    } catch (const std::runtime_error& e) {
+      throw gsj::Error(GSJ_E_IO, e.what());
+    }
+  });
+}
+int gsj_gset_write(const char* path, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr, uint64_t nv) {
+  return guarded([&] {
+    GSJ_REQUIRE(path && gro && rvo, GSJ_E_ARG, "null argument");
+    try { gsj::write_gset(path, gro, rvo, xy, n, nr, nv); } catch (const std::runtime_error& e) { throw gsj::Error(GSJ_E_IO, e.what()); }
+  });
+}
+int gsj_pset_read(const char* path, double** xy, uint64_t* n) {
+  return guarded([&] {
+    GSJ_REQUIRE(path && xy && n, GSJ_E_ARG, "null argument");
+    std::vector<double> v;
+    try { v = gsj::read_pset(path); } catch (const std::runtime_error& e) { throw gsj::Error(GSJ_E_IO, e.what()); }
+    *n = v.size() / 2;
+    *xy = static_cast<double*>(std::malloc(std::max<size_t>(v.size(), 1) * 8));
+    if (!*xy) throw std::bad_alloc();
+    std::memcpy(*xy, v.data(), v.size() * 8);
+  });
+}
+int gsj_pset_write(const char* path, const double* xy, uint64_t n) {
+  return guarded([&] {
+    GSJ_REQUIRE(path && (xy || n == 0), GSJ_E_ARG, "null argument");
+    try { gsj::write_pset(path, xy, n); } catch (const std::runtime_error& e) { throw gsj::Error(GSJ_E_IO, e.what()); }
+  });
+}
+void gsj_free(void* p) { std::free(p); }
+
+int gsj_subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv, gsj_hostset** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(gro && rvo && out, GSJ_E_ARG, "null argument");
+    GSJ_REQUIRE(maxv >= 4, GSJ_E_ARG, "subdivide: maxv must be >= 4");
+    auto* h = new gsj_hostset();
+    h->s = gsj::subdivide(gro, rvo, xy, n, maxv);
+    *out = h;
+  });
+}
+
+int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct, uint64_t* moff, uint32_t* midx,
+              uint64_t* n_distinct) {
+  return guarded([&] {
+    GSJ_REQUIRE(gro && rvo && distinct && moff && midx && n_distinct, GSJ_E_ARG, "null argument");
+    std::vector<uint64_t> mo;
+    std::vector<uint32_t> mi;
+    auto* h = new gsj_hostset();
+    h->s = gsj::dedup(gro, rvo, xy, n, mo, mi);
+    std::memcpy(moff, mo.data(), mo.size() * 8);
+    std::memcpy(midx, mi.data(), mi.size() * 4);
+    *n_distinct = h->s.n();
+    *distinct = h;
+  });
+}
+
+int gsj_generate(const char* kind, const char* spec, gsj_hostset** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(kind && out, GSJ_E_ARG, "null argument");
+    auto* h = new gsj_hostset();
+    try { h->s = gsj::generate(kind, spec); } catch (...) { delete h; throw; }
+    *out = h;
+  });
+}
+
+int gsj_generate_points(const char* spec, double** xy, uint64_t* n) {
+  return guarded([&] {
+    GSJ_REQUIRE(xy && n, GSJ_E_ARG, "null argument");
+    std::vector<double> v = gsj::generate_points(spec);
+    *n = v.size() / 2;
+    *xy = static_cast<double*>(std::malloc(std::max<size_t>(v.size(), 1) * 8));
+    if (!*xy) throw std::bad_alloc();
+    std::memcpy(*xy, v.data(), v.size() * 8);
+  });
+}
+
+}  // extern "C"

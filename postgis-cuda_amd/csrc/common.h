@@ -1,0 +1,100 @@
+// Engine-internal plumbing: status codes, HIP error checks, grow-only device scratch, stream timers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <mutex>
+#include "../../include/gsj.h"
+
+namespace gsj {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define GSJ_HIP(x)                                                                                     \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess)                                                                              \
+      throw ::gsj::Error(e_ == hipErrorOutOfMemory ? GSJ_E_NOMEM : GSJ_E_HIP,                          \
+                         std::string(#x) + " at " + __FILE__ + ":" + std::to_string(__LINE__) + ": " + \
+                             hipGetErrorString(e_));                                                   \
+  } while (0)
+
+#define GSJ_REQUIRE(cond, code, msg) \
+  do {                               \
+    if (!(cond)) throw ::gsj::Error(code, msg); \
+  } while (0)
+
+// grow-only device buffer; contents are not preserved on growth
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void* get(size_t n) {
+    if (n == 0) n = 1;
+    if (n > cap) {
+      if (p) GSJ_HIP(hipFree(p));
+      p = nullptr; cap = 0;
+      size_t want = n + n / 8;   // headroom so slowly growing requests do not reallocate every call
+      GSJ_HIP(hipMalloc(&p, want));
+      cap = want;
+    }
+    return p;
+  }
+  template <class T> T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// grow-only pinned host buffer
+struct PinBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void* get(size_t n) {
+    if (n == 0) n = 1;
+    if (n > cap) {
+      if (p) GSJ_HIP(hipHostFree(p));
+      p = nullptr; cap = 0;
+      size_t want = n + n / 8;
+      GSJ_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+      cap = want;
+    }
+    return p;
+  }
+  template <class T> T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
+  void release() { if (p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+  ~PinBuf() { release(); }
+  PinBuf() = default;
+  PinBuf(const PinBuf&) = delete;
+  PinBuf& operator=(const PinBuf&) = delete;
+};
+
+// event pair on a stream; elapsed() synchronises on the stop event
+struct StreamTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s = nullptr;
+  explicit StreamTimer(hipStream_t st) : s(st) { GSJ_HIP(hipEventCreate(&a)); GSJ_HIP(hipEventCreate(&b)); }
+  ~StreamTimer() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); }
+  void start() { GSJ_HIP(hipEventRecord(a, s)); }
+  float stop() {
+    GSJ_HIP(hipEventRecord(b, s));
+    GSJ_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+};
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// kernel launch check (launch errors surface on the next runtime call otherwise)
+#define GSJ_LAUNCHED() GSJ_HIP(hipGetLastError())
+
+}  // namespace gsj

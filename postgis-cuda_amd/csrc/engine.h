@@ -1,0 +1,137 @@
+// Engine internals shared by the .hip translation units: device layer layout, BVH layout, context, primitives.
+#pragma once
+#include "common.h"
+#include "geom.h"
+#include <functional>
+#include <memory>
+#include <string>
+
+namespace gsj {
+
+// ------------------------------------------------------------------------------------------- device layouts
+// Polygon layer resident in HBM (SoA). Sizes at 162M parcels x ~10 vertices: xy 26 GB, per-geometry arrays
+// ~60 B x n, BVH 32 B x (2n - 1).
+struct DPolys {
+  uint32_t n = 0, nr = 0;
+  uint64_t nv = 0;
+  const double2* origin = nullptr;   // [n] bbox min (FP64)
+  const double2* ext = nullptr;      // [n] bbox extent
+  const uint32_t* gro = nullptr;     // [n+1] geometry -> first ring
+  const uint64_t* rvo = nullptr;     // [nr+1] ring -> first vertex
+  const double2* xy = nullptr;       // [nv] v - origin of its geometry
+  const float4* bb = nullptr;        // [n] outward-rounded float box (minx, miny, maxx, maxy), world coords
+  const uint32_t* nedge = nullptr;   // [n] number of edges (vertices - rings), for work classification
+  const uint32_t* parent = nullptr;  // [n] piece -> parent, nullptr if not subdivided
+  const uint32_t* synb = nullptr;    // [(nv+31)/32] bit v: edge v -> v+1 was made by subdivision; nullptr if none
+  const double* blen = nullptr;      // [nparent] real boundary length per parent
+  const uint32_t* pfirst = nullptr;  // [nparent+1] first piece of each parent (subdivided layers only)
+  uint32_t nparent = 0;
+};
+
+struct DPoints {
+  uint32_t n = 0;
+  const double2* xy = nullptr;   // [n] world coordinates
+  const float4* bb = nullptr;    // [n] outward-rounded float point boxes
+};
+
+// LBVH node in depth-first (preorder) layout: the left child of an internal node is the next node; `skip` is the
+// first node after the subtree (== node count: done); `prim` is the primitive of a leaf, -1 for internal nodes.
+struct alignas(16) BNode {
+  float4 box;
+  int32_t skip;
+  int32_t prim;
+  int32_t pad0, pad1;
+};
+
+struct DBvh {
+  uint32_t n = 0;           // primitives
+  uint32_t nnodes = 0;      // 2n - 1
+  const BNode* nodes = nullptr;
+  const uint32_t* order = nullptr;   // [n] primitives in Morton (leaf) order
+};
+
+// ------------------------------------------------------------------------------------------- host objects
+struct Context;
+
+struct Layer {
+  Context* ctx = nullptr;
+  bool points = false;
+  uint64_t n = 0, nr = 0, nv = 0;
+  uint32_t nparent = 0;
+  // owned device memory
+  void* mem = nullptr;          // one allocation holding every layer array
+  size_t mem_bytes = 0;
+  void* bvh_mem = nullptr;
+  size_t bvh_bytes = 0;
+  DPolys poly;
+  DPoints pts;
+  DBvh bvh;
+  bool has_bvh = false;
+  std::vector<uint32_t> h_parent;   // host copy: keeps a parent's pieces in one probe batch
+  ~Layer();
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;                  // one call at a time per context
+  std::string status;             // last status JSON
+  // scratch, grow-only, reused across calls
+  DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
+      s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc;
+  PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;
+  int cu_count = 256;
+  ~Context();
+};
+
+// ------------------------------------------------------------------------------------------- primitives (prims.hip)
+// exclusive prefix sum of n u32 values into u64 offsets; returns the total (synchronises)
+uint64_t scan_u32_to_u64(Context& c, const uint32_t* in, uint64_t* out, uint64_t n);
+// exclusive prefix sum without host sync: total written to *d_total
+void scan_u32_to_u64_async(Context& c, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* d_total);
+// stable LSD radix sort of (key, value) by key; only the digits in [bit0, bit1) are sorted; returns pointers to the
+// sorted arrays (either the inputs or the alternates)
+void radix_sort_pairs(Context& c, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint64_t n, int bit0,
+                      int bit1);
+// OR and AND of all keys (for choosing digits); synchronises
+void key_bits(Context& c, const uint64_t* keys, uint64_t n, uint64_t& or_all, uint64_t& and_all);
+
+// ------------------------------------------------------------------------------------------- layers / bvh
+void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr, uint64_t nv,
+                        const uint32_t* parent, const uint8_t* syn);
+void layer_upload_points(Layer& L, const double* xy, uint64_t n);
+void bvh_build(Layer& L);
+
+// ------------------------------------------------------------------------------------------- traversal
+// count pass (write == false) or write pass over probe boxes qbb[qbase .. qbase+nq)
+void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
+              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b);
+
+// ------------------------------------------------------------------------------------------- refinement
+struct RefineParams {   // engine/engine.cuh:517-518
+  double tol_rel = 1e-12, eps_par = 1e-7, tol_abs = 1e-11;
+  int want_area = 1, want_relate = 0, snap_half_ulp = 0;
+};
+struct RefineOut {
+  uint8_t* isect;   // [np] 0 / 1 real contact or containment / 2 area-only piece overlap
+  double* area;     // [np] or nullptr
+  double4* ev;      // [np] (A in, A on, B in, B on) or nullptr
+  uint32_t* evb;    // [np] evidence bits or nullptr
+};
+// refine np candidate pairs (pa[i], pb[i]); returns the number of pairs that needed global-memory edge lists
+uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
+                      const RefineOut& out);
+
+// ------------------------------------------------------------------------------------------- join driver
+struct JoinHooks {
+  // consumer of each finished sub-batch while on the device (rows: 2n interleaved parent ids, area or nullptr, n)
+  std::function<void(const uint32_t*, const double*, uint64_t)> on_device;
+  bool device_only = false;
+};
+void join_run(Context& c, const Layer& P, const Layer& R, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st,
+              const JoinHooks* hooks);
+void points_join_run(Context& c, const Layer& pts, const Layer& R, const gsj_join_opts& o, gsj_points_cb cb, void* user,
+                     gsj_join_stats* st);
+void cluster_run(Context& c, const Layer& P, const gsj_join_opts& o, uint32_t* root, uint64_t nroot, gsj_join_stats* st);
+
+}  // namespace gsj

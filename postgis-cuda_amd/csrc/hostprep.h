@@ -1,0 +1,32 @@
+// Host-side layer preparation (files, subdivision, dedup, generators). No HIP dependency.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace gsj {
+
+// flat polygon set in the .gset layout (reference HostSet, engine/gset.h:5-11), plus subdivision metadata
+struct HostSet {
+  std::vector<uint64_t> gro, rvo;   // [n+1], [nr+1]
+  std::vector<double> xy;           // [2 nv]
+  bool has_parent = false;
+  std::vector<uint32_t> parent;     // [n] piece -> source geometry (subdivided sets)
+  std::vector<uint8_t> syn;         // [nv] edge v -> v+1 lies on a cut (subdivided sets)
+  uint64_t n() const { return gro.empty() ? 0 : gro.size() - 1; }
+  uint64_t nr() const { return rvo.empty() ? 0 : rvo.size() - 1; }
+  uint64_t nv() const { return xy.size() / 2; }
+};
+
+HostSet read_gset(const std::string& path);
+void write_gset(const std::string& path, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr, uint64_t nv);
+std::vector<double> read_pset(const std::string& path);
+void write_pset(const std::string& path, const double* xy, uint64_t n);
+
+HostSet subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv);
+HostSet dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, std::vector<uint64_t>& moff, std::vector<uint32_t>& midx);
+
+HostSet generate(const std::string& kind, const char* spec);
+std::vector<double> generate_points(const char* spec);
+
+}  // namespace gsj

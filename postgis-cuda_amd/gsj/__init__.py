@@ -1,0 +1,401 @@
+"""Python bindings (ctypes) for libgsj, the MI355X spatial join engine.
+
+Mirrors the engine entry points of the reference (landstats-us/postgis-cuda engine/engine.cuh):
+``Context.join`` ~ join_stream, ``Context.points_in_polygons`` ~ join_points_stream, ``Context.cluster`` ~
+cluster_intersecting; host-side preparation ``subdivide`` ~ engine/subdivide.h, ``dedup`` ~ engine/dedup.h,
+``read_gset``/``read_pset`` ~ engine/gset.h. The compute path is the HIP library only: if ``libgsj.so`` is not
+built, importing the bindings raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import weakref
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgsj.so")
+
+GSJ_OK = 0
+ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "LIMIT", -5: "CANCEL", -6: "IO", -7: "INTERNAL"}
+DE9IM_PT_INTERIOR = 1 | 3 << 12 | 2 << 14 | 3 << 16
+DE9IM_PT_BOUNDARY = 1 << 2 | 3 << 12 | 2 << 14 | 3 << 16
+
+
+class GsjError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gsj error {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class JoinOpts(C.Structure):
+    _fields_ = [("want_area", C.c_int), ("want_relate", C.c_int), ("upper", C.c_int), ("snap_half_ulp", C.c_int),
+                ("batch_q", C.c_uint32), ("max_pairs", C.c_uint64), ("no_agg", C.c_int), ("cancel", C.POINTER(C.c_int))]
+
+
+class JoinStats(C.Structure):
+    _fields_ = [("traverse_ms", C.c_double), ("refine_ms", C.c_double), ("compact_ms", C.c_double), ("copy_ms", C.c_double),
+                ("candidates", C.c_uint64), ("hits", C.c_uint64), ("heavy", C.c_uint64), ("batches", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LayerInfo(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("nr", C.c_uint64), ("nv", C.c_uint64), ("nparent", C.c_uint32), ("is_points", C.c_int),
+                ("has_bvh", C.c_int), ("device_bytes", C.c_uint64)]
+
+
+ROWS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t)
+POINTS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_size_t)
+
+_lib = None
+
+# every symbol include/gsj.h declares (tests check the library exports all of them)
+EXPORTS = [
+    "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
+    "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
+    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
+    "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
+    "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_generate", "gsj_generate_points",
+]
+
+
+def load(path: str | None = None):
+    """Load libgsj.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or os.environ.get("GSJ_LIB", LIB_PATH)
+    if not os.path.exists(p):
+        raise ImportError(f"libgsj.so not found at {p}: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(p)
+    vp, u64, u32, i32, dp = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_double)
+    u64p, u32p, u8p, fp = C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.POINTER(C.c_float)
+    sig = {
+        "gsj_last_error": (C.c_char_p, []), "gsj_version": (C.c_char_p, []),
+        "gsj_ctx_create": (i32, [i32, C.POINTER(vp)]), "gsj_ctx_destroy": (None, [vp]), "gsj_ctx_status_json": (C.c_char_p, [vp]),
+        "gsj_ctx_trim": (i32, [vp]),
+        "gsj_layer_create_polys": (i32, [vp, u64p, u64p, dp, u64, u64, u64, u32p, u8p, C.POINTER(vp)]),
+        "gsj_layer_create_points": (i32, [vp, dp, u64, C.POINTER(vp)]), "gsj_layer_free": (None, [vp]), "gsj_bvh_build": (i32, [vp]),
+        "gsj_layer_info_get": (i32, [vp, C.POINTER(LayerInfo)]), "gsj_layer_export": (i32, [vp, dp, dp, fp, dp, dp]),
+        "gsj_join": (i32, [vp, vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_points_in_polygons": (i32, [vp, vp, vp, C.POINTER(JoinOpts), POINTS_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_cluster": (i32, [vp, vp, C.POINTER(JoinOpts), u32p, u64, C.POINTER(JoinStats)]),
+        "gsj_de9im_str": (None, [u32, C.c_char_p]), "gsj_de9im_transpose": (u32, [u32]), "gsj_de9im_match": (i32, [u32, C.c_char_p]),
+        "gsj_hostset_free": (None, [vp]), "gsj_hostset_sizes": (i32, [vp, u64p, u64p, u64p]),
+        "gsj_hostset_arrays": (i32, [vp, C.POINTER(u64p), C.POINTER(u64p), C.POINTER(dp), C.POINTER(u32p), C.POINTER(u8p)]),
+        "gsj_gset_read": (i32, [C.c_char_p, C.POINTER(vp)]), "gsj_gset_write": (i32, [C.c_char_p, u64p, u64p, dp, u64, u64, u64]),
+        "gsj_pset_read": (i32, [C.c_char_p, C.POINTER(dp), u64p]), "gsj_pset_write": (i32, [C.c_char_p, dp, u64]),
+        "gsj_free": (None, [vp]), "gsj_subdivide": (i32, [u64p, u64p, dp, u64, i32, C.POINTER(vp)]),
+        "gsj_dedup": (i32, [u64p, u64p, dp, u64, C.POINTER(vp), u64p, u32p, u64p]),
+        "gsj_generate": (i32, [C.c_char_p, C.c_char_p, C.POINTER(vp)]), "gsj_generate_points": (i32, [C.c_char_p, C.POINTER(dp), u64p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != GSJ_OK:
+        raise GsjError(rc, _lib.gsj_last_error().decode())
+
+
+def _ptr(a, ctype):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ctype))
+
+
+# ------------------------------------------------------------------------------------------- host sets
+@dataclass
+class HostSet:
+    """Flat polygon set in the .gset layout (engine/gset.h:5-11) plus optional subdivision metadata."""
+    gro: np.ndarray
+    rvo: np.ndarray
+    xy: np.ndarray                  # (nv, 2) float64
+    parent: np.ndarray | None = None
+    syn: np.ndarray | None = None
+    ids: list = field(default_factory=list)
+
+    @property
+    def n(self):
+        return len(self.gro) - 1
+
+    @property
+    def nr(self):
+        return len(self.rvo) - 1
+
+    @property
+    def nv(self):
+        return len(self.xy)
+
+    def normalized(self):
+        self.gro = np.ascontiguousarray(self.gro, dtype=np.uint64)
+        self.rvo = np.ascontiguousarray(self.rvo, dtype=np.uint64)
+        self.xy = np.ascontiguousarray(np.asarray(self.xy, dtype=np.float64).reshape(-1, 2))
+        if self.parent is not None:
+            self.parent = np.ascontiguousarray(self.parent, dtype=np.uint32)
+        if self.syn is not None:
+            self.syn = np.ascontiguousarray(self.syn, dtype=np.uint8)
+        return self
+
+    def geometry(self, g):
+        """rings of geometry g as a list of (k, 2) arrays (closed)"""
+        return [self.xy[self.rvo[r]:self.rvo[r + 1]] for r in range(int(self.gro[g]), int(self.gro[g + 1]))]
+
+
+def _take_hostset(h) -> HostSet:
+    lib = load()
+    n, nr, nv = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    _check(lib.gsj_hostset_sizes(h, C.byref(n), C.byref(nr), C.byref(nv)))
+    gro, rvo, xy = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint64)(), C.POINTER(C.c_double)()
+    par, syn = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint8)()
+    _check(lib.gsj_hostset_arrays(h, C.byref(gro), C.byref(rvo), C.byref(xy), C.byref(par), C.byref(syn)))
+    try:
+        out = HostSet(np.ctypeslib.as_array(gro, (n.value + 1,)).copy(), np.ctypeslib.as_array(rvo, (nr.value + 1,)).copy(),
+                      (np.ctypeslib.as_array(xy, (2 * nv.value,)).copy() if nv.value else np.zeros(0)).reshape(-1, 2))
+        if par:
+            out.parent = np.ctypeslib.as_array(par, (n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+            out.syn = np.ctypeslib.as_array(syn, (nv.value,)).copy() if nv.value else np.zeros(0, np.uint8)
+    finally:
+        lib.gsj_hostset_free(h)
+    return out.normalized()
+
+
+def _spec(kw):
+    return ";".join(f"{k}={v!r}" if isinstance(v, float) else f"{k}={v}" for k, v in kw.items())
+
+
+def generate(kind: str, **spec) -> HostSet:
+    """Deterministic synthetic layer: kind in parcels_grid (config 1 parcels), regions_grid (config 1 regions),
+    parcels_mesh (config 2 parcels)."""
+    lib = load()
+    h = C.c_void_p()
+    _check(lib.gsj_generate(kind.encode(), _spec(spec).encode(), C.byref(h)))
+    return _take_hostset(h)
+
+
+def generate_points(**spec) -> np.ndarray:
+    lib = load()
+    p, n = C.POINTER(C.c_double)(), C.c_uint64()
+    _check(lib.gsj_generate_points(_spec(spec).encode(), C.byref(p), C.byref(n)))
+    try:
+        return np.ctypeslib.as_array(p, (2 * n.value,)).copy().reshape(-1, 2) if n.value else np.zeros((0, 2))
+    finally:
+        lib.gsj_free(p)
+
+
+def subdivide(hs: HostSet, maxv: int) -> HostSet:
+    lib = load()
+    hs = hs.normalized()
+    h = C.c_void_p()
+    _check(lib.gsj_subdivide(_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, maxv, C.byref(h)))
+    return _take_hostset(h)
+
+
+def dedup(hs: HostSet):
+    """(distinct set, moff[un+1], midx[n])"""
+    lib = load()
+    hs = hs.normalized()
+    h = C.c_void_p()
+    moff = np.zeros(hs.n + 1, np.uint64)
+    midx = np.zeros(max(hs.n, 1), np.uint32)
+    un = C.c_uint64()
+    _check(lib.gsj_dedup(_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, C.byref(h),
+                         _ptr(moff, C.c_uint64), _ptr(midx, C.c_uint32), C.byref(un)))
+    return _take_hostset(h), moff[: un.value + 1].copy(), midx[: hs.n].copy()
+
+
+def read_gset(path: str) -> HostSet:
+    lib = load()
+    h = C.c_void_p()
+    _check(lib.gsj_gset_read(path.encode(), C.byref(h)))
+    hs = _take_hostset(h)
+    ids = path + ".ids"
+    if os.path.exists(ids):
+        with open(ids) as f:
+            hs.ids = f.read().splitlines()
+    return hs
+
+
+def write_gset(path: str, hs: HostSet, ids=None):
+    lib = load()
+    hs = hs.normalized()
+    _check(lib.gsj_gset_write(path.encode(), _ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, hs.nr, hs.nv))
+    ids = ids if ids is not None else (hs.ids or [str(i) for i in range(hs.n)])
+    with open(path + ".ids", "w") as f:
+        f.write("\n".join(map(str, ids)) + "\n")
+
+
+def read_pset(path: str) -> np.ndarray:
+    lib = load()
+    p, n = C.POINTER(C.c_double)(), C.c_uint64()
+    _check(lib.gsj_pset_read(path.encode(), C.byref(p), C.byref(n)))
+    try:
+        return np.ctypeslib.as_array(p, (2 * n.value,)).copy().reshape(-1, 2) if n.value else np.zeros((0, 2))
+    finally:
+        lib.gsj_free(p)
+
+
+def write_pset(path: str, xy: np.ndarray, ids=None):
+    lib = load()
+    xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+    _check(lib.gsj_pset_write(path.encode(), _ptr(xy, C.c_double), len(xy)))
+    ids = ids if ids is not None else [str(i) for i in range(len(xy))]
+    with open(path + ".ids", "w") as f:
+        f.write("\n".join(map(str, ids)) + "\n")
+
+
+def de9im_str(code: int) -> str:
+    return "".join("F012"[(int(code) >> (2 * k)) & 3] for k in range(9))
+
+
+def de9im_transpose(code: int) -> int:
+    return int(load().gsj_de9im_transpose(int(code)))
+
+
+def de9im_match(code: int, pattern: str) -> bool:
+    return bool(load().gsj_de9im_match(int(code), pattern.encode()))
+
+
+# ------------------------------------------------------------------------------------------- device
+class Layer:
+    def __init__(self, ctx: "Context", handle, kind: str, hs: HostSet | None = None):
+        self.ctx, self.h, self.kind, self.host = ctx, handle, kind, hs
+
+    def build_bvh(self):
+        _check(load().gsj_bvh_build(self.h))
+        return self
+
+    def info(self) -> LayerInfo:
+        i = LayerInfo()
+        _check(load().gsj_layer_info_get(self.h, C.byref(i)))
+        return i
+
+    def export(self):
+        """prepared device arrays: origin (n,2), ext (n,2), bb (n,4) float32, q (nv,2), blen (nparent,)"""
+        i = self.info()
+        if i.is_points:
+            bb = np.zeros((i.n, 4), np.float32)
+            _check(load().gsj_layer_export(self.h, None, None, _ptr(bb, C.c_float), None, None))
+            return {"bb": bb}
+        origin, ext = np.zeros((i.n, 2)), np.zeros((i.n, 2))
+        bb, q, blen = np.zeros((i.n, 4), np.float32), np.zeros((i.nv, 2)), np.zeros(i.nparent)
+        _check(load().gsj_layer_export(self.h, _ptr(origin, C.c_double), _ptr(ext, C.c_double), _ptr(bb, C.c_float), _ptr(q, C.c_double),
+                                       _ptr(blen, C.c_double)))
+        return {"origin": origin, "ext": ext, "bb": bb, "q": q, "blen": blen}
+
+    def free(self):
+        if self.h and self.ctx.h:
+            load().gsj_layer_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class Rows:
+    pairs: np.ndarray           # (n, 2) uint32 parent ids
+    area: np.ndarray | None
+    code: np.ndarray | None
+    where: np.ndarray | None
+    stats: dict
+
+
+def _opts(want_area=False, want_relate=False, upper=False, snap_half_ulp=False, batch_q=0, max_pairs=0, no_agg=False):
+    o = JoinOpts()
+    o.want_area, o.want_relate, o.upper, o.snap_half_ulp = int(want_area), int(want_relate), int(upper), int(snap_half_ulp)
+    o.batch_q, o.max_pairs, o.no_agg = batch_q, max_pairs, int(no_agg)
+    return o
+
+
+class Context:
+    """One HIP device + stream; layers and joins live on it."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.gsj_ctx_create(device, C.byref(h)))
+        self.h = h
+        self._layers = weakref.WeakSet()
+
+    def status(self) -> str:
+        return load().gsj_ctx_status_json(self.h).decode()
+
+    def polys(self, hs: HostSet, build_bvh=True) -> Layer:
+        hs = hs.normalized()
+        h = C.c_void_p()
+        _check(load().gsj_layer_create_polys(self.h, _ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, hs.nr,
+                                            hs.nv, _ptr(hs.parent, C.c_uint32), _ptr(hs.syn, C.c_uint8), C.byref(h)))
+        L = Layer(self, h, "polygon", hs)
+        self._layers.add(L)
+        return L.build_bvh() if build_bvh else L
+
+    def points(self, xy: np.ndarray, build_bvh=False) -> Layer:
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        h = C.c_void_p()
+        _check(load().gsj_layer_create_points(self.h, _ptr(xy, C.c_double), len(xy), C.byref(h)))
+        L = Layer(self, h, "point")
+        self._layers.add(L)
+        return L.build_bvh() if build_bvh else L
+
+    def join(self, P: Layer, R: Layer, **kw) -> Rows:
+        o = _opts(**kw)
+        chunks = []
+
+        def cb(user, pairs, area, code, n):
+            p = np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2)
+            a = np.ctypeslib.as_array(area, (n,)).copy() if area else None
+            c = np.ctypeslib.as_array(code, (n,)).copy() if code else None
+            chunks.append((p, a, c))
+            return 0
+
+        st = JoinStats()
+        _check(load().gsj_join(self.h, P.h, R.h, C.byref(o), ROWS_CB(cb), None, C.byref(st)))
+        pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
+        area = np.concatenate([c[1] for c in chunks]) if chunks and o.want_area else (np.zeros(0) if o.want_area else None)
+        code = np.concatenate([c[2] for c in chunks]) if chunks and o.want_relate else (np.zeros(0, np.uint32) if o.want_relate else None)
+        return Rows(pairs, area, code, None, st.as_dict())
+
+    def points_in_polygons(self, pts: Layer, R: Layer, **kw) -> Rows:
+        o = _opts(**kw)
+        chunks = []
+
+        def cb(user, pairs, where, n):
+            chunks.append((np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2), np.ctypeslib.as_array(where, (n,)).copy()))
+            return 0
+
+        st = JoinStats()
+        _check(load().gsj_points_in_polygons(self.h, pts.h, R.h, C.byref(o), POINTS_CB(cb), None, C.byref(st)))
+        pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
+        where = np.concatenate([c[1] for c in chunks]) if chunks else np.zeros(0, np.uint8)
+        return Rows(pairs, None, None, where, st.as_dict())
+
+    def cluster(self, P: Layer, **kw):
+        o = _opts(**kw)
+        n = P.info().nparent
+        root = np.zeros(max(n, 1), np.uint32)
+        st = JoinStats()
+        _check(load().gsj_cluster(self.h, P.h, C.byref(o), _ptr(root, C.c_uint32), n, C.byref(st)))
+        return root[:n], st.as_dict()
+
+    def close(self):
+        if self.h:
+            for L in list(self._layers):   # layers are device objects of this context: free them first
+                L.free()
+            load().gsj_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,135 @@
+"""HIP engine vs the oracle (CPU restatement) through libgsj's C ABI: layer preparation, candidates, intersects,
+intersection areas, DE-9IM, piece -> parent aggregation, batching, points in polygons, clustering.
+
+Bar: pairs, flags and DE-9IM codes bit-exact; areas bit-exact as well (the engine adds every sum in the same order as
+the oracle), with the 1e-9 relative tolerance of the metric kept as the documented fallback bound."""
+import numpy as np
+import pytest
+
+import gsj
+from oracle import oracle as O
+from conftest import sorted_rows
+
+pytestmark = pytest.mark.gpu
+
+AREA_RTOL = 1e-9   # documented bound; the engine is expected to match to the bit
+
+
+def _cfg1(n=100, segs=500, k=4):
+    P = gsj.generate("parcels_grid", nx=n, ny=n, seed=0)
+    R = gsj.generate("regions_grid", nx=n, ny=n, kx=k, ky=k, edge_segs=segs, seed=0)
+    return P, R
+
+
+def _mesh(n=120, seed=1, **kw):
+    return gsj.generate("parcels_mesh", nx=n, ny=n, w=1 / 1549, h=1 / 1549, seed=seed, **kw)
+
+
+def _check_rows(got, exp, area=False, code=False):
+    g = sorted_rows(got.pairs, got.area if area else None, got.code if code else None)
+    e = sorted_rows(exp["pairs"], exp.get("area") if area else None, exp.get("code") if code else None)
+    assert g[0].shape == e[0].shape, f"row count {len(g[0])} != oracle {len(e[0])}"
+    assert np.array_equal(g[0], e[0]), "pair sets differ"
+    if area:
+        np.testing.assert_allclose(g[1], e[1], rtol=AREA_RTOL, atol=0)
+        assert np.array_equal(g[1], e[1]), f"areas not bit-identical (max rel {np.max(np.abs(g[1] - e[1]) / np.maximum(np.abs(e[1]), 1e-300))})"
+    if code:
+        assert np.array_equal(g[2], e[2]), "DE-9IM codes differ"
+
+
+def test_layer_prep_matches_oracle(ctx):
+    P = gsj.subdivide(_mesh(60), 16)
+    d = ctx.polys(P, build_bvh=False).export()
+    o = O.Layer.polys(P).export()
+    for k in ("origin", "ext", "bb", "q"):
+        assert np.array_equal(d[k], o[k]), k
+    np.testing.assert_allclose(d["blen"], o["blen"], rtol=1e-15)
+
+
+def test_points_prep_matches_oracle(ctx):
+    xy = gsj.generate_points(n=20000, x0=-95.5, y0=38.5, w=0.05, h=0.05, clusters=20, seed=2)
+    assert np.array_equal(ctx.points(xy).export()["bb"], O.Layer.points_layer(xy).export()["bb"])
+
+
+def test_cfg1_intersects_area(ctx):
+    P, R = _cfg1()
+    got = ctx.join(ctx.polys(P, build_bvh=False), ctx.polys(R), want_area=True)
+    exp = O.join(O.Layer.polys(P), O.Layer.polys(R), want_area=True)
+    assert got.stats["candidates"] == exp["candidates"]
+    _check_rows(got, exp, area=True)
+    assert len(got.pairs) == P.n   # every parcel lies in exactly one region
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_mesh_self_relate(ctx, upper):
+    M = _mesh(120)
+    L = ctx.polys(M)
+    got = ctx.join(L, L, want_area=True, want_relate=True, upper=upper)
+    OM = O.Layer.polys(M)
+    exp = O.join(OM, OM, want_area=True, want_relate=True, upper=upper)
+    assert got.stats["candidates"] == exp["candidates"]
+    _check_rows(got, exp, area=True, code=True)
+
+
+@pytest.mark.parametrize("maxv", [8, 64])
+def test_mesh_subdivided_relate(ctx, maxv):
+    M = gsj.subdivide(_mesh(100, seed=5), maxv)
+    assert M.parent is not None and M.n > 100 * 100 or maxv == 64
+    L = ctx.polys(M)
+    got = ctx.join(L, L, want_area=True, want_relate=True, upper=True)
+    OM = O.Layer.polys(M)
+    exp = O.join(OM, OM, want_area=True, want_relate=True, upper=True)
+    assert got.stats["candidates"] == exp["candidates"]
+    _check_rows(got, exp, area=True, code=True)
+
+
+def test_intersects_only_and_regions_relate(ctx):
+    P, R = _cfg1(60, segs=300, k=3)
+    R = gsj.subdivide(R, 64)
+    dP, dR = ctx.polys(P, build_bvh=False), ctx.polys(R)
+    oP, oR = O.Layer.polys(P), O.Layer.polys(R)
+    _check_rows(ctx.join(dP, dR), O.join(oP, oR, want_area=False))
+    _check_rows(ctx.join(dP, dR, want_relate=True), O.join(oP, oR, want_area=False, want_relate=True), code=True)
+
+
+def test_batching_is_invisible(ctx):
+    M = gsj.subdivide(_mesh(80, seed=7), 12)
+    L = ctx.polys(M)
+    ref = ctx.join(L, L, want_area=True, want_relate=True, upper=True)
+    small = ctx.join(L, L, want_area=True, want_relate=True, upper=True, batch_q=333, max_pairs=1000)
+    assert small.stats["batches"] > 3
+    _check_rows(small, {"pairs": ref.pairs, "area": ref.area, "code": ref.code}, area=True, code=True)
+
+
+def test_points_in_polygons(ctx):
+    M = _mesh(80, seed=3)
+    x0, y0 = -95.5, 38.5
+    xy = gsj.generate_points(n=50000, x0=x0, y0=y0, w=80 / 1549, h=80 / 1549, clusters=30, sigma_m=30.0, seed=2)
+    # points exactly on vertices and on edge midpoints of the layer: boundary cases
+    xy = np.concatenate([xy, M.xy[::7], 0.5 * (M.xy[:-1:5] + M.xy[1::5])])
+    pts = ctx.points(xy)
+    for layer in (M, gsj.subdivide(M, 10)):
+        got = ctx.points_in_polygons(pts, ctx.polys(layer))
+        exp = O.points_join(O.Layer.points_layer(xy), O.Layer.polys(layer))
+        g = sorted_rows(got.pairs, got.where)
+        e = sorted_rows(exp["pairs"], exp["where"])
+        assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1])
+        assert (e[1] == 2).sum() > 0   # boundary cases exercised
+
+
+def test_cluster(ctx):
+    for M in (_mesh(90, seed=11), gsj.subdivide(_mesh(50, seed=12), 10)):
+        L = ctx.polys(M)
+        root, st = ctx.cluster(L)
+        exp = O.cluster(O.Layer.polys(M))
+        assert np.array_equal(root, exp["root"])
+
+
+def test_empty_and_tiny_layers(ctx):
+    one = gsj.HostSet(np.array([0, 1]), np.array([0, 5]), np.array([[0, 0], [1, 0], [1, 1], [0, 1], [0, 0]], float))
+    L1 = ctx.polys(one)
+    r = ctx.join(L1, L1, want_area=True, want_relate=True)
+    assert r.pairs.tolist() == [[0, 0]] and r.area[0] == 1.0 and gsj.de9im_str(r.code[0]) == "2FFF1FFF2"
+    empty = gsj.HostSet(np.array([0]), np.array([0]), np.zeros((0, 2)))
+    L0 = ctx.polys(empty)
+    assert len(ctx.join(L0, L1).pairs) == 0 and len(ctx.join(L1, L0).pairs) == 0
