@@ -98,6 +98,11 @@ typedef struct {
   uint64_t hits;           /* result rows delivered (parent pairs) */
   uint64_t heavy;          /* pairs whose local edge lists exceeded on-chip capacity */
   uint64_t batches;
+  /* refinement kernels, timed with HIP events on the context's stream: 16-lane class (both polygons <= 32 edges),
+   * 64-lane class, and the global-memory overflow pass; summed durations and pairs refined by each */
+  double k_small_ms, k_large_ms, k_over_ms;
+  uint64_t n_small, n_large, n_over;
+  uint64_t launches;       /* refinement launches (sub-batches) */
 } gsj_join_stats;
 
 /* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;

@@ -33,31 +33,70 @@
 
 namespace orc {
 
+// ------------------------------------------------------------------------------------------- arithmetic type
+// Real is double; the FLOP-counting build (-DORC_COUNT_FLOPS, liboracle_flops.so) swaps in a double wrapper that
+// counts every add/sub, mul, fma, div and sqrt the refinement executes: the algorithmic FLOPs of a pair.
+#ifdef ORC_COUNT_FLOPS
+struct FlopCount { uint64_t add = 0, mul = 0, fma = 0, div = 0, sqrt = 0; };
+static thread_local FlopCount g_fc;
+struct Real {
+  double v;
+  Real() = default;
+  Real(double x) : v(x) {}
+};
+static inline Real operator+(Real a, Real b) { g_fc.add++; return Real(a.v + b.v); }
+static inline Real operator-(Real a, Real b) { g_fc.add++; return Real(a.v - b.v); }
+static inline Real operator-(Real a) { return Real(-a.v); }
+static inline Real operator*(Real a, Real b) { g_fc.mul++; return Real(a.v * b.v); }
+static inline Real operator/(Real a, Real b) { g_fc.div++; return Real(a.v / b.v); }
+static inline Real& operator+=(Real& a, Real b) { a = a + b; return a; }
+static inline bool operator<(Real a, Real b) { return a.v < b.v; }
+static inline bool operator>(Real a, Real b) { return a.v > b.v; }
+static inline bool operator<=(Real a, Real b) { return a.v <= b.v; }
+static inline bool operator>=(Real a, Real b) { return a.v >= b.v; }
+static inline bool operator==(Real a, Real b) { return a.v == b.v; }
+static inline bool operator!=(Real a, Real b) { return a.v != b.v; }
+static inline Real r_fma(Real a, Real b, Real c) { g_fc.fma++; return Real(std::fma(a.v, b.v, c.v)); }
+static inline Real r_min(Real a, Real b) { return Real(std::fmin(a.v, b.v)); }
+static inline Real r_max(Real a, Real b) { return Real(std::fmax(a.v, b.v)); }
+static inline Real r_abs(Real a) { return Real(std::fabs(a.v)); }
+static inline Real r_sqrt(Real a) { g_fc.sqrt++; return Real(std::sqrt(a.v)); }
+static inline double val(Real a) { return a.v; }
+#else
+using Real = double;
+static inline Real r_fma(Real a, Real b, Real c) { return std::fma(a, b, c); }
+static inline Real r_min(Real a, Real b) { return std::fmin(a, b); }
+static inline Real r_max(Real a, Real b) { return std::fmax(a, b); }
+static inline Real r_abs(Real a) { return std::fabs(a); }
+static inline Real r_sqrt(Real a) { return std::sqrt(a); }
+static inline double val(Real a) { return a; }
+#endif
+
 // ------------------------------------------------------------------------------------------- predicates
 // engine/engine.cuh:283-289: 2x2 determinant (b-a) x (c-a) with the product rounding error recovered by fma
-static inline double orient(double ax, double ay, double bx, double by, double cx, double cy) {
-  double a = bx - ax, b = cy - ay, c = by - ay, d = cx - ax;
-  double w = c * d;
-  double err = std::fma(-c, d, w);
-  double f = std::fma(a, b, -w);
+static inline Real orient(Real ax, Real ay, Real bx, Real by, Real cx, Real cy) {
+  Real a = bx - ax, b = cy - ay, c = by - ay, d = cx - ax;
+  Real w = c * d;
+  Real err = r_fma(-c, d, w);
+  Real f = r_fma(a, b, -w);
   return f + err;
 }
 // engine/engine.cuh:290-292: closed bbox test for a point known to be collinear
-static inline bool onseg(double ax, double ay, double bx, double by, double px, double py) {
-  return px >= std::fmin(ax, bx) && px <= std::fmax(ax, bx) && py >= std::fmin(ay, by) && py <= std::fmax(ay, by);
+static inline bool onseg(Real ax, Real ay, Real bx, Real by, Real px, Real py) {
+  return px >= r_min(ax, bx) && px <= r_max(ax, bx) && py >= r_min(ay, by) && py <= r_max(ay, by);
 }
 // engine/engine.cuh:298-304: p left of a->b, with p displaced by (eps, eps^2) when exactly on the line
-static inline bool left_of(double ax, double ay, double bx, double by, double px, double py) {
-  double o = orient(ax, ay, bx, by, px, py);
+static inline bool left_of(Real ax, Real ay, Real bx, Real by, Real px, Real py) {
+  Real o = orient(ax, ay, bx, by, px, py);
   if (o != 0) return o > 0;
-  double dy = by - ay;
+  Real dy = by - ay;
   if (dy != 0) return dy < 0;
   return bx > ax;
 }
 // engine/engine.cuh:307-316: closed-segment intersection (touching counts)
-static inline bool seg_isect(double p0x, double p0y, double p1x, double p1y, double q0x, double q0y, double q1x, double q1y) {
-  double d1 = orient(q0x, q0y, q1x, q1y, p0x, p0y), d2 = orient(q0x, q0y, q1x, q1y, p1x, p1y);
-  double d3 = orient(p0x, p0y, p1x, p1y, q0x, q0y), d4 = orient(p0x, p0y, p1x, p1y, q1x, q1y);
+static inline bool seg_isect(Real p0x, Real p0y, Real p1x, Real p1y, Real q0x, Real q0y, Real q1x, Real q1y) {
+  Real d1 = orient(q0x, q0y, q1x, q1y, p0x, p0y), d2 = orient(q0x, q0y, q1x, q1y, p1x, p1y);
+  Real d3 = orient(p0x, p0y, p1x, p1y, q0x, q0y), d4 = orient(p0x, p0y, p1x, p1y, q1x, q1y);
   bool straddle_q = (d1 > 0 && d2 < 0) || (d1 < 0 && d2 > 0);
   bool straddle_p = (d3 > 0 && d4 < 0) || (d3 < 0 && d4 > 0);
   if (straddle_q && straddle_p) return true;
@@ -225,20 +264,20 @@ struct Params {   // engine/engine.cuh:517-518 (RefineParams)
   int want_area = 1, want_relate = 0, snap_half_ulp = 0;
 };
 
-struct Edge { double x0, y0, x1, y1; int syn, cut; };   // engine/engine.cuh:356
-struct Box { double x0, y0, x1, y1; };
+struct Edge { Real x0, y0, x1, y1; int syn, cut; };   // engine/engine.cuh:356
+struct Box { Real x0, y0, x1, y1; };
 
 // a geometry seen from the frame of another geometry's origin (engine/engine.cuh:265-279)
 struct View {
-  const Layer* L; uint64_t g; double shx, shy;
-  void v(uint64_t i, double& x, double& y) const { x = shx + L->q[2 * i]; y = shy + L->q[2 * i + 1]; }
+  const Layer* L; uint64_t g; Real shx, shy;
+  void v(uint64_t i, Real& x, Real& y) const { x = shx + L->q[2 * i]; y = shy + L->q[2 * i + 1]; }
   int syn(uint64_t i) const { return L->is_syn(i) ? 1 : 0; }
   uint64_t r0() const { return L->gro[g]; }
   uint64_t r1() const { return L->gro[g + 1]; }
 };
 
-static inline bool edge_outside(double px, double py, double qx, double qy, const Box& o) {   // engine/engine.cuh:329-331
-  return std::fmax(px, qx) < o.x0 || std::fmin(px, qx) > o.x1 || std::fmax(py, qy) < o.y0 || std::fmin(py, qy) > o.y1;
+static inline bool edge_outside(Real px, Real py, Real qx, Real qy, const Box& o) {   // engine/engine.cuh:329-331
+  return r_max(px, qx) < o.x0 || r_min(px, qx) > o.x1 || r_max(py, qy) < o.y0 || r_min(py, qy) > o.y1;
 }
 
 // engine/engine.cuh:358-377 without the capacity limit: edges meeting the box, in ring order
@@ -257,13 +296,13 @@ static void collect(const View& A, const Box& ov, std::vector<Edge>& out) {
 }
 
 // engine/engine.cuh:382-397: parity of the segment from a point left of the geometry's bbox to m, over every edge
-static bool pip_full(const View& B, double mx, double my) {
-  double ox = B.shx - 1.0, oy = my;
+static bool pip_full(const View& B, Real mx, Real my) {
+  Real ox = B.shx - 1.0, oy = my;
   bool par = false;
   for (uint64_t r = B.r0(); r < B.r1(); r++) {
     uint64_t s = B.L->rvo[r], e = B.L->rvo[r + 1];
     for (uint64_t j = s + 1; j < e; j++) {
-      double x0, y0, x1, y1; B.v(j - 1, x0, y0); B.v(j, x1, y1);
+      Real x0, y0, x1, y1; B.v(j - 1, x0, y0); B.v(j, x1, y1);
       if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;
       bool so = left_of(x0, y0, x1, y1, ox, oy), sc = left_of(x0, y0, x1, y1, mx, my);
       if (so == sc) continue;
@@ -275,11 +314,11 @@ static bool pip_full(const View& B, double mx, double my) {
 }
 
 // engine/engine.cuh:400-412: inside(x) = inside(c) xor parity of segment c->x across the local edges
-static bool inside_from_c(const std::vector<Edge>& L, double cx, double cy, bool in_c, double x, double y) {
+static bool inside_from_c(const std::vector<Edge>& L, Real cx, Real cy, bool in_c, Real x, Real y) {
   bool in = in_c;
-  double sx0 = std::fmin(cx, x), sx1 = std::fmax(cx, x), sy0 = std::fmin(cy, y), sy1 = std::fmax(cy, y);
+  Real sx0 = r_min(cx, x), sx1 = r_max(cx, x), sy0 = r_min(cy, y), sy1 = r_max(cy, y);
   for (const Edge& e : L) {
-    if (std::fmax(e.x0, e.x1) < sx0 || std::fmin(e.x0, e.x1) > sx1 || std::fmax(e.y0, e.y1) < sy0 || std::fmin(e.y0, e.y1) > sy1) continue;
+    if (r_max(e.x0, e.x1) < sx0 || r_min(e.x0, e.x1) > sx1 || r_max(e.y0, e.y1) < sy0 || r_min(e.y0, e.y1) > sy1) continue;
     bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
     if (sc == sx) continue;
     bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
@@ -293,56 +332,56 @@ static bool inside_from_c(const std::vector<Edge>& L, double cx, double cy, bool
 // real. Each edge's partial sums start at zero; the caller adds the per-edge partials in list order. (The reference
 // accumulates per lane and reduces across a warp, an order with no canonical form; this per-edge order is the one
 // the HIP engine uses, so both produce bit-identical sums.)
-struct Contrib { double acc = 0, lin = 0, lon = 0; uint32_t bits = 0; };
-static Contrib edge_contrib(double px, double py, double qx, double qy, const std::vector<Edge>& L, double cx, double cy, bool in_c,
-                            const Box& ov, double tol, double eps_par, bool real, bool rel, int cut, double snap) {
+struct Contrib { Real acc = 0, lin = 0, lon = 0; uint32_t bits = 0; };
+static Contrib edge_contrib(Real px, Real py, Real qx, Real qy, const std::vector<Edge>& L, Real cx, Real cy, bool in_c,
+                            const Box& ov, Real tol, Real eps_par, bool real, bool rel, int cut, Real snap) {
   Contrib out;
-  const double dx = qx - px, dy = qy - py;
-  const double len2 = dx * dx + dy * dy;
+  const Real dx = qx - px, dy = qy - py;
+  const Real len2 = dx * dx + dy * dy;
   if (len2 == 0) return out;
-  const double tol2len2 = tol * tol * len2, eps2 = eps_par * eps_par;
-  double ts[MAXS]; int ns = 0;
-  double ol0[MAXO], ol1[MAXO]; bool osame[MAXO]; int oidx[MAXO]; int no = 0;
+  const Real tol2len2 = tol * tol * len2, eps2 = eps_par * eps_par;
+  Real ts[MAXS]; int ns = 0;
+  Real ol0[MAXO], ol1[MAXO]; bool osame[MAXO]; int oidx[MAXO]; int no = 0;
   ts[ns++] = 0; ts[ns++] = 1;
-  const double bx0 = std::fmin(px, qx) - tol, bx1 = std::fmax(px, qx) + tol, by0 = std::fmin(py, qy) - tol, by1 = std::fmax(py, qy) + tol;
-  const double snap2 = snap * snap;
+  const Real bx0 = r_min(px, qx) - tol, bx1 = r_max(px, qx) + tol, by0 = r_min(py, qy) - tol, by1 = r_max(py, qy) + tol;
+  const Real snap2 = snap * snap;
   const int cnt = (int)L.size();
   for (int k = 0; k < cnt; k++) {
     const Edge& e = L[k];
-    if (std::fmax(e.x0, e.x1) < bx0 || std::fmin(e.x0, e.x1) > bx1 || std::fmax(e.y0, e.y1) < by0 || std::fmin(e.y0, e.y1) > by1) continue;
-    double fx = e.x1 - e.x0, fy = e.y1 - e.y0;
-    double r = dx * fy - dy * fx;
-    double flen2 = fx * fx + fy * fy;
-    double c0 = orient(px, py, qx, qy, e.x0, e.y0), c1 = orient(px, py, qx, qy, e.x1, e.y1);
+    if (r_max(e.x0, e.x1) < bx0 || r_min(e.x0, e.x1) > bx1 || r_max(e.y0, e.y1) < by0 || r_min(e.y0, e.y1) > by1) continue;
+    Real fx = e.x1 - e.x0, fy = e.y1 - e.y0;
+    Real r = dx * fy - dy * fx;
+    Real flen2 = fx * fx + fy * fy;
+    Real c0 = orient(px, py, qx, qy, e.x0, e.y0), c1 = orient(px, py, qx, qy, e.x1, e.y1);
     bool col = (c0 * c0 <= snap2 * len2 || ((e.cut & 1) && c0 * c0 <= tol2len2)) &&
                (c1 * c1 <= snap2 * len2 || ((e.cut & 2) && c1 * c1 <= tol2len2));
     if (!col && flen2 > 0) {
-      double g0 = orient(e.x0, e.y0, e.x1, e.y1, px, py), g1 = orient(e.x0, e.y0, e.x1, e.y1, qx, qy), tf = tol * tol * flen2;
+      Real g0 = orient(e.x0, e.y0, e.x1, e.y1, px, py), g1 = orient(e.x0, e.y0, e.x1, e.y1, qx, qy), tf = tol * tol * flen2;
       col = (g0 * g0 <= snap2 * flen2 || ((cut & 1) && g0 * g0 <= tf)) && (g1 * g1 <= snap2 * flen2 || ((cut & 2) && g1 * g1 <= tf));
     }
     if (col) {
-      double tc = ((e.x0 - px) * dx + (e.y0 - py) * dy) / len2, te = ((e.x1 - px) * dx + (e.y1 - py) * dy) / len2;
-      double lo = std::fmax(0.0, std::fmin(tc, te)), hi = std::fmin(1.0, std::fmax(tc, te));
+      Real tc = ((e.x0 - px) * dx + (e.y0 - py) * dy) / len2, te = ((e.x1 - px) * dx + (e.y1 - py) * dy) / len2;
+      Real lo = r_max(0.0, r_min(tc, te)), hi = r_min(1.0, r_max(tc, te));
       if (hi > lo) {
         if (lo > 0 && ns < MAXS) ts[ns++] = lo;
         if (hi < 1 && ns < MAXS) ts[ns++] = hi;
         if (no < MAXO) { ol0[no] = lo; ol1[no] = hi; osame[no] = (dx * fx + dy * fy) > 0; oidx[no] = k; no++; }
       }
     } else if (r * r > eps2 * len2 * flen2) {
-      double wx = e.x0 - px, wy = e.y0 - py;
-      double tn = wx * fy - wy * fx;
+      Real wx = e.x0 - px, wy = e.y0 - py;
+      Real tn = wx * fy - wy * fx;
       if (r > 0 ? (tn <= 0 || tn >= r) : (tn >= 0 || tn <= r)) continue;
-      double t = tn / r, u = (wx * dy - wy * dx) / r;
-      double du = tol / (flen2 > 0 ? std::sqrt(flen2) : 1.0);
+      Real t = tn / r, u = (wx * dy - wy * dx) / r;
+      Real du = tol / (flen2 > 0 ? r_sqrt(flen2) : 1.0);
       if (u >= -du && u <= 1 + du && ns < MAXS) ts[ns++] = t;
     }
   }
-  for (int a = 1; a < ns; a++) { double v = ts[a]; int b = a - 1; while (b >= 0 && ts[b] > v) { ts[b + 1] = ts[b]; b--; } ts[b + 1] = v; }
+  for (int a = 1; a < ns; a++) { Real v = ts[a]; int b = a - 1; while (b >= 0 && ts[b] > v) { ts[b + 1] = ts[b]; b--; } ts[b + 1] = v; }
   for (int k = 0; k + 1 < ns; k++) {
-    double t0 = ts[k], t1 = ts[k + 1];
+    Real t0 = ts[k], t1 = ts[k + 1];
     if (!(t1 > t0)) continue;
-    double tm = 0.5 * (t0 + t1);
-    double mx = px + dx * tm, my = py + dy * tm;
+    Real tm = 0.5 * (t0 + t1);
+    Real mx = px + dx * tm, my = py + dy * tm;
     bool in_ov = mx >= ov.x0 && mx <= ov.x1 && my >= ov.y0 && my <= ov.y1;
     int net = 0, first = -1, net_real = 0, n_real = 0;
     for (int o = 0; o < no; o++)
@@ -354,12 +393,12 @@ static Contrib edge_contrib(double px, double py, double qx, double qy, const st
     int vin = -1;
     auto inside_m = [&]() { if (vin < 0) vin = (in_ov && inside_from_c(L, cx, cy, in_c, mx, my)) ? 1 : 0; return vin; };
     if (rel && real) {
-      double seg = std::sqrt(len2) * (t1 - t0);
+      Real seg = r_sqrt(len2) * (t1 - t0);
       if (net_real != 0) { out.lon += seg; out.bits |= (net_real > 0 ? REL_COIN_SAME : REL_COIN_OPP) | REL_TOUCH; }
       else if (n_real == 0 && (net == 1 || net == -1)) out.lin += 0.5 * seg;
       else if (inside_m()) out.lin += seg;
     }
-    double w;
+    Real w;
     if (first < 0) w = inside_m() ? 1.0 : 0.0;
     else if (net == 1 || net == -1) w = 0.5;
     else if (net == 0) w = inside_m() ? 1.0 : 0.0;
@@ -373,7 +412,7 @@ static Contrib edge_contrib(double px, double py, double qx, double qy, const st
       w = 0.5 * (v + other);
     }
     if (w != 0.0) {
-      double ax = px + dx * t0 - cx, ay = py + dy * t0 - cy, bx = px + dx * t1 - cx, by = py + dy * t1 - cy;
+      Real ax = px + dx * t0 - cx, ay = py + dy * t0 - cy, bx = px + dx * t1 - cx, by = py + dy * t1 - cy;
       out.acc += w * (ax * by - ay * bx);
     }
   }
@@ -381,13 +420,13 @@ static Contrib edge_contrib(double px, double py, double qx, double qy, const st
 }
 
 // engine/engine.cuh:524-527: half an ulp of the largest world coordinate of the pair
-static inline double pair_snap(double ox, double oy, double ex, double ey) {
-  double m = std::fmax(std::fmax(std::fabs(ox), std::fabs(ox + ex)), std::fmax(std::fabs(oy), std::fabs(oy + ey)));
-  return 0.5 * (std::nextafter(m, INFINITY) - m);
+static inline Real pair_snap(Real ox, Real oy, Real ex, Real ey) {
+  Real m = r_max(r_max(r_abs(ox), r_abs(ox + ex)), r_max(r_abs(oy), r_abs(oy + ey)));
+  return 0.5 * (Real(std::nextafter(val(m), INFINITY)) - m);
 }
 
 // probe vertex of ring r (engine/engine.cuh:322-327): first vertex that starts a real edge
-static inline bool probe_vertex(const View& A, uint64_t r, double& x, double& y) {
+static inline bool probe_vertex(const View& A, uint64_t r, Real& x, Real& y) {
   uint64_t v = A.L->rvo[r], e = A.L->rvo[r + 1];
   if (A.L->has_syn()) { while (v + 1 < e && A.syn(v)) v++; if (v + 1 >= e) return false; }
   A.v(v, x, y);
@@ -400,20 +439,20 @@ struct PairOut { uint8_t isect; double area; double ev[4]; uint32_t bits; };
 static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t b, const Params& rp) {
   std::vector<Edge> LA, LB;
   PairOut out{0, 0.0, {0, 0, 0, 0}, 0};
-  const double oax = P.ox[a], oay = P.oy[a], obx = R.ox[b], oby = R.oy[b];
-  const double eax = P.ex[a], eay = P.ey[a], ebx = R.ex[b], eby = R.ey[b];
-  const double sx = obx - oax, sy = oby - oay;
-  const double pad = 4.0 * std::fmax(rp.tol_abs, rp.tol_rel * std::fmax(std::fmax(eax, eay), std::fmax(ebx, eby)));
-  Box ov{std::fmax(0.0, sx) - pad, std::fmax(0.0, sy) - pad, std::fmin(eax, sx + ebx) + pad, std::fmin(eay, sy + eby) + pad};
+  const Real oax = P.ox[a], oay = P.oy[a], obx = R.ox[b], oby = R.oy[b];
+  const Real eax = P.ex[a], eay = P.ey[a], ebx = R.ex[b], eby = R.ey[b];
+  const Real sx = obx - oax, sy = oby - oay;
+  const Real pad = 4.0 * r_max(rp.tol_abs, rp.tol_rel * r_max(r_max(eax, eay), r_max(ebx, eby)));
+  Box ov{r_max(0.0, sx) - pad, r_max(0.0, sy) - pad, r_min(eax, sx + ebx) + pad, r_min(eay, sy + eby) + pad};
   View A{&P, a, oax - oax, oay - oay}, B{&R, b, obx - oax, oby - oay};
-  const double tol = std::fmax(rp.tol_abs, rp.tol_rel * std::fmax(ov.x1 - ov.x0, ov.y1 - ov.y0));
-  const double snap = rp.snap_half_ulp ? pair_snap(oax, oay, eax, eay) : 0.0;
+  const Real tol = r_max(rp.tol_abs, rp.tol_rel * r_max(ov.x1 - ov.x0, ov.y1 - ov.y0));
+  const Real snap = rp.snap_half_ulp ? pair_snap(oax, oay, eax, eay) : 0.0;
   collect(A, ov, LA); collect(B, ov, LB);
   bool hit = false, hit_any = false;
   for (const Edge& ea : LA) {
-    double ax0 = std::fmin(ea.x0, ea.x1), ax1 = std::fmax(ea.x0, ea.x1), ay0 = std::fmin(ea.y0, ea.y1), ay1 = std::fmax(ea.y0, ea.y1);
+    Real ax0 = r_min(ea.x0, ea.x1), ax1 = r_max(ea.x0, ea.x1), ay0 = r_min(ea.y0, ea.y1), ay1 = r_max(ea.y0, ea.y1);
     for (const Edge& eb : LB) {
-      if (std::fmax(eb.x0, eb.x1) < ax0 || std::fmin(eb.x0, eb.x1) > ax1 || std::fmax(eb.y0, eb.y1) < ay0 || std::fmin(eb.y0, eb.y1) > ay1) continue;
+      if (r_max(eb.x0, eb.x1) < ax0 || r_min(eb.x0, eb.x1) > ax1 || r_max(eb.y0, eb.y1) < ay0 || r_min(eb.y0, eb.y1) > ay1) continue;
       if (seg_isect(ea.x0, ea.y0, ea.x1, ea.y1, eb.x0, eb.y0, eb.x1, eb.y1)) {
         hit_any = true;
         if (!(ea.syn | eb.syn)) { hit = true; break; }
@@ -422,7 +461,7 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
     if (hit) break;
   }
   const bool rtouch = hit;
-  const double cx = ov.x0 + 0.381966 * (ov.x1 - ov.x0), cy = ov.y0 + 0.414213 * (ov.y1 - ov.y0);
+  const Real cx = ov.x0 + 0.381966 * (ov.x1 - ov.x0), cy = ov.y0 + 0.414213 * (ov.y1 - ov.y0);
   bool inB_c = false, inA_c = false;
   if (!hit || rp.want_area || rp.want_relate) { inB_c = pip_full(B, cx, cy); inA_c = pip_full(A, cx, cy); }
   bool contain_any = false;
@@ -430,7 +469,7 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
     bool h = false, ha = false;
     auto probe = [&](const View& S, const std::vector<Edge>& Lother, bool in_other) {
       for (uint64_t r = S.r0(); r < S.r1() && !h; r++) {
-        double x, y; bool real = probe_vertex(S, r, x, y);
+        Real x, y; bool real = probe_vertex(S, r, x, y);
         if (!real) S.v(S.L->rvo[r], x, y);
         if (x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(Lother, cx, cy, in_other, x, y)) { ha = true; if (real) h = true; }
       }
@@ -439,9 +478,9 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
     probe(B, LA, inA_c);
     hit = h; contain_any = ha;
   }
-  double ar = 0.0, e4[4] = {0, 0, 0, 0}; uint32_t eb4 = 0;
+  Real ar = 0.0, e4[4] = {0, 0, 0, 0}; uint32_t eb4 = 0;
   if ((hit || hit_any || contain_any) && (rp.want_area || rp.want_relate)) {
-    double s2 = 0.0, ai = 0, ao = 0, bi = 0, bo = 0; uint32_t bits = rtouch ? REL_TOUCH : 0;
+    Real s2 = 0.0, ai = 0, ao = 0, bi = 0, bo = 0; uint32_t bits = rtouch ? REL_TOUCH : 0;
     const bool rel = rp.want_relate != 0;
     for (const Edge& e : LA) {
       Contrib c = edge_contrib(e.x0, e.y0, e.x1, e.y1, LB, cx, cy, inB_c, ov, tol, rp.eps_par, !e.syn, rel, e.cut, snap);
@@ -457,8 +496,8 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
   } else if (rtouch) eb4 = REL_TOUCH;
   bool ev_any = e4[0] != 0 || e4[1] != 0 || e4[2] != 0 || e4[3] != 0;
   out.isect = hit ? 1 : ((ar != 0.0 || ev_any) ? 2 : 0);
-  out.area = ar;
-  for (int k = 0; k < 4; k++) out.ev[k] = e4[k];
+  out.area = val(ar);
+  for (int k = 0; k < 4; k++) out.ev[k] = val(e4[k]);
   out.bits = eb4;
   return out;
 }
@@ -542,18 +581,18 @@ static void join(const Layer& P, const Layer& R, const Params& rp, bool upper, b
 // ------------------------------------------------------------------------------------------- points in polygons
 // engine/engine.cuh:1079-1106: 1 = interior, 2 = on a real edge, 0 = outside (the piece)
 static uint8_t pip_pair(const Layer& pts, const Layer& R, uint32_t pi, uint32_t g) {
-  const double ox_ = R.ox[g], oy_ = R.oy[g];
-  const double px = pts.xw[2 * pi] - ox_, py = pts.xw[2 * pi + 1] - oy_;
+  const Real ox_ = R.ox[g], oy_ = R.oy[g];
+  const Real px = pts.xw[2 * pi] - ox_, py = pts.xw[2 * pi + 1] - oy_;
   View B{&R, g, ox_ - ox_, oy_ - oy_};
-  const double ox = B.shx - 1.0;
+  const Real ox = B.shx - 1.0;
   bool par = false, on = false;
   for (uint64_t r = B.r0(); r < B.r1() && !on; r++) {
     uint64_t s0 = R.rvo[r], e = R.rvo[r + 1];
-    double x0, y0; B.v(s0, x0, y0);
+    Real x0, y0; B.v(s0, x0, y0);
     for (uint64_t j = s0 + 1; j < e; j++) {
-      double x1, y1; B.v(j, x1, y1);
+      Real x1, y1; B.v(j, x1, y1);
       if (!((y0 > py) == (y1 > py) && y0 != py && y1 != py)) {
-        double o_ = orient(x0, y0, x1, y1, px, py);
+        Real o_ = orient(x0, y0, x1, y1, px, py);
         if (o_ == 0 && !B.syn(j - 1) && onseg(x0, y0, x1, y1, px, py)) { on = true; break; }
         bool so = left_of(x0, y0, x1, y1, ox, py), sp = left_of(x0, y0, x1, y1, px, py);
         if (so != sp) {
@@ -749,12 +788,23 @@ using namespace orc;
 
 extern "C" {
 
-double orc_orient(double ax, double ay, double bx, double by, double cx, double cy) { return orient(ax, ay, bx, by, cx, cy); }
+double orc_orient(double ax, double ay, double bx, double by, double cx, double cy) { return val(orient(ax, ay, bx, by, cx, cy)); }
 int orc_left_of(double ax, double ay, double bx, double by, double px, double py) { return left_of(ax, ay, bx, by, px, py); }
 int orc_seg_isect(double a, double b, double c, double d, double e, double f, double g, double h) { return seg_isect(a, b, c, d, e, f, g, h); }
 float orc_f_down(double v) { return f_down(v); }
 float orc_f_up(double v) { return f_up(v); }
 uint32_t orc_de9im_code(const double* ev, uint32_t bits, double la, double lb) { return de9im_code(ev, bits, la, lb); }
+
+// FLOP counters of this thread (counting build only; zeros otherwise): add/sub, mul, fma, div, sqrt
+void orc_flops(uint64_t* out, int reset) {
+#ifdef ORC_COUNT_FLOPS
+  out[0] = g_fc.add; out[1] = g_fc.mul; out[2] = g_fc.fma; out[3] = g_fc.div; out[4] = g_fc.sqrt;
+  if (reset) g_fc = FlopCount();
+#else
+  (void)reset;
+  for (int k = 0; k < 5; k++) out[k] = 0;
+#endif
+}
 
 void orc_set_threads(int t) {
 #ifdef _OPENMP

@@ -10,26 +10,29 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+FLOPS_LIB_PATH = os.path.join(_HERE, "build", "liboracle_flops.so")
 _lib = None
+_flib = None
 
 
 def build():
     subprocess.run(["make", "-C", _HERE], check=True, capture_output=True)
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def load(flops=False):
+    global _lib, _flib
+    if (_flib if flops else _lib) is not None:
+        return _flib if flops else _lib
+    path = FLOPS_LIB_PATH if flops else LIB_PATH
+    if not os.path.exists(path):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     d, u64, u32, i32, vp = C.c_double, C.c_uint64, C.c_uint32, C.c_int, C.c_void_p
     dp, u64p, u32p, u8p, fp = C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.POINTER(C.c_float)
     sig = {
         "orc_orient": (d, [d] * 6), "orc_left_of": (i32, [d] * 6), "orc_seg_isect": (i32, [d] * 8),
         "orc_f_down": (C.c_float, [d]), "orc_f_up": (C.c_float, [d]), "orc_de9im_code": (u32, [dp, u32, d, d]),
-        "orc_set_threads": (None, [i32]),
+        "orc_set_threads": (None, [i32]), "orc_flops": (None, [u64p, i32]),
         "orc_layer_polys": (vp, [u64p, u64p, dp, u64, u64, u64, u32p, u8p]), "orc_layer_points": (vp, [dp, u64]),
         "orc_layer_free": (None, [vp]), "orc_layer_export": (None, [vp, dp, dp, fp, dp, dp]), "orc_layer_nparent": (u32, [vp]),
         "orc_result_new": (vp, []), "orc_result_free": (None, [vp]), "orc_result_rows": (u64, [vp]), "orc_result_candidates": (u64, [vp]),
@@ -46,7 +49,11 @@ def load():
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype, f.argtypes = res, args
-    _lib = lib
+    if flops:
+        lib.orc_set_threads(1)   # counters are per thread
+        _flib = lib
+    else:
+        _lib = lib
     return lib
 
 
@@ -231,3 +238,43 @@ def dedup(hs):
         return first[:un], moff, midx[:n]
     finally:
         lib.orc_dedup_free(h)
+
+
+FLOP_KINDS = ("add", "mul", "fma", "div", "sqrt")
+
+
+def refine_flops(hs_P, hs_R, pairs, want_area=True, want_relate=False):
+    """Algorithmic FLOPs of refining `pairs` (counting build of the oracle, one thread): dict of add/sub, mul, fma,
+    div, sqrt counts and `flops` = add + mul + 2 fma + div + sqrt."""
+    lib = load(flops=True)
+    P = _FlopLayer(lib, hs_P)
+    R = P if hs_R is hs_P else _FlopLayer(lib, hs_R)
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    n = len(pairs)
+    isect, area, ev, evb = np.zeros(n, np.uint8), np.zeros(n), np.zeros((n, 4)), np.zeros(n, np.uint32)
+    cnt = np.zeros(5, np.uint64)
+    lib.orc_flops(_p(cnt, C.c_uint64), 1)
+    lib.orc_refine(P.h, R.h, _p(pairs, C.c_uint32), n, int(want_area), int(want_relate), 0, _p(isect, C.c_uint8), _p(area, C.c_double),
+                   _p(ev, C.c_double), _p(evb, C.c_uint32))
+    lib.orc_flops(_p(cnt, C.c_uint64), 1)
+    out = dict(zip(FLOP_KINDS, (int(v) for v in cnt)))
+    out["flops"] = out["add"] + out["mul"] + 2 * out["fma"] + out["div"] + out["sqrt"]
+    out["pairs"] = n
+    return out
+
+
+class _FlopLayer:
+    def __init__(self, lib, hs):
+        self.lib = lib
+        gro, rvo = np.ascontiguousarray(hs.gro, np.uint64), np.ascontiguousarray(hs.rvo, np.uint64)
+        xy = np.ascontiguousarray(hs.xy, np.float64).reshape(-1, 2)
+        par = None if hs.parent is None else np.ascontiguousarray(hs.parent, np.uint32)
+        syn = None if hs.syn is None else np.ascontiguousarray(hs.syn, np.uint8)
+        self.h = lib.orc_layer_polys(_p(gro, C.c_uint64), _p(rvo, C.c_uint64), _p(xy, C.c_double), len(gro) - 1, len(rvo) - 1, len(xy),
+                                     _p(par, C.c_uint32), _p(syn, C.c_uint8))
+
+    def __del__(self):
+        try:
+            self.lib.orc_layer_free(self.h)
+        except Exception:
+            pass

@@ -45,6 +45,7 @@ void set_device(const gsj::Context& c) { GSJ_HIP(hipSetDevice(c.device)); }
 
 namespace gsj {
 Context::~Context() {
+  for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 }  // namespace gsj
@@ -70,6 +71,8 @@ int gsj_ctx_create(int device, gsj_ctx** out) {
       delete ctx;
       throw gsj::Error(GSJ_E_HIP, "hipStreamCreate failed");
     }
+    for (hipEvent_t& e : ctx->c.ev)
+      if (hipEventCreate(&e) != hipSuccess) { delete ctx; throw gsj::Error(GSJ_E_HIP, "hipEventCreate failed"); }
     *out = ctx;
   });
 }

@@ -81,6 +81,7 @@ struct Context {
       s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;
   int cu_count = 256;
+  hipEvent_t ev[8] = {};   // kernel timing events (created with the context)
   ~Context();
 };
 
@@ -118,9 +119,10 @@ struct RefineOut {
   double4* ev;      // [np] (A in, A on, B in, B on) or nullptr
   uint32_t* evb;    // [np] evidence bits or nullptr
 };
-// refine np candidate pairs (pa[i], pb[i]); returns the number of pairs that needed global-memory edge lists
+// refine np candidate pairs (pa[i], pb[i]); returns the number of pairs that needed global-memory edge lists.
+// Kernel durations and per-class pair counts are added to *st (if given).
 uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
-                      const RefineOut& out);
+                      const RefineOut& out, gsj_join_stats* st = nullptr);
 
 // ------------------------------------------------------------------------------------------- join driver
 struct JoinHooks {

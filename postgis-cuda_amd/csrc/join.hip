@@ -235,7 +235,7 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
       ro.area = rp.want_area ? c.s_area.as<double>(np + 16) : nullptr;
       ro.ev = rp.want_relate ? c.s_ev.as<double4>(np + 16) : nullptr;
       ro.evb = rp.want_relate ? c.s_evb.as<uint32_t>(np + 16) : nullptr;
-      s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro);
+      s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro, &s);
       GSJ_HIP(hipStreamSynchronize(stream));
       s.refine_ms += t2.ms();
       Timer t3;

@@ -453,7 +453,7 @@ __global__ void k_classify(const DPolys P, const DPolys R, const uint32_t* __res
 }  // namespace
 
 uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
-                      const RefineOut& out) {
+                      const RefineOut& out, gsj_join_stats* stats) {
   if (np == 0) return 0;
   GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "refine: more than 2^32 pairs in one launch");
   hipStream_t st = c.stream;
@@ -468,30 +468,49 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_LAUNCHED();
   const int T = 256;
   const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
+  GSJ_HIP(hipEventRecord(c.ev[0], st));
   {   // 16-lane groups: 16 pairs per block
     const size_t sm = (size_t)(T / 16) * scratch_bytes(CAP_SMALL, rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 16), max_blocks);
     hipLaunchKernelGGL(k_refine<16>, dim3(g), dim3(T), sm, st, P, R, pa, pb, small, 0, rp, out, CAP_SMALL, ovf, ctl);
     GSJ_LAUNCHED();
   }
+  GSJ_HIP(hipEventRecord(c.ev[1], st));
   {   // 64-lane groups: 4 pairs per block
     const size_t sm = (size_t)(T / 64) * scratch_bytes(CAP_LARGE, rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), max_blocks);
     hipLaunchKernelGGL(k_refine<64>, dim3(g), dim3(T), sm, st, P, R, pa, pb, large, 1, rp, out, CAP_LARGE, ovf, ctl);
     GSJ_LAUNCHED();
   }
+  GSJ_HIP(hipEventRecord(c.ev[2], st));
   OverflowCtl* h = c.h_scal.as<OverflowCtl>(2);
   GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
   const uint32_t nover = h->count, need = h->need;
+  if (stats) {
+    float ms_s = 0, ms_l = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
+    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[1], c.ev[2]));
+    stats->k_small_ms += ms_s; stats->k_large_ms += ms_l;
+    stats->n_small += h->n_small; stats->n_large += h->n_large; stats->n_over += nover;
+    stats->launches++;
+  }
   if (nover) {
     const size_t per = scratch_bytes((int)need, rel);
     const uint64_t budget = 1ull << 30;
     uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nover, budget / std::max<size_t>(per, 1)));
     waves = (waves + 3) / 4 * 4;
     unsigned char* slab = c.s_heavy_edges.as<unsigned char>(per * waves + 64);
+    GSJ_HIP(hipEventRecord(c.ev[3], st));
     hipLaunchKernelGGL(k_refine_overflow, dim3((uint32_t)(waves / 4)), dim3(T), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
     GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[4], st));
+    if (stats) {
+      GSJ_HIP(hipEventSynchronize(c.ev[4]));
+      float ms_o = 0;
+      GSJ_HIP(hipEventElapsedTime(&ms_o, c.ev[3], c.ev[4]));
+      stats->k_over_ms += ms_o;
+    }
   }
   return nover;
 }

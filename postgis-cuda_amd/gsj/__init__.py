@@ -37,7 +37,9 @@ class JoinOpts(C.Structure):
 
 class JoinStats(C.Structure):
     _fields_ = [("traverse_ms", C.c_double), ("refine_ms", C.c_double), ("compact_ms", C.c_double), ("copy_ms", C.c_double),
-                ("candidates", C.c_uint64), ("hits", C.c_uint64), ("heavy", C.c_uint64), ("batches", C.c_uint64)]
+                ("candidates", C.c_uint64), ("hits", C.c_uint64), ("heavy", C.c_uint64), ("batches", C.c_uint64),
+                ("k_small_ms", C.c_double), ("k_large_ms", C.c_double), ("k_over_ms", C.c_double),
+                ("n_small", C.c_uint64), ("n_large", C.c_uint64), ("n_over", C.c_uint64), ("launches", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -347,11 +349,14 @@ class Context:
         self._layers.add(L)
         return L.build_bvh() if build_bvh else L
 
-    def join(self, P: Layer, R: Layer, **kw) -> Rows:
+    def join(self, P: Layer, R: Layer, collect=True, **kw) -> Rows:
+        """Rows of the join; collect=False delivers them to a no-op callback (counts and stats only)."""
         o = _opts(**kw)
         chunks = []
 
         def cb(user, pairs, area, code, n):
+            if not collect:
+                return 0
             p = np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2)
             a = np.ctypeslib.as_array(area, (n,)).copy() if area else None
             c = np.ctypeslib.as_array(code, (n,)).copy() if code else None
