@@ -39,10 +39,11 @@ def mesh(gsj, n, seed=1, subdiv=256):
     return M, gsj.subdivide(M, subdiv)
 
 
-def nedges(hs):
+def small_class(hs):
+    """Per piece: fits the small refinement class (<= 32 stored vertices and <= 8 rings)."""
     rings = np.diff(hs.gro.astype(np.int64))
     verts = hs.rvo[hs.gro[1:]].astype(np.int64) - hs.rvo[hs.gro[:-1]].astype(np.int64)
-    return verts - rings
+    return (verts <= 32) & (rings <= 8)
 
 
 def main():
@@ -93,7 +94,7 @@ def main():
         f"over {st['k_over_ms']:.2f} ms (pairs {st['n_small']}/{st['n_large']}/{st['n_over']})")
 
     # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator)
-    kern = max((("k_refine<16>", "k_small_ms", "n_small"), ("k_refine<64>", "k_large_ms", "n_large")), key=lambda k: st[k[1]])
+    kern = max((("k_refine_small", "k_small_ms", "n_small"), ("k_refine_large", "k_large_ms", "n_large")), key=lambda k: st[k[1]])
     roofline = None
     cpu = None
     parity = None
@@ -103,9 +104,9 @@ def main():
         Ms, Ss = mesh(gsj, args.sample)
         Os = O.Layer.polys(Ss)
         cand_s = O.candidates(Os, Os, upper=True)
-        ne = nedges(Ss)
-        small = np.maximum(ne[cand_s[:, 0]], ne[cand_s[:, 1]]) <= 32
-        pick = cand_s[small] if kern[0] == "k_refine<16>" else cand_s[~small]
+        sc = small_class(Ss)
+        small = sc[cand_s[:, 0]] & sc[cand_s[:, 1]]
+        pick = cand_s[small] if kern[0] == "k_refine_small" else cand_s[~small]
         rng = np.random.default_rng(0)
         pick = pick[rng.choice(len(pick), size=min(len(pick), 40000), replace=False)]
         fl = O.refine_flops(Ss, Ss, pick, want_area=False, want_relate=True)

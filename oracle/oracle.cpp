@@ -469,6 +469,7 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
     bool h = false, ha = false;
     auto probe = [&](const View& S, const std::vector<Edge>& Lother, bool in_other) {
       for (uint64_t r = S.r0(); r < S.r1() && !h; r++) {
+        if (S.L->rvo[r] == S.L->rvo[r + 1]) continue;   // an empty ring has no probe vertex
         Real x, y; bool real = probe_vertex(S, r, x, y);
         if (!real) S.v(S.L->rvo[r], x, y);
         if (x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(Lother, cx, cy, in_other, x, y)) { ha = true; if (real) h = true; }

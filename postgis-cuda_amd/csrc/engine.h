@@ -20,7 +20,7 @@ struct DPolys {
   const uint64_t* rvo = nullptr;     // [nr+1] ring -> first vertex
   const double2* xy = nullptr;       // [nv] v - origin of its geometry
   const float4* bb = nullptr;        // [n] outward-rounded float box (minx, miny, maxx, maxy), world coords
-  const uint32_t* nedge = nullptr;   // [n] number of edges (vertices - rings), for work classification
+  const uint32_t* ginfo = nullptr;   // [n] stored vertices (low 16 bits, saturated) | rings << 16, for work classes
   const uint32_t* parent = nullptr;  // [n] piece -> parent, nullptr if not subdivided
   const uint32_t* synb = nullptr;    // [(nv+31)/32] bit v: edge v -> v+1 was made by subdivision; nullptr if none
   const double* blen = nullptr;      // [nparent] real boundary length per parent

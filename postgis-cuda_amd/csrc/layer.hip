@@ -28,7 +28,7 @@ __device__ __forceinline__ int syn_of(const uint8_t* syn, uint64_t v) { return s
 
 // thread per geometry: bbox, float box, edge count, boundary length; then shift the vertices
 __global__ void k_poly_prep(uint32_t n, const uint32_t* gro, const uint64_t* rvo, double2* xy, const uint8_t* syn, double2* origin, double2* ext,
-                            float4* bb, uint32_t* nedge, double* glen) {
+                            float4* bb, uint32_t* ginfo, double* glen) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const uint32_t r0 = gro[g], r1 = gro[g + 1];
@@ -42,7 +42,7 @@ __global__ void k_poly_prep(uint32_t n, const uint32_t* gro, const uint64_t* rvo
   ext[g] = make_double2(x1 - x0, y1 - y0);
   // largest float <= lower corner, smallest float >= upper corner (engine/engine.cuh:34-35, 59)
   bb[g] = make_float4(__double2float_rd(x0), __double2float_rd(y0), __double2float_ru(x1), __double2float_ru(y1));
-  nedge[g] = (uint32_t)((v1 - v0) - (uint64_t)(r1 - r0));
+  ginfo[g] = (uint32_t)min<uint64_t>(v1 - v0, 0xFFFFu) | (uint32_t)min<uint32_t>(r1 - r0, 0xFFFFu) << 16;
   double L = 0;
   for (uint32_t r = r0; r < r1; r++)
     for (uint64_t v = rvo[r]; v + 1 < rvo[r + 1]; v++)
@@ -111,7 +111,7 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
   // sizes (pass 1) and assignment (pass 2)
   auto carve = [&](Slicer& s, DPolys& d, uint32_t*& gro32, uint8_t*& syn8, double*& glen) {
     d.origin = s.take<double2>(n); d.ext = s.take<double2>(n); gro32 = s.take<uint32_t>(n + 1); d.rvo = s.take<uint64_t>(nr + 1);
-    d.xy = s.take<double2>(nv); d.bb = s.take<float4>(n); d.nedge = s.take<uint32_t>(n);
+    d.xy = s.take<double2>(nv); d.bb = s.take<float4>(n); d.ginfo = s.take<uint32_t>(n);
     d.parent = parent ? s.take<uint32_t>(n) : nullptr;
     d.synb = syn ? s.take<uint32_t>((nv + 31) / 32) : nullptr;
     d.blen = s.take<double>(nparent);
@@ -137,7 +137,7 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
     // without parents the boundary length of a geometry is its parent's
     double* glen_out = parent ? glen : const_cast<double*>(d.blen);
     hipLaunchKernelGGL(k_poly_prep, dim3(div_up(n, 256)), dim3(256), 0, st, (uint32_t)n, gro32, d.rvo, const_cast<double2*>(d.xy), syn8,
-                       const_cast<double2*>(d.origin), const_cast<double2*>(d.ext), const_cast<float4*>(d.bb), const_cast<uint32_t*>(d.nedge),
+                       const_cast<double2*>(d.origin), const_cast<double2*>(d.ext), const_cast<float4*>(d.bb), const_cast<uint32_t*>(d.ginfo),
                        glen_out);
     GSJ_LAUNCHED();
     if (syn && nv) {

@@ -1,19 +1,24 @@
 // Exact FP64 refinement of candidate polygon pairs (reference: k_refine and the heavy path,
-// engine/engine.cuh:342-792). Per pair: local edge lists of both polygons inside the pair's overlap box, boundary
-// contact by exact segment tests, containment by crossing parity from a probe point c, and (for area / relate)
-// Green's-theorem contributions of every local edge weighted by its position relative to the other polygon.
+// engine/engine.cuh:342-792). Per pair: the edges of both polygons inside the pair's overlap box, boundary contact by
+// exact segment tests, containment by crossing parity from a probe point c, and (for area / relate) Green's-theorem
+// contributions of every such edge weighted by its position relative to the other polygon.
 //
 // MI355X design:
-//  * a lane GROUP per pair, not a 32-lane warp: 16 lanes for pairs whose polygons have <= 32 edges each (four pairs
-//    per wave64; the bulk of parcel data), a full wave64 for larger pairs. Group-local ballots are slices of the
-//    64-bit wave ballot.
-//  * edge lists live in LDS (40 B per edge); pairs whose lists exceed the LDS capacity go to a second pass with
-//    per-wave lists in global memory (sized by the largest overflow), all on the device without host round trips
-//    except one read of the overflow count.
-//  * sums are deterministic and independent of the group size: each lane stores its edges' contributions in LDS and
-//    group lanes 0..4 add them sequentially in list order (A's edges, then B's). The result is bit-identical between
-//    the 16-lane, 64-lane and overflow paths, and with the CPU restatement, so areas compare exactly and a pair gives
-//    the same bits alone or batched with other requests.
+//  * three work classes, chosen per pair from the vertex and ring counts of its polygons:
+//      small (both <= 32 stored vertices, <= 8 rings): a 16-lane group per pair (four pairs per wave64); both
+//        polygons are loaded once into LDS as vertex lists with per-vertex flags (edge exists / synthetic / cut ends /
+//        meets the overlap box), so the full-ring parity scans read LDS, not HBM; the contact test covers the whole
+//        A x B edge grid across the group; A's and B's edge contributions run in one combined pass;
+//      large: a wave64 per pair, compacted lists of the edges meeting the overlap box in LDS;
+//      overflow (lists beyond the LDS capacity): a wave64 per pair with lists in global memory sized by the largest
+//        overflow.
+//  * split points of an edge live in eight named registers kept sorted by a branch-free min/max insertion, with the
+//    overlaps in two register slots. A pair with an edge of more than six interior splits or two overlaps is handed to
+//    the overflow kernel, which runs the array path with the reference's caps (24 / 8); the two paths give the same
+//    bits wherever both apply, and the fast kernels carry no call and no scratch arrays.
+//  * deterministic sums: each edge's partial sums start at zero and are stored per edge; group lanes 0..4 add them in
+//    list order (A's edges, then B's). The result is the same in every class and bit-identical to the CPU
+//    restatement, so a pair gives the same bits alone or batched with any other request.
 //  * FP64 on the vector ALUs only; nothing here is a dense contraction, so MFMA is not used.
 #include "engine.h"
 
@@ -49,6 +54,489 @@ __device__ __forceinline__ bool edge_outside(double px, double py, double qx, do
   return fmax(px, qx) < o.x0 || fmin(px, qx) > o.x1 || fmax(py, qy) < o.y0 || fmin(py, qy) > o.y1;
 }
 
+// ------------------------------------------------------------------------------------------- edge sources
+// Both sources enumerate "the other polygon's edges meeting the overlap box" in ring order.
+struct ListSrc {   // compacted list (large / overflow classes)
+  const Ed* L;
+  int n;
+  __device__ __forceinline__ int size() const { return n; }
+  __device__ __forceinline__ bool get(int k, Ed& e) const { e = L[k]; return true; }
+};
+
+constexpr uint8_t F_VALID = 1, F_SYN = 2, F_CUT0 = 4, F_CUT1 = 8, F_INOV = 16;
+constexpr int VCAP = 32, RCAP = 8;   // small class: vertex slots and rings per polygon
+
+struct VertSrc {   // all vertices of a small polygon; edge k = (V[k], V[k+1]) exists iff F[k] & F_VALID
+  const double2* V;
+  const uint8_t* F;
+  int n;
+  __device__ __forceinline__ int size() const { return n; }
+  __device__ __forceinline__ bool get(int k, Ed& e) const {
+    const uint8_t f = F[k];
+    if ((f & (F_VALID | F_INOV)) != (F_VALID | F_INOV)) return false;
+    const double2 a = V[k], b = V[k + 1];
+    e.x0 = a.x; e.y0 = a.y; e.x1 = b.x; e.y1 = b.y;
+    e.syn = (f & F_SYN) ? 1 : 0;
+    e.cut = (f >> 2) & 3;
+    return true;
+  }
+};
+
+// inside(x) = inside(c) xor parity of segment c->x across the local edges (engine/engine.cuh:400-412)
+template <class Src>
+__device__ bool inside_from_c(const Src& L, double cx, double cy, bool in_c, double x, double y) {
+  bool in = in_c;
+  const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
+  const int n = L.size();
+  for (int k = 0; k < n; k++) {
+    Ed e;
+    if (!L.get(k, e)) continue;
+    if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) continue;
+    const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
+    if (sc == sx) continue;
+    const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
+    if (su != sv) in = !in;
+  }
+  return in;
+}
+
+struct Contrib { double acc, lin, lon; uint32_t bits; };
+constexpr uint32_t SPILL = 0x80000000u;   // Contrib.bits: the edge exceeded the register path (pair goes to overflow)
+
+struct EdgeCtx {   // what an edge's contribution needs besides the other polygon's edges
+  double cx, cy, tol, eps_par, snap;
+  Box ov;
+  bool in_c, rel;
+};
+
+// One sub-segment [t0, t1] of edge p->q: relate lengths and the Green's-theorem term (engine/engine.cuh:464-513).
+// `net`, `first_*` describe the collinear overlaps containing the midpoint.
+template <class Src>
+__device__ __forceinline__ void subsegment(const Src& L, const EdgeCtx& c, double px, double py, double dx, double dy, double len2, bool real,
+                                           double t0, double t1, int net, int net_real, int n_real, int first_idx, bool first_same,
+                                           Contrib& out) {
+  const double tm = 0.5 * (t0 + t1);
+  const double mx = px + dx * tm, my = py + dy * tm;
+  const bool in_ov = mx >= c.ov.x0 && mx <= c.ov.x1 && my >= c.ov.y0 && my <= c.ov.y1;
+  int vin = -1;   // midpoint inside the other polygon, computed on first use
+  auto inside_m = [&]() {
+    if (vin < 0) vin = (in_ov && inside_from_c(L, c.cx, c.cy, c.in_c, mx, my)) ? 1 : 0;
+    return vin;
+  };
+  if (c.rel && real) {
+    const double seg = sqrt(len2) * (t1 - t0);
+    if (net_real != 0) { out.lon += seg; out.bits |= (net_real > 0 ? REL_COIN_SAME : REL_COIN_OPP) | REL_TOUCH; }
+    else if (n_real == 0 && (net == 1 || net == -1)) out.lin += 0.5 * seg;
+    else if (inside_m()) out.lin += seg;
+  }
+  double w;
+  if (first_idx < 0) w = inside_m() ? 1.0 : 0.0;
+  else if (net == 1 || net == -1) w = 0.5;
+  else if (net == 0) w = inside_m() ? 1.0 : 0.0;
+  else {   // |net| >= 2 only for inconsistent input: derive the other side from the classified one and clamp
+    Ed eb;
+    L.get(first_idx, eb);
+    const bool sb = left_of(eb.x0, eb.y0, eb.x1, eb.y1, mx, my);
+    const bool m_left = first_same ? sb : !sb;
+    const int v = inside_m();
+    int other = m_left ? v - net : v + net;
+    other = other < 0 ? 0 : (other > 1 ? 1 : other);
+    w = 0.5 * (v + other);
+  }
+  if (w != 0.0) {
+    const double ax = px + dx * t0 - c.cx, ay = py + dy * t0 - c.cy, bx = px + dx * t1 - c.cx, by = py + dy * t1 - c.cy;
+    out.acc += w * (ax * by - ay * bx);
+  }
+}
+
+// Classification of one other-polygon edge against edge p->q (engine/engine.cuh:427-461): returns 0 (nothing),
+// 1 (collinear overlap [lo, hi] in edge parameter, same direction in *same), 2 (proper split at *t).
+__device__ __forceinline__ int classify_edge(const Ed& e, double px, double py, double qx, double qy, double dx, double dy, double len2, int cut,
+                                             const EdgeCtx& c, double& lo, double& hi, bool& same, double& t) {
+  const double tol = c.tol;
+  const double bx0 = fmin(px, qx) - tol, bx1 = fmax(px, qx) + tol, by0 = fmin(py, qy) - tol, by1 = fmax(py, qy) + tol;
+  if (fmax(e.x0, e.x1) < bx0 || fmin(e.x0, e.x1) > bx1 || fmax(e.y0, e.y1) < by0 || fmin(e.y0, e.y1) > by1) return 0;
+  const double fx = e.x1 - e.x0, fy = e.y1 - e.y0;
+  const double r = dx * fy - dy * fx;
+  const double flen2 = fx * fx + fy * fy;
+  const double snap2 = c.snap * c.snap, tol2len2 = tol * tol * len2, eps2 = c.eps_par * c.eps_par;
+  // coincident if either edge's endpoints lie on the other's line (within snap, or within tol at cut points)
+  const double c0 = orient(px, py, qx, qy, e.x0, e.y0), c1 = orient(px, py, qx, qy, e.x1, e.y1);
+  bool col = (c0 * c0 <= snap2 * len2 || ((e.cut & 1) && c0 * c0 <= tol2len2)) && (c1 * c1 <= snap2 * len2 || ((e.cut & 2) && c1 * c1 <= tol2len2));
+  if (!col && flen2 > 0) {
+    const double g0 = orient(e.x0, e.y0, e.x1, e.y1, px, py), g1 = orient(e.x0, e.y0, e.x1, e.y1, qx, qy), tf = tol * tol * flen2;
+    col = (g0 * g0 <= snap2 * flen2 || ((cut & 1) && g0 * g0 <= tf)) && (g1 * g1 <= snap2 * flen2 || ((cut & 2) && g1 * g1 <= tf));
+  }
+  if (col) {
+    const double tc = ((e.x0 - px) * dx + (e.y0 - py) * dy) / len2, te = ((e.x1 - px) * dx + (e.y1 - py) * dy) / len2;
+    lo = fmax(0.0, fmin(tc, te));
+    hi = fmin(1.0, fmax(tc, te));
+    same = (dx * fx + dy * fy) > 0;
+    return hi > lo ? 1 : 0;
+  }
+  if (r * r > eps2 * len2 * flen2) {
+    const double wx = e.x0 - px, wy = e.y0 - py;
+    const double tn = wx * fy - wy * fx;
+    if (r > 0 ? (tn <= 0 || tn >= r) : (tn >= 0 || tn <= r)) return 0;
+    t = tn / r;
+    const double u = (wx * dy - wy * dx) / r;
+    const double du = tol / (flen2 > 0 ? sqrt(flen2) : 1.0);
+    return (u >= -du && u <= 1 + du) ? 2 : 0;
+  }
+  return 0;
+}
+
+// Array path with the reference's caps (MAXS split points, MAXO overlaps per edge, in list order): taken only by
+// edges whose splits or overlaps exceed the register path's capacity.
+template <class Src>
+__device__ Contrib edge_contrib_capped(const Ed& E, const Src& L, const EdgeCtx& c) {
+  Contrib out{0.0, 0.0, 0.0, 0u};
+  const double px = E.x0, py = E.y0, qx = E.x1, qy = E.y1;
+  const double dx = qx - px, dy = qy - py, len2 = dx * dx + dy * dy;
+  if (len2 == 0) return out;
+  double ts[MAXS];
+  int ns = 0;
+  double ol0[MAXO], ol1[MAXO];
+  bool osame[MAXO], oreal[MAXO];
+  int oidx[MAXO];
+  int no = 0;
+  ts[ns++] = 0; ts[ns++] = 1;
+  const int n = L.size();
+  for (int k = 0; k < n; k++) {
+    Ed e;
+    if (!L.get(k, e)) continue;
+    double lo, hi, t;
+    bool same;
+    const int kind = classify_edge(e, px, py, qx, qy, dx, dy, len2, E.cut, c, lo, hi, same, t);
+    if (kind == 1) {
+      if (lo > 0 && ns < MAXS) ts[ns++] = lo;
+      if (hi < 1 && ns < MAXS) ts[ns++] = hi;
+      if (no < MAXO) { ol0[no] = lo; ol1[no] = hi; osame[no] = same; oreal[no] = !e.syn; oidx[no] = k; no++; }
+    } else if (kind == 2 && ns < MAXS) {
+      ts[ns++] = t;
+    }
+  }
+  for (int a = 1; a < ns; a++) {
+    const double v = ts[a];
+    int b = a - 1;
+    while (b >= 0 && ts[b] > v) { ts[b + 1] = ts[b]; b--; }
+    ts[b + 1] = v;
+  }
+  for (int k = 0; k + 1 < ns; k++) {
+    const double t0 = ts[k], t1 = ts[k + 1];
+    if (!(t1 > t0)) continue;
+    const double tm = 0.5 * (t0 + t1);
+    int net = 0, first = -1, net_real = 0, n_real = 0;
+    for (int o = 0; o < no; o++)
+      if (tm > ol0[o] && tm < ol1[o]) {
+        const int sgn = osame[o] ? 1 : -1;
+        net += sgn;
+        if (first < 0) first = o;
+        if (oreal[o]) { net_real += sgn; n_real++; }
+      }
+    subsegment<Src>(L, c, px, py, dx, dy, len2, !E.syn, t0, t1, net, net_real, n_real, first < 0 ? -1 : oidx[first], first < 0 ? false : osame[first],
+                    out);
+  }
+  return out;
+}
+
+// One edge's Green's-theorem term (relative to c) and, for a real edge with relate on, its length inside / on the
+// other polygon (engine/engine.cuh:416-515). Partial sums start at zero for every edge. Register path: up to eight
+// split values (0 and 1 included) kept sorted, up to two overlaps; beyond that the result carries SPILL.
+template <class Src, bool CAPPED>
+__device__ Contrib edge_contrib(const Ed& E, const Src& L, const EdgeCtx& c) {
+  if constexpr (CAPPED) return edge_contrib_capped<Src>(E, L, c);
+  Contrib out{0.0, 0.0, 0.0, 0u};
+  const double px = E.x0, py = E.y0, qx = E.x1, qy = E.y1;
+  const double dx = qx - px, dy = qy - py, len2 = dx * dx + dy * dy;
+  if (len2 == 0) return out;
+  // sorted split values; 2.0 marks an empty slot (every split lies in [0, 1])
+  double s0 = 0.0, s1 = 1.0, s2 = 2.0, s3 = 2.0, s4 = 2.0, s5 = 2.0, s6 = 2.0, s7 = 2.0;
+  int ns = 2, no = 0;
+  double o0lo = 0, o0hi = 0, o1lo = 0, o1hi = 0;
+  bool o0same = false, o1same = false, o0real = false, o1real = false;
+  int o0idx = 0, o1idx = 0;
+  bool spill = false;
+  auto insert = [&](double v) {   // insertion into the sorted slots: b0 = min(a0, v), bj = max(a(j-1), min(aj, v))
+    if (ns >= 8) { spill = true; return; }
+    const double n7 = fmax(s6, fmin(s7, v)), n6 = fmax(s5, fmin(s6, v)), n5 = fmax(s4, fmin(s5, v)), n4 = fmax(s3, fmin(s4, v));
+    const double n3 = fmax(s2, fmin(s3, v)), n2 = fmax(s1, fmin(s2, v)), n1 = fmax(s0, fmin(s1, v)), n0 = fmin(s0, v);
+    s0 = n0; s1 = n1; s2 = n2; s3 = n3; s4 = n4; s5 = n5; s6 = n6; s7 = n7;
+    ns++;
+  };
+  const int n = L.size();
+  for (int k = 0; k < n && !spill; k++) {
+    Ed e;
+    if (!L.get(k, e)) continue;
+    double lo, hi, t;
+    bool same;
+    const int kind = classify_edge(e, px, py, qx, qy, dx, dy, len2, E.cut, c, lo, hi, same, t);
+    if (kind == 1) {
+      if (lo > 0) insert(lo);
+      if (hi < 1) insert(hi);
+      if (no == 0) { o0lo = lo; o0hi = hi; o0same = same; o0real = !e.syn; o0idx = k; }
+      else if (no == 1) { o1lo = lo; o1hi = hi; o1same = same; o1real = !e.syn; o1idx = k; }
+      else spill = true;
+      no++;
+    } else if (kind == 2) {
+      insert(t);
+    }
+  }
+  if (spill) { out.bits = SPILL; return out; }
+  const bool real = !E.syn;
+  while (ns >= 2) {
+    const double t0 = s0, t1 = s1;
+    if (t1 > t0) {
+      const double tm = 0.5 * (t0 + t1);
+      int net = 0, net_real = 0, n_real = 0, first = -1;
+      bool first_same = false;
+      if (no > 0 && tm > o0lo && tm < o0hi) {
+        const int sgn = o0same ? 1 : -1;
+        net += sgn; first = o0idx; first_same = o0same;
+        if (o0real) { net_real += sgn; n_real++; }
+      }
+      if (no > 1 && tm > o1lo && tm < o1hi) {
+        const int sgn = o1same ? 1 : -1;
+        net += sgn;
+        if (first < 0) { first = o1idx; first_same = o1same; }
+        if (o1real) { net_real += sgn; n_real++; }
+      }
+      subsegment<Src>(L, c, px, py, dx, dy, len2, real, t0, t1, net, net_real, n_real, first, first_same, out);
+    }
+    s0 = s1; s1 = s2; s2 = s3; s3 = s4; s4 = s5; s5 = s6; s6 = s7; s7 = 2.0;
+    ns--;
+  }
+  return out;
+}
+
+__device__ __forceinline__ double pair_snap(double2 o, double2 e) {   // engine/engine.cuh:524-527
+  const double m = fmax(fmax(fabs(o.x), fabs(o.x + e.x)), fmax(fabs(o.y), fabs(o.y + e.y)));
+  return 0.5 * (nextafter(m, INFINITY) - m);
+}
+
+struct PairSetup {
+  Box ov;
+  double tol, snap, cx, cy, bshx, bshy, ashx, ashy;
+};
+
+__device__ __forceinline__ PairSetup setup_pair(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, const RefineParams& rp) {
+  PairSetup s;
+  const double2 oa = P.origin[a], ob = R.origin[b], ea = P.ext[a], eb = R.ext[b];
+  const double sx = ob.x - oa.x, sy = ob.y - oa.y;
+  const double pad = 4.0 * fmax(rp.tol_abs, rp.tol_rel * fmax(fmax(ea.x, ea.y), fmax(eb.x, eb.y)));
+  s.ov = Box{fmax(0.0, sx) - pad, fmax(0.0, sy) - pad, fmin(ea.x, sx + eb.x) + pad, fmin(ea.y, sy + eb.y) + pad};
+  s.bshx = sx; s.bshy = sy;
+  s.ashx = oa.x - oa.x; s.ashy = oa.y - oa.y;   // A in its own frame
+  s.tol = fmax(rp.tol_abs, rp.tol_rel * fmax(s.ov.x1 - s.ov.x0, s.ov.y1 - s.ov.y0));
+  s.snap = rp.snap_half_ulp ? pair_snap(oa, ea) : 0.0;
+  s.cx = s.ov.x0 + 0.381966 * (s.ov.x1 - s.ov.x0);
+  s.cy = s.ov.y0 + 0.414213 * (s.ov.y1 - s.ov.y0);
+  return s;
+}
+
+// group epilogue: deterministic sums of the per-edge partials (stored at acc/lin/lon[0 .. n)), lane 0 writes
+template <int G>
+__device__ __forceinline__ void finish_pair(int gl, uint64_t p, bool hit, bool rtouch, bool computed, bool rel, int nsum, int na, const double* acc,
+                                            const double* lin, const double* lon, uint32_t bits, const RefineParams& rp, const RefineOut& out) {
+  double ar = 0.0, ai = 0.0, ao = 0.0, bi = 0.0, bo = 0.0;
+  uint32_t eb4 = rtouch ? REL_TOUCH : 0u;
+  if (computed) {
+    double s = 0.0;
+    if (gl == 0) { for (int k = 0; k < nsum; k++) s += acc[k]; }
+    else if (rel && gl == 1) { for (int k = 0; k < na; k++) s += lin[k]; }
+    else if (rel && gl == 2) { for (int k = 0; k < na; k++) s += lon[k]; }
+    else if (rel && gl == 3) { for (int k = na; k < nsum; k++) s += lin[k]; }
+    else if (rel && gl == 4) { for (int k = na; k < nsum; k++) s += lon[k]; }
+    ar = 0.5 * __shfl(s, 0, G);
+    if (rel) { ai = __shfl(s, 1, G); ao = __shfl(s, 2, G); bi = __shfl(s, 3, G); bo = __shfl(s, 4, G); }
+    eb4 |= (gany<G>(bits & REL_COIN_SAME) ? REL_COIN_SAME : 0u) | (gany<G>(bits & REL_COIN_OPP) ? REL_COIN_OPP : 0u) |
+           (gany<G>(bits & REL_TOUCH) ? REL_TOUCH : 0u);
+  }
+  if (gl == 0) {
+    const bool ev_any = ai != 0 || ao != 0 || bi != 0 || bo != 0;
+    out.isect[p] = hit ? 1 : ((ar != 0.0 || ev_any) ? 2 : 0);
+    if (out.area) out.area[p] = ar;
+    if (out.ev) { out.ev[p] = make_double4(ai, ao, bi, bo); out.evb[p] = eb4; }
+  }
+}
+
+// ------------------------------------------------------------------------------------------- small class
+struct SmallPoly {
+  double2 V[VCAP];
+  uint8_t F[VCAP];
+  uint8_t rs[RCAP + 1];   // ring start slots, rs[nrings] = vertex count
+  uint8_t nv, nr;
+};
+constexpr size_t small_bytes(bool rel) { return ((2 * sizeof(SmallPoly) + 15) & ~size_t(15)) + (size_t)2 * VCAP * 8 * (rel ? 3 : 1); }
+
+// all vertices of geometry g into LDS with per-vertex flags (reference edge collection, engine/engine.cuh:358-377)
+template <int G>
+__device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, SmallPoly& sp, int gl) {
+  const uint32_t r0 = S.gro[g], nrings = S.gro[g + 1] - r0;
+  const uint64_t v0 = S.rvo[r0];
+  const int nv = (int)(S.rvo[r0 + nrings] - v0);
+  for (int k = gl; k < nv; k += G) { const double2 p = S.xy[v0 + k]; sp.V[k] = make_double2(shx + p.x, shy + p.y); }
+  for (int r = gl; r <= (int)nrings; r += G) sp.rs[r] = (uint8_t)(S.rvo[r0 + r] - v0);
+  if (gl == 0) { sp.nv = (uint8_t)nv; sp.nr = (uint8_t)nrings; }
+  wave_lds_sync();
+  for (int k = gl; k < nv; k += G) {
+    int r = 0;
+    while (r + 1 < (int)nrings && sp.rs[r + 1] <= k) r++;
+    const int s = sp.rs[r], e = sp.rs[r + 1];
+    uint8_t f = 0;
+    if (k + 1 < e) {
+      const uint64_t i = v0 + k, si = v0 + s, ei = v0 + e;
+      f = F_VALID;
+      if (syn_bit(S.synb, i)) f |= F_SYN;
+      if (S.synb) {
+        if (syn_bit(S.synb, i > si ? i - 1 : ei - 2)) f |= F_CUT0;
+        if (syn_bit(S.synb, i + 2 < ei ? i + 1 : si)) f |= F_CUT1;
+      }
+      const double2 a = sp.V[k], b = sp.V[k + 1];
+      if (!edge_outside(a.x, a.y, b.x, b.y, ov)) f |= F_INOV;
+    }
+    sp.F[k] = f;
+  }
+  wave_lds_sync();
+}
+
+// parity of the segment from a point left of the geometry's bbox to m over every edge (engine/engine.cuh:382-397)
+template <int G>
+__device__ bool pip_small(const SmallPoly& sp, double shx, double mx, double my, int gl) {
+  const double ox = shx - 1.0, oy = my;
+  bool par = false;
+  for (int k = gl; k < sp.nv; k += G) {
+    if (!(sp.F[k] & F_VALID)) continue;
+    const double2 a = sp.V[k], b = sp.V[k + 1];
+    const double x0 = a.x, y0 = a.y, x1 = b.x, y1 = b.y;
+    if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;
+    const bool so = left_of(x0, y0, x1, y1, ox, oy), sc = left_of(x0, y0, x1, y1, mx, my);
+    if (so == sc) continue;
+    const bool su = orient(ox, oy, mx, my, x0, y0) > 0, sv = orient(ox, oy, mx, my, x1, y1) > 0;
+    if (su != sv) par = !par;
+  }
+  return (__popcll(gballot<G>(par)) & 1) != 0;
+}
+
+// containment probe of ring r: first vertex that starts a real edge, else the first vertex (engine/engine.cuh:322-327);
+// returns 0 for an empty ring (no probe), 1 for a probe on a synthetic-only ring, 2 for a real probe
+__device__ __forceinline__ int probe_small(const SmallPoly& sp, int r, bool has_syn, double& x, double& y) {
+  const int s = sp.rs[r], e = sp.rs[r + 1];
+  if (s == e) return 0;
+  int v = s;
+  bool real = true;
+  if (has_syn) {
+    while (v + 1 < e && (sp.F[v] & F_SYN)) v++;
+    real = v + 1 < e;
+  }
+  const double2 p = sp.V[real ? v : s];
+  x = p.x; y = p.y;
+  return real ? 2 : 1;
+}
+
+struct OverflowCtl { uint32_t count; uint32_t need; uint32_t pad0, pad1; };
+
+// hand pair p to the overflow kernel (lists of up to `need` edges per side); called by one lane
+__device__ __forceinline__ void queue_overflow(uint32_t p, int need, uint32_t* ovf_list, OverflowCtl* ctl) {
+  const uint32_t slot = atomicAdd(&ctl->count, 1u);
+  ovf_list[slot] = p;
+  atomicMax(&ctl->need, (uint32_t)need);
+}
+
+template <int G>
+__global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                         const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist_ptr, RefineParams rp,
+                                                         RefineOut out, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int gpb = blockDim.x / G;
+  const int gib = threadIdx.x / G, gl = threadIdx.x % G;
+  const bool rel = rp.want_relate != 0;
+  unsigned char* base = smem + (size_t)gib * small_bytes(rel);
+  SmallPoly& SA = *reinterpret_cast<SmallPoly*>(base);
+  SmallPoly& SB = *(reinterpret_cast<SmallPoly*>(base) + 1);
+  double* acc = reinterpret_cast<double*>(base + ((2 * sizeof(SmallPoly) + 15) & ~size_t(15)));
+  double* lin = acc + 2 * VCAP;
+  double* lon = lin + 2 * VCAP;
+  const uint32_t nlist = *nlist_ptr;
+  for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
+    const uint32_t p = list[k];
+    const uint32_t a = pa[p], b = pb[p];
+    const PairSetup f = setup_pair(P, R, a, b, rp);
+    load_small<G>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
+    load_small<G>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
+    const int na = SA.nv, nb = SB.nv;
+    const VertSrc LA{SA.V, SA.F, na}, LB{SB.V, SB.F, nb};
+    // boundary contact over the A x B edge grid
+    bool hit = false, hit_any = false;
+    const int grid = na * nb;
+    for (int q0 = 0; q0 < grid; q0 += G) {
+      const int q = q0 + gl;
+      if (q < grid) {
+        Ed ea, eb;
+        if (LA.get(q / nb, ea) && LB.get(q % nb, eb)) {
+          if (!(fmax(eb.x0, eb.x1) < fmin(ea.x0, ea.x1) || fmin(eb.x0, eb.x1) > fmax(ea.x0, ea.x1) || fmax(eb.y0, eb.y1) < fmin(ea.y0, ea.y1) ||
+                fmin(eb.y0, eb.y1) > fmax(ea.y0, ea.y1)) &&
+              seg_isect(ea.x0, ea.y0, ea.x1, ea.y1, eb.x0, eb.y0, eb.x1, eb.y1)) {
+            hit_any = true;
+            if (!(ea.syn | eb.syn)) hit = true;
+          }
+        }
+      }
+      if (gany<G>(hit)) break;
+    }
+    hit = gany<G>(hit);
+    hit_any = gany<G>(hit_any || hit);
+    const bool rtouch = hit;
+    bool inB_c = false, inA_c = false;
+    if (!hit || rp.want_area || rel) {
+      inB_c = pip_small<G>(SB, f.bshx, f.cx, f.cy, gl);
+      inA_c = pip_small<G>(SA, f.ashx, f.cx, f.cy, gl);
+    }
+    bool contain_any = false;
+    if (!hit) {
+      bool h = false, ha = false;
+      const int nra = SA.nr, nrb = SB.nr;
+      for (int r = gl; r < nra + nrb; r += G) {
+        const bool sideA = r < nra;
+        double x, y;
+        const int pr = probe_small(sideA ? SA : SB, sideA ? r : r - nra, (sideA ? P.synb : R.synb) != nullptr, x, y);
+        const bool real = pr == 2;
+        if (pr && x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 &&
+            inside_from_c(sideA ? LB : LA, f.cx, f.cy, sideA ? inB_c : inA_c, x, y)) {
+          ha = true;
+          if (real) h = true;
+        }
+      }
+      hit = gany<G>(h);
+      contain_any = gany<G>(ha);
+    }
+    const bool computed = (hit || hit_any || contain_any) && (rp.want_area || rel);
+    uint32_t bits = 0;
+    if (computed) {
+      EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, inB_c, rel}, cb = ca;
+      cb.in_c = inA_c;
+      for (int s = gl; s < na + nb; s += G) {
+        const bool sideA = s < na;
+        Ed e;
+        Contrib c{0.0, 0.0, 0.0, 0u};
+        if ((sideA ? LA : LB).get(sideA ? s : s - na, e))
+          c = sideA ? edge_contrib<VertSrc, false>(e, LB, ca) : edge_contrib<VertSrc, false>(e, LA, cb);
+        acc[s] = c.acc;
+        if (rel) { lin[s] = c.lin; lon[s] = c.lon; }
+        bits |= c.bits;
+      }
+      wave_lds_sync();
+      if (gany<G>(bits & SPILL)) {
+        if (gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
+        continue;
+      }
+    }
+    finish_pair<G>(gl, p, hit, rtouch, computed, rel, na + nb, na, acc, lin, lon, bits, rp, out);
+    wave_lds_sync();
+  }
+}
+
+// ------------------------------------------------------------------------------------------- large / overflow
 // edges of geometry g meeting box ov, in ring order, compacted by group ballot (engine/engine.cuh:358-377);
 // returns the full count (may exceed cap: then the list is incomplete and the pair overflows)
 template <int G>
@@ -77,8 +565,7 @@ __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, cons
   return cnt;
 }
 
-// inside(m) for geometry g: parity of the segment from a point left of its bbox to m, over every edge
-// (engine/engine.cuh:382-397)
+// inside(m) over every edge of geometry g, read from global memory (engine/engine.cuh:382-397)
 template <int G>
 __device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, double mx, double my, int gl) {
   const double ox = shx - 1.0, oy = my;
@@ -99,132 +586,6 @@ __device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, do
   return (__popcll(gballot<G>(par)) & 1) != 0;
 }
 
-// inside(x) = inside(c) xor parity of segment c->x across the local edges (engine/engine.cuh:400-412)
-__device__ bool inside_from_c(const Ed* L, int cnt, double cx, double cy, bool in_c, double x, double y) {
-  bool in = in_c;
-  const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
-  for (int k = 0; k < cnt; k++) {
-    const Ed e = L[k];
-    if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) continue;
-    const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
-    if (sc == sx) continue;
-    const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
-    if (su != sv) in = !in;
-  }
-  return in;
-}
-
-struct Contrib { double acc, lin, lon; uint32_t bits; };
-
-// One edge's Green's-theorem term (relative to c) and, for a real edge with relate on, its length inside / on the
-// other polygon (engine/engine.cuh:416-515). Partial sums start at zero for every edge.
-__device__ Contrib edge_contrib(const Ed& E, const Ed* L, int cnt, double cx, double cy, bool in_c, const Box& ov, double tol, double eps_par,
-                                bool rel, double snap) {
-  Contrib out{0.0, 0.0, 0.0, 0u};
-  const double px = E.x0, py = E.y0, qx = E.x1, qy = E.y1;
-  const int cut = E.cut;
-  const bool real = !E.syn;
-  const double dx = qx - px, dy = qy - py;
-  const double len2 = dx * dx + dy * dy;
-  if (len2 == 0) return out;
-  const double tol2len2 = tol * tol * len2, eps2 = eps_par * eps_par;
-  const double snap2 = snap * snap;
-  double ts[MAXS];
-  int ns = 0;
-  double ol0[MAXO], ol1[MAXO];
-  bool osame[MAXO];
-  int oidx[MAXO];
-  int no = 0;
-  ts[ns++] = 0; ts[ns++] = 1;
-  const double bx0 = fmin(px, qx) - tol, bx1 = fmax(px, qx) + tol, by0 = fmin(py, qy) - tol, by1 = fmax(py, qy) + tol;
-  for (int k = 0; k < cnt; k++) {
-    const Ed e = L[k];
-    if (fmax(e.x0, e.x1) < bx0 || fmin(e.x0, e.x1) > bx1 || fmax(e.y0, e.y1) < by0 || fmin(e.y0, e.y1) > by1) continue;
-    const double fx = e.x1 - e.x0, fy = e.y1 - e.y0;
-    const double r = dx * fy - dy * fx;
-    const double flen2 = fx * fx + fy * fy;
-    // coincident if either edge's endpoints lie on the other's line (within snap, or within tol at cut points)
-    const double c0 = orient(px, py, qx, qy, e.x0, e.y0), c1 = orient(px, py, qx, qy, e.x1, e.y1);
-    bool col = (c0 * c0 <= snap2 * len2 || ((e.cut & 1) && c0 * c0 <= tol2len2)) && (c1 * c1 <= snap2 * len2 || ((e.cut & 2) && c1 * c1 <= tol2len2));
-    if (!col && flen2 > 0) {
-      const double g0 = orient(e.x0, e.y0, e.x1, e.y1, px, py), g1 = orient(e.x0, e.y0, e.x1, e.y1, qx, qy), tf = tol * tol * flen2;
-      col = (g0 * g0 <= snap2 * flen2 || ((cut & 1) && g0 * g0 <= tf)) && (g1 * g1 <= snap2 * flen2 || ((cut & 2) && g1 * g1 <= tf));
-    }
-    if (col) {
-      const double tc = ((e.x0 - px) * dx + (e.y0 - py) * dy) / len2, te = ((e.x1 - px) * dx + (e.y1 - py) * dy) / len2;
-      const double lo = fmax(0.0, fmin(tc, te)), hi = fmin(1.0, fmax(tc, te));
-      if (hi > lo) {
-        if (lo > 0 && ns < MAXS) ts[ns++] = lo;
-        if (hi < 1 && ns < MAXS) ts[ns++] = hi;
-        if (no < MAXO) { ol0[no] = lo; ol1[no] = hi; osame[no] = (dx * fx + dy * fy) > 0; oidx[no] = k; no++; }
-      }
-    } else if (r * r > eps2 * len2 * flen2) {
-      const double wx = e.x0 - px, wy = e.y0 - py;
-      const double tn = wx * fy - wy * fx;
-      if (r > 0 ? (tn <= 0 || tn >= r) : (tn >= 0 || tn <= r)) continue;
-      const double t = tn / r, u = (wx * dy - wy * dx) / r;
-      const double du = tol / (flen2 > 0 ? sqrt(flen2) : 1.0);
-      if (u >= -du && u <= 1 + du && ns < MAXS) ts[ns++] = t;
-    }
-  }
-  for (int a = 1; a < ns; a++) {
-    const double v = ts[a];
-    int b = a - 1;
-    while (b >= 0 && ts[b] > v) { ts[b + 1] = ts[b]; b--; }
-    ts[b + 1] = v;
-  }
-  for (int k = 0; k + 1 < ns; k++) {
-    const double t0 = ts[k], t1 = ts[k + 1];
-    if (!(t1 > t0)) continue;
-    const double tm = 0.5 * (t0 + t1);
-    const double mx = px + dx * tm, my = py + dy * tm;
-    const bool in_ov = mx >= ov.x0 && mx <= ov.x1 && my >= ov.y0 && my <= ov.y1;
-    int net = 0, first = -1, net_real = 0, n_real = 0;
-    for (int o = 0; o < no; o++)
-      if (tm > ol0[o] && tm < ol1[o]) {
-        const int sgn = osame[o] ? 1 : -1;
-        net += sgn;
-        if (first < 0) first = o;
-        if (!L[oidx[o]].syn) { net_real += sgn; n_real++; }
-      }
-    int vin = -1;   // midpoint inside the other polygon, computed on first use
-    auto inside_m = [&]() {
-      if (vin < 0) vin = (in_ov && inside_from_c(L, cnt, cx, cy, in_c, mx, my)) ? 1 : 0;
-      return vin;
-    };
-    if (rel && real) {
-      const double seg = sqrt(len2) * (t1 - t0);
-      if (net_real != 0) { out.lon += seg; out.bits |= (net_real > 0 ? REL_COIN_SAME : REL_COIN_OPP) | REL_TOUCH; }
-      else if (n_real == 0 && (net == 1 || net == -1)) out.lin += 0.5 * seg;
-      else if (inside_m()) out.lin += seg;
-    }
-    double w;
-    if (first < 0) w = inside_m() ? 1.0 : 0.0;
-    else if (net == 1 || net == -1) w = 0.5;
-    else if (net == 0) w = inside_m() ? 1.0 : 0.0;
-    else {
-      const Ed eb = L[oidx[first]];
-      const bool sb = left_of(eb.x0, eb.y0, eb.x1, eb.y1, mx, my);
-      const bool m_left = osame[first] ? sb : !sb;
-      const int v = inside_m();
-      int other = m_left ? v - net : v + net;
-      other = other < 0 ? 0 : (other > 1 ? 1 : other);
-      w = 0.5 * (v + other);
-    }
-    if (w != 0.0) {
-      const double ax = px + dx * t0 - cx, ay = py + dy * t0 - cy, bx = px + dx * t1 - cx, by = py + dy * t1 - cy;
-      out.acc += w * (ax * by - ay * bx);
-    }
-  }
-  return out;
-}
-
-__device__ __forceinline__ double pair_snap(double2 o, double2 e) {   // engine/engine.cuh:524-527
-  const double m = fmax(fmax(fabs(o.x), fabs(o.x + e.x)), fmax(fabs(o.y), fabs(o.y + e.y)));
-  return 0.5 * (nextafter(m, INFINITY) - m);
-}
-
-// containment probe of ring r: first vertex that starts a real edge (engine/engine.cuh:322-327)
 __device__ __forceinline__ bool probe_vertex(const DPolys& S, uint32_t r, double shx, double shy, double& x, double& y) {
   uint64_t v = S.rvo[r];
   const uint64_t e = S.rvo[r + 1];
@@ -237,53 +598,29 @@ __device__ __forceinline__ bool probe_vertex(const DPolys& S, uint32_t r, double
   return true;
 }
 
-// Per-group scratch: edge lists and per-edge partial sums. LDS for the fast kernels, global memory for overflow.
+// Per-group scratch for compacted lists and per-edge partial sums (LDS in the large kernel, global in overflow).
 struct Scratch {
   Ed* LA; Ed* LB;
-  double* cacc;   // [capA + capB] area terms: A edges then B edges
-  double* clin;   // [capA + capB] inside lengths (relate only)
-  double* clon;   // [capA + capB] on lengths (relate only)
-  int cap;
+  double* acc;   // [2 cap] area terms: A edges then B edges
+  double* lin;   // [2 cap] inside lengths (relate only)
+  double* lon;   // [2 cap] on lengths (relate only)
 };
-
 __device__ __forceinline__ Scratch carve(unsigned char* base, int cap, bool rel) {
   Scratch s;
   s.LA = reinterpret_cast<Ed*>(base);
   s.LB = s.LA + cap;
-  s.cacc = reinterpret_cast<double*>(s.LB + cap);
-  s.clin = rel ? s.cacc + 2 * cap : nullptr;
-  s.clon = rel ? s.clin + 2 * cap : nullptr;
-  s.cap = cap;
+  s.acc = reinterpret_cast<double*>(s.LB + cap);
+  s.lin = rel ? s.acc + 2 * cap : nullptr;
+  s.lon = rel ? s.lin + 2 * cap : nullptr;
   return s;
 }
-__host__ __device__ constexpr size_t scratch_bytes(int cap, bool rel) { return (size_t)cap * (2 * 40 + 2 * 8 + (rel ? 4 * 8 : 0)); }
+__host__ __device__ constexpr size_t list_bytes(int cap, bool rel) { return (size_t)cap * (2 * 40 + 2 * 8 + (rel ? 4 * 8 : 0)); }
 
-struct PairSetup {
-  Box ov;
-  double tol, snap, cx, cy, bshx, bshy;
-};
-
-__device__ __forceinline__ PairSetup setup_pair(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, const RefineParams& rp) {
-  PairSetup s;
-  const double2 oa = P.origin[a], ob = R.origin[b], ea = P.ext[a], eb = R.ext[b];
-  const double sx = ob.x - oa.x, sy = ob.y - oa.y;
-  const double pad = 4.0 * fmax(rp.tol_abs, rp.tol_rel * fmax(fmax(ea.x, ea.y), fmax(eb.x, eb.y)));
-  s.ov = Box{fmax(0.0, sx) - pad, fmax(0.0, sy) - pad, fmin(ea.x, sx + eb.x) + pad, fmin(ea.y, sy + eb.y) + pad};
-  s.bshx = sx; s.bshy = sy;
-  s.tol = fmax(rp.tol_abs, rp.tol_rel * fmax(s.ov.x1 - s.ov.x0, s.ov.y1 - s.ov.y0));
-  s.snap = rp.snap_half_ulp ? pair_snap(oa, ea) : 0.0;
-  s.cx = s.ov.x0 + 0.381966 * (s.ov.x1 - s.ov.x0);
-  s.cy = s.ov.y0 + 0.414213 * (s.ov.y1 - s.ov.y0);
-  return s;
-}
-
-// the refinement of one pair by one lane group after its lists are collected; lane 0 of the group writes
-template <int G>
-__device__ void refine_body(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, uint64_t p, const RefineParams& rp, const PairSetup& f,
-                            const Scratch& S, int na, int nb, int gl, const RefineOut& out) {
-  const double2 oa = P.origin[a];
-  const double a_shx = oa.x - oa.x, a_shy = oa.y - oa.y;   // A in its own frame
-  // boundary contact
+// returns true (group-uniform) when an edge spilled the register path: nothing is written for the pair then
+template <int G, bool CAPPED>
+__device__ bool refine_lists(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, uint64_t p, const RefineParams& rp, const PairSetup& f,
+                             const Scratch& S, int na, int nb, int gl, const RefineOut& out) {
+  const ListSrc LA{S.LA, na}, LB{S.LB, nb};
   bool hit = false, hit_any = false;
   for (int i = gl; i < na && !hit; i += G) {
     const Ed ea = S.LA[i];
@@ -300,155 +637,120 @@ __device__ void refine_body(const DPolys& P, const DPolys& R, uint32_t a, uint32
   hit = gany<G>(hit);
   hit_any = gany<G>(hit_any || hit);
   const bool rtouch = hit;
+  const bool rel = rp.want_relate != 0;
   bool inB_c = false, inA_c = false;
-  if (!hit || rp.want_area || rp.want_relate) {
+  if (!hit || rp.want_area || rel) {
     inB_c = pip_full<G>(R, b, f.bshx, f.bshy, f.cx, f.cy, gl);
-    inA_c = pip_full<G>(P, a, a_shx, a_shy, f.cx, f.cy, gl);
+    inA_c = pip_full<G>(P, a, f.ashx, f.ashy, f.cx, f.cy, gl);
   }
   bool contain_any = false;
   if (!hit) {
-    // no real boundary contact: intersects iff one contains the other; one probe vertex per ring
     bool h = false, ha = false;
     for (uint32_t r = P.gro[a] + gl; r < P.gro[a + 1] && !h; r += G) {
+      if (P.rvo[r] == P.rvo[r + 1]) continue;   // an empty ring has no probe vertex
       double x, y;
-      const bool real = probe_vertex(P, r, a_shx, a_shy, x, y);
-      if (!real) { const double2 v = P.xy[P.rvo[r]]; x = a_shx + v.x; y = a_shy + v.y; }
-      if (x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 && inside_from_c(S.LB, nb, f.cx, f.cy, inB_c, x, y)) { ha = true; if (real) h = true; }
+      const bool real = probe_vertex(P, r, f.ashx, f.ashy, x, y);
+      if (!real) { const double2 v = P.xy[P.rvo[r]]; x = f.ashx + v.x; y = f.ashy + v.y; }
+      if (x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 && inside_from_c(LB, f.cx, f.cy, inB_c, x, y)) { ha = true; if (real) h = true; }
     }
     for (uint32_t r = R.gro[b] + gl; r < R.gro[b + 1] && !h; r += G) {
+      if (R.rvo[r] == R.rvo[r + 1]) continue;   // an empty ring has no probe vertex
       double x, y;
       const bool real = probe_vertex(R, r, f.bshx, f.bshy, x, y);
       if (!real) { const double2 v = R.xy[R.rvo[r]]; x = f.bshx + v.x; y = f.bshy + v.y; }
-      if (x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 && inside_from_c(S.LA, na, f.cx, f.cy, inA_c, x, y)) { ha = true; if (real) h = true; }
+      if (x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 && inside_from_c(LA, f.cx, f.cy, inA_c, x, y)) { ha = true; if (real) h = true; }
     }
     hit = gany<G>(h);
     contain_any = gany<G>(ha);
   }
-  double ar = 0.0, ai = 0.0, ao = 0.0, bi = 0.0, bo = 0.0;
-  uint32_t eb4 = 0;
-  const bool rel = rp.want_relate != 0;
-  if ((hit || hit_any || contain_any) && (rp.want_area || rel)) {
-    uint32_t bits = 0;
-    for (int i = gl; i < na; i += G) {
-      const Contrib c = edge_contrib(S.LA[i], S.LB, nb, f.cx, f.cy, inB_c, f.ov, f.tol, rp.eps_par, rel, f.snap);
-      S.cacc[i] = c.acc;
-      if (rel) { S.clin[i] = c.lin; S.clon[i] = c.lon; }
-      bits |= c.bits;
-    }
-    for (int i = gl; i < nb; i += G) {
-      const Contrib c = edge_contrib(S.LB[i], S.LA, na, f.cx, f.cy, inA_c, f.ov, f.tol, rp.eps_par, rel, f.snap);
-      S.cacc[na + i] = c.acc;
-      if (rel) { S.clin[na + i] = c.lin; S.clon[na + i] = c.lon; }
+  const bool computed = (hit || hit_any || contain_any) && (rp.want_area || rel);
+  uint32_t bits = 0;
+  if (computed) {
+    EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, inB_c, rel}, cb = ca;
+    cb.in_c = inA_c;
+    for (int s = gl; s < na + nb; s += G) {
+      const bool sideA = s < na;
+      const Contrib c = sideA ? edge_contrib<ListSrc, CAPPED>(S.LA[s], LB, ca) : edge_contrib<ListSrc, CAPPED>(S.LB[s - na], LA, cb);
+      S.acc[s] = c.acc;
+      if (rel) { S.lin[s] = c.lin; S.lon[s] = c.lon; }
       bits |= c.bits;
     }
     wave_lds_sync();
-    // deterministic sums in list order: lane 0 area, lanes 1..4 the relate lengths
-    double s = 0.0;
-    if (gl == 0) { for (int k = 0; k < na + nb; k++) s += S.cacc[k]; }
-    else if (rel && gl == 1) { for (int k = 0; k < na; k++) s += S.clin[k]; }
-    else if (rel && gl == 2) { for (int k = 0; k < na; k++) s += S.clon[k]; }
-    else if (rel && gl == 3) { for (int k = na; k < na + nb; k++) s += S.clin[k]; }
-    else if (rel && gl == 4) { for (int k = na; k < na + nb; k++) s += S.clon[k]; }
-    ar = 0.5 * __shfl(s, 0, G);
-    ai = __shfl(s, 1, G); ao = __shfl(s, 2, G); bi = __shfl(s, 3, G); bo = __shfl(s, 4, G);
-    if (!rel) { ai = ao = bi = bo = 0.0; }
-    eb4 = (rtouch ? REL_TOUCH : 0u) | (gany<G>(bits & REL_COIN_SAME) ? REL_COIN_SAME : 0u) | (gany<G>(bits & REL_COIN_OPP) ? REL_COIN_OPP : 0u) |
-          (gany<G>(bits & REL_TOUCH) ? REL_TOUCH : 0u);
-  } else if (rtouch) {
-    eb4 = REL_TOUCH;
+    if (!CAPPED && gany<G>(bits & SPILL)) return true;
   }
-  if (gl == 0) {
-    const bool ev_any = ai != 0 || ao != 0 || bi != 0 || bo != 0;
-    out.isect[p] = hit ? 1 : ((ar != 0.0 || ev_any) ? 2 : 0);
-    if (out.area) out.area[p] = ar;
-    if (out.ev) { out.ev[p] = make_double4(ai, ao, bi, bo); out.evb[p] = eb4; }
-  }
+  finish_pair<G>(gl, p, hit, rtouch, computed, rel, na + nb, na, S.acc, S.lin, S.lon, bits, rp, out);
+  return false;
 }
 
-struct OverflowCtl { uint32_t count; uint32_t need; uint32_t n_small; uint32_t n_large; };
-
-// fast path: a group of G lanes per pair, lists in LDS (cap edges per side); overflowing pairs are queued
-template <int G>
-__global__ void __launch_bounds__(256) k_refine(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
-                                                const uint32_t* __restrict__ list, int which, RefineParams rp,
-                                                RefineOut out, int cap, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+// large class: a wave per pair, compacted lists in LDS (cap edges per side); overflowing pairs are queued
+__global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                      const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist_ptr, RefineParams rp,
+                                                      RefineOut out, int cap, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int gpb = blockDim.x / G;
-  const int gib = threadIdx.x / G, gl = threadIdx.x % G;
+  const int wpb = blockDim.x / 64;
+  const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
   const bool rel = rp.want_relate != 0;
-  const Scratch S = carve(smem + (size_t)gib * scratch_bytes(cap, rel), cap, rel);
-  const uint32_t nlist = which == 0 ? ctl->n_small : ctl->n_large;
-  for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
-    const uint32_t p = list[k];
+  const Scratch S = carve(smem + (size_t)wib * list_bytes(cap, rel), cap, rel);
+  const uint32_t nlist = nlist_ptr[1];
+  const uint32_t* large = list + nlist_ptr[0];   // the large class follows the small one
+  for (uint32_t k = blockIdx.x * wpb + wib; k < nlist; k += gridDim.x * wpb) {
+    const uint32_t p = large[k];
     const uint32_t a = pa[p], b = pb[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    const double2 oa = P.origin[a];
-    const int na = collect<G>(P, a, oa.x - oa.x, oa.y - oa.y, f.ov, S.LA, cap, gl);
-    const int nb = collect<G>(R, b, f.bshx, f.bshy, f.ov, S.LB, cap, gl);
+    const int na = collect<64>(P, a, f.ashx, f.ashy, f.ov, S.LA, cap, gl);
+    const int nb = collect<64>(R, b, f.bshx, f.bshy, f.ov, S.LB, cap, gl);
     if (na > cap || nb > cap) {
-      if (gl == 0) {
-        const uint32_t slot = atomicAdd(&ctl->count, 1u);
-        ovf_list[slot] = p;
-        atomicMax(&ctl->need, (uint32_t)max(na, nb));
-      }
+      if (gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
       continue;
     }
     wave_lds_sync();
-    refine_body<G>(P, R, a, b, p, rp, f, S, na, nb, gl, out);
+    if (refine_lists<64, false>(P, R, a, b, p, rp, f, S, na, nb, gl, out) && gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
     wave_lds_sync();
   }
 }
 
-// overflow path: one wave per pair, lists and partial sums in a per-wave slab of global memory
-__global__ void __launch_bounds__(256) k_refine_overflow(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+// overflow: one wave per pair, lists and partial sums in a per-wave slab of global memory, array path for every edge
+__global__ void __launch_bounds__(256, 2) k_refine_overflow(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                          const uint32_t* __restrict__ list, uint32_t nlist, RefineParams rp, RefineOut out, int cap,
                                                          unsigned char* __restrict__ slab) {
   const int wpb = blockDim.x / 64;
   const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
   const bool rel = rp.want_relate != 0;
   const uint64_t wave = (uint64_t)blockIdx.x * wpb + wib;
-  const Scratch S = carve(slab + wave * scratch_bytes(cap, rel), cap, rel);
+  const Scratch S = carve(slab + wave * list_bytes(cap, rel), cap, rel);
   for (uint32_t k = (uint32_t)wave; k < nlist; k += gridDim.x * wpb) {
     const uint32_t p = list[k];
     const uint32_t a = pa[p], b = pb[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    const double2 oa = P.origin[a];
-    const int na = collect<64>(P, a, oa.x - oa.x, oa.y - oa.y, f.ov, S.LA, cap, gl);
+    const int na = collect<64>(P, a, f.ashx, f.ashy, f.ov, S.LA, cap, gl);
     const int nb = collect<64>(R, b, f.bshx, f.bshy, f.ov, S.LB, cap, gl);
     wave_lds_sync();
-    refine_body<64>(P, R, a, b, p, rp, f, S, min(na, cap), min(nb, cap), gl, out);
+    refine_lists<64, true>(P, R, a, b, p, rp, f, S, min(na, cap), min(nb, cap), gl, out);
     wave_lds_sync();
   }
 }
 
-constexpr uint32_t SMALL_EDGES = 32;   // both polygons at most this many edges: 16-lane groups, LDS cap 32
-constexpr int CAP_SMALL = 32, CAP_LARGE = 128;
-
-__global__ void k_ctl_reset(OverflowCtl* ctl) { ctl->count = 0; ctl->need = 0; ctl->n_small = 0; ctl->n_large = 0; }
-
-// split candidates into the 16-lane class and the 64-lane class (wave-aggregated appends)
-__global__ void k_classify(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                           uint32_t* __restrict__ small, uint32_t* __restrict__ large, OverflowCtl* ctl) {
+// class flag per candidate: 1 = small (both polygons fit the LDS vertex lists)
+__global__ void k_class_flags(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
+                              uint32_t* __restrict__ fsmall) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < np;
-  bool is_small = false;
-  if (valid) is_small = max(P.nedge[pa[i]], R.nedge[pb[i]]) <= SMALL_EDGES;
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  const uint64_t ms = __ballot(valid && is_small), ml = __ballot(valid && !is_small);
-  uint32_t bs = 0, bl = 0;
-  const int leader = __ffsll((long long)(ms | ml)) - 1;
-  if (lane == leader) {
-    if (ms) bs = atomicAdd(&ctl->n_small, (uint32_t)__popcll(ms));
-    if (ml) bl = atomicAdd(&ctl->n_large, (uint32_t)__popcll(ml));
-  }
-  if (leader < 0) return;
-  bs = __shfl(bs, leader, 64);
-  bl = __shfl(bl, leader, 64);
-  if (!valid) return;
-  if (is_small) small[bs + __popcll(ms & lt)] = (uint32_t)i;
-  else large[bl + __popcll(ml & lt)] = (uint32_t)i;
+  if (i >= np) return;
+  const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
+  fsmall[i] = ((ia & 0xFFFFu) <= VCAP && (ib & 0xFFFFu) <= VCAP && (ia >> 16) <= RCAP && (ib >> 16) <= RCAP) ? 1u : 0u;
 }
+
+// stable partition from the scan of the small flags: small pairs first, large pairs after
+__global__ void k_partition(const uint32_t* __restrict__ fsmall, const uint64_t* __restrict__ off, uint64_t np, const uint64_t* __restrict__ nsmall_p,
+                            uint32_t* __restrict__ list, uint32_t* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nsmall = *nsmall_p;
+  if (i == 0) { counts[0] = (uint32_t)nsmall; counts[1] = (uint32_t)(np - nsmall); }
+  if (i >= np) return;
+  const uint64_t o = off[i];
+  list[fsmall[i] ? o : nsmall + (i - o)] = (uint32_t)i;
+}
+
+__global__ void k_ctl_reset(OverflowCtl* ctl) { ctl->count = 0; ctl->need = 0; }
 
 }  // namespace
 
@@ -458,45 +760,54 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "refine: more than 2^32 pairs in one launch");
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
-  uint32_t* lists = c.s_cls.as<uint32_t>(2 * np + 64);
-  uint32_t* small = lists;
-  uint32_t* large = lists + np;
+  uint32_t* fsmall = c.s_cls.as<uint32_t>(2 * np + 64);
+  uint32_t* list = fsmall + np + 32;
+  uint64_t* off = c.s_tmp2.as<uint64_t>(np + 16);
   uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
-  OverflowCtl* ctl = c.s_misc.as<OverflowCtl>(4) + 1;
+  char* misc = c.s_misc.as<char>(256);
+  uint64_t* d_nsmall = reinterpret_cast<uint64_t*>(misc);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(misc + 64);
+  OverflowCtl* ctl = reinterpret_cast<OverflowCtl*>(misc + 128);
   hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, ctl);
-  hipLaunchKernelGGL(k_classify, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, small, large, ctl);
+  hipLaunchKernelGGL(k_class_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, fsmall);
+  GSJ_LAUNCHED();
+  scan_u32_to_u64_async(c, fsmall, off, np, d_nsmall);
+  hipLaunchKernelGGL(k_partition, dim3(div_up(np, 256)), dim3(256), 0, st, fsmall, off, np, d_nsmall, list, counts);
   GSJ_LAUNCHED();
   const int T = 256;
   const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
   GSJ_HIP(hipEventRecord(c.ev[0], st));
-  {   // 16-lane groups: 16 pairs per block
-    const size_t sm = (size_t)(T / 16) * scratch_bytes(CAP_SMALL, rel);
+  {   // small: 16-lane groups, 16 pairs per block
+    const size_t sm = (size_t)(T / 16) * small_bytes(rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 16), max_blocks);
-    hipLaunchKernelGGL(k_refine<16>, dim3(g), dim3(T), sm, st, P, R, pa, pb, small, 0, rp, out, CAP_SMALL, ovf, ctl);
+    hipLaunchKernelGGL(k_refine_small<16>, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[1], st));
-  {   // 64-lane groups: 4 pairs per block
-    const size_t sm = (size_t)(T / 64) * scratch_bytes(CAP_LARGE, rel);
+  constexpr int CAP_LARGE = 128;
+  {   // large: one wave per pair, 4 pairs per block
+    const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), max_blocks);
-    hipLaunchKernelGGL(k_refine<64>, dim3(g), dim3(T), sm, st, P, R, pa, pb, large, 1, rp, out, CAP_LARGE, ovf, ctl);
+    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, CAP_LARGE, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], st));
-  OverflowCtl* h = c.h_scal.as<OverflowCtl>(2);
-  GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
+  struct Host { uint32_t counts[2]; OverflowCtl ctl; };
+  Host* h = c.h_scal.as<Host>(2);
+  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 8, hipMemcpyDeviceToHost, st));
+  GSJ_HIP(hipMemcpyAsync(&h->ctl, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
-  const uint32_t nover = h->count, need = h->need;
+  const uint32_t nover = h->ctl.count, need = h->ctl.need;
   if (stats) {
     float ms_s = 0, ms_l = 0;
     GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
     GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[1], c.ev[2]));
     stats->k_small_ms += ms_s; stats->k_large_ms += ms_l;
-    stats->n_small += h->n_small; stats->n_large += h->n_large; stats->n_over += nover;
+    stats->n_small += h->counts[0]; stats->n_large += h->counts[1]; stats->n_over += nover;
     stats->launches++;
   }
   if (nover) {
-    const size_t per = scratch_bytes((int)need, rel);
+    const size_t per = list_bytes((int)need, rel);
     const uint64_t budget = 1ull << 30;
     uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nover, budget / std::max<size_t>(per, 1)));
     waves = (waves + 3) / 4 * 4;
