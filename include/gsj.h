@@ -98,8 +98,9 @@ typedef struct {
   uint64_t hits;           /* result rows delivered (parent pairs) */
   uint64_t heavy;          /* pairs whose local edge lists exceeded on-chip capacity */
   uint64_t batches;
-  /* refinement kernels, timed with HIP events on the context's stream: 16-lane class (both polygons <= 32 edges),
-   * 64-lane class, and the global-memory overflow pass; summed durations and pairs refined by each */
+  /* pair kernels, timed with HIP events on the context's stream: 16-lane class (both polygons <= 32 stored vertices
+   * and <= 8 rings; point joins: the per-pair kernel), 64-lane class, and the global-memory overflow pass; summed
+   * durations and pairs handled by each */
   double k_small_ms, k_large_ms, k_over_ms;
   uint64_t n_small, n_large, n_over;
   uint64_t launches;       /* refinement launches (sub-batches) */
@@ -124,6 +125,25 @@ int gsj_points_in_polygons(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* 
 /* ST_ClusterIntersecting over the parents of P (replaces cluster_intersecting, engine/engine.cuh:1216-1234):
  * root[g] = smallest parent index of g's connected component, for g in [0, nparent). P needs a BVH. */
 int gsj_cluster(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, uint32_t* root, uint64_t nroot, gsj_join_stats* stats);
+
+/* ------------------------------------------------------------------ distance joins (lon/lat, metres, WGS84)
+ * Row callback of the distance joins: pairs[2i], pairs[2i+1] = (a, b) (polygon joins: parent ids of P and R; point
+ * joins: point index, parent polygon id); dist[i] = the geography distance in metres when report_distance was set
+ * (0 when the geometries intersect or one contains the other), else NULL. Return nonzero to cancel. */
+typedef int (*gsj_dist_cb)(void* user, const uint32_t* pairs, const double* dist, size_t n);
+
+/* ST_DWithin(a::geography, b::geography, d) of polygon layers (replaces buffer_join_stream in DWITHIN mode,
+ * engine/buffer.cuh:892-991, 994-998; pair test dw_pair, :185-237). P's probe boxes are widened by d metres; R needs
+ * a BVH. `same`: P and R are the same layer (pieces of one parent are within any distance of each other; pair with
+ * opts->upper = 1 for each unordered pair once). report_distance: new output (the reference reports NaN,
+ * daemon/gpujoind.cu:468). Only opts->upper, batch_q, max_pairs and cancel are used. */
+int gsj_dwithin(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, double d, int same, int report_distance, const gsj_join_opts* opts,
+                gsj_dist_cb cb, void* user, gsj_join_stats* stats);
+/* ST_DWithin of points against polygons (replaces points_buffer_join_stream in DWITHIN mode, engine/buffer.cuh:
+ * 1137-1229; per pair k_pt_dwithin, :1018-1049): rows (point index, parent polygon id), inside or within d of the
+ * real boundary. */
+int gsj_points_dwithin(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, double d, int report_distance, const gsj_join_opts* opts,
+                       gsj_dist_cb cb, void* user, gsj_join_stats* stats);
 
 /* ------------------------------------------------------------------ DE-9IM helpers (engine/engine.cuh:871-888) */
 void gsj_de9im_str(uint32_t code, char out[9]);
@@ -160,7 +180,7 @@ int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64
  *   kind "parcels_grid": config-1 parcels (one 4..12-vertex star polygon per cell of an nx x ny grid)
  *   kind "regions_grid": config-1 regions (kx x ky tiles, `edge_verts` vertices per side, shared jittered borders)
  *   kind "parcels_mesh": config-2 parcels (jittered lattice, lognormal vertex counts, shared / overlapping edges)
- * `spec` is "key=value;key=value" (see DESIGN notes in gen.cpp for keys); returns the polygon set. */
+ * `spec` is "key=value;key=value" (keys and defaults in hostprep.cpp, gen_*); returns the polygon set. */
 int gsj_generate(const char* kind, const char* spec, gsj_hostset** out);
 /* config-3 points: "n=..;seed=..;x0=..;y0=..;w=..;h=..;clusters=..;sigma_m=..;frac_clustered=.." */
 int gsj_generate_points(const char* spec, double** xy, uint64_t* n);

@@ -45,6 +45,14 @@ def load(flops=False):
         "orc_flat_copy": (None, [vp, u64p, u64p, dp, u8p, u32p]), "orc_flat_free": (None, [vp]),
         "orc_dedup": (vp, [u64p, u64p, dp, u64]), "orc_dedup_n": (u64, [vp]), "orc_dedup_copy": (None, [vp, u32p, u64p, u32p]),
         "orc_dedup_free": (None, [vp]),
+        # distance joins and their geodesy (buffer.cpp)
+        "orc_gm_sin": (d, [d]), "orc_gm_cos": (d, [d]), "orc_gm_tan": (d, [d]), "orc_gm_atan": (d, [d]), "orc_gm_atan2": (d, [d, d]),
+        "orc_geo_m_per_deg": (None, [d, dp, dp]), "orc_geo_deg_cover": (None, [d, d, d, i32, dp, dp]),
+        "orc_geo_utm_k": (d, [d] * 4), "orc_geo_vincenty": (d, [d] * 4), "orc_geo_geog_dist": (d, [d] * 8 + [i32]),
+        "orc_dw_pair": (i32, [vp, vp, u32, u32, d, i32, dp]),
+        "orc_dw_new": (vp, []), "orc_dw_free": (None, [vp]), "orc_dw_rows": (u64, [vp]), "orc_dw_candidates": (u64, [vp]),
+        "orc_dw_copy": (None, [vp, u32p, dp]), "orc_dwithin": (None, [vp, vp, d, i32, i32, i32, vp]),
+        "orc_points_dwithin": (None, [vp, vp, d, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -278,3 +286,67 @@ class _FlopLayer:
             self.lib.orc_layer_free(self.h)
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------------------------------- distance joins
+def gm(name, *a):
+    """Portable elementary function of the engine (sin, cos, tan, atan, atan2) as built on the host."""
+    return getattr(load(), "orc_gm_" + name)(*a)
+
+
+def m_per_deg(lat):
+    kx, ky = C.c_double(), C.c_double()
+    load().orc_geo_m_per_deg(lat, C.byref(kx), C.byref(ky))
+    return kx.value, ky.value
+
+
+def deg_cover(y0, y1, d, larger=True):
+    dx, dy = C.c_double(), C.c_double()
+    load().orc_geo_deg_cover(y0, y1, d, int(larger), C.byref(dx), C.byref(dy))
+    return dx.value, dy.value
+
+
+def utm_k(glon, glat, lon, lat):
+    return load().orc_geo_utm_k(glon, glat, lon, lat)
+
+
+def vincenty(lon1, lat1, lon2, lat2):
+    return load().orc_geo_vincenty(lon1, lat1, lon2, lat2)
+
+
+def geog_dist(a0, a1, b0, b1=None):
+    """Geography distance (m) of edge a0-a1 from edge b0-b1, or from point b0 when b1 is None."""
+    pt = b1 is None
+    b1 = b0 if pt else b1
+    return load().orc_geo_geog_dist(a0[0], a0[1], a1[0], a1[1], b0[0], b0[1], b1[0], b1[1], int(pt))
+
+
+def dw_pair(P: Layer, R: Layer, a: int, b: int, d: float, report=True):
+    dist = C.c_double()
+    hit = load().orc_dw_pair(P.h, R.h, a, b, d, int(report), C.byref(dist))
+    return bool(hit), dist.value
+
+
+def _dw_result(r):
+    lib = load()
+    n = lib.orc_dw_rows(r)
+    pairs = np.zeros((n, 2), np.uint32)
+    dist = np.zeros(n, np.float64)
+    lib.orc_dw_copy(r, _p(pairs, C.c_uint32), _p(dist, C.c_double))
+    out = {"pairs": pairs, "dist": dist, "candidates": lib.orc_dw_candidates(r)}
+    lib.orc_dw_free(r)
+    return out
+
+
+def dwithin(P: Layer, R: Layer, d: float, same=False, upper=False, report=False):
+    lib = load()
+    r = lib.orc_dw_new()
+    lib.orc_dwithin(P.h, R.h, d, int(same), int(upper), int(report), r)
+    return _dw_result(r)
+
+
+def points_dwithin(pts: Layer, R: Layer, d: float, report=False):
+    lib = load()
+    r = lib.orc_dw_new()
+    lib.orc_points_dwithin(pts.h, R.h, d, int(report), r)
+    return _dw_result(r)

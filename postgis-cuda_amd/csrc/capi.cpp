@@ -231,6 +231,28 @@ int gsj_points_in_polygons(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* 
   });
 }
 
+int gsj_dwithin(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, double d, int same, int report_distance, const gsj_join_opts* opts,
+                gsj_dist_cb cb, void* user, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && P && R, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(P->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::dwithin_run(ctx->c, P->L, R->L, d, same != 0, report_distance != 0, defaults(opts), cb, user, stats);
+  });
+}
+
+int gsj_points_dwithin(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, double d, int report_distance, const gsj_join_opts* opts,
+                       gsj_dist_cb cb, void* user, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && pts && R, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(pts->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::points_dwithin_run(ctx->c, pts->L, R->L, d, report_distance != 0, defaults(opts), cb, user, stats);
+  });
+}
+
 int gsj_cluster(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, uint32_t* root, uint64_t nroot, gsj_join_stats* stats) {
   return guarded([&] {
     GSJ_REQUIRE(ctx && P, GSJ_E_ARG, "null context or layer");

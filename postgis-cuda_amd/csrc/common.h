@@ -1,5 +1,6 @@
 // Engine-internal plumbing: status codes, HIP error checks, grow-only device scratch, stream timers.
 #pragma once
+#include <chrono>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
@@ -77,6 +78,11 @@ struct PinBuf {
 };
 
 // event pair on a stream; elapsed() synchronises on the stop event
+struct WallTimer {   // host wall clock around synchronised device work
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
 struct StreamTimer {
   hipEvent_t a = nullptr, b = nullptr;
   hipStream_t s = nullptr;

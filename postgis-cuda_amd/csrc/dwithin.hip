@@ -1,0 +1,475 @@
+// Distance joins on lon/lat layers: ST_DWithin(A::geography, B::geography, d) for polygon x polygon and
+// point x polygon (reference: engine/buffer.cuh:185-273 dw_pair / k_dwithin / k_dwithin_heavy, 703-715 k_expand_bb,
+// 892-991 dwithin_join_stream, 1004-1049 k_expand_pts / k_pt_dwithin, 1137-1229 points_buffer_join_stream).
+//
+// Candidates: probe boxes widened by d metres (degrees from the WGS84 radii at the box's worst latitude), rounded
+// outward to float, traversed against the build layer's LBVH. Per candidate piece pair: the real edges of both
+// pieces inside the widened overlap box (LDS lists, ballot-compacted), the closest distance of every near edge pair
+// in a local metric frame at the window's latitude, re-measured as a geography distance (great-circle closest points
+// + Vincenty) when it falls inside the frame's error band around d; containment by crossing parity when no boundary
+// is near. With report_distance, the distance of every row: 0 when the boundaries cross or one piece contains the
+// other, else the minimum geography distance over the edge pairs that could hold the minimum (new output; the
+// reference reports NaN). Piece pairs fold to unique parent pairs with the minimum distance.
+//
+// Work classes as in the refinement: small pairs (both pieces <= 32 stored vertices) on 16-lane groups with 32-edge
+// LDS lists, larger pairs on a wave with 128-edge lists, overflowing pairs on a wave with lists in global memory.
+// All transcendental math is the portable set of gmath.h, so every threshold decision and distance has the bits of
+// the CPU restatement.
+#include "pairs.h"
+#include "geodesy.h"
+
+namespace gsj {
+
+namespace {
+
+// probe boxes of polygons widened by d metres (buffer.cuh:703-715)
+__global__ void k_expand_polys(const DPolys P, double d, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const double2 o = P.origin[i], e = P.ext[i];
+  double dx, dy;
+  geo::deg_cover(o.y, o.y + e.y, d, true, dx, dy);
+  out[i] = make_float4(geo::f_down(o.x - dx), geo::f_down(o.y - dy), geo::f_up(o.x + e.x + dx), geo::f_up(o.y + e.y + dy));
+}
+
+// probe boxes of points widened by d metres (buffer.cuh:1004-1016)
+__global__ void k_expand_points(const DPoints pts, double d, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pts.n) return;
+  const double2 p = pts.xy[i];
+  double dx, dy;
+  geo::deg_cover(p.y, p.y, d, true, dx, dy);
+  out[i] = make_float4(geo::f_down(p.x - dx), geo::f_down(p.y - dy), geo::f_up(p.x + dx), geo::f_up(p.y + dy));
+}
+
+// real probe vertex of ring r (engine/engine.cuh:322-327); false for an empty ring or one without real edges
+__device__ __forceinline__ bool probe_real(const DPolys& S, uint32_t r, double shx, double shy, double& x, double& y) {
+  if (S.rvo[r] == S.rvo[r + 1]) return false;
+  return probe_vertex(S, r, shx, shy, x, y);
+}
+
+// One piece pair on a lane group (buffer.cuh:185-237). Returns 1 / 0, or -1 when a side has more than cap edges in
+// the window (need = edges required). With report and a hit, dist = the pair's distance (see the file comment).
+template <int G>
+__device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, double d, bool report, Ed* LA, Ed* LB, int cap, int gl,
+                       double& dist, int& need) {
+  const double2 oa = P.origin[a], ob = R.origin[b], ea = P.ext[a], eb = R.ext[b];
+  const double sx = ob.x - oa.x, sy = ob.y - oa.y;
+  double gx, gy;
+  geo::deg_cover(fmin(oa.y, ob.y), fmax(oa.y + ea.y, ob.y + eb.y), d, true, gx, gy);
+  const Box ov{fmax(-gx, sx - gx), fmax(-gy, sy - gy), fmin(ea.x + gx, sx + eb.x + gx), fmin(ea.y + gy, sy + eb.y + gy)};
+  if (ov.x0 > ov.x1 || ov.y0 > ov.y1) return 0;
+  const double ash = oa.x - oa.x, ashy = oa.y - oa.y;   // A in its own frame
+  const int na = collect<G>(P, a, ash, ashy, ov, LA, cap, gl);
+  const int nb = collect<G>(R, b, sx, sy, ov, LB, cap, gl);
+  if (na > cap || nb > cap) { need = max(na, nb); return -1; }
+  wave_lds_sync();
+  const double phi0 = oa.y + 0.5 * (ov.y0 + ov.y1);
+  double kx, ky;
+  geo::m_per_deg(phi0, kx, ky);
+  const double tl = fabs(gm::tan(phi0 * geo::DEG)) + 0.01;
+  // relative error of the frame metric over the window (scale varies with latitude), with a 2x margin
+  const double delta = 2.0 * 0.5 * (ov.y1 - ov.y0) * geo::DEG * (tl + 0.04) + 1e-9;
+  bool hit = false, cross = false;
+  double m_up = INFINITY;
+  for (int i = gl; i < na && (report || !hit); i += G) {
+    const Ed e = LA[i];
+    if (e.syn) continue;
+    const double bx0 = fmin(e.x0, e.x1) - gx, bx1 = fmax(e.x0, e.x1) + gx, by0 = fmin(e.y0, e.y1) - gy, by1 = fmax(e.y0, e.y1) + gy;
+    const double ux = (e.x1 - e.x0) * kx, uy = (e.y1 - e.y0) * ky, La2 = ux * ux + uy * uy;
+    for (int j = 0; j < nb; j++) {
+      const Ed f = LB[j];
+      if (f.syn || fmax(f.x0, f.x1) < bx0 || fmin(f.x0, f.x1) > bx1 || fmax(f.y0, f.y1) < by0 || fmin(f.y0, f.y1) > by1) continue;
+      if (seg_isect(e.x0, e.y0, e.x1, e.y1, f.x0, f.y0, f.x1, f.y1)) {
+        hit = cross = true;
+        if (!report) break;
+        continue;
+      }
+      double px, py, qx, qy;
+      const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
+      const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
+      const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+      const double dlo = d - band, dhi = d + band;
+      if (d2 <= dlo * dlo) hit = true;
+      else if (d2 <= dhi * dhi &&
+               geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1, false) <= d)
+        hit = true;
+      if (report) m_up = fmin(m_up, sqrt(d2) + band);
+      else if (hit) break;
+    }
+  }
+  hit = gany<G>(hit);
+  cross = gany<G>(cross);
+  bool contained = false;
+  if (!hit || report) {
+    // no boundary within d: within d iff one contains the other (a real vertex of one inside the other piece)
+    const double cx = ov.x0 + 0.381966 * (ov.x1 - ov.x0), cy = ov.y0 + 0.414213 * (ov.y1 - ov.y0);
+    const bool inB_c = pip_full<G>(R, b, sx, sy, cx, cy, gl), inA_c = pip_full<G>(P, a, ash, ashy, cx, cy, gl);
+    const ListSrc SA{LA, na}, SB{LB, nb};
+    bool h = false;
+    for (uint32_t r = P.gro[a] + gl; r < P.gro[a + 1] && !h; r += G) {
+      double x, y;
+      if (probe_real(P, r, ash, ashy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(SB, cx, cy, inB_c, x, y)) h = true;
+    }
+    for (uint32_t r = R.gro[b] + gl; r < R.gro[b + 1] && !h; r += G) {
+      double x, y;
+      if (probe_real(R, r, sx, sy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(SA, cx, cy, inA_c, x, y)) h = true;
+    }
+    contained = gany<G>(h);
+  }
+  const bool within = hit || contained;
+  if (report && within) {
+    if (cross || contained) dist = 0.0;
+    else {
+      // minimum geography distance over the edge pairs whose frame distance could hold it
+      const double m = group_min<G>(m_up);
+      double best = INFINITY;
+      for (int i = gl; i < na; i += G) {
+        const Ed e = LA[i];
+        if (e.syn) continue;
+        const double bx0 = fmin(e.x0, e.x1) - gx, bx1 = fmax(e.x0, e.x1) + gx, by0 = fmin(e.y0, e.y1) - gy, by1 = fmax(e.y0, e.y1) + gy;
+        const double ux = (e.x1 - e.x0) * kx, uy = (e.y1 - e.y0) * ky, La2 = ux * ux + uy * uy;
+        for (int j = 0; j < nb; j++) {
+          const Ed f = LB[j];
+          if (f.syn || fmax(f.x0, f.x1) < bx0 || fmin(f.x0, f.x1) > bx1 || fmax(f.y0, f.y1) < by0 || fmin(f.y0, f.y1) > by1) continue;
+          double px, py, qx, qy;
+          const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
+          const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
+          const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+          if (sqrt(d2) - band > m) continue;
+          best = fmin(best, geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1,
+                                           false));
+        }
+      }
+      dist = group_min<G>(best);
+    }
+  }
+  return within ? 1 : 0;
+}
+
+struct DwArgs {
+  DPolys P, R;
+  const uint32_t* pa;
+  const uint32_t* pb;
+  double d;
+  int same, report;
+  uint8_t* hit;
+  double* dist;
+};
+
+// pieces of one parent are within any distance of each other on a self-join (buffer.cuh:246-249)
+__device__ __forceinline__ bool self_piece_pair(const DwArgs& A, uint32_t a, uint32_t b) {
+  if (!A.same) return false;
+  const uint32_t qa = A.P.parent ? A.P.parent[a] : a, qb = A.R.parent ? A.R.parent[b] : b;
+  return qa == qb;
+}
+
+// list classes: a G-lane group per pair, CAP-edge LDS lists per side; class `which` (0 small, 1 large) of the
+// partition; pairs whose lists overflow are queued
+template <int G, int CAP>
+__global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, int which,
+                                                  uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+  __shared__ Ed lds[256 / G][2 * CAP];
+  const int gpb = blockDim.x / G;
+  const int gib = threadIdx.x / G, gl = threadIdx.x % G;
+  Ed* LA = lds[gib];
+  Ed* LB = LA + CAP;
+  const uint32_t n = counts[which];
+  const uint32_t* L = list + (which ? counts[0] : 0u);
+  for (uint32_t k = blockIdx.x * gpb + gib; k < n; k += gridDim.x * gpb) {
+    const uint32_t p = L[k];
+    const uint32_t a = A.pa[p], b = A.pb[p];
+    if (self_piece_pair(A, a, b)) {
+      if (gl == 0) { A.hit[p] = 1; if (A.dist) A.dist[p] = 0.0; }
+      continue;
+    }
+    double dist = NAN;
+    int need = 0;
+    const int r = dw_eval<G>(A.P, A.R, a, b, A.d, A.report != 0, LA, LB, CAP, gl, dist, need);
+    if (gl == 0) {
+      if (r < 0) queue_overflow(p, need, ovf_list, ctl);
+      else {
+        A.hit[p] = (uint8_t)r;
+        if (A.dist) A.dist[p] = r ? dist : NAN;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// overflow: a wave per pair, lists in a per-wave slab of global memory
+__global__ void __launch_bounds__(256) k_dw_overflow(const DwArgs A, const uint32_t* __restrict__ list, uint32_t n, int cap, Ed* __restrict__ slab) {
+  const int wpb = blockDim.x / 64;
+  const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
+  const uint64_t wave = (uint64_t)blockIdx.x * wpb + wib;
+  Ed* LA = slab + wave * 2 * (uint64_t)cap;
+  Ed* LB = LA + cap;
+  for (uint32_t k = (uint32_t)wave; k < n; k += gridDim.x * wpb) {
+    const uint32_t p = list[k];
+    double dist = NAN;
+    int need = 0;
+    const int r = dw_eval<64>(A.P, A.R, A.pa[p], A.pb[p], A.d, A.report != 0, LA, LB, cap, gl, dist, need);
+    if (gl == 0) {
+      A.hit[p] = r > 0 ? 1 : 0;
+      if (A.dist) A.dist[p] = r > 0 ? dist : NAN;
+    }
+    wave_lds_sync();
+  }
+}
+
+// thread per (point, piece): inside the piece, or within d of its real boundary (buffer.cuh:1018-1049); with report,
+// the distance (0 inside, else the minimum geography distance over the edges that could hold it)
+__global__ void __launch_bounds__(256) k_pt_dwithin(const DPoints pts, const DPolys R, const uint32_t* __restrict__ pa,
+                                                    const uint32_t* __restrict__ pb, uint64_t np, double d, int report, uint8_t* __restrict__ hit,
+                                                    double* __restrict__ dist) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint32_t g = pb[i];
+  const double2 pt = pts.xy[pa[i]], o = R.origin[g];
+  const double px = pt.x - o.x, py = pt.y - o.y;
+  double kx, ky;
+  geo::m_per_deg(pt.y, kx, ky);
+  const double gy = d / ky;
+  const double tl = fabs(gm::tan(pt.y * geo::DEG)) + 0.01;
+  const double delta = 2.0 * gy * geo::DEG * (tl + 0.04) + 1e-9;
+  const double ox = (o.x - o.x) - 1.0;
+  bool par = false, h = false;
+  double m_up = INFINITY;
+  const uint32_t r1 = R.gro[g + 1];
+  for (uint32_t r = R.gro[g]; r < r1 && (report || !h); r++) {
+    const uint64_t s0 = R.rvo[r], e = R.rvo[r + 1];
+    double2 v0 = R.xy[s0];
+    for (uint64_t j = s0 + 1; j < e && (report || !h); j++) {
+      const double2 v1 = R.xy[j];
+      const double x0 = v0.x, y0 = v0.y, x1 = v1.x, y1 = v1.y;
+      if (!syn_bit(R.synb, j - 1)) {
+        double qx, qy;
+        const double d2 = geo::pt_seg_d2(px * kx, py * ky, x0 * kx, y0 * ky, x1 * kx, y1 * ky, qx, qy);
+        const double ex = (x1 - x0) * kx, ey = (y1 - y0) * ky;
+        const double band = geo::geog_band(d, delta, ex * ex + ey * ey, 0.0, tl), dlo = d - band, dhi = d + band;
+        if (d2 <= dlo * dlo || (d2 <= dhi * dhi && geo::geog_dist(o.x + x0, o.y + y0, o.x + x1, o.y + y1, pt.x, pt.y, pt.x, pt.y, true) <= d)) h = true;
+        if (report) m_up = fmin(m_up, sqrt(d2) + band);
+      }
+      if (!((y0 > py) == (y1 > py) && y0 != py && y1 != py)) {
+        const bool so = left_of(x0, y0, x1, y1, ox, py), sp = left_of(x0, y0, x1, y1, px, py);
+        if (so != sp) {
+          const bool su = orient(ox, py, px, py, x0, y0) > 0, sv = orient(ox, py, px, py, x1, y1) > 0;
+          if (su != sv) par = !par;
+        }
+      }
+      v0 = v1;
+    }
+  }
+  const bool within = h || par;
+  hit[i] = within ? 1 : 0;
+  if (!dist) return;
+  if (!within) { dist[i] = NAN; return; }
+  if (par) { dist[i] = 0.0; return; }
+  double best = INFINITY;
+  for (uint32_t r = R.gro[g]; r < r1; r++) {
+    const uint64_t s0 = R.rvo[r], e = R.rvo[r + 1];
+    for (uint64_t j = s0 + 1; j < e; j++) {
+      if (syn_bit(R.synb, j - 1)) continue;
+      const double2 v0 = R.xy[j - 1], v1 = R.xy[j];
+      double qx, qy;
+      const double d2 = geo::pt_seg_d2(px * kx, py * ky, v0.x * kx, v0.y * ky, v1.x * kx, v1.y * ky, qx, qy);
+      const double ex = (v1.x - v0.x) * kx, ey = (v1.y - v0.y) * ky;
+      const double band = geo::geog_band(d, delta, ex * ex + ey * ey, 0.0, tl);
+      if (sqrt(d2) - band > m_up) continue;
+      best = fmin(best, geo::geog_dist(o.x + v0.x, o.y + v0.y, o.x + v1.x, o.y + v1.y, pt.x, pt.y, pt.x, pt.y, true));
+    }
+  }
+  dist[i] = best;
+}
+
+// ---- compaction: unique (parent a, parent b) rows of the hit pairs, minimum distance -----------------------------
+__global__ void k_hit_u32(const uint8_t* __restrict__ hit, uint64_t n, uint32_t* __restrict__ f) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = hit[i] ? 1u : 0u;
+}
+__global__ void k_dw_keys(const uint32_t* __restrict__ Ppar, const uint32_t* __restrict__ Rpar, const uint32_t* __restrict__ pa,
+                          const uint32_t* __restrict__ pb, const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint64_t n,
+                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !f[i]) return;
+  const uint64_t a = Ppar ? Ppar[pa[i]] : pa[i], b = Rpar ? Rpar[pb[i]] : pb[i];
+  keys[off[i]] = (a << 32) | b;
+  vals[off[i]] = (uint32_t)i;
+}
+__global__ void k_key_heads(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ head) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) head[j] = (j == 0 || keys[j] != keys[j - 1]) ? 1u : 0u;
+}
+__global__ void k_dw_fold(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ head,
+                          const uint64_t* __restrict__ hoff, uint64_t m, const double* __restrict__ dist, uint32_t* __restrict__ orow,
+                          double* __restrict__ odist) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m || !head[j]) return;
+  const uint64_t o = hoff[j];
+  orow[2 * o] = (uint32_t)(keys[j] >> 32);
+  orow[2 * o + 1] = (uint32_t)keys[j];
+  if (!dist) return;
+  double v = INFINITY;
+  for (uint64_t k = j; k < m && keys[k] == keys[j]; k++) v = fmin(v, dist[vals[k]]);
+  odist[o] = v;
+}
+
+// rows of one sub-batch on the device; returns the row count
+uint64_t dw_compact(Context& c, const uint32_t* Ppar, const uint32_t* Rpar, const uint32_t* pa, const uint32_t* pb, uint64_t np, const uint8_t* hit,
+                    const double* dist, uint32_t* orow, double* odist) {
+  hipStream_t st = c.stream;
+  uint32_t* f = c.s_tmp.as<uint32_t>(np + 16);
+  uint64_t* off = c.s_offs.as<uint64_t>(np + 16);
+  hipLaunchKernelGGL(k_hit_u32, dim3(div_up(np, 256)), dim3(256), 0, st, hit, np, f);
+  GSJ_LAUNCHED();
+  const uint64_t nh = scan_u32_to_u64(c, f, off, np);
+  if (nh == 0) return 0;
+  uint64_t* keys = c.s_keys.as<uint64_t>(nh + 16);
+  uint64_t* keys2 = c.s_keys2.as<uint64_t>(nh + 16);
+  uint32_t* vals = c.s_vals.as<uint32_t>(nh + 16);
+  uint32_t* vals2 = c.s_vals2.as<uint32_t>(nh + 16);
+  hipLaunchKernelGGL(k_dw_keys, dim3(div_up(np, 256)), dim3(256), 0, st, Ppar, Rpar, pa, pb, f, off, np, keys, vals);
+  GSJ_LAUNCHED();
+  radix_sort_pairs(c, keys, keys2, vals, vals2, nh, 0, 64);
+  uint32_t* head = c.s_seg.as<uint32_t>(nh + 16);
+  uint64_t* hoff = c.s_tmp2.as<uint64_t>(nh + 16);
+  hipLaunchKernelGGL(k_key_heads, dim3(div_up(nh, 256)), dim3(256), 0, st, keys, nh, head);
+  GSJ_LAUNCHED();
+  const uint64_t nrows = scan_u32_to_u64(c, head, hoff, nh);
+  hipLaunchKernelGGL(k_dw_fold, dim3(div_up(nh, 256)), dim3(256), 0, st, keys, vals, head, hoff, nh, dist, orow, odist);
+  GSJ_LAUNCHED();
+  return nrows;
+}
+
+// deliver one sub-batch's rows through the callback
+void dw_deliver(Context& c, const uint32_t* orow, const double* odist, uint64_t nrows, gsj_join_stats& s, gsj_dist_cb cb, void* user) {
+  if (nrows == 0) return;
+  WallTimer t4;
+  uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nrows);
+  double* hd = odist ? c.h_area.as<double>(nrows) : nullptr;
+  GSJ_HIP(hipMemcpyAsync(hrow, orow, nrows * 8, hipMemcpyDeviceToHost, c.stream));
+  if (hd) GSJ_HIP(hipMemcpyAsync(hd, odist, nrows * 8, hipMemcpyDeviceToHost, c.stream));
+  GSJ_HIP(hipStreamSynchronize(c.stream));
+  s.copy_ms += t4.ms();
+  if (cb && cb(user, hrow, hd, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "distance join cancelled by the row callback");
+}
+
+}  // namespace
+
+void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool same, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
+                 gsj_join_stats* st) {
+  GSJ_REQUIRE(!PL.points && !RL.points, GSJ_E_ARG, "dwithin: both layers must be polygon layers (points: gsj_points_dwithin)");
+  GSJ_REQUIRE(RL.has_bvh, GSJ_E_ARG, "dwithin: build layer has no BVH (call gsj_bvh_build)");
+  GSJ_REQUIRE(d >= 0 && d < 1e7, GSJ_E_ARG, "dwithin: distance must be in [0, 1e7) metres");
+  gsj_join_stats s{};
+  if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
+  hipStream_t stream = c.stream;
+  const float4* qbb = PL.poly.bb;
+  if (d > 0) {
+    float4* e = c.s_qbb.as<float4>(PL.n + 16);
+    hipLaunchKernelGGL(k_expand_polys, dim3(div_up(PL.n, 256)), dim3(256), 0, stream, PL.poly, d, e);
+    GSJ_LAUNCHED();
+    qbb = e;
+  }
+  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, PL.h_parent, RL.bvh, o.upper != 0, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+    WallTimer t2;
+    DwArgs A{PL.poly, RL.poly, pa, pb, d, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr};
+    const PairClasses pc = partition_pairs(c, PL.poly, RL.poly, pa, pb, np);
+    uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
+    const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
+    GSJ_HIP(hipEventRecord(c.ev[0], stream));
+    hipLaunchKernelGGL((k_dw_lists<16, 32>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 0, ovf,
+                       pc.ctl);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[1], stream));
+    hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 1, ovf,
+                       pc.ctl);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[2], stream));
+    struct Host { uint32_t counts[2]; OverflowCtl ctl; };
+    Host* h = c.h_scal.as<Host>(2);
+    GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipMemcpyAsync(&h->ctl, pc.ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    float ms_s = 0, ms_l = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
+    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[1], c.ev[2]));
+    s.k_small_ms += ms_s; s.k_large_ms += ms_l;
+    s.n_small += h->counts[0]; s.n_large += h->counts[1];
+    s.launches++;
+    const uint32_t nover = h->ctl.count;
+    if (nover) {
+      const int cap = (int)h->ctl.need;
+      const size_t per = 2 * (size_t)cap * sizeof(Ed);
+      uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nover, (1ull << 30) / per));
+      waves = (waves + 3) / 4 * 4;
+      Ed* slab = c.s_heavy_edges.as<Ed>(2 * (uint64_t)cap * waves + 16);
+      GSJ_HIP(hipEventRecord(c.ev[3], stream));
+      hipLaunchKernelGGL(k_dw_overflow, dim3((uint32_t)(waves / 4)), dim3(256), 0, stream, A, ovf, nover, cap, slab);
+      GSJ_LAUNCHED();
+      GSJ_HIP(hipEventRecord(c.ev[4], stream));
+      GSJ_HIP(hipEventSynchronize(c.ev[4]));
+      float ms_o = 0;
+      GSJ_HIP(hipEventElapsedTime(&ms_o, c.ev[3], c.ev[4]));
+      s.k_over_ms += ms_o;
+      s.n_over += nover;
+      s.heavy += nover;
+    }
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.refine_ms += t2.ms();
+    WallTimer t3;
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
+    double* odist = report ? c.s_out_area.as<double>(np + 16) : nullptr;
+    const uint64_t nrows = dw_compact(c, PL.poly.parent, RL.poly.parent, pa, pb, np, A.hit, A.dist, orow, odist);
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.compact_ms += t3.ms();
+    s.hits += nrows;
+    dw_deliver(c, orow, odist, nrows, s, cb, user);
+  });
+  if (st) *st = s;
+}
+
+void points_dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
+                        gsj_join_stats* st) {
+  GSJ_REQUIRE(PL.points && !RL.points, GSJ_E_ARG, "points_dwithin: needs a point layer and a polygon layer");
+  GSJ_REQUIRE(RL.has_bvh, GSJ_E_ARG, "points_dwithin: polygon layer has no BVH");
+  GSJ_REQUIRE(d >= 0 && d < 1e7, GSJ_E_ARG, "points_dwithin: distance must be in [0, 1e7) metres");
+  gsj_join_stats s{};
+  if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
+  hipStream_t stream = c.stream;
+  const float4* qbb = PL.pts.bb;
+  if (d > 0) {
+    float4* e = c.s_qbb.as<float4>(PL.n + 16);
+    hipLaunchKernelGGL(k_expand_points, dim3(div_up(PL.n, 256)), dim3(256), 0, stream, PL.pts, d, e);
+    GSJ_LAUNCHED();
+    qbb = e;
+  }
+  static const std::vector<uint32_t> no_parent;
+  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+    WallTimer t2;
+    uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
+    double* dist = report ? c.s_area.as<double>(np + 16) : nullptr;
+    GSJ_HIP(hipEventRecord(c.ev[0], stream));
+    hipLaunchKernelGGL(k_pt_dwithin, dim3(div_up(np, 256)), dim3(256), 0, stream, PL.pts, RL.poly, pa, pb, np, d, report ? 1 : 0, hit, dist);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[1], stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+    s.k_small_ms += ms;
+    s.n_small += np;
+    s.launches++;
+    s.refine_ms += t2.ms();
+    WallTimer t3;
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
+    double* odist = report ? c.s_out_area.as<double>(np + 16) : nullptr;
+    const uint64_t nrows = dw_compact(c, nullptr, RL.poly.parent, pa, pb, np, hit, dist, orow, odist);
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.compact_ms += t3.ms();
+    s.hits += nrows;
+    dw_deliver(c, orow, odist, nrows, s, cb, user);
+  });
+  if (st) *st = s;
+}
+
+}  // namespace gsj

@@ -108,6 +108,14 @@ void bvh_build(Layer& L);
 void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
               const uint64_t* offs, uint32_t* out_a, uint32_t* out_b);
 
+// Candidate pairs of the probe boxes qbb[0 .. nq) (probe index = box index) against tree t, in probe batches of at
+// most o.batch_q probes that keep a parent's pieces together (h_parent, may be empty) and sub-batches of at most
+// o.max_pairs candidates (engine/engine.cuh:903-937). fn(pa, pb, np) runs on every non-empty sub-batch with the pairs
+// on the device and the stream synchronised. Adds traverse_ms, batches and candidates to s; polls o.cancel.
+using PairsFn = std::function<void(const uint32_t* pa, const uint32_t* pb, uint64_t np)>;
+void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nq, const std::vector<uint32_t>& h_parent, const DBvh& t, bool upper,
+                              const gsj_join_opts& o, gsj_join_stats& s, const PairsFn& fn);
+
 // ------------------------------------------------------------------------------------------- refinement
 struct RefineParams {   // engine/engine.cuh:517-518
   double tol_rel = 1e-12, eps_par = 1e-7, tol_abs = 1e-11;
@@ -135,5 +143,11 @@ void join_run(Context& c, const Layer& P, const Layer& R, const gsj_join_opts& o
 void points_join_run(Context& c, const Layer& pts, const Layer& R, const gsj_join_opts& o, gsj_points_cb cb, void* user,
                      gsj_join_stats* st);
 void cluster_run(Context& c, const Layer& P, const gsj_join_opts& o, uint32_t* root, uint64_t nroot, gsj_join_stats* st);
+
+// ------------------------------------------------------------------------------------------- distance joins (dwithin.hip)
+void dwithin_run(Context& c, const Layer& P, const Layer& R, double d, bool same, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
+                 gsj_join_stats* st);
+void points_dwithin_run(Context& c, const Layer& pts, const Layer& R, double d, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
+                        gsj_join_stats* st);
 
 }  // namespace gsj

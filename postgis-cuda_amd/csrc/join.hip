@@ -112,11 +112,6 @@ __global__ void k_emit_multi(const DPolys P, const DPolys R, const uint64_t* __r
 
 inline int bits_for(uint64_t maxval) { int b = 1; while (b < 64 && (maxval >> b)) b++; return b; }
 
-struct Timer {   // host wall clock around synchronised device work
-  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-  double ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
-};
-
 }  // namespace
 
 // Compact the refined candidates of one sub-batch into parent-pair rows on the device; returns the row count.
@@ -180,86 +175,41 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
   gsj_join_stats s{};
   if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
   hipStream_t stream = c.stream;
-  const uint32_t batch_q = o.batch_q ? o.batch_q : (1u << 22);
-  const uint64_t max_pairs = o.max_pairs ? o.max_pairs : (128ull << 20);
   RefineParams rp;
   rp.want_area = o.want_area ? 1 : 0;
   rp.want_relate = o.want_relate ? 1 : 0;
   rp.snap_half_ulp = o.snap_half_ulp ? 1 : 0;
-  const std::vector<uint32_t>& hp = PL.h_parent;
-  const uint32_t nP = (uint32_t)PL.n;
-  std::vector<uint32_t> hcounts;
-  for (uint32_t q0 = 0, qn; q0 < nP; q0 = qn) {
-    qn = (uint32_t)std::min<uint64_t>((uint64_t)q0 + batch_q, nP);
-    if (!hp.empty()) while (qn < nP && hp[qn] == hp[qn - 1]) qn++;
-    if (o.cancel && *o.cancel) throw Error(GSJ_E_CANCEL, "join cancelled");
-    const uint32_t nq = qn - q0;
-    Timer tt;
-    uint32_t* counts = c.s_counts.as<uint32_t>(nq + 16);
-    traverse(c, PL.poly.bb, nullptr, q0, nq, RL.bvh, o.upper != 0, counts, nullptr, nullptr, nullptr);
-    uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);   // probe offsets (s_offs is compaction's)
-    const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);
-    s.traverse_ms += tt.ms();
-    // split into sub-batches of at most max_pairs candidates (a probe parent's pieces stay together)
-    std::vector<std::pair<uint32_t, uint32_t>> subs;
-    if (total <= max_pairs) subs.push_back({0, nq});
-    else {
-      hcounts.resize(nq);
-      GSJ_HIP(hipMemcpyAsync(hcounts.data(), counts, (size_t)nq * 4, hipMemcpyDeviceToHost, stream));
-      GSJ_HIP(hipStreamSynchronize(stream));
-      for (uint32_t s0 = 0; s0 < nq;) {
-        uint64_t np = 0;
-        uint32_t s1 = s0;
-        while (s1 < nq && (np + hcounts[s1] <= max_pairs || s1 == s0 || (!hp.empty() && hp[q0 + s1] == hp[q0 + s1 - 1]))) np += hcounts[s1++];
-        GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a probe batch has >= 2^32 candidates (lower max_pairs)");
-        subs.push_back({s0, s1});
-        s0 = s1;
-      }
-    }
-    for (auto [s0, s1] : subs) {
-      if (o.cancel && *o.cancel) throw Error(GSJ_E_CANCEL, "join cancelled");
-      Timer t1;
-      uint64_t np = total;
-      if (subs.size() > 1) np = scan_u32_to_u64(c, counts + s0, offs, s1 - s0);
-      s.batches++;
-      s.candidates += np;
-      if (np == 0) continue;
-      uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
-      uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
-      traverse(c, PL.poly.bb, nullptr, q0 + s0, s1 - s0, RL.bvh, o.upper != 0, nullptr, offs, pa, pb);
-      GSJ_HIP(hipStreamSynchronize(stream));
-      s.traverse_ms += t1.ms();
-      Timer t2;
-      RefineOut ro;
-      ro.isect = c.s_flag.as<uint8_t>(np + 16);
-      ro.area = rp.want_area ? c.s_area.as<double>(np + 16) : nullptr;
-      ro.ev = rp.want_relate ? c.s_ev.as<double4>(np + 16) : nullptr;
-      ro.evb = rp.want_relate ? c.s_evb.as<uint32_t>(np + 16) : nullptr;
-      s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro, &s);
-      GSJ_HIP(hipStreamSynchronize(stream));
-      s.refine_ms += t2.ms();
-      Timer t3;
-      uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
-      double* oarea = c.s_out_area.as<double>(np + 16);
-      uint32_t* ocode = c.s_out_code.as<uint32_t>(np + 16);
-      const uint64_t nrows = compact_rows(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode);
-      GSJ_HIP(hipStreamSynchronize(stream));
-      s.compact_ms += t3.ms();
-      s.hits += nrows;
-      if (hooks && hooks->on_device && nrows) hooks->on_device(orow, rp.want_area ? oarea : nullptr, nrows);
-      if ((hooks && hooks->device_only) || nrows == 0) continue;
-      Timer t4;
-      uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nrows);
-      double* harea = rp.want_area ? c.h_area.as<double>(nrows) : nullptr;
-      uint32_t* hcode = rp.want_relate ? c.h_code.as<uint32_t>(nrows) : nullptr;
-      GSJ_HIP(hipMemcpyAsync(hrow, orow, nrows * 8, hipMemcpyDeviceToHost, stream));
-      if (harea) GSJ_HIP(hipMemcpyAsync(harea, oarea, nrows * 8, hipMemcpyDeviceToHost, stream));
-      if (hcode) GSJ_HIP(hipMemcpyAsync(hcode, ocode, nrows * 4, hipMemcpyDeviceToHost, stream));
-      GSJ_HIP(hipStreamSynchronize(stream));
-      s.copy_ms += t4.ms();
-      if (cb && cb(user, hrow, harea, hcode, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "join cancelled by the row callback");
-    }
-  }
+  for_each_candidate_batch(c, PL.poly.bb, (uint32_t)PL.n, PL.h_parent, RL.bvh, o.upper != 0, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+    WallTimer t2;
+    RefineOut ro;
+    ro.isect = c.s_flag.as<uint8_t>(np + 16);
+    ro.area = rp.want_area ? c.s_area.as<double>(np + 16) : nullptr;
+    ro.ev = rp.want_relate ? c.s_ev.as<double4>(np + 16) : nullptr;
+    ro.evb = rp.want_relate ? c.s_evb.as<uint32_t>(np + 16) : nullptr;
+    s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro, &s);
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.refine_ms += t2.ms();
+    WallTimer t3;
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
+    double* oarea = c.s_out_area.as<double>(np + 16);
+    uint32_t* ocode = c.s_out_code.as<uint32_t>(np + 16);
+    const uint64_t nrows = compact_rows(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode);
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.compact_ms += t3.ms();
+    s.hits += nrows;
+    if (hooks && hooks->on_device && nrows) hooks->on_device(orow, rp.want_area ? oarea : nullptr, nrows);
+    if ((hooks && hooks->device_only) || nrows == 0) return;
+    WallTimer t4;
+    uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nrows);
+    double* harea = rp.want_area ? c.h_area.as<double>(nrows) : nullptr;
+    uint32_t* hcode = rp.want_relate ? c.h_code.as<uint32_t>(nrows) : nullptr;
+    GSJ_HIP(hipMemcpyAsync(hrow, orow, nrows * 8, hipMemcpyDeviceToHost, stream));
+    if (harea) GSJ_HIP(hipMemcpyAsync(harea, oarea, nrows * 8, hipMemcpyDeviceToHost, stream));
+    if (hcode) GSJ_HIP(hipMemcpyAsync(hcode, ocode, nrows * 4, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.copy_ms += t4.ms();
+    if (cb && cb(user, hrow, harea, hcode, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "join cancelled by the row callback");
+  });
   if (st) *st = s;
 }
 

@@ -1,0 +1,157 @@
+// Device building blocks shared by the pair kernels (polygon refinement, distance joins): edge records, lane-group
+// ballots, ballot-compacted edge lists of a geometry inside a box, crossing-parity containment (reference:
+// engine/engine.cuh:265-412), the overflow queue, and the work-class partition of candidate pairs.
+#pragma once
+#include "engine.h"
+
+namespace gsj {
+
+struct Ed { double x0, y0, x1, y1; int syn, cut; };   // 40 B; cut bit 1/2: start/end vertex is a cut point
+struct Box { double x0, y0, x1, y1; };
+
+__device__ __forceinline__ int syn_bit(const uint32_t* synb, uint64_t i) {
+  return synb ? (int)((synb[i >> 5] >> (i & 31)) & 1u) : 0;
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t gballot(bool p) {
+  const uint64_t m = __ballot(p);
+  if constexpr (G == 64) return m;
+  const int sh = (threadIdx.x & 63) & ~(G - 1);
+  return (m >> sh) & ((1ull << G) - 1);
+}
+template <int G> __device__ __forceinline__ bool gany(bool p) { return gballot<G>(p) != 0; }
+
+// make this wave's scratch writes (LDS or global) visible to its other lanes before they read them; also a code
+// motion barrier. Scratch is never shared between waves.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ bool edge_outside(double px, double py, double qx, double qy, const Box& o) {
+  return fmax(px, qx) < o.x0 || fmin(px, qx) > o.x1 || fmax(py, qy) < o.y0 || fmin(py, qy) > o.y1;
+}
+
+// ------------------------------------------------------------------------------------------- edge sources
+// Sources enumerate "the other polygon's edges meeting the overlap box" in ring order.
+struct ListSrc {   // compacted list (large / overflow classes)
+  const Ed* L;
+  int n;
+  __device__ __forceinline__ int size() const { return n; }
+  __device__ __forceinline__ bool get(int k, Ed& e) const { e = L[k]; return true; }
+};
+
+// inside(x) = inside(c) xor parity of segment c->x across the local edges (engine/engine.cuh:400-412)
+template <class Src>
+__device__ bool inside_from_c(const Src& L, double cx, double cy, bool in_c, double x, double y) {
+  bool in = in_c;
+  const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
+  const int n = L.size();
+  for (int k = 0; k < n; k++) {
+    Ed e;
+    if (!L.get(k, e)) continue;
+    if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) continue;
+    const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
+    if (sc == sx) continue;
+    const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
+    if (su != sv) in = !in;
+  }
+  return in;
+}
+
+// ------------------------------------------------------------------------------------------- whole geometries
+// edges of geometry g meeting box ov, in ring order, compacted by group ballot (engine/engine.cuh:358-377);
+// returns the full count (may exceed cap: then the list is incomplete and the pair overflows)
+template <int G>
+__device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, Ed* out, int cap, int gl) {
+  int cnt = 0;
+  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
+  for (uint32_t r = r0; r < r1; r++) {
+    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
+    for (uint64_t base = s; base + 1 < e; base += G) {
+      const uint64_t i = base + gl;
+      bool v = false;
+      Ed ed;
+      if (i + 1 < e) {
+        const double2 p = S.xy[i], q = S.xy[i + 1];
+        ed.x0 = shx + p.x; ed.y0 = shy + p.y; ed.x1 = shx + q.x; ed.y1 = shy + q.y;
+        ed.syn = syn_bit(S.synb, i);
+        ed.cut = S.synb ? ((syn_bit(S.synb, i > s ? i - 1 : e - 2) ? 1 : 0) | (syn_bit(S.synb, i + 2 < e ? i + 1 : s) ? 2 : 0)) : 0;
+        v = !edge_outside(ed.x0, ed.y0, ed.x1, ed.y1, ov);
+      }
+      const uint64_t m = gballot<G>(v);
+      const int pos = cnt + __popcll(m & ((1ull << gl) - 1));
+      if (v && pos < cap) out[pos] = ed;
+      cnt += __popcll(m);
+    }
+  }
+  return cnt;
+}
+
+// inside(m) over every edge of geometry g, read from global memory (engine/engine.cuh:382-397)
+template <int G>
+__device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, double mx, double my, int gl) {
+  const double ox = shx - 1.0, oy = my;
+  bool par = false;
+  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
+  for (uint32_t r = r0; r < r1; r++) {
+    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
+    for (uint64_t j = s + 1 + gl; j < e; j += G) {
+      const double2 p = S.xy[j - 1], q = S.xy[j];
+      const double x0 = shx + p.x, y0 = shy + p.y, x1 = shx + q.x, y1 = shy + q.y;
+      if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;
+      const bool so = left_of(x0, y0, x1, y1, ox, oy), sc = left_of(x0, y0, x1, y1, mx, my);
+      if (so == sc) continue;
+      const bool su = orient(ox, oy, mx, my, x0, y0) > 0, sv = orient(ox, oy, mx, my, x1, y1) > 0;
+      if (su != sv) par = !par;
+    }
+  }
+  return (__popcll(gballot<G>(par)) & 1) != 0;
+}
+
+__device__ __forceinline__ bool probe_vertex(const DPolys& S, uint32_t r, double shx, double shy, double& x, double& y) {
+  uint64_t v = S.rvo[r];
+  const uint64_t e = S.rvo[r + 1];
+  if (S.synb) {
+    while (v + 1 < e && syn_bit(S.synb, v)) v++;
+    if (v + 1 >= e) return false;
+  }
+  const double2 p = S.xy[v];
+  x = shx + p.x; y = shy + p.y;
+  return true;
+}
+
+struct OverflowCtl { uint32_t count; uint32_t need; uint32_t pad0, pad1; };
+
+// hand pair p to the overflow kernel (lists of up to `need` edges per side); called by one lane
+__device__ __forceinline__ void queue_overflow(uint32_t p, int need, uint32_t* ovf_list, OverflowCtl* ctl) {
+  const uint32_t slot = atomicAdd(&ctl->count, 1u);
+  ovf_list[slot] = p;
+  atomicMax(&ctl->need, (uint32_t)need);
+}
+
+
+// ------------------------------------------------------------------------------------------- work classes
+constexpr int VCAP = 32, RCAP = 8;   // small class: stored vertices and rings per polygon
+
+// Candidate pairs split by work class, on the device: list[0 .. counts[0]) the small pairs (both polygons within
+// VCAP / RCAP), list[counts[0] .. counts[0] + counts[1]) the rest, each in candidate order; ctl cleared for the
+// overflow queue. Uses the context's s_cls, s_tmp2 and s_misc scratch.
+struct PairClasses {
+  uint32_t* list;
+  uint32_t* counts;
+  OverflowCtl* ctl;
+};
+PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np);
+
+// group-wide minimum (order-independent, so deterministic)
+template <int G>
+__device__ __forceinline__ double group_min(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, G));
+  return v;
+}
+
+}  // namespace gsj

@@ -114,11 +114,6 @@ __global__ void k_uf_roots(uint32_t* parent, uint32_t n, uint32_t* root) {
   if (i < n) root[i] = uf_find(parent, i);
 }
 
-struct Timer {
-  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-  double ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
-};
-
 }  // namespace
 
 void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts& o, gsj_points_cb cb, void* user, gsj_join_stats* st) {
@@ -127,51 +122,15 @@ void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_joi
   gsj_join_stats s{};
   if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
   hipStream_t stream = c.stream;
-  const uint32_t batch_q = o.batch_q ? o.batch_q : (1u << 22);
-  const uint64_t max_pairs = o.max_pairs ? o.max_pairs : (128ull << 20);
-  const uint32_t n = (uint32_t)PL.n;
-  std::vector<uint32_t> hcounts;
-  for (uint32_t q0 = 0; q0 < n; q0 += batch_q) {
-    if (o.cancel && *o.cancel) throw Error(GSJ_E_CANCEL, "points join cancelled");
-    const uint32_t nq = std::min<uint32_t>(batch_q, n - q0);
-    Timer tt;
-    uint32_t* counts = c.s_counts.as<uint32_t>(nq + 16);
-    traverse(c, PL.pts.bb, nullptr, q0, nq, RL.bvh, false, counts, nullptr, nullptr, nullptr);
-    uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);
-    const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);
-    s.traverse_ms += tt.ms();
-    std::vector<std::pair<uint32_t, uint32_t>> subs;
-    if (total <= max_pairs) subs.push_back({0, nq});
-    else {
-      hcounts.resize(nq);
-      GSJ_HIP(hipMemcpyAsync(hcounts.data(), counts, (size_t)nq * 4, hipMemcpyDeviceToHost, stream));
-      GSJ_HIP(hipStreamSynchronize(stream));
-      for (uint32_t s0 = 0; s0 < nq;) {
-        uint64_t np = 0; uint32_t s1 = s0;
-        while (s1 < nq && (np + hcounts[s1] <= max_pairs || s1 == s0)) np += hcounts[s1++];
-        subs.push_back({s0, s1});
-        s0 = s1;
-      }
-    }
-    for (auto [s0, s1] : subs) {
-      Timer t1;
-      uint64_t np = total;
-      if (subs.size() > 1) np = scan_u32_to_u64(c, counts + s0, offs, s1 - s0);
-      s.batches++;
-      s.candidates += np;
-      if (np == 0) continue;
-      uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
-      uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
-      traverse(c, PL.pts.bb, nullptr, q0 + s0, s1 - s0, RL.bvh, false, nullptr, offs, pa, pb);
-      GSJ_HIP(hipStreamSynchronize(stream));
-      s.traverse_ms += t1.ms();
-      Timer t2;
+  static const std::vector<uint32_t> no_parent;
+  for_each_candidate_batch(c, PL.pts.bb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+      WallTimer t2;
       uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
       hipLaunchKernelGGL(k_pip, dim3(div_up(np, 256)), dim3(256), 0, stream, PL.pts, RL.poly, pa, pb, np, hit);
       GSJ_LAUNCHED();
       GSJ_HIP(hipStreamSynchronize(stream));
       s.refine_ms += t2.ms();
-      Timer t3;
+      WallTimer t3;
       uint32_t* f = c.s_tmp.as<uint32_t>(np + 16);
       uint64_t* hoffs = c.s_offs.as<uint64_t>(np + 16);
       hipLaunchKernelGGL(k_hit_flags, dim3(div_up(np, 256)), dim3(256), 0, stream, hit, np, f);
@@ -202,8 +161,8 @@ void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_joi
       GSJ_HIP(hipStreamSynchronize(stream));
       s.compact_ms += t3.ms();
       s.hits += nrows;
-      if (!nrows) continue;
-      Timer t4;
+      if (!nrows) return;
+      WallTimer t4;
       uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nrows);
       uint8_t* hw = c.h_where.as<uint8_t>(nrows);
       GSJ_HIP(hipMemcpyAsync(hrow, orow, nrows * 8, hipMemcpyDeviceToHost, stream));
@@ -211,8 +170,7 @@ void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_joi
       GSJ_HIP(hipStreamSynchronize(stream));
       s.copy_ms += t4.ms();
       if (cb && cb(user, hrow, hw, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "points join cancelled by the row callback");
-    }
-  }
+  });
   if (st) *st = s;
 }
 

@@ -20,51 +20,15 @@
 //    list order (A's edges, then B's). The result is the same in every class and bit-identical to the CPU
 //    restatement, so a pair gives the same bits alone or batched with any other request.
 //  * FP64 on the vector ALUs only; nothing here is a dense contraction, so MFMA is not used.
-#include "engine.h"
+#include "pairs.h"
 
 namespace gsj {
 
 namespace {
 
-struct Ed { double x0, y0, x1, y1; int syn, cut; };   // 40 B; cut bit 1/2: start/end vertex is a cut point
-struct Box { double x0, y0, x1, y1; };
-
-__device__ __forceinline__ int syn_bit(const uint32_t* synb, uint64_t i) {
-  return synb ? (int)((synb[i >> 5] >> (i & 31)) & 1u) : 0;
-}
-
-template <int G>
-__device__ __forceinline__ uint64_t gballot(bool p) {
-  const uint64_t m = __ballot(p);
-  if constexpr (G == 64) return m;
-  const int sh = (threadIdx.x & 63) & ~(G - 1);
-  return (m >> sh) & ((1ull << G) - 1);
-}
-template <int G> __device__ __forceinline__ bool gany(bool p) { return gballot<G>(p) != 0; }
-
-// make this wave's scratch writes (LDS or global) visible to its other lanes before they read them; also a code
-// motion barrier. Scratch is never shared between waves.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-__device__ __forceinline__ bool edge_outside(double px, double py, double qx, double qy, const Box& o) {
-  return fmax(px, qx) < o.x0 || fmin(px, qx) > o.x1 || fmax(py, qy) < o.y0 || fmin(py, qy) > o.y1;
-}
-
-// ------------------------------------------------------------------------------------------- edge sources
-// Both sources enumerate "the other polygon's edges meeting the overlap box" in ring order.
-struct ListSrc {   // compacted list (large / overflow classes)
-  const Ed* L;
-  int n;
-  __device__ __forceinline__ int size() const { return n; }
-  __device__ __forceinline__ bool get(int k, Ed& e) const { e = L[k]; return true; }
-};
-
+// ------------------------------------------------------------------------------------------- small-class source
+// The small class reads a polygon's edges from its LDS vertex list (the list classes use ListSrc).
 constexpr uint8_t F_VALID = 1, F_SYN = 2, F_CUT0 = 4, F_CUT1 = 8, F_INOV = 16;
-constexpr int VCAP = 32, RCAP = 8;   // small class: vertex slots and rings per polygon
 
 struct VertSrc {   // all vertices of a small polygon; edge k = (V[k], V[k+1]) exists iff F[k] & F_VALID
   const double2* V;
@@ -81,24 +45,6 @@ struct VertSrc {   // all vertices of a small polygon; edge k = (V[k], V[k+1]) e
     return true;
   }
 };
-
-// inside(x) = inside(c) xor parity of segment c->x across the local edges (engine/engine.cuh:400-412)
-template <class Src>
-__device__ bool inside_from_c(const Src& L, double cx, double cy, bool in_c, double x, double y) {
-  bool in = in_c;
-  const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
-  const int n = L.size();
-  for (int k = 0; k < n; k++) {
-    Ed e;
-    if (!L.get(k, e)) continue;
-    if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) continue;
-    const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
-    if (sc == sx) continue;
-    const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
-    if (su != sv) in = !in;
-  }
-  return in;
-}
 
 struct Contrib { double acc, lin, lon; uint32_t bits; };
 constexpr uint32_t SPILL = 0x80000000u;   // Contrib.bits: the edge exceeded the register path (pair goes to overflow)
@@ -434,15 +380,6 @@ __device__ __forceinline__ int probe_small(const SmallPoly& sp, int r, bool has_
   return real ? 2 : 1;
 }
 
-struct OverflowCtl { uint32_t count; uint32_t need; uint32_t pad0, pad1; };
-
-// hand pair p to the overflow kernel (lists of up to `need` edges per side); called by one lane
-__device__ __forceinline__ void queue_overflow(uint32_t p, int need, uint32_t* ovf_list, OverflowCtl* ctl) {
-  const uint32_t slot = atomicAdd(&ctl->count, 1u);
-  ovf_list[slot] = p;
-  atomicMax(&ctl->need, (uint32_t)need);
-}
-
 template <int G>
 __global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                          const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist_ptr, RefineParams rp,
@@ -537,67 +474,6 @@ __global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const D
 }
 
 // ------------------------------------------------------------------------------------------- large / overflow
-// edges of geometry g meeting box ov, in ring order, compacted by group ballot (engine/engine.cuh:358-377);
-// returns the full count (may exceed cap: then the list is incomplete and the pair overflows)
-template <int G>
-__device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, Ed* out, int cap, int gl) {
-  int cnt = 0;
-  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
-  for (uint32_t r = r0; r < r1; r++) {
-    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
-    for (uint64_t base = s; base + 1 < e; base += G) {
-      const uint64_t i = base + gl;
-      bool v = false;
-      Ed ed;
-      if (i + 1 < e) {
-        const double2 p = S.xy[i], q = S.xy[i + 1];
-        ed.x0 = shx + p.x; ed.y0 = shy + p.y; ed.x1 = shx + q.x; ed.y1 = shy + q.y;
-        ed.syn = syn_bit(S.synb, i);
-        ed.cut = S.synb ? ((syn_bit(S.synb, i > s ? i - 1 : e - 2) ? 1 : 0) | (syn_bit(S.synb, i + 2 < e ? i + 1 : s) ? 2 : 0)) : 0;
-        v = !edge_outside(ed.x0, ed.y0, ed.x1, ed.y1, ov);
-      }
-      const uint64_t m = gballot<G>(v);
-      const int pos = cnt + __popcll(m & ((1ull << gl) - 1));
-      if (v && pos < cap) out[pos] = ed;
-      cnt += __popcll(m);
-    }
-  }
-  return cnt;
-}
-
-// inside(m) over every edge of geometry g, read from global memory (engine/engine.cuh:382-397)
-template <int G>
-__device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, double mx, double my, int gl) {
-  const double ox = shx - 1.0, oy = my;
-  bool par = false;
-  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
-  for (uint32_t r = r0; r < r1; r++) {
-    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
-    for (uint64_t j = s + 1 + gl; j < e; j += G) {
-      const double2 p = S.xy[j - 1], q = S.xy[j];
-      const double x0 = shx + p.x, y0 = shy + p.y, x1 = shx + q.x, y1 = shy + q.y;
-      if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;
-      const bool so = left_of(x0, y0, x1, y1, ox, oy), sc = left_of(x0, y0, x1, y1, mx, my);
-      if (so == sc) continue;
-      const bool su = orient(ox, oy, mx, my, x0, y0) > 0, sv = orient(ox, oy, mx, my, x1, y1) > 0;
-      if (su != sv) par = !par;
-    }
-  }
-  return (__popcll(gballot<G>(par)) & 1) != 0;
-}
-
-__device__ __forceinline__ bool probe_vertex(const DPolys& S, uint32_t r, double shx, double shy, double& x, double& y) {
-  uint64_t v = S.rvo[r];
-  const uint64_t e = S.rvo[r + 1];
-  if (S.synb) {
-    while (v + 1 < e && syn_bit(S.synb, v)) v++;
-    if (v + 1 >= e) return false;
-  }
-  const double2 p = S.xy[v];
-  x = shx + p.x; y = shy + p.y;
-  return true;
-}
-
 // Per-group scratch for compacted lists and per-edge partial sums (LDS in the large kernel, global in overflow).
 struct Scratch {
   Ed* LA; Ed* LB;
@@ -730,28 +606,6 @@ __global__ void __launch_bounds__(256, 2) k_refine_overflow(const DPolys P, cons
   }
 }
 
-// class flag per candidate: 1 = small (both polygons fit the LDS vertex lists)
-__global__ void k_class_flags(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                              uint32_t* __restrict__ fsmall) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
-  fsmall[i] = ((ia & 0xFFFFu) <= VCAP && (ib & 0xFFFFu) <= VCAP && (ia >> 16) <= RCAP && (ib >> 16) <= RCAP) ? 1u : 0u;
-}
-
-// stable partition from the scan of the small flags: small pairs first, large pairs after
-__global__ void k_partition(const uint32_t* __restrict__ fsmall, const uint64_t* __restrict__ off, uint64_t np, const uint64_t* __restrict__ nsmall_p,
-                            uint32_t* __restrict__ list, uint32_t* __restrict__ counts) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t nsmall = *nsmall_p;
-  if (i == 0) { counts[0] = (uint32_t)nsmall; counts[1] = (uint32_t)(np - nsmall); }
-  if (i >= np) return;
-  const uint64_t o = off[i];
-  list[fsmall[i] ? o : nsmall + (i - o)] = (uint32_t)i;
-}
-
-__global__ void k_ctl_reset(OverflowCtl* ctl) { ctl->count = 0; ctl->need = 0; }
-
 }  // namespace
 
 uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
@@ -760,20 +614,11 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "refine: more than 2^32 pairs in one launch");
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
-  uint32_t* fsmall = c.s_cls.as<uint32_t>(2 * np + 64);
-  uint32_t* list = fsmall + np + 32;
-  uint64_t* off = c.s_tmp2.as<uint64_t>(np + 16);
+  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np);
+  uint32_t* list = pc.list;
+  uint32_t* counts = pc.counts;
+  OverflowCtl* ctl = pc.ctl;
   uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
-  char* misc = c.s_misc.as<char>(256);
-  uint64_t* d_nsmall = reinterpret_cast<uint64_t*>(misc);
-  uint32_t* counts = reinterpret_cast<uint32_t*>(misc + 64);
-  OverflowCtl* ctl = reinterpret_cast<OverflowCtl*>(misc + 128);
-  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, ctl);
-  hipLaunchKernelGGL(k_class_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, fsmall);
-  GSJ_LAUNCHED();
-  scan_u32_to_u64_async(c, fsmall, off, np, d_nsmall);
-  hipLaunchKernelGGL(k_partition, dim3(div_up(np, 256)), dim3(256), 0, st, fsmall, off, np, d_nsmall, list, counts);
-  GSJ_LAUNCHED();
   const int T = 256;
   const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
   GSJ_HIP(hipEventRecord(c.ev[0], st));
@@ -824,6 +669,51 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     }
   }
   return nover;
+}
+
+namespace {
+// class flag per candidate: 1 = small (both polygons fit the LDS vertex lists)
+__global__ void k_class_flags(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
+                              uint32_t* __restrict__ fsmall) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
+  fsmall[i] = ((ia & 0xFFFFu) <= VCAP && (ib & 0xFFFFu) <= VCAP && (ia >> 16) <= RCAP && (ib >> 16) <= RCAP) ? 1u : 0u;
+}
+
+// stable partition from the scan of the small flags: small pairs first, large pairs after
+__global__ void k_partition(const uint32_t* __restrict__ fsmall, const uint64_t* __restrict__ off, uint64_t np, const uint64_t* __restrict__ nsmall_p,
+                            uint32_t* __restrict__ list, uint32_t* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nsmall = *nsmall_p;
+  if (i == 0) { counts[0] = (uint32_t)nsmall; counts[1] = (uint32_t)(np - nsmall); }
+  if (i >= np) return;
+  const uint64_t o = off[i];
+  list[fsmall[i] ? o : nsmall + (i - o)] = (uint32_t)i;
+}
+
+__global__ void k_ctl_reset(OverflowCtl* ctl) { ctl->count = 0; ctl->need = 0; }
+}  // namespace
+
+PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+  hipStream_t st = c.stream;
+  uint32_t* fsmall = c.s_cls.as<uint32_t>(2 * np + 64);
+  uint64_t* off = c.s_tmp2.as<uint64_t>(np + 16);
+  char* misc = c.s_misc.as<char>(256);
+  PairClasses pc;
+  pc.list = fsmall + np + 32;
+  pc.counts = reinterpret_cast<uint32_t*>(misc + 64);
+  pc.ctl = reinterpret_cast<OverflowCtl*>(misc + 128);
+  uint64_t* d_nsmall = reinterpret_cast<uint64_t*>(misc);
+  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, pc.ctl);
+  GSJ_LAUNCHED();
+  if (np == 0) return pc;
+  hipLaunchKernelGGL(k_class_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, fsmall);
+  GSJ_LAUNCHED();
+  scan_u32_to_u64_async(c, fsmall, off, np, d_nsmall);
+  hipLaunchKernelGGL(k_partition, dim3(div_up(np, 256)), dim3(256), 0, st, fsmall, off, np, d_nsmall, pc.list, pc.counts);
+  GSJ_LAUNCHED();
+  return pc;
 }
 
 }  // namespace gsj

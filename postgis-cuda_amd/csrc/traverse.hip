@@ -64,4 +64,56 @@ void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbas
   GSJ_LAUNCHED();
 }
 
+void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nQ, const std::vector<uint32_t>& hp, const DBvh& t, bool upper,
+                              const gsj_join_opts& o, gsj_join_stats& s, const PairsFn& fn) {
+  hipStream_t stream = c.stream;
+  const uint32_t batch_q = o.batch_q ? o.batch_q : (1u << 22);
+  const uint64_t max_pairs = o.max_pairs ? o.max_pairs : (128ull << 20);
+  std::vector<uint32_t> hcounts;
+  for (uint32_t q0 = 0, qn; q0 < nQ; q0 = qn) {
+    qn = (uint32_t)std::min<uint64_t>((uint64_t)q0 + batch_q, nQ);
+    if (!hp.empty()) while (qn < nQ && hp[qn] == hp[qn - 1]) qn++;
+    if (o.cancel && *o.cancel) throw Error(GSJ_E_CANCEL, "cancelled");
+    const uint32_t nq = qn - q0;
+    WallTimer tt;
+    uint32_t* counts = c.s_counts.as<uint32_t>(nq + 16);
+    traverse(c, qbb, nullptr, q0, nq, t, upper, counts, nullptr, nullptr, nullptr);
+    uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);   // probe offsets (s_offs belongs to compaction)
+    const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);
+    s.traverse_ms += tt.ms();
+    // sub-batches of at most max_pairs candidates (a probe parent's pieces stay together)
+    std::vector<std::pair<uint32_t, uint32_t>> subs;
+    if (total <= max_pairs) subs.push_back({0, nq});
+    else {
+      hcounts.resize(nq);
+      GSJ_HIP(hipMemcpyAsync(hcounts.data(), counts, (size_t)nq * 4, hipMemcpyDeviceToHost, stream));
+      GSJ_HIP(hipStreamSynchronize(stream));
+      for (uint32_t s0 = 0; s0 < nq;) {
+        uint64_t np = 0;
+        uint32_t s1 = s0;
+        while (s1 < nq && (np + hcounts[s1] <= max_pairs || s1 == s0 || (!hp.empty() && hp[q0 + s1] == hp[q0 + s1 - 1]))) np += hcounts[s1++];
+        GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a probe batch has >= 2^32 candidates (lower max_pairs)");
+        subs.push_back({s0, s1});
+        s0 = s1;
+      }
+    }
+    for (auto [s0, s1] : subs) {
+      if (o.cancel && *o.cancel) throw Error(GSJ_E_CANCEL, "cancelled");
+      WallTimer t1;
+      uint64_t np = total;
+      if (subs.size() > 1) np = scan_u32_to_u64(c, counts + s0, offs, s1 - s0);
+      s.batches++;
+      s.candidates += np;
+      if (np == 0) continue;
+      GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a sub-batch has >= 2^32 candidates (lower max_pairs)");
+      uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
+      uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
+      traverse(c, qbb, nullptr, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb);
+      GSJ_HIP(hipStreamSynchronize(stream));
+      s.traverse_ms += t1.ms();
+      fn(pa, pb, np);
+    }
+  }
+}
+
 }  // namespace gsj

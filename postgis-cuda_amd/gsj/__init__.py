@@ -52,6 +52,7 @@ class LayerInfo(C.Structure):
 
 ROWS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t)
 POINTS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_size_t)
+DIST_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.c_size_t)
 
 _lib = None
 
@@ -59,7 +60,7 @@ _lib = None
 EXPORTS = [
     "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
-    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
+    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
     "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_generate", "gsj_generate_points",
 ]
@@ -86,6 +87,8 @@ def load(path: str | None = None):
         "gsj_join": (i32, [vp, vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_in_polygons": (i32, [vp, vp, vp, C.POINTER(JoinOpts), POINTS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_cluster": (i32, [vp, vp, C.POINTER(JoinOpts), u32p, u64, C.POINTER(JoinStats)]),
+        "gsj_dwithin": (i32, [vp, vp, vp, C.c_double, i32, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_points_dwithin": (i32, [vp, vp, vp, C.c_double, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_de9im_str": (None, [u32, C.c_char_p]), "gsj_de9im_transpose": (u32, [u32]), "gsj_de9im_match": (i32, [u32, C.c_char_p]),
         "gsj_hostset_free": (None, [vp]), "gsj_hostset_sizes": (i32, [vp, u64p, u64p, u64p]),
         "gsj_hostset_arrays": (i32, [vp, C.POINTER(u64p), C.POINTER(u64p), C.POINTER(dp), C.POINTER(u32p), C.POINTER(u8p)]),
@@ -310,6 +313,7 @@ class Rows:
     code: np.ndarray | None
     where: np.ndarray | None
     stats: dict
+    dist: np.ndarray | None = None   # distance joins with report_distance: metres per row
 
 
 def _opts(want_area=False, want_relate=False, upper=False, snap_half_ulp=False, batch_q=0, max_pairs=0, no_agg=False):
@@ -383,6 +387,35 @@ class Context:
         pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
         where = np.concatenate([c[1] for c in chunks]) if chunks else np.zeros(0, np.uint8)
         return Rows(pairs, None, None, where, st.as_dict())
+
+    def _dist_rows(self, fn, *args, **kw):
+        o = _opts(**kw)
+        chunks = []
+
+        def cb(user, pairs, dist, n):
+            p = np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2)
+            chunks.append((p, np.ctypeslib.as_array(dist, (n,)).copy() if dist else None))
+            return 0
+
+        st = JoinStats()
+        _check(fn(self.h, *args, C.byref(o), DIST_CB(cb), None, C.byref(st)))
+        pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
+        dist = np.concatenate([c[1] for c in chunks]) if chunks and chunks[0][1] is not None else None
+        return Rows(pairs, None, None, None, st.as_dict(), dist)
+
+    def dwithin(self, P: Layer, R: Layer, d: float, same=False, report_distance=False, **kw) -> Rows:
+        """ST_DWithin(geography) rows (parent pairs); dist per row in metres with report_distance."""
+        r = self._dist_rows(load().gsj_dwithin, P.h, R.h, float(d), int(same), int(report_distance), **kw)
+        if report_distance and r.dist is None:
+            r.dist = np.zeros(0)
+        return r
+
+    def points_dwithin(self, pts: Layer, R: Layer, d: float, report_distance=False, **kw) -> Rows:
+        """ST_DWithin(geography) of points against polygons: rows (point index, parent polygon)."""
+        r = self._dist_rows(load().gsj_points_dwithin, pts.h, R.h, float(d), int(report_distance), **kw)
+        if report_distance and r.dist is None:
+            r.dist = np.zeros(0)
+        return r
 
     def cluster(self, P: Layer, **kw):
         o = _opts(**kw)

@@ -53,6 +53,26 @@ def test_candidates_equal_brute_force():
         assert got == set(zip(q.tolist(), r.tolist()))
 
 
+def test_point_candidates_equal_brute_force_at_the_extent_border():
+    """Point boxes touching the far border of the build layer's extent (a regression: the grid enumeration dropped
+    them when the query's first cell index rounded to one past the last cell)."""
+    rng = np.random.default_rng(4)
+    for n, seed in ((40, 6), (25, 1), (33, 2), (18, 3)):
+        M = gsj.generate("parcels_mesh", nx=n, ny=n, x0=-95.2, y0=38.6, w=0.6 / 1095, h=0.6 / 1095, seed=seed)
+        L = O.Layer.polys(M)
+        bb = L.export()["bb"]
+        edge = np.nonzero(bb[:, 2] == bb[:, 2].max())[0]
+        xy = np.concatenate([M.xy[::3], np.stack([bb[edge, 2].astype(np.float64), 0.5 * (bb[edge, 1] + bb[edge, 3]).astype(np.float64)], 1),
+                             np.stack([rng.uniform(bb[:, 0].min(), bb[:, 2].max(), 50), np.full(50, bb[:, 3].max())], 1)])
+        P = O.Layer.points_layer(xy)
+        pb = P.export()["bb"]
+        got = {tuple(p) for p in O.candidates(P, L).tolist()}
+        ov = ~((bb[None, :, 0] > pb[:, None, 2]) | (bb[None, :, 2] < pb[:, None, 0]) | (bb[None, :, 1] > pb[:, None, 3]) |
+               (bb[None, :, 3] < pb[:, None, 1]))
+        q, r = np.nonzero(ov)
+        assert got == set(zip(q.tolist(), r.tolist())), (n, seed)
+
+
 # ------------------------------------------------------------------------------------------- cluster
 def test_cluster_equals_bfs_components():
     M = gsj.subdivide(mesh(50, seed=4), 10)
