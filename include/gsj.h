@@ -145,6 +145,58 @@ int gsj_dwithin(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, double d, 
 int gsj_points_dwithin(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, double d, int report_distance, const gsj_join_opts* opts,
                        gsj_dist_cb cb, void* user, gsj_join_stats* stats);
 
+/* ------------------------------------------------------------------ served path: continuous batching
+ * No counterpart function in the reference, which runs one request at a time under a global GPU mutex
+ * (daemon/gpujoind.cu:35, 189-206): the reference's entry points above, run through a scheduler instead.
+ * A scheduler owns `lanes` execution lanes on the context's device (one HIP stream and one scratch pool each) and
+ * one worker thread per lane. Submitted requests queue in arrival order; the head of the queue is admitted as soon
+ * as a lane is free and its estimated scratch fits the budget (a request runs alone when nothing else is admitted),
+ * so requests that arrive while others are on the GPU overlap with them on the device. A request's rows are exactly
+ * those of the same call made directly on the context. Callbacks run on the lane's worker thread; one request's
+ * callbacks never run concurrently. Layers (of the scheduler's context) are pinned until their requests finish. */
+typedef struct gsj_sched gsj_sched;
+
+#define GSJ_REQ_JOIN 1                /* gsj_join: rows_cb */
+#define GSJ_REQ_POINTS_IN_POLYGONS 2  /* gsj_points_in_polygons: points_cb */
+#define GSJ_REQ_CLUSTER 3             /* gsj_cluster: root / nroot */
+#define GSJ_REQ_DWITHIN 4             /* gsj_dwithin: d, same, report_distance, dist_cb */
+#define GSJ_REQ_POINTS_DWITHIN 5      /* gsj_points_dwithin: d, report_distance, dist_cb */
+
+typedef struct {
+  int kind;                       /* GSJ_REQ_* */
+  const gsj_layer* P;             /* probe layer (points for the point requests) */
+  const gsj_layer* R;             /* build layer (needs a BVH); unused by GSJ_REQ_CLUSTER */
+  gsj_join_opts opts;             /* cancel: polled by the lane as in the direct call */
+  double d;                       /* distance requests: metres */
+  int same, report_distance;      /* distance requests */
+  gsj_rows_cb rows_cb;
+  gsj_points_cb points_cb;
+  gsj_dist_cb dist_cb;
+  void* user;
+  uint32_t* root;                 /* GSJ_REQ_CLUSTER output, nroot >= parents of P */
+  uint64_t nroot;
+} gsj_request;
+
+typedef struct {
+  double queue_ms;   /* submit -> admitted to a lane */
+  double run_ms;     /* admitted -> finished (device work, copies and callbacks) */
+  double total_ms;   /* submit -> finished */
+  int lane;
+} gsj_request_times;
+
+/* lanes in [1, 16]; scratch_budget_bytes: admission budget over the estimated scratch of running requests (0 = no
+ * limit besides the lanes) */
+int gsj_sched_create(gsj_ctx* ctx, int lanes, uint64_t scratch_budget_bytes, gsj_sched** out);
+/* waits for every submitted request, then stops the lanes and frees their scratch */
+void gsj_sched_destroy(gsj_sched* s);
+/* queue a request; *ticket identifies it for gsj_wait / gsj_poll */
+int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket);
+/* wait for the request and forget the ticket; returns the request's own status (GSJ_OK or GSJ_E_*, message in
+ * gsj_last_error()); stats and times may be NULL */
+int gsj_wait(gsj_sched* s, uint64_t ticket, gsj_join_stats* stats, gsj_request_times* times);
+/* 1 finished, 0 queued or running, GSJ_E_ARG for an unknown ticket */
+int gsj_poll(gsj_sched* s, uint64_t ticket);
+
 /* ------------------------------------------------------------------ DE-9IM helpers (engine/engine.cuh:871-888) */
 void gsj_de9im_str(uint32_t code, char out[9]);
 uint32_t gsj_de9im_transpose(uint32_t code);

@@ -1,22 +1,19 @@
 // extern "C" boundary of libgsj (declared in include/gsj.h). Every entry point converts exceptions into GSJ_E_*
 // codes plus a thread-local message; nothing here exits the process.
-#include "engine.h"
+#include "capi_internal.h"
 #include "hostprep.h"
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <sstream>
 
-struct gsj_ctx { gsj::Context c; };
-struct gsj_layer { gsj::Layer L; };
 struct gsj_hostset { gsj::HostSet s; };
 
-namespace {
+namespace gsj {
 
 thread_local std::string g_last_error;
 
-template <class F>
-int guarded(F&& f) {
+int guarded_status(const std::function<void()>& f) {
   try {
     f();
     g_last_error.clear();
@@ -39,16 +36,43 @@ int guarded(F&& f) {
   }
 }
 
-void set_device(const gsj::Context& c) { GSJ_HIP(hipSetDevice(c.device)); }
-
-}  // namespace
-
-namespace gsj {
 Context::~Context() {
   for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
+
+void context_init(Context& c, int device) {
+  int ndev = 0;
+  GSJ_HIP(hipGetDeviceCount(&ndev));
+  GSJ_REQUIRE(device >= 0 && device < ndev, GSJ_E_ARG, "no such HIP device: " + std::to_string(device));
+  GSJ_HIP(hipSetDevice(device));
+  c.device = device;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) == hipSuccess) c.cu_count = p.multiProcessorCount;
+  GSJ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (hipEvent_t& e : c.ev) GSJ_HIP(hipEventCreate(&e));
+}
+
+void layer_release(gsj_layer* layer) {
+  if (!layer || layer->refs.fetch_sub(1) != 1) return;
+  if (layer->L.ctx) {
+    std::lock_guard<std::mutex> lk(layer->L.ctx->mu);
+    (void)hipSetDevice(layer->L.ctx->device);
+    (void)hipStreamSynchronize(layer->L.ctx->stream);
+    delete layer;
+    return;
+  }
+  delete layer;
+}
+
 }  // namespace gsj
+
+namespace {
+template <class F>
+int guarded(F&& f) { return gsj::guarded_status(std::function<void()>(std::forward<F>(f))); }
+void set_device(const gsj::Context& c) { GSJ_HIP(hipSetDevice(c.device)); }
+using gsj::g_last_error;
+}  // namespace
 
 extern "C" {
 
@@ -59,21 +83,9 @@ int gsj_ctx_create(int device, gsj_ctx** out) {
   return guarded([&] {
     GSJ_REQUIRE(out, GSJ_E_ARG, "null output pointer");
     *out = nullptr;
-    int ndev = 0;
-    GSJ_HIP(hipGetDeviceCount(&ndev));
-    GSJ_REQUIRE(device >= 0 && device < ndev, GSJ_E_ARG, "no such HIP device: " + std::to_string(device));
-    GSJ_HIP(hipSetDevice(device));
-    auto* ctx = new gsj_ctx();
-    ctx->c.device = device;
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, device) == hipSuccess) ctx->c.cu_count = p.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
-      delete ctx;
-      throw gsj::Error(GSJ_E_HIP, "hipStreamCreate failed");
-    }
-    for (hipEvent_t& e : ctx->c.ev)
-      if (hipEventCreate(&e) != hipSuccess) { delete ctx; throw gsj::Error(GSJ_E_HIP, "hipEventCreate failed"); }
-    *out = ctx;
+    auto ctx = std::make_unique<gsj_ctx>();
+    gsj::context_init(ctx->c, device);
+    *out = ctx.release();
   });
 }
 
@@ -153,17 +165,7 @@ int gsj_layer_create_points(gsj_ctx* ctx, const double* xy, uint64_t n, gsj_laye
   });
 }
 
-void gsj_layer_free(gsj_layer* layer) {
-  if (!layer) return;
-  if (layer->L.ctx) {
-    std::lock_guard<std::mutex> lk(layer->L.ctx->mu);
-    (void)hipSetDevice(layer->L.ctx->device);
-    (void)hipStreamSynchronize(layer->L.ctx->stream);
-    delete layer;
-    return;
-  }
-  delete layer;
-}
+void gsj_layer_free(gsj_layer* layer) { gsj::layer_release(layer); }
 
 int gsj_bvh_build(gsj_layer* layer) {
   return guarded([&] {

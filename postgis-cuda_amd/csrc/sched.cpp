@@ -1,0 +1,214 @@
+// Served path: continuous batching of join requests over resident layers on one device.
+//
+// The reference serves one request at a time under a global GPU mutex (daemon/gpujoind.cu:35, 189-206), so the
+// device idles during every host phase of a request (count copies, scans, row copies, callbacks, socket sends) and a
+// small request cannot fill 256 CUs. Here a scheduler owns N lanes, each a stream with its own grow-only scratch
+// pool and a worker thread. Requests queue in arrival order; the queue head is admitted when a lane is free and its
+// scratch estimate fits the budget, so requests that arrive while others run are admitted as lanes and memory free
+// up, and their kernels overlap on the device (HIP runs kernels of different streams concurrently). Each lane runs
+// the same code as the direct entry points, so a request's rows are the same alone and in the mix.
+#include "capi_internal.h"
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_between(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
+
+struct Req {
+  gsj_request r{};
+  uint64_t ticket = 0;
+  uint64_t est = 0;   // scratch estimate, bytes
+  Clock::time_point t_submit, t_start, t_end;
+  int lane = -1;
+  bool done = false;
+  int rc = GSJ_OK;
+  std::string msg;
+  gsj_join_stats stats{};
+};
+
+// Scratch estimate of a request: candidate pairs bounded by the sub-batch limit, ~120 B of pair state each
+// (pairs, flags, areas, evidence, partition lists, compaction keys), plus per-probe traversal counters.
+uint64_t estimate_scratch(const gsj_request& r) {
+  const uint64_t nq = r.P ? r.P->L.n : 0;
+  const uint64_t max_pairs = r.opts.max_pairs ? r.opts.max_pairs : (128ull << 20);
+  const uint64_t pairs = std::min<uint64_t>(max_pairs, 8 * nq + 1024);
+  return pairs * 120 + nq * 24;
+}
+
+}  // namespace
+
+struct gsj_sched {
+  gsj_ctx* owner = nullptr;
+  std::vector<std::unique_ptr<gsj::Context>> lanes;
+  std::vector<std::thread> workers;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_ticket = 1;
+  uint64_t budget = 0, in_use = 0;
+  int running = 0;
+  bool stop = false;
+
+  void worker(int lane);
+  void run(gsj::Context& c, Req& q);
+};
+
+void gsj_sched::run(gsj::Context& c, Req& q) {
+  const gsj_request& r = q.r;
+  switch (r.kind) {
+    case GSJ_REQ_JOIN:
+      gsj::join_run(c, r.P->L, r.R->L, r.opts, r.rows_cb, r.user, &q.stats, nullptr);
+      break;
+    case GSJ_REQ_POINTS_IN_POLYGONS:
+      gsj::points_join_run(c, r.P->L, r.R->L, r.opts, r.points_cb, r.user, &q.stats);
+      break;
+    case GSJ_REQ_CLUSTER:
+      gsj::cluster_run(c, r.P->L, r.opts, r.root, r.nroot, &q.stats);
+      break;
+    case GSJ_REQ_DWITHIN:
+      gsj::dwithin_run(c, r.P->L, r.R->L, r.d, r.same != 0, r.report_distance != 0, r.opts, r.dist_cb, r.user, &q.stats);
+      break;
+    case GSJ_REQ_POINTS_DWITHIN:
+      gsj::points_dwithin_run(c, r.P->L, r.R->L, r.d, r.report_distance != 0, r.opts, r.dist_cb, r.user, &q.stats);
+      break;
+    default:
+      throw gsj::Error(GSJ_E_ARG, "unknown request kind " + std::to_string(r.kind));
+  }
+}
+
+void gsj_sched::worker(int lane) {
+  gsj::Context& c = *lanes[lane];
+  (void)hipSetDevice(c.device);
+  for (;;) {
+    Req* q = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] {
+        if (queue.empty()) return stop;
+        return running == 0 || budget == 0 || in_use + queue.front()->est <= budget;
+      });
+      if (queue.empty()) return;   // stop requested and nothing left
+      q = queue.front();
+      queue.pop_front();
+      in_use += q->est;
+      running++;
+      q->lane = lane;
+      q->t_start = Clock::now();
+    }
+    const int rc = gsj::guarded_status([&] { run(c, *q); });
+    const std::string msg = rc == GSJ_OK ? std::string() : gsj::g_last_error;
+    gsj::layer_release(const_cast<gsj_layer*>(q->r.P));
+    gsj::layer_release(const_cast<gsj_layer*>(q->r.R));
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      in_use -= q->est;
+      running--;
+      q->rc = rc;
+      q->msg = msg;
+      q->t_end = Clock::now();
+      q->done = true;
+    }
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+}
+
+extern "C" {
+
+int gsj_sched_create(gsj_ctx* ctx, int nlanes, uint64_t scratch_budget_bytes, gsj_sched** out) {
+  return gsj::guarded_status([&] {
+    GSJ_REQUIRE(ctx && out, GSJ_E_ARG, "null context or output pointer");
+    GSJ_REQUIRE(nlanes >= 1 && nlanes <= 16, GSJ_E_ARG, "lanes must be in [1, 16]");
+    *out = nullptr;
+    auto s = std::make_unique<gsj_sched>();
+    s->owner = ctx;
+    s->budget = scratch_budget_bytes;
+    for (int i = 0; i < nlanes; i++) {
+      s->lanes.push_back(std::make_unique<gsj::Context>());
+      gsj::context_init(*s->lanes.back(), ctx->c.device);
+    }
+    for (int i = 0; i < nlanes; i++) s->workers.emplace_back([p = s.get(), i] { p->worker(i); });
+    *out = s.release();
+  });
+}
+
+void gsj_sched_destroy(gsj_sched* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (std::thread& t : s->workers) t.join();
+  for (auto& l : s->lanes) {
+    (void)hipSetDevice(l->device);
+    (void)hipStreamSynchronize(l->stream);
+  }
+  delete s;
+}
+
+int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket) {
+  return gsj::guarded_status([&] {
+    GSJ_REQUIRE(s && req && ticket, GSJ_E_ARG, "null scheduler, request or ticket");
+    const gsj_request& r = *req;
+    GSJ_REQUIRE(r.P, GSJ_E_ARG, "request without a probe layer");
+    GSJ_REQUIRE(r.kind == GSJ_REQ_CLUSTER || r.R, GSJ_E_ARG, "request without a build layer");
+    GSJ_REQUIRE(r.P->L.ctx == &s->owner->c && (!r.R || r.R->L.ctx == &s->owner->c), GSJ_E_ARG,
+                "layer belongs to another context than the scheduler's");
+    auto q = std::make_unique<Req>();
+    q->r = r;
+    q->est = estimate_scratch(r);
+    q->t_submit = Clock::now();
+    const_cast<gsj_layer*>(r.P)->refs.fetch_add(1);
+    if (r.R) const_cast<gsj_layer*>(r.R)->refs.fetch_add(1);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      GSJ_REQUIRE(!s->stop, GSJ_E_ARG, "scheduler is stopping");
+      q->ticket = s->next_ticket++;
+      *ticket = q->ticket;
+      s->queue.push_back(q.get());
+      s->reqs.emplace(q->ticket, std::move(q));
+    }
+    s->cv_work.notify_one();
+  });
+}
+
+int gsj_wait(gsj_sched* s, uint64_t ticket, gsj_join_stats* stats, gsj_request_times* times) {
+  std::unique_ptr<Req> q;
+  int rc = gsj::guarded_status([&] {
+    GSJ_REQUIRE(s, GSJ_E_ARG, "null scheduler");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    GSJ_REQUIRE(it != s->reqs.end(), GSJ_E_ARG, "unknown ticket " + std::to_string(ticket));
+    Req* p = it->second.get();
+    s->cv_done.wait(lk, [&] { return p->done; });
+    q = std::move(it->second);
+    s->reqs.erase(it);
+  });
+  if (rc != GSJ_OK) return rc;
+  if (stats) *stats = q->stats;
+  if (times) {
+    times->queue_ms = ms_between(q->t_submit, q->t_start);
+    times->run_ms = ms_between(q->t_start, q->t_end);
+    times->total_ms = ms_between(q->t_submit, q->t_end);
+    times->lane = q->lane;
+  }
+  gsj::g_last_error = q->msg;
+  return q->rc;
+}
+
+int gsj_poll(gsj_sched* s, uint64_t ticket) {
+  if (!s) return GSJ_E_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return GSJ_E_ARG;
+  return it->second->done ? 1 : 0;
+}
+
+}  // extern "C"

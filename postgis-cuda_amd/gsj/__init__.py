@@ -54,13 +54,27 @@ ROWS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_
 POINTS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_size_t)
 DIST_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.c_size_t)
 
+REQ_JOIN, REQ_POINTS_IN_POLYGONS, REQ_CLUSTER, REQ_DWITHIN, REQ_POINTS_DWITHIN = 1, 2, 3, 4, 5
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("P", C.c_void_p), ("R", C.c_void_p), ("opts", JoinOpts), ("d", C.c_double), ("same", C.c_int),
+                ("report_distance", C.c_int), ("rows_cb", ROWS_CB), ("points_cb", POINTS_CB), ("dist_cb", DIST_CB), ("user", C.c_void_p),
+                ("root", C.POINTER(C.c_uint32)), ("nroot", C.c_uint64)]
+
+
+class RequestTimes(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double), ("lane", C.c_int)]
+
+
 _lib = None
 
 # every symbol include/gsj.h declares (tests check the library exports all of them)
 EXPORTS = [
     "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
-    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
+    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin",
+    "gsj_sched_create", "gsj_sched_destroy", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
     "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_generate", "gsj_generate_points",
 ]
@@ -89,6 +103,9 @@ def load(path: str | None = None):
         "gsj_cluster": (i32, [vp, vp, C.POINTER(JoinOpts), u32p, u64, C.POINTER(JoinStats)]),
         "gsj_dwithin": (i32, [vp, vp, vp, C.c_double, i32, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_dwithin": (i32, [vp, vp, vp, C.c_double, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_sched_create": (i32, [vp, i32, u64, C.POINTER(vp)]), "gsj_sched_destroy": (None, [vp]),
+        "gsj_submit": (i32, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+        "gsj_wait": (i32, [vp, u64, C.POINTER(JoinStats), C.POINTER(RequestTimes)]), "gsj_poll": (i32, [vp, u64]),
         "gsj_de9im_str": (None, [u32, C.c_char_p]), "gsj_de9im_transpose": (u32, [u32]), "gsj_de9im_match": (i32, [u32, C.c_char_p]),
         "gsj_hostset_free": (None, [vp]), "gsj_hostset_sizes": (i32, [vp, u64p, u64p, u64p]),
         "gsj_hostset_arrays": (i32, [vp, C.POINTER(u64p), C.POINTER(u64p), C.POINTER(dp), C.POINTER(u32p), C.POINTER(u8p)]),
@@ -314,6 +331,7 @@ class Rows:
     where: np.ndarray | None
     stats: dict
     dist: np.ndarray | None = None   # distance joins with report_distance: metres per row
+    root: np.ndarray | None = None   # cluster: smallest parent index of each parent's component
 
 
 def _opts(want_area=False, want_relate=False, upper=False, snap_half_ulp=False, batch_q=0, max_pairs=0, no_agg=False):
@@ -437,3 +455,98 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Scheduler:
+    """Served path: requests on resident layers of `ctx`, run by `lanes` lanes with continuous admission.
+
+    submit() returns a ticket; wait(ticket) returns the request's Rows (as the direct call would) with
+    `times` (queue / run / total ms, lane) in its stats. collect=False passes no callback: rows are computed, counted
+    (stats["hits"]) and copied to the host, but not delivered."""
+
+    def __init__(self, ctx: "Context", lanes: int = 4, scratch_budget: int = 0):
+        h = C.c_void_p()
+        _check(load().gsj_sched_create(ctx.h, lanes, scratch_budget, C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self._pending = {}
+
+    def submit(self, kind: int, P: "Layer", R: "Layer" = None, collect=True, d=0.0, same=False, report_distance=False, **kw) -> int:
+        import threading
+
+        req = Request()
+        req.kind, req.P, req.R, req.opts = kind, P.h, (R.h if R is not None else None), _opts(**kw)
+        req.d, req.same, req.report_distance = float(d), int(same), int(report_distance)
+        chunks, lock = [], threading.Lock()
+        keep = [req]
+        root = None
+        if kind == REQ_CLUSTER:
+            n = P.info().nparent
+            root = np.zeros(max(n, 1), np.uint32)
+            req.root, req.nroot = _ptr(root, C.c_uint32), n
+        elif collect and kind == REQ_JOIN:
+            def cb(user, pairs, area, code, n):
+                p = np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2)
+                with lock:
+                    chunks.append((p, np.ctypeslib.as_array(area, (n,)).copy() if area else None,
+                                   np.ctypeslib.as_array(code, (n,)).copy() if code else None))
+                return 0
+            req.rows_cb = ROWS_CB(cb)
+            keep.append(req.rows_cb)
+        elif collect and kind == REQ_POINTS_IN_POLYGONS:
+            def cb(user, pairs, where, n):
+                with lock:
+                    chunks.append((np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2), np.ctypeslib.as_array(where, (n,)).copy()))
+                return 0
+            req.points_cb = POINTS_CB(cb)
+            keep.append(req.points_cb)
+        elif collect:
+            def cb(user, pairs, dist, n):
+                with lock:
+                    chunks.append((np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2),
+                                   np.ctypeslib.as_array(dist, (n,)).copy() if dist else None))
+                return 0
+            req.dist_cb = DIST_CB(cb)
+            keep.append(req.dist_cb)
+        t = C.c_uint64()
+        _check(load().gsj_submit(self.h, C.byref(req), C.byref(t)))
+        self._pending[t.value] = (kind, chunks, keep, root, req.opts)
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        rc = load().gsj_poll(self.h, ticket)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def wait(self, ticket: int) -> Rows:
+        kind, chunks, keep, root, o = self._pending.pop(ticket)
+        st, tm = JoinStats(), RequestTimes()
+        rc = load().gsj_wait(self.h, ticket, C.byref(st), C.byref(tm))
+        _check(rc)
+        stats = st.as_dict()
+        stats["times"] = {"queue_ms": tm.queue_ms, "run_ms": tm.run_ms, "total_ms": tm.total_ms, "lane": tm.lane}
+        pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
+        if kind == REQ_CLUSTER:
+            return Rows(np.zeros((0, 2), np.uint32), None, None, None, stats, None, root[:keep[0].nroot])
+        if kind == REQ_JOIN:
+            area = np.concatenate([c[1] for c in chunks]) if chunks and o.want_area else None
+            code = np.concatenate([c[2] for c in chunks]) if chunks and o.want_relate else None
+            return Rows(pairs, area, code, None, stats)
+        if kind == REQ_POINTS_IN_POLYGONS:
+            where = np.concatenate([c[1] for c in chunks]) if chunks else np.zeros(0, np.uint8)
+            return Rows(pairs, None, None, where, stats)
+        dist = np.concatenate([c[1] for c in chunks]) if chunks and chunks[0][1] is not None else None
+        return Rows(pairs, None, None, None, stats, dist)
+
+    def close(self):
+        if self.h:
+            load().gsj_sched_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -1,0 +1,119 @@
+"""Served path (continuous batching over lanes) through libgsj's C ABI: every request's rows are the rows of the same
+direct call, alone and in a mix that overlaps on the device; layers stay valid while requests hold them; errors
+stay with their request."""
+import numpy as np
+import pytest
+
+import gsj
+from conftest import sorted_rows
+
+pytestmark = pytest.mark.gpu
+
+CELL = 0.6 / 1095
+
+
+def _mesh(n, seed):
+    return gsj.generate("parcels_mesh", nx=n, ny=n, w=1 / 1549, h=1 / 1549, seed=seed)
+
+
+def _same(a, b):
+    ka = sorted_rows(a.pairs, a.area, a.code, a.where, a.dist)
+    kb = sorted_rows(b.pairs, b.area, b.code, b.where, b.dist)
+    for x, y in zip(ka, kb):
+        assert (x is None) == (y is None)
+        if x is not None:
+            assert np.array_equal(x, y)
+
+
+def _workload(ctx):
+    M1 = ctx.polys(_mesh(60, 1))
+    M2 = ctx.polys(gsj.subdivide(_mesh(50, 2), 10))
+    P = ctx.polys(gsj.generate("parcels_grid", nx=40, ny=40, seed=0), build_bvh=False)
+    R = ctx.polys(gsj.generate("regions_grid", nx=40, ny=40, kx=2, ky=2, edge_segs=200, seed=0))
+    D = ctx.polys(gsj.generate("parcels_mesh", nx=40, ny=40, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=2))
+    pts = ctx.points(gsj.generate_points(n=20000, x0=-95.5, y0=38.5, w=60 / 1549, h=60 / 1549, clusters=20, seed=2))
+    reqs = [
+        (gsj.REQ_JOIN, M1, M1, dict(want_relate=True, upper=True)),
+        (gsj.REQ_JOIN, P, R, dict(want_area=True)),
+        (gsj.REQ_JOIN, M2, M2, dict(want_area=True, want_relate=True, upper=True)),
+        (gsj.REQ_DWITHIN, D, D, dict(d=5.0, same=True, report_distance=True, upper=True)),
+        (gsj.REQ_POINTS_IN_POLYGONS, pts, M1, {}),
+        (gsj.REQ_POINTS_DWITHIN, pts, M2, dict(d=3.0, report_distance=True)),
+        (gsj.REQ_JOIN, M2, M1, dict(want_area=True)),
+        (gsj.REQ_CLUSTER, M2, None, {}),
+    ]
+    return reqs, [M1, M2, P, R, D, pts]
+
+
+def _direct(ctx, kind, A, B, kw):
+    kw = dict(kw)
+    if kind == gsj.REQ_JOIN:
+        return ctx.join(A, B, **kw)
+    if kind == gsj.REQ_POINTS_IN_POLYGONS:
+        return ctx.points_in_polygons(A, B, **kw)
+    if kind == gsj.REQ_DWITHIN:
+        return ctx.dwithin(A, B, kw.pop("d"), **kw)
+    if kind == gsj.REQ_POINTS_DWITHIN:
+        return ctx.points_dwithin(A, B, kw.pop("d"), **kw)
+    root, st = ctx.cluster(A)
+    return gsj.Rows(np.zeros((0, 2), np.uint32), None, None, None, st, None, root)
+
+
+@pytest.mark.parametrize("lanes,budget", [(4, 0), (3, 1 << 20)])
+def test_mix_equals_direct_calls(ctx, lanes, budget):
+    reqs, _layers = _workload(ctx)
+    ref = [_direct(ctx, k, A, B, kw) for k, A, B, kw in reqs]
+    s = gsj.Scheduler(ctx, lanes=lanes, scratch_budget=budget)
+    try:
+        for rnd in range(2):   # every request twice, all in flight together
+            tickets = [s.submit(k, A, B, **kw) for k, A, B, kw in reqs] + [s.submit(k, A, B, **kw) for k, A, B, kw in reqs]
+            got = [s.wait(t) for t in tickets]
+            for i, g in enumerate(got):
+                r = ref[i % len(reqs)]
+                if reqs[i % len(reqs)][0] == gsj.REQ_CLUSTER:
+                    assert np.array_equal(g.root, r.root)
+                else:
+                    _same(g, r)
+                assert g.stats["times"]["total_ms"] >= g.stats["times"]["run_ms"] >= 0
+            lanes_used = {g.stats["times"]["lane"] for g in got}
+            # no budget: requests spread over the lanes; a budget below every estimate admits one request at a time
+            assert len(lanes_used) > 1 if budget == 0 else max(g.stats["times"]["queue_ms"] for g in got) > 0
+    finally:
+        s.close()
+
+
+def test_alone_and_in_the_mix_are_identical(ctx):
+    reqs, _layers = _workload(ctx)
+    s = gsj.Scheduler(ctx, lanes=4)
+    try:
+        alone = []
+        for k, A, B, kw in reqs:
+            alone.append(s.wait(s.submit(k, A, B, **kw)))
+        tickets = [s.submit(k, A, B, **kw) for _ in range(3) for k, A, B, kw in reqs]
+        for i, t in enumerate(tickets):
+            g, a = s.wait(t), alone[i % len(reqs)]
+            if reqs[i % len(reqs)][0] == gsj.REQ_CLUSTER:
+                assert np.array_equal(g.root, a.root)
+            else:
+                _same(g, a)
+    finally:
+        s.close()
+
+
+def test_layers_outlive_their_owner_handle_and_errors_stay_with_the_request(ctx):
+    M = ctx.polys(_mesh(40, 5))
+    ref = ctx.join(M, M, want_area=True, upper=True)
+    s = gsj.Scheduler(ctx, lanes=2)
+    try:
+        tickets = [s.submit(gsj.REQ_JOIN, M, M, want_area=True, upper=True) for _ in range(4)]
+        M.free()   # the queued requests still hold the layer
+        for t in tickets:
+            _same(s.wait(t), ref)
+        N = ctx.polys(_mesh(10, 6), build_bvh=False)
+        bad = s.submit(gsj.REQ_JOIN, N, N)   # build layer without a BVH: this request fails, the lanes keep going
+        good = s.submit(gsj.REQ_JOIN, N, ctx.polys(_mesh(10, 6)))
+        with pytest.raises(gsj.GsjError):
+            s.wait(bad)
+        assert len(s.wait(good).pairs) > 0
+    finally:
+        s.close()
