@@ -46,6 +46,68 @@ def small_class(hs):
     return (verts <= 32) & (rings <= 8)
 
 
+def serving_leg(gsj, ctx, args):
+    """Served vs serial on a mixed stream of uneven requests (the batched-requests configuration: sizes lognormal in
+    [1k, 2M] polygons of the config-1 generator on sqrt(s) x sqrt(s) cells, seed 3; 40% intersects + area and 20%
+    relate of parcels x regions, 20% dwithin self-joins with d ~ U(1, 50) m, 20% eroded intersects, which run as
+    intersects + area until that entry point exists). Every request arrives at t = 0; serial = one lane (one request
+    at a time), served = `lanes` lanes with continuous admission. Latency = submit -> finished."""
+    rng = np.random.default_rng(3)
+    n = args.requests
+    mu, sig = np.log(np.sqrt(1e3 * args.smax)), np.log(args.smax / 1e3) / 6
+    sizes = np.clip(np.exp(rng.normal(mu, sig, n)), 1e3, args.smax).astype(np.int64)
+    kinds = rng.choice(4, size=n, p=[0.4, 0.2, 0.2, 0.2])
+    dists = rng.uniform(1.0, 50.0, n)
+    t0 = time.time()
+    reqs, layers = [], []
+    for i in range(n):
+        side = max(1, int(round(np.sqrt(sizes[i]))))
+        P = gsj.generate("parcels_grid", nx=side, ny=side, seed=3000 + i)
+        if kinds[i] == 2:
+            LP = ctx.polys(P)
+            reqs.append((gsj.REQ_DWITHIN, LP, LP, dict(d=float(dists[i]), same=True, upper=True)))
+            layers.append(LP)
+        else:
+            R = gsj.generate("regions_grid", nx=side, ny=side, kx=4, ky=4, edge_segs=500, seed=3000 + i)
+            LP, LR = ctx.polys(P, build_bvh=False), ctx.polys(R)
+            reqs.append((gsj.REQ_JOIN, LP, LR, dict(want_area=kinds[i] != 1, want_relate=kinds[i] == 1)))
+            layers += [LP, LR]
+        del P
+    log(f"serving mix: {n} requests, {int(sizes.sum())} polygons, generated + resident in {time.time() - t0:.1f}s")
+
+    def run(lanes):
+        s = gsj.Scheduler(ctx, lanes=lanes)
+        try:
+            for k, A, B, kw in reqs:   # warm-up: every lane's scratch grows to its working size
+                s.submit(k, A, B, collect=False, **kw)
+            for t in list(s._pending):
+                s.wait(t)
+            tw = time.perf_counter()
+            tickets = [s.submit(k, A, B, collect=False, **kw) for k, A, B, kw in reqs]
+            res = [s.wait(t) for t in tickets]
+            wall = time.perf_counter() - tw
+        finally:
+            s.close()
+        cand = sum(r.stats["candidates"] for r in res)
+        rows = sum(r.stats["hits"] for r in res)
+        lat = np.array([r.stats["times"]["total_ms"] for r in res])
+        return {"lanes": lanes, "wall_s": round(wall, 4), "refined_pairs_per_s": round(cand / wall, 1), "rows_per_s": round(rows / wall, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "candidates": int(cand), "rows": int(rows)}, [(r.stats["candidates"], r.stats["hits"]) for r in res]
+
+    serial, cs = run(1)
+    served, cv = run(args.lanes)
+    assert cs == cv, "served and serial request results differ"
+    out = {"workload": f"batched requests: {n} requests, sizes lognormal in [1e3, {args.smax:.0e}] polygons (config-1 generator), seed 3",
+           "mix": "40% intersects+area, 20% relate, 20% dwithin d~U(1,50) m, 20% eroded intersects (run as intersects+area)",
+           "arrivals": "all at t=0", "serial": serial, "served": served,
+           "speedup_pairs_per_s": round(served["refined_pairs_per_s"] / serial["refined_pairs_per_s"], 3),
+           "p99_ratio_served_over_serial": round(served["p99_ms"] / serial["p99_ms"], 3)}
+    for L in layers:
+        L.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +116,10 @@ def main():
     ap.add_argument("--scale", type=int, default=1549, help="lattice cells per side (1549 = the 2.4M-parcel config)")
     ap.add_argument("--sample", type=int, default=400, help="cells per side of the CPU/parity sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline and parity legs (profiling runs)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-vs-serial leg")
+    ap.add_argument("--requests", type=int, default=256, help="requests in the served-vs-serial mix")
+    ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
+    ap.add_argument("--lanes", type=int, default=4, help="lanes of the served path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is described, not built)",
@@ -137,6 +203,9 @@ def main():
                   "candidates_equal": got.stats["candidates"] == exp["candidates"]}
         log(f"cpu baseline {cpu['value']:.0f} pairs/s on {threads} threads; parity {parity}; roofline {roofline}")
 
+    serving = None if args.no_serve else serving_leg(gsj, ctx, args)
+    if serving:
+        log(f"serving: serial {serving['serial']}; served {serving['served']}; speedup {serving['speedup_pairs_per_s']}")
     lat = np.array(times) * 1e3
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "refined pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -148,7 +217,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
         "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_small_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1)},
-        "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity,
+        "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

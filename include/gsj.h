@@ -145,6 +145,20 @@ int gsj_dwithin(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, double d, 
 int gsj_points_dwithin(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, double d, int report_distance, const gsj_join_opts* opts,
                        gsj_dist_cb cb, void* user, gsj_join_stats* stats);
 
+/* ST_Intersects(ST_Buffer(a::geography, -ra)::geometry, ST_Buffer(b::geography, -rb)::geometry) of polygon layers
+ * without building buffers (replaces buffer_join_stream in ERODE mode, engine/buffer.cuh:779-889; pair test er_pair,
+ * :502-588). ra, rb >= 0 metres (0 = that side not buffered). Pass 1 is the intersects + area join; parent pairs whose
+ * overlap can hold a disk of radius min(ra, rb) are tested on their eroded boundaries. `same` / opts->upper as for
+ * gsj_dwithin; need_self (host, one flag per parent of P, may be NULL): with `same`, a parent's pair with itself is
+ * tested only where its flag is set (daemon/gpujoind.cu:438-443). Rows: parent pairs (dist NULL). */
+int gsj_intersects_buffered(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, double ra, double rb, int same, const uint8_t* need_self,
+                            const gsj_join_opts* opts, gsj_dist_cb cb, void* user, gsj_join_stats* stats);
+/* Points inside ST_Buffer(polygon::geography, -r) (replaces points_buffer_join_stream in ERODE mode,
+ * engine/buffer.cuh:1137-1229; per pair k_pt_erode, :1080-1128): rows (point index, parent polygon) where the point is
+ * inside some piece and within r of no real boundary. */
+int gsj_points_intersects_buffered(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, double r, const gsj_join_opts* opts, gsj_dist_cb cb,
+                                   void* user, gsj_join_stats* stats);
+
 /* ------------------------------------------------------------------ served path: continuous batching
  * No counterpart function in the reference, which runs one request at a time under a global GPU mutex
  * (daemon/gpujoind.cu:35, 189-206): the reference's entry points above, run through a scheduler instead.
@@ -161,6 +175,8 @@ typedef struct gsj_sched gsj_sched;
 #define GSJ_REQ_CLUSTER 3             /* gsj_cluster: root / nroot */
 #define GSJ_REQ_DWITHIN 4             /* gsj_dwithin: d, same, report_distance, dist_cb */
 #define GSJ_REQ_POINTS_DWITHIN 5      /* gsj_points_dwithin: d, report_distance, dist_cb */
+#define GSJ_REQ_INTERSECTS_BUFFERED 6 /* gsj_intersects_buffered: ra, rb, same, need_self, dist_cb */
+#define GSJ_REQ_POINTS_INTERSECTS_BUFFERED 7  /* gsj_points_intersects_buffered: rb (the polygon radius), dist_cb */
 
 typedef struct {
   int kind;                       /* GSJ_REQ_* */
@@ -169,6 +185,8 @@ typedef struct {
   gsj_join_opts opts;             /* cancel: polled by the lane as in the direct call */
   double d;                       /* distance requests: metres */
   int same, report_distance;      /* distance requests */
+  double ra, rb;                  /* buffered requests: erosion radii, metres */
+  const uint8_t* need_self;       /* buffered requests: see gsj_intersects_buffered (host, must outlive the request) */
   gsj_rows_cb rows_cb;
   gsj_points_cb points_cb;
   gsj_dist_cb dist_cb;

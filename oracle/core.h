@@ -11,6 +11,7 @@
 #include <vector>
 #include <string>
 #include <unordered_map>
+#include <functional>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -312,6 +313,17 @@ static bool inside_from_c(const std::vector<Edge>& L, Real cx, Real cy, bool in_
   }
   return in;
 }
-
+// ------------------------------------------------------------------------------------------- join (oracle.cpp)
+struct JoinResult {
+  std::vector<uint32_t> pairs;   // (a, b) parent ids
+  std::vector<double> area;
+  std::vector<uint32_t> code;
+  std::vector<uint8_t> where;    // points: 1 interior, 2 boundary
+  std::vector<uint32_t> root;    // cluster: root per parent
+  uint64_t candidates = 0;
+};
+struct PairOut;
+void join(const Layer& P, const Layer& R, const Params& rp, bool upper, bool no_agg, JoinResult& res, std::vector<uint32_t>* cand_out,
+          std::vector<PairOut>* refined_out, const std::function<bool(uint32_t, uint32_t)>& prefilter = {});
 
 }  // namespace orc

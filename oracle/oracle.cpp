@@ -209,23 +209,21 @@ static inline uint32_t de9im_code(const double* e, uint32_t bits, double LA, dou
 }
 
 // ------------------------------------------------------------------------------------------- join
-struct JoinResult {
-  std::vector<uint32_t> pairs;   // (a, b) parent ids
-  std::vector<double> area;
-  std::vector<uint32_t> code;
-  std::vector<uint8_t> where;    // points: 1 interior, 2 boundary
-  std::vector<uint32_t> root;    // cluster: root per parent
-  uint64_t candidates = 0;
-};
-
 // engine/engine.cuh:895-1061 (join_stream) without batching: every piece pair of one parent pair is aggregated in
 // one place, which the reference guarantees by keeping a probe parent's pieces in one sub-batch (lines 909-913, 925)
-static void join(const Layer& P, const Layer& R, const Params& rp, bool upper, bool no_agg, JoinResult& res, std::vector<uint32_t>* cand_out,
-                 std::vector<PairOut>* refined_out) {
+// (prefilter: candidate piece pairs it rejects are dropped before refinement, JoinOpts::prefilter, engine.cuh:935)
+void join(const Layer& P, const Layer& R, const Params& rp, bool upper, bool no_agg, JoinResult& res, std::vector<uint32_t>* cand_out,
+          std::vector<PairOut>* refined_out, const std::function<bool(uint32_t, uint32_t)>& prefilter) {
   std::vector<uint32_t> cand;
   candidates(P.bb, P.n, R.bb, R.n, upper, cand);
+  res.candidates = cand.size() / 2;
+  if (prefilter) {
+    size_t k = 0;
+    for (size_t i = 0; i < cand.size(); i += 2)
+      if (prefilter(cand[i], cand[i + 1])) { cand[k++] = cand[i]; cand[k++] = cand[i + 1]; }
+    cand.resize(k);
+  }
   const uint64_t np = cand.size() / 2;
-  res.candidates = np;
   std::vector<PairOut> ref(np);
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t i = 0; i < (int64_t)np; i++) ref[i] = refine_pair(P, R, cand[2 * i], cand[2 * i + 1], rp);

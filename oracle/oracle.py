@@ -53,6 +53,8 @@ def load(flops=False):
         "orc_dw_new": (vp, []), "orc_dw_free": (None, [vp]), "orc_dw_rows": (u64, [vp]), "orc_dw_candidates": (u64, [vp]),
         "orc_dw_copy": (None, [vp, u32p, dp]), "orc_dwithin": (None, [vp, vp, d, i32, i32, i32, vp]),
         "orc_points_dwithin": (None, [vp, vp, d, i32, vp]),
+        "orc_er_pair": (i32, [vp, vp, u32, u32, d, d]), "orc_erode": (None, [vp, vp, d, d, i32, i32, u8p, vp]),
+        "orc_points_erode": (None, [vp, vp, d, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -350,3 +352,30 @@ def points_dwithin(pts: Layer, R: Layer, d: float, report=False):
     r = lib.orc_dw_new()
     lib.orc_points_dwithin(pts.h, R.h, d, int(report), r)
     return _dw_result(r)
+
+
+def er_pair(P: Layer, R: Layer, a: int, b: int, ra: float, rb: float) -> bool:
+    """Eroded intersects of parent a of P and parent b of R (all their pieces)."""
+    return bool(load().orc_er_pair(P.h, R.h, a, b, ra, rb))
+
+
+def erode(P: Layer, R: Layer, ra: float, rb: float, same=False, upper=False, need_self=None):
+    """ST_Intersects(ST_Buffer(A, -ra), ST_Buffer(B, -rb)) rows (parent pairs)."""
+    lib = load()
+    r = lib.orc_dw_new()
+    ns = None if need_self is None else np.ascontiguousarray(need_self, np.uint8)
+    lib.orc_erode(P.h, R.h, ra, rb, int(same), int(upper), _p(ns, C.c_uint8), r)
+    out = _dw_result(r)
+    del out["dist"]
+    return out
+
+
+def points_erode(pts: Layer, R: Layer, r: float):
+    """Points inside ST_Buffer(polygon, -r): rows (point index, parent polygon)."""
+    lib = load()
+    h = lib.orc_dw_new()
+    lib.orc_points_erode(pts.h, R.h, r, h)
+    out = _dw_result(h)
+    del out["dist"]
+    return out
+

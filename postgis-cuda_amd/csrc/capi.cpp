@@ -123,7 +123,8 @@ int gsj_ctx_trim(gsj_ctx* ctx) {
     gsj::Context& c = ctx->c;
     for (gsj::DevBuf* b : {&c.s_counts, &c.s_qoffs, &c.s_offs, &c.s_pairs_a, &c.s_pairs_b, &c.s_flag, &c.s_area, &c.s_ev, &c.s_evb, &c.s_tmp,
                            &c.s_tmp2, &c.s_cls, &c.s_out_a, &c.s_out_b, &c.s_out_area, &c.s_out_code, &c.s_out_flag, &c.s_keys, &c.s_keys2,
-                           &c.s_vals, &c.s_vals2, &c.s_seg, &c.s_heavy, &c.s_heavy_edges, &c.s_scal, &c.s_qbb, &c.s_uf, &c.s_root, &c.s_misc})
+                           &c.s_vals, &c.s_vals2, &c.s_seg, &c.s_heavy, &c.s_heavy_edges, &c.s_scal, &c.s_qbb, &c.s_uf, &c.s_root, &c.s_misc,
+                           &c.s_er_keep, &c.s_er_off, &c.s_er_pairs, &c.s_er_flag, &c.s_er_list, &c.s_er_slab, &c.s_er_out, &c.s_er_need, &c.s_er_pf})
       b->release();
   });
 }
@@ -252,6 +253,28 @@ int gsj_points_dwithin(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, d
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     set_device(ctx->c);
     gsj::points_dwithin_run(ctx->c, pts->L, R->L, d, report_distance != 0, defaults(opts), cb, user, stats);
+  });
+}
+
+int gsj_intersects_buffered(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, double ra, double rb, int same, const uint8_t* need_self,
+                            const gsj_join_opts* opts, gsj_dist_cb cb, void* user, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && P && R, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(P->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::erode_run(ctx->c, P->L, R->L, ra, rb, same != 0, need_self, defaults(opts), cb, user, stats);
+  });
+}
+
+int gsj_points_intersects_buffered(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* R, double r, const gsj_join_opts* opts, gsj_dist_cb cb,
+                                   void* user, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && pts && R, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(pts->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::points_erode_run(ctx->c, pts->L, R->L, r, defaults(opts), cb, user, stats);
   });
 }
 

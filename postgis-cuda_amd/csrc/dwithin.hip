@@ -1,5 +1,5 @@
 // Distance joins on lon/lat layers: ST_DWithin(A::geography, B::geography, d) for polygon x polygon and
-// point x polygon (reference: engine/buffer.cuh:185-273 dw_pair / k_dwithin / k_dwithin_heavy, 703-715 k_expand_bb,
+// point x polygon, and points in eroded polygons (reference: engine/buffer.cuh:185-273 dw_pair / k_dwithin / k_dwithin_heavy, 703-715 k_expand_bb,
 // 892-991 dwithin_join_stream, 1004-1049 k_expand_pts / k_pt_dwithin, 1137-1229 points_buffer_join_stream).
 //
 // Candidates: probe boxes widened by d metres (degrees from the WGS84 radii at the box's worst latitude), rounded
@@ -282,6 +282,145 @@ __global__ void __launch_bounds__(256) k_pt_dwithin(const DPoints pts, const DPo
   dist[i] = best;
 }
 
+// ---- points in eroded polygons (buffer.cuh:1053-1131) ---------------------------------------------------------------
+constexpr double FILLET_QUANTUM = gm::PI_HI / 16.0;   // GEOS: 8 quadrant segments
+
+// w (metres from vertex v) strictly inside v's region between unit directions e -> f: convex, the disk outside the
+// interior wedge; reflex, the GEOS fillet polygon (buffer.cuh:1056-1078)
+__device__ bool vertex_excl(double ex, double ey, double fx, double fy, double r, double wx, double wy, double eps) {
+  const double d2 = wx * wx + wy * wy;
+  if (d2 >= r * r) return false;
+  const double cr = ex * fy - ey * fx, dt = ex * fx + ey * fy;
+  if (cr > 0) {
+    const double ri = r - eps, s1 = wy * ex - wx * ey, s2 = wy * fx - wx * fy;
+    return d2 < ri * ri && !(s1 > eps && s2 > eps);
+  }
+  if (cr == 0 && dt >= 0) return false;   // straight
+  const double th = cr == 0 ? gm::PI_HI : gm::atan2(-cr, dt);
+  int m = (int)(th / FILLET_QUANTUM + 0.5);
+  if (m < 1) m = 1;
+  const double inc = th / m;
+  double si, ci;
+  gm::sincos(inc, si, ci);
+  const double nkx = -ey, nky = ex, nsx = -fy, nsy = fx;
+  if (nkx * wy - nky * wx > eps || nsx * wy - nsy * wx < -eps) return false;
+  const double chh = sqrt(0.5 * (1.0 + ci)), shh = sqrt(fmax(0.0, 0.5 * (1.0 - ci)));
+  const double ch = r * chh - eps;
+  if (d2 < ch * ch) return true;
+  double mx = nkx * chh + nky * shh, my = -nkx * shh + nky * chh;
+  for (int i = 0; i < m; i++) {
+    if (mx * wx + my * wy >= ch) return false;
+    const double t2 = mx * ci + my * si;
+    my = -mx * si + my * ci;
+    mx = t2;
+  }
+  return true;
+}
+
+// thread per (point, piece) widened by r: 1 = inside the piece, 2 = within r of its real boundary (excluded),
+// 0 = outside (buffer.cuh:1080-1128)
+__global__ void __launch_bounds__(256) k_pt_erode(const DPoints pts, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                  uint64_t np, double r_nom, uint8_t* __restrict__ bits) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint32_t g = pb[i];
+  const double2 pt = pts.xy[pa[i]], o = R.origin[g], ext = R.ext[g];
+  const double r = r_nom / geo::utm_k(o.x + 0.5 * ext.x, o.y + 0.5 * ext.y, pt.x, pt.y);
+  double kx, ky;
+  geo::m_per_deg(pt.y, kx, ky);
+  const double eps = 1e-9;
+  const double shx = o.x - pt.x, shy = o.y - pt.y, ox = shx * kx - 1.0;   // frame: metres relative to the point
+  auto vx = [&](uint64_t j, double& x, double& y) { const double2 q = R.xy[j]; x = (shx + q.x) * kx; y = (shy + q.y) * ky; };
+  bool par = false, ex = false;
+  for (uint32_t rr = R.gro[g]; rr < R.gro[g + 1] && !ex; rr++) {
+    const uint64_t s0 = R.rvo[rr], e = R.rvo[rr + 1];
+    if (e - s0 < 2) continue;
+    // previous real edge direction (region of the vertex that starts the current edge); the ring closes
+    bool pv = false;
+    double pux = 0, puy = 0;
+    {
+      const uint64_t j = e - 2;
+      if (!syn_bit(R.synb, j)) {
+        double a, b, c, dd;
+        vx(j, a, b);
+        vx(j + 1, c, dd);
+        const double L = sqrt((c - a) * (c - a) + (dd - b) * (dd - b));
+        if (L > 0) { pv = true; pux = (c - a) / L; puy = (dd - b) / L; }
+      }
+    }
+    double x0, y0;
+    vx(s0, x0, y0);
+    for (uint64_t j = s0; j + 1 < e && !ex; j++) {
+      double x1, y1;
+      vx(j + 1, x1, y1);
+      if (!((y0 > 0) == (y1 > 0) && y0 != 0 && y1 != 0)) {
+        const bool so = left_of(x0, y0, x1, y1, ox, 0.0), sp = left_of(x0, y0, x1, y1, 0.0, 0.0);
+        if (so != sp) {
+          const bool su = orient(ox, 0.0, 0.0, 0.0, x0, y0) > 0, sv = orient(ox, 0.0, 0.0, 0.0, x1, y1) > 0;
+          if (su != sv) par = !par;
+        }
+      }
+      const double L = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+      const bool sj = syn_bit(R.synb, j) != 0;
+      if (sj || L == 0) {
+        if (sj) pv = false;
+        x0 = x1; y0 = y1;
+        continue;
+      }
+      const double ux = (x1 - x0) / L, uy = (y1 - y0) / L;
+      if (r > 0) {
+        const double t = -x0 * ux - y0 * uy, s = -y0 * ux + x0 * uy;   // the point (0, 0) relative to the edge
+        if (fabs(s) < r - eps && t > -eps && t < L + eps) ex = true;
+        else if (pv && vertex_excl(pux, puy, ux, uy, r, -x0, -y0, eps)) ex = true;
+      }
+      pv = true; pux = ux; puy = uy;
+      x0 = x1; y0 = y1;
+    }
+  }
+  bits[i] = ex ? 2 : (par ? 1 : 0);   // the scan stops early only once the point is known to be excluded
+}
+
+__global__ void k_nonzero_u32(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ f) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = b[i] ? 1u : 0u;
+}
+__global__ void k_one_u32(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ f) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = b[i] == 1 ? 1u : 0u;
+}
+// (point, parent) keys of the pairs with nonzero bits
+__global__ void k_pe_keys(const uint32_t* __restrict__ Rpar, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                          const uint8_t* __restrict__ bits, const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint64_t n,
+                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !f[i]) return;
+  keys[off[i]] = ((uint64_t)pa[i] << 32) | (Rpar ? Rpar[pb[i]] : pb[i]);
+  vals[off[i]] = bits[i];
+}
+// OR of the bits per key; a row where the OR is 1 (inside some piece, near no real boundary)
+__global__ void k_pe_fold(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint64_t m, uint32_t* __restrict__ keep) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  if (j > 0 && keys[j] == keys[j - 1]) { keep[j] = 0; return; }
+  uint32_t b = 0;
+  for (uint64_t k = j; k < m && keys[k] == keys[j]; k++) b |= vals[k];
+  keep[j] = b == 1 ? 1u : 0u;
+}
+__global__ void k_pe_emit(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ keep, const uint64_t* __restrict__ off, uint64_t m,
+                          uint32_t* __restrict__ orow) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m || !keep[j]) return;
+  orow[2 * off[j]] = (uint32_t)(keys[j] >> 32);
+  orow[2 * off[j] + 1] = (uint32_t)keys[j];
+}
+__global__ void k_pe_direct(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, const uint32_t* __restrict__ f,
+                            const uint64_t* __restrict__ off, uint64_t n, uint32_t* __restrict__ orow) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !f[i]) return;
+  orow[2 * off[i]] = pa[i];
+  orow[2 * off[i] + 1] = pb[i];
+}
+
 // ---- compaction: unique (parent a, parent b) rows of the hit pairs, minimum distance -----------------------------
 __global__ void k_hit_u32(const uint8_t* __restrict__ hit, uint64_t n, uint32_t* __restrict__ f) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -371,7 +510,7 @@ void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool sa
     GSJ_LAUNCHED();
     qbb = e;
   }
-  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, PL.h_parent, RL.bvh, o.upper != 0, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, PL.h_parent, RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
     DwArgs A{PL.poly, RL.poly, pa, pb, d, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr};
     const PairClasses pc = partition_pairs(c, PL.poly, RL.poly, pa, pb, np);
@@ -445,7 +584,7 @@ void points_dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, 
     qbb = e;
   }
   static const std::vector<uint32_t> no_parent;
-  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
     uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
     double* dist = report ? c.s_area.as<double>(np + 16) : nullptr;
@@ -468,6 +607,77 @@ void points_dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, 
     s.compact_ms += t3.ms();
     s.hits += nrows;
     dw_deliver(c, orow, odist, nrows, s, cb, user);
+  });
+  if (st) *st = s;
+}
+
+void points_erode_run(Context& c, const Layer& PL, const Layer& RL, double r, const gsj_join_opts& o, gsj_dist_cb cb, void* user, gsj_join_stats* st) {
+  GSJ_REQUIRE(PL.points && !RL.points, GSJ_E_ARG, "points_intersects_buffered: needs a point layer and a polygon layer");
+  GSJ_REQUIRE(RL.has_bvh, GSJ_E_ARG, "points_intersects_buffered: polygon layer has no BVH");
+  GSJ_REQUIRE(r >= 0 && r < 1e5, GSJ_E_ARG, "points_intersects_buffered: radius must be in [0, 1e5) metres");
+  gsj_join_stats s{};
+  if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
+  hipStream_t stream = c.stream;
+  // every piece whose boundary may be within r of the point (buffer.cuh:1145-1152)
+  const double grow = r * 1.002;
+  const float4* qbb = PL.pts.bb;
+  if (grow > 0) {
+    float4* e = c.s_qbb.as<float4>(PL.n + 16);
+    hipLaunchKernelGGL(k_expand_points, dim3(div_up(PL.n, 256)), dim3(256), 0, stream, PL.pts, grow, e);
+    GSJ_LAUNCHED();
+    qbb = e;
+  }
+  static const std::vector<uint32_t> no_parent;
+  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+    WallTimer t2;
+    uint8_t* bits = c.s_flag.as<uint8_t>(np + 16);
+    GSJ_HIP(hipEventRecord(c.ev[0], stream));
+    hipLaunchKernelGGL(k_pt_erode, dim3(div_up(np, 256)), dim3(256), 0, stream, PL.pts, RL.poly, pa, pb, np, r, bits);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[1], stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+    s.k_small_ms += ms;
+    s.n_small += np;
+    s.launches++;
+    s.refine_ms += t2.ms();
+    WallTimer t3;
+    uint32_t* f = c.s_tmp.as<uint32_t>(np + 16);
+    uint64_t* off = c.s_offs.as<uint64_t>(np + 16);
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
+    uint64_t nrows = 0;
+    if (!RL.poly.parent) {
+      hipLaunchKernelGGL(k_one_u32, dim3(div_up(np, 256)), dim3(256), 0, stream, bits, np, f);
+      GSJ_LAUNCHED();
+      nrows = scan_u32_to_u64(c, f, off, np);
+      hipLaunchKernelGGL(k_pe_direct, dim3(div_up(np, 256)), dim3(256), 0, stream, pa, pb, f, off, np, orow);
+      GSJ_LAUNCHED();
+    } else {
+      hipLaunchKernelGGL(k_nonzero_u32, dim3(div_up(np, 256)), dim3(256), 0, stream, bits, np, f);
+      GSJ_LAUNCHED();
+      const uint64_t nz = scan_u32_to_u64(c, f, off, np);
+      if (nz) {
+        uint64_t* keys = c.s_keys.as<uint64_t>(nz + 16);
+        uint64_t* keys2 = c.s_keys2.as<uint64_t>(nz + 16);
+        uint32_t* vals = c.s_vals.as<uint32_t>(nz + 16);
+        uint32_t* vals2 = c.s_vals2.as<uint32_t>(nz + 16);
+        hipLaunchKernelGGL(k_pe_keys, dim3(div_up(np, 256)), dim3(256), 0, stream, RL.poly.parent, pa, pb, bits, f, off, np, keys, vals);
+        GSJ_LAUNCHED();
+        radix_sort_pairs(c, keys, keys2, vals, vals2, nz, 0, 64);
+        uint32_t* keep = c.s_seg.as<uint32_t>(nz + 16);
+        uint64_t* koff = c.s_tmp2.as<uint64_t>(nz + 16);
+        hipLaunchKernelGGL(k_pe_fold, dim3(div_up(nz, 256)), dim3(256), 0, stream, keys, vals, nz, keep);
+        GSJ_LAUNCHED();
+        nrows = scan_u32_to_u64(c, keep, koff, nz);
+        hipLaunchKernelGGL(k_pe_emit, dim3(div_up(nz, 256)), dim3(256), 0, stream, keys, keep, koff, nz, orow);
+        GSJ_LAUNCHED();
+      }
+    }
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.compact_ms += t3.ms();
+    s.hits += nrows;
+    dw_deliver(c, orow, nullptr, nrows, s, cb, user);
   });
   if (st) *st = s;
 }

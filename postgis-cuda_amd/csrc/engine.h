@@ -79,6 +79,8 @@ struct Context {
   // scratch, grow-only, reused across calls
   DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
       s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc;
+  // erosion pass 2 (it runs inside the join's on-device hook, beside the join's own scratch)
+  DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;
   int cu_count = 256;
   hipEvent_t ev[8] = {};   // kernel timing events (created with the context)
@@ -112,7 +114,7 @@ void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbas
 // most o.batch_q probes that keep a parent's pieces together (h_parent, may be empty) and sub-batches of at most
 // o.max_pairs candidates (engine/engine.cuh:903-937). fn(pa, pb, np) runs on every non-empty sub-batch with the pairs
 // on the device and the stream synchronised. Adds traverse_ms, batches and candidates to s; polls o.cancel.
-using PairsFn = std::function<void(const uint32_t* pa, const uint32_t* pb, uint64_t np)>;
+using PairsFn = std::function<void(uint32_t* pa, uint32_t* pb, uint64_t np)>;
 void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nq, const std::vector<uint32_t>& h_parent, const DBvh& t, bool upper,
                               const gsj_join_opts& o, gsj_join_stats& s, const PairsFn& fn);
 
@@ -137,6 +139,9 @@ struct JoinHooks {
   // consumer of each finished sub-batch while on the device (rows: 2n interleaved parent ids, area or nullptr, n)
   std::function<void(const uint32_t*, const double*, uint64_t)> on_device;
   bool device_only = false;
+  // candidate filter before refinement (JoinOpts::prefilter, engine/engine.cuh:812, 935): compacts the sub-batch's
+  // pairs in place on the device and returns how many are kept
+  std::function<uint64_t(uint32_t* pa, uint32_t* pb, uint64_t np)> prefilter;
 };
 void join_run(Context& c, const Layer& P, const Layer& R, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st,
               const JoinHooks* hooks);
@@ -149,5 +154,9 @@ void dwithin_run(Context& c, const Layer& P, const Layer& R, double d, bool same
                  gsj_join_stats* st);
 void points_dwithin_run(Context& c, const Layer& pts, const Layer& R, double d, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
                         gsj_join_stats* st);
+// eroded intersects (erode.hip, dwithin.hip)
+void erode_run(Context& c, const Layer& P, const Layer& R, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
+               gsj_dist_cb cb, void* user, gsj_join_stats* st);
+void points_erode_run(Context& c, const Layer& pts, const Layer& R, double r, const gsj_join_opts& o, gsj_dist_cb cb, void* user, gsj_join_stats* st);
 
 }  // namespace gsj

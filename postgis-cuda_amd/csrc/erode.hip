@@ -1,0 +1,638 @@
+// Eroded intersects of polygon layers: ST_Intersects(ST_Buffer(A::geography, -ra)::geometry,
+// ST_Buffer(B::geography, -rb)::geometry) without building buffers (reference: engine/buffer.cuh:275-588 er_pair and
+// its parts, 590-684 main pass, 717-889 filters and erode_join_stream).
+//
+// X_A = A minus E_A, E_A = the points within r of A's real boundary: a two-sided strip per edge, a disk per convex
+// vertex, GEOS's chord fillet (round(angle / (pi/16)) chords) per reflex vertex. X_A and X_B meet iff their boundaries
+// (inward offset edges and fillet chords, the "elements") cross outside E_A and E_B, or an element vertex or element
+// self-crossing of one lies inside the other eroded polygon, all inside the window W = shrink(bbox A, ra) and
+// shrink(bbox B, rb), in a metric frame at W's centre latitude with r divided by the UTM point scale factor.
+//
+// Pipeline: pass 1 is the exact intersects + area join (refine.hip) over the candidates that survive a box prefilter;
+// its parent rows that could hold a disk of radius min(ra, rb) in their overlap go to pass 2, one wave per parent pair
+// (all pieces of both parents at once: a piece's erosion can come from a neighbouring piece's boundary). Lists live
+// in LDS (64 edges / 128 elements per side); pairs that need more are re-run with global-memory lists of the size they
+// reported, on the same window. The reference splits such windows into tiles; evaluating the whole window instead
+// gives exactly the CPU restatement's answer.
+#include "pairs.h"
+#include "geodesy.h"
+
+namespace gsj {
+
+namespace {
+
+constexpr double FILLET_QUANTUM = gm::PI_HI / 16.0;   // GEOS: 8 quadrant segments
+constexpr int ER_CAPE = 64, ER_CAPL = 128;            // LDS list capacities per side
+constexpr int ER_WAVES = 2;                           // waves per block
+
+// real boundary edge in the frame (metres); vt: 0 no region at the end vertex, 1 convex, m + 1 reflex (m chords)
+struct BEd { double x0, y0, x1, y1, ux, uy, L, ci, si; int succ, vt; };
+struct BEl { double x0, y0, x1, y1; };   // element: piece of the raw offset curve
+
+struct ErScratch { BEd* EA; BEd* EB; BEl* LA; BEl* LB; int capE, capL; };
+__host__ __device__ constexpr size_t er_bytes(int capE, int capL) { return 2 * (size_t)capE * sizeof(BEd) + 2 * (size_t)capL * sizeof(BEl); }
+__device__ __forceinline__ ErScratch er_carve(unsigned char* base, int capE, int capL) {
+  ErScratch s;
+  s.EA = reinterpret_cast<BEd*>(base);
+  s.EB = s.EA + capE;
+  s.LA = reinterpret_cast<BEl*>(s.EB + capE);
+  s.LB = s.LA + capL;
+  s.capE = capE;
+  s.capL = capL;
+  return s;
+}
+
+// strictly inside (by more than eps) the region within r of the boundary (buffer.cuh:285-320)
+__device__ bool in_excl(const BEd* E, int n, double r, double x, double y, double eps) {
+  if (r <= 0) return false;
+  const double ri = r - eps;
+  for (int k = 0; k < n; k++) {
+    const BEd& e = E[k];
+    if (x < fmin(e.x0, e.x1) - r || x > fmax(e.x0, e.x1) + r || y < fmin(e.y0, e.y1) - r || y > fmax(e.y0, e.y1) + r) continue;
+    const double dx = x - e.x0, dy = y - e.y0;
+    const double t = dx * e.ux + dy * e.uy, s = dy * e.ux - dx * e.uy;   // s > 0: interior (left) side
+    if (fabs(s) < ri && t > -eps && t < e.L + eps) return true;         // two-sided strip along the edge
+    if (e.vt == 0) continue;
+    const double vx = x - e.x1, vy = y - e.y1, d2 = vx * vx + vy * vy;
+    if (d2 >= r * r) continue;
+    const BEd& f = E[e.succ];
+    if (e.vt == 1) {   // convex vertex: disk outside the interior wedge
+      const double s2 = vy * f.ux - vx * f.uy;
+      if (d2 < ri * ri && !(s > eps && s2 > eps)) return true;
+      continue;
+    }
+    // reflex vertex: fillet polygon (v, P0 .. Pm) from n_k clockwise to n_s
+    const double nkx = -e.uy, nky = e.ux, nsx = -f.uy, nsy = f.ux;
+    if (nkx * vy - nky * vx > eps) continue;
+    if (nsx * vy - nsy * vx < -eps) continue;
+    const double chh = sqrt(0.5 * (1.0 + e.ci)), shh = sqrt(fmax(0.0, 0.5 * (1.0 - e.ci)));   // cos, sin of half the increment
+    const double ch = r * chh - eps;                                                         // chord distance from v
+    if (d2 < ch * ch) return true;
+    double mx = nkx * chh + nky * shh, my = -nkx * shh + nky * chh;   // chord normals
+    bool in = true;
+    for (int i = 0, m = e.vt - 1; i < m && in; i++) {
+      if (mx * vx + my * vy >= ch) in = false;
+      const double t2 = mx * e.ci + my * e.si;
+      my = -mx * e.si + my * e.ci;
+      mx = t2;
+    }
+    if (in) return true;
+  }
+  return false;
+}
+
+// inside(x) = inside(c) xor parity of the real edges crossing c -> x (buffer.cuh:323-335)
+__device__ bool inside_m(const BEd* E, int n, double cx, double cy, bool in_c, double x, double y) {
+  bool in = in_c;
+  const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
+  for (int k = 0; k < n; k++) {
+    const BEd& e = E[k];
+    if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) continue;
+    const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
+    if (sc == sx) continue;
+    const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
+    if (su != sv) in = !in;
+  }
+  return in;
+}
+
+struct ESide { uint32_t g0, g1; double r, bx0, by0, bx1, by1; };
+struct EFrame { double ox, oy, kx, ky; };
+
+__device__ __forceinline__ void fv(const DPolys& S, double shx, double shy, uint64_t v, const EFrame& F, double& x, double& y) {
+  const double2 q = S.xy[v];
+  x = (shx + q.x) * F.kx;
+  y = (shy + q.y) * F.ky;
+}
+
+// c inside the parent: parity over the pieces whose bbox holds c (buffer.cuh:351-374)
+__device__ bool pip_parent(const DPolys& S, const ESide& s, const EFrame& F, double cx, double cy, int lane) {
+  bool in = false;
+  for (uint32_t g = s.g0; g < s.g1 && !in; g++) {
+    const double2 o = S.origin[g], e = S.ext[g];
+    const double bx0 = (o.x - F.ox) * F.kx, by0 = (o.y - F.oy) * F.ky, bx1 = (o.x + e.x - F.ox) * F.kx, by1 = (o.y + e.y - F.oy) * F.ky;
+    if (cx < bx0 - 1e-6 || cx > bx1 + 1e-6 || cy < by0 - 1e-6 || cy > by1 + 1e-6) continue;
+    const double shx = o.x - F.ox, shy = o.y - F.oy, ox = bx0 - 1.0;
+    bool par = false;
+    for (uint32_t r = S.gro[g]; r < S.gro[g + 1]; r++) {
+      const uint64_t s0 = S.rvo[r], ee = S.rvo[r + 1];
+      for (uint64_t j = s0 + 1 + lane; j < ee; j += 64) {
+        double x0, y0, x1, y1;
+        fv(S, shx, shy, j - 1, F, x0, y0);
+        fv(S, shx, shy, j, F, x1, y1);
+        if ((y0 > cy) == (y1 > cy) && y0 != cy && y1 != cy) continue;
+        const bool so = left_of(x0, y0, x1, y1, ox, cy), sc = left_of(x0, y0, x1, y1, cx, cy);
+        if (so == sc) continue;
+        const bool su = orient(ox, cy, cx, cy, x0, y0) > 0, sv = orient(ox, cy, cx, cy, x1, y1) > 0;
+        if (su != sv) par = !par;
+      }
+    }
+    in = (__popcll(__ballot(par)) & 1) != 0;
+  }
+  return in;
+}
+
+// the parent's real, non-degenerate edges meeting box C, ballot-compacted; returns the full count (buffer.cuh:377-405)
+__device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, double cx0, double cy0, double cx1, double cy1, BEd* E, int cap,
+                            int lane) {
+  int cnt = 0;
+  for (uint32_t g = s.g0; g < s.g1; g++) {
+    const double2 o = S.origin[g], ex = S.ext[g];
+    if ((o.x + ex.x - F.ox) * F.kx < cx0 || (o.x - F.ox) * F.kx > cx1 || (o.y + ex.y - F.oy) * F.ky < cy0 || (o.y - F.oy) * F.ky > cy1) continue;
+    const double shx = o.x - F.ox, shy = o.y - F.oy;
+    for (uint32_t r = S.gro[g]; r < S.gro[g + 1]; r++) {
+      const uint64_t s0 = S.rvo[r], e = S.rvo[r + 1];
+      for (uint64_t base = s0; base + 1 < e; base += 64) {
+        const uint64_t i = base + lane;
+        bool v = false;
+        BEd ed;
+        if (i + 1 < e && !syn_bit(S.synb, i)) {
+          fv(S, shx, shy, i, F, ed.x0, ed.y0);
+          fv(S, shx, shy, i + 1, F, ed.x1, ed.y1);
+          v = !(ed.x0 == ed.x1 && ed.y0 == ed.y1) &&
+              !(fmax(ed.x0, ed.x1) < cx0 || fmin(ed.x0, ed.x1) > cx1 || fmax(ed.y0, ed.y1) < cy0 || fmin(ed.y0, ed.y1) > cy1);
+        }
+        const uint64_t m = __ballot(v);
+        const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
+        if (v && pos < cap) E[pos] = ed;
+        cnt += __popcll(m);
+      }
+    }
+  }
+  return cnt;
+}
+
+// directions, successors and end-vertex regions (buffer.cuh:408-438)
+__device__ void prepare_side(BEd* E, int n, double r, int lane) {
+  for (int k = lane; k < n; k += 64) {
+    const double dx = E[k].x1 - E[k].x0, dy = E[k].y1 - E[k].y0, L = sqrt(dx * dx + dy * dy);
+    E[k].ux = dx / L;
+    E[k].uy = dy / L;
+    E[k].L = L;
+  }
+  wave_lds_sync();
+  // the successor search reads only positions and directions; the fields written below are disjoint from them
+  for (int k = lane; k < n; k += 64) {
+    const BEd e = E[k];
+    int s = -1;
+    if (k + 1 < n && E[k + 1].x0 == e.x1 && E[k + 1].y0 == e.y1) s = k + 1;
+    else
+      for (int j = 0; j < n; j++)
+        if (j != k && E[j].x0 == e.x1 && E[j].y0 == e.y1) { s = j; break; }
+    int vt = 0;
+    double ci = 1, si = 0;
+    if (s >= 0 && r > 0) {
+      const BEd& f = E[s];
+      const double cr = e.ux * f.uy - e.uy * f.ux, dt = e.ux * f.ux + e.uy * f.uy;
+      if (cr > 0) vt = 1;
+      else if (cr < 0 || dt < 0) {   // right turn, or a reversal: fillet (GEOS joins a reversal with a half circle)
+        double th = gm::atan2(-cr, dt);
+        if (cr == 0) th = gm::PI_HI;
+        int m = (int)(th / FILLET_QUANTUM + 0.5);
+        if (m < 1) m = 1;
+        const double inc = th / m;
+        gm::sincos(inc, si, ci);
+        vt = m + 1;
+      }
+    }
+    E[k].succ = s;
+    E[k].vt = vt;
+    E[k].ci = ci;
+    E[k].si = si;
+  }
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// offset edges and reflex fillet chords meeting the window (+- eps), ballot-compacted; returns the full count
+// (buffer.cuh:441-475)
+__device__ int build_elements(const BEd* E, int n, double r, double wx0, double wy0, double wx1, double wy1, double eps, BEl* L, int cap, int lane) {
+  int cnt = 0;
+  for (int base = 0; base < n; base += 64) {
+    const int k = base + lane;
+    const bool valid = k < n;
+    BEd e{};
+    if (valid) e = E[k];
+    const int ne = valid ? 1 + (e.vt >= 2 ? e.vt - 1 : 0) : 0;
+    const int mx = wave_max(ne);
+    double nkx = 0, nky = 0, wxr = 0, wyr = 0, px = 0, py = 0, nsx = 0, nsy = 0;
+    if (valid) {
+      nkx = -e.uy; nky = e.ux; wxr = nkx; wyr = nky;
+      px = e.x1 + r * nkx; py = e.y1 + r * nky;
+      if (e.vt >= 2) { nsx = -E[e.succ].uy; nsy = E[e.succ].ux; }
+    }
+    for (int it = 0; it < mx; it++) {
+      bool has = it < ne;
+      BEl el{};
+      if (has) {
+        if (it == 0) { el.x0 = e.x0 + r * nkx; el.y0 = e.y0 + r * nky; el.x1 = e.x1 + r * nkx; el.y1 = e.y1 + r * nky; }
+        else {
+          const double t2 = wxr * e.ci + wyr * e.si;
+          wyr = -wxr * e.si + wyr * e.ci;
+          wxr = t2;
+          double qx, qy;
+          if (it == ne - 1) { qx = e.x1 + r * nsx; qy = e.y1 + r * nsy; }
+          else { qx = e.x1 + r * wxr; qy = e.y1 + r * wyr; }
+          el.x0 = px; el.y0 = py; el.x1 = qx; el.y1 = qy;
+          px = qx; py = qy;
+        }
+        has = !(fmax(el.x0, el.x1) < wx0 - eps || fmin(el.x0, el.x1) > wx1 + eps || fmax(el.y0, el.y1) < wy0 - eps || fmin(el.y0, el.y1) > wy1 + eps);
+      }
+      const uint64_t m = __ballot(has);
+      const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
+      if (has && pos < cap) L[pos] = el;
+      cnt += __popcll(m);
+    }
+  }
+  return cnt;
+}
+
+// intersection points of two intersecting segments: 1 (proper / touching) or 3 (collinear overlap) (buffer.cuh:478-498)
+__device__ __forceinline__ int seg_points(const BEl& p, const BEl& q, double* xs, double* ys) {
+  const double rx = p.x1 - p.x0, ry = p.y1 - p.y0, sx = q.x1 - q.x0, sy = q.y1 - q.y0;
+  const double den = rx * sy - ry * sx;
+  if (den != 0) {
+    double t = ((q.x0 - p.x0) * sy - (q.y0 - p.y0) * sx) / den;
+    t = fmin(1.0, fmax(0.0, t));
+    xs[0] = p.x0 + t * rx; ys[0] = p.y0 + t * ry;
+    return 1;
+  }
+  const double l2 = rx * rx + ry * ry;
+  if (l2 == 0) { xs[0] = p.x0; ys[0] = p.y0; return 1; }
+  const double t0 = ((q.x0 - p.x0) * rx + (q.y0 - p.y0) * ry) / l2, t1 = ((q.x1 - p.x0) * rx + (q.y1 - p.y0) * ry) / l2;
+  double lo = fmax(0.0, fmin(t0, t1)), hi = fmin(1.0, fmax(t0, t1));
+  if (lo > hi) { const double m = 0.5 * (lo + hi); lo = hi = fmin(1.0, fmax(0.0, m)); }
+  xs[0] = p.x0 + lo * rx; ys[0] = p.y0 + lo * ry;
+  xs[1] = p.x0 + hi * rx; ys[1] = p.y0 + hi * ry;
+  xs[2] = p.x0 + 0.5 * (lo + hi) * rx; ys[2] = p.y0 + 0.5 * (lo + hi) * ry;
+  return 3;
+}
+
+__device__ __forceinline__ void piece_range(const uint32_t* pf, uint32_t p, uint32_t& g0, uint32_t& g1) {
+  if (pf) { g0 = pf[p]; g1 = pf[p + 1]; }
+  else { g0 = p; g1 = p + 1; }
+}
+
+// X_A and X_B intersect within the window (buffer.cuh:502-588). One wave; returns 1 / 0, or -1 when a list overflows
+// (needE / needL: sizes required; needL is 0 when the edge lists already overflowed).
+__device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, uint32_t pa, uint32_t pb, double ra_nom,
+                       double rb_nom, const ErScratch& sc, int lane, int& needE, int& needL) {
+  ESide A{0, 0, 0, 1e300, 1e300, -1e300, -1e300}, B{0, 0, 0, 1e300, 1e300, -1e300, -1e300};
+  piece_range(pfP, pa, A.g0, A.g1);
+  piece_range(pfR, pb, B.g0, B.g1);
+  if (A.g0 >= A.g1 || B.g0 >= B.g1) return 0;
+  for (uint32_t g = A.g0; g < A.g1; g++) {
+    const double2 o = P.origin[g], e = P.ext[g];
+    A.bx0 = fmin(A.bx0, o.x); A.by0 = fmin(A.by0, o.y); A.bx1 = fmax(A.bx1, o.x + e.x); A.by1 = fmax(A.by1, o.y + e.y);
+  }
+  for (uint32_t g = B.g0; g < B.g1; g++) {
+    const double2 o = R.origin[g], e = R.ext[g];
+    B.bx0 = fmin(B.bx0, o.x); B.by0 = fmin(B.by0, o.y); B.bx1 = fmax(B.bx1, o.x + e.x); B.by1 = fmax(B.by1, o.y + e.y);
+  }
+  // window (world degrees): both bboxes shrunk by a lower bound of the true radius (k <= 1.001 in a UTM zone)
+  double dax, day, dbx, dby;
+  geo::deg_cover(A.by0, A.by1, ra_nom * 0.998, false, dax, day);
+  geo::deg_cover(B.by0, B.by1, rb_nom * 0.998, false, dbx, dby);
+  const double wx0 = fmax(A.bx0 + dax, B.bx0 + dbx), wx1 = fmin(A.bx1 - dax, B.bx1 - dbx);
+  const double wy0 = fmax(A.by0 + day, B.by0 + dby), wy1 = fmin(A.by1 - day, B.by1 - dby);
+  if (wx0 > wx1 || wy0 > wy1) return 0;
+  const double clon = 0.5 * (wx0 + wx1), clat = 0.5 * (wy0 + wy1);
+  A.r = ra_nom > 0 ? ra_nom / geo::utm_k(0.5 * (A.bx0 + A.bx1), 0.5 * (A.by0 + A.by1), clon, clat) : 0.0;
+  B.r = rb_nom > 0 ? rb_nom / geo::utm_k(0.5 * (B.bx0 + B.bx1), 0.5 * (B.by0 + B.by1), clon, clat) : 0.0;
+  const double2 fo = P.origin[A.g0];
+  EFrame F{fo.x, fo.y, 0, 0};
+  geo::m_per_deg(clat, F.kx, F.ky);
+  const double Wx0 = (wx0 - F.ox) * F.kx, Wx1 = (wx1 - F.ox) * F.kx, Wy0 = (wy0 - F.oy) * F.ky, Wy1 = (wy1 - F.oy) * F.ky;
+  const double ext = fmax(fmax(fabs(Wx0), fabs(Wx1)), fmax(fabs(Wy0), fabs(Wy1))) + fmax(A.r, B.r);
+  const double eps = 1e-9 + 1e-12 * ext;
+  // the frame metric is exact at the window's centre latitude; widen the collect margin for its error
+  const double fe = 1.0 + 2.0 * 0.5 * (wy1 - wy0) * geo::DEG * (fabs(gm::tan(clat * geo::DEG)) + 0.05);
+  const double ma = A.r * fe + 1e-6, mb = B.r * fe + 1e-6;
+  const int na = collect_side(P, A, F, Wx0 - ma, Wy0 - ma, Wx1 + ma, Wy1 + ma, sc.EA, sc.capE, lane);
+  const int nb = collect_side(R, B, F, Wx0 - mb, Wy0 - mb, Wx1 + mb, Wy1 + mb, sc.EB, sc.capE, lane);
+  if (na > sc.capE || nb > sc.capE) { needE = max(na, nb); needL = 0; return -1; }
+  wave_lds_sync();
+  prepare_side(sc.EA, na, A.r, lane);
+  prepare_side(sc.EB, nb, B.r, lane);
+  const int la = build_elements(sc.EA, na, A.r, Wx0, Wy0, Wx1, Wy1, eps, sc.LA, sc.capL, lane);
+  const int lb = build_elements(sc.EB, nb, B.r, Wx0, Wy0, Wx1, Wy1, eps, sc.LB, sc.capL, lane);
+  if (la > sc.capL || lb > sc.capL) { needE = max(na, nb); needL = max(la, lb); return -1; }
+  wave_lds_sync();
+  const BEd* EA = sc.EA;
+  const BEd* EB = sc.EB;
+  const BEl* LA = sc.LA;
+  const BEl* LB = sc.LB;
+  auto inW = [&](double x, double y) { return x >= Wx0 - eps && x <= Wx1 + eps && y >= Wy0 - eps && y <= Wy1 + eps; };
+  // (a) the eroded boundaries cross outside both exclusion zones
+  bool hit = false;
+  for (int i = lane; i < la && !hit; i += 64) {
+    const BEl p = LA[i];
+    const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
+    for (int j = 0; j < lb && !hit; j++) {
+      const BEl q = LB[j];
+      if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
+      if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
+      double xs[3], ys[3];
+      const int k = seg_points(p, q, xs, ys);
+      for (int t = 0; t < k && !hit; t++)
+        if (inW(xs[t], ys[t]) && !in_excl(EA, na, A.r, xs[t], ys[t], eps) && !in_excl(EB, nb, B.r, xs[t], ys[t], eps)) hit = true;
+    }
+  }
+  if (__ballot(hit)) return 1;
+  // (b), (c): a vertex of one eroded boundary inside the other eroded polygon
+  const double cx = Wx0 + 0.381966 * (Wx1 - Wx0), cy = Wy0 + 0.414213 * (Wy1 - Wy0);
+  const bool inA_c = pip_parent(P, A, F, cx, cy, lane);
+  const bool inB_c = pip_parent(R, B, F, cx, cy, lane);
+  for (int side = 0; side < 2; side++) {
+    const BEd* ES = side ? EB : EA;
+    const BEd* EO = side ? EA : EB;
+    const BEl* LS = side ? LB : LA;
+    const int ns = side ? nb : na, no = side ? na : nb, ls = side ? lb : la;
+    const double rs = side ? B.r : A.r, ro = side ? A.r : B.r;
+    const bool inO_c = side ? inA_c : inB_c;
+    auto vertex_in_other = [&](double x, double y) {
+      return inW(x, y) && !in_excl(ES, ns, rs, x, y, eps) && inside_m(EO, no, cx, cy, inO_c, x, y) && !in_excl(EO, no, ro, x, y, eps);
+    };
+    for (int i = lane; i < ls && !hit; i += 64) {
+      const BEl p = LS[i];
+      if (vertex_in_other(p.x0, p.y0) || vertex_in_other(p.x1, p.y1)) { hit = true; break; }
+      const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
+      for (int j = i + 1; j < ls && !hit; j++) {
+        const BEl q = LS[j];
+        if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
+        if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
+        double xs[3], ys[3];
+        const int k = seg_points(p, q, xs, ys);
+        for (int t = 0; t < k && !hit; t++)
+          if (vertex_in_other(xs[t], ys[t])) hit = true;
+      }
+    }
+    if (__ballot(hit)) return 1;
+  }
+  return 0;
+}
+
+struct ErCtl { uint32_t n; int needE, needL; uint32_t pad; };
+
+// main pass: a wave per parent pair, LDS lists; overflowing pairs are queued with the sizes they need
+__global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
+                                                              const uint32_t* __restrict__ pfR, const uint32_t* __restrict__ pairs, uint32_t np,
+                                                              double ra, double rb, uint8_t* __restrict__ flag, uint32_t* __restrict__ heavy,
+                                                              ErCtl* ctl) {
+  __shared__ __align__(16) unsigned char lds[ER_WAVES][er_bytes(ER_CAPE, ER_CAPL)];
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const ErScratch sc = er_carve(lds[wib], ER_CAPE, ER_CAPL);
+  for (uint32_t p = blockIdx.x * ER_WAVES + wib; p < np; p += gridDim.x * ER_WAVES) {
+    int ne = 0, nl = 0;
+    const int r = er_pair(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
+    if (lane == 0) {
+      flag[p] = r > 0 ? 1 : 0;
+      if (r < 0) {
+        heavy[atomicAdd(&ctl->n, 1u)] = p;
+        atomicMax(&ctl->needE, ne);
+        atomicMax(&ctl->needL, nl);
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// pairs that overflowed: per-wave lists in global memory of the size the queue reported; those that overflow the
+// element lists again are re-queued with their exact element count
+__global__ void __launch_bounds__(256) k_erode_slab(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP, const uint32_t* __restrict__ pfR,
+                                                    const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ items, uint32_t ni, double ra,
+                                                    double rb, uint8_t* __restrict__ flag, unsigned char* __restrict__ slab, int capE, int capL,
+                                                    uint32_t* __restrict__ again, ErCtl* ctl) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const ErScratch sc = er_carve(slab + wave * er_bytes(capE, capL), capE, capL);
+  for (uint64_t i = wave; i < ni; i += nwaves) {
+    const uint32_t p = items[i];
+    int ne = 0, nl = 0;
+    const int r = er_pair(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
+    if (lane == 0) {
+      flag[p] = r > 0 ? 1 : 0;
+      if (r < 0) {
+        again[atomicAdd(&ctl->n, 1u)] = p;
+        atomicMax(&ctl->needE, ne);
+        atomicMax(&ctl->needL, nl);
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+__global__ void k_ctl_clear(ErCtl* ctl) { ctl->n = 0; ctl->needE = 0; ctl->needL = 0; }
+
+// candidate piece pairs whose eroded polygons can meet: not both whole geometries, or shrunk bboxes that overlap
+// (buffer.cuh:751-776)
+__global__ void k_er_prefilter(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t n,
+                               double ra, double rb, uint32_t* __restrict__ keep) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t a = pa[i], b = pb[i];
+  auto whole = [](const DPolys& S, uint32_t g) {
+    if (!S.parent) return true;
+    const uint32_t q = S.parent[g];
+    return (g == 0 || S.parent[g - 1] != q) && (g + 1 >= S.n || S.parent[g + 1] != q);
+  };
+  bool k = true;
+  if (whole(P, a) && whole(R, b)) {
+    const double2 oa = P.origin[a], ea = P.ext[a], ob = R.origin[b], eb = R.ext[b];
+    double dax, day, dbx, dby;
+    geo::deg_cover(oa.y, oa.y + ea.y, ra * 0.998, false, dax, day);
+    geo::deg_cover(ob.y, ob.y + eb.y, rb * 0.998, false, dbx, dby);
+    const double wx0 = fmax(oa.x + dax, ob.x + dbx), wx1 = fmin(oa.x + ea.x - dax, ob.x + eb.x - dbx);
+    const double wy0 = fmax(oa.y + day, ob.y + dby), wy1 = fmin(oa.y + ea.y - day, ob.y + eb.y - dby);
+    k = wx0 <= wx1 && wy0 <= wy1;
+  }
+  keep[i] = k ? 1u : 0u;
+}
+__global__ void k_compact_pairs(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, const uint32_t* __restrict__ keep,
+                                const uint64_t* __restrict__ off, uint64_t n, uint32_t* __restrict__ oa, uint32_t* __restrict__ ob) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  oa[off[i]] = pa[i];
+  ob[off[i]] = pb[i];
+}
+__global__ void k_copy_u32(const uint32_t* __restrict__ a, uint64_t n, uint32_t* __restrict__ b) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// parent rows of pass 1 that go to pass 2: overlap area able to hold a disk of radius rmin (rmin 0: any overlap,
+// < 0: every intersecting pair), self pairs only where needed (buffer.cuh:722-748)
+__global__ void k_er_filter(const uint32_t* __restrict__ rows, const double* __restrict__ area, uint64_t n, const DPolys P,
+                            const uint32_t* __restrict__ pfP, double rmin, int same, const uint8_t* __restrict__ need_self, uint32_t* __restrict__ keep) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t a = rows[2 * i], b = rows[2 * i + 1];
+  bool k = true;
+  if (same && a == b && need_self) k = need_self[a] != 0;
+  if (k && rmin >= 0) {
+    const uint32_t g = pfP ? pfP[a] : a;
+    const double y0 = P.origin[g].y, y1 = y0 + P.ext[g].y;
+    const double lat = (y0 <= 0 && y1 >= 0) ? 0.0 : (fabs(y0) < fabs(y1) ? y0 : y1);
+    double kx, ky;
+    geo::m_per_deg(lat, kx, ky);
+    const double a_m2 = area[i] * kx * ky * 1.001;   // upper bound over the pair's latitudes
+    const double r = 0.98 * rmin;                    // GEOS's eroded polygon reaches up to ~2% closer to the boundary
+    k = rmin > 0 ? a_m2 >= gm::PI_HI * r * r : area[i] > 0;
+  }
+  keep[i] = k ? 1u : 0u;
+}
+__global__ void k_compact_rows(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ keep, const uint64_t* __restrict__ off, uint64_t n,
+                               uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  out[2 * off[i]] = rows[2 * i];
+  out[2 * off[i] + 1] = rows[2 * i + 1];
+}
+__global__ void k_flag_u32(const uint8_t* __restrict__ f, uint64_t n, uint32_t* __restrict__ o) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = f[i] ? 1u : 0u;
+}
+
+}  // namespace
+
+void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
+               gsj_dist_cb cb, void* user, gsj_join_stats* st) {
+  GSJ_REQUIRE(!PL.points && !RL.points, GSJ_E_ARG, "intersects_buffered: both layers must be polygon layers (points: the points entry point)");
+  GSJ_REQUIRE(RL.has_bvh, GSJ_E_ARG, "intersects_buffered: build layer has no BVH");
+  GSJ_REQUIRE(ra >= 0 && rb >= 0 && ra < 1e5 && rb < 1e5, GSJ_E_ARG, "intersects_buffered: radii must be in [0, 1e5) metres");
+  gsj_join_stats s{};
+  if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
+  hipStream_t stream = c.stream;
+  const DPolys& P = PL.poly;
+  const DPolys& R = RL.poly;
+  const uint32_t* pfP = P.pfirst;   // first piece per parent (subdivided layers), as k_piece_first builds it
+  const uint32_t* pfR = R.pfirst;
+  // one allocation for the need_self flags and the overflow queue's control block (allocated once per call)
+  const size_t need_bytes = ((size_t)P.nparent + 16 + 63) & ~size_t(63);
+  uint8_t* er_misc = c.s_er_need.as<uint8_t>(need_bytes + 64);
+  ErCtl* ctl = reinterpret_cast<ErCtl*>(er_misc + need_bytes);
+  const uint8_t* d_need = nullptr;
+  if (same && need_self) {
+    GSJ_HIP(hipMemcpyAsync(er_misc, need_self, P.nparent, hipMemcpyHostToDevice, stream));
+    d_need = er_misc;
+  }
+  const double rmin = (ra > 0 && rb > 0) ? std::min(ra, rb) : (ra > 0 || rb > 0 ? 0.0 : -1.0);
+  gsj_join_opts o1 = o;
+  o1.want_area = 1;
+  o1.want_relate = 0;
+  o1.no_agg = 0;
+  JoinHooks hooks;
+  hooks.device_only = true;
+  if (ra > 0 || rb > 0)
+    hooks.prefilter = [&](uint32_t* pa, uint32_t* pb, uint64_t n) -> uint64_t {
+      uint32_t* keep = c.s_er_keep.as<uint32_t>(n + 16);
+      uint64_t* off = c.s_er_off.as<uint64_t>(n + 16);
+      uint32_t* tmp = c.s_er_pairs.as<uint32_t>(2 * n + 16);
+      hipLaunchKernelGGL(k_er_prefilter, dim3(div_up(n, 256)), dim3(256), 0, stream, P, R, pa, pb, n, ra, rb, keep);
+      GSJ_LAUNCHED();
+      const uint64_t k = scan_u32_to_u64(c, keep, off, n);
+      hipLaunchKernelGGL(k_compact_pairs, dim3(div_up(n, 256)), dim3(256), 0, stream, pa, pb, keep, off, n, tmp, tmp + n);
+      GSJ_LAUNCHED();
+      if (k) {
+        hipLaunchKernelGGL(k_copy_u32, dim3(div_up(k, 256)), dim3(256), 0, stream, tmp, k, pa);
+        hipLaunchKernelGGL(k_copy_u32, dim3(div_up(k, 256)), dim3(256), 0, stream, tmp + n, k, pb);
+        GSJ_LAUNCHED();
+      }
+      GSJ_HIP(hipStreamSynchronize(stream));
+      return k;
+    };
+  uint64_t rows_out = 0;
+  double er_ms = 0;
+  hooks.on_device = [&](const uint32_t* rows, const double* area, uint64_t n) {
+    WallTimer t0;
+    uint32_t* keep = c.s_er_keep.as<uint32_t>(n + 16);
+    uint64_t* off = c.s_er_off.as<uint64_t>(n + 16);
+    hipLaunchKernelGGL(k_er_filter, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, area, n, P, pfP, rmin, same ? 1 : 0, d_need, keep);
+    GSJ_LAUNCHED();
+    const uint64_t ne = scan_u32_to_u64(c, keep, off, n);
+    if (ne == 0) { er_ms += t0.ms(); return; }
+    uint32_t* ep = c.s_er_pairs.as<uint32_t>(2 * ne + 16);
+    hipLaunchKernelGGL(k_compact_rows, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, keep, off, n, ep);
+    GSJ_LAUNCHED();
+    uint8_t* flag = c.s_er_flag.as<uint8_t>(ne + 16);
+    uint32_t* lists = c.s_er_list.as<uint32_t>(2 * ne + 32);
+    uint32_t* heavy = lists;
+    uint32_t* again = lists + ne + 16;
+    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
+    GSJ_HIP(hipEventRecord(c.ev[0], stream));
+    const uint32_t g = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES), (uint64_t)c.cu_count * 32);
+    hipLaunchKernelGGL(k_erode_main, dim3(g), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, ra, rb, flag, heavy, ctl);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[1], stream));
+    ErCtl* h = c.h_scal.as<ErCtl>(2);
+    GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+    s.k_large_ms += ms;
+    s.n_large += ne;
+    s.launches++;
+    uint32_t nh = h->n;
+    int capE = std::max(h->needE, ER_CAPE), capL = std::max(h->needL, 4 * capE);
+    s.heavy += nh;
+    s.n_over += nh;
+    for (int round = 0; nh; round++) {
+      GSJ_REQUIRE(round < 4, GSJ_E_INTERNAL, "erosion: overflowing pairs did not converge");
+      const size_t per = er_bytes(capE, capL);
+      uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nh, (1ull << 30) / per));
+      waves = (waves + 3) / 4 * 4;
+      unsigned char* slab = c.s_er_slab.as<unsigned char>(per * waves + 64);
+      hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
+      GSJ_HIP(hipEventRecord(c.ev[2], stream));
+      hipLaunchKernelGGL(k_erode_slab, dim3((uint32_t)(waves / 4)), dim3(256), 0, stream, P, R, pfP, pfR, ep, heavy, nh, ra, rb, flag, slab, capE,
+                         capL, again, ctl);
+      GSJ_LAUNCHED();
+      GSJ_HIP(hipEventRecord(c.ev[3], stream));
+      GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
+      GSJ_HIP(hipStreamSynchronize(stream));
+      float mo = 0;
+      GSJ_HIP(hipEventElapsedTime(&mo, c.ev[2], c.ev[3]));
+      s.k_over_ms += mo;
+      nh = h->n;
+      capE = std::max(capE, h->needE);
+      capL = std::max(capL, h->needL);
+      std::swap(heavy, again);
+    }
+    // rows: flagged parent pairs in pass-1 order
+    uint32_t* f = c.s_er_keep.as<uint32_t>(ne + 16);
+    hipLaunchKernelGGL(k_flag_u32, dim3(div_up(ne, 256)), dim3(256), 0, stream, flag, ne, f);
+    GSJ_LAUNCHED();
+    const uint64_t nhit = scan_u32_to_u64(c, f, off, ne);
+    er_ms += t0.ms();
+    if (nhit == 0) return;
+    uint32_t* out = c.s_er_out.as<uint32_t>(2 * nhit + 16);
+    hipLaunchKernelGGL(k_compact_rows, dim3(div_up(ne, 256)), dim3(256), 0, stream, ep, f, off, ne, out);
+    GSJ_LAUNCHED();
+    rows_out += nhit;
+    WallTimer t4;
+    uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nhit);
+    GSJ_HIP(hipMemcpyAsync(hrow, out, nhit * 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.copy_ms += t4.ms();
+    if (cb && cb(user, hrow, nullptr, (size_t)nhit)) throw Error(GSJ_E_CANCEL, "intersects_buffered cancelled by the row callback");
+  };
+  gsj_join_stats js{};
+  join_run(c, PL, RL, o1, nullptr, nullptr, &js, &hooks);
+  s.traverse_ms = js.traverse_ms;
+  s.candidates = js.candidates;
+  s.batches = js.batches;
+  s.refine_ms = js.refine_ms + er_ms;
+  s.compact_ms += js.compact_ms;
+  s.k_small_ms = js.k_small_ms;
+  s.n_small = js.n_small + js.n_large;
+  s.hits = rows_out;
+  if (st) *st = s;
+}
+
+}  // namespace gsj

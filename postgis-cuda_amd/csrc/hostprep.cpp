@@ -321,24 +321,37 @@ HostSet gen_parcels_grid(const Spec& sp) {
   const uint64_t seed = (uint64_t)sp.num("seed", 0);
   if (nx <= 0 || ny <= 0 || kmin < 3 || kmax < kmin) throw std::invalid_argument("parcels_grid: bad spec");
   Rng rng(seed, 1);
-  Builder b;
   const double two_pi = 6.283185307179586;
-  for (int j = 0; j < ny; j++)
-    for (int i = 0; i < nx; i++) {
-      const uint64_t id = (uint64_t)j * nx + i, base = id * 64;
-      const int k = kmin + (int)(rng.uni(base) * (kmax - kmin + 1));
-      const double phase = rng.uni(base + 1) * two_pi / k;
-      const double cx = x0 + (i + 0.5) * cell, cy = y0 + (j + 0.5) * cell;
-      const size_t first = b.s.xy.size() / 2;
-      for (int v = 0; v < k; v++) {
-        const double ang = phase + two_pi * v / k;
-        const double r = rfrac * cell * (rmin + (rmax - rmin) * rng.uni(base + 2 + v));
-        b.vertex(cx + r * std::cos(ang), cy + r * std::sin(ang));
-      }
-      b.close_ring(first);
-      b.end_geometry();
+  // two passes over the cells (vertex counts, then vertices at their offsets); every cell draws from its own
+  // counter range, so the result does not depend on the number of threads
+  const uint64_t n = (uint64_t)nx * ny;
+  HostSet s;
+  s.gro.resize(n + 1);
+  s.rvo.resize(n + 1);
+  std::vector<int> kk(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t id = 0; id < (int64_t)n; id++) kk[id] = kmin + (int)(rng.uni((uint64_t)id * 64) * (kmax - kmin + 1));
+  s.gro[0] = 0; s.rvo[0] = 0;
+  for (uint64_t id = 0; id < n; id++) { s.gro[id + 1] = id + 1; s.rvo[id + 1] = s.rvo[id] + kk[id] + 1; }
+  s.xy.resize(2 * s.rvo[n]);
+#pragma omp parallel for schedule(static)
+  for (int64_t id = 0; id < (int64_t)n; id++) {
+    const int i = (int)(id % nx), j = (int)(id / nx);
+    const uint64_t base = (uint64_t)id * 64;
+    const int k = kk[id];
+    const double phase = rng.uni(base + 1) * two_pi / k;
+    const double cx = x0 + (i + 0.5) * cell, cy = y0 + (j + 0.5) * cell;
+    double* out = &s.xy[2 * s.rvo[id]];
+    for (int v = 0; v < k; v++) {
+      const double ang = phase + two_pi * v / k;
+      const double r = rfrac * cell * (rmin + (rmax - rmin) * rng.uni(base + 2 + v));
+      out[2 * v] = cx + r * std::cos(ang);
+      out[2 * v + 1] = cy + r * std::sin(ang);
     }
-  return std::move(b.s);
+    out[2 * k] = out[0];
+    out[2 * k + 1] = out[1];
+  }
+  return s;
 }
 
 // config 1 regions: kx x ky tiles over [x0, x0 + nx*cell] x [y0, y0 + ny*cell]; every tile side has `edge_segs`

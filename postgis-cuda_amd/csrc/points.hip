@@ -123,7 +123,7 @@ void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_joi
   if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
   hipStream_t stream = c.stream;
   static const std::vector<uint32_t> no_parent;
-  for_each_candidate_batch(c, PL.pts.bb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, PL.pts.bb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
       WallTimer t2;
       uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
       hipLaunchKernelGGL(k_pip, dim3(div_up(np, 256)), dim3(256), 0, stream, PL.pts, RL.poly, pa, pb, np, hit);

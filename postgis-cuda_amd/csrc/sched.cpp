@@ -77,6 +77,12 @@ void gsj_sched::run(gsj::Context& c, Req& q) {
     case GSJ_REQ_POINTS_DWITHIN:
       gsj::points_dwithin_run(c, r.P->L, r.R->L, r.d, r.report_distance != 0, r.opts, r.dist_cb, r.user, &q.stats);
       break;
+    case GSJ_REQ_INTERSECTS_BUFFERED:
+      gsj::erode_run(c, r.P->L, r.R->L, r.ra, r.rb, r.same != 0, r.need_self, r.opts, r.dist_cb, r.user, &q.stats);
+      break;
+    case GSJ_REQ_POINTS_INTERSECTS_BUFFERED:
+      gsj::points_erode_run(c, r.P->L, r.R->L, r.rb, r.opts, r.dist_cb, r.user, &q.stats);
+      break;
     default:
       throw gsj::Error(GSJ_E_ARG, "unknown request kind " + std::to_string(r.kind));
   }
@@ -165,11 +171,11 @@ int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket) {
     q->r = r;
     q->est = estimate_scratch(r);
     q->t_submit = Clock::now();
-    const_cast<gsj_layer*>(r.P)->refs.fetch_add(1);
-    if (r.R) const_cast<gsj_layer*>(r.R)->refs.fetch_add(1);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       GSJ_REQUIRE(!s->stop, GSJ_E_ARG, "scheduler is stopping");
+      const_cast<gsj_layer*>(r.P)->refs.fetch_add(1);   // released by the lane when the request finishes
+      if (r.R) const_cast<gsj_layer*>(r.R)->refs.fetch_add(1);
       q->ticket = s->next_ticket++;
       *ticket = q->ticket;
       s->queue.push_back(q.get());

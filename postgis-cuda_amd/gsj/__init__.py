@@ -55,11 +55,13 @@ POINTS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.
 DIST_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.c_size_t)
 
 REQ_JOIN, REQ_POINTS_IN_POLYGONS, REQ_CLUSTER, REQ_DWITHIN, REQ_POINTS_DWITHIN = 1, 2, 3, 4, 5
+REQ_INTERSECTS_BUFFERED, REQ_POINTS_INTERSECTS_BUFFERED = 6, 7
 
 
 class Request(C.Structure):
     _fields_ = [("kind", C.c_int), ("P", C.c_void_p), ("R", C.c_void_p), ("opts", JoinOpts), ("d", C.c_double), ("same", C.c_int),
-                ("report_distance", C.c_int), ("rows_cb", ROWS_CB), ("points_cb", POINTS_CB), ("dist_cb", DIST_CB), ("user", C.c_void_p),
+                ("report_distance", C.c_int), ("ra", C.c_double), ("rb", C.c_double), ("need_self", C.POINTER(C.c_uint8)),
+                ("rows_cb", ROWS_CB), ("points_cb", POINTS_CB), ("dist_cb", DIST_CB), ("user", C.c_void_p),
                 ("root", C.POINTER(C.c_uint32)), ("nroot", C.c_uint64)]
 
 
@@ -73,7 +75,8 @@ _lib = None
 EXPORTS = [
     "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
-    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin",
+    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
+    "gsj_points_intersects_buffered",
     "gsj_sched_create", "gsj_sched_destroy", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
     "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_generate", "gsj_generate_points",
@@ -103,6 +106,8 @@ def load(path: str | None = None):
         "gsj_cluster": (i32, [vp, vp, C.POINTER(JoinOpts), u32p, u64, C.POINTER(JoinStats)]),
         "gsj_dwithin": (i32, [vp, vp, vp, C.c_double, i32, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_dwithin": (i32, [vp, vp, vp, C.c_double, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_intersects_buffered": (i32, [vp, vp, vp, C.c_double, C.c_double, i32, u8p, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_points_intersects_buffered": (i32, [vp, vp, vp, C.c_double, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_sched_create": (i32, [vp, i32, u64, C.POINTER(vp)]), "gsj_sched_destroy": (None, [vp]),
         "gsj_submit": (i32, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "gsj_wait": (i32, [vp, u64, C.POINTER(JoinStats), C.POINTER(RequestTimes)]), "gsj_poll": (i32, [vp, u64]),
@@ -435,6 +440,15 @@ class Context:
             r.dist = np.zeros(0)
         return r
 
+    def intersects_buffered(self, P: Layer, R: Layer, ra: float, rb: float, same=False, need_self=None, **kw) -> Rows:
+        """ST_Intersects(ST_Buffer(a::geography, -ra), ST_Buffer(b::geography, -rb)) rows (parent pairs)."""
+        ns = None if need_self is None else np.ascontiguousarray(need_self, np.uint8)
+        return self._dist_rows(load().gsj_intersects_buffered, P.h, R.h, float(ra), float(rb), int(same), _ptr(ns, C.c_uint8), **kw)
+
+    def points_intersects_buffered(self, pts: Layer, R: Layer, r: float, **kw) -> Rows:
+        """Points inside ST_Buffer(polygon::geography, -r): rows (point index, parent polygon)."""
+        return self._dist_rows(load().gsj_points_intersects_buffered, pts.h, R.h, float(r), **kw)
+
     def cluster(self, P: Layer, **kw):
         o = _opts(**kw)
         n = P.info().nparent
@@ -471,14 +485,19 @@ class Scheduler:
         self.ctx = ctx
         self._pending = {}
 
-    def submit(self, kind: int, P: "Layer", R: "Layer" = None, collect=True, d=0.0, same=False, report_distance=False, **kw) -> int:
+    def submit(self, kind: int, P: "Layer", R: "Layer" = None, collect=True, d=0.0, same=False, report_distance=False, ra=0.0, rb=0.0,
+               need_self=None, **kw) -> int:
         import threading
 
         req = Request()
         req.kind, req.P, req.R, req.opts = kind, P.h, (R.h if R is not None else None), _opts(**kw)
-        req.d, req.same, req.report_distance = float(d), int(same), int(report_distance)
+        req.d, req.same, req.report_distance, req.ra, req.rb = float(d), int(same), int(report_distance), float(ra), float(rb)
         chunks, lock = [], threading.Lock()
         keep = [req]
+        if need_self is not None:
+            ns = np.ascontiguousarray(need_self, np.uint8)
+            req.need_self = _ptr(ns, C.c_uint8)
+            keep.append(ns)
         root = None
         if kind == REQ_CLUSTER:
             n = P.info().nparent

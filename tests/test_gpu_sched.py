@@ -41,6 +41,8 @@ def _workload(ctx):
         (gsj.REQ_POINTS_DWITHIN, pts, M2, dict(d=3.0, report_distance=True)),
         (gsj.REQ_JOIN, M2, M1, dict(want_area=True)),
         (gsj.REQ_CLUSTER, M2, None, {}),
+        (gsj.REQ_INTERSECTS_BUFFERED, D, D, dict(ra=1.0, rb=1.5, same=True, upper=True)),
+        (gsj.REQ_POINTS_INTERSECTS_BUFFERED, pts, M2, dict(rb=2.0)),
     ]
     return reqs, [M1, M2, P, R, D, pts]
 
@@ -55,6 +57,10 @@ def _direct(ctx, kind, A, B, kw):
         return ctx.dwithin(A, B, kw.pop("d"), **kw)
     if kind == gsj.REQ_POINTS_DWITHIN:
         return ctx.points_dwithin(A, B, kw.pop("d"), **kw)
+    if kind == gsj.REQ_INTERSECTS_BUFFERED:
+        return ctx.intersects_buffered(A, B, kw.pop("ra"), kw.pop("rb"), **kw)
+    if kind == gsj.REQ_POINTS_INTERSECTS_BUFFERED:
+        return ctx.points_intersects_buffered(A, B, kw.pop("rb"), **kw)
     root, st = ctx.cluster(A)
     return gsj.Rows(np.zeros((0, 2), np.uint32), None, None, None, st, None, root)
 
