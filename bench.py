@@ -49,15 +49,16 @@ def small_class(hs):
 def serving_leg(gsj, ctx, args):
     """Served vs serial on a mixed stream of uneven requests (the batched-requests configuration: sizes lognormal in
     [1k, 2M] polygons of the config-1 generator on sqrt(s) x sqrt(s) cells, seed 3; 40% intersects + area and 20%
-    relate of parcels x regions, 20% dwithin self-joins with d ~ U(1, 50) m, 20% eroded intersects, which run as
-    intersects + area until that entry point exists). Every request arrives at t = 0; serial = one lane (one request
-    at a time), served = `lanes` lanes with continuous admission. Latency = submit -> finished."""
+    relate of parcels x regions, 20% dwithin self-joins with d ~ U(1, 50) m, 20% eroded intersects of parcels x
+    regions with ra, rb ~ U(0.5, 3) m). Every request arrives at t = 0; serial = one lane (one request at a time),
+    served = `lanes` lanes with continuous admission. Latency = submit -> finished."""
     rng = np.random.default_rng(3)
     n = args.requests
     mu, sig = np.log(np.sqrt(1e3 * args.smax)), np.log(args.smax / 1e3) / 6
     sizes = np.clip(np.exp(rng.normal(mu, sig, n)), 1e3, args.smax).astype(np.int64)
     kinds = rng.choice(4, size=n, p=[0.4, 0.2, 0.2, 0.2])
     dists = rng.uniform(1.0, 50.0, n)
+    radii = rng.uniform(0.5, 3.0, (n, 2))
     t0 = time.time()
     reqs, layers = [], []
     for i in range(n):
@@ -70,7 +71,10 @@ def serving_leg(gsj, ctx, args):
         else:
             R = gsj.generate("regions_grid", nx=side, ny=side, kx=4, ky=4, edge_segs=500, seed=3000 + i)
             LP, LR = ctx.polys(P, build_bvh=False), ctx.polys(R)
-            reqs.append((gsj.REQ_JOIN, LP, LR, dict(want_area=kinds[i] != 1, want_relate=kinds[i] == 1)))
+            if kinds[i] == 3:
+                reqs.append((gsj.REQ_INTERSECTS_BUFFERED, LP, LR, dict(ra=float(radii[i, 0]), rb=float(radii[i, 1]))))
+            else:
+                reqs.append((gsj.REQ_JOIN, LP, LR, dict(want_area=kinds[i] != 1, want_relate=kinds[i] == 1)))
             layers += [LP, LR]
         del P
     log(f"serving mix: {n} requests, {int(sizes.sum())} polygons, generated + resident in {time.time() - t0:.1f}s")
@@ -99,7 +103,7 @@ def serving_leg(gsj, ctx, args):
     served, cv = run(args.lanes)
     assert cs == cv, "served and serial request results differ"
     out = {"workload": f"batched requests: {n} requests, sizes lognormal in [1e3, {args.smax:.0e}] polygons (config-1 generator), seed 3",
-           "mix": "40% intersects+area, 20% relate, 20% dwithin d~U(1,50) m, 20% eroded intersects (run as intersects+area)",
+           "mix": "40% intersects+area, 20% relate, 20% dwithin d~U(1,50) m, 20% eroded intersects ra,rb~U(0.5,3) m",
            "arrivals": "all at t=0", "serial": serial, "served": served,
            "speedup_pairs_per_s": round(served["refined_pairs_per_s"] / serial["refined_pairs_per_s"], 3),
            "p99_ratio_served_over_serial": round(served["p99_ms"] / serial["p99_ms"], 3)}

@@ -1,0 +1,239 @@
+"""gpujoind end to end on the GPU: the daemon binary (on libgsj's C ABI and scheduler) speaking the reference's wire
+protocol. Every response is checked against the library's direct calls on the same distinct, subdivided layer, with
+the reference's row semantics restated here: dedup member expansion, self-join mirroring ("all") or ordering
+("distinct"), transposed DE-9IM for mirrored rows, point/polygon swap, dense cluster labels."""
+import math
+import os
+import subprocess
+import tempfile
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import gsj
+from gsj import client as G
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DAEMON = os.path.join(ROOT, "postgis-cuda_amd", "bin", "gpujoind")
+TOKEN = "s3cret"
+CELL = 0.6 / 1095
+
+
+def _dup_set(M, seed):
+    """M with some geometries repeated (identical rows) in shuffled order, and text ids"""
+    idx = np.concatenate([np.arange(M.n), np.arange(0, M.n, 7), np.arange(3, M.n, 13)])
+    np.random.default_rng(seed).shuffle(idx)
+    gro, rvo, xy = [0], [0], []
+    for g in idx:
+        for r in range(int(M.gro[g]), int(M.gro[g + 1])):
+            xy.append(M.xy[int(M.rvo[r]):int(M.rvo[r + 1])])
+            rvo.append(rvo[-1] + len(xy[-1]))
+        gro.append(len(rvo) - 1)
+    return gsj.HostSet(np.array(gro), np.array(rvo), np.concatenate(xy))
+
+
+@pytest.fixture(scope="module")
+def daemon():
+    d = tempfile.mkdtemp()
+    A = _dup_set(gsj.generate("parcels_mesh", nx=30, ny=30, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=21), 1)
+    B = _dup_set(gsj.generate("parcels_mesh", nx=24, ny=24, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=22), 2)
+    pts = gsj.generate_points(n=5000, x0=-95.2, y0=38.6, w=30 * CELL, h=30 * CELL, clusters=5, sigma_m=40.0, seed=3)
+    for name, hs in (("a", A), ("b", B)):
+        gsj.write_gset(os.path.join(d, name + ".gset"), hs)
+        with open(os.path.join(d, name + ".gset.ids"), "w") as f:
+            f.write("".join(f"{name}{i}\n" for i in range(hs.n)))
+    gsj.write_pset(os.path.join(d, "p.pset"), pts)
+    with open(os.path.join(d, "p.pset.ids"), "w") as f:
+        f.write("".join(f"pt{i}\n" for i in range(len(pts))))
+    with open(os.path.join(d, "token"), "w") as f:
+        f.write(TOKEN + "\n")
+    sock = os.path.join(d, "gpujoind.sock")
+    log = open(os.path.join(d, "daemon.log"), "w")
+    proc = subprocess.Popen([DAEMON, "--socket", sock, "--data-dir", d, "--token-file", os.path.join(d, "token"), "--lanes", "3", "--threads", "4"],
+                            stdout=log, stderr=log)
+    for _ in range(600):
+        if os.path.exists(sock) or proc.poll() is not None:
+            break
+        time.sleep(0.1)
+    assert proc.poll() is None, open(os.path.join(d, "daemon.log")).read()
+    c = G.Client(sock, token=TOKEN)
+    info = {"a": c.load("a", "gset:a.gset", subdivide=8), "b": c.load("b", "gset:b.gset", subdivide=0), "p": c.load("p", "pset:p.pset")}
+    yield {"client": c, "dir": d, "sets": {"a": A, "b": B}, "pts": pts, "info": info, "sock": sock}
+    proc.terminate()
+    proc.wait(timeout=60)
+    log.close()
+
+
+class Side:
+    """the daemon's view of a loaded polygon layer, rebuilt through the library: distinct set, members, pieces"""
+
+    def __init__(self, ctx, hs, name, subdiv):
+        U, moff, midx = gsj.dedup(hs)
+        self.moff, self.midx = moff, midx
+        self.ids = [f"{name}{i}" for i in range(hs.n)]
+        self.layer = ctx.polys(gsj.subdivide(U, subdiv) if subdiv else U)
+
+    def members(self, g):
+        return [int(m) for m in self.midx[int(self.moff[g]):int(self.moff[g + 1])]]
+
+
+def expand(A, B, pairs, vals, same, distinct, transpose=None):
+    """reference row semantics (daemon/gpujoind.cu:231-283, 365-379)"""
+    out = []
+    for (pa, pb), v in zip(pairs.tolist(), vals):
+        diag = same and pa == pb
+        for ra in A.members(pa):
+            for rb in B.members(pb):
+                if not same:
+                    out.append((A.ids[ra], B.ids[rb], v))
+                elif distinct:
+                    if ra == rb or (diag and ra > rb):
+                        continue
+                    if ra < rb:
+                        out.append((A.ids[ra], A.ids[rb], v))
+                    else:
+                        out.append((A.ids[rb], A.ids[ra], v if diag or transpose is None else transpose(v)))
+                else:
+                    out.append((A.ids[ra], A.ids[rb], v))
+                    if not diag:
+                        out.append((A.ids[rb], A.ids[ra], v if transpose is None else transpose(v)))
+    return out
+
+
+def _key(rows):
+    return sorted(rows, key=lambda r: (r[0], r[1], str(r[2])))
+
+
+def _same_rows(got, exp):
+    g, e = _key(got), _key(exp)
+    assert len(g) == len(e)
+    for x, y in zip(g, e):
+        assert x[:2] == y[:2]
+        if isinstance(y[2], float) and math.isnan(y[2]):
+            assert math.isnan(x[2])
+        else:
+            assert x[2] == y[2]
+
+
+def test_load_list_status_and_auth(daemon):
+    c = daemon["client"]
+    assert daemon["info"]["a"]["dedup"] and daemon["info"]["a"]["distinct"] < daemon["info"]["a"]["rows"] == daemon["sets"]["a"].n
+    assert daemon["info"]["p"]["kind"] == "point"
+    names = {L["layer"] for L in c.call(G.OP_LIST)}
+    assert names == {"a", "b", "p"}
+    st = c.call(G.OP_STATUS)
+    assert set(st["layers"]) == names and st["mem_total_mb"] > 0
+    with pytest.raises(G.DaemonError, match="authentication"):
+        G.Client(daemon["sock"], token="wrong").call(G.OP_STATUS)
+    with pytest.raises(G.DaemonError, match="no such layer"):
+        c.join("a", "zz")
+    with pytest.raises(G.DaemonError, match="self_pairs"):
+        c.join("a", "a", self_pairs="some")
+
+
+@pytest.mark.parametrize("self_pairs", ["all", "distinct"])
+def test_join_self_and_two_layers(ctx, daemon, self_pairs):
+    c = daemon["client"]
+    A = Side(ctx, daemon["sets"]["a"], "a", 8)
+    B = Side(ctx, daemon["sets"]["b"], "b", 0)
+    r = ctx.join(A.layer, A.layer, want_area=True, upper=True)
+    rows, st, _ = c.join("a", "a", want_area=True, self_pairs=self_pairs)
+    _same_rows(rows, expand(A, A, r.pairs, r.area.tolist(), True, self_pairs == "distinct"))
+    _, st2, _ = c.join("a", "a", want_area=True, self_pairs=self_pairs, stats=True)
+    # stats JSON numbers carry 6 significant digits, as the reference prints them
+    assert st2["rows"] == st["rows"] == len(rows) and abs(st2["area_sum"] - sum(x[2] for x in rows)) <= 1e-5 * abs(st2["area_sum"])
+    r = ctx.join(A.layer, B.layer)
+    rows, _, _ = c.join("a", "b", self_pairs=self_pairs)
+    _same_rows(rows, expand(A, B, r.pairs, [math.nan] * len(r.pairs), False, False))
+
+
+def test_relate_with_pattern_and_transposes(ctx, daemon):
+    c = daemon["client"]
+    A = Side(ctx, daemon["sets"]["a"], "a", 8)
+    r = ctx.join(A.layer, A.layer, want_relate=True, upper=True)
+    codes = [gsj.de9im_str(x) for x in r.code]
+    tr = {s: gsj.de9im_str(gsj.de9im_transpose(x)) for s, x in zip(codes, r.code)}
+    for sp in ("all", "distinct"):
+        rows, _, _ = c.relate("a", "a", "", sp)
+        _same_rows(rows, expand(A, A, r.pairs, codes, True, sp == "distinct", transpose=lambda s: tr[s]))
+    rows, _, _ = c.relate("a", "a", "T********", "all")
+    assert rows and all(x[2][0] != "F" for x in rows)
+
+
+def test_points_and_cluster(ctx, daemon):
+    c = daemon["client"]
+    B = Side(ctx, daemon["sets"]["b"], "b", 0)
+    pts = ctx.points(daemon["pts"])
+    pip = ctx.points_in_polygons(pts, B.layer)
+    exp = [(f"pt{p}", B.ids[m], math.nan) for (p, g) in pip.pairs.tolist() for m in B.members(g)]
+    rows, st, _ = c.join("p", "b")
+    _same_rows(rows, exp)
+    rows, _, _ = c.join("b", "p")   # polygon x point runs as point x polygon, rows in request order
+    _same_rows(rows, [(b, a, v) for a, b, v in exp])
+    rel, _, _ = c.relate("p", "b", "", "all")
+    assert {(a, b) for a, b, _ in rel} == {(a, b) for a, b, _ in exp} and {m for _, _, m in rel} <= {"0FFFFF212", "F0FFFF212"}
+    root, _ = ctx.cluster(B.layer)
+    rows, st, _ = c.cluster("b")
+    dense, lab = {}, {}
+    for g in range(len(B.moff) - 1):
+        for m in B.members(g):
+            lab[m] = int(root[g])
+    exp = []
+    for rr in range(len(B.ids)):
+        dense.setdefault(lab[rr], len(dense))
+        exp.append((B.ids[rr], dense[lab[rr]]))
+    assert rows == exp and st["clusters"] == len(set(dense.values()))
+
+
+def test_dwithin_modes(ctx, daemon):
+    c = daemon["client"]
+    A = Side(ctx, daemon["sets"]["a"], "a", 8)
+    r = ctx.dwithin(A.layer, A.layer, 4.0, same=True, report_distance=True, upper=True)
+    rows, st, _ = c.dwithin("a", "a", "dwithin", 4, 0, "all", extra="distance")
+    _same_rows(rows, expand(A, A, r.pairs, r.dist.tolist(), True, False))
+    rows, _, _ = c.dwithin("a", "a", "buffer", 1.5, 2.5, "distinct")   # two positive buffers: dwithin 4 m
+    _same_rows(rows, expand(A, A, r.pairs, [math.nan] * len(r.pairs), True, True))
+    need = (np.diff(A.moff) > 1).astype(np.uint8)
+    e = ctx.intersects_buffered(A.layer, A.layer, 1.0, 1.0, same=True, need_self=need, upper=True)
+    rows, st, _ = c.dwithin("a", "a", "buffer", -1, -1, "distinct")
+    _same_rows(rows, expand(A, A, e.pairs, [math.nan] * len(e.pairs), True, True))
+    assert st["kind"] == "buffer_intersects"
+    B = Side(ctx, daemon["sets"]["b"], "b", 0)
+    pts = ctx.points(daemon["pts"])
+    pe = ctx.points_intersects_buffered(pts, B.layer, 2.0)
+    rows, _, _ = c.dwithin("p", "b", "buffer", 0, -2, "all")
+    _same_rows(rows, [(f"pt{p}", B.ids[m], math.nan) for (p, g) in pe.pairs.tolist() for m in B.members(g)])
+    with pytest.raises(G.DaemonError, match="opposite sign"):
+        c.dwithin("a", "b", "buffer", 1, -1, "all")
+    with pytest.raises(G.DaemonError, match="negative buffer"):
+        c.dwithin("p", "b", "buffer", -1, -1, "all")
+
+
+def test_concurrent_clients_get_their_own_rows(daemon):
+    c = daemon["client"]
+    reqs = [("join", ("a", "a", True, "all")), ("join", ("a", "b", False, "all")), ("relate", ("a", "a", "", "distinct")),
+            ("dwithin", ("a", "a", "dwithin", 3, 0, "all")), ("join", ("p", "b", False, "all"))]
+    alone = [getattr(c, k)(*a)[0] for k, a in reqs]
+    out = [None] * (3 * len(reqs))
+
+    def run(i):
+        k, a = reqs[i % len(reqs)]
+        out[i] = getattr(c, k)(*a)[0]
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(out))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i, rows in enumerate(out):
+        _same_rows(rows, alone[i % len(reqs)])
+
+
+def test_drop(daemon):
+    c = daemon["client"]
+    assert c.load("tmp", "gen:parcels_grid:nx=10;ny=10;seed=1")["rows"] == 100
+    assert c.call(G.OP_DROP, "tmp") is True and c.call(G.OP_DROP, "tmp") is False
