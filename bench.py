@@ -46,6 +46,23 @@ def small_class(hs):
     return (verts <= 32) & (rings <= 8)
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size through the HIP runtime (its copy kernel), so that the HBM byte
+    counters of a profiling run can be scaled: FETCH_SIZE / WRITE_SIZE of these dispatches should read nbytes each."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {reps} device copies of {nbytes} bytes")
+
+
 def serving_leg(gsj, ctx, args):
     """Served vs serial on a mixed stream of uneven requests (the batched-requests configuration: sizes lognormal in
     [1k, 2M] polygons of the config-1 generator on sqrt(s) x sqrt(s) cells, seed 3; 40% intersects + area and 20%
@@ -124,6 +141,7 @@ def main():
     ap.add_argument("--requests", type=int, default=256, help="requests in the served-vs-serial mix")
     ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
     ap.add_argument("--lanes", type=int, default=4, help="lanes of the served path")
+    ap.add_argument("--calibrate-copy", action="store_true", help="also run 1 GiB device copies (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is described, not built)",
@@ -136,6 +154,8 @@ def main():
     M, S = mesh(gsj, args.scale)
     log(f"generated {M.n} parcels ({M.nv} vertices), {S.n} pieces after subdivide 256 in {time.time() - t0:.1f}s")
     ctx = gsj.Context(0)
+    if args.calibrate_copy:
+        copy_calibration()
     t0 = time.time()
     L = ctx.polys(S, build_bvh=False)
     t_up = time.time() - t0
