@@ -90,6 +90,12 @@ typedef struct {
   uint64_t max_pairs;      /* candidate pairs per refine launch; 0 = default 128M (JoinOpts::max_pairs) */
   int no_agg;              /* debug: report piece pairs instead of parent pairs (JoinOpts::no_agg) */
   const volatile int* cancel;   /* polled between batches (JoinOpts::cancel); may be NULL */
+  /* Query window (no counterpart in the reference; the windowed serving requests use it): with has_window set,
+   * only probe-side parents with a piece whose float box meets the closed rectangle window = {x0, y0, x1, y1}
+   * (layer coordinates) are probed, so the rows are exactly the unwindowed rows whose probe parent meets it.
+   * Joins, point joins and the distance / eroded joins honour it; gsj_cluster rejects it. */
+  int has_window;
+  double window[4];
 } gsj_join_opts;
 
 typedef struct {

@@ -510,7 +510,7 @@ void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool sa
     GSJ_LAUNCHED();
     qbb = e;
   }
-  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, PL.h_parent, RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, probes_of(PL, qbb), RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
     DwArgs A{PL.poly, RL.poly, pa, pb, d, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr};
     const PairClasses pc = partition_pairs(c, PL.poly, RL.poly, pa, pb, np);
@@ -583,8 +583,7 @@ void points_dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, 
     GSJ_LAUNCHED();
     qbb = e;
   }
-  static const std::vector<uint32_t> no_parent;
-  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, probes_of(PL, qbb), RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
     uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
     double* dist = report ? c.s_area.as<double>(np + 16) : nullptr;
@@ -627,8 +626,7 @@ void points_erode_run(Context& c, const Layer& PL, const Layer& RL, double r, co
     GSJ_LAUNCHED();
     qbb = e;
   }
-  static const std::vector<uint32_t> no_parent;
-  for_each_candidate_batch(c, qbb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, probes_of(PL, qbb), RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
     uint8_t* bits = c.s_flag.as<uint8_t>(np + 16);
     GSJ_HIP(hipEventRecord(c.ev[0], stream));

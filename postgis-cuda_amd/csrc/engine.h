@@ -26,7 +26,14 @@ struct DPolys {
   const double* blen = nullptr;      // [nparent] real boundary length per parent
   const uint32_t* pfirst = nullptr;  // [nparent+1] first piece of each parent (subdivided layers only)
   uint32_t nparent = 0;
+  // edge chunks of geometries with more than CHUNK edges: chunk k of geometry g covers the vertex pairs
+  // (i, i+1), i in [v0 + CHUNK k, v0 + CHUNK k + CHUNK), v0 its first vertex; cbb[cfirst[g] + k] bounds those
+  // vertices (origin-relative, rounded outward to float). cfirst[g] == NO_CHUNKS for smaller geometries.
+  const uint32_t* cfirst = nullptr;  // [n]
+  const float4* cbb = nullptr;       // [nchunks]
 };
+constexpr int CHUNK = 64;
+constexpr uint32_t NO_CHUNKS = 0xFFFFFFFFu;
 
 struct DPoints {
   uint32_t n = 0;
@@ -78,7 +85,8 @@ struct Context {
   std::string status;             // last status JSON
   // scratch, grow-only, reused across calls
   DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
-      s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc;
+      s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc, s_win, s_qidx,
+      s_win_off;
   // erosion pass 2 (it runs inside the join's on-device hook, beside the join's own scratch)
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;
@@ -110,13 +118,26 @@ void bvh_build(Layer& L);
 void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
               const uint64_t* offs, uint32_t* out_a, uint32_t* out_b);
 
-// Candidate pairs of the probe boxes qbb[0 .. nq) (probe index = box index) against tree t, in probe batches of at
-// most o.batch_q probes that keep a parent's pieces together (h_parent, may be empty) and sub-batches of at most
-// o.max_pairs candidates (engine/engine.cuh:903-937). fn(pa, pb, np) runs on every non-empty sub-batch with the pairs
-// on the device and the stream synchronised. Adds traverse_ms, batches and candidates to s; polls o.cancel.
+// The probe side of a candidate search: traversal boxes qbb[0 .. n) (probe index = box index), the probes' own boxes
+// (the window test; differ from qbb when the search box is widened), and the parent structure of a subdivided layer
+// (host h_parent keeps a parent's pieces in one batch; device pfirst lets a window take whole parents).
+struct Probes {
+  const float4* qbb = nullptr;
+  const float4* own_bb = nullptr;
+  uint32_t n = 0;
+  const std::vector<uint32_t>* h_parent = nullptr;
+  const uint32_t* pfirst = nullptr;
+  uint32_t nparent = 0;
+};
+Probes probes_of(const Layer& L, const float4* qbb = nullptr);
+
+// Candidate pairs of the probes against tree t, in probe batches of at most o.batch_q probes that keep a parent's
+// pieces together and sub-batches of at most o.max_pairs candidates (engine/engine.cuh:903-937); with o.has_window
+// only the probes of parents meeting the window. fn(pa, pb, np) runs on every non-empty sub-batch with the pairs on
+// the device and the stream synchronised. Adds traverse_ms, batches and candidates to s; polls o.cancel.
 using PairsFn = std::function<void(uint32_t* pa, uint32_t* pb, uint64_t np)>;
-void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nq, const std::vector<uint32_t>& h_parent, const DBvh& t, bool upper,
-                              const gsj_join_opts& o, gsj_join_stats& s, const PairsFn& fn);
+void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
+                              const PairsFn& fn);
 
 // ------------------------------------------------------------------------------------------- refinement
 struct RefineParams {   // engine/engine.cuh:517-518

@@ -435,78 +435,106 @@ HostSet gen_parcels_mesh(const Spec& sp) {
     const uint64_t id = horiz ? ((uint64_t)j * nx + i) * 2 : ((uint64_t)j * (nx + 1) + i) * 2 + 1;
     return unit * (0.01 + 0.04 * r_side.uni(2 * id + 1000000007ull));
   };
-  Builder b;
-  std::vector<int> per_side;
-  std::vector<double> ts;
-  for (int j = 0; j < ny; j++)
-    for (int i = 0; i < nx; i++) {
-      const uint64_t id = (uint64_t)j * nx + i;
-      const uint64_t pbase = id;
-      double cx[4], cy[4];
-      lattice(i, j, cx[0], cy[0]); lattice(i + 1, j, cx[1], cy[1]); lattice(i + 1, j + 1, cx[2], cy[2]); lattice(i, j + 1, cx[3], cy[3]);
-      // sides in CCW order: bottom, right, top, left; this cell is the lower/left neighbour of its top/right sides
-      const int st[4] = {side_state(true, i, j), side_state(false, i + 1, j), side_state(true, i, j + 1), side_state(false, i, j)};
-      const bool lower_of[4] = {false, true, true, false};
-      int kind[4];   // 0 gap, 1 straight exact, 2 bulge (this cell owns the overlap)
-      double bdepth[4] = {0, 0, 0, 0};
-      for (int s = 0; s < 4; s++) {
-        if (st[s] == GAP) kind[s] = 0;
-        else if (st[s] == SHARED) kind[s] = 1;
-        else {
-          const bool owner = (st[s] == OVER_LO) == lower_of[s];
-          kind[s] = owner ? 2 : 1;
-          if (owner) {
-            const int si = s == 1 ? i + 1 : i, sj = s == 2 ? j + 1 : j;
-            bdepth[s] = side_depth(s == 0 || s == 2, si, sj);
-          }
+  // per cell: side kinds, bulge depths, corners and the vertex budget (everything the vertex count depends on)
+  struct Plan {
+    int kind[4];   // 0 gap, 1 straight exact, 2 bulge (this cell owns the overlap)
+    double bdepth[4], cx[4], cy[4], px[4], py[4];
+    long extra;
+    int ngap, nverts;   // nverts: stored vertices including the closing one
+  };
+  auto plan = [&](int i, int j, Plan& P) {
+    lattice(i, j, P.cx[0], P.cy[0]); lattice(i + 1, j, P.cx[1], P.cy[1]); lattice(i + 1, j + 1, P.cx[2], P.cy[2]); lattice(i, j + 1, P.cx[3], P.cy[3]);
+    // sides in CCW order: bottom, right, top, left; this cell is the lower/left neighbour of its top/right sides
+    const int st[4] = {side_state(true, i, j), side_state(false, i + 1, j), side_state(true, i, j + 1), side_state(false, i, j)};
+    const bool lower_of[4] = {false, true, true, false};
+    for (int s = 0; s < 4; s++) {
+      P.bdepth[s] = 0;
+      if (st[s] == GAP) P.kind[s] = 0;
+      else if (st[s] == SHARED) P.kind[s] = 1;
+      else {
+        const bool owner = (st[s] == OVER_LO) == lower_of[s];
+        P.kind[s] = owner ? 2 : 1;
+        if (owner) {
+          const int si = s == 1 ? i + 1 : i, sj = s == 2 ? j + 1 : j;
+          P.bdepth[s] = side_depth(s == 0 || s == 2, si, sj);
         }
       }
-      const double mcx = 0.25 * (cx[0] + cx[1] + cx[2] + cx[3]), mcy = 0.25 * (cy[0] + cy[1] + cy[2] + cy[3]);
-      double px[4], py[4];   // polygon corners
-      for (int k = 0; k < 4; k++) {
-        const bool exact = kind[(k + 3) & 3] != 0 || kind[k] != 0;   // sides before and after corner k
-        px[k] = exact ? cx[k] : cx[k] + 0.06 * (mcx - cx[k]);
-        py[k] = exact ? cy[k] : cy[k] + 0.06 * (mcy - cy[k]);
-      }
-      // vertex budget
-      const double z = r_poly.normal(pbase);
-      long kv = std::lround(std::exp(std::log(med) + sigma * z));
-      kv = std::max<long>(kmin, std::min<long>(kmax, kv));
-      int base_v = 4, ngap = 0;
-      for (int s = 0; s < 4; s++) { if (kind[s] == 2) base_v += 2; if (kind[s] == 0) ngap++; }
-      long extra = std::max<long>(0, kv - base_v);
-      per_side.assign(4, 0);
-      if (ngap > 0)
-        for (long e = 0; e < extra; e++) {
-          int pick = (int)(r_pick.uni(((uint64_t)id << 21) + (uint64_t)e) * ngap);
-          for (int s = 0; s < 4; s++) if (kind[s] == 0 && pick-- == 0) { per_side[s]++; break; }
+    }
+    const double mcx = 0.25 * (P.cx[0] + P.cx[1] + P.cx[2] + P.cx[3]), mcy = 0.25 * (P.cy[0] + P.cy[1] + P.cy[2] + P.cy[3]);
+    for (int k = 0; k < 4; k++) {   // polygon corners
+      const bool exact = P.kind[(k + 3) & 3] != 0 || P.kind[k] != 0;   // sides before and after corner k
+      P.px[k] = exact ? P.cx[k] : P.cx[k] + 0.06 * (mcx - P.cx[k]);
+      P.py[k] = exact ? P.cy[k] : P.cy[k] + 0.06 * (mcy - P.cy[k]);
+    }
+    const uint64_t id = (uint64_t)j * nx + i;
+    const double z = r_poly.normal(id);
+    long kv = std::lround(std::exp(std::log(med) + sigma * z));
+    kv = std::max<long>(kmin, std::min<long>(kmax, kv));
+    int base_v = 4;
+    P.ngap = 0;
+    for (int s = 0; s < 4; s++) { if (P.kind[s] == 2) base_v += 2; if (P.kind[s] == 0) P.ngap++; }
+    P.extra = std::max<long>(0, kv - base_v);
+    P.nverts = base_v + (P.ngap > 0 ? (int)P.extra : 0) + 1;
+  };
+  // two passes over the cells (vertex counts, then vertices at their offsets); every cell draws from its own
+  // counter ranges, so the result does not depend on the number of threads
+  const uint64_t n = (uint64_t)nx * ny;
+  HostSet out;
+  out.gro.resize(n + 1);
+  out.rvo.resize(n + 1);
+  std::vector<int> nvc(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t id = 0; id < (int64_t)n; id++) {
+    Plan P;
+    plan((int)(id % nx), (int)(id / nx), P);
+    nvc[id] = P.nverts;
+  }
+  out.gro[0] = 0; out.rvo[0] = 0;
+  for (uint64_t id = 0; id < n; id++) { out.gro[id + 1] = id + 1; out.rvo[id + 1] = out.rvo[id] + nvc[id]; }
+  std::vector<int>().swap(nvc);
+  out.xy.resize(2 * out.rvo[n]);
+#pragma omp parallel
+  {
+    int per_side[4];
+    std::vector<double> ts;
+#pragma omp for schedule(static)
+    for (int64_t id = 0; id < (int64_t)n; id++) {
+      Plan P;
+      plan((int)(id % nx), (int)(id / nx), P);
+      double* o = &out.xy[2 * out.rvo[id]];
+      int nv = 0;
+      auto vertex = [&](double x, double y) { o[2 * nv] = x; o[2 * nv + 1] = y; nv++; };
+      for (int s = 0; s < 4; s++) per_side[s] = 0;
+      if (P.ngap > 0)
+        for (long e = 0; e < P.extra; e++) {
+          int pick = (int)(r_pick.uni(((uint64_t)id << 21) + (uint64_t)e) * P.ngap);
+          for (int s = 0; s < 4; s++) if (P.kind[s] == 0 && pick-- == 0) { per_side[s]++; break; }
         }
-      const size_t first = b.s.xy.size() / 2;
       uint64_t ctr = (uint64_t)id << 21;   // per-polygon counters for vertex positions
       for (int s = 0; s < 4; s++) {
         const int e = (s + 1) & 3;
-        b.vertex(px[s], py[s]);
+        vertex(P.px[s], P.py[s]);
         // side direction on the lattice and its inward (left) unit normal, scaled to the cell
-        const double dx = cx[e] - cx[s], dy = cy[e] - cy[s], len = std::sqrt(dx * dx + dy * dy);
+        const double dx = P.cx[e] - P.cx[s], dy = P.cy[e] - P.cy[s], len = std::sqrt(dx * dx + dy * dy);
         const double nxu = -dy / len, nyu = dx / len;
-        if (kind[s] == 2) {
+        if (P.kind[s] == 2) {
           const double t0 = 0.3, t1 = 0.7;
-          b.vertex(cx[s] + t0 * dx - bdepth[s] * nxu, cy[s] + t0 * dy - bdepth[s] * nyu);
-          b.vertex(cx[s] + t1 * dx - bdepth[s] * nxu, cy[s] + t1 * dy - bdepth[s] * nyu);
-        } else if (kind[s] == 0 && per_side[s] > 0) {
+          vertex(P.cx[s] + t0 * dx - P.bdepth[s] * nxu, P.cy[s] + t0 * dy - P.bdepth[s] * nyu);
+          vertex(P.cx[s] + t1 * dx - P.bdepth[s] * nxu, P.cy[s] + t1 * dy - P.bdepth[s] * nyu);
+        } else if (P.kind[s] == 0 && per_side[s] > 0) {
           ts.resize(per_side[s]);
           for (int k = 0; k < per_side[s]; k++) ts[k] = 0.1 + 0.8 * r_pos.uni(ctr++);
           std::sort(ts.begin(), ts.end());
           for (int k = 0; k < per_side[s]; k++) {
             const double d = unit * (0.005 + 0.015 * r_pos.uni(ctr++));
-            b.vertex(cx[s] + ts[k] * dx + d * nxu, cy[s] + ts[k] * dy + d * nyu);
+            vertex(P.cx[s] + ts[k] * dx + d * nxu, P.cy[s] + ts[k] * dy + d * nyu);
           }
         }
       }
-      b.close_ring(first);
-      b.end_geometry();
+      vertex(o[0], o[1]);   // closing vertex
     }
-  return std::move(b.s);
+  }
+  return out;
 }
 
 }  // namespace

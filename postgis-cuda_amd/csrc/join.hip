@@ -179,7 +179,7 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
   rp.want_area = o.want_area ? 1 : 0;
   rp.want_relate = o.want_relate ? 1 : 0;
   rp.snap_half_ulp = o.snap_half_ulp ? 1 : 0;
-  for_each_candidate_batch(c, PL.poly.bb, (uint32_t)PL.n, PL.h_parent, RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, probes_of(PL), RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     if (hooks && hooks->prefilter) {
       WallTimer tp;
       np = hooks->prefilter(pa, pb, np);

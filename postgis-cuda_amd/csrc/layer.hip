@@ -77,6 +77,22 @@ __global__ void k_blen(uint32_t nparent, const uint32_t* pf, const double* glen,
   blen[p] = s;
 }
 
+// outward float boxes of the edge chunks (after the vertices were shifted to their origin); thread per geometry
+__global__ void k_chunk_boxes(uint32_t n, const uint32_t* gro, const uint64_t* rvo, const double2* xy, const uint32_t* cfirst, float4* cbb) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n || cfirst[g] == NO_CHUNKS) return;
+  const uint64_t v0 = rvo[gro[g]], v1 = rvo[gro[g + 1]];
+  for (uint64_t k = 0; v0 + k * CHUNK + 1 < v1; k++) {
+    const uint64_t a = v0 + k * CHUNK, b = min(a + CHUNK, v1 - 1);   // vertices a..b
+    double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+    for (uint64_t v = a; v <= b; v++) {
+      const double2 p = xy[v];
+      x0 = fmin(x0, p.x); x1 = fmax(x1, p.x); y0 = fmin(y0, p.y); y1 = fmax(y1, p.y);
+    }
+    cbb[cfirst[g] + k] = make_float4(__double2float_rd(x0), __double2float_rd(y0), __double2float_ru(x1), __double2float_ru(y1));
+  }
+}
+
 __global__ void k_point_prep(uint32_t n, const double2* xy, float4* bb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -108,6 +124,19 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
     L.h_parent.assign(parent, parent + n);
   }
   L.nparent = nparent;
+  // edge chunks of the geometries with more than CHUNK edges (vertex pairs)
+  std::vector<uint32_t> hcf(n);
+  uint64_t nchunks = 0;
+  for (uint64_t g = 0; g < n; g++) {
+    const uint64_t nvg = rvo[gro[g + 1]] - rvo[gro[g]];
+    if (nvg > CHUNK + 1) {
+      hcf[g] = (uint32_t)nchunks;
+      nchunks += (nvg - 1 + CHUNK - 1) / CHUNK;
+    } else {
+      hcf[g] = NO_CHUNKS;
+    }
+  }
+  GSJ_REQUIRE(nchunks < NO_CHUNKS, GSJ_E_LIMIT, "layer too large: edge chunks >= 2^32");
   // sizes (pass 1) and assignment (pass 2)
   auto carve = [&](Slicer& s, DPolys& d, uint32_t*& gro32, uint8_t*& syn8, double*& glen) {
     d.origin = s.take<double2>(n); d.ext = s.take<double2>(n); gro32 = s.take<uint32_t>(n + 1); d.rvo = s.take<uint64_t>(nr + 1);
@@ -118,6 +147,8 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
     d.pfirst = parent ? s.take<uint32_t>((uint64_t)nparent + 1) : nullptr;
     syn8 = syn ? s.take<uint8_t>(nv) : nullptr;
     glen = parent ? s.take<double>(n) : nullptr;
+    d.cfirst = nchunks ? s.take<uint32_t>(n) : nullptr;
+    d.cbb = nchunks ? s.take<float4>(nchunks) : nullptr;
   };
   Slicer s1; DPolys d; uint32_t* gro32; uint8_t* syn8; double* glen;
   carve(s1, d, gro32, syn8, glen);
@@ -133,6 +164,7 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
   if (nv) GSJ_HIP(hipMemcpyAsync((void*)d.xy, xy, nv * 16, hipMemcpyHostToDevice, st));
   if (parent) GSJ_HIP(hipMemcpyAsync((void*)d.parent, parent, n * 4, hipMemcpyHostToDevice, st));
   if (syn) GSJ_HIP(hipMemcpyAsync(syn8, syn, nv, hipMemcpyHostToDevice, st));
+  if (nchunks) GSJ_HIP(hipMemcpyAsync((void*)d.cfirst, hcf.data(), n * 4, hipMemcpyHostToDevice, st));
   if (n) {
     // without parents the boundary length of a geometry is its parent's
     double* glen_out = parent ? glen : const_cast<double*>(d.blen);
@@ -140,6 +172,10 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
                        const_cast<double2*>(d.origin), const_cast<double2*>(d.ext), const_cast<float4*>(d.bb), const_cast<uint32_t*>(d.ginfo),
                        glen_out);
     GSJ_LAUNCHED();
+    if (nchunks) {
+      hipLaunchKernelGGL(k_chunk_boxes, dim3(div_up(n, 256)), dim3(256), 0, st, (uint32_t)n, gro32, d.rvo, d.xy, d.cfirst, const_cast<float4*>(d.cbb));
+      GSJ_LAUNCHED();
+    }
     if (syn && nv) {
       hipLaunchKernelGGL(k_synb, dim3(div_up((nv + 31) / 32, 256)), dim3(256), 0, st, nv, syn8, const_cast<uint32_t*>(d.synb));
       GSJ_LAUNCHED();

@@ -61,6 +61,31 @@ __device__ bool inside_from_c(const Src& L, double cx, double cy, bool in_c, dou
   return in;
 }
 
+// ------------------------------------------------------------------------------------------- edge chunks
+// Vertex pairs (i, i+1) with i in [a, b) of geometry g (b - a <= CHUNK) all lie in the union of at most two chunk
+// boxes; true when both boxes, shifted by (shx, shy), miss the box o. Rounding is monotonic, so a chunk box that
+// misses o in floating point holds only vertices whose shifted coordinates miss it too: every edge it covers is one
+// the caller would reject, and skipping it changes no result.
+__device__ __forceinline__ bool chunk_misses(const float4 c, double shx, double shy, const Box& o) {
+  return shx + (double)c.z < o.x0 || shx + (double)c.x > o.x1 || shy + (double)c.w < o.y0 || shy + (double)c.y > o.y1;
+}
+__device__ __forceinline__ bool chunks_miss(const DPolys& S, uint32_t g, uint64_t v0, uint64_t a, uint64_t b, double shx, double shy, const Box& o) {
+  if (!S.cfirst) return false;
+  const uint32_t cf = S.cfirst[g];
+  if (cf == NO_CHUNKS) return false;
+  const uint64_t k0 = (a - v0) / CHUNK, k1 = (b - 1 - v0) / CHUNK;
+  return chunk_misses(S.cbb[cf + k0], shx, shy, o) && (k1 == k0 || chunk_misses(S.cbb[cf + k1], shx, shy, o));
+}
+// all vertices of pairs i in [a, b) strictly above or strictly below the horizontal line y = my (shifted by shy)
+__device__ __forceinline__ bool chunks_off_row(const DPolys& S, uint32_t g, uint64_t v0, uint64_t a, uint64_t b, double shy, double my) {
+  if (!S.cfirst) return false;
+  const uint32_t cf = S.cfirst[g];
+  if (cf == NO_CHUNKS) return false;
+  const uint64_t k0 = (a - v0) / CHUNK, k1 = (b - 1 - v0) / CHUNK;
+  auto off = [&](const float4 c) { return shy + (double)c.y > my || shy + (double)c.w < my; };
+  return off(S.cbb[cf + k0]) && (k1 == k0 || off(S.cbb[cf + k1]));
+}
+
 // ------------------------------------------------------------------------------------------- whole geometries
 // edges of geometry g meeting box ov, in ring order, compacted by group ballot (engine/engine.cuh:358-377);
 // returns the full count (may exceed cap: then the list is incomplete and the pair overflows)
@@ -68,9 +93,11 @@ template <int G>
 __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, Ed* out, int cap, int gl) {
   int cnt = 0;
   const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
+  const uint64_t v0 = S.rvo[r0];
   for (uint32_t r = r0; r < r1; r++) {
     const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
     for (uint64_t base = s; base + 1 < e; base += G) {
+      if (chunks_miss(S, g, v0, base, min(base + G, e - 1), shx, shy, ov)) continue;   // group-uniform
       const uint64_t i = base + gl;
       bool v = false;
       Ed ed;
@@ -96,9 +123,13 @@ __device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, do
   const double ox = shx - 1.0, oy = my;
   bool par = false;
   const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
+  const uint64_t v0 = S.rvo[r0];
   for (uint32_t r = r0; r < r1; r++) {
     const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
-    for (uint64_t j = s + 1 + gl; j < e; j += G) {
+    for (uint64_t jb = s + 1; jb < e; jb += G) {
+      if (chunks_off_row(S, g, v0, jb - 1, min(jb - 1 + G, e - 1), shy, my)) continue;   // group-uniform
+      const uint64_t j = jb + gl;
+      if (j >= e) continue;
       const double2 p = S.xy[j - 1], q = S.xy[j];
       const double x0 = shx + p.x, y0 = shy + p.y, x1 = shx + q.x, y1 = shy + q.y;
       if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;

@@ -21,21 +21,35 @@ __global__ void __launch_bounds__(256) k_pip(const DPoints pts, const DPolys R, 
   const double ox = (o.x - o.x) - 1.0;
   bool par = false, on = false;
   const uint32_t r1 = R.gro[g + 1];
+  const uint64_t vg = R.rvo[R.gro[g]];
+  const uint32_t cf = R.cfirst ? R.cfirst[g] : NO_CHUNKS;
   for (uint32_t r = R.gro[g]; r < r1 && !on; r++) {
     const uint64_t s0 = R.rvo[r], e = R.rvo[r + 1];
-    double2 v0 = R.xy[s0];
-    for (uint64_t j = s0 + 1; j < e; j++) {
-      const double2 v1 = R.xy[j];
-      const double x0 = v0.x, y0 = v0.y, x1 = v1.x, y1 = v1.y;
-      if (!((y0 > py) == (y1 > py) && y0 != py && y1 != py)) {
-        if (orient(x0, y0, x1, y1, px, py) == 0 && !syn_bit(R.synb, j - 1) && onseg(x0, y0, x1, y1, px, py)) { on = true; break; }
-        const bool so = left_of(x0, y0, x1, y1, ox, py), sp = left_of(x0, y0, x1, y1, px, py);
-        if (so != sp) {
-          const bool su = orient(ox, py, px, py, x0, y0) > 0, sv = orient(ox, py, px, py, x1, y1) > 0;
-          if (su != sv) par = !par;
-        }
+    // edges (j-1, j) taken an edge chunk at a time; a chunk whose vertices all lie strictly above or strictly below
+    // the row y = py holds only edges the parity test skips
+    for (uint64_t a = s0; a + 1 < e && !on;) {
+      uint64_t b = e - 1;
+      if (cf != NO_CHUNKS) {
+        const uint64_t k = (a - vg) / CHUNK;
+        b = min(b, vg + (k + 1) * CHUNK);
+        const float4 c = R.cbb[cf + k];
+        if ((double)c.y > py || (double)c.w < py) { a = b; continue; }
       }
-      v0 = v1;
+      double2 v0 = R.xy[a];
+      for (uint64_t j = a + 1; j <= b; j++) {
+        const double2 v1 = R.xy[j];
+        const double x0 = v0.x, y0 = v0.y, x1 = v1.x, y1 = v1.y;
+        if (!((y0 > py) == (y1 > py) && y0 != py && y1 != py)) {
+          if (orient(x0, y0, x1, y1, px, py) == 0 && !syn_bit(R.synb, j - 1) && onseg(x0, y0, x1, y1, px, py)) { on = true; break; }
+          const bool so = left_of(x0, y0, x1, y1, ox, py), sp = left_of(x0, y0, x1, y1, px, py);
+          if (so != sp) {
+            const bool su = orient(ox, py, px, py, x0, y0) > 0, sv = orient(ox, py, px, py, x1, y1) > 0;
+            if (su != sv) par = !par;
+          }
+        }
+        v0 = v1;
+      }
+      a = b;
     }
   }
   hit[i] = on ? 2 : (par ? 1 : 0);
@@ -122,8 +136,7 @@ void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_joi
   gsj_join_stats s{};
   if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
   hipStream_t stream = c.stream;
-  static const std::vector<uint32_t> no_parent;
-  for_each_candidate_batch(c, PL.pts.bb, (uint32_t)PL.n, no_parent, RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+  for_each_candidate_batch(c, probes_of(PL), RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
       WallTimer t2;
       uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
       hipLaunchKernelGGL(k_pip, dim3(div_up(np, 256)), dim3(256), 0, stream, PL.pts, RL.poly, pa, pb, np, hit);
@@ -184,6 +197,7 @@ void cluster_run(Context& c, const Layer& PL, const gsj_join_opts& o, uint32_t* 
   uint32_t* droot = c.s_root.as<uint32_t>((uint64_t)n + 16);
   if (n) hipLaunchKernelGGL(k_uf_init, dim3(div_up(n, 256)), dim3(256), 0, stream, parent, n);
   GSJ_LAUNCHED();
+  GSJ_REQUIRE(!o.has_window, GSJ_E_ARG, "cluster takes no query window");
   gsj_join_opts jo = o;
   jo.want_area = 0; jo.want_relate = 0; jo.upper = 1;   // intersects is symmetric: each unordered pair once
   JoinHooks hooks;

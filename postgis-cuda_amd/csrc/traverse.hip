@@ -46,7 +46,45 @@ __global__ void __launch_bounds__(256) k_traverse(const float4* __restrict__ qbb
   if (!WRITE) counts[t] = cnt;
 }
 
+// window membership of every probe: its own (outward float) box meets the closed window, compared in double
+__global__ void k_win_flags(const float4* __restrict__ bb, uint32_t n, double x0, double y0, double x1, double y1, uint32_t* __restrict__ f) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 b = bb[i];
+  f[i] = ((double)b.x <= x1 && (double)b.z >= x0 && (double)b.y <= y1 && (double)b.w >= y0) ? 1u : 0u;
+}
+
+// a parent is in when any of its pieces is; then all its pieces are (thread per parent, pieces pfirst[p] .. pfirst[p+1])
+__global__ void k_win_parents(const uint32_t* __restrict__ pfirst, uint32_t nparent, uint32_t* __restrict__ f) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nparent) return;
+  const uint32_t a = pfirst[p], b = pfirst[p + 1];
+  if (b - a < 2) return;
+  uint32_t any = 0;
+  for (uint32_t i = a; i < b; i++) any |= f[i];
+  if (any)
+    for (uint32_t i = a; i < b; i++) f[i] = 1u;
+}
+
+__global__ void k_win_scatter(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && f[i]) out[off[i]] = i;
+}
+
 }  // namespace
+
+Probes probes_of(const Layer& L, const float4* qbb) {
+  Probes p;
+  p.own_bb = L.points ? L.pts.bb : L.poly.bb;
+  p.qbb = qbb ? qbb : p.own_bb;
+  p.n = (uint32_t)L.n;
+  if (!L.points && !L.h_parent.empty()) {
+    p.h_parent = &L.h_parent;
+    p.pfirst = L.poly.pfirst;
+    p.nparent = L.nparent;
+  }
+  return p;
+}
 
 void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
               const uint64_t* offs, uint32_t* out_a, uint32_t* out_b) {
@@ -64,20 +102,51 @@ void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbas
   GSJ_LAUNCHED();
 }
 
-void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nQ, const std::vector<uint32_t>& hp, const DBvh& t, bool upper,
-                              const gsj_join_opts& o, gsj_join_stats& s, const PairsFn& fn) {
+void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
+                              const PairsFn& fn) {
   hipStream_t stream = c.stream;
   const uint32_t batch_q = o.batch_q ? o.batch_q : (1u << 22);
   const uint64_t max_pairs = o.max_pairs ? o.max_pairs : (128ull << 20);
+  const float4* qbb = pr.qbb;
+  static const std::vector<uint32_t> none;
+  const std::vector<uint32_t>& hp = pr.h_parent ? *pr.h_parent : none;
+  // probe positions 0 .. nQ: probe ids directly, or through the window's member list (ascending ids)
+  uint32_t nQ = pr.n;
+  const uint32_t* qidx = nullptr;
+  std::vector<uint32_t> hq;
+  if (o.has_window && pr.n) {
+    const double* w = o.window;
+    GSJ_REQUIRE(w[0] <= w[2] && w[1] <= w[3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
+    WallTimer tw;
+    uint32_t* f = c.s_win.as<uint32_t>((uint64_t)pr.n + 16);
+    hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, pr.n, w[0], w[1], w[2], w[3], f);
+    GSJ_LAUNCHED();
+    if (pr.pfirst && pr.nparent) {
+      hipLaunchKernelGGL(k_win_parents, dim3(div_up(pr.nparent, 256)), dim3(256), 0, stream, pr.pfirst, pr.nparent, f);
+      GSJ_LAUNCHED();
+    }
+    uint64_t* off = c.s_win_off.as<uint64_t>((uint64_t)pr.n + 16);
+    nQ = (uint32_t)scan_u32_to_u64(c, f, off, pr.n);
+    // members written behind the flags (the flags are read by the scatter itself, so they need their own region)
+    uint32_t* list = c.s_qidx.as<uint32_t>((uint64_t)nQ + 16);
+    hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, pr.n, list);
+    GSJ_LAUNCHED();
+    hq.resize(nQ);
+    if (nQ) GSJ_HIP(hipMemcpyAsync(hq.data(), list, (size_t)nQ * 4, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    qidx = list;
+    s.traverse_ms += tw.ms();
+  }
+  auto pid = [&](uint32_t pos) { return hp[qidx ? hq[pos] : pos]; };   // parent of the probe at a position
   std::vector<uint32_t> hcounts;
   for (uint32_t q0 = 0, qn; q0 < nQ; q0 = qn) {
     qn = (uint32_t)std::min<uint64_t>((uint64_t)q0 + batch_q, nQ);
-    if (!hp.empty()) while (qn < nQ && hp[qn] == hp[qn - 1]) qn++;
+    if (!hp.empty()) while (qn < nQ && pid(qn) == pid(qn - 1)) qn++;
     if (o.cancel && *o.cancel) throw Error(GSJ_E_CANCEL, "cancelled");
     const uint32_t nq = qn - q0;
     WallTimer tt;
     uint32_t* counts = c.s_counts.as<uint32_t>(nq + 16);
-    traverse(c, qbb, nullptr, q0, nq, t, upper, counts, nullptr, nullptr, nullptr);
+    traverse(c, qbb, qidx, q0, nq, t, upper, counts, nullptr, nullptr, nullptr);
     uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);   // probe offsets (s_offs belongs to compaction)
     const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);
     s.traverse_ms += tt.ms();
@@ -91,7 +160,7 @@ void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nQ, const 
       for (uint32_t s0 = 0; s0 < nq;) {
         uint64_t np = 0;
         uint32_t s1 = s0;
-        while (s1 < nq && (np + hcounts[s1] <= max_pairs || s1 == s0 || (!hp.empty() && hp[q0 + s1] == hp[q0 + s1 - 1]))) np += hcounts[s1++];
+        while (s1 < nq && (np + hcounts[s1] <= max_pairs || s1 == s0 || (!hp.empty() && pid(q0 + s1) == pid(q0 + s1 - 1)))) np += hcounts[s1++];
         GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a probe batch has >= 2^32 candidates (lower max_pairs)");
         subs.push_back({s0, s1});
         s0 = s1;
@@ -108,7 +177,7 @@ void for_each_candidate_batch(Context& c, const float4* qbb, uint32_t nQ, const 
       GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a sub-batch has >= 2^32 candidates (lower max_pairs)");
       uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
       uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
-      traverse(c, qbb, nullptr, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb);
+      traverse(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb);
       GSJ_HIP(hipStreamSynchronize(stream));
       s.traverse_ms += t1.ms();
       fn(pa, pb, np);

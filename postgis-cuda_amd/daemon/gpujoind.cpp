@@ -446,10 +446,51 @@ bool self_pairs_distinct(const std::string& s) {
   return s == "distinct";
 }
 
+double to_double(const std::string& s) {
+  char* e;
+  const double v = strtod(s.c_str(), &e);
+  if (s.empty() || *e || !std::isfinite(v)) throw std::runtime_error("not a number: " + s);
+  return v;
+}
+
+// Optional trailing arguments after an op's own (additive: the unchanged extension never sends them): "stats"
+// (stats-only counting), "distance" (DWITHIN: the geography distance in the f64 slot), "window=x0,y0,x1,y1" (probe
+// only the first layer's geometries, or the points of a point join, whose box meets the rectangle; a self-join keeps
+// the mirrored rows of the pairs found).
+struct Trailing {
+  bool stats = false, distance = false, has_window = false;
+  double w[4] = {0, 0, 0, 0};
+  void apply(gsj_join_opts& o) const {
+    o.has_window = has_window;
+    for (int k = 0; k < 4; k++) o.window[k] = w[k];
+  }
+};
+Trailing trailing(const std::vector<std::string>& a, size_t from, bool allow_distance) {
+  Trailing t;
+  for (size_t i = from; i < a.size(); i++) {
+    if (a[i] == "stats") t.stats = true;
+    else if (a[i] == "distance" && allow_distance) t.distance = true;
+    else if (a[i].compare(0, 7, "window=") == 0) {
+      std::vector<std::string> f;
+      size_t p = 7;
+      for (size_t q; (q = a[i].find(',', p)) != std::string::npos; p = q + 1) f.push_back(a[i].substr(p, q - p));
+      f.push_back(a[i].substr(p));
+      if (f.size() != 4) throw std::runtime_error("window needs x0,y0,x1,y1");
+      for (int k = 0; k < 4; k++) t.w[k] = to_double(f[k]);
+      if (!(t.w[0] <= t.w[2] && t.w[1] <= t.w[3])) throw std::runtime_error("window needs x0 <= x1 and y0 <= y1");
+      t.has_window = true;
+    } else {
+      throw std::runtime_error("unknown option: " + a[i]);
+    }
+  }
+  return t;
+}
+
 // JOIN (reference op_join, daemon/gpujoind.cu:210-307)
 void op_join(Conn& c, const std::vector<std::string>& a) {
-  if (a.size() < 4) throw std::runtime_error("JOIN needs layer_a, layer_b, want_area, self_pairs[, stats]");
-  const bool stats_only = a.size() > 4 && a[4] == "stats";
+  if (a.size() < 4) throw std::runtime_error("JOIN needs layer_a, layer_b, want_area, self_pairs[, stats][, window=x0,y0,x1,y1]");
+  const Trailing tr = trailing(a, 4, false);
+  const bool stats_only = tr.stats;
   LayerPair lp = layer_pair(a[0], a[1], "joins");
   const bool want_area = a[2] == "1" || a[2] == "t" || a[2] == "true";
   const bool distinct = self_pairs_distinct(a[3]);
@@ -464,6 +505,7 @@ void op_join(Conn& c, const std::vector<std::string>& a) {
     const Layer* G = lp.swapped ? lp.A.get() : lp.B.get();
     s.points = true;
     r.kind = GSJ_REQ_POINTS_IN_POLYGONS; r.P = P->gl; r.R = G->gl; r.points_cb = join_points_cb; r.user = &s;
+    tr.apply(r.opts);
     const gsj_join_stats st = run_request(r, c);
     stats = Json().k("kind").v("point_in_polygon").k("candidates").u(st.candidates).k("gpu_hits").u(st.hits).k("rows").u(s.total)
         .k("traverse_ms").n(st.traverse_ms).k("refine_ms").n(st.refine_ms).k("host_encode_send_s").n(s.host_s).k("total_s").n(secs_since(t0))
@@ -471,6 +513,7 @@ void op_join(Conn& c, const std::vector<std::string>& a) {
   } else {
     r.kind = GSJ_REQ_JOIN; r.P = lp.A->gl; r.R = lp.B->gl; r.opts.want_area = want_area; r.opts.upper = lp.same;
     r.rows_cb = join_rows_cb; r.user = &s;
+    tr.apply(r.opts);
     const gsj_join_stats st = run_request(r, c);
     Json j;
     j.k("kind").v("polygon_polygon").k("candidates").u(st.candidates).k("gpu_hits").u(st.hits).k("rows").u(s.total).k("heavy").u(st.heavy)
@@ -552,7 +595,9 @@ int relate_points_cb(void* u, const uint32_t* pairs, const uint8_t* where, size_
   return s.c->broken ? 1 : 0;
 }
 void op_relate(Conn& c, const std::vector<std::string>& a) {
-  if (a.size() < 4) throw std::runtime_error("RELATE needs layer_a, layer_b, pattern, self_pairs");
+  if (a.size() < 4) throw std::runtime_error("RELATE needs layer_a, layer_b, pattern, self_pairs[, window=x0,y0,x1,y1]");
+  const Trailing tr = trailing(a, 4, false);
+  if (tr.stats) throw std::runtime_error("RELATE has no stats mode");
   LayerPair lp = layer_pair(a[0], a[1], "relate");
   const std::string pat = a[2];
   if (!pat.empty()) {
@@ -570,10 +615,12 @@ void op_relate(Conn& c, const std::vector<std::string>& a) {
   if (lp.A->kind == "point" || lp.swapped) {
     r.kind = GSJ_REQ_POINTS_IN_POLYGONS; r.P = (lp.swapped ? lp.B : lp.A)->gl; r.R = (lp.swapped ? lp.A : lp.B)->gl;
     r.points_cb = relate_points_cb; r.user = &s;
+    tr.apply(r.opts);
     st = run_request(r, c);
   } else {
     r.kind = GSJ_REQ_JOIN; r.P = lp.A->gl; r.R = lp.B->gl; r.opts.want_relate = 1; r.opts.upper = lp.same;
     r.rows_cb = relate_rows_cb; r.user = &s;
+    tr.apply(r.opts);
     st = run_request(r, c);
   }
   const std::string stats = Json().k("kind").v("relate").k("pattern").v(pat).k("candidates").u(st.candidates).k("gpu_hits").u(st.hits)
@@ -627,16 +674,10 @@ int dist_rows_cb(void* u, const uint32_t* pairs, const double* dist, size_t n) {
   s.host_s += secs_since(th);
   return s.c->broken ? 1 : 0;
 }
-double to_double(const std::string& s) {
-  char* e;
-  const double v = strtod(s.c_str(), &e);
-  if (s.empty() || *e || !std::isfinite(v)) throw std::runtime_error("not a number: " + s);
-  return v;
-}
 void op_dwithin(Conn& c, const std::vector<std::string>& a) {
-  if (a.size() < 6) throw std::runtime_error("DWITHIN needs layer_a, layer_b, mode, dist_a, dist_b, self_pairs[, stats]");
-  const bool stats_only = a.size() > 6 && a[6] == "stats";
-  const bool with_distance = a.size() > 6 && a[6] == "distance";
+  if (a.size() < 6) throw std::runtime_error("DWITHIN needs layer_a, layer_b, mode, dist_a, dist_b, self_pairs[, stats|distance][, window=x0,y0,x1,y1]");
+  const Trailing tr = trailing(a, 6, true);
+  const bool stats_only = tr.stats, with_distance = tr.distance;
   LayerPair lp = layer_pair(a[0], a[1], "distance joins");
   const std::string mode = a[2];
   const double da = to_double(a[3]), db = to_double(a[4]);
@@ -684,6 +725,7 @@ void op_dwithin(Conn& c, const std::vector<std::string>& a) {
       r.kind = GSJ_REQ_DWITHIN; r.d = d; r.report_distance = s.with_distance;
     }
   }
+  tr.apply(r.opts);
   const gsj_join_stats st = run_request(r, c);
   const std::string stats = Json().k("kind").v(erode ? "buffer_intersects" : "dwithin").k("distance_m").n(d)
       .k("erode_a_m").n(lp.swapped ? rb : ra).k("erode_b_m").n(lp.swapped ? ra : rb).k("candidates").u(st.candidates).k("gpu_hits").u(st.hits)

@@ -32,7 +32,8 @@ class GsjError(RuntimeError):
 
 class JoinOpts(C.Structure):
     _fields_ = [("want_area", C.c_int), ("want_relate", C.c_int), ("upper", C.c_int), ("snap_half_ulp", C.c_int),
-                ("batch_q", C.c_uint32), ("max_pairs", C.c_uint64), ("no_agg", C.c_int), ("cancel", C.POINTER(C.c_int))]
+                ("batch_q", C.c_uint32), ("max_pairs", C.c_uint64), ("no_agg", C.c_int), ("cancel", C.POINTER(C.c_int)),
+                ("has_window", C.c_int), ("window", C.c_double * 4)]
 
 
 class JoinStats(C.Structure):
@@ -175,21 +176,44 @@ class HostSet:
         return [self.xy[self.rvo[r]:self.rvo[r + 1]] for r in range(int(self.gro[g]), int(self.gro[g + 1]))]
 
 
+class _NativeSet:
+    """Owns a library-allocated host set; freed when the last array viewing it is gone."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if self.h:
+            load().gsj_hostset_free(self.h)
+            self.h = None
+
+
+class _View:
+    """numpy view of library memory (the array's base keeps the owner alive)"""
+
+    def __init__(self, owner, ptr, shape, typestr):
+        self.owner = owner
+        self.__array_interface__ = {"data": (C.cast(ptr, C.c_void_p).value, False), "shape": shape, "typestr": typestr, "version": 3}
+
+
 def _take_hostset(h) -> HostSet:
+    """wrap a library host set without copying it (national-scale layers are tens of GB)"""
     lib = load()
+    owner = _NativeSet(h)
     n, nr, nv = C.c_uint64(), C.c_uint64(), C.c_uint64()
     _check(lib.gsj_hostset_sizes(h, C.byref(n), C.byref(nr), C.byref(nv)))
     gro, rvo, xy = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint64)(), C.POINTER(C.c_double)()
     par, syn = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint8)()
     _check(lib.gsj_hostset_arrays(h, C.byref(gro), C.byref(rvo), C.byref(xy), C.byref(par), C.byref(syn)))
-    try:
-        out = HostSet(np.ctypeslib.as_array(gro, (n.value + 1,)).copy(), np.ctypeslib.as_array(rvo, (nr.value + 1,)).copy(),
-                      (np.ctypeslib.as_array(xy, (2 * nv.value,)).copy() if nv.value else np.zeros(0)).reshape(-1, 2))
-        if par:
-            out.parent = np.ctypeslib.as_array(par, (n.value,)).copy() if n.value else np.zeros(0, np.uint32)
-            out.syn = np.ctypeslib.as_array(syn, (nv.value,)).copy() if nv.value else np.zeros(0, np.uint8)
-    finally:
-        lib.gsj_hostset_free(h)
+
+    def view(ptr, count, typestr, empty_dtype):
+        return np.asarray(_View(owner, ptr, (count,), typestr)) if count and ptr else np.zeros(0, empty_dtype)
+
+    out = HostSet(view(gro, n.value + 1, "<u8", np.uint64), view(rvo, nr.value + 1, "<u8", np.uint64),
+                  view(xy, 2 * nv.value, "<f8", np.float64).reshape(-1, 2))
+    if par:
+        out.parent = view(par, n.value, "<u4", np.uint32)
+        out.syn = view(syn, nv.value, "|u1", np.uint8)
     return out.normalized()
 
 
@@ -339,10 +363,14 @@ class Rows:
     root: np.ndarray | None = None   # cluster: smallest parent index of each parent's component
 
 
-def _opts(want_area=False, want_relate=False, upper=False, snap_half_ulp=False, batch_q=0, max_pairs=0, no_agg=False):
+def _opts(want_area=False, want_relate=False, upper=False, snap_half_ulp=False, batch_q=0, max_pairs=0, no_agg=False, window=None):
+    """window = (x0, y0, x1, y1): probe only the parents with a piece box meeting the rectangle"""
     o = JoinOpts()
     o.want_area, o.want_relate, o.upper, o.snap_half_ulp = int(want_area), int(want_relate), int(upper), int(snap_half_ulp)
     o.batch_q, o.max_pairs, o.no_agg = batch_q, max_pairs, int(no_agg)
+    if window is not None:
+        o.has_window = 1
+        o.window[:] = [float(v) for v in window]
     return o
 
 

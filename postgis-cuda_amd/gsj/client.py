@@ -47,6 +47,11 @@ class _Reader:
         return self.take(self.u32()).decode()
 
 
+def _window_arg(window):
+    """the optional trailing query-window argument: probe only geometries whose box meets (x0, y0, x1, y1)"""
+    return [] if window is None else ["window=" + ",".join(repr(float(v)) for v in window)]
+
+
 def _row_parser(op: int):
     def ids(r):
         la = struct.unpack("<H", r.take(2))[0]
@@ -109,14 +114,14 @@ class Client:
     def load(self, layer, source, subdivide=0, dedup=True):
         return self.call(OP_LOAD, layer, source, "id", "geom", "", subdivide, "true" if dedup else "false")
 
-    def join(self, a, b, want_area=False, self_pairs="all", stats=False):
-        return self.call(OP_JOIN, a, b, "1" if want_area else "0", self_pairs, *(["stats"] if stats else []))
+    def join(self, a, b, want_area=False, self_pairs="all", stats=False, window=None):
+        return self.call(OP_JOIN, a, b, "1" if want_area else "0", self_pairs, *(["stats"] if stats else []), *_window_arg(window))
 
-    def relate(self, a, b, pattern="", self_pairs="all"):
-        return self.call(OP_RELATE, a, b, pattern, self_pairs)
+    def relate(self, a, b, pattern="", self_pairs="all", window=None):
+        return self.call(OP_RELATE, a, b, pattern, self_pairs, *_window_arg(window))
 
-    def dwithin(self, a, b, mode, dist_a, dist_b, self_pairs="all", extra=None):
-        return self.call(OP_DWITHIN, a, b, mode, dist_a, dist_b, self_pairs, *([extra] if extra else []))
+    def dwithin(self, a, b, mode, dist_a, dist_b, self_pairs="all", extra=None, window=None):
+        return self.call(OP_DWITHIN, a, b, mode, dist_a, dist_b, self_pairs, *([extra] if extra else []), *_window_arg(window))
 
     def cluster(self, layer):
         return self.call(OP_CLUSTER, layer)

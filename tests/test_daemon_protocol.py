@@ -27,6 +27,17 @@ def test_request_layout_matches_the_reference_protocol():
     assert args == ["tok", "a", "b", "1", "all"] and off == len(b)
 
 
+def test_window_is_an_additive_trailing_argument():
+    from gsj.client import _window_arg
+
+    assert _window_arg(None) == []
+    (w,) = _window_arg((-95.5, 38.5, -95.25, 38.75))
+    assert w == "window=-95.5,38.5,-95.25,38.75"
+    # floats round-trip exactly through the text form
+    x = 0.1 + 0.2
+    assert float(_window_arg((x, 0, 1, 1))[0][7:].split(",")[0]) == x
+
+
 def _serve_once(payload: bytes):
     d = tempfile.mkdtemp()
     path = os.path.join(d, "s.sock")
