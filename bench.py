@@ -129,6 +129,125 @@ def serving_leg(gsj, ctx, args):
     return out
 
 
+NATIONAL_NX, NATIONAL_NY = 19737, 8224   # 162.3M cells of 60/19737 degrees over 60 x 25 degrees
+
+
+def national_leg(gsj, ctx, args):
+    """Resident national-scale layers and a windowed request stream (the serving configuration).
+
+    Parcels: the config-2 lattice generator (cells share or overlap their sides, as real parcels do, so that distance
+    joins find neighbours; median 10 vertices, at most 256 so no piece needs subdivision, seed 4) with cells of
+    60/19737 degrees tiled from (-125, 24): 19737 x 8224 cells = 162.3M parcels at scale 1, both sides scaled by
+    sqrt(scale). Points: 161M at scale 1, half uniform over the extent, half in clusters of sigma 50 m, seed 4.
+    On the device: upload, LBVH, self-dwithin at 5 m of the whole parcel layer and points-in-polygons of every point
+    (rows counted on the host). Then a request stream over the resident parcels with Poisson arrivals: self joins in
+    query windows of area lognormal in [1e-4, 10] square degrees, mix 40% intersects + area, 20% relate, 20% dwithin
+    d ~ U(1, 50) m, 20% eroded intersects ra, rb ~ U(0.5, 3) m; served (`lanes` lanes) vs serial (one lane) on the
+    same arrival times; latency = arrival -> finished."""
+    f = args.national
+    cell = 60.0 / NATIONAL_NX
+    nx, ny = max(1, round(NATIONAL_NX * f ** 0.5)), max(1, round(NATIONAL_NY * f ** 0.5))
+    x0, y0, W, H = -125.0, 24.0, nx * cell, ny * cell
+    out = {"workload": f"resident parcels {nx}x{ny} cells of {cell:.6g} deg (config-2 generator, median 10, kmax 256, seed 4) + "
+                       f"{int(161e6 * f)} clustered points; self-dwithin 5 m, points-in-polygons, windowed request stream"}
+    t0 = time.time()
+    M = gsj.generate("parcels_mesh", nx=nx, ny=ny, x0=x0, y0=y0, w=cell, h=cell, median=10.0, kmax=256, seed=4)
+    t_gen = time.time() - t0
+    log(f"national: generated {M.n} parcels, {M.nv} vertices in {t_gen:.1f}s")
+    t0 = time.time()
+    L = ctx.polys(M, build_bvh=False)
+    t_up = time.time() - t0
+    n_par, n_v = M.n, M.nv
+    L.host = None
+    del M
+    t0 = time.time()
+    L.build_bvh()
+    t_bvh = time.time() - t0
+    info = L.info()
+    log(f"national: upload {t_up:.1f}s, bvh {t_bvh:.1f}s, {info.device_bytes / 2**30:.1f} GiB resident")
+    npts = int(161e6 * f)
+    t0 = time.time()
+    xy = gsj.generate_points(n=npts, x0=x0, y0=y0, w=W, h=H, clusters=max(1, int(32000 * f)), sigma_m=50.0, seed=4)
+    P = ctx.points(xy)
+    del xy
+    t_pts = time.time() - t0
+    pinfo = P.info()
+    out["resident"] = {"parcels": n_par, "vertices": n_v, "points": npts, "parcel_layer_gib": round(info.device_bytes / 2**30, 2),
+                       "point_layer_gib": round(pinfo.device_bytes / 2**30, 2), "generate_s": round(t_gen, 1), "upload_s": round(t_up, 1),
+                       "bvh_s": round(t_bvh, 2), "points_generate_upload_s": round(t_pts, 1)}
+    log(f"national: points {npts} in {t_pts:.1f}s; {ctx.status()}")
+    t0 = time.perf_counter()
+    r = ctx.dwithin(L, L, 5.0, same=True, upper=True, collect=False)
+    t_dw = time.perf_counter() - t0
+    out["self_dwithin_5m"] = {"s": round(t_dw, 3), "candidates": r.stats["candidates"], "rows": r.stats["hits"],
+                              "refined_pairs_per_s": round(r.stats["candidates"] / t_dw, 1)}
+    log(f"national: self-dwithin 5 m {t_dw:.2f}s {r.stats}")
+    t0 = time.perf_counter()
+    r = ctx.points_in_polygons(P, L, collect=False)
+    t_pip = time.perf_counter() - t0
+    out["points_in_polygons"] = {"s": round(t_pip, 3), "candidates": r.stats["candidates"], "rows": r.stats["hits"],
+                                 "refined_pairs_per_s": round(r.stats["candidates"] / t_pip, 1)}
+    log(f"national: points-in-polygons {t_pip:.2f}s {r.stats}")
+    P.free()
+
+    # windowed request stream, Poisson arrivals
+    rng = np.random.default_rng(4)
+    nreq = args.stream_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, nreq))
+    arrivals -= arrivals[0]
+    mu, sig = np.log(np.sqrt(1e-4 * 10.0)), np.log(10.0 / 1e-4) / 6
+    area = np.clip(np.exp(rng.normal(mu, sig, nreq)), 1e-4, 10.0)
+    side = np.minimum(np.sqrt(area), min(W, H))
+    cx, cy = x0 + rng.uniform(0, 1, nreq) * (W - side) + side / 2, y0 + rng.uniform(0, 1, nreq) * (H - side) + side / 2
+    kinds = rng.choice(4, size=nreq, p=[0.4, 0.2, 0.2, 0.2])
+    dists, radii = rng.uniform(1.0, 50.0, nreq), rng.uniform(0.5, 3.0, (nreq, 2))
+    reqs = []
+    for i in range(nreq):
+        win = (cx[i] - side[i] / 2, cy[i] - side[i] / 2, cx[i] + side[i] / 2, cy[i] + side[i] / 2)
+        if kinds[i] == 2:
+            reqs.append((gsj.REQ_DWITHIN, dict(d=float(dists[i]), same=True, upper=True, window=win)))
+        elif kinds[i] == 3:
+            reqs.append((gsj.REQ_INTERSECTS_BUFFERED, dict(ra=float(radii[i, 0]), rb=float(radii[i, 1]), same=True, upper=True, window=win)))
+        else:
+            reqs.append((gsj.REQ_JOIN, dict(want_area=kinds[i] == 0, want_relate=kinds[i] == 1, upper=True, window=win)))
+
+    def run(lanes):
+        s = gsj.Scheduler(ctx, lanes=lanes)
+        try:
+            for k, kw in reqs[: min(nreq, 16)]:   # warm-up: lanes' scratch
+                s.wait(s.submit(k, L, L, collect=False, **kw))
+            t_start = time.perf_counter()
+            tickets, sub = [], []
+            for (k, kw), ta in zip(reqs, arrivals):
+                dt = t_start + ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                sub.append(time.perf_counter())
+                tickets.append(s.submit(k, L, L, collect=False, **kw))
+            res = [s.wait(t) for t in tickets]
+            wall = time.perf_counter() - t_start
+        finally:
+            s.close()
+        cand = sum(r.stats["candidates"] for r in res)
+        rows = sum(r.stats["hits"] for r in res)
+        lat = np.array([r.stats["times"]["total_ms"] for r in res])
+        return {"lanes": lanes, "wall_s": round(wall, 3), "refined_pairs_per_s": round(cand / wall, 1), "rows_per_s": round(rows / wall, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3), "candidates": int(cand),
+                "rows": int(rows)}, [(r.stats["candidates"], r.stats["hits"]) for r in res]
+
+    serial, cs = run(1)
+    served, cv = run(args.lanes)
+    assert cs == cv, "served and serial request results differ"
+    out["stream"] = {"requests": nreq, "arrivals": f"Poisson, {args.rate:g}/s", "window_area_deg2": "lognormal in [1e-4, 10]",
+                     "mix": "40% intersects+area, 20% relate, 20% dwithin d~U(1,50) m, 20% eroded intersects ra,rb~U(0.5,3) m (self joins)",
+                     "serial": serial, "served": served,
+                     "speedup_pairs_per_s": round(served["refined_pairs_per_s"] / serial["refined_pairs_per_s"], 3),
+                     "p99_ratio_served_over_serial": round(served["p99_ms"] / serial["p99_ms"], 3)}
+    log(f"national stream: serial {serial}; served {served}")
+    L.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +261,10 @@ def main():
     ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
     ap.add_argument("--lanes", type=int, default=4, help="lanes of the served path")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run 1 GiB device copies (HBM counter calibration)")
+    ap.add_argument("--national", type=float, default=0.0,
+                    help="run only the resident national-scale leg at this scale (1.0 = 162M parcels + 161M points)")
+    ap.add_argument("--stream-requests", type=int, default=400, help="requests in the national windowed stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate (requests/s) of the national stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is described, not built)",
@@ -149,6 +272,18 @@ def main():
         sys.exit(2)
 
     import gsj
+
+    if args.national > 0:
+        ctx = gsj.Context(0)
+        t0 = time.perf_counter()
+        nat = national_leg(gsj, ctx, args)
+        st = nat["stream"]["served"]
+        print(json.dumps({"metric": METRIC, "value": st["refined_pairs_per_s"], "unit": "refined pairs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+                          "ms_per_step": round(1e3 * (time.perf_counter() - t0), 1), "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic (deterministic generators)",
+                          "config": {"workload": nat["workload"], "parallelism": "1 GPU"}, "national": nat}), flush=True)
+        ctx.close()
+        return
 
     t0 = time.time()
     M, S = mesh(gsj, args.scale)

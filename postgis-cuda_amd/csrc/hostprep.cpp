@@ -567,7 +567,9 @@ std::vector<double> generate_points(const char* spec) {
   const Rng r(seed, 20), rc(seed, 21);
   std::vector<double> xy(2 * n);
   const uint64_t ncl = nc ? (uint64_t)std::llround(frac * (double)n) : 0;
-  for (uint64_t i = 0; i < n; i++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t ii = 0; ii < (int64_t)n; ii++) {   // every point draws from its own counters
+    const uint64_t i = (uint64_t)ii;
     if (i < n - ncl) {
       xy[2 * i] = x0 + W * r.uni(2 * i);
       xy[2 * i + 1] = y0 + H * r.uni(2 * i + 1);

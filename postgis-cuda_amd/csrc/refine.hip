@@ -30,14 +30,15 @@ namespace {
 // The small class reads a polygon's edges from its LDS vertex list (the list classes use ListSrc).
 constexpr uint8_t F_VALID = 1, F_SYN = 2, F_CUT0 = 4, F_CUT1 = 8, F_INOV = 16;
 
-struct VertSrc {   // all vertices of a small polygon; edge k = (V[k], V[k+1]) exists iff F[k] & F_VALID
+struct VertSrc {   // the edges of a small polygon meeting the overlap box: list entry j is edge (V[I[j]], V[I[j] + 1])
   const double2* V;
   const uint8_t* F;
+  const uint8_t* I;
   int n;
   __device__ __forceinline__ int size() const { return n; }
-  __device__ __forceinline__ bool get(int k, Ed& e) const {
+  __device__ __forceinline__ bool get(int j, Ed& e) const {
+    const int k = I[j];
     const uint8_t f = F[k];
-    if ((f & (F_VALID | F_INOV)) != (F_VALID | F_INOV)) return false;
     const double2 a = V[k], b = V[k + 1];
     e.x0 = a.x; e.y0 = a.y; e.x1 = b.x; e.y1 = b.y;
     e.syn = (f & F_SYN) ? 1 : 0;
@@ -310,8 +311,9 @@ __device__ __forceinline__ void finish_pair(int gl, uint64_t p, bool hit, bool r
 struct SmallPoly {
   double2 V[VCAP];
   uint8_t F[VCAP];
+  uint8_t I[VCAP];        // edges meeting the overlap box, ascending (the reference's compacted edge list)
   uint8_t rs[RCAP + 1];   // ring start slots, rs[nrings] = vertex count
-  uint8_t nv, nr;
+  uint8_t nv, nr, ni;
 };
 constexpr size_t small_bytes(bool rel) { return ((2 * sizeof(SmallPoly) + 15) & ~size_t(15)) + (size_t)2 * VCAP * 8 * (rel ? 3 : 1); }
 
@@ -343,6 +345,17 @@ __device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, 
     }
     sp.F[k] = f;
   }
+  wave_lds_sync();
+  // compacted list of the edges meeting the overlap box, in vertex order (group ballot)
+  int cnt = 0;
+  for (int base = 0; base < nv; base += G) {
+    const int k = base + gl;
+    const bool in = k < nv && (sp.F[k] & (F_VALID | F_INOV)) == (F_VALID | F_INOV);
+    const uint64_t m = gballot<G>(in);
+    if (in) sp.I[cnt + __popcll(m & ((1ull << gl) - 1))] = (uint8_t)k;
+    cnt += __popcll(m);
+  }
+  if (gl == 0) sp.ni = (uint8_t)cnt;
   wave_lds_sync();
 }
 
@@ -401,8 +414,8 @@ __global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const D
     const PairSetup f = setup_pair(P, R, a, b, rp);
     load_small<G>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
     load_small<G>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
-    const int na = SA.nv, nb = SB.nv;
-    const VertSrc LA{SA.V, SA.F, na}, LB{SB.V, SB.F, nb};
+    const int na = SA.ni, nb = SB.ni;   // edges meeting the overlap box
+    const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SB.V, SB.F, SB.I, nb};
     // boundary contact over the A x B edge grid
     bool hit = false, hit_any = false;
     const int grid = na * nb;

@@ -188,6 +188,18 @@ class _NativeSet:
             self.h = None
 
 
+class _NativeBuffer:
+    """Owns a buffer the library allocated with malloc (gsj_free)."""
+
+    def __init__(self, p):
+        self.p = p
+
+    def __del__(self):
+        if self.p:
+            load().gsj_free(self.p)
+            self.p = None
+
+
 class _View:
     """numpy view of library memory (the array's base keeps the owner alive)"""
 
@@ -218,7 +230,14 @@ def _take_hostset(h) -> HostSet:
 
 
 def _spec(kw):
-    return ";".join(f"{k}={v!r}" if isinstance(v, float) else f"{k}={v}" for k, v in kw.items())
+    def fmt(v):
+        if isinstance(v, (float, np.floating)):
+            return repr(float(v))   # shortest text that reads back to the same double
+        if isinstance(v, np.integer):
+            return str(int(v))
+        return str(v)
+
+    return ";".join(f"{k}={fmt(v)}" for k, v in kw.items())
 
 
 def generate(kind: str, **spec) -> HostSet:
@@ -234,10 +253,10 @@ def generate_points(**spec) -> np.ndarray:
     lib = load()
     p, n = C.POINTER(C.c_double)(), C.c_uint64()
     _check(lib.gsj_generate_points(_spec(spec).encode(), C.byref(p), C.byref(n)))
-    try:
-        return np.ctypeslib.as_array(p, (2 * n.value,)).copy().reshape(-1, 2) if n.value else np.zeros((0, 2))
-    finally:
+    if not n.value:
         lib.gsj_free(p)
+        return np.zeros((0, 2))
+    return np.asarray(_View(_NativeBuffer(p), p, (2 * n.value,), "<f8")).reshape(-1, 2)
 
 
 def subdivide(hs: HostSet, maxv: int) -> HostSet:
@@ -425,12 +444,13 @@ class Context:
         code = np.concatenate([c[2] for c in chunks]) if chunks and o.want_relate else (np.zeros(0, np.uint32) if o.want_relate else None)
         return Rows(pairs, area, code, None, st.as_dict())
 
-    def points_in_polygons(self, pts: Layer, R: Layer, **kw) -> Rows:
+    def points_in_polygons(self, pts: Layer, R: Layer, collect=True, **kw) -> Rows:
         o = _opts(**kw)
         chunks = []
 
         def cb(user, pairs, where, n):
-            chunks.append((np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2), np.ctypeslib.as_array(where, (n,)).copy()))
+            if collect:
+                chunks.append((np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2), np.ctypeslib.as_array(where, (n,)).copy()))
             return 0
 
         st = JoinStats()
@@ -439,11 +459,13 @@ class Context:
         where = np.concatenate([c[1] for c in chunks]) if chunks else np.zeros(0, np.uint8)
         return Rows(pairs, None, None, where, st.as_dict())
 
-    def _dist_rows(self, fn, *args, **kw):
+    def _dist_rows(self, fn, *args, collect=True, **kw):
         o = _opts(**kw)
         chunks = []
 
         def cb(user, pairs, dist, n):
+            if not collect:
+                return 0
             p = np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2)
             chunks.append((p, np.ctypeslib.as_array(dist, (n,)).copy() if dist else None))
             return 0
