@@ -39,11 +39,12 @@ def mesh(gsj, n, seed=1, subdiv=256):
     return M, gsj.subdivide(M, subdiv)
 
 
-def small_class(hs):
-    """Per piece: fits the small refinement class (<= 32 stored vertices and <= 8 rings)."""
+def piece_class(hs):
+    """Per piece: the largest vertex-list refinement class it fits: 0 small (<= 32 stored vertices), 1 medium (<= 64),
+    2 large (more vertices or more than 8 rings). A pair's class is the larger of its two pieces'."""
     rings = np.diff(hs.gro.astype(np.int64))
     verts = hs.rvo[hs.gro[1:]].astype(np.int64) - hs.rvo[hs.gro[:-1]].astype(np.int64)
-    return (verts <= 32) & (rings <= 8)
+    return np.where(rings > 8, 2, np.where(verts <= 32, 0, np.where(verts <= 64, 1, 2)))
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -315,11 +316,14 @@ def main():
     ms_step = 1e3 * t_all / args.steps
     value = cand * args.steps / t_all
     log(f"step {ms_step:.2f} ms: {cand} candidates, {rows} rows; traverse {st['traverse_ms']:.2f} refine {st['refine_ms']:.2f} "
-        f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels small {st['k_small_ms']:.2f} large {st['k_large_ms']:.2f} "
-        f"over {st['k_over_ms']:.2f} ms (pairs {st['n_small']}/{st['n_large']}/{st['n_over']})")
+        f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels small {st['k_small_ms']:.2f} medium {st['k_medium_ms']:.2f} "
+        f"large {st['k_large_ms']:.2f} over {st['k_over_ms']:.2f} ms (pairs {st['n_small']}/{st['n_medium']}/{st['n_large']}/{st['n_over']})")
 
     # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator)
-    kern = max((("k_refine_small", "k_small_ms", "n_small"), ("k_refine_large", "k_large_ms", "n_large")), key=lambda k: st[k[1]])
+    classes = (("k_refine_small<16,32>", "k_small_ms", "n_small"), ("k_refine_small<32,64>", "k_medium_ms", "n_medium"),
+               ("k_refine_large", "k_large_ms", "n_large"))
+    kc = max(range(3), key=lambda i: st[classes[i][1]])
+    kern = classes[kc]
     roofline = None
     cpu = None
     parity = None
@@ -329,9 +333,8 @@ def main():
         Ms, Ss = mesh(gsj, args.sample)
         Os = O.Layer.polys(Ss)
         cand_s = O.candidates(Os, Os, upper=True)
-        sc = small_class(Ss)
-        small = sc[cand_s[:, 0]] & sc[cand_s[:, 1]]
-        pick = cand_s[small] if kern[0] == "k_refine_small" else cand_s[~small]
+        pc = piece_class(Ss)
+        pick = cand_s[np.maximum(pc[cand_s[:, 0]], pc[cand_s[:, 1]]) == kc]
         rng = np.random.default_rng(0)
         pick = pick[rng.choice(len(pick), size=min(len(pick), 40000), replace=False)]
         fl = O.refine_flops(Ss, Ss, pick, want_area=False, want_relate=True)
@@ -374,7 +377,7 @@ def main():
                    "pieces": S.n, "vertices": S.nv, "candidates_per_step": cand, "rows_per_step": rows, "parallelism": "1 GPU"},
         "rows_per_s": round(rows * args.steps / t_all, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
-        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_small_ms", "k_large_ms", "k_over_ms")},
+        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_small_ms", "k_medium_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1)},
         "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving,
     }

@@ -110,6 +110,8 @@ typedef struct {
   double k_small_ms, k_large_ms, k_over_ms;
   uint64_t n_small, n_large, n_over;
   uint64_t launches;       /* refinement launches (sub-batches) */
+  double k_medium_ms;      /* 32-lane class (not small, both polygons <= 64 stored vertices and <= 8 rings) */
+  uint64_t n_medium;
 } gsj_join_stats;
 
 /* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;

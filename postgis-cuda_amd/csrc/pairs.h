@@ -166,16 +166,18 @@ __device__ __forceinline__ void queue_overflow(uint32_t p, int need, uint32_t* o
 
 // ------------------------------------------------------------------------------------------- work classes
 constexpr int VCAP = 32, RCAP = 8;   // small class: stored vertices and rings per polygon
+constexpr int MCAP = 64;             // medium class: stored vertices per polygon (rings within RCAP)
 
 // Candidate pairs split by work class, on the device: list[0 .. counts[0]) the small pairs (both polygons within
-// VCAP / RCAP), list[counts[0] .. counts[0] + counts[1]) the rest, each in candidate order; ctl cleared for the
-// overflow queue. Uses the context's s_cls, s_tmp2 and s_misc scratch.
+// VCAP / RCAP), then counts[1] medium pairs (not small, both within mcap / RCAP; none when mcap is 0), then the
+// counts[2] others, each class in candidate order; ctl cleared for the overflow queue. Uses the context's s_cls,
+// s_tmp2 and s_misc scratch.
 struct PairClasses {
   uint32_t* list;
   uint32_t* counts;
   OverflowCtl* ctl;
 };
-PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np);
+PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap);
 
 // group-wide minimum (order-independent, so deterministic)
 template <int G>

@@ -308,18 +308,20 @@ __device__ __forceinline__ void finish_pair(int gl, uint64_t p, bool hit, bool r
 }
 
 // ------------------------------------------------------------------------------------------- small class
-struct SmallPoly {
-  double2 V[VCAP];
-  uint8_t F[VCAP];
-  uint8_t I[VCAP];        // edges meeting the overlap box, ascending (the reference's compacted edge list)
+template <int CAP>
+struct SmallPoly {        // a polygon of at most CAP stored vertices (<= 255) and RCAP rings
+  double2 V[CAP];
+  uint8_t F[CAP];
+  uint8_t I[CAP];         // edges meeting the overlap box, ascending (the reference's compacted edge list)
   uint8_t rs[RCAP + 1];   // ring start slots, rs[nrings] = vertex count
   uint8_t nv, nr, ni;
 };
-constexpr size_t small_bytes(bool rel) { return ((2 * sizeof(SmallPoly) + 15) & ~size_t(15)) + (size_t)2 * VCAP * 8 * (rel ? 3 : 1); }
+template <int CAP>
+constexpr size_t small_bytes(bool rel) { return ((2 * sizeof(SmallPoly<CAP>) + 15) & ~size_t(15)) + (size_t)2 * CAP * 8 * (rel ? 3 : 1); }
 
 // all vertices of geometry g into LDS with per-vertex flags (reference edge collection, engine/engine.cuh:358-377)
-template <int G>
-__device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, SmallPoly& sp, int gl) {
+template <int G, int CAP>
+__device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, SmallPoly<CAP>& sp, int gl) {
   const uint32_t r0 = S.gro[g], nrings = S.gro[g + 1] - r0;
   const uint64_t v0 = S.rvo[r0];
   const int nv = (int)(S.rvo[r0 + nrings] - v0);
@@ -360,8 +362,8 @@ __device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, 
 }
 
 // parity of the segment from a point left of the geometry's bbox to m over every edge (engine/engine.cuh:382-397)
-template <int G>
-__device__ bool pip_small(const SmallPoly& sp, double shx, double mx, double my, int gl) {
+template <int G, int CAP>
+__device__ bool pip_small(const SmallPoly<CAP>& sp, double shx, double mx, double my, int gl) {
   const double ox = shx - 1.0, oy = my;
   bool par = false;
   for (int k = gl; k < sp.nv; k += G) {
@@ -379,7 +381,8 @@ __device__ bool pip_small(const SmallPoly& sp, double shx, double mx, double my,
 
 // containment probe of ring r: first vertex that starts a real edge, else the first vertex (engine/engine.cuh:322-327);
 // returns 0 for an empty ring (no probe), 1 for a probe on a synthetic-only ring, 2 for a real probe
-__device__ __forceinline__ int probe_small(const SmallPoly& sp, int r, bool has_syn, double& x, double& y) {
+template <int CAP>
+__device__ __forceinline__ int probe_small(const SmallPoly<CAP>& sp, int r, bool has_syn, double& x, double& y) {
   const int s = sp.rs[r], e = sp.rs[r + 1];
   if (s == e) return 0;
   int v = s;
@@ -393,27 +396,31 @@ __device__ __forceinline__ int probe_small(const SmallPoly& sp, int r, bool has_
   return real ? 2 : 1;
 }
 
-template <int G>
+// vertex-list classes: a G-lane group per pair, both polygons in LDS (at most CAP stored vertices each); class
+// WHICH of the partition (0 small, 1 medium)
+template <int G, int CAP, int WHICH>
 __global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
-                                                         const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist_ptr, RefineParams rp,
+                                                         const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
                                                          RefineOut out, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
   extern __shared__ __align__(16) unsigned char smem[];
+  using SP = SmallPoly<CAP>;
   const int gpb = blockDim.x / G;
   const int gib = threadIdx.x / G, gl = threadIdx.x % G;
   const bool rel = rp.want_relate != 0;
-  unsigned char* base = smem + (size_t)gib * small_bytes(rel);
-  SmallPoly& SA = *reinterpret_cast<SmallPoly*>(base);
-  SmallPoly& SB = *(reinterpret_cast<SmallPoly*>(base) + 1);
-  double* acc = reinterpret_cast<double*>(base + ((2 * sizeof(SmallPoly) + 15) & ~size_t(15)));
-  double* lin = acc + 2 * VCAP;
-  double* lon = lin + 2 * VCAP;
-  const uint32_t nlist = *nlist_ptr;
+  unsigned char* base = smem + (size_t)gib * small_bytes<CAP>(rel);
+  SP& SA = *reinterpret_cast<SP*>(base);
+  SP& SB = *(reinterpret_cast<SP*>(base) + 1);
+  double* acc = reinterpret_cast<double*>(base + ((2 * sizeof(SP) + 15) & ~size_t(15)));
+  double* lin = acc + 2 * CAP;
+  double* lon = lin + 2 * CAP;
+  const uint32_t nlist = counts[WHICH];
+  const uint32_t* cl = list + (WHICH == 0 ? 0u : counts[0]);
   for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
-    const uint32_t p = list[k];
+    const uint32_t p = cl[k];
     const uint32_t a = pa[p], b = pb[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    load_small<G>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
-    load_small<G>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
+    load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
+    load_small<G, CAP>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
     const int na = SA.ni, nb = SB.ni;   // edges meeting the overlap box
     const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SB.V, SB.F, SB.I, nb};
     // boundary contact over the A x B edge grid
@@ -439,8 +446,8 @@ __global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const D
     const bool rtouch = hit;
     bool inB_c = false, inA_c = false;
     if (!hit || rp.want_area || rel) {
-      inB_c = pip_small<G>(SB, f.bshx, f.cx, f.cy, gl);
-      inA_c = pip_small<G>(SA, f.ashx, f.cx, f.cy, gl);
+      inB_c = pip_small<G, CAP>(SB, f.bshx, f.cx, f.cy, gl);
+      inA_c = pip_small<G, CAP>(SA, f.ashx, f.cx, f.cy, gl);
     }
     bool contain_any = false;
     if (!hit) {
@@ -580,8 +587,8 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
   const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
   const bool rel = rp.want_relate != 0;
   const Scratch S = carve(smem + (size_t)wib * list_bytes(cap, rel), cap, rel);
-  const uint32_t nlist = nlist_ptr[1];
-  const uint32_t* large = list + nlist_ptr[0];   // the large class follows the small one
+  const uint32_t nlist = nlist_ptr[2];
+  const uint32_t* large = list + nlist_ptr[0] + nlist_ptr[1];   // the large class follows the small and medium ones
   for (uint32_t k = blockIdx.x * wpb + wib; k < nlist; k += gridDim.x * wpb) {
     const uint32_t p = large[k];
     const uint32_t a = pa[p], b = pb[p];
@@ -627,7 +634,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "refine: more than 2^32 pairs in one launch");
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
-  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np);
+  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, MCAP);
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
@@ -636,12 +643,19 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
   GSJ_HIP(hipEventRecord(c.ev[0], st));
   {   // small: 16-lane groups, 16 pairs per block
-    const size_t sm = (size_t)(T / 16) * small_bytes(rel);
+    const size_t sm = (size_t)(T / 16) * small_bytes<VCAP>(rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 16), max_blocks);
-    hipLaunchKernelGGL(k_refine_small<16>, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, ovf, ctl);
+    hipLaunchKernelGGL((k_refine_small<16, VCAP, 0>), dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[1], st));
+  {   // medium: 32-lane groups, 8 pairs per block
+    const size_t sm = (size_t)(T / 32) * small_bytes<MCAP>(rel);
+    const uint32_t g = std::min<uint32_t>(div_up(np, T / 32), max_blocks);
+    hipLaunchKernelGGL((k_refine_small<32, MCAP, 1>), dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, ovf, ctl);
+    GSJ_LAUNCHED();
+  }
+  GSJ_HIP(hipEventRecord(c.ev[5], st));
   constexpr int CAP_LARGE = 128;
   {   // large: one wave per pair, 4 pairs per block
     const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
@@ -650,18 +664,19 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], st));
-  struct Host { uint32_t counts[2]; OverflowCtl ctl; };
+  struct Host { uint32_t counts[3]; OverflowCtl ctl; };
   Host* h = c.h_scal.as<Host>(2);
-  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 8, hipMemcpyDeviceToHost, st));
+  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 12, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipMemcpyAsync(&h->ctl, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
   const uint32_t nover = h->ctl.count, need = h->ctl.need;
   if (stats) {
-    float ms_s = 0, ms_l = 0;
+    float ms_s = 0, ms_m = 0, ms_l = 0;
     GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
-    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[1], c.ev[2]));
-    stats->k_small_ms += ms_s; stats->k_large_ms += ms_l;
-    stats->n_small += h->counts[0]; stats->n_large += h->counts[1]; stats->n_over += nover;
+    GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
+    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[5], c.ev[2]));
+    stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_large_ms += ms_l;
+    stats->n_small += h->counts[0]; stats->n_medium += h->counts[1]; stats->n_large += h->counts[2]; stats->n_over += nover;
     stats->launches++;
   }
   if (nover) {
@@ -685,46 +700,55 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
 }
 
 namespace {
-// class flag per candidate: 1 = small (both polygons fit the LDS vertex lists)
+// class flags per candidate: fsmall = both polygons within VCAP stored vertices, fmed = not small and both within
+// mcap (0: no medium class); rings within RCAP for both
 __global__ void k_class_flags(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                              uint32_t* __restrict__ fsmall) {
+                              uint32_t mcap, uint32_t* __restrict__ fsmall, uint32_t* __restrict__ fmed) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
-  fsmall[i] = ((ia & 0xFFFFu) <= VCAP && (ib & 0xFFFFu) <= VCAP && (ia >> 16) <= RCAP && (ib >> 16) <= RCAP) ? 1u : 0u;
+  const uint32_t va = ia & 0xFFFFu, vb = ib & 0xFFFFu;
+  const bool rings = (ia >> 16) <= RCAP && (ib >> 16) <= RCAP;
+  const bool sm = rings && va <= VCAP && vb <= VCAP;
+  fsmall[i] = sm ? 1u : 0u;
+  fmed[i] = (!sm && rings && va <= mcap && vb <= mcap) ? 1u : 0u;
 }
 
-// stable partition from the scan of the small flags: small pairs first, large pairs after
-__global__ void k_partition(const uint32_t* __restrict__ fsmall, const uint64_t* __restrict__ off, uint64_t np, const uint64_t* __restrict__ nsmall_p,
-                            uint32_t* __restrict__ list, uint32_t* __restrict__ counts) {
+// stable partition from the scans of the flags: small pairs, then medium, then large, each in candidate order
+__global__ void k_partition(const uint32_t* __restrict__ fsmall, const uint32_t* __restrict__ fmed, const uint64_t* __restrict__ offs,
+                            const uint64_t* __restrict__ offm, uint64_t np, const uint64_t* __restrict__ tot, uint32_t* __restrict__ list,
+                            uint32_t* __restrict__ counts) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t nsmall = *nsmall_p;
-  if (i == 0) { counts[0] = (uint32_t)nsmall; counts[1] = (uint32_t)(np - nsmall); }
+  const uint64_t nsmall = tot[0], nmed = tot[1];
+  if (i == 0) { counts[0] = (uint32_t)nsmall; counts[1] = (uint32_t)nmed; counts[2] = (uint32_t)(np - nsmall - nmed); }
   if (i >= np) return;
-  const uint64_t o = off[i];
-  list[fsmall[i] ? o : nsmall + (i - o)] = (uint32_t)i;
+  const uint64_t os = offs[i], om = offm[i];
+  list[fsmall[i] ? os : (fmed[i] ? nsmall + om : nsmall + nmed + (i - os - om))] = (uint32_t)i;
 }
 
 __global__ void k_ctl_reset(OverflowCtl* ctl) { ctl->count = 0; ctl->need = 0; }
 }  // namespace
 
-PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np) {
+PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap) {
   hipStream_t st = c.stream;
-  uint32_t* fsmall = c.s_cls.as<uint32_t>(2 * np + 64);
-  uint64_t* off = c.s_tmp2.as<uint64_t>(np + 16);
+  uint32_t* fsmall = c.s_cls.as<uint32_t>(3 * np + 96);
+  uint32_t* fmed = fsmall + np + 32;
+  uint64_t* offs = c.s_tmp2.as<uint64_t>(2 * np + 32);
+  uint64_t* offm = offs + np + 16;
   char* misc = c.s_misc.as<char>(256);
   PairClasses pc;
-  pc.list = fsmall + np + 32;
+  pc.list = fmed + np + 32;
   pc.counts = reinterpret_cast<uint32_t*>(misc + 64);
   pc.ctl = reinterpret_cast<OverflowCtl*>(misc + 128);
-  uint64_t* d_nsmall = reinterpret_cast<uint64_t*>(misc);
+  uint64_t* tot = reinterpret_cast<uint64_t*>(misc);   // [0] small, [1] medium
   hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, pc.ctl);
   GSJ_LAUNCHED();
   if (np == 0) return pc;
-  hipLaunchKernelGGL(k_class_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, fsmall);
+  hipLaunchKernelGGL(k_class_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, fsmall, fmed);
   GSJ_LAUNCHED();
-  scan_u32_to_u64_async(c, fsmall, off, np, d_nsmall);
-  hipLaunchKernelGGL(k_partition, dim3(div_up(np, 256)), dim3(256), 0, st, fsmall, off, np, d_nsmall, pc.list, pc.counts);
+  scan_u32_to_u64_async(c, fsmall, offs, np, tot);
+  scan_u32_to_u64_async(c, fmed, offm, np, tot + 1);
+  hipLaunchKernelGGL(k_partition, dim3(div_up(np, 256)), dim3(256), 0, st, fsmall, fmed, offs, offm, np, tot, pc.list, pc.counts);
   GSJ_LAUNCHED();
   return pc;
 }

@@ -71,6 +71,19 @@ def test_mesh_self_relate(ctx, upper):
     _check_rows(got, exp, area=True, code=True)
 
 
+@pytest.mark.parametrize("want_area,want_relate", [(True, True), (True, False), (False, True)])
+def test_every_work_class_matches_oracle(ctx, want_area, want_relate):
+    # median 24 stored vertices: pairs in the 16-lane (<= 32), 32-lane (<= 64) and wave (larger) classes
+    M = _mesh(70, seed=8, median=24.0)
+    L = ctx.polys(M)
+    got = ctx.join(L, L, want_area=want_area, want_relate=want_relate, upper=True)
+    assert min(got.stats["n_small"], got.stats["n_medium"], got.stats["n_large"]) > 1000
+    OM = O.Layer.polys(M)
+    exp = O.join(OM, OM, want_area=want_area, want_relate=want_relate, upper=True)
+    assert got.stats["candidates"] == exp["candidates"]
+    _check_rows(got, exp, area=want_area, code=want_relate)
+
+
 @pytest.mark.parametrize("maxv", [8, 64])
 def test_mesh_subdivided_relate(ctx, maxv):
     M = gsj.subdivide(_mesh(100, seed=5), maxv)
