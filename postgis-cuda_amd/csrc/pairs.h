@@ -170,8 +170,9 @@ constexpr int MCAP = 64;             // medium class: stored vertices per polygo
 
 // Candidate pairs split by work class, on the device: list[0 .. counts[0]) the small pairs (both polygons within
 // VCAP / RCAP), then counts[1] medium pairs (not small, both within mcap / RCAP; none when mcap is 0), then the
-// counts[2] others, each class in candidate order; ctl cleared for the overflow queue. Uses the context's s_cls,
-// s_tmp2 and s_misc scratch.
+// counts[2] others; within a class by mean vertex count, then candidate order; ctl cleared for the overflow queue.
+// Uses the context's s_keys, s_keys2, s_vals, s_vals2, s_tmp2 and s_misc scratch (the list lives in one of the
+// s_vals buffers until their next use).
 struct PairClasses {
   uint32_t* list;
   uint32_t* counts;

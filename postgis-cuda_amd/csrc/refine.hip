@@ -700,56 +700,64 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
 }
 
 namespace {
-// class flags per candidate: fsmall = both polygons within VCAP stored vertices, fmed = not small and both within
-// mcap (0: no medium class); rings within RCAP for both
-__global__ void k_class_flags(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                              uint32_t mcap, uint32_t* __restrict__ fsmall, uint32_t* __restrict__ fmed) {
+// sort key per candidate: class (0 small: both polygons within VCAP stored vertices; 1 medium: not small, both within
+// mcap, none when mcap is 0; 2 large; rings within RCAP for both) in bits 6-7 and the mean stored vertex count
+// (saturated at 63) below, so that the groups of a wave get pairs of similar work
+__global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
+                             uint32_t mcap, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
   const uint32_t va = ia & 0xFFFFu, vb = ib & 0xFFFFu;
   const bool rings = (ia >> 16) <= RCAP && (ib >> 16) <= RCAP;
-  const bool sm = rings && va <= VCAP && vb <= VCAP;
-  fsmall[i] = sm ? 1u : 0u;
-  fmed[i] = (!sm && rings && va <= mcap && vb <= mcap) ? 1u : 0u;
+  const uint32_t cls = (rings && va <= VCAP && vb <= VCAP) ? 0u : ((rings && va <= mcap && vb <= mcap) ? 1u : 2u);
+  key[i] = (uint64_t)((cls << 6) | min(63u, (va + vb) >> 1));
+  val[i] = (uint32_t)i;
 }
 
-// stable partition from the scans of the flags: small pairs, then medium, then large, each in candidate order
-__global__ void k_partition(const uint32_t* __restrict__ fsmall, const uint32_t* __restrict__ fmed, const uint64_t* __restrict__ offs,
-                            const uint64_t* __restrict__ offm, uint64_t np, const uint64_t* __restrict__ tot, uint32_t* __restrict__ list,
-                            uint32_t* __restrict__ counts) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t nsmall = tot[0], nmed = tot[1];
-  if (i == 0) { counts[0] = (uint32_t)nsmall; counts[1] = (uint32_t)nmed; counts[2] = (uint32_t)(np - nsmall - nmed); }
-  if (i >= np) return;
-  const uint64_t os = offs[i], om = offm[i];
-  list[fsmall[i] ? os : (fmed[i] ? nsmall + om : nsmall + nmed + (i - os - om))] = (uint32_t)i;
+// class sizes from the sorted keys: first index of class 1 and of class 2 by binary search
+__global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, uint32_t* __restrict__ counts) {
+  uint64_t first[2];
+  for (int c = 1; c <= 2; c++) {
+    uint64_t lo = 0, hi = np;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      if ((key[mid] >> 6) < (uint64_t)c) lo = mid + 1;
+      else hi = mid;
+    }
+    first[c - 1] = lo;
+  }
+  counts[0] = (uint32_t)first[0];
+  counts[1] = (uint32_t)(first[1] - first[0]);
+  counts[2] = (uint32_t)(np - first[1]);
 }
 
-__global__ void k_ctl_reset(OverflowCtl* ctl) { ctl->count = 0; ctl->need = 0; }
+__global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
+  ctl->count = 0; ctl->need = 0;
+  counts[0] = counts[1] = counts[2] = 0;
+}
 }  // namespace
 
 PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap) {
   hipStream_t st = c.stream;
-  uint32_t* fsmall = c.s_cls.as<uint32_t>(3 * np + 96);
-  uint32_t* fmed = fsmall + np + 32;
-  uint64_t* offs = c.s_tmp2.as<uint64_t>(2 * np + 32);
-  uint64_t* offm = offs + np + 16;
-  char* misc = c.s_misc.as<char>(256);
+  char* misc = c.s_misc.as<char>(256);   // [0, 32) belongs to the scan / sort primitives
   PairClasses pc;
-  pc.list = fmed + np + 32;
   pc.counts = reinterpret_cast<uint32_t*>(misc + 64);
   pc.ctl = reinterpret_cast<OverflowCtl*>(misc + 128);
-  uint64_t* tot = reinterpret_cast<uint64_t*>(misc);   // [0] small, [1] medium
-  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, pc.ctl);
+  pc.list = nullptr;
+  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, st, pc.ctl, pc.counts);
   GSJ_LAUNCHED();
   if (np == 0) return pc;
-  hipLaunchKernelGGL(k_class_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, fsmall, fmed);
+  uint64_t* keys = c.s_keys.as<uint64_t>(np + 16);
+  uint64_t* keys2 = c.s_keys2.as<uint64_t>(np + 16);
+  uint32_t* vals = c.s_vals.as<uint32_t>(np + 16);
+  uint32_t* vals2 = c.s_vals2.as<uint32_t>(np + 16);
+  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, keys, vals);
   GSJ_LAUNCHED();
-  scan_u32_to_u64_async(c, fsmall, offs, np, tot);
-  scan_u32_to_u64_async(c, fmed, offm, np, tot + 1);
-  hipLaunchKernelGGL(k_partition, dim3(div_up(np, 256)), dim3(256), 0, st, fsmall, fmed, offs, offm, np, tot, pc.list, pc.counts);
+  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 8);   // stable: candidate order within a key
+  hipLaunchKernelGGL(k_class_counts, dim3(1), dim3(1), 0, st, keys, np, pc.counts);
   GSJ_LAUNCHED();
+  pc.list = vals;
   return pc;
 }
 
