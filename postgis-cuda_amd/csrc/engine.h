@@ -86,7 +86,7 @@ struct Context {
   // scratch, grow-only, reused across calls
   DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
       s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc, s_win, s_qidx,
-      s_win_off;
+      s_win_off, s_edgeq;
   // erosion pass 2 (it runs inside the join's on-device hook, beside the join's own scratch)
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;

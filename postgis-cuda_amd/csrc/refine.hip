@@ -396,47 +396,62 @@ __device__ __forceinline__ int probe_small(const SmallPoly<CAP>& sp, int r, bool
   return real ? 2 : 1;
 }
 
-// vertex-list classes: a G-lane group per pair, both polygons in LDS (at most CAP stored vertices each); class
-// WHICH of the partition (0 small, 1 medium)
+// Vertex-list classes: a G-lane group per pair, both polygons in LDS (at most CAP stored vertices each); class
+// WHICH of the partition (0 small, 1 medium). Two kernels: the screen decides contact and containment for every pair
+// and finishes the pairs that need no edge contributions; the pairs that do are queued for the edge kernel, so that
+// its waves hold only pairs with edge work (the screen's verdict travels in isect[p]).
+constexpr uint8_t ST_HIT = 1, ST_RTOUCH = 2, ST_INA = 4, ST_INB = 8;
+
+struct EdgeQueue {   // pairs with edge work: flag[k] per position k of the class list (set by the screen), then the
+  uint32_t* flag;    // flagged pairs in class-list order: class c's at list[first[c] .. first[c] + count[c])
+  uint64_t* off;
+  uint32_t* list;
+  uint32_t* count;   // [2]
+  uint32_t* first;   // [2]
+};
+
+template <int G, int CAP>
+__device__ __forceinline__ void load_pair(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, const PairSetup& f, SmallPoly<CAP>& SA,
+                                          SmallPoly<CAP>& SB, int gl) {
+  load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
+  load_small<G, CAP>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
+}
+
 template <int G, int CAP, int WHICH>
-__global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
-                                                         const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
-                                                         RefineOut out, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+__global__ void __launch_bounds__(256) k_screen_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                      const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
+                                                      RefineOut out, EdgeQueue q) {
   extern __shared__ __align__(16) unsigned char smem[];
   using SP = SmallPoly<CAP>;
   const int gpb = blockDim.x / G;
   const int gib = threadIdx.x / G, gl = threadIdx.x % G;
   const bool rel = rp.want_relate != 0;
-  unsigned char* base = smem + (size_t)gib * small_bytes<CAP>(rel);
-  SP& SA = *reinterpret_cast<SP*>(base);
-  SP& SB = *(reinterpret_cast<SP*>(base) + 1);
-  double* acc = reinterpret_cast<double*>(base + ((2 * sizeof(SP) + 15) & ~size_t(15)));
-  double* lin = acc + 2 * CAP;
-  double* lon = lin + 2 * CAP;
+  SP& SA = *reinterpret_cast<SP*>(smem + (size_t)gib * 2 * sizeof(SP));
+  SP& SB = *(&SA + 1);
   const uint32_t nlist = counts[WHICH];
-  const uint32_t* cl = list + (WHICH == 0 ? 0u : counts[0]);
+  const uint32_t base = WHICH == 0 ? 0u : counts[0];
+  const uint32_t* cl = list + base;
   for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
     const uint32_t p = cl[k];
     const uint32_t a = pa[p], b = pb[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
-    load_small<G, CAP>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
+    load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
     const int na = SA.ni, nb = SB.ni;   // edges meeting the overlap box
     const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SB.V, SB.F, SB.I, nb};
     // boundary contact over the A x B edge grid
     bool hit = false, hit_any = false;
     const int grid = na * nb;
     for (int q0 = 0; q0 < grid; q0 += G) {
-      const int q = q0 + gl;
-      if (q < grid) {
+      const int qi = q0 + gl;
+      if (qi < grid) {
         Ed ea, eb;
-        if (LA.get(q / nb, ea) && LB.get(q % nb, eb)) {
-          if (!(fmax(eb.x0, eb.x1) < fmin(ea.x0, ea.x1) || fmin(eb.x0, eb.x1) > fmax(ea.x0, ea.x1) || fmax(eb.y0, eb.y1) < fmin(ea.y0, ea.y1) ||
-                fmin(eb.y0, eb.y1) > fmax(ea.y0, ea.y1)) &&
-              seg_isect(ea.x0, ea.y0, ea.x1, ea.y1, eb.x0, eb.y0, eb.x1, eb.y1)) {
-            hit_any = true;
-            if (!(ea.syn | eb.syn)) hit = true;
-          }
+        LA.get(qi / nb, ea);
+        LB.get(qi % nb, eb);
+        if (!(fmax(eb.x0, eb.x1) < fmin(ea.x0, ea.x1) || fmin(eb.x0, eb.x1) > fmax(ea.x0, ea.x1) || fmax(eb.y0, eb.y1) < fmin(ea.y0, ea.y1) ||
+              fmin(eb.y0, eb.y1) > fmax(ea.y0, ea.y1)) &&
+            seg_isect(ea.x0, ea.y0, ea.x1, ea.y1, eb.x0, eb.y0, eb.x1, eb.y1)) {
+          hit_any = true;
+          if (!(ea.syn | eb.syn)) hit = true;
         }
       }
       if (gany<G>(hit)) break;
@@ -467,30 +482,73 @@ __global__ void __launch_bounds__(256, 3) k_refine_small(const DPolys P, const D
       hit = gany<G>(h);
       contain_any = gany<G>(ha);
     }
-    const bool computed = (hit || hit_any || contain_any) && (rp.want_area || rel);
-    uint32_t bits = 0;
-    if (computed) {
-      EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, inB_c, rel}, cb = ca;
-      cb.in_c = inA_c;
-      for (int s = gl; s < na + nb; s += G) {
-        const bool sideA = s < na;
-        Ed e;
-        Contrib c{0.0, 0.0, 0.0, 0u};
-        if ((sideA ? LA : LB).get(sideA ? s : s - na, e))
-          c = sideA ? edge_contrib<VertSrc, false>(e, LB, ca) : edge_contrib<VertSrc, false>(e, LA, cb);
-        acc[s] = c.acc;
-        if (rel) { lin[s] = c.lin; lon[s] = c.lon; }
-        bits |= c.bits;
-      }
-      wave_lds_sync();
-      if (gany<G>(bits & SPILL)) {
-        if (gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
-        continue;
-      }
+    const bool computed = (hit || hit_any || contain_any) && (rp.want_area || rel) && na + nb > 0;
+    if (!computed) {
+      finish_pair<G>(gl, p, hit, rtouch, false, rel, 0, 0, nullptr, nullptr, nullptr, 0u, rp, out);
+    } else if (gl == 0) {
+      out.isect[p] = (hit ? ST_HIT : 0) | (rtouch ? ST_RTOUCH : 0) | (inA_c ? ST_INA : 0) | (inB_c ? ST_INB : 0);
     }
-    finish_pair<G>(gl, p, hit, rtouch, computed, rel, na + nb, na, acc, lin, lon, bits, rp, out);
+    if (gl == 0) q.flag[base + k] = computed ? 1u : 0u;
     wave_lds_sync();
   }
+}
+
+template <int G, int CAP, int WHICH>
+__global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                        const uint32_t* __restrict__ counts, RefineParams rp, RefineOut out, EdgeQueue q,
+                                                        uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using SP = SmallPoly<CAP>;
+  const int gpb = blockDim.x / G;
+  const int gib = threadIdx.x / G, gl = threadIdx.x % G;
+  const bool rel = rp.want_relate != 0;
+  unsigned char* gbase = smem + (size_t)gib * small_bytes<CAP>(rel);
+  SP& SA = *reinterpret_cast<SP*>(gbase);
+  SP& SB = *(reinterpret_cast<SP*>(gbase) + 1);
+  double* acc = reinterpret_cast<double*>(gbase + ((2 * sizeof(SP) + 15) & ~size_t(15)));
+  double* lin = acc + 2 * CAP;
+  double* lon = lin + 2 * CAP;
+  const uint32_t nlist = q.count[WHICH];
+  const uint32_t* cl = q.list + q.first[WHICH];
+  for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
+    const uint32_t p = cl[k];
+    const uint32_t a = pa[p], b = pb[p];
+    const uint8_t st = out.isect[p];
+    const PairSetup f = setup_pair(P, R, a, b, rp);
+    load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
+    const int na = SA.ni, nb = SB.ni;
+    const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SB.V, SB.F, SB.I, nb};
+    EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, (st & ST_INB) != 0, rel}, cb = ca;
+    cb.in_c = (st & ST_INA) != 0;
+    uint32_t bits = 0;
+    for (int s = gl; s < na + nb; s += G) {
+      const bool sideA = s < na;
+      Ed e;
+      (sideA ? LA : LB).get(sideA ? s : s - na, e);
+      const Contrib c = sideA ? edge_contrib<VertSrc, false>(e, LB, ca) : edge_contrib<VertSrc, false>(e, LA, cb);
+      acc[s] = c.acc;
+      if (rel) { lin[s] = c.lin; lon[s] = c.lon; }
+      bits |= c.bits;
+    }
+    wave_lds_sync();
+    if (gany<G>(bits & SPILL)) {
+      if (gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
+    } else {
+      finish_pair<G>(gl, p, (st & ST_HIT) != 0, (st & ST_RTOUCH) != 0, true, rel, na + nb, na, acc, lin, lon, bits, rp, out);
+    }
+    wave_lds_sync();
+  }
+}
+
+// the flagged pairs of the vertex-list classes in class-list order (stable compaction of the screen's flags)
+__global__ void k_queue_scatter(const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, EdgeQueue q) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = (uint64_t)counts[0] + counts[1];
+  if (k == 0) {
+    const uint32_t n0 = (uint32_t)q.off[counts[0]], n1 = (uint32_t)(q.off[n] - q.off[counts[0]]);
+    q.first[0] = 0; q.count[0] = n0; q.first[1] = n0; q.count[1] = n1;
+  }
+  if (k < n && q.flag[k]) q.list[q.off[k]] = list[k];
 }
 
 // ------------------------------------------------------------------------------------------- large / overflow
@@ -642,19 +700,32 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   const int T = 256;
   const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
   GSJ_HIP(hipEventRecord(c.ev[0], st));
-  {   // small: 16-lane groups, 16 pairs per block
-    const size_t sm = (size_t)(T / 16) * small_bytes<VCAP>(rel);
-    const uint32_t g = std::min<uint32_t>(div_up(np, T / 16), max_blocks);
-    hipLaunchKernelGGL((k_refine_small<16, VCAP, 0>), dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, ovf, ctl);
-    GSJ_LAUNCHED();
-  }
+  EdgeQueue eq;
+  eq.flag = c.s_edgeq.as<uint32_t>(2 * np + 32);
+  eq.list = eq.flag + np + 16;
+  eq.off = c.s_tmp2.as<uint64_t>(np + 17);
+  eq.count = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(counts) + 32);
+  eq.first = eq.count + 2;
+  const uint32_t gs = std::min<uint32_t>(div_up(np, T / 16), max_blocks), gm = std::min<uint32_t>(div_up(np, T / 32), max_blocks);
+  // screens (every small and medium pair), then the edge work of the flagged ones in class-list order
+  hipLaunchKernelGGL((k_screen_small<16, VCAP, 0>), dim3(gs), dim3(T), (T / 16) * 2 * sizeof(SmallPoly<VCAP>), st, P, R, pa, pb, list, counts, rp,
+                     out, eq);
+  GSJ_LAUNCHED();
+  hipLaunchKernelGGL((k_screen_small<32, MCAP, 1>), dim3(gm), dim3(T), (T / 32) * 2 * sizeof(SmallPoly<MCAP>), st, P, R, pa, pb, list, counts, rp,
+                     out, eq);
+  GSJ_LAUNCHED();
+  // over np + 1 positions so that off[n] exists for n = np; flags past the two classes are stale and only reach
+  // offsets that are never read
+  scan_u32_to_u64_async(c, eq.flag, eq.off, np + 1, nullptr);
+  hipLaunchKernelGGL(k_queue_scatter, dim3(div_up(np, 256)), dim3(256), 0, st, list, counts, eq);
+  GSJ_LAUNCHED();
+  hipLaunchKernelGGL((k_edges_small<16, VCAP, 0>), dim3(gs), dim3(T), (T / 16) * small_bytes<VCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
+                     ovf, ctl);
+  GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[1], st));
-  {   // medium: 32-lane groups, 8 pairs per block
-    const size_t sm = (size_t)(T / 32) * small_bytes<MCAP>(rel);
-    const uint32_t g = std::min<uint32_t>(div_up(np, T / 32), max_blocks);
-    hipLaunchKernelGGL((k_refine_small<32, MCAP, 1>), dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, ovf, ctl);
-    GSJ_LAUNCHED();
-  }
+  hipLaunchKernelGGL((k_edges_small<32, MCAP, 1>), dim3(gm), dim3(T), (T / 32) * small_bytes<MCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
+                     ovf, ctl);
+  GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[5], st));
   constexpr int CAP_LARGE = 128;
   {   // large: one wave per pair, 4 pairs per block
@@ -734,7 +805,7 @@ __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, ui
 
 __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
   ctl->count = 0; ctl->need = 0;
-  counts[0] = counts[1] = counts[2] = 0;
+  for (int k = 0; k < 16; k++) counts[k] = 0;   // class sizes [0, 3) and the edge queues' counts [8, 10)
 }
 }  // namespace
 
