@@ -47,6 +47,25 @@ def piece_class(hs):
     return np.where(rings > 8, 2, np.where(verts <= 32, 0, np.where(verts <= 64, 1, 2)))
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "refine_relate_2.4M.md")
+
+
+def hbm_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` on this workload, from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE passes, each scaled by the copy calibration recorded there); None when the summary lacks the kernel.
+    Rows of the summary's traffic table read: | `kernel` | fetch KiB | write KiB | HBM bytes |."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+            try:
+                return int(float(cells[3]))
+            except ValueError:
+                return None
+    return None
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """Device-to-device copies of a known size through the HIP runtime (its copy kernel), so that the HBM byte
     counters of a profiling run can be scaled: FETCH_SIZE / WRITE_SIZE of these dispatches should read nbytes each."""
@@ -316,14 +335,17 @@ def main():
     ms_step = 1e3 * t_all / args.steps
     value = cand * args.steps / t_all
     log(f"step {ms_step:.2f} ms: {cand} candidates, {rows} rows; traverse {st['traverse_ms']:.2f} refine {st['refine_ms']:.2f} "
-        f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels small {st['k_small_ms']:.2f} medium {st['k_medium_ms']:.2f} "
-        f"large {st['k_large_ms']:.2f} over {st['k_over_ms']:.2f} ms (pairs {st['n_small']}/{st['n_medium']}/{st['n_large']}/{st['n_over']})")
+        f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels screen {st['k_screen_ms']:.2f} small {st['k_small_ms']:.2f} "
+        f"medium {st['k_medium_ms']:.2f} large {st['k_large_ms']:.2f} over {st['k_over_ms']:.2f} ms (pairs {st['n_small']}/{st['n_medium']}/"
+        f"{st['n_large']}/{st['n_over']}, edge work {st['n_small_edge']}/{st['n_medium_edge']})")
 
-    # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator)
-    classes = (("k_refine_small<16,32>", "k_small_ms", "n_small"), ("k_refine_small<32,64>", "k_medium_ms", "n_medium"),
-               ("k_refine_large", "k_large_ms", "n_large"))
-    kc = max(range(3), key=lambda i: st[classes[i][1]])
-    kern = classes[kc]
+    # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator):
+    # (name, duration field, pair class of the sample, pairs field, FLOPs counted: edge phase / screen part / all)
+    kernels = (("k_screen_small", "k_screen_ms", (0, 1), ("n_small", "n_medium"), "screen"),
+               ("k_edges_small<16, 32, 0>", "k_small_ms", (0,), ("n_small_edge",), "edge"),
+               ("k_edges_small<32, 64, 1>", "k_medium_ms", (1,), ("n_medium_edge",), "edge"),
+               ("k_refine_large", "k_large_ms", (2,), ("n_large",), "all"))
+    kern = max(kernels, key=lambda k: st[k[1]])
     roofline = None
     cpu = None
     parity = None
@@ -334,18 +356,25 @@ def main():
         Os = O.Layer.polys(Ss)
         cand_s = O.candidates(Os, Os, upper=True)
         pc = piece_class(Ss)
-        pick = cand_s[np.maximum(pc[cand_s[:, 0]], pc[cand_s[:, 1]]) == kc]
+        pick = cand_s[np.isin(np.maximum(pc[cand_s[:, 0]], pc[cand_s[:, 1]]), kern[2])]
         rng = np.random.default_rng(0)
         pick = pick[rng.choice(len(pick), size=min(len(pick), 40000), replace=False)]
         fl = O.refine_flops(Ss, Ss, pick, want_area=False, want_relate=True)
-        flops_per_pair = fl["flops"] / max(fl["pairs"], 1)
-        per_launch_pairs = st[kern[2]] / max(st["launches"], 1)
+        if kern[4] == "edge":     # per pair that reaches the edge contributions
+            flops_per_pair = fl["edge_flops"] / max(fl["edge_pairs"], 1)
+        elif kern[4] == "screen":   # contact + containment: everything but the edge contributions
+            flops_per_pair = (fl["flops"] - fl["edge_flops"]) / max(fl["pairs"], 1)
+        else:
+            flops_per_pair = fl["flops"] / max(fl["pairs"], 1)
+        per_launch_pairs = sum(st[f] for f in kern[3]) / max(st["launches"], 1)
         per_launch_ms = st[kern[1]] / max(st["launches"], 1)
         achieved = flops_per_pair * per_launch_pairs / (per_launch_ms * 1e-3) / 1e12
         roofline = {"kernel": kern[0], "bound": "valu", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None, "flops_per_pair": round(flops_per_pair, 1),
+                    "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": hbm_traffic(kern[0]), "flops_per_pair": round(flops_per_pair, 1),
                     "pairs_per_launch": int(per_launch_pairs), "launch_ms": round(per_launch_ms, 4),
-                    "flop_mix": {k: fl[k] for k in ("add", "mul", "fma", "div", "sqrt")}}
+                    "flops_counted": {"edge": "edge contributions of the pairs that need them", "screen": "contact and containment",
+                                      "all": "the whole refinement"}[kern[4]],
+                    "flop_mix": {k: (fl["edge"] if kern[4] == "edge" else fl)[k] for k in ("add", "mul", "fma", "div", "sqrt")}}
         # cpu baseline: the oracle's relate self-join of the sample on every host core
         threads = os.cpu_count() or 1
         O.set_threads(threads)
@@ -377,7 +406,7 @@ def main():
                    "pieces": S.n, "vertices": S.nv, "candidates_per_step": cand, "rows_per_step": rows, "parallelism": "1 GPU"},
         "rows_per_s": round(rows * args.steps / t_all, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
-        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_small_ms", "k_medium_ms", "k_large_ms", "k_over_ms")},
+        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1)},
         "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving,
     }

@@ -104,14 +104,19 @@ typedef struct {
   uint64_t hits;           /* result rows delivered (parent pairs) */
   uint64_t heavy;          /* pairs whose local edge lists exceeded on-chip capacity */
   uint64_t batches;
-  /* pair kernels, timed with HIP events on the context's stream: 16-lane class (both polygons <= 32 stored vertices
-   * and <= 8 rings; point joins: the per-pair kernel), 64-lane class, and the global-memory overflow pass; summed
-   * durations and pairs handled by each */
+  /* pair kernels, timed with HIP events on the context's stream, summed durations and pairs handled by each. Polygon
+   * joins: small = 16-lane class (both polygons <= 32 stored vertices and <= 8 rings), medium = 32-lane class (not
+   * small, both <= 64 stored vertices and <= 8 rings); these two classes are screened first (contact, containment;
+   * k_screen_ms, every small and medium pair) and k_small_ms / k_medium_ms time their edge-contribution kernels over
+   * the n_small_edge / n_medium_edge pairs that need one; large = a wave per pair; over = the global-memory overflow
+   * pass. Distance joins: small and large list classes. Point joins: the per-pair kernel as small. */
   double k_small_ms, k_large_ms, k_over_ms;
   uint64_t n_small, n_large, n_over;
   uint64_t launches;       /* refinement launches (sub-batches) */
-  double k_medium_ms;      /* 32-lane class (not small, both polygons <= 64 stored vertices and <= 8 rings) */
+  double k_medium_ms;
   uint64_t n_medium;
+  double k_screen_ms;
+  uint64_t n_small_edge, n_medium_edge;
 } gsj_join_stats;
 
 /* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;

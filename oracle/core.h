@@ -24,6 +24,8 @@ namespace orc {
 #ifdef ORC_COUNT_FLOPS
 struct FlopCount { uint64_t add = 0, mul = 0, fma = 0, div = 0, sqrt = 0; };
 static thread_local FlopCount g_fc;
+[[maybe_unused]] static thread_local FlopCount g_fc_edge;   // the part spent in edge contributions (area / relate terms)
+[[maybe_unused]] static thread_local uint64_t g_edge_pairs;  // pairs that reached the edge contributions
 struct Real {
   double v;
   Real() = default;

@@ -175,6 +175,16 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
   }
   Real ar = 0.0, e4[4] = {0, 0, 0, 0}; uint32_t eb4 = 0;
   if ((hit || hit_any || contain_any) && (rp.want_area || rp.want_relate)) {
+#ifdef ORC_COUNT_FLOPS
+    const FlopCount f0 = g_fc;
+    struct EdgePhase {   // adds this block's counts to the edge-phase counters when it ends
+      FlopCount f0;
+      ~EdgePhase() {
+        g_fc_edge.add += g_fc.add - f0.add; g_fc_edge.mul += g_fc.mul - f0.mul; g_fc_edge.fma += g_fc.fma - f0.fma;
+        g_fc_edge.div += g_fc.div - f0.div; g_fc_edge.sqrt += g_fc.sqrt - f0.sqrt; g_edge_pairs++;
+      }
+    } edge_phase{f0};
+#endif
     Real s2 = 0.0, ai = 0, ao = 0, bi = 0, bo = 0; uint32_t bits = rtouch ? REL_TOUCH : 0;
     const bool rel = rp.want_relate != 0;
     for (const Edge& e : LA) {
@@ -489,13 +499,16 @@ float orc_f_up(double v) { return f_up(v); }
 uint32_t orc_de9im_code(const double* ev, uint32_t bits, double la, double lb) { return de9im_code(ev, bits, la, lb); }
 
 // FLOP counters of this thread (counting build only; zeros otherwise): add/sub, mul, fma, div, sqrt
+// out[5 .. 10): the same for the edge-contribution phase alone, out[10]: pairs that reached it
 void orc_flops(uint64_t* out, int reset) {
 #ifdef ORC_COUNT_FLOPS
   out[0] = g_fc.add; out[1] = g_fc.mul; out[2] = g_fc.fma; out[3] = g_fc.div; out[4] = g_fc.sqrt;
-  if (reset) g_fc = FlopCount();
+  out[5] = g_fc_edge.add; out[6] = g_fc_edge.mul; out[7] = g_fc_edge.fma; out[8] = g_fc_edge.div; out[9] = g_fc_edge.sqrt;
+  out[10] = g_edge_pairs;
+  if (reset) { g_fc = FlopCount(); g_fc_edge = FlopCount(); g_edge_pairs = 0; }
 #else
   (void)reset;
-  for (int k = 0; k < 5; k++) out[k] = 0;
+  for (int k = 0; k < 11; k++) out[k] = 0;
 #endif
 }
 

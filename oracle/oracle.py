@@ -262,14 +262,19 @@ def refine_flops(hs_P, hs_R, pairs, want_area=True, want_relate=False):
     pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
     n = len(pairs)
     isect, area, ev, evb = np.zeros(n, np.uint8), np.zeros(n), np.zeros((n, 4)), np.zeros(n, np.uint32)
-    cnt = np.zeros(5, np.uint64)
+    cnt = np.zeros(11, np.uint64)
     lib.orc_flops(_p(cnt, C.c_uint64), 1)
     lib.orc_refine(P.h, R.h, _p(pairs, C.c_uint32), n, int(want_area), int(want_relate), 0, _p(isect, C.c_uint8), _p(area, C.c_double),
                    _p(ev, C.c_double), _p(evb, C.c_uint32))
     lib.orc_flops(_p(cnt, C.c_uint64), 1)
-    out = dict(zip(FLOP_KINDS, (int(v) for v in cnt)))
+    out = dict(zip(FLOP_KINDS, (int(v) for v in cnt[:5])))
     out["flops"] = out["add"] + out["mul"] + 2 * out["fma"] + out["div"] + out["sqrt"]
     out["pairs"] = n
+    # the edge-contribution phase alone, and the pairs that reached it
+    e = dict(zip(FLOP_KINDS, (int(v) for v in cnt[5:10])))
+    out["edge"] = e
+    out["edge_flops"] = e["add"] + e["mul"] + 2 * e["fma"] + e["div"] + e["sqrt"]
+    out["edge_pairs"] = int(cnt[10])
     return out
 
 

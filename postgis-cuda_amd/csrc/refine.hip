@@ -719,6 +719,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   scan_u32_to_u64_async(c, eq.flag, eq.off, np + 1, nullptr);
   hipLaunchKernelGGL(k_queue_scatter, dim3(div_up(np, 256)), dim3(256), 0, st, list, counts, eq);
   GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[6], st));
   hipLaunchKernelGGL((k_edges_small<16, VCAP, 0>), dim3(gs), dim3(T), (T / 16) * small_bytes<VCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
                      ovf, ctl);
   GSJ_LAUNCHED();
@@ -735,19 +736,21 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], st));
-  struct Host { uint32_t counts[3]; OverflowCtl ctl; };
+  struct Host { uint32_t counts[12]; OverflowCtl ctl; };   // class sizes [0, 3), edge queue sizes [8, 10)
   Host* h = c.h_scal.as<Host>(2);
-  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 12, hipMemcpyDeviceToHost, st));
+  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 48, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipMemcpyAsync(&h->ctl, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
   const uint32_t nover = h->ctl.count, need = h->ctl.need;
   if (stats) {
-    float ms_s = 0, ms_m = 0, ms_l = 0;
-    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
+    float ms_q = 0, ms_s = 0, ms_m = 0, ms_l = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms_q, c.ev[0], c.ev[6]));
+    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[6], c.ev[1]));
     GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
     GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[5], c.ev[2]));
-    stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_large_ms += ms_l;
+    stats->k_screen_ms += ms_q; stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_large_ms += ms_l;
     stats->n_small += h->counts[0]; stats->n_medium += h->counts[1]; stats->n_large += h->counts[2]; stats->n_over += nover;
+    stats->n_small_edge += h->counts[8]; stats->n_medium_edge += h->counts[9];
     stats->launches++;
   }
   if (nover) {

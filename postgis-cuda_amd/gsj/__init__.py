@@ -41,7 +41,8 @@ class JoinStats(C.Structure):
                 ("candidates", C.c_uint64), ("hits", C.c_uint64), ("heavy", C.c_uint64), ("batches", C.c_uint64),
                 ("k_small_ms", C.c_double), ("k_large_ms", C.c_double), ("k_over_ms", C.c_double),
                 ("n_small", C.c_uint64), ("n_large", C.c_uint64), ("n_over", C.c_uint64), ("launches", C.c_uint64),
-                ("k_medium_ms", C.c_double), ("n_medium", C.c_uint64)]
+                ("k_medium_ms", C.c_double), ("n_medium", C.c_uint64), ("k_screen_ms", C.c_double), ("n_small_edge", C.c_uint64),
+                ("n_medium_edge", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
