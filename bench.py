@@ -276,6 +276,7 @@ def main():
     ap.add_argument("--scale", type=int, default=1549, help="lattice cells per side (1549 = the 2.4M-parcel config)")
     ap.add_argument("--sample", type=int, default=400, help="cells per side of the CPU/parity sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline and parity legs (profiling runs)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="host work of the cpu_baseline leg (repeats of the join)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-vs-serial leg")
     ap.add_argument("--requests", type=int, default=256, help="requests in the served-vs-serial mix")
     ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
@@ -375,15 +376,23 @@ def main():
                     "flops_counted": {"edge": "edge contributions of the pairs that need them", "screen": "contact and containment",
                                       "all": "the whole refinement"}[kern[4]],
                     "flop_mix": {k: (fl["edge"] if kern[4] == "edge" else fl)[k] for k in ("add", "mul", "fma", "div", "sqrt")}}
-        # cpu baseline: the oracle's relate self-join of the sample on every host core
-        threads = os.cpu_count() or 1
+        # cpu baseline: the oracle's relate self-join of the same 2.4M-parcel layer on the host's share of cores,
+        # repeated until about --cpu-seconds of work
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
         O.set_threads(threads)
-        t1 = time.perf_counter()
+        Of = O.Layer.polys(S)
+        reps, t_cpu, n_cpu = 0, 0.0, 0
+        while reps == 0 or t_cpu < args.cpu_seconds:
+            t1 = time.perf_counter()
+            r = O.join(Of, Of, want_area=False, want_relate=True, upper=True)
+            t_cpu += time.perf_counter() - t1
+            n_cpu += r["candidates"]
+            reps += 1
+            del r
+        del Of
+        cpu = {"value": round(n_cpu / t_cpu, 1), "unit": "refined pairs/s", "cores": threads, "kind": "port",
+               "sample": f"the same {S.n}-piece relate self-join ({n_cpu // reps} candidates), run {reps} times in {t_cpu:.1f} s"}
         exp = O.join(Os, Os, want_area=False, want_relate=True, upper=True)
-        t_cpu = time.perf_counter() - t1
-        cpu = {"value": round(exp["candidates"] / t_cpu, 1), "unit": "refined pairs/s", "cores": threads, "kind": "port",
-               "sample": f"config-2 generator {args.sample}x{args.sample} cells ({Ss.n} pieces, {exp['candidates']} candidates), relate self-join, "
-                         f"{t_cpu:.2f} s"}
         # parity of the engine on the same sample
         Ls = ctx.polys(Ss)
         got = ctx.join(Ls, Ls, **opts)

@@ -278,11 +278,23 @@ __device__ __forceinline__ void piece_range(const uint32_t* pf, uint32_t p, uint
   else { g0 = p; g1 = p + 1; }
 }
 
-// X_A and X_B intersect within the window (buffer.cuh:502-588). One wave; returns 1 / 0, or -1 when a list overflows
-// (needE / needL: sizes required; needL is 0 when the edge lists already overflowed).
-__device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, uint32_t pa, uint32_t pb, double ra_nom,
-                       double rb_nom, const ErScratch& sc, int lane, int& needE, int& needL) {
-  ESide A{0, 0, 0, 1e300, 1e300, -1e300, -1e300}, B{0, 0, 0, 1e300, 1e300, -1e300, -1e300};
+// A parent pair made ready for the tests: the window and frame, both sides' edge and element lists in the scratch.
+struct ErPair {
+  ESide A, B;
+  EFrame F;
+  int na, nb, la, lb;
+  double Wx0, Wy0, Wx1, Wy1, eps, cx, cy;
+};
+
+// Window, frame, edge lists and element lists of a parent pair (buffer.cuh:502-541); one wave. Returns 1 when ready,
+// 0 when the window is empty (no intersection), -1 when a list overflows (needE / needL: sizes required; needL is 0
+// when the edge lists already overflowed).
+__device__ int er_setup(const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, uint32_t pa, uint32_t pb, double ra_nom,
+                        double rb_nom, const ErScratch& sc, int lane, ErPair& E, int& needE, int& needL) {
+  ESide& A = E.A;
+  ESide& B = E.B;
+  A = ESide{0, 0, 0, 1e300, 1e300, -1e300, -1e300};
+  B = ESide{0, 0, 0, 1e300, 1e300, -1e300, -1e300};
   piece_range(pfP, pa, A.g0, A.g1);
   piece_range(pfR, pb, B.g0, B.g1);
   if (A.g0 >= A.g1 || B.g0 >= B.g1) return 0;
@@ -305,75 +317,92 @@ __device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, co
   A.r = ra_nom > 0 ? ra_nom / geo::utm_k(0.5 * (A.bx0 + A.bx1), 0.5 * (A.by0 + A.by1), clon, clat) : 0.0;
   B.r = rb_nom > 0 ? rb_nom / geo::utm_k(0.5 * (B.bx0 + B.bx1), 0.5 * (B.by0 + B.by1), clon, clat) : 0.0;
   const double2 fo = P.origin[A.g0];
-  EFrame F{fo.x, fo.y, 0, 0};
+  EFrame& F = E.F;
+  F = EFrame{fo.x, fo.y, 0, 0};
   geo::m_per_deg(clat, F.kx, F.ky);
-  const double Wx0 = (wx0 - F.ox) * F.kx, Wx1 = (wx1 - F.ox) * F.kx, Wy0 = (wy0 - F.oy) * F.ky, Wy1 = (wy1 - F.oy) * F.ky;
-  const double ext = fmax(fmax(fabs(Wx0), fabs(Wx1)), fmax(fabs(Wy0), fabs(Wy1))) + fmax(A.r, B.r);
-  const double eps = 1e-9 + 1e-12 * ext;
+  E.Wx0 = (wx0 - F.ox) * F.kx; E.Wx1 = (wx1 - F.ox) * F.kx; E.Wy0 = (wy0 - F.oy) * F.ky; E.Wy1 = (wy1 - F.oy) * F.ky;
+  const double ext = fmax(fmax(fabs(E.Wx0), fabs(E.Wx1)), fmax(fabs(E.Wy0), fabs(E.Wy1))) + fmax(A.r, B.r);
+  E.eps = 1e-9 + 1e-12 * ext;
   // the frame metric is exact at the window's centre latitude; widen the collect margin for its error
   const double fe = 1.0 + 2.0 * 0.5 * (wy1 - wy0) * geo::DEG * (fabs(gm::tan(clat * geo::DEG)) + 0.05);
   const double ma = A.r * fe + 1e-6, mb = B.r * fe + 1e-6;
-  const int na = collect_side(P, A, F, Wx0 - ma, Wy0 - ma, Wx1 + ma, Wy1 + ma, sc.EA, sc.capE, lane);
-  const int nb = collect_side(R, B, F, Wx0 - mb, Wy0 - mb, Wx1 + mb, Wy1 + mb, sc.EB, sc.capE, lane);
-  if (na > sc.capE || nb > sc.capE) { needE = max(na, nb); needL = 0; return -1; }
+  E.na = collect_side(P, A, F, E.Wx0 - ma, E.Wy0 - ma, E.Wx1 + ma, E.Wy1 + ma, sc.EA, sc.capE, lane);
+  E.nb = collect_side(R, B, F, E.Wx0 - mb, E.Wy0 - mb, E.Wx1 + mb, E.Wy1 + mb, sc.EB, sc.capE, lane);
+  if (E.na > sc.capE || E.nb > sc.capE) { needE = max(E.na, E.nb); needL = 0; return -1; }
   wave_lds_sync();
-  prepare_side(sc.EA, na, A.r, lane);
-  prepare_side(sc.EB, nb, B.r, lane);
-  const int la = build_elements(sc.EA, na, A.r, Wx0, Wy0, Wx1, Wy1, eps, sc.LA, sc.capL, lane);
-  const int lb = build_elements(sc.EB, nb, B.r, Wx0, Wy0, Wx1, Wy1, eps, sc.LB, sc.capL, lane);
-  if (la > sc.capL || lb > sc.capL) { needE = max(na, nb); needL = max(la, lb); return -1; }
+  prepare_side(sc.EA, E.na, A.r, lane);
+  prepare_side(sc.EB, E.nb, B.r, lane);
+  E.la = build_elements(sc.EA, E.na, A.r, E.Wx0, E.Wy0, E.Wx1, E.Wy1, E.eps, sc.LA, sc.capL, lane);
+  E.lb = build_elements(sc.EB, E.nb, B.r, E.Wx0, E.Wy0, E.Wx1, E.Wy1, E.eps, sc.LB, sc.capL, lane);
+  if (E.la > sc.capL || E.lb > sc.capL) { needE = max(E.na, E.nb); needL = max(E.la, E.lb); return -1; }
+  E.cx = E.Wx0 + 0.381966 * (E.Wx1 - E.Wx0);
+  E.cy = E.Wy0 + 0.414213 * (E.Wy1 - E.Wy0);
   wave_lds_sync();
-  const BEd* EA = sc.EA;
-  const BEd* EB = sc.EB;
-  const BEl* LA = sc.LA;
-  const BEl* LB = sc.LB;
-  auto inW = [&](double x, double y) { return x >= Wx0 - eps && x <= Wx1 + eps && y >= Wy0 - eps && y <= Wy1 + eps; };
-  // (a) the eroded boundaries cross outside both exclusion zones
-  bool hit = false;
-  for (int i = lane; i < la && !hit; i += 64) {
-    const BEl p = LA[i];
+  return 1;
+}
+
+// (a) the eroded boundaries cross outside both exclusion zones: this thread's share (element i of A, i = t, t + NT, ...)
+__device__ bool er_cross(const ErScratch& sc, const ErPair& E, int t, int NT, const volatile int* stop) {
+  auto inW = [&](double x, double y) { return x >= E.Wx0 - E.eps && x <= E.Wx1 + E.eps && y >= E.Wy0 - E.eps && y <= E.Wy1 + E.eps; };
+  for (int i = t; i < E.la; i += NT) {
+    if (stop && *stop) return false;
+    const BEl p = sc.LA[i];
     const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
-    for (int j = 0; j < lb && !hit; j++) {
-      const BEl q = LB[j];
+    for (int j = 0; j < E.lb; j++) {
+      const BEl q = sc.LB[j];
       if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
       if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
       double xs[3], ys[3];
       const int k = seg_points(p, q, xs, ys);
-      for (int t = 0; t < k && !hit; t++)
-        if (inW(xs[t], ys[t]) && !in_excl(EA, na, A.r, xs[t], ys[t], eps) && !in_excl(EB, nb, B.r, xs[t], ys[t], eps)) hit = true;
+      for (int u = 0; u < k; u++)
+        if (inW(xs[u], ys[u]) && !in_excl(sc.EA, E.na, E.A.r, xs[u], ys[u], E.eps) && !in_excl(sc.EB, E.nb, E.B.r, xs[u], ys[u], E.eps)) return true;
     }
   }
-  if (__ballot(hit)) return 1;
-  // (b), (c): a vertex of one eroded boundary inside the other eroded polygon
-  const double cx = Wx0 + 0.381966 * (Wx1 - Wx0), cy = Wy0 + 0.414213 * (Wy1 - Wy0);
-  const bool inA_c = pip_parent(P, A, F, cx, cy, lane);
-  const bool inB_c = pip_parent(R, B, F, cx, cy, lane);
-  for (int side = 0; side < 2; side++) {
-    const BEd* ES = side ? EB : EA;
-    const BEd* EO = side ? EA : EB;
-    const BEl* LS = side ? LB : LA;
-    const int ns = side ? nb : na, no = side ? na : nb, ls = side ? lb : la;
-    const double rs = side ? B.r : A.r, ro = side ? A.r : B.r;
-    const bool inO_c = side ? inA_c : inB_c;
-    auto vertex_in_other = [&](double x, double y) {
-      return inW(x, y) && !in_excl(ES, ns, rs, x, y, eps) && inside_m(EO, no, cx, cy, inO_c, x, y) && !in_excl(EO, no, ro, x, y, eps);
-    };
-    for (int i = lane; i < ls && !hit; i += 64) {
-      const BEl p = LS[i];
-      if (vertex_in_other(p.x0, p.y0) || vertex_in_other(p.x1, p.y1)) { hit = true; break; }
-      const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
-      for (int j = i + 1; j < ls && !hit; j++) {
-        const BEl q = LS[j];
-        if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
-        if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
-        double xs[3], ys[3];
-        const int k = seg_points(p, q, xs, ys);
-        for (int t = 0; t < k && !hit; t++)
-          if (vertex_in_other(xs[t], ys[t])) hit = true;
-      }
+  return false;
+}
+
+// (b), (c) a vertex or self-crossing of one side's eroded boundary inside the other eroded polygon: this thread's share
+__device__ bool er_inside(const ErScratch& sc, const ErPair& E, bool inA_c, bool inB_c, int side, int t, int NT, const volatile int* stop) {
+  auto inW = [&](double x, double y) { return x >= E.Wx0 - E.eps && x <= E.Wx1 + E.eps && y >= E.Wy0 - E.eps && y <= E.Wy1 + E.eps; };
+  const BEd* ES = side ? sc.EB : sc.EA;
+  const BEd* EO = side ? sc.EA : sc.EB;
+  const BEl* LS = side ? sc.LB : sc.LA;
+  const int ns = side ? E.nb : E.na, no = side ? E.na : E.nb, ls = side ? E.lb : E.la;
+  const double rs = side ? E.B.r : E.A.r, ro = side ? E.A.r : E.B.r;
+  const bool inO_c = side ? inA_c : inB_c;
+  auto vertex_in_other = [&](double x, double y) {
+    return inW(x, y) && !in_excl(ES, ns, rs, x, y, E.eps) && inside_m(EO, no, E.cx, E.cy, inO_c, x, y) && !in_excl(EO, no, ro, x, y, E.eps);
+  };
+  for (int i = t; i < ls; i += NT) {
+    if (stop && *stop) return false;
+    const BEl p = LS[i];
+    if (vertex_in_other(p.x0, p.y0) || vertex_in_other(p.x1, p.y1)) return true;
+    const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
+    for (int j = i + 1; j < ls; j++) {
+      const BEl q = LS[j];
+      if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
+      if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
+      double xs[3], ys[3];
+      const int k = seg_points(p, q, xs, ys);
+      for (int u = 0; u < k; u++)
+        if (vertex_in_other(xs[u], ys[u])) return true;
     }
-    if (__ballot(hit)) return 1;
   }
+  return false;
+}
+
+// X_A and X_B intersect within the window (buffer.cuh:502-588). One wave; returns 1 / 0, or -1 when a list overflows
+// (needE / needL: sizes required; needL is 0 when the edge lists already overflowed).
+__device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, uint32_t pa, uint32_t pb, double ra_nom,
+                       double rb_nom, const ErScratch& sc, int lane, int& needE, int& needL) {
+  ErPair E;
+  const int st = er_setup(P, R, pfP, pfR, pa, pb, ra_nom, rb_nom, sc, lane, E, needE, needL);
+  if (st <= 0) return st;
+  if (__ballot(er_cross(sc, E, lane, 64, nullptr))) return 1;
+  const bool inA_c = pip_parent(P, E.A, E.F, E.cx, E.cy, lane);
+  const bool inB_c = pip_parent(R, E.B, E.F, E.cx, E.cy, lane);
+  for (int side = 0; side < 2; side++)
+    if (__ballot(er_inside(sc, E, inA_c, inB_c, side, lane, 64, nullptr))) return 1;
   return 0;
 }
 
@@ -402,29 +431,59 @@ __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, co
   }
 }
 
-// pairs that overflowed: per-wave lists in global memory of the size the queue reported; those that overflow the
-// element lists again are re-queued with their exact element count
-__global__ void __launch_bounds__(256) k_erode_slab(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP, const uint32_t* __restrict__ pfR,
-                                                    const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ items, uint32_t ni, double ra,
-                                                    double rb, uint8_t* __restrict__ flag, unsigned char* __restrict__ slab, int capE, int capL,
-                                                    uint32_t* __restrict__ again, ErCtl* ctl) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const ErScratch sc = er_carve(slab + wave * er_bytes(capE, capL), capE, capL);
-  for (uint64_t i = wave; i < ni; i += nwaves) {
+// pairs that overflowed: a block per pair (ER_SLAB_T threads) with lists in global memory of the size the queue
+// reported; its first wave builds the lists, then the whole block runs the crossing and containment tests; pairs that
+// overflow the element lists again are re-queued with their exact element count
+constexpr int ER_SLAB_T = 512;
+__global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
+                                                          const uint32_t* __restrict__ pfR, const uint32_t* __restrict__ pairs,
+                                                          const uint32_t* __restrict__ items, uint32_t ni, double ra, double rb,
+                                                          uint8_t* __restrict__ flag, unsigned char* __restrict__ slab, int capE, int capL,
+                                                          uint32_t* __restrict__ again, ErCtl* ctl) {
+  __shared__ ErPair SE;
+  __shared__ int s_state, s_hit, s_inA, s_inB;
+  const int t = threadIdx.x, lane = t & 63;
+  const ErScratch sc = er_carve(slab + (uint64_t)blockIdx.x * er_bytes(capE, capL), capE, capL);
+  for (uint32_t i = blockIdx.x; i < ni; i += gridDim.x) {
     const uint32_t p = items[i];
-    int ne = 0, nl = 0;
-    const int r = er_pair(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
-    if (lane == 0) {
-      flag[p] = r > 0 ? 1 : 0;
-      if (r < 0) {
-        again[atomicAdd(&ctl->n, 1u)] = p;
-        atomicMax(&ctl->needE, ne);
-        atomicMax(&ctl->needL, nl);
+    if (t < 64) {   // first wave: window, frame and lists
+      ErPair E;
+      int ne = 0, nl = 0;
+      const int st = er_setup(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, E, ne, nl);
+      if (lane == 0) {
+        SE = E;
+        s_state = st;
+        s_hit = 0;
+        if (st < 0) {
+          flag[p] = 0;
+          again[atomicAdd(&ctl->n, 1u)] = p;
+          atomicMax(&ctl->needE, ne);
+          atomicMax(&ctl->needL, nl);
+        } else if (st == 0) {
+          flag[p] = 0;
+        }
       }
     }
-    wave_lds_sync();
+    __syncthreads();
+    if (s_state > 0) {
+      const ErPair E = SE;
+      if (er_cross(sc, E, t, ER_SLAB_T, &s_hit)) s_hit = 1;
+      __syncthreads();
+      if (!s_hit) {
+        if (t < 64) {
+          const bool a = pip_parent(P, E.A, E.F, E.cx, E.cy, lane);
+          const bool b = pip_parent(R, E.B, E.F, E.cx, E.cy, lane);
+          if (lane == 0) { s_inA = a; s_inB = b; }
+        }
+        __syncthreads();
+        for (int side = 0; side < 2; side++) {
+          if (!s_hit && er_inside(sc, E, s_inA != 0, s_inB != 0, side, t, ER_SLAB_T, &s_hit)) s_hit = 1;
+          __syncthreads();
+        }
+      }
+      if (t == 0) flag[p] = s_hit ? 1 : 0;
+    }
+    __syncthreads();   // the lists and SE are rewritten for the next pair
   }
 }
 
@@ -585,12 +644,11 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
     for (int round = 0; nh; round++) {
       GSJ_REQUIRE(round < 4, GSJ_E_INTERNAL, "erosion: overflowing pairs did not converge");
       const size_t per = er_bytes(capE, capL);
-      uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nh, (1ull << 30) / per));
-      waves = (waves + 3) / 4 * 4;
-      unsigned char* slab = c.s_er_slab.as<unsigned char>(per * waves + 64);
+      const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(nh, (1ull << 30) / per));   // one list set per block
+      unsigned char* slab = c.s_er_slab.as<unsigned char>(per * blocks + 64);
       hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
       GSJ_HIP(hipEventRecord(c.ev[2], stream));
-      hipLaunchKernelGGL(k_erode_slab, dim3((uint32_t)(waves / 4)), dim3(256), 0, stream, P, R, pfP, pfR, ep, heavy, nh, ra, rb, flag, slab, capE,
+      hipLaunchKernelGGL(k_erode_slab, dim3((uint32_t)blocks), dim3(ER_SLAB_T), 0, stream, P, R, pfP, pfR, ep, heavy, nh, ra, rb, flag, slab, capE,
                          capL, again, ctl);
       GSJ_LAUNCHED();
       GSJ_HIP(hipEventRecord(c.ev[3], stream));
