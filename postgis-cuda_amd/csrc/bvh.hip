@@ -159,6 +159,24 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 
 }  // namespace
 
+void morton_order(Context& c, const float4* bb, uint32_t n, uint32_t* order) {
+  if (n == 0) return;
+  hipStream_t st = c.stream;
+  DevBuf t_keys, t_keys2, t_vals, t_vals2, t_misc;
+  uint64_t* keys = t_keys.as<uint64_t>(n);
+  uint64_t* keys2 = t_keys2.as<uint64_t>(n);
+  uint32_t* vals = t_vals.as<uint32_t>(n);
+  uint32_t* vals2 = t_vals2.as<uint32_t>(n);
+  int32_t* bounds = t_misc.as<int32_t>(16);
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, st, bounds);
+  hipLaunchKernelGGL(k_centre_bounds, dim3(std::min<uint32_t>(div_up(n, 256), 2048)), dim3(256), 0, st, bb, n, bounds);
+  hipLaunchKernelGGL(k_morton64, dim3(div_up(n, 256)), dim3(256), 0, st, bb, n, bounds, keys, vals);
+  GSJ_LAUNCHED();
+  radix_sort_pairs(c, keys, keys2, vals, vals2, n, 0, 64);
+  GSJ_HIP(hipMemcpyAsync(order, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  GSJ_HIP(hipStreamSynchronize(st));   // the temporaries are freed on return
+}
+
 void bvh_build(Layer& L) {
   Context& c = *L.ctx;
   hipStream_t st = c.stream;

@@ -39,6 +39,7 @@ struct DPoints {
   uint32_t n = 0;
   const double2* xy = nullptr;   // [n] world coordinates
   const float4* bb = nullptr;    // [n] outward-rounded float point boxes
+  const uint32_t* order = nullptr;   // [n] point ids in Morton order of their boxes (the probe walk order)
 };
 
 // LBVH node in depth-first (preorder) layout: the left child of an internal node is the next node; `skip` is the
@@ -112,6 +113,8 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
                         const uint32_t* parent, const uint8_t* syn);
 void layer_upload_points(Layer& L, const double* xy, uint64_t n);
 void bvh_build(Layer& L);
+// ids 0 .. n-1 of the boxes bb in the order of their centres' 64-bit Morton keys (stable), into order[n]
+void morton_order(Context& c, const float4* bb, uint32_t n, uint32_t* order);
 
 // ------------------------------------------------------------------------------------------- traversal
 // count pass (write == false) or write pass over probe boxes qbb[qbase .. qbase+nq)
@@ -124,6 +127,7 @@ void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbas
 struct Probes {
   const float4* qbb = nullptr;
   const float4* own_bb = nullptr;
+  const uint32_t* order = nullptr;   // probe walk order (ids at positions 0 .. n-1), or id order when null
   uint32_t n = 0;
   const std::vector<uint32_t>* h_parent = nullptr;
   const uint32_t* pfirst = nullptr;

@@ -46,11 +46,13 @@ __global__ void __launch_bounds__(256) k_traverse(const float4* __restrict__ qbb
   if (!WRITE) counts[t] = cnt;
 }
 
-// window membership of every probe: its own (outward float) box meets the closed window, compared in double
-__global__ void k_win_flags(const float4* __restrict__ bb, uint32_t n, double x0, double y0, double x1, double y1, uint32_t* __restrict__ f) {
+// window membership of the probe at every walk position: its own (outward float) box meets the closed window,
+// compared in double
+__global__ void k_win_flags(const float4* __restrict__ bb, const uint32_t* __restrict__ order, uint32_t n, double x0, double y0, double x1, double y1,
+                            uint32_t* __restrict__ f) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 b = bb[i];
+  const float4 b = bb[order ? order[i] : i];
   f[i] = ((double)b.x <= x1 && (double)b.z >= x0 && (double)b.y <= y1 && (double)b.w >= y0) ? 1u : 0u;
 }
 
@@ -66,9 +68,10 @@ __global__ void k_win_parents(const uint32_t* __restrict__ pfirst, uint32_t npar
     for (uint32_t i = a; i < b; i++) f[i] = 1u;
 }
 
-__global__ void k_win_scatter(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint32_t n, uint32_t* __restrict__ out) {
+__global__ void k_win_scatter(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, const uint32_t* __restrict__ order, uint32_t n,
+                              uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && f[i]) out[off[i]] = i;
+  if (i < n && f[i]) out[off[i]] = order ? order[i] : i;
 }
 
 }  // namespace
@@ -78,6 +81,7 @@ Probes probes_of(const Layer& L, const float4* qbb) {
   p.own_bb = L.points ? L.pts.bb : L.poly.bb;
   p.qbb = qbb ? qbb : p.own_bb;
   p.n = (uint32_t)L.n;
+  if (L.points) p.order = L.pts.order;
   if (!L.points && !L.h_parent.empty()) {
     p.h_parent = &L.h_parent;
     p.pfirst = L.poly.pfirst;
@@ -110,16 +114,17 @@ void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool 
   const float4* qbb = pr.qbb;
   static const std::vector<uint32_t> none;
   const std::vector<uint32_t>& hp = pr.h_parent ? *pr.h_parent : none;
-  // probe positions 0 .. nQ: probe ids directly, or through the window's member list (ascending ids)
+  // probe positions 0 .. nQ: probe ids directly, through the walk order, or through the window's member list (in walk
+  // order); a subdivided layer walks in id order (its parents' pieces are contiguous)
   uint32_t nQ = pr.n;
-  const uint32_t* qidx = nullptr;
+  const uint32_t* qidx = pr.h_parent && !pr.h_parent->empty() ? nullptr : pr.order;
   std::vector<uint32_t> hq;
   if (o.has_window && pr.n) {
     const double* w = o.window;
     GSJ_REQUIRE(w[0] <= w[2] && w[1] <= w[3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
     WallTimer tw;
     uint32_t* f = c.s_win.as<uint32_t>((uint64_t)pr.n + 16);
-    hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, pr.n, w[0], w[1], w[2], w[3], f);
+    hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, w[0], w[1], w[2], w[3], f);
     GSJ_LAUNCHED();
     if (pr.pfirst && pr.nparent) {
       hipLaunchKernelGGL(k_win_parents, dim3(div_up(pr.nparent, 256)), dim3(256), 0, stream, pr.pfirst, pr.nparent, f);
@@ -129,7 +134,7 @@ void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool 
     nQ = (uint32_t)scan_u32_to_u64(c, f, off, pr.n);
     // members written behind the flags (the flags are read by the scatter itself, so they need their own region)
     uint32_t* list = c.s_qidx.as<uint32_t>((uint64_t)nQ + 16);
-    hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, pr.n, list);
+    hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, qidx, pr.n, list);
     GSJ_LAUNCHED();
     hq.resize(nQ);
     if (nQ) GSJ_HIP(hipMemcpyAsync(hq.data(), list, (size_t)nQ * 4, hipMemcpyDeviceToHost, stream));
