@@ -82,6 +82,7 @@ Probes probes_of(const Layer& L, const float4* qbb) {
   p.qbb = qbb ? qbb : p.own_bb;
   p.n = (uint32_t)L.n;
   if (L.points) p.order = L.pts.order;
+  else if (L.h_parent.empty() && L.has_bvh) p.order = L.bvh.order;   // the tree's leaf order: Morton order of the boxes
   if (!L.points && !L.h_parent.empty()) {
     p.h_parent = &L.h_parent;
     p.pfirst = L.poly.pfirst;
