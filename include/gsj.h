@@ -109,7 +109,7 @@ typedef struct {
    * small, both <= 64 stored vertices and <= 8 rings); these two classes are screened first (contact, containment;
    * k_screen_ms, every small and medium pair) and k_small_ms / k_medium_ms time their edge-contribution kernels over
    * the n_small_edge / n_medium_edge pairs that need one; large = a wave per pair; over = the global-memory overflow
-   * pass. Distance joins: small and large list classes. Point joins: the per-pair kernel as small. */
+   * pass. Distance joins: small, medium and large list classes. Point joins: the per-pair kernel as small. */
   double k_small_ms, k_large_ms, k_over_ms;
   uint64_t n_small, n_large, n_over;
   uint64_t launches;       /* refinement launches (sub-batches) */
