@@ -39,12 +39,17 @@ def mesh(gsj, n, seed=1, subdiv=256):
     return M, gsj.subdivide(M, subdiv)
 
 
-def piece_class(hs):
-    """Per piece: the largest vertex-list refinement class it fits: 0 small (<= 32 stored vertices), 1 medium (<= 64),
-    2 large (more vertices or more than 8 rings). A pair's class is the larger of its two pieces'."""
+def pair_class(hs, pairs):
+    """Refinement work class of each candidate pair of a self-join of hs: 0 small (both pieces <= 32 stored vertices),
+    1 medium (both <= 64), both with <= 8 rings; 2 mixed (one piece <= 32 vertices); 3 large."""
     rings = np.diff(hs.gro.astype(np.int64))
     verts = hs.rvo[hs.gro[1:]].astype(np.int64) - hs.rvo[hs.gro[:-1]].astype(np.int64)
-    return np.where(rings > 8, 2, np.where(verts <= 32, 0, np.where(verts <= 64, 1, 2)))
+    va, vb = verts[pairs[:, 0]], verts[pairs[:, 1]]
+    rok = (rings[pairs[:, 0]] <= 8) & (rings[pairs[:, 1]] <= 8)
+    small = rok & (va <= 32) & (vb <= 32)
+    med = ~small & rok & (va <= 64) & (vb <= 64)
+    mixed = ~small & ~med & (np.minimum(va, vb) <= 32)
+    return np.where(small, 0, np.where(med, 1, np.where(mixed, 2, 3)))
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "refine_relate_2.4M.md")
@@ -337,15 +342,16 @@ def main():
     value = cand * args.steps / t_all
     log(f"step {ms_step:.2f} ms: {cand} candidates, {rows} rows; traverse {st['traverse_ms']:.2f} refine {st['refine_ms']:.2f} "
         f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels screen {st['k_screen_ms']:.2f} small {st['k_small_ms']:.2f} "
-        f"medium {st['k_medium_ms']:.2f} large {st['k_large_ms']:.2f} over {st['k_over_ms']:.2f} ms (pairs {st['n_small']}/{st['n_medium']}/"
-        f"{st['n_large']}/{st['n_over']}, edge work {st['n_small_edge']}/{st['n_medium_edge']})")
+        f"medium {st['k_medium_ms']:.2f} mixed {st['k_mixed_ms']:.2f} large {st['k_large_ms']:.2f} over {st['k_over_ms']:.2f} ms (pairs "
+        f"{st['n_small']}/{st['n_medium']}/{st['n_mixed']}/{st['n_large']}/{st['n_over']}, edge work {st['n_small_edge']}/{st['n_medium_edge']})")
 
     # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator):
     # (name, duration field, pair class of the sample, pairs field, FLOPs counted: edge phase / screen part / all)
     kernels = (("k_screen_small", "k_screen_ms", (0, 1), ("n_small", "n_medium"), "screen"),
                ("k_edges_small<16, 32, 0>", "k_small_ms", (0,), ("n_small_edge",), "edge"),
                ("k_edges_small<32, 64, 1>", "k_medium_ms", (1,), ("n_medium_edge",), "edge"),
-               ("k_refine_large", "k_large_ms", (2,), ("n_large",), "all"))
+               ("k_refine_mixed", "k_mixed_ms", (2,), ("n_mixed",), "all"),
+               ("k_refine_large", "k_large_ms", (3,), ("n_large",), "all"))
     kern = max(kernels, key=lambda k: st[k[1]])
     roofline = None
     cpu = None
@@ -356,8 +362,7 @@ def main():
         Ms, Ss = mesh(gsj, args.sample)
         Os = O.Layer.polys(Ss)
         cand_s = O.candidates(Os, Os, upper=True)
-        pc = piece_class(Ss)
-        pick = cand_s[np.isin(np.maximum(pc[cand_s[:, 0]], pc[cand_s[:, 1]]), kern[2])]
+        pick = cand_s[np.isin(pair_class(Ss, cand_s), kern[2])]
         rng = np.random.default_rng(0)
         pick = pick[rng.choice(len(pick), size=min(len(pick), 40000), replace=False)]
         fl = O.refine_flops(Ss, Ss, pick, want_area=False, want_relate=True)
@@ -415,7 +420,7 @@ def main():
                    "pieces": S.n, "vertices": S.nv, "candidates_per_step": cand, "rows_per_step": rows, "parallelism": "1 GPU"},
         "rows_per_s": round(rows * args.steps / t_all, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
-        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_large_ms", "k_over_ms")},
+        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1)},
         "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving,
     }

@@ -109,7 +109,8 @@ typedef struct {
    * small, both <= 64 stored vertices and <= 8 rings); these two classes are screened first (contact, containment;
    * k_screen_ms, every small and medium pair) and k_small_ms / k_medium_ms time their edge-contribution kernels over
    * the n_small_edge / n_medium_edge pairs that need one; large = a wave per pair; over = the global-memory overflow
-   * pass. Distance joins: small, medium and large list classes. Point joins: the per-pair kernel as small. */
+   * pass; mixed = 16-lane groups with edge lists for pairs of a small polygon (<= 32 vertices) and a larger one.
+   * Distance joins: small, medium, mixed and large list classes. Point joins: the per-pair kernel as small. */
   double k_small_ms, k_large_ms, k_over_ms;
   uint64_t n_small, n_large, n_over;
   uint64_t launches;       /* refinement launches (sub-batches) */
@@ -117,6 +118,8 @@ typedef struct {
   uint64_t n_medium;
   double k_screen_ms;
   uint64_t n_small_edge, n_medium_edge;
+  double k_mixed_ms;
+  uint64_t n_mixed;
 } gsj_join_stats;
 
 /* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;

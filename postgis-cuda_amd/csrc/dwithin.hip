@@ -164,8 +164,8 @@ __device__ __forceinline__ bool self_piece_pair(const DwArgs& A, uint32_t a, uin
   return qa == qb;
 }
 
-// list classes: a G-lane group per pair, CAP-edge LDS lists per side; class `which` (0 small, 1 medium, 2 large) of
-// the partition; pairs whose lists overflow are queued
+// list classes: a G-lane group per pair, CAP-edge LDS lists per side; class `which` (0 small, 1 medium, 2 mixed,
+// 3 large) of the partition; pairs whose lists overflow are queued
 template <int G, int CAP>
 __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, int which,
                                                   uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
@@ -175,7 +175,9 @@ __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t
   Ed* LA = lds[gib];
   Ed* LB = LA + CAP;
   const uint32_t n = counts[which];
-  const uint32_t* L = list + (which == 0 ? 0u : (which == 1 ? counts[0] : counts[0] + counts[1]));
+  uint32_t first = 0;
+  for (int w = 0; w < which; w++) first += counts[w];
+  const uint32_t* L = list + first;
   for (uint32_t k = blockIdx.x * gpb + gib; k < n; k += gridDim.x * gpb) {
     const uint32_t p = L[k];
     const uint32_t a = A.pa[p], b = A.pb[p];
@@ -513,7 +515,7 @@ void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool sa
   for_each_candidate_batch(c, probes_of(PL, qbb), RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
     DwArgs A{PL.poly, RL.poly, pa, pb, d, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr};
-    const PairClasses pc = partition_pairs(c, PL.poly, RL.poly, pa, pb, np, MCAP);
+    const PairClasses pc = partition_pairs(c, PL.poly, RL.poly, pa, pb, np, MCAP, true);
     uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
     const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
     GSJ_HIP(hipEventRecord(c.ev[0], stream));
@@ -525,21 +527,26 @@ void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool sa
                        pc.ctl);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[5], stream));
-    hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 2, ovf,
+    hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 2, ovf,
+                       pc.ctl);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[7], stream));
+    hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 3, ovf,
                        pc.ctl);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[2], stream));
-    struct Host { uint32_t counts[3]; OverflowCtl ctl; };
+    struct Host { uint32_t counts[4]; OverflowCtl ctl; };
     Host* h = c.h_scal.as<Host>(2);
-    GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 12, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 16, hipMemcpyDeviceToHost, stream));
     GSJ_HIP(hipMemcpyAsync(&h->ctl, pc.ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, stream));
     GSJ_HIP(hipStreamSynchronize(stream));
-    float ms_s = 0, ms_m = 0, ms_l = 0;
+    float ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
     GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
     GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
-    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[5], c.ev[2]));
-    s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_large_ms += ms_l;
-    s.n_small += h->counts[0]; s.n_medium += h->counts[1]; s.n_large += h->counts[2];
+    GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
+    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
+    s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_mixed_ms += ms_x; s.k_large_ms += ms_l;
+    s.n_small += h->counts[0]; s.n_medium += h->counts[1]; s.n_mixed += h->counts[2]; s.n_large += h->counts[3];
     s.launches++;
     const uint32_t nover = h->ctl.count;
     if (nover) {

@@ -114,19 +114,20 @@ __device__ bool pip_parent(const DPolys& S, const ESide& s, const EFrame& F, dou
     if (cx < bx0 - 1e-6 || cx > bx1 + 1e-6 || cy < by0 - 1e-6 || cy > by1 + 1e-6) continue;
     const double shx = o.x - F.ox, shy = o.y - F.oy, ox = bx0 - 1.0;
     bool par = false;
-    for (uint32_t r = S.gro[g]; r < S.gro[g + 1]; r++) {
-      const uint64_t s0 = S.rvo[r], ee = S.rvo[r + 1];
-      for (uint64_t j = s0 + 1 + lane; j < ee; j += 64) {
+    // runs whose chunk lies strictly above or below the row y = cy in the frame cross nothing
+    auto skip = [&](const float4 c) { return (shy + (double)c.y) * F.ky > cy || (shy + (double)c.w) * F.ky < cy; };
+    walk_edge_runs(S, g, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+      for (uint64_t i = i0 + lane; i < i1; i += 64) {
         double x0, y0, x1, y1;
-        fv(S, shx, shy, j - 1, F, x0, y0);
-        fv(S, shx, shy, j, F, x1, y1);
+        fv(S, shx, shy, i, F, x0, y0);
+        fv(S, shx, shy, i + 1, F, x1, y1);
         if ((y0 > cy) == (y1 > cy) && y0 != cy && y1 != cy) continue;
         const bool so = left_of(x0, y0, x1, y1, ox, cy), sc = left_of(x0, y0, x1, y1, cx, cy);
         if (so == sc) continue;
         const bool su = orient(ox, cy, cx, cy, x0, y0) > 0, sv = orient(ox, cy, cx, cy, x1, y1) > 0;
         if (su != sv) par = !par;
       }
-    }
+    });
     in = (__popcll(__ballot(par)) & 1) != 0;
   }
   return in;
@@ -140,13 +141,16 @@ __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, do
     const double2 o = S.origin[g], ex = S.ext[g];
     if ((o.x + ex.x - F.ox) * F.kx < cx0 || (o.x - F.ox) * F.kx > cx1 || (o.y + ex.y - F.oy) * F.ky < cy0 || (o.y - F.oy) * F.ky > cy1) continue;
     const double shx = o.x - F.ox, shy = o.y - F.oy;
-    for (uint32_t r = S.gro[g]; r < S.gro[g + 1]; r++) {
-      const uint64_t s0 = S.rvo[r], e = S.rvo[r + 1];
-      for (uint64_t base = s0; base + 1 < e; base += 64) {
+    // edge-chunk runs whose box, in the frame, misses C hold no edge meeting C (the frame map is monotonic)
+    auto skip = [&](const float4 c) {
+      return (shx + (double)c.z) * F.kx < cx0 || (shx + (double)c.x) * F.kx > cx1 || (shy + (double)c.w) * F.ky < cy0 || (shy + (double)c.y) * F.ky > cy1;
+    };
+    walk_edge_runs(S, g, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+      for (uint64_t base = i0; base < i1; base += 64) {
         const uint64_t i = base + lane;
         bool v = false;
         BEd ed;
-        if (i + 1 < e && !syn_bit(S.synb, i)) {
+        if (i < i1 && !syn_bit(S.synb, i)) {
           fv(S, shx, shy, i, F, ed.x0, ed.y0);
           fv(S, shx, shy, i + 1, F, ed.x1, ed.y1);
           v = !(ed.x0 == ed.x1 && ed.y0 == ed.y1) &&
@@ -157,7 +161,7 @@ __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, do
         if (v && pos < cap) E[pos] = ed;
         cnt += __popcll(m);
       }
-    }
+    });
   }
   return cnt;
 }

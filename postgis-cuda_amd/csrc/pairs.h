@@ -62,28 +62,37 @@ __device__ bool inside_from_c(const Src& L, double cx, double cy, bool in_c, dou
 }
 
 // ------------------------------------------------------------------------------------------- edge chunks
-// Vertex pairs (i, i+1) with i in [a, b) of geometry g (b - a <= CHUNK) all lie in the union of at most two chunk
-// boxes; true when both boxes, shifted by (shx, shy), miss the box o. Rounding is monotonic, so a chunk box that
-// misses o in floating point holds only vertices whose shifted coordinates miss it too: every edge it covers is one
-// the caller would reject, and skipping it changes no result.
+// Edges (i, i+1) of a geometry are walked in runs: a ring's edges split at chunk boundaries (chunk k of a chunked
+// geometry holds i in [v0 + CHUNK k, v0 + CHUNK (k+1))), so a run lies in one chunk and one ring. A run whose chunk box,
+// shifted by (shx, shy), misses the box of interest is skipped whole. Rounding is monotonic, so a chunk box that misses
+// the box in floating point holds only vertices whose shifted coordinates miss it too: every edge it covers is one the
+// caller would reject, and skipping it changes no result.
 __device__ __forceinline__ bool chunk_misses(const float4 c, double shx, double shy, const Box& o) {
   return shx + (double)c.z < o.x0 || shx + (double)c.x > o.x1 || shy + (double)c.w < o.y0 || shy + (double)c.y > o.y1;
 }
-__device__ __forceinline__ bool chunks_miss(const DPolys& S, uint32_t g, uint64_t v0, uint64_t a, uint64_t b, double shx, double shy, const Box& o) {
-  if (!S.cfirst) return false;
-  const uint32_t cf = S.cfirst[g];
-  if (cf == NO_CHUNKS) return false;
-  const uint64_t k0 = (a - v0) / CHUNK, k1 = (b - 1 - v0) / CHUNK;
-  return chunk_misses(S.cbb[cf + k0], shx, shy, o) && (k1 == k0 || chunk_misses(S.cbb[cf + k1], shx, shy, o));
-}
-// all vertices of pairs i in [a, b) strictly above or strictly below the horizontal line y = my (shifted by shy)
-__device__ __forceinline__ bool chunks_off_row(const DPolys& S, uint32_t g, uint64_t v0, uint64_t a, uint64_t b, double shy, double my) {
-  if (!S.cfirst) return false;
-  const uint32_t cf = S.cfirst[g];
-  if (cf == NO_CHUNKS) return false;
-  const uint64_t k0 = (a - v0) / CHUNK, k1 = (b - 1 - v0) / CHUNK;
-  auto off = [&](const float4 c) { return shy + (double)c.y > my || shy + (double)c.w < my; };
-  return off(S.cbb[cf + k0]) && (k1 == k0 || off(S.cbb[cf + k1]));
+// the chunk's vertices all strictly above or strictly below the horizontal line y = my (shifted by shy)
+__device__ __forceinline__ bool chunk_off_row(const float4 c, double shy, double my) { return shy + (double)c.y > my || shy + (double)c.w < my; }
+
+// calls run(i0, i1) for the runs of edges i in [i0, i1) of geometry g, in ring and edge order; skip(chunk box) decides
+// whether a chunked run can be skipped (group-uniform arguments, so the walk is group-uniform)
+template <class Skip, class Run>
+__device__ __forceinline__ void walk_edge_runs(const DPolys& S, uint32_t g, const Skip& skip, const Run& run) {
+  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
+  const uint64_t v0 = S.rvo[r0];
+  const uint32_t cf = S.cfirst ? S.cfirst[g] : NO_CHUNKS;
+  for (uint32_t r = r0; r < r1; r++) {
+    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
+    for (uint64_t i0 = s; i0 + 1 < e;) {
+      uint64_t i1 = e - 1;
+      if (cf != NO_CHUNKS) {
+        const uint64_t k = (i0 - v0) / CHUNK;
+        i1 = min(i1, v0 + (k + 1) * CHUNK);
+        if (skip(S.cbb[cf + k])) { i0 = i1; continue; }
+      }
+      run(s, e, i0, i1);
+      i0 = i1;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------- whole geometries
@@ -92,16 +101,12 @@ __device__ __forceinline__ bool chunks_off_row(const DPolys& S, uint32_t g, uint
 template <int G>
 __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, Ed* out, int cap, int gl) {
   int cnt = 0;
-  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
-  const uint64_t v0 = S.rvo[r0];
-  for (uint32_t r = r0; r < r1; r++) {
-    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
-    for (uint64_t base = s; base + 1 < e; base += G) {
-      if (chunks_miss(S, g, v0, base, min(base + G, e - 1), shx, shy, ov)) continue;   // group-uniform
+  walk_edge_runs(S, g, [&](const float4 c) { return chunk_misses(c, shx, shy, ov); }, [&](uint64_t s, uint64_t e, uint64_t i0, uint64_t i1) {
+    for (uint64_t base = i0; base < i1; base += G) {
       const uint64_t i = base + gl;
       bool v = false;
       Ed ed;
-      if (i + 1 < e) {
+      if (i < i1) {
         const double2 p = S.xy[i], q = S.xy[i + 1];
         ed.x0 = shx + p.x; ed.y0 = shy + p.y; ed.x1 = shx + q.x; ed.y1 = shy + q.y;
         ed.syn = syn_bit(S.synb, i);
@@ -113,7 +118,7 @@ __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, cons
       if (v && pos < cap) out[pos] = ed;
       cnt += __popcll(m);
     }
-  }
+  });
   return cnt;
 }
 
@@ -122,15 +127,9 @@ template <int G>
 __device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, double mx, double my, int gl) {
   const double ox = shx - 1.0, oy = my;
   bool par = false;
-  const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
-  const uint64_t v0 = S.rvo[r0];
-  for (uint32_t r = r0; r < r1; r++) {
-    const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
-    for (uint64_t jb = s + 1; jb < e; jb += G) {
-      if (chunks_off_row(S, g, v0, jb - 1, min(jb - 1 + G, e - 1), shy, my)) continue;   // group-uniform
-      const uint64_t j = jb + gl;
-      if (j >= e) continue;
-      const double2 p = S.xy[j - 1], q = S.xy[j];
+  walk_edge_runs(S, g, [&](const float4 c) { return chunk_off_row(c, shy, my); }, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+    for (uint64_t i = i0 + gl; i < i1; i += G) {
+      const double2 p = S.xy[i], q = S.xy[i + 1];
       const double x0 = shx + p.x, y0 = shy + p.y, x1 = shx + q.x, y1 = shy + q.y;
       if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;
       const bool so = left_of(x0, y0, x1, y1, ox, oy), sc = left_of(x0, y0, x1, y1, mx, my);
@@ -138,7 +137,7 @@ __device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, do
       const bool su = orient(ox, oy, mx, my, x0, y0) > 0, sv = orient(ox, oy, mx, my, x1, y1) > 0;
       if (su != sv) par = !par;
     }
-  }
+  });
   return (__popcll(gballot<G>(par)) & 1) != 0;
 }
 
@@ -169,8 +168,9 @@ constexpr int VCAP = 32, RCAP = 8;   // small class: stored vertices and rings p
 constexpr int MCAP = 64;             // medium class: stored vertices per polygon (rings within RCAP)
 
 // Candidate pairs split by work class, on the device: list[0 .. counts[0]) the small pairs (both polygons within
-// VCAP / RCAP), then counts[1] medium pairs (not small, both within mcap / RCAP; none when mcap is 0), then the
-// counts[2] others; within a class by mean vertex count, then candidate order; ctl cleared for the overflow queue.
+// VCAP / RCAP), then counts[1] medium pairs (not small, both within mcap / RCAP; none when mcap is 0), counts[2]
+// mixed pairs (with `mixed`: one polygon within VCAP vertices), then the counts[3] others; within a class by mean
+// vertex count, then candidate order; ctl cleared for the overflow queue.
 // Uses the context's s_keys, s_keys2, s_vals, s_vals2, s_tmp2 and s_misc scratch (the list lives in one of the
 // s_vals buffers until their next use).
 struct PairClasses {
@@ -178,7 +178,8 @@ struct PairClasses {
   uint32_t* counts;
   OverflowCtl* ctl;
 };
-PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap);
+PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap,
+                            bool mixed);
 
 // group-wide minimum (order-independent, so deterministic)
 template <int G>

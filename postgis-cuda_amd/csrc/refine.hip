@@ -645,8 +645,8 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
   const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
   const bool rel = rp.want_relate != 0;
   const Scratch S = carve(smem + (size_t)wib * list_bytes(cap, rel), cap, rel);
-  const uint32_t nlist = nlist_ptr[2];
-  const uint32_t* large = list + nlist_ptr[0] + nlist_ptr[1];   // the large class follows the small and medium ones
+  const uint32_t nlist = nlist_ptr[3];
+  const uint32_t* large = list + nlist_ptr[0] + nlist_ptr[1] + nlist_ptr[2];   // the large class comes last
   for (uint32_t k = blockIdx.x * wpb + wib; k < nlist; k += gridDim.x * wpb) {
     const uint32_t p = large[k];
     const uint32_t a = pa[p], b = pb[p];
@@ -659,6 +659,37 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
     }
     wave_lds_sync();
     if (refine_lists<64, false>(P, R, a, b, p, rp, f, S, na, nb, gl, out) && gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
+    wave_lds_sync();
+  }
+}
+
+// mixed class (one polygon small, the other large): a 16-lane group per pair, compacted lists of CAP_MIXED edges per
+// side in LDS (the large polygon's edges meeting the overlap box, found through its edge-chunk boxes); pairs whose
+// lists overflow are queued
+constexpr int CAP_MIXED = 32, MIXED_T = 128;
+__global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                          const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
+                                                          RefineOut out, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int G = 16;
+  const int gpb = blockDim.x / G;
+  const int gib = threadIdx.x / G, gl = threadIdx.x % G;
+  const bool rel = rp.want_relate != 0;
+  const Scratch S = carve(smem + (size_t)gib * list_bytes(CAP_MIXED, rel), CAP_MIXED, rel);
+  const uint32_t nlist = counts[2];
+  const uint32_t* cl = list + counts[0] + counts[1];
+  for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
+    const uint32_t p = cl[k];
+    const uint32_t a = pa[p], b = pb[p];
+    const PairSetup f = setup_pair(P, R, a, b, rp);
+    const int na = collect<G>(P, a, f.ashx, f.ashy, f.ov, S.LA, CAP_MIXED, gl);
+    const int nb = collect<G>(R, b, f.bshx, f.bshy, f.ov, S.LB, CAP_MIXED, gl);
+    if (na > CAP_MIXED || nb > CAP_MIXED) {
+      if (gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
+      continue;
+    }
+    wave_lds_sync();
+    if (refine_lists<G, false>(P, R, a, b, p, rp, f, S, na, nb, gl, out) && gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
     wave_lds_sync();
   }
 }
@@ -692,7 +723,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "refine: more than 2^32 pairs in one launch");
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
-  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, MCAP);
+  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, MCAP, true);
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
@@ -728,6 +759,13 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
                      ovf, ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[5], st));
+  {   // mixed: 16-lane groups with edge lists, 8 pairs per block
+    const uint32_t g = std::min<uint32_t>(div_up(np, MIXED_T / 16), max_blocks * 2);
+    hipLaunchKernelGGL(k_refine_mixed, dim3(g), dim3(MIXED_T), (MIXED_T / 16) * list_bytes(CAP_MIXED, rel), st, P, R, pa, pb, list, counts, rp, out, ovf,
+                       ctl);
+    GSJ_LAUNCHED();
+  }
+  GSJ_HIP(hipEventRecord(c.ev[7], st));
   constexpr int CAP_LARGE = 128;
   {   // large: one wave per pair, 4 pairs per block
     const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
@@ -743,13 +781,15 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_HIP(hipStreamSynchronize(st));
   const uint32_t nover = h->ctl.count, need = h->ctl.need;
   if (stats) {
-    float ms_q = 0, ms_s = 0, ms_m = 0, ms_l = 0;
+    float ms_q = 0, ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
     GSJ_HIP(hipEventElapsedTime(&ms_q, c.ev[0], c.ev[6]));
     GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[6], c.ev[1]));
     GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
-    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[5], c.ev[2]));
-    stats->k_screen_ms += ms_q; stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_large_ms += ms_l;
-    stats->n_small += h->counts[0]; stats->n_medium += h->counts[1]; stats->n_large += h->counts[2]; stats->n_over += nover;
+    GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
+    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
+    stats->k_screen_ms += ms_q; stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_mixed_ms += ms_x; stats->k_large_ms += ms_l;
+    stats->n_small += h->counts[0]; stats->n_medium += h->counts[1]; stats->n_mixed += h->counts[2]; stats->n_large += h->counts[3];
+    stats->n_over += nover;
     stats->n_small_edge += h->counts[8]; stats->n_medium_edge += h->counts[9];
     stats->launches++;
   }
@@ -774,36 +814,39 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
 }
 
 namespace {
-// sort key per candidate: class (0 small: both polygons within VCAP stored vertices; 1 medium: not small, both within
-// mcap, none when mcap is 0; 2 large; rings within RCAP for both) in bits 6-7 and the mean stored vertex count
-// (saturated at 63) below, so that the groups of a wave get pairs of similar work
+// sort key per candidate: class in bits 6-7 — 0 small (both polygons within VCAP stored vertices), 1 medium (not
+// small, both within mcap; none when mcap is 0), both with rings within RCAP; 2 mixed (when enabled: neither, one
+// polygon within VCAP stored vertices), 3 large — and the mean stored vertex count (saturated at 63) below, so that
+// the groups of a wave get pairs of similar work
 __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                             uint32_t mcap, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+                             uint32_t mcap, int mixed, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
   const uint32_t va = ia & 0xFFFFu, vb = ib & 0xFFFFu;
   const bool rings = (ia >> 16) <= RCAP && (ib >> 16) <= RCAP;
-  const uint32_t cls = (rings && va <= VCAP && vb <= VCAP) ? 0u : ((rings && va <= mcap && vb <= mcap) ? 1u : 2u);
+  const uint32_t cls = (rings && va <= VCAP && vb <= VCAP) ? 0u
+                       : (rings && va <= mcap && vb <= mcap) ? 1u
+                       : (mixed && min(va, vb) <= (uint32_t)VCAP) ? 2u : 3u;
   key[i] = (uint64_t)((cls << 6) | min(63u, (va + vb) >> 1));
   val[i] = (uint32_t)i;
 }
 
-// class sizes from the sorted keys: first index of class 1 and of class 2 by binary search
+// class sizes from the sorted keys: first index of classes 1, 2 and 3 by binary search
 __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, uint32_t* __restrict__ counts) {
-  uint64_t first[2];
-  for (int c = 1; c <= 2; c++) {
+  uint64_t first[4];
+  first[0] = 0;
+  for (int c = 1; c <= 3; c++) {
     uint64_t lo = 0, hi = np;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) / 2;
       if ((key[mid] >> 6) < (uint64_t)c) lo = mid + 1;
       else hi = mid;
     }
-    first[c - 1] = lo;
+    first[c] = lo;
   }
-  counts[0] = (uint32_t)first[0];
-  counts[1] = (uint32_t)(first[1] - first[0]);
-  counts[2] = (uint32_t)(np - first[1]);
+  for (int c = 0; c < 3; c++) counts[c] = (uint32_t)(first[c + 1] - first[c]);
+  counts[3] = (uint32_t)(np - first[3]);
 }
 
 __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
@@ -812,7 +855,8 @@ __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
 }
 }  // namespace
 
-PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap) {
+PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap,
+                            bool mixed) {
   hipStream_t st = c.stream;
   char* misc = c.s_misc.as<char>(256);   // [0, 32) belongs to the scan / sort primitives
   PairClasses pc;
@@ -826,7 +870,7 @@ PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const 
   uint64_t* keys2 = c.s_keys2.as<uint64_t>(np + 16);
   uint32_t* vals = c.s_vals.as<uint32_t>(np + 16);
   uint32_t* vals2 = c.s_vals2.as<uint32_t>(np + 16);
-  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, keys, vals);
+  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, mixed ? 1 : 0, keys, vals);
   GSJ_LAUNCHED();
   radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 8);   // stable: candidate order within a key
   hipLaunchKernelGGL(k_class_counts, dim3(1), dim3(1), 0, st, keys, np, pc.counts);

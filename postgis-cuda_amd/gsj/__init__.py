@@ -42,7 +42,7 @@ class JoinStats(C.Structure):
                 ("k_small_ms", C.c_double), ("k_large_ms", C.c_double), ("k_over_ms", C.c_double),
                 ("n_small", C.c_uint64), ("n_large", C.c_uint64), ("n_over", C.c_uint64), ("launches", C.c_uint64),
                 ("k_medium_ms", C.c_double), ("n_medium", C.c_uint64), ("k_screen_ms", C.c_double), ("n_small_edge", C.c_uint64),
-                ("n_medium_edge", C.c_uint64)]
+                ("n_medium_edge", C.c_uint64), ("k_mixed_ms", C.c_double), ("n_mixed", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
