@@ -63,7 +63,7 @@ def hbm_traffic(kernel):
         return None
     for line in open(TRAFFIC_SUMMARY):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+        if len(cells) == 4 and cells[0] == f"`{kernel}`":   # the traffic table (the trace table has 5 columns)
             try:
                 return int(float(cells[3]))
             except ValueError:

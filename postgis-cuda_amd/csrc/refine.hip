@@ -636,19 +636,21 @@ __device__ bool refine_lists(const DPolys& P, const DPolys& R, uint32_t a, uint3
   return false;
 }
 
-// large class: a wave per pair, compacted lists in LDS (cap edges per side); overflowing pairs are queued
+// large class: a wave per pair, compacted lists in LDS (cap edges per side); overflowing pairs are queued. Its pairs
+// are the partition's last class followed by the mixed pairs promoted because their lists outgrew the mixed class.
 __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
-                                                      const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist_ptr, RefineParams rp,
-                                                      RefineOut out, int cap, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+                                                      const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ promo,
+                                                      const OverflowCtl* __restrict__ promo_ctl, RefineParams rp, RefineOut out, int cap,
+                                                      uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wpb = blockDim.x / 64;
   const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
   const bool rel = rp.want_relate != 0;
   const Scratch S = carve(smem + (size_t)wib * list_bytes(cap, rel), cap, rel);
-  const uint32_t nlist = nlist_ptr[3];
-  const uint32_t* large = list + nlist_ptr[0] + nlist_ptr[1] + nlist_ptr[2];   // the large class comes last
+  const uint32_t nclass = counts[3], nlist = nclass + promo_ctl->count;
+  const uint32_t* large = list + counts[0] + counts[1] + counts[2];   // the large class comes last
   for (uint32_t k = blockIdx.x * wpb + wib; k < nlist; k += gridDim.x * wpb) {
-    const uint32_t p = large[k];
+    const uint32_t p = k < nclass ? large[k] : promo[k - nclass];
     const uint32_t a = pa[p], b = pb[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
     const int na = collect<64>(P, a, f.ashx, f.ashy, f.ov, S.LA, cap, gl);
@@ -669,7 +671,8 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
 constexpr int CAP_MIXED = 32, MIXED_T = 128;
 __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                           const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
-                                                          RefineOut out, uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
+                                                          RefineOut out, uint32_t* __restrict__ promo, OverflowCtl* promo_ctl,
+                                                          uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int G = 16;
   const int gpb = blockDim.x / G;
@@ -684,8 +687,8 @@ __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const 
     const PairSetup f = setup_pair(P, R, a, b, rp);
     const int na = collect<G>(P, a, f.ashx, f.ashy, f.ov, S.LA, CAP_MIXED, gl);
     const int nb = collect<G>(R, b, f.bshx, f.bshy, f.ov, S.LB, CAP_MIXED, gl);
-    if (na > CAP_MIXED || nb > CAP_MIXED) {
-      if (gl == 0) queue_overflow(p, max(na, nb), ovf_list, ctl);
+    if (na > CAP_MIXED || nb > CAP_MIXED) {   // to the large class (which sends it on if its lists overflow too)
+      if (gl == 0) queue_overflow(p, max(na, nb), promo, promo_ctl);
       continue;
     }
     wave_lds_sync();
@@ -727,7 +730,9 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
-  uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
+  uint32_t* ovf = c.s_heavy.as<uint32_t>(2 * np + 128);
+  uint32_t* promo = ovf + np + 64;   // mixed pairs promoted to the large class
+  OverflowCtl* promo_ctl = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);
   const int T = 256;
   const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
   GSJ_HIP(hipEventRecord(c.ev[0], st));
@@ -761,8 +766,8 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_HIP(hipEventRecord(c.ev[5], st));
   {   // mixed: 16-lane groups with edge lists, 8 pairs per block
     const uint32_t g = std::min<uint32_t>(div_up(np, MIXED_T / 16), max_blocks * 2);
-    hipLaunchKernelGGL(k_refine_mixed, dim3(g), dim3(MIXED_T), (MIXED_T / 16) * list_bytes(CAP_MIXED, rel), st, P, R, pa, pb, list, counts, rp, out, ovf,
-                       ctl);
+    hipLaunchKernelGGL(k_refine_mixed, dim3(g), dim3(MIXED_T), (MIXED_T / 16) * list_bytes(CAP_MIXED, rel), st, P, R, pa, pb, list, counts, rp, out,
+                       promo, promo_ctl, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[7], st));
@@ -770,7 +775,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   {   // large: one wave per pair, 4 pairs per block
     const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), max_blocks);
-    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, rp, out, CAP_LARGE, ovf, ctl);
+    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, CAP_LARGE, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], st));
@@ -851,6 +856,8 @@ __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, ui
 
 __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
   ctl->count = 0; ctl->need = 0;
+  OverflowCtl* promo = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);   // the promotion queue's
+  promo->count = 0; promo->need = 0;
   for (int k = 0; k < 16; k++) counts[k] = 0;   // class sizes [0, 3) and the edge queues' counts [8, 10)
 }
 }  // namespace
