@@ -22,7 +22,8 @@ namespace gsj {
 namespace {
 
 constexpr double FILLET_QUANTUM = gm::PI_HI / 16.0;   // GEOS: 8 quadrant segments
-constexpr int ER_CAPE = 64, ER_CAPL = 128;            // LDS list capacities per side
+constexpr int ER_CAPE = 64, ER_CAPL = 128;            // LDS list capacities per side (wave pass)
+constexpr int ER_CAPE16 = 16, ER_CAPL16 = 48;         // the same for the 16-lane first pass
 constexpr int ER_WAVES = 2;                           // waves per block
 
 // real boundary edge in the frame (metres); vt: 0 no region at the end vertex, 1 convex, m + 1 reflex (m chords)
@@ -105,7 +106,8 @@ __device__ __forceinline__ void fv(const DPolys& S, double shx, double shy, uint
   y = (shy + q.y) * F.ky;
 }
 
-// c inside the parent: parity over the pieces whose bbox holds c (buffer.cuh:351-374)
+// c inside the parent: parity over the pieces whose bbox holds c (buffer.cuh:351-374); a G-lane group
+template <int G>
 __device__ bool pip_parent(const DPolys& S, const ESide& s, const EFrame& F, double cx, double cy, int lane) {
   bool in = false;
   for (uint32_t g = s.g0; g < s.g1 && !in; g++) {
@@ -117,7 +119,7 @@ __device__ bool pip_parent(const DPolys& S, const ESide& s, const EFrame& F, dou
     // runs whose chunk lies strictly above or below the row y = cy in the frame cross nothing
     auto skip = [&](const float4 c) { return (shy + (double)c.y) * F.ky > cy || (shy + (double)c.w) * F.ky < cy; };
     walk_edge_runs(S, g, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
-      for (uint64_t i = i0 + lane; i < i1; i += 64) {
+      for (uint64_t i = i0 + lane; i < i1; i += G) {
         double x0, y0, x1, y1;
         fv(S, shx, shy, i, F, x0, y0);
         fv(S, shx, shy, i + 1, F, x1, y1);
@@ -128,12 +130,13 @@ __device__ bool pip_parent(const DPolys& S, const ESide& s, const EFrame& F, dou
         if (su != sv) par = !par;
       }
     });
-    in = (__popcll(__ballot(par)) & 1) != 0;
+    in = (__popcll(gballot<G>(par)) & 1) != 0;
   }
   return in;
 }
 
 // the parent's real, non-degenerate edges meeting box C, ballot-compacted; returns the full count (buffer.cuh:377-405)
+template <int G>
 __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, double cx0, double cy0, double cx1, double cy1, BEd* E, int cap,
                             int lane) {
   int cnt = 0;
@@ -146,7 +149,7 @@ __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, do
       return (shx + (double)c.z) * F.kx < cx0 || (shx + (double)c.x) * F.kx > cx1 || (shy + (double)c.w) * F.ky < cy0 || (shy + (double)c.y) * F.ky > cy1;
     };
     walk_edge_runs(S, g, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
-      for (uint64_t base = i0; base < i1; base += 64) {
+      for (uint64_t base = i0; base < i1; base += G) {
         const uint64_t i = base + lane;
         bool v = false;
         BEd ed;
@@ -156,7 +159,7 @@ __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, do
           v = !(ed.x0 == ed.x1 && ed.y0 == ed.y1) &&
               !(fmax(ed.x0, ed.x1) < cx0 || fmin(ed.x0, ed.x1) > cx1 || fmax(ed.y0, ed.y1) < cy0 || fmin(ed.y0, ed.y1) > cy1);
         }
-        const uint64_t m = __ballot(v);
+        const uint64_t m = gballot<G>(v);
         const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
         if (v && pos < cap) E[pos] = ed;
         cnt += __popcll(m);
@@ -167,8 +170,9 @@ __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, do
 }
 
 // directions, successors and end-vertex regions (buffer.cuh:408-438)
+template <int G>
 __device__ void prepare_side(BEd* E, int n, double r, int lane) {
-  for (int k = lane; k < n; k += 64) {
+  for (int k = lane; k < n; k += G) {
     const double dx = E[k].x1 - E[k].x0, dy = E[k].y1 - E[k].y0, L = sqrt(dx * dx + dy * dy);
     E[k].ux = dx / L;
     E[k].uy = dy / L;
@@ -176,7 +180,7 @@ __device__ void prepare_side(BEd* E, int n, double r, int lane) {
   }
   wave_lds_sync();
   // the successor search reads only positions and directions; the fields written below are disjoint from them
-  for (int k = lane; k < n; k += 64) {
+  for (int k = lane; k < n; k += G) {
     const BEd e = E[k];
     int s = -1;
     if (k + 1 < n && E[k + 1].x0 == e.x1 && E[k + 1].y0 == e.y1) s = k + 1;
@@ -207,23 +211,25 @@ __device__ void prepare_side(BEd* E, int n, double r, int lane) {
   wave_lds_sync();
 }
 
-__device__ __forceinline__ int wave_max(int v) {
+template <int G>
+__device__ __forceinline__ int group_max(int v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  for (int o = G / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, G));
   return v;
 }
 
 // offset edges and reflex fillet chords meeting the window (+- eps), ballot-compacted; returns the full count
 // (buffer.cuh:441-475)
+template <int G>
 __device__ int build_elements(const BEd* E, int n, double r, double wx0, double wy0, double wx1, double wy1, double eps, BEl* L, int cap, int lane) {
   int cnt = 0;
-  for (int base = 0; base < n; base += 64) {
+  for (int base = 0; base < n; base += G) {
     const int k = base + lane;
     const bool valid = k < n;
     BEd e{};
     if (valid) e = E[k];
     const int ne = valid ? 1 + (e.vt >= 2 ? e.vt - 1 : 0) : 0;
-    const int mx = wave_max(ne);
+    const int mx = group_max<G>(ne);
     double nkx = 0, nky = 0, wxr = 0, wyr = 0, px = 0, py = 0, nsx = 0, nsy = 0;
     if (valid) {
       nkx = -e.uy; nky = e.ux; wxr = nkx; wyr = nky;
@@ -247,7 +253,7 @@ __device__ int build_elements(const BEd* E, int n, double r, double wx0, double 
         }
         has = !(fmax(el.x0, el.x1) < wx0 - eps || fmin(el.x0, el.x1) > wx1 + eps || fmax(el.y0, el.y1) < wy0 - eps || fmin(el.y0, el.y1) > wy1 + eps);
       }
-      const uint64_t m = __ballot(has);
+      const uint64_t m = gballot<G>(has);
       const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
       if (has && pos < cap) L[pos] = el;
       cnt += __popcll(m);
@@ -290,9 +296,10 @@ struct ErPair {
   double Wx0, Wy0, Wx1, Wy1, eps, cx, cy;
 };
 
-// Window, frame, edge lists and element lists of a parent pair (buffer.cuh:502-541); one wave. Returns 1 when ready,
+// Window, frame, edge lists and element lists of a parent pair (buffer.cuh:502-541); a G-lane group. Returns 1 when ready,
 // 0 when the window is empty (no intersection), -1 when a list overflows (needE / needL: sizes required; needL is 0
 // when the edge lists already overflowed).
+template <int G>
 __device__ int er_setup(const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, uint32_t pa, uint32_t pb, double ra_nom,
                         double rb_nom, const ErScratch& sc, int lane, ErPair& E, int& needE, int& needL) {
   ESide& A = E.A;
@@ -330,14 +337,14 @@ __device__ int er_setup(const DPolys& P, const DPolys& R, const uint32_t* pfP, c
   // the frame metric is exact at the window's centre latitude; widen the collect margin for its error
   const double fe = 1.0 + 2.0 * 0.5 * (wy1 - wy0) * geo::DEG * (fabs(gm::tan(clat * geo::DEG)) + 0.05);
   const double ma = A.r * fe + 1e-6, mb = B.r * fe + 1e-6;
-  E.na = collect_side(P, A, F, E.Wx0 - ma, E.Wy0 - ma, E.Wx1 + ma, E.Wy1 + ma, sc.EA, sc.capE, lane);
-  E.nb = collect_side(R, B, F, E.Wx0 - mb, E.Wy0 - mb, E.Wx1 + mb, E.Wy1 + mb, sc.EB, sc.capE, lane);
+  E.na = collect_side<G>(P, A, F, E.Wx0 - ma, E.Wy0 - ma, E.Wx1 + ma, E.Wy1 + ma, sc.EA, sc.capE, lane);
+  E.nb = collect_side<G>(R, B, F, E.Wx0 - mb, E.Wy0 - mb, E.Wx1 + mb, E.Wy1 + mb, sc.EB, sc.capE, lane);
   if (E.na > sc.capE || E.nb > sc.capE) { needE = max(E.na, E.nb); needL = 0; return -1; }
   wave_lds_sync();
-  prepare_side(sc.EA, E.na, A.r, lane);
-  prepare_side(sc.EB, E.nb, B.r, lane);
-  E.la = build_elements(sc.EA, E.na, A.r, E.Wx0, E.Wy0, E.Wx1, E.Wy1, E.eps, sc.LA, sc.capL, lane);
-  E.lb = build_elements(sc.EB, E.nb, B.r, E.Wx0, E.Wy0, E.Wx1, E.Wy1, E.eps, sc.LB, sc.capL, lane);
+  prepare_side<G>(sc.EA, E.na, A.r, lane);
+  prepare_side<G>(sc.EB, E.nb, B.r, lane);
+  E.la = build_elements<G>(sc.EA, E.na, A.r, E.Wx0, E.Wy0, E.Wx1, E.Wy1, E.eps, sc.LA, sc.capL, lane);
+  E.lb = build_elements<G>(sc.EB, E.nb, B.r, E.Wx0, E.Wy0, E.Wx1, E.Wy1, E.eps, sc.LB, sc.capL, lane);
   if (E.la > sc.capL || E.lb > sc.capL) { needE = max(E.na, E.nb); needL = max(E.la, E.lb); return -1; }
   E.cx = E.Wx0 + 0.381966 * (E.Wx1 - E.Wx0);
   E.cy = E.Wy0 + 0.414213 * (E.Wy1 - E.Wy0);
@@ -395,34 +402,40 @@ __device__ bool er_inside(const ErScratch& sc, const ErPair& E, bool inA_c, bool
   return false;
 }
 
-// X_A and X_B intersect within the window (buffer.cuh:502-588). One wave; returns 1 / 0, or -1 when a list overflows
-// (needE / needL: sizes required; needL is 0 when the edge lists already overflowed).
+// X_A and X_B intersect within the window (buffer.cuh:502-588). A G-lane group; returns 1 / 0, or -1 when a list
+// overflows (needE / needL: sizes required; needL is 0 when the edge lists already overflowed).
+template <int G>
 __device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, uint32_t pa, uint32_t pb, double ra_nom,
                        double rb_nom, const ErScratch& sc, int lane, int& needE, int& needL) {
   ErPair E;
-  const int st = er_setup(P, R, pfP, pfR, pa, pb, ra_nom, rb_nom, sc, lane, E, needE, needL);
+  const int st = er_setup<G>(P, R, pfP, pfR, pa, pb, ra_nom, rb_nom, sc, lane, E, needE, needL);
   if (st <= 0) return st;
-  if (__ballot(er_cross(sc, E, lane, 64, nullptr))) return 1;
-  const bool inA_c = pip_parent(P, E.A, E.F, E.cx, E.cy, lane);
-  const bool inB_c = pip_parent(R, E.B, E.F, E.cx, E.cy, lane);
+  if (gany<G>(er_cross(sc, E, lane, G, nullptr))) return 1;
+  const bool inA_c = pip_parent<G>(P, E.A, E.F, E.cx, E.cy, lane);
+  const bool inB_c = pip_parent<G>(R, E.B, E.F, E.cx, E.cy, lane);
   for (int side = 0; side < 2; side++)
-    if (__ballot(er_inside(sc, E, inA_c, inB_c, side, lane, 64, nullptr))) return 1;
+    if (gany<G>(er_inside(sc, E, inA_c, inB_c, side, lane, G, nullptr))) return 1;
   return 0;
 }
 
 struct ErCtl { uint32_t n; int needE, needL; uint32_t pad; };
 
-// main pass: a wave per parent pair, LDS lists; overflowing pairs are queued with the sizes they need
+// LDS passes: a G-lane group per parent pair with CAPE / CAPL lists per side; overflowing pairs are queued with the
+// sizes they need. The pairs are 0 .. np (items null) or items[0 .. items_ctl->n) (the previous pass's queue).
+template <int G, int CAPE, int CAPL>
 __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
                                                               const uint32_t* __restrict__ pfR, const uint32_t* __restrict__ pairs, uint32_t np,
-                                                              double ra, double rb, uint8_t* __restrict__ flag, uint32_t* __restrict__ heavy,
-                                                              ErCtl* ctl) {
-  __shared__ __align__(16) unsigned char lds[ER_WAVES][er_bytes(ER_CAPE, ER_CAPL)];
-  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
-  const ErScratch sc = er_carve(lds[wib], ER_CAPE, ER_CAPL);
-  for (uint32_t p = blockIdx.x * ER_WAVES + wib; p < np; p += gridDim.x * ER_WAVES) {
+                                                              const uint32_t* __restrict__ items, const ErCtl* __restrict__ items_ctl, double ra,
+                                                              double rb, uint8_t* __restrict__ flag, uint32_t* __restrict__ heavy, ErCtl* ctl) {
+  constexpr int NG = ER_WAVES * 64 / G;
+  __shared__ __align__(16) unsigned char lds[NG][er_bytes(CAPE, CAPL)];
+  const int lane = threadIdx.x % G, gib = threadIdx.x / G;
+  const ErScratch sc = er_carve(lds[gib], CAPE, CAPL);
+  const uint32_t n = items ? items_ctl->n : np;
+  for (uint32_t k = blockIdx.x * NG + gib; k < n; k += gridDim.x * NG) {
+    const uint32_t p = items ? items[k] : k;
     int ne = 0, nl = 0;
-    const int r = er_pair(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
+    const int r = er_pair<G>(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
     if (lane == 0) {
       flag[p] = r > 0 ? 1 : 0;
       if (r < 0) {
@@ -453,7 +466,7 @@ __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const 
     if (t < 64) {   // first wave: window, frame and lists
       ErPair E;
       int ne = 0, nl = 0;
-      const int st = er_setup(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, E, ne, nl);
+      const int st = er_setup<64>(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, E, ne, nl);
       if (lane == 0) {
         SE = E;
         s_state = st;
@@ -475,8 +488,8 @@ __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const 
       __syncthreads();
       if (!s_hit) {
         if (t < 64) {
-          const bool a = pip_parent(P, E.A, E.F, E.cx, E.cy, lane);
-          const bool b = pip_parent(R, E.B, E.F, E.cx, E.cy, lane);
+          const bool a = pip_parent<64>(P, E.A, E.F, E.cx, E.cy, lane);
+          const bool b = pip_parent<64>(R, E.B, E.F, E.cx, E.cy, lane);
           if (lane == 0) { s_inA = a; s_inB = b; }
         }
         __syncthreads();
@@ -624,13 +637,22 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
     hipLaunchKernelGGL(k_compact_rows, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, keep, off, n, ep);
     GSJ_LAUNCHED();
     uint8_t* flag = c.s_er_flag.as<uint8_t>(ne + 16);
-    uint32_t* lists = c.s_er_list.as<uint32_t>(2 * ne + 32);
+    uint32_t* lists = c.s_er_list.as<uint32_t>(3 * ne + 48);
     uint32_t* heavy = lists;
     uint32_t* again = lists + ne + 16;
+    uint32_t* first = again + ne + 16;   // the 16-lane pass's overflow, for the wave pass
+    ErCtl* ctl16 = ctl + 1;
     hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
+    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl16);
     GSJ_HIP(hipEventRecord(c.ev[0], stream));
+    // 16-lane groups with small lists first (most parent pairs), then a wave per pair for those that overflowed them
+    const uint32_t g16 = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES * 4), (uint64_t)c.cu_count * 32);
+    hipLaunchKernelGGL((k_erode_main<16, ER_CAPE16, ER_CAPL16>), dim3(g16), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, nullptr,
+                       nullptr, ra, rb, flag, first, ctl16);
+    GSJ_LAUNCHED();
     const uint32_t g = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES), (uint64_t)c.cu_count * 32);
-    hipLaunchKernelGGL(k_erode_main, dim3(g), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, ra, rb, flag, heavy, ctl);
+    hipLaunchKernelGGL((k_erode_main<64, ER_CAPE, ER_CAPL>), dim3(g), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, first, ctl16,
+                       ra, rb, flag, heavy, ctl);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[1], stream));
     ErCtl* h = c.h_scal.as<ErCtl>(2);
