@@ -402,6 +402,32 @@ __device__ bool er_inside(const ErScratch& sc, const ErPair& E, bool inA_c, bool
   return false;
 }
 
+// The vertex part of (b) / (c) for one side: an element end of that side's eroded boundary inside the other eroded
+// polygon. A subset of er_inside's tests, so a hit here is a hit of the pair.
+__device__ bool er_vertices(const ErScratch& sc, const ErPair& E, bool inA_c, bool inB_c, int side, int t, int NT, const volatile int* stop) {
+  auto inW = [&](double x, double y) { return x >= E.Wx0 - E.eps && x <= E.Wx1 + E.eps && y >= E.Wy0 - E.eps && y <= E.Wy1 + E.eps; };
+  const BEd* ES = side ? sc.EB : sc.EA;
+  const BEd* EO = side ? sc.EA : sc.EB;
+  const BEl* LS = side ? sc.LB : sc.LA;
+  const int ns = side ? E.nb : E.na, no = side ? E.na : E.nb, ls = side ? E.lb : E.la;
+  const double rs = side ? E.B.r : E.A.r, ro = side ? E.A.r : E.B.r;
+  const bool inO_c = side ? inA_c : inB_c;
+  auto vertex_in_other = [&](double x, double y) {
+    return inW(x, y) && !in_excl(ES, ns, rs, x, y, E.eps) && inside_m(EO, no, E.cx, E.cy, inO_c, x, y) && !in_excl(EO, no, ro, x, y, E.eps);
+  };
+  for (int i = t; i < ls; i += NT) {
+    if (stop && *stop) return false;
+    const BEl p = LS[i];
+    if (vertex_in_other(p.x0, p.y0) || vertex_in_other(p.x1, p.y1)) return true;
+  }
+  return false;
+}
+
+// A parent paired with itself: X_A(ra) and X_A(rb) meet wherever the more eroded one is non-empty, so an element end of
+// the larger radius's boundary inside the other usually settles it at once; only when none is found do the full tests
+// run (their answer is an OR over all tests, so taking this subset first changes no result).
+__device__ __forceinline__ int self_side(const ErPair& E) { return E.B.r > E.A.r ? 1 : 0; }
+
 // X_A and X_B intersect within the window (buffer.cuh:502-588). A G-lane group; returns 1 / 0, or -1 when a list
 // overflows (needE / needL: sizes required; needL is 0 when the edge lists already overflowed).
 template <int G>
@@ -410,9 +436,19 @@ __device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, co
   ErPair E;
   const int st = er_setup<G>(P, R, pfP, pfR, pa, pb, ra_nom, rb_nom, sc, lane, E, needE, needL);
   if (st <= 0) return st;
+  const bool self = pa == pb && P.xy == R.xy;
+  bool inA_c = false, inB_c = false, have_c = false;
+  if (self) {
+    inA_c = pip_parent<G>(P, E.A, E.F, E.cx, E.cy, lane);
+    inB_c = pip_parent<G>(R, E.B, E.F, E.cx, E.cy, lane);
+    have_c = true;
+    if (gany<G>(er_vertices(sc, E, inA_c, inB_c, self_side(E), lane, G, nullptr))) return 1;
+  }
   if (gany<G>(er_cross(sc, E, lane, G, nullptr))) return 1;
-  const bool inA_c = pip_parent<G>(P, E.A, E.F, E.cx, E.cy, lane);
-  const bool inB_c = pip_parent<G>(R, E.B, E.F, E.cx, E.cy, lane);
+  if (!have_c) {
+    inA_c = pip_parent<G>(P, E.A, E.F, E.cx, E.cy, lane);
+    inB_c = pip_parent<G>(R, E.B, E.F, E.cx, E.cy, lane);
+  }
   for (int side = 0; side < 2; side++)
     if (gany<G>(er_inside(sc, E, inA_c, inB_c, side, lane, G, nullptr))) return 1;
   return 0;
@@ -484,15 +520,18 @@ __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const 
     __syncthreads();
     if (s_state > 0) {
       const ErPair E = SE;
-      if (er_cross(sc, E, t, ER_SLAB_T, &s_hit)) s_hit = 1;
+      const bool self = pairs[2 * p] == pairs[2 * p + 1] && P.xy == R.xy;
+      if (t < 64) {
+        const bool a = pip_parent<64>(P, E.A, E.F, E.cx, E.cy, lane);
+        const bool b = pip_parent<64>(R, E.B, E.F, E.cx, E.cy, lane);
+        if (lane == 0) { s_inA = a; s_inB = b; }
+      }
+      __syncthreads();
+      if (self && er_vertices(sc, E, s_inA != 0, s_inB != 0, self_side(E), t, ER_SLAB_T, &s_hit)) s_hit = 1;
+      __syncthreads();
+      if (!s_hit && er_cross(sc, E, t, ER_SLAB_T, &s_hit)) s_hit = 1;
       __syncthreads();
       if (!s_hit) {
-        if (t < 64) {
-          const bool a = pip_parent<64>(P, E.A, E.F, E.cx, E.cy, lane);
-          const bool b = pip_parent<64>(R, E.B, E.F, E.cx, E.cy, lane);
-          if (lane == 0) { s_inA = a; s_inB = b; }
-        }
-        __syncthreads();
         for (int side = 0; side < 2; side++) {
           if (!s_hit && er_inside(sc, E, s_inA != 0, s_inB != 0, side, t, ER_SLAB_T, &s_hit)) s_hit = 1;
           __syncthreads();
