@@ -159,6 +159,26 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 
 }  // namespace
 
+__global__ void k_tile_boxes(const float4* __restrict__ bb, const uint32_t* __restrict__ order, uint32_t n, float4* __restrict__ tiles) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t p0 = (uint64_t)t * PROBE_TILE;
+  if (p0 >= n) return;
+  const uint64_t p1 = min<uint64_t>(p0 + PROBE_TILE, n);
+  float4 u = bb[order[p0]];
+  for (uint64_t p = p0 + 1; p < p1; p++) {
+    const float4 b = bb[order[p]];
+    u.x = fminf(u.x, b.x); u.y = fminf(u.y, b.y); u.z = fmaxf(u.z, b.z); u.w = fmaxf(u.w, b.w);
+  }
+  tiles[t] = u;
+}
+
+void tile_boxes(Context& c, const float4* bb, const uint32_t* order, uint32_t n, float4* tiles) {
+  const uint32_t nt = (n + PROBE_TILE - 1) / PROBE_TILE;
+  if (nt == 0) return;
+  hipLaunchKernelGGL(k_tile_boxes, dim3(div_up(nt, 256)), dim3(256), 0, c.stream, bb, order, n, tiles);
+  GSJ_LAUNCHED();
+}
+
 void morton_order(Context& c, const float4* bb, uint32_t n, uint32_t* order) {
   if (n == 0) return;
   hipStream_t st = c.stream;
@@ -187,7 +207,8 @@ void bvh_build(Layer& L) {
   L.bvh.n = n;
   if (n == 0) { L.has_bvh = true; return; }
   const uint32_t nn = 2 * n - 1;
-  L.bvh_bytes = (size_t)nn * sizeof(BNode) + 256 + (size_t)n * 4;
+  const size_t tiles_off = (((size_t)nn * sizeof(BNode) + 255) & ~size_t(255)) + (((size_t)n * 4 + 255) & ~size_t(255));
+  L.bvh_bytes = tiles_off + (size_t)((n + PROBE_TILE - 1) / PROBE_TILE) * sizeof(float4) + 256;
   GSJ_HIP(hipMalloc(&L.bvh_mem, L.bvh_bytes));
   BNode* nodes = static_cast<BNode*>(L.bvh_mem);
   uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(L.bvh_mem) + (((size_t)nn * sizeof(BNode) + 255) & ~size_t(255)));
@@ -227,9 +248,13 @@ void bvh_build(Layer& L) {
   GSJ_HIP(hipMemcpyAsync(herr, err, 4, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
   GSJ_REQUIRE(herr[0] == 0, GSJ_E_INTERNAL, "bvh build: tree deeper than expected");
+  float4* tiles = reinterpret_cast<float4*>(static_cast<char*>(L.bvh_mem) + tiles_off);
+  tile_boxes(c, bb, order, n, tiles);
+  GSJ_HIP(hipStreamSynchronize(st));
   L.bvh.nnodes = nn;
   L.bvh.nodes = nodes;
   L.bvh.order = order;
+  L.bvh.tiles = tiles;
   L.has_bvh = true;
 }
 

@@ -35,11 +35,14 @@ struct DPolys {
 constexpr int CHUNK = 64;
 constexpr uint32_t NO_CHUNKS = 0xFFFFFFFFu;
 
+constexpr uint32_t PROBE_TILE = 1024;   // probes per tile box (window membership)
+
 struct DPoints {
   uint32_t n = 0;
   const double2* xy = nullptr;   // [n] world coordinates
   const float4* bb = nullptr;    // [n] outward-rounded float point boxes
   const uint32_t* order = nullptr;   // [n] point ids in Morton order of their boxes (the probe walk order)
+  const float4* tiles = nullptr;     // [(n + PROBE_TILE - 1) / PROBE_TILE] box of each run of PROBE_TILE walk positions
 };
 
 // LBVH node in depth-first (preorder) layout: the left child of an internal node is the next node; `skip` is the
@@ -56,6 +59,7 @@ struct DBvh {
   uint32_t nnodes = 0;      // 2n - 1
   const BNode* nodes = nullptr;
   const uint32_t* order = nullptr;   // [n] primitives in Morton (leaf) order
+  const float4* tiles = nullptr;     // [(n + PROBE_TILE - 1) / PROBE_TILE] box of each run of PROBE_TILE leaves in that order
 };
 
 // ------------------------------------------------------------------------------------------- host objects
@@ -87,7 +91,7 @@ struct Context {
   // scratch, grow-only, reused across calls
   DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
       s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc, s_win, s_qidx,
-      s_win_off, s_edgeq;
+      s_win_off, s_edgeq, s_qlist;
   // erosion pass 2 (it runs inside the join's on-device hook, beside the join's own scratch)
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;
@@ -115,6 +119,8 @@ void layer_upload_points(Layer& L, const double* xy, uint64_t n);
 void bvh_build(Layer& L);
 // ids 0 .. n-1 of the boxes bb in the order of their centres' 64-bit Morton keys (stable), into order[n]
 void morton_order(Context& c, const float4* bb, uint32_t n, uint32_t* order);
+// tiles[t] = union of the boxes bb[order[pos]] for walk positions pos in [PROBE_TILE t, PROBE_TILE (t + 1)) (async)
+void tile_boxes(Context& c, const float4* bb, const uint32_t* order, uint32_t n, float4* tiles);
 
 // ------------------------------------------------------------------------------------------- traversal
 // count pass (write == false) or write pass over probe boxes qbb[qbase .. qbase+nq)
@@ -128,6 +134,7 @@ struct Probes {
   const float4* qbb = nullptr;
   const float4* own_bb = nullptr;
   const uint32_t* order = nullptr;   // probe walk order (ids at positions 0 .. n-1), or id order when null
+  const float4* tiles = nullptr;     // boxes of PROBE_TILE consecutive walk positions (with order), or null
   uint32_t n = 0;
   const std::vector<uint32_t>* h_parent = nullptr;
   const uint32_t* pfirst = nullptr;

@@ -196,22 +196,25 @@ void layer_upload_points(Layer& L, const double* xy, uint64_t n) {
   GSJ_REQUIRE(xy || n == 0, GSJ_E_ARG, "null point array");
   Context& c = *L.ctx;
   L.points = true; L.n = n; L.nparent = (uint32_t)n;
-  Slicer s1; s1.take<double2>(n); s1.take<float4>(n); s1.take<uint32_t>(n);
+  const uint32_t nt = (uint32_t)((n + PROBE_TILE - 1) / PROBE_TILE);
+  Slicer s1; s1.take<double2>(n); s1.take<float4>(n); s1.take<uint32_t>(n); s1.take<float4>(nt);
   L.mem_bytes = s1.off + 64;
   GSJ_HIP(hipMalloc(&L.mem, L.mem_bytes));
   Slicer s2; s2.base = static_cast<char*>(L.mem);
   double2* dxy = s2.take<double2>(n);
   float4* dbb = s2.take<float4>(n);
   uint32_t* order = s2.take<uint32_t>(n);
+  float4* tiles = s2.take<float4>(nt);
   if (n) {
     GSJ_HIP(hipMemcpyAsync(dxy, xy, n * 16, hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(k_point_prep, dim3(div_up(n, 256)), dim3(256), 0, c.stream, (uint32_t)n, dxy, dbb);
     GSJ_LAUNCHED();
     // probes walk the points in Morton order: neighbouring queries visit the same tree nodes and polygons
     morton_order(c, dbb, (uint32_t)n, order);
+    tile_boxes(c, dbb, order, (uint32_t)n, tiles);
   }
   GSJ_HIP(hipStreamSynchronize(c.stream));
-  L.pts.n = (uint32_t)n; L.pts.xy = dxy; L.pts.bb = dbb; L.pts.order = n ? order : nullptr;
+  L.pts.n = (uint32_t)n; L.pts.xy = dxy; L.pts.bb = dbb; L.pts.order = n ? order : nullptr; L.pts.tiles = n ? tiles : nullptr;
 }
 
 }  // namespace gsj

@@ -74,6 +74,36 @@ __global__ void k_win_scatter(const uint32_t* __restrict__ f, const uint64_t* __
   if (i < n && f[i]) out[off[i]] = order ? order[i] : i;
 }
 
+// tiles whose box meets the window (a probe meeting it lies in such a tile)
+__global__ void k_win_tile_flags(const float4* __restrict__ tiles, uint32_t nt, double x0, double y0, double x1, double y1, uint32_t* __restrict__ f) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const float4 b = tiles[t];
+  f[t] = ((double)b.x <= x1 && (double)b.z >= x0 && (double)b.y <= y1 && (double)b.w >= y0) ? 1u : 0u;
+}
+__global__ void k_win_tile_list(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint32_t nt, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nt && f[t]) out[off[t]] = t;
+}
+// membership of the probes of the selected tiles: slot k * PROBE_TILE + j holds walk position tl[k] * PROBE_TILE + j
+__global__ void k_win_tile_probes(const float4* __restrict__ bb, const uint32_t* __restrict__ order, uint32_t n, const uint32_t* __restrict__ tl,
+                                  uint64_t nslots, double x0, double y0, double x1, double y1, uint32_t* __restrict__ f) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nslots) return;
+  const uint64_t pos = (uint64_t)tl[k / PROBE_TILE] * PROBE_TILE + k % PROBE_TILE;
+  bool in = false;
+  if (pos < n) {
+    const float4 b = bb[order[pos]];
+    in = (double)b.x <= x1 && (double)b.z >= x0 && (double)b.y <= y1 && (double)b.w >= y0;
+  }
+  f[k] = in ? 1u : 0u;
+}
+__global__ void k_win_tile_scatter(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, const uint32_t* __restrict__ order,
+                                   const uint32_t* __restrict__ tl, uint64_t nslots, uint32_t* __restrict__ out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nslots && f[k]) out[off[k]] = order[(uint64_t)tl[k / PROBE_TILE] * PROBE_TILE + k % PROBE_TILE];
+}
+
 }  // namespace
 
 Probes probes_of(const Layer& L, const float4* qbb) {
@@ -81,8 +111,8 @@ Probes probes_of(const Layer& L, const float4* qbb) {
   p.own_bb = L.points ? L.pts.bb : L.poly.bb;
   p.qbb = qbb ? qbb : p.own_bb;
   p.n = (uint32_t)L.n;
-  if (L.points) p.order = L.pts.order;
-  else if (L.h_parent.empty() && L.has_bvh) p.order = L.bvh.order;   // the tree's leaf order: Morton order of the boxes
+  if (L.points) { p.order = L.pts.order; p.tiles = L.pts.tiles; }
+  else if (L.h_parent.empty() && L.has_bvh) { p.order = L.bvh.order; p.tiles = L.bvh.tiles; }   // the tree's leaf order
   if (!L.points && !L.h_parent.empty()) {
     p.h_parent = &L.h_parent;
     p.pfirst = L.poly.pfirst;
@@ -124,19 +154,47 @@ void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool 
     const double* w = o.window;
     GSJ_REQUIRE(w[0] <= w[2] && w[1] <= w[3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
     WallTimer tw;
-    uint32_t* f = c.s_win.as<uint32_t>((uint64_t)pr.n + 16);
-    hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, w[0], w[1], w[2], w[3], f);
-    GSJ_LAUNCHED();
-    if (pr.pfirst && pr.nparent) {
-      hipLaunchKernelGGL(k_win_parents, dim3(div_up(pr.nparent, 256)), dim3(256), 0, stream, pr.pfirst, pr.nparent, f);
+    uint32_t* list = nullptr;
+    if (qidx && pr.tiles) {
+      // through the tile boxes of the walk order: only the tiles meeting the window are scanned, and their members
+      // come out in walk order
+      const uint32_t nt = (pr.n + PROBE_TILE - 1) / PROBE_TILE;
+      uint32_t* tf = c.s_win.as<uint32_t>((uint64_t)nt + 16);
+      uint64_t* toff = c.s_win_off.as<uint64_t>((uint64_t)nt + 16);
+      hipLaunchKernelGGL(k_win_tile_flags, dim3(div_up(nt, 256)), dim3(256), 0, stream, pr.tiles, nt, w[0], w[1], w[2], w[3], tf);
+      GSJ_LAUNCHED();
+      const uint32_t nsel = (uint32_t)scan_u32_to_u64(c, tf, toff, nt);
+      uint32_t* tl = c.s_qidx.as<uint32_t>((uint64_t)nsel + 16);
+      hipLaunchKernelGGL(k_win_tile_list, dim3(div_up(nt, 256)), dim3(256), 0, stream, tf, toff, nt, tl);
+      GSJ_LAUNCHED();
+      const uint64_t nslots = (uint64_t)nsel * PROBE_TILE;
+      nQ = 0;
+      if (nslots) {
+        uint32_t* f = c.s_win.as<uint32_t>(nslots + 16);
+        uint64_t* off = c.s_win_off.as<uint64_t>(nslots + 16);
+        hipLaunchKernelGGL(k_win_tile_probes, dim3(div_up(nslots, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, tl, nslots, w[0], w[1], w[2],
+                           w[3], f);
+        GSJ_LAUNCHED();
+        nQ = (uint32_t)scan_u32_to_u64(c, f, off, nslots);
+        list = c.s_qlist.as<uint32_t>((uint64_t)nQ + 16);
+        hipLaunchKernelGGL(k_win_tile_scatter, dim3(div_up(nslots, 256)), dim3(256), 0, stream, f, off, qidx, tl, nslots, list);
+        GSJ_LAUNCHED();
+      }
+    } else {
+      uint32_t* f = c.s_win.as<uint32_t>((uint64_t)pr.n + 16);
+      hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, w[0], w[1], w[2], w[3], f);
+      GSJ_LAUNCHED();
+      if (pr.pfirst && pr.nparent) {
+        hipLaunchKernelGGL(k_win_parents, dim3(div_up(pr.nparent, 256)), dim3(256), 0, stream, pr.pfirst, pr.nparent, f);
+        GSJ_LAUNCHED();
+      }
+      uint64_t* off = c.s_win_off.as<uint64_t>((uint64_t)pr.n + 16);
+      nQ = (uint32_t)scan_u32_to_u64(c, f, off, pr.n);
+      // members written behind the flags (the flags are read by the scatter itself, so they need their own region)
+      list = c.s_qlist.as<uint32_t>((uint64_t)nQ + 16);
+      hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, qidx, pr.n, list);
       GSJ_LAUNCHED();
     }
-    uint64_t* off = c.s_win_off.as<uint64_t>((uint64_t)pr.n + 16);
-    nQ = (uint32_t)scan_u32_to_u64(c, f, off, pr.n);
-    // members written behind the flags (the flags are read by the scatter itself, so they need their own region)
-    uint32_t* list = c.s_qidx.as<uint32_t>((uint64_t)nQ + 16);
-    hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, qidx, pr.n, list);
-    GSJ_LAUNCHED();
     hq.resize(nQ);
     if (nQ) GSJ_HIP(hipMemcpyAsync(hq.data(), list, (size_t)nQ * 4, hipMemcpyDeviceToHost, stream));
     GSJ_HIP(hipStreamSynchronize(stream));
