@@ -216,6 +216,7 @@ typedef struct {
   double run_ms;     /* admitted -> finished (device work, copies and callbacks) */
   double total_ms;   /* submit -> finished */
   int lane;
+  int batch;         /* requests that ran together in its pass (1 = alone; see gsj_sched_set_batch) */
 } gsj_request_times;
 
 /* lanes in [1, 16]; scratch_budget_bytes: admission budget over the estimated scratch of running requests (0 = no
@@ -223,6 +224,13 @@ typedef struct {
 int gsj_sched_create(gsj_ctx* ctx, int lanes, uint64_t scratch_budget_bytes, gsj_sched** out);
 /* waits for every submitted request, then stops the lanes and frees their scratch */
 void gsj_sched_destroy(gsj_sched* s);
+/* Batching of windowed joins (GSJ_REQ_JOIN with has_window, the same P and R, the same want_area / want_relate /
+ * upper / snap_half_ulp / no_agg / batch_q / max_pairs, no cancel flag, probe layer with a BVH and neither layer
+ * subdivided): a lane admitting one such request also takes up to max_requests - 1 more from the queue and runs them
+ * in one pass. Each request's rows (in order), candidates and hits are those it gets alone. max_requests in [1, 32],
+ * default 16; 1 turns batching off. No counterpart in the reference, which serves one request at a time
+ * (daemon/gpujoind.cu:35). */
+int gsj_sched_set_batch(gsj_sched* s, int max_requests);
 /* queue a request; *ticket identifies it for gsj_wait / gsj_poll */
 int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket);
 /* wait for the request and forget the ticket; returns the request's own status (GSJ_OK or GSJ_E_*, message in

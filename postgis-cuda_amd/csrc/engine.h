@@ -91,10 +91,10 @@ struct Context {
   // scratch, grow-only, reused across calls
   DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
       s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc, s_win, s_qidx,
-      s_win_off, s_edgeq, s_qlist;
+      s_win_off, s_edgeq, s_qlist, s_wtile, s_wtoff, s_wseg;
   // erosion pass 2 (it runs inside the join's on-device hook, beside the join's own scratch)
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
-  PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts;
+  PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg;
   int cu_count = 256;
   hipEvent_t ev[8] = {};   // kernel timing events (created with the context)
   ~Context();
@@ -150,6 +150,25 @@ using PairsFn = std::function<void(uint32_t* pa, uint32_t* pb, uint64_t np)>;
 void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
                               const PairsFn& fn);
 
+// Several windowed requests over the same probes in one candidate search (the batched served path). Window r's
+// members come after window r-1's, each in walk order, so request r's candidates are a contiguous run of every
+// sub-batch, in the order its own search would produce them. Needs probes with tile boxes (a layer without parents).
+constexpr uint32_t MAX_WINDOWS = 32;
+struct WindowSet {   // passed to kernels by value
+  uint32_t k = 0;
+  double w[MAX_WINDOWS][4];
+};
+// candidate-index boundaries of the k requests inside one sub-batch: request r owns pairs [lo[r], lo[r + 1])
+struct SegBounds {
+  uint32_t k = 0;
+  const uint64_t* h = nullptr;   // host, k + 1 entries
+  const uint64_t* d = nullptr;   // device copy
+};
+using SegPairsFn = std::function<void(uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds& sb)>;
+bool probes_batchable(const Probes& pr);
+void for_each_candidate_multi(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, const WindowSet& ws,
+                              gsj_join_stats& s, const SegPairsFn& fn);
+
 // ------------------------------------------------------------------------------------------- refinement
 struct RefineParams {   // engine/engine.cuh:517-518
   double tol_rel = 1e-12, eps_par = 1e-7, tol_abs = 1e-11;
@@ -177,6 +196,18 @@ struct JoinHooks {
 };
 void join_run(Context& c, const Layer& P, const Layer& R, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st,
               const JoinHooks* hooks);
+// windowed joins of the same layers and refinement options in one pass (see for_each_candidate_multi); each
+// request's rows, in order, and its candidates and hits are those of its own join_run; timing fields are the batch's
+struct BatchJoin {
+  const gsj_join_opts* o = nullptr;
+  gsj_rows_cb cb = nullptr;
+  void* user = nullptr;
+  gsj_join_stats* st = nullptr;
+  int rc = GSJ_OK;   // GSJ_E_CANCEL when its callback asked to stop
+};
+bool join_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o);
+bool join_compatible(const gsj_join_opts& a, const gsj_join_opts& b);
+void join_batch_run(Context& c, const Layer& P, const Layer& R, std::vector<BatchJoin>& reqs);
 void points_join_run(Context& c, const Layer& pts, const Layer& R, const gsj_join_opts& o, gsj_points_cb cb, void* user,
                      gsj_join_stats* st);
 void cluster_run(Context& c, const Layer& P, const gsj_join_opts& o, uint32_t* root, uint64_t nroot, gsj_join_stats* st);

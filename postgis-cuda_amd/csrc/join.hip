@@ -112,11 +112,21 @@ __global__ void k_emit_multi(const DPolys P, const DPolys R, const uint64_t* __r
 
 inline int bits_for(uint64_t maxval) { int b = 1; while (b < 64 && (maxval >> b)) b++; return b; }
 
+// row boundaries of the batched requests: rows of candidates [lo[r], lo[r + 1]) are [rlo[r], rlo[r + 1]), from the
+// exclusive scan offs of the kept flags (single-piece rows only)
+__global__ void k_row_bounds(const uint64_t* __restrict__ lo, uint32_t k, const uint64_t* __restrict__ offs, uint64_t np, uint64_t nrows,
+                             uint64_t* __restrict__ rlo) {
+  const uint32_t r = threadIdx.x;
+  if (r > k) return;
+  const uint64_t i = lo[r];
+  rlo[r] = i < np ? offs[i] : nrows;
+}
+
 }  // namespace
 
 // Compact the refined candidates of one sub-batch into parent-pair rows on the device; returns the row count.
 static uint64_t compact_rows(Context& c, const Layer& PL, const Layer& RL, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
-                             const RefineOut& ro, bool no_agg, uint32_t* orow, double* oarea, uint32_t* ocode) {
+                             const RefineOut& ro, bool no_agg, uint32_t* orow, double* oarea, uint32_t* ocode, const uint64_t** single_offs = nullptr) {
   hipStream_t st = c.stream;
   const DPolys& P = PL.poly;
   const DPolys& R = RL.poly;
@@ -126,6 +136,7 @@ static uint64_t compact_rows(Context& c, const Layer& PL, const Layer& RL, const
   hipLaunchKernelGGL(k_row_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, ro.isect, np, no_agg ? 1 : 0, fs, fm);
   GSJ_LAUNCHED();
   const uint64_t nsingle = scan_u32_to_u64(c, fs, offs, np);
+  if (single_offs) *single_offs = offs;
   hipLaunchKernelGGL(k_emit_single, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, fs, offs, np, no_agg ? 1 : 0, ro.area, ro.ev, ro.evb,
                      orow, rp.want_area ? oarea : nullptr, rp.want_relate ? ocode : nullptr);
   GSJ_LAUNCHED();
@@ -217,6 +228,85 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
     if (cb && cb(user, hrow, harea, hcode, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "join cancelled by the row callback");
   });
   if (st) *st = s;
+}
+
+bool join_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o) {
+  // rows map one to one onto candidates (no piece -> parent folding), so a request's rows are a contiguous run
+  return o.has_window && !o.cancel && !P.points && !R.points && R.has_bvh && P.n && R.n && probes_batchable(probes_of(P)) &&
+         (o.no_agg || (!P.poly.pfirst && !R.poly.pfirst));
+}
+
+bool join_compatible(const gsj_join_opts& a, const gsj_join_opts& b) {
+  return a.want_area == b.want_area && a.want_relate == b.want_relate && a.upper == b.upper && a.snap_half_ulp == b.snap_half_ulp &&
+         a.no_agg == b.no_agg && a.batch_q == b.batch_q && a.max_pairs == b.max_pairs;
+}
+
+void join_batch_run(Context& c, const Layer& PL, const Layer& RL, std::vector<BatchJoin>& reqs) {
+  const uint32_t k = (uint32_t)reqs.size();
+  GSJ_REQUIRE(k >= 1 && k <= MAX_WINDOWS, GSJ_E_ARG, "batch size out of range");
+  const gsj_join_opts& o = *reqs[0].o;
+  WindowSet ws;
+  ws.k = k;
+  for (uint32_t r = 0; r < k; r++) {
+    GSJ_REQUIRE(join_batchable(PL, RL, *reqs[r].o) && join_compatible(o, *reqs[r].o), GSJ_E_ARG, "request cannot join this batch");
+    for (int j = 0; j < 4; j++) ws.w[r][j] = reqs[r].o->window[j];
+  }
+  hipStream_t stream = c.stream;
+  RefineParams rp;
+  rp.want_area = o.want_area ? 1 : 0;
+  rp.want_relate = o.want_relate ? 1 : 0;
+  rp.snap_half_ulp = o.snap_half_ulp ? 1 : 0;
+  gsj_join_stats s{};
+  std::vector<uint64_t> cand(k, 0), hits(k, 0);
+  for_each_candidate_multi(c, probes_of(PL), RL.bvh, o.upper != 0, o, ws, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds& sb) {
+    for (uint32_t r = 0; r < k; r++) cand[r] += sb.h[r + 1] - sb.h[r];
+    WallTimer t2;
+    RefineOut ro;
+    ro.isect = c.s_flag.as<uint8_t>(np + 16);
+    ro.area = rp.want_area ? c.s_area.as<double>(np + 16) : nullptr;
+    ro.ev = rp.want_relate ? c.s_ev.as<double4>(np + 16) : nullptr;
+    ro.evb = rp.want_relate ? c.s_evb.as<uint32_t>(np + 16) : nullptr;
+    s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro, &s);
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.refine_ms += t2.ms();
+    WallTimer t3;
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
+    double* oarea = c.s_out_area.as<double>(np + 16);
+    uint32_t* ocode = c.s_out_code.as<uint32_t>(np + 16);
+    const uint64_t* offs = nullptr;
+    const uint64_t nrows = compact_rows(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode, &offs);
+    uint64_t* drlo = c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1)) + 3 * (MAX_WINDOWS + 1);   // after the windows' bounds
+    hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, stream, sb.d, k, offs, np, nrows, drlo);
+    GSJ_LAUNCHED();
+    std::vector<uint64_t> rlo(k + 1);
+    GSJ_HIP(hipMemcpyAsync(rlo.data(), drlo, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.compact_ms += t3.ms();
+    if (nrows == 0) return;
+    WallTimer t4;
+    uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nrows);
+    double* harea = rp.want_area ? c.h_area.as<double>(nrows) : nullptr;
+    uint32_t* hcode = rp.want_relate ? c.h_code.as<uint32_t>(nrows) : nullptr;
+    GSJ_HIP(hipMemcpyAsync(hrow, orow, nrows * 8, hipMemcpyDeviceToHost, stream));
+    if (harea) GSJ_HIP(hipMemcpyAsync(harea, oarea, nrows * 8, hipMemcpyDeviceToHost, stream));
+    if (hcode) GSJ_HIP(hipMemcpyAsync(hcode, ocode, nrows * 4, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.copy_ms += t4.ms();
+    for (uint32_t r = 0; r < k; r++) {
+      const uint64_t a = rlo[r], n = rlo[r + 1] - rlo[r];
+      hits[r] += n;
+      BatchJoin& q = reqs[r];
+      if (n == 0 || !q.cb || q.rc != GSJ_OK) continue;
+      if (q.cb(q.user, hrow + 2 * a, harea ? harea + a : nullptr, hcode ? hcode + a : nullptr, (size_t)n)) q.rc = GSJ_E_CANCEL;
+    }
+  });
+  for (uint32_t r = 0; r < k; r++) {
+    if (!reqs[r].st) continue;
+    gsj_join_stats& st = *reqs[r].st;
+    st = s;
+    st.candidates = cand[r];
+    st.hits = hits[r];
+  }
 }
 
 }  // namespace gsj

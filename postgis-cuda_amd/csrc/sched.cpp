@@ -7,6 +7,10 @@
 // scratch estimate fits the budget, so requests that arrive while others run are admitted as lanes and memory free
 // up, and their kernels overlap on the device (HIP runs kernels of different streams concurrently). Each lane runs
 // the same code as the direct entry points, so a request's rows are the same alone and in the mix.
+//
+// Windowed joins of the same layers and options are also batched: a lane that admits one takes up to max_batch - 1
+// more such requests from anywhere in the queue and runs them as one candidate search and one refinement pass
+// (join_batch_run), which pays a small request's chain of launches and synchronisations once per batch.
 #include "capi_internal.h"
 #include <chrono>
 #include <condition_variable>
@@ -25,6 +29,7 @@ struct Req {
   uint64_t est = 0;   // scratch estimate, bytes
   Clock::time_point t_submit, t_start, t_end;
   int lane = -1;
+  int batch = 1;   // requests run together in its pass
   bool done = false;
   int rc = GSJ_OK;
   std::string msg;
@@ -53,11 +58,38 @@ struct gsj_sched {
   uint64_t next_ticket = 1;
   uint64_t budget = 0, in_use = 0;
   int running = 0;
+  int max_batch = 16;
   bool stop = false;
 
   void worker(int lane);
   void run(gsj::Context& c, Req& q);
+  void run_batch(gsj::Context& c, std::vector<Req*>& group);
+  bool batch_head(const Req& q) const {
+    return max_batch > 1 && q.r.kind == GSJ_REQ_JOIN && gsj::join_batchable(q.r.P->L, q.r.R->L, q.r.opts);
+  }
+  bool batch_with(const Req& head, const Req& x) const {
+    return x.r.kind == GSJ_REQ_JOIN && x.r.P == head.r.P && x.r.R == head.r.R && gsj::join_batchable(x.r.P->L, x.r.R->L, x.r.opts) &&
+           gsj::join_compatible(head.r.opts, x.r.opts);
+  }
 };
+
+void gsj_sched::run_batch(gsj::Context& c, std::vector<Req*>& group) {
+  std::vector<gsj::BatchJoin> b(group.size());
+  for (size_t i = 0; i < group.size(); i++) {
+    b[i].o = &group[i]->r.opts;
+    b[i].cb = group[i]->r.rows_cb;
+    b[i].user = group[i]->r.user;
+    b[i].st = &group[i]->stats;
+  }
+  const int rc = gsj::guarded_status([&] { gsj::join_batch_run(c, group[0]->r.P->L, group[0]->r.R->L, b); });
+  const std::string msg = rc == GSJ_OK ? std::string() : gsj::g_last_error;
+  for (size_t i = 0; i < group.size(); i++) {
+    Req& q = *group[i];
+    q.batch = (int)group.size();
+    q.rc = rc != GSJ_OK ? rc : b[i].rc;
+    q.msg = rc != GSJ_OK ? msg : (b[i].rc != GSJ_OK ? std::string("join cancelled by the row callback") : std::string());
+  }
+}
 
 void gsj_sched::run(gsj::Context& c, Req& q) {
   const gsj_request& r = q.r;
@@ -91,6 +123,7 @@ void gsj_sched::run(gsj::Context& c, Req& q) {
 void gsj_sched::worker(int lane) {
   gsj::Context& c = *lanes[lane];
   (void)hipSetDevice(c.device);
+  std::vector<Req*> group;
   for (;;) {
     Req* q = nullptr;
     {
@@ -102,23 +135,41 @@ void gsj_sched::worker(int lane) {
       if (queue.empty()) return;   // stop requested and nothing left
       q = queue.front();
       queue.pop_front();
+      group.assign(1, q);
+      if (batch_head(*q))
+        for (auto it = queue.begin(); it != queue.end() && (int)group.size() < max_batch;) {
+          if (batch_with(*q, **it) && (budget == 0 || in_use + q->est + (*it)->est <= budget)) {
+            group.push_back(*it);
+            in_use += (*it)->est;
+            it = queue.erase(it);
+          } else {
+            ++it;
+          }
+        }
       in_use += q->est;
-      running++;
-      q->lane = lane;
-      q->t_start = Clock::now();
+      running += (int)group.size();
+      const Clock::time_point now = Clock::now();
+      for (Req* x : group) { x->lane = lane; x->t_start = now; }
     }
-    const int rc = gsj::guarded_status([&] { run(c, *q); });
-    const std::string msg = rc == GSJ_OK ? std::string() : gsj::g_last_error;
-    gsj::layer_release(const_cast<gsj_layer*>(q->r.P));
-    gsj::layer_release(const_cast<gsj_layer*>(q->r.R));
+    if (group.size() == 1) {
+      q->rc = gsj::guarded_status([&] { run(c, *q); });
+      q->msg = q->rc == GSJ_OK ? std::string() : gsj::g_last_error;
+    } else {
+      run_batch(c, group);
+    }
+    for (Req* x : group) {
+      gsj::layer_release(const_cast<gsj_layer*>(x->r.P));
+      gsj::layer_release(const_cast<gsj_layer*>(x->r.R));
+    }
     {
       std::lock_guard<std::mutex> lk(mu);
-      in_use -= q->est;
-      running--;
-      q->rc = rc;
-      q->msg = msg;
-      q->t_end = Clock::now();
-      q->done = true;
+      const Clock::time_point now = Clock::now();
+      for (Req* x : group) {
+        in_use -= x->est;
+        running--;
+        x->t_end = now;
+        x->done = true;
+      }
     }
     cv_done.notify_all();
     cv_work.notify_all();
@@ -157,6 +208,15 @@ void gsj_sched_destroy(gsj_sched* s) {
     (void)hipStreamSynchronize(l->stream);
   }
   delete s;
+}
+
+int gsj_sched_set_batch(gsj_sched* s, int max_requests) {
+  return gsj::guarded_status([&] {
+    GSJ_REQUIRE(s, GSJ_E_ARG, "null scheduler");
+    GSJ_REQUIRE(max_requests >= 1 && max_requests <= (int)gsj::MAX_WINDOWS, GSJ_E_ARG, "batch size must be in [1, 32]");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_batch = max_requests;
+  });
 }
 
 int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket) {
@@ -204,6 +264,7 @@ int gsj_wait(gsj_sched* s, uint64_t ticket, gsj_join_stats* stats, gsj_request_t
     times->run_ms = ms_between(q->t_start, q->t_end);
     times->total_ms = ms_between(q->t_submit, q->t_end);
     times->lane = q->lane;
+    times->batch = q->batch;
   }
   gsj::g_last_error = q->msg;
   return q->rc;

@@ -74,34 +74,104 @@ __global__ void k_win_scatter(const uint32_t* __restrict__ f, const uint64_t* __
   if (i < n && f[i]) out[off[i]] = order ? order[i] : i;
 }
 
-// tiles whose box meets the window (a probe meeting it lies in such a tile)
-__global__ void k_win_tile_flags(const float4* __restrict__ tiles, uint32_t nt, double x0, double y0, double x1, double y1, uint32_t* __restrict__ f) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  const float4 b = tiles[t];
-  f[t] = ((double)b.x <= x1 && (double)b.z >= x0 && (double)b.y <= y1 && (double)b.w >= y0) ? 1u : 0u;
+__device__ __forceinline__ bool box_meets(const float4 b, const double* w) {
+  return (double)b.x <= w[2] && (double)b.z >= w[0] && (double)b.y <= w[3] && (double)b.w >= w[1];
 }
-__global__ void k_win_tile_list(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint32_t nt, uint32_t* __restrict__ out) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nt && f[t]) out[off[t]] = t;
+
+// Windows over the tile boxes. Entry e = r * nt + t stands for tile t of window r; a probe meeting window r lies in
+// a tile whose box meets it.
+__global__ void k_win_tile_flags(const float4* __restrict__ tiles, uint32_t nt, const WindowSet ws, uint32_t* __restrict__ f) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ws.k * nt) return;
+  const uint32_t r = e / nt;
+  f[e] = box_meets(tiles[e - r * nt], ws.w[r]) ? 1u : 0u;
 }
-// membership of the probes of the selected tiles: slot k * PROBE_TILE + j holds walk position tl[k] * PROBE_TILE + j
-__global__ void k_win_tile_probes(const float4* __restrict__ bb, const uint32_t* __restrict__ order, uint32_t n, const uint32_t* __restrict__ tl,
-                                  uint64_t nslots, double x0, double y0, double x1, double y1, uint32_t* __restrict__ f) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nslots) return;
-  const uint64_t pos = (uint64_t)tl[k / PROBE_TILE] * PROBE_TILE + k % PROBE_TILE;
-  bool in = false;
-  if (pos < n) {
-    const float4 b = bb[order[pos]];
-    in = (double)b.x <= x1 && (double)b.z >= x0 && (double)b.y <= y1 && (double)b.w >= y0;
-  }
-  f[k] = in ? 1u : 0u;
+__global__ void k_win_tile_list(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, uint32_t ne, uint32_t* __restrict__ out) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne && f[e]) out[off[e]] = e;
 }
-__global__ void k_win_tile_scatter(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, const uint32_t* __restrict__ order,
+// first selected tile of each window: tstart[r] = selected entries before r * nt (tstart[k] = all of them)
+__global__ void k_win_tstart(const uint64_t* __restrict__ off, uint32_t nt, uint32_t k, uint64_t nsel, uint64_t* __restrict__ tstart) {
+  const uint32_t r = threadIdx.x;
+  if (r <= k) tstart[r] = r < k ? off[(uint64_t)r * nt] : nsel;
+}
+// membership of the probes of the selected tiles: slot s holds walk position t * PROBE_TILE + s % PROBE_TILE of the
+// entry tl[s / PROBE_TILE] = r * nt + t, tested against window r
+__global__ void k_win_tile_probes(const float4* __restrict__ bb, const uint32_t* __restrict__ order, uint32_t n, uint32_t nt,
+                                  const uint32_t* __restrict__ tl, uint64_t nslots, const WindowSet ws, uint32_t* __restrict__ f) {
+  const uint64_t sl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslots) return;
+  const uint32_t e = tl[sl / PROBE_TILE], r = e / nt;
+  const uint64_t pos = (uint64_t)(e - r * nt) * PROBE_TILE + sl % PROBE_TILE;
+  f[sl] = (pos < n && box_meets(bb[order[pos]], ws.w[r])) ? 1u : 0u;
+}
+__global__ void k_win_tile_scatter(const uint32_t* __restrict__ f, const uint64_t* __restrict__ off, const uint32_t* __restrict__ order, uint32_t nt,
                                    const uint32_t* __restrict__ tl, uint64_t nslots, uint32_t* __restrict__ out) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nslots && f[k]) out[off[k]] = order[(uint64_t)tl[k / PROBE_TILE] * PROBE_TILE + k % PROBE_TILE];
+  const uint64_t sl = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslots || !f[sl]) return;
+  const uint32_t e = tl[sl / PROBE_TILE];
+  out[off[sl]] = order[(uint64_t)(e % nt) * PROBE_TILE + sl % PROBE_TILE];
+}
+// first member position of each window: mstart[r] = members before slot tstart[r] * PROBE_TILE (mstart[k] = all)
+__global__ void k_win_mstart(const uint64_t* __restrict__ tstart, uint32_t k, const uint64_t* __restrict__ off, uint64_t nslots, uint64_t nmem,
+                             uint64_t* __restrict__ mstart) {
+  const uint32_t r = threadIdx.x;
+  if (r > k) return;
+  const uint64_t sl = tstart[r] * PROBE_TILE;
+  mstart[r] = (r < k && sl < nslots) ? off[sl] : nmem;
+}
+// candidate boundaries of the windows inside the sub-batch of walk positions [base, base + nsub) with candidate
+// offsets offs[0 .. nsub) and np candidates in all
+__global__ void k_seg_bounds(const uint64_t* __restrict__ mstart, uint32_t k, uint64_t base, uint64_t nsub, const uint64_t* __restrict__ offs, uint64_t np,
+                             uint64_t* __restrict__ lo) {
+  const uint32_t r = threadIdx.x;
+  if (r > k) return;
+  const uint64_t m = mstart[r];
+  lo[r] = m <= base ? 0 : (m >= base + nsub ? np : offs[m - base]);
+}
+
+// Members of the k windows through the tile boxes: their probe ids (window after window, each in walk order) in
+// *list and the member boundaries mstart[0 .. k] on the device; returns the member count.
+uint32_t window_members_tiled(Context& c, const Probes& pr, const WindowSet& ws, uint32_t*& list, uint64_t*& mstart) {
+  hipStream_t stream = c.stream;
+  const uint32_t nt = (pr.n + PROBE_TILE - 1) / PROBE_TILE;
+  GSJ_REQUIRE(ws.k >= 1 && ws.k <= MAX_WINDOWS, GSJ_E_ARG, "window count out of range");
+  GSJ_REQUIRE((uint64_t)ws.k * nt < (1ull << 32), GSJ_E_LIMIT, "too many windows for the layer's tile count");
+  for (uint32_t r = 0; r < ws.k; r++)
+    GSJ_REQUIRE(ws.w[r][0] <= ws.w[r][2] && ws.w[r][1] <= ws.w[r][3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
+  const uint32_t ne = ws.k * nt;
+  uint32_t* tf = c.s_wtile.as<uint32_t>((uint64_t)ne + 16);
+  uint64_t* toff = c.s_wtoff.as<uint64_t>((uint64_t)ne + 16);
+  uint64_t* tstart = c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1));
+  mstart = tstart + 2 * (MAX_WINDOWS + 1);
+  hipLaunchKernelGGL(k_win_tile_flags, dim3(div_up(ne, 256)), dim3(256), 0, stream, pr.tiles, nt, ws, tf);
+  GSJ_LAUNCHED();
+  const uint64_t nsel = scan_u32_to_u64(c, tf, toff, ne);
+  hipLaunchKernelGGL(k_win_tstart, dim3(1), dim3(64), 0, stream, toff, nt, ws.k, nsel, tstart);
+  GSJ_LAUNCHED();
+  uint32_t* tl = c.s_qidx.as<uint32_t>(nsel + 16);
+  hipLaunchKernelGGL(k_win_tile_list, dim3(div_up(ne, 256)), dim3(256), 0, stream, tf, toff, ne, tl);
+  GSJ_LAUNCHED();
+  const uint64_t nslots = nsel * PROBE_TILE;
+  uint64_t nmem = 0;
+  list = nullptr;
+  if (nslots) {
+    uint32_t* f = c.s_win.as<uint32_t>(nslots + 16);
+    uint64_t* off = c.s_win_off.as<uint64_t>(nslots + 16);
+    hipLaunchKernelGGL(k_win_tile_probes, dim3(div_up(nslots, 256)), dim3(256), 0, stream, pr.own_bb, pr.order, pr.n, nt, tl, nslots, ws, f);
+    GSJ_LAUNCHED();
+    nmem = scan_u32_to_u64(c, f, off, nslots);
+    GSJ_REQUIRE(nmem < (1ull << 32), GSJ_E_LIMIT, "window members exceed 2^32");
+    list = c.s_qlist.as<uint32_t>(nmem + 16);
+    hipLaunchKernelGGL(k_win_tile_scatter, dim3(div_up(nslots, 256)), dim3(256), 0, stream, f, off, pr.order, nt, tl, nslots, list);
+    GSJ_LAUNCHED();
+    hipLaunchKernelGGL(k_win_mstart, dim3(1), dim3(64), 0, stream, tstart, ws.k, off, nslots, nmem, mstart);
+    GSJ_LAUNCHED();
+  } else {
+    hipLaunchKernelGGL(k_win_mstart, dim3(1), dim3(64), 0, stream, tstart, ws.k, nullptr, (uint64_t)0, (uint64_t)0, mstart);
+    GSJ_LAUNCHED();
+  }
+  return (uint32_t)nmem;
 }
 
 }  // namespace
@@ -137,72 +207,23 @@ void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbas
   GSJ_LAUNCHED();
 }
 
-void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
-                              const PairsFn& fn) {
+namespace {
+
+// The probe batches and candidate sub-batches over probe positions 0 .. nQ (probe ids directly, or qidx[pos]); with
+// mstart (k windows' member boundaries on the device), every sub-batch also carries the windows' candidate
+// boundaries.
+void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s, const uint32_t* qidx,
+                  uint32_t nQ, const std::vector<uint32_t>& hq, const uint64_t* mstart, uint32_t k, const SegPairsFn& fn) {
   hipStream_t stream = c.stream;
   const uint32_t batch_q = o.batch_q ? o.batch_q : (1u << 22);
   const uint64_t max_pairs = o.max_pairs ? o.max_pairs : (128ull << 20);
   const float4* qbb = pr.qbb;
   static const std::vector<uint32_t> none;
   const std::vector<uint32_t>& hp = pr.h_parent ? *pr.h_parent : none;
-  // probe positions 0 .. nQ: probe ids directly, through the walk order, or through the window's member list (in walk
-  // order); a subdivided layer walks in id order (its parents' pieces are contiguous)
-  uint32_t nQ = pr.n;
-  const uint32_t* qidx = pr.h_parent && !pr.h_parent->empty() ? nullptr : pr.order;
-  std::vector<uint32_t> hq;
-  if (o.has_window && pr.n) {
-    const double* w = o.window;
-    GSJ_REQUIRE(w[0] <= w[2] && w[1] <= w[3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
-    WallTimer tw;
-    uint32_t* list = nullptr;
-    if (qidx && pr.tiles) {
-      // through the tile boxes of the walk order: only the tiles meeting the window are scanned, and their members
-      // come out in walk order
-      const uint32_t nt = (pr.n + PROBE_TILE - 1) / PROBE_TILE;
-      uint32_t* tf = c.s_win.as<uint32_t>((uint64_t)nt + 16);
-      uint64_t* toff = c.s_win_off.as<uint64_t>((uint64_t)nt + 16);
-      hipLaunchKernelGGL(k_win_tile_flags, dim3(div_up(nt, 256)), dim3(256), 0, stream, pr.tiles, nt, w[0], w[1], w[2], w[3], tf);
-      GSJ_LAUNCHED();
-      const uint32_t nsel = (uint32_t)scan_u32_to_u64(c, tf, toff, nt);
-      uint32_t* tl = c.s_qidx.as<uint32_t>((uint64_t)nsel + 16);
-      hipLaunchKernelGGL(k_win_tile_list, dim3(div_up(nt, 256)), dim3(256), 0, stream, tf, toff, nt, tl);
-      GSJ_LAUNCHED();
-      const uint64_t nslots = (uint64_t)nsel * PROBE_TILE;
-      nQ = 0;
-      if (nslots) {
-        uint32_t* f = c.s_win.as<uint32_t>(nslots + 16);
-        uint64_t* off = c.s_win_off.as<uint64_t>(nslots + 16);
-        hipLaunchKernelGGL(k_win_tile_probes, dim3(div_up(nslots, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, tl, nslots, w[0], w[1], w[2],
-                           w[3], f);
-        GSJ_LAUNCHED();
-        nQ = (uint32_t)scan_u32_to_u64(c, f, off, nslots);
-        list = c.s_qlist.as<uint32_t>((uint64_t)nQ + 16);
-        hipLaunchKernelGGL(k_win_tile_scatter, dim3(div_up(nslots, 256)), dim3(256), 0, stream, f, off, qidx, tl, nslots, list);
-        GSJ_LAUNCHED();
-      }
-    } else {
-      uint32_t* f = c.s_win.as<uint32_t>((uint64_t)pr.n + 16);
-      hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, w[0], w[1], w[2], w[3], f);
-      GSJ_LAUNCHED();
-      if (pr.pfirst && pr.nparent) {
-        hipLaunchKernelGGL(k_win_parents, dim3(div_up(pr.nparent, 256)), dim3(256), 0, stream, pr.pfirst, pr.nparent, f);
-        GSJ_LAUNCHED();
-      }
-      uint64_t* off = c.s_win_off.as<uint64_t>((uint64_t)pr.n + 16);
-      nQ = (uint32_t)scan_u32_to_u64(c, f, off, pr.n);
-      // members written behind the flags (the flags are read by the scatter itself, so they need their own region)
-      list = c.s_qlist.as<uint32_t>((uint64_t)nQ + 16);
-      hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, qidx, pr.n, list);
-      GSJ_LAUNCHED();
-    }
-    hq.resize(nQ);
-    if (nQ) GSJ_HIP(hipMemcpyAsync(hq.data(), list, (size_t)nQ * 4, hipMemcpyDeviceToHost, stream));
-    GSJ_HIP(hipStreamSynchronize(stream));
-    qidx = list;
-    s.traverse_ms += tw.ms();
-  }
   auto pid = [&](uint32_t pos) { return hp[qidx ? hq[pos] : pos]; };   // parent of the probe at a position
   std::vector<uint32_t> hcounts;
+  uint64_t* dlo = mstart ? c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1)) + (MAX_WINDOWS + 1) : nullptr;   // beside tstart, mstart
+  uint64_t* hlo = mstart ? c.h_seg.as<uint64_t>(MAX_WINDOWS + 1) : nullptr;
   for (uint32_t q0 = 0, qn; q0 < nQ; q0 = qn) {
     qn = (uint32_t)std::min<uint64_t>((uint64_t)q0 + batch_q, nQ);
     if (!hp.empty()) while (qn < nQ && pid(qn) == pid(qn - 1)) qn++;
@@ -242,11 +263,83 @@ void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool 
       uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
       uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
       traverse(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb);
+      SegBounds sb;
+      if (mstart) {
+        hipLaunchKernelGGL(k_seg_bounds, dim3(1), dim3(64), 0, stream, mstart, k, (uint64_t)(q0 + s0), (uint64_t)(s1 - s0), offs, np, dlo);
+        GSJ_LAUNCHED();
+        GSJ_HIP(hipMemcpyAsync(hlo, dlo, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, stream));
+        sb.k = k; sb.h = hlo; sb.d = dlo;
+      }
       GSJ_HIP(hipStreamSynchronize(stream));
       s.traverse_ms += t1.ms();
-      fn(pa, pb, np);
+      fn(pa, pb, np, sb);
     }
   }
+}
+
+}  // namespace
+
+bool probes_batchable(const Probes& pr) { return pr.order && pr.tiles && !pr.h_parent; }
+
+void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
+                              const PairsFn& fn) {
+  hipStream_t stream = c.stream;
+  // probe positions 0 .. nQ: probe ids directly, through the walk order, or through the window's member list (in walk
+  // order); a subdivided layer walks in id order (its parents' pieces are contiguous)
+  uint32_t nQ = pr.n;
+  const uint32_t* qidx = pr.h_parent && !pr.h_parent->empty() ? nullptr : pr.order;
+  std::vector<uint32_t> hq;
+  if (o.has_window && pr.n) {
+    const double* w = o.window;
+    GSJ_REQUIRE(w[0] <= w[2] && w[1] <= w[3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
+    WallTimer tw;
+    uint32_t* list = nullptr;
+    if (qidx && pr.tiles) {
+      // through the tile boxes of the walk order: only the tiles meeting the window are scanned, and their members
+      // come out in walk order
+      WindowSet ws;
+      ws.k = 1;
+      for (int j = 0; j < 4; j++) ws.w[0][j] = w[j];
+      uint64_t* mstart = nullptr;
+      nQ = window_members_tiled(c, pr, ws, list, mstart);
+    } else {
+      uint32_t* f = c.s_win.as<uint32_t>((uint64_t)pr.n + 16);
+      hipLaunchKernelGGL(k_win_flags, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, pr.own_bb, qidx, pr.n, w[0], w[1], w[2], w[3], f);
+      GSJ_LAUNCHED();
+      if (pr.pfirst && pr.nparent) {
+        hipLaunchKernelGGL(k_win_parents, dim3(div_up(pr.nparent, 256)), dim3(256), 0, stream, pr.pfirst, pr.nparent, f);
+        GSJ_LAUNCHED();
+      }
+      uint64_t* off = c.s_win_off.as<uint64_t>((uint64_t)pr.n + 16);
+      nQ = (uint32_t)scan_u32_to_u64(c, f, off, pr.n);
+      // members written behind the flags (the flags are read by the scatter itself, so they need their own region)
+      list = c.s_qlist.as<uint32_t>((uint64_t)nQ + 16);
+      hipLaunchKernelGGL(k_win_scatter, dim3(div_up(pr.n, 256)), dim3(256), 0, stream, f, off, qidx, pr.n, list);
+      GSJ_LAUNCHED();
+    }
+    if (pr.h_parent && !pr.h_parent->empty()) {   // parents of the members, for keeping their pieces together
+      hq.resize(nQ);
+      if (nQ) GSJ_HIP(hipMemcpyAsync(hq.data(), list, (size_t)nQ * 4, hipMemcpyDeviceToHost, stream));
+    }
+    GSJ_HIP(hipStreamSynchronize(stream));
+    qidx = list;
+    s.traverse_ms += tw.ms();
+  }
+  walk_batches(c, pr, t, upper, o, s, qidx, nQ, hq, nullptr, 0, [&](uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds&) { fn(pa, pb, np); });
+}
+
+void for_each_candidate_multi(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, const WindowSet& ws,
+                              gsj_join_stats& s, const SegPairsFn& fn) {
+  GSJ_REQUIRE(probes_batchable(pr), GSJ_E_ARG, "batched windows need probes with tile boxes and no parents");
+  if (pr.n == 0) return;
+  WallTimer tw;
+  uint32_t* list = nullptr;
+  uint64_t* mstart = nullptr;
+  const uint32_t nQ = window_members_tiled(c, pr, ws, list, mstart);
+  GSJ_HIP(hipStreamSynchronize(c.stream));
+  s.traverse_ms += tw.ms();
+  static const std::vector<uint32_t> none;
+  walk_batches(c, pr, t, upper, o, s, list, nQ, none, mstart, ws.k, fn);
 }
 
 }  // namespace gsj

@@ -69,7 +69,7 @@ class Request(C.Structure):
 
 
 class RequestTimes(C.Structure):
-    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double), ("lane", C.c_int)]
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double), ("lane", C.c_int), ("batch", C.c_int)]
 
 
 _lib = None
@@ -80,7 +80,7 @@ EXPORTS = [
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
     "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
     "gsj_points_intersects_buffered",
-    "gsj_sched_create", "gsj_sched_destroy", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
+    "gsj_sched_create", "gsj_sched_destroy", "gsj_sched_set_batch", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
     "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_generate", "gsj_generate_points",
 ]
@@ -111,7 +111,7 @@ def load(path: str | None = None):
         "gsj_points_dwithin": (i32, [vp, vp, vp, C.c_double, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_intersects_buffered": (i32, [vp, vp, vp, C.c_double, C.c_double, i32, u8p, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_intersects_buffered": (i32, [vp, vp, vp, C.c_double, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
-        "gsj_sched_create": (i32, [vp, i32, u64, C.POINTER(vp)]), "gsj_sched_destroy": (None, [vp]),
+        "gsj_sched_create": (i32, [vp, i32, u64, C.POINTER(vp)]), "gsj_sched_destroy": (None, [vp]), "gsj_sched_set_batch": (i32, [vp, i32]),
         "gsj_submit": (i32, [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "gsj_wait": (i32, [vp, u64, C.POINTER(JoinStats), C.POINTER(RequestTimes)]), "gsj_poll": (i32, [vp, u64]),
         "gsj_de9im_str": (None, [u32, C.c_char_p]), "gsj_de9im_transpose": (u32, [u32]), "gsj_de9im_match": (i32, [u32, C.c_char_p]),
@@ -527,12 +527,19 @@ class Scheduler:
     """Served path: requests on resident layers of `ctx`, run by `lanes` lanes with continuous admission.
 
     submit() returns a ticket; wait(ticket) returns the request's Rows (as the direct call would) with
-    `times` (queue / run / total ms, lane) in its stats. collect=False passes no callback: rows are computed, counted
-    (stats["hits"]) and copied to the host, but not delivered."""
+    `times` (queue / run / total ms, lane, batch) in its stats. collect=False passes no callback: rows are computed,
+    counted (stats["hits"]) and copied to the host, but not delivered. batch: most windowed joins run in one pass
+    (gsj_sched_set_batch; 1 = each alone)."""
 
-    def __init__(self, ctx: "Context", lanes: int = 4, scratch_budget: int = 0):
+    def __init__(self, ctx: "Context", lanes: int = 4, scratch_budget: int = 0, batch: int = 16):
+        self.h = None
         h = C.c_void_p()
         _check(load().gsj_sched_create(ctx.h, lanes, scratch_budget, C.byref(h)))
+        rc = load().gsj_sched_set_batch(h, batch)
+        if rc != 0:
+            msg = _lib.gsj_last_error().decode()
+            load().gsj_sched_destroy(h)
+            raise GsjError(rc, msg)
         self.h = h
         self.ctx = ctx
         self._pending = {}
@@ -596,7 +603,7 @@ class Scheduler:
         rc = load().gsj_wait(self.h, ticket, C.byref(st), C.byref(tm))
         _check(rc)
         stats = st.as_dict()
-        stats["times"] = {"queue_ms": tm.queue_ms, "run_ms": tm.run_ms, "total_ms": tm.total_ms, "lane": tm.lane}
+        stats["times"] = {"queue_ms": tm.queue_ms, "run_ms": tm.run_ms, "total_ms": tm.total_ms, "lane": tm.lane, "batch": tm.batch}
         pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
         if kind == REQ_CLUSTER:
             return Rows(np.zeros((0, 2), np.uint32), None, None, None, stats, None, root[:keep[0].nroot])

@@ -123,3 +123,81 @@ def test_layers_outlive_their_owner_handle_and_errors_stay_with_the_request(ctx)
         assert len(s.wait(good).pairs) > 0
     finally:
         s.close()
+
+
+def _rows_in_order(a, b):
+    """identical rows in identical order (batching keeps every request's own row order)"""
+    assert np.array_equal(a.pairs, b.pairs)
+    for f in ("area", "code"):
+        x, y = getattr(a, f), getattr(b, f)
+        if x is None or y is None:   # an empty result may carry no column
+            assert (x is None or len(x) == 0) and (y is None or len(y) == 0) or (x is None and y is None)
+        else:
+            assert np.array_equal(x, y)
+
+
+def test_batched_windowed_joins_equal_their_own_calls(ctx):
+    """Windowed joins queued behind a long request run as batches (times["batch"] > 1); each one's rows, in order,
+    and its candidates and hits equal its direct call, across area / relate groups, overlapping, empty, point-sized and
+    layer-covering windows, and tiny probe batches and sub-batches that cut through the requests."""
+    M = _mesh(70, 7)
+    L = ctx.polys(M)
+    big = ctx.polys(gsj.subdivide(_mesh(150, 8), 10))
+    x0, y0 = M.xy[:, 0].min(), M.xy[:, 1].min()
+    W, H = M.xy[:, 0].max() - x0, M.xy[:, 1].max() - y0
+    rng = np.random.default_rng(11)
+    wins = []
+    for i in range(44):
+        f = rng.uniform(0, 1, 2)
+        s = rng.uniform(0.02, 0.4)
+        wins.append((x0 + f[0] * W, y0 + f[1] * H, x0 + f[0] * W + s * W, y0 + f[1] * H + s * H))
+    wins[3] = (x0 - 1, y0 - 1, x0 + W + 1, y0 + H + 1)                 # every probe
+    wins[7] = (x0 + W + 1, y0 + H + 1, x0 + W + 2, y0 + H + 2)         # none
+    wins[12] = (x0 + W / 2, y0 + H / 2, x0 + W / 2, y0 + H / 2)        # a point
+    wins[20] = wins[21]                                                # twice the same window in one batch
+    kws = []
+    for i, w in enumerate(wins):
+        kw = dict(window=w, upper=True)
+        if i % 3 == 0:
+            kw["want_area"] = True
+        elif i % 3 == 1:
+            kw["want_relate"] = True
+        if i >= 36:
+            kw.update(batch_q=5, max_pairs=40)
+        kws.append(kw)
+    s = gsj.Scheduler(ctx, lanes=1, batch=16)
+    try:
+        t0 = s.submit(gsj.REQ_JOIN, big, big, want_relate=True, want_area=True, upper=True)   # holds the lane
+        tickets = [s.submit(gsj.REQ_JOIN, L, L, **kw) for kw in kws]
+        s.wait(t0)
+        got = [s.wait(t) for t in tickets]
+    finally:
+        s.close()
+    batched = [g.stats["times"]["batch"] for g in got]
+    assert max(batched) > 1, "no request ran in a batch"
+    assert sum(b > 1 for b in batched) >= len(got) // 2
+    for kw, g in zip(kws, got):
+        d = ctx.join(L, L, **kw)
+        _rows_in_order(g, d)
+        assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
+    assert len(got[3].pairs) > 0 and len(got[7].pairs) == 0
+
+
+def test_batching_off_and_unbatchable_requests_run_alone(ctx):
+    M = _mesh(30, 9)
+    L = ctx.polys(M)
+    S = ctx.polys(gsj.subdivide(_mesh(30, 10), 8))
+    x0, y0 = M.xy[:, 0].min(), M.xy[:, 1].min()
+    w = (x0, y0, x0 + 0.01, y0 + 0.01)
+    for batch, reqs in ((1, [(L, dict(window=w, upper=True))] * 6),
+                        (16, [(L, dict(upper=True))] * 3 + [(S, dict(window=w, upper=True))] * 3)):
+        s = gsj.Scheduler(ctx, lanes=1, batch=batch)
+        try:
+            got = [s.wait(t) for t in [s.submit(gsj.REQ_JOIN, A, A, **kw) for A, kw in reqs]]
+        finally:
+            s.close()
+        assert all(g.stats["times"]["batch"] == 1 for g in got)
+        for (A, kw), g in zip(reqs, got):
+            _rows_in_order(g, ctx.join(A, A, **kw))
+    with pytest.raises(gsj.GsjError, match="batch"):
+        gsj.Scheduler(ctx, lanes=1, batch=33)
