@@ -122,7 +122,7 @@ def serving_leg(gsj, ctx, args):
     log(f"serving mix: {n} requests, {int(sizes.sum())} polygons, generated + resident in {time.time() - t0:.1f}s")
 
     def run(lanes, arrivals):
-        s = gsj.Scheduler(ctx, lanes=lanes)
+        s = gsj.Scheduler(ctx, lanes=lanes, batch=1 if lanes == 1 else 16)   # serial: one request at a time
         try:
             for k, A, B, kw in reqs:   # warm-up: every lane's scratch grows to its working size
                 s.submit(k, A, B, collect=False, **kw)
@@ -235,7 +235,8 @@ def national_leg(gsj, ctx, args):
             reqs.append((gsj.REQ_JOIN, dict(want_area=kinds[i] == 0, want_relate=kinds[i] == 1, upper=True, window=win)))
 
     def run(lanes, arrivals):
-        s = gsj.Scheduler(ctx, lanes=lanes)
+        # serial: one request at a time, as the reference serves them; served: lanes plus batches of windowed joins
+        s = gsj.Scheduler(ctx, lanes=lanes, batch=1 if lanes == 1 else 16)
         try:
             for k, kw in reqs[: min(nreq, 16)]:   # warm-up: lanes' scratch
                 s.wait(s.submit(k, L, L, collect=False, **kw))
@@ -254,9 +255,11 @@ def national_leg(gsj, ctx, args):
         cand = sum(r.stats["candidates"] for r in res)
         rows = sum(r.stats["hits"] for r in res)
         lat = np.array([r.stats["times"]["total_ms"] for r in res])
+        nb = np.array([r.stats["times"]["batch"] for r in res])
         return {"lanes": lanes, "wall_s": round(wall, 3), "refined_pairs_per_s": round(cand / wall, 1), "rows_per_s": round(rows / wall, 1),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3), "candidates": int(cand),
-                "rows": int(rows)}, [(r.stats["candidates"], r.stats["hits"]) for r in res]
+                "rows": int(rows), "batched_requests": int((nb > 1).sum()),
+                "mean_batch": round(float(nb[nb > 1].mean()), 2) if (nb > 1).any() else 1.0}, [(r.stats["candidates"], r.stats["hits"]) for r in res]
 
     # the same requests at the configuration's top arrival rate and at an overload rate (both paths saturated: the
     # refined pairs/s are then each path's capacity)

@@ -23,13 +23,34 @@ namespace gsj {
 namespace {
 
 // probe boxes of polygons widened by d metres (buffer.cuh:703-715)
-__global__ void k_expand_polys(const DPolys P, double d, float4* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+__device__ __forceinline__ float4 expand_poly_box(const DPolys& P, uint32_t i, double d) {
   const double2 o = P.origin[i], e = P.ext[i];
   double dx, dy;
   geo::deg_cover(o.y, o.y + e.y, d, true, dx, dy);
-  out[i] = make_float4(geo::f_down(o.x - dx), geo::f_down(o.y - dy), geo::f_up(o.x + e.x + dx), geo::f_up(o.y + e.y + dy));
+  return make_float4(geo::f_down(o.x - dx), geo::f_down(o.y - dy), geo::f_up(o.x + e.x + dx), geo::f_up(o.y + e.y + dy));
+}
+__global__ void k_expand_polys(const DPolys P, double d, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.n) out[i] = expand_poly_box(P, i, d);
+}
+// only the listed probes (a window's members, distinct ids), at their ids
+__global__ void k_expand_polys_list(const DPolys P, double d, const uint32_t* __restrict__ list, uint32_t n, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[list[i]] = expand_poly_box(P, list[i], d);
+}
+// batched windows: the member at position i belongs to request r (mstart[r] <= i < mstart[r + 1]) and is widened by
+// that request's distance; boxes by position
+struct DistSet {
+  uint32_t k;
+  double d[MAX_WINDOWS];
+};
+__global__ void k_expand_members(const DPolys P, const uint32_t* __restrict__ list, uint32_t n, const uint64_t* __restrict__ mstart, const DistSet ds,
+                                 float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t r = 0;
+  while (r + 1 < ds.k && mstart[r + 1] <= i) r++;
+  out[i] = expand_poly_box(P, list[i], ds.d[r]);
 }
 
 // probe boxes of points widened by d metres (buffer.cuh:1004-1016)
@@ -155,7 +176,16 @@ struct DwArgs {
   int same, report;
   uint8_t* hit;
   double* dist;
+  // batched requests: candidate p belongs to request r when seg[r] <= p < seg[r + 1], distance ds.d[r]
+  const uint64_t* seg;
+  DistSet ds;
 };
+__device__ __forceinline__ double pair_distance(const DwArgs& A, uint32_t p) {
+  if (!A.seg) return A.d;
+  uint32_t r = 0;
+  while (r + 1 < A.ds.k && A.seg[r + 1] <= p) r++;
+  return A.ds.d[r];
+}
 
 // pieces of one parent are within any distance of each other on a self-join (buffer.cuh:246-249)
 __device__ __forceinline__ bool self_piece_pair(const DwArgs& A, uint32_t a, uint32_t b) {
@@ -187,7 +217,7 @@ __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t
     }
     double dist = NAN;
     int need = 0;
-    const int r = dw_eval<G>(A.P, A.R, a, b, A.d, A.report != 0, LA, LB, CAP, gl, dist, need);
+    const int r = dw_eval<G>(A.P, A.R, a, b, pair_distance(A, p), A.report != 0, LA, LB, CAP, gl, dist, need);
     if (gl == 0) {
       if (r < 0) queue_overflow(p, need, ovf_list, ctl);
       else {
@@ -210,7 +240,7 @@ __global__ void __launch_bounds__(256) k_dw_overflow(const DwArgs A, const uint3
     const uint32_t p = list[k];
     double dist = NAN;
     int need = 0;
-    const int r = dw_eval<64>(A.P, A.R, A.pa[p], A.pb[p], A.d, A.report != 0, LA, LB, cap, gl, dist, need);
+    const int r = dw_eval<64>(A.P, A.R, A.pa[p], A.pb[p], pair_distance(A, p), A.report != 0, LA, LB, cap, gl, dist, need);
     if (gl == 0) {
       A.hit[p] = r > 0 ? 1 : 0;
       if (A.dist) A.dist[p] = r > 0 ? dist : NAN;
@@ -437,6 +467,17 @@ __global__ void k_dw_keys(const uint32_t* __restrict__ Ppar, const uint32_t* __r
   keys[off[i]] = (a << 32) | b;
   vals[off[i]] = (uint32_t)i;
 }
+// layers without parents: every candidate is its own parent pair, rows in candidate order
+__global__ void k_dw_direct(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, const uint32_t* __restrict__ f,
+                            const uint64_t* __restrict__ off, uint64_t n, const double* __restrict__ dist, uint32_t* __restrict__ orow,
+                            double* __restrict__ odist) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !f[i]) return;
+  const uint64_t o = off[i];
+  orow[2 * o] = pa[i];
+  orow[2 * o + 1] = pb[i];
+  if (odist) odist[o] = dist[i];
+}
 __global__ void k_key_heads(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ head) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < m) head[j] = (j == 0 || keys[j] != keys[j - 1]) ? 1u : 0u;
@@ -457,13 +498,22 @@ __global__ void k_dw_fold(const uint64_t* __restrict__ keys, const uint32_t* __r
 
 // rows of one sub-batch on the device; returns the row count
 uint64_t dw_compact(Context& c, const uint32_t* Ppar, const uint32_t* Rpar, const uint32_t* pa, const uint32_t* pb, uint64_t np, const uint8_t* hit,
-                    const double* dist, uint32_t* orow, double* odist) {
+                    const double* dist, uint32_t* orow, double* odist, const uint64_t** direct_offs = nullptr) {
   hipStream_t st = c.stream;
   uint32_t* f = c.s_tmp.as<uint32_t>(np + 16);
   uint64_t* off = c.s_offs.as<uint64_t>(np + 16);
   hipLaunchKernelGGL(k_hit_u32, dim3(div_up(np, 256)), dim3(256), 0, st, hit, np, f);
   GSJ_LAUNCHED();
   const uint64_t nh = scan_u32_to_u64(c, f, off, np);
+  if (!Ppar && !Rpar) {   // a probe meets each leaf once, so the candidates are distinct parent pairs
+    if (direct_offs) *direct_offs = off;
+    if (nh) {
+      hipLaunchKernelGGL(k_dw_direct, dim3(div_up(np, 256)), dim3(256), 0, st, pa, pb, f, off, np, dist, orow, odist);
+      GSJ_LAUNCHED();
+    }
+    return nh;
+  }
+  GSJ_REQUIRE(!direct_offs, GSJ_E_INTERNAL, "candidate-ordered rows need layers without parents");
   if (nh == 0) return 0;
   uint64_t* keys = c.s_keys.as<uint64_t>(nh + 16);
   uint64_t* keys2 = c.s_keys2.as<uint64_t>(nh + 16);
@@ -497,6 +547,62 @@ void dw_deliver(Context& c, const uint32_t* orow, const double* odist, uint64_t 
 
 }  // namespace
 
+// refine one sub-batch's candidates into A.hit / A.dist (list classes, then the overflow queue)
+void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
+  hipStream_t stream = c.stream;
+  const PairClasses pc = partition_pairs(c, A.P, A.R, A.pa, A.pb, np, MCAP, true);
+  uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
+  const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
+  GSJ_HIP(hipEventRecord(c.ev[0], stream));
+  hipLaunchKernelGGL((k_dw_lists<16, 32>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 0, ovf,
+                     pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[1], stream));
+  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 1, ovf,
+                     pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[5], stream));
+  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 2, ovf,
+                     pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[7], stream));
+  hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 3, ovf,
+                     pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[2], stream));
+  struct Host { uint32_t counts[4]; OverflowCtl ctl; };
+  Host* h = c.h_scal.as<Host>(2);
+  GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 16, hipMemcpyDeviceToHost, stream));
+  GSJ_HIP(hipMemcpyAsync(&h->ctl, pc.ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, stream));
+  GSJ_HIP(hipStreamSynchronize(stream));
+  float ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
+  GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
+  GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
+  GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
+  GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
+  s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_mixed_ms += ms_x; s.k_large_ms += ms_l;
+  s.n_small += h->counts[0]; s.n_medium += h->counts[1]; s.n_mixed += h->counts[2]; s.n_large += h->counts[3];
+  s.launches++;
+  const uint32_t nover = h->ctl.count;
+  if (nover) {
+    const int cap = (int)h->ctl.need;
+    const size_t per = 2 * (size_t)cap * sizeof(Ed);
+    uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nover, (1ull << 30) / per));
+    waves = (waves + 3) / 4 * 4;
+    Ed* slab = c.s_heavy_edges.as<Ed>(2 * (uint64_t)cap * waves + 16);
+    GSJ_HIP(hipEventRecord(c.ev[3], stream));
+    hipLaunchKernelGGL(k_dw_overflow, dim3((uint32_t)(waves / 4)), dim3(256), 0, stream, A, ovf, nover, cap, slab);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[4], stream));
+    GSJ_HIP(hipEventSynchronize(c.ev[4]));
+    float ms_o = 0;
+    GSJ_HIP(hipEventElapsedTime(&ms_o, c.ev[3], c.ev[4]));
+    s.k_over_ms += ms_o;
+    s.n_over += nover;
+    s.heavy += nover;
+  }
+}
+
 void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool same, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
                  gsj_join_stats* st) {
   GSJ_REQUIRE(!PL.points && !RL.points, GSJ_E_ARG, "dwithin: both layers must be polygon layers (points: gsj_points_dwithin)");
@@ -505,67 +611,20 @@ void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool sa
   gsj_join_stats s{};
   if (PL.n == 0 || RL.n == 0) { if (st) *st = s; return; }
   hipStream_t stream = c.stream;
-  const float4* qbb = PL.poly.bb;
-  if (d > 0) {
+  // search boxes widened by d, for the probes that will be walked only (a window's members)
+  const MembersFn widen = [&](const uint32_t* list, uint32_t n, const uint64_t*) -> const float4* {
+    if (d == 0 || n == 0) return nullptr;
     float4* e = c.s_qbb.as<float4>(PL.n + 16);
-    hipLaunchKernelGGL(k_expand_polys, dim3(div_up(PL.n, 256)), dim3(256), 0, stream, PL.poly, d, e);
+    if (list) hipLaunchKernelGGL(k_expand_polys_list, dim3(div_up(n, 256)), dim3(256), 0, stream, PL.poly, d, list, n, e);
+    else hipLaunchKernelGGL(k_expand_polys, dim3(div_up(PL.n, 256)), dim3(256), 0, stream, PL.poly, d, e);
     GSJ_LAUNCHED();
-    qbb = e;
-  }
-  for_each_candidate_batch(c, probes_of(PL, qbb), RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
+    return e;
+  };
+  for_each_candidate_batch(c, probes_of(PL), RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     WallTimer t2;
-    DwArgs A{PL.poly, RL.poly, pa, pb, d, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr};
-    const PairClasses pc = partition_pairs(c, PL.poly, RL.poly, pa, pb, np, MCAP, true);
-    uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
-    const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
-    GSJ_HIP(hipEventRecord(c.ev[0], stream));
-    hipLaunchKernelGGL((k_dw_lists<16, 32>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 0, ovf,
-                       pc.ctl);
-    GSJ_LAUNCHED();
-    GSJ_HIP(hipEventRecord(c.ev[1], stream));
-    hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 1, ovf,
-                       pc.ctl);
-    GSJ_LAUNCHED();
-    GSJ_HIP(hipEventRecord(c.ev[5], stream));
-    hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 2, ovf,
-                       pc.ctl);
-    GSJ_LAUNCHED();
-    GSJ_HIP(hipEventRecord(c.ev[7], stream));
-    hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 3, ovf,
-                       pc.ctl);
-    GSJ_LAUNCHED();
-    GSJ_HIP(hipEventRecord(c.ev[2], stream));
-    struct Host { uint32_t counts[4]; OverflowCtl ctl; };
-    Host* h = c.h_scal.as<Host>(2);
-    GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 16, hipMemcpyDeviceToHost, stream));
-    GSJ_HIP(hipMemcpyAsync(&h->ctl, pc.ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, stream));
-    GSJ_HIP(hipStreamSynchronize(stream));
-    float ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
-    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
-    GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
-    GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
-    GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
-    s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_mixed_ms += ms_x; s.k_large_ms += ms_l;
-    s.n_small += h->counts[0]; s.n_medium += h->counts[1]; s.n_mixed += h->counts[2]; s.n_large += h->counts[3];
-    s.launches++;
-    const uint32_t nover = h->ctl.count;
-    if (nover) {
-      const int cap = (int)h->ctl.need;
-      const size_t per = 2 * (size_t)cap * sizeof(Ed);
-      uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nover, (1ull << 30) / per));
-      waves = (waves + 3) / 4 * 4;
-      Ed* slab = c.s_heavy_edges.as<Ed>(2 * (uint64_t)cap * waves + 16);
-      GSJ_HIP(hipEventRecord(c.ev[3], stream));
-      hipLaunchKernelGGL(k_dw_overflow, dim3((uint32_t)(waves / 4)), dim3(256), 0, stream, A, ovf, nover, cap, slab);
-      GSJ_LAUNCHED();
-      GSJ_HIP(hipEventRecord(c.ev[4], stream));
-      GSJ_HIP(hipEventSynchronize(c.ev[4]));
-      float ms_o = 0;
-      GSJ_HIP(hipEventElapsedTime(&ms_o, c.ev[3], c.ev[4]));
-      s.k_over_ms += ms_o;
-      s.n_over += nover;
-      s.heavy += nover;
-    }
+    DwArgs A{PL.poly, RL.poly, pa, pb, d, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr,
+             nullptr, DistSet{}};
+    dw_refine(c, A, np, s);
     GSJ_HIP(hipStreamSynchronize(stream));
     s.refine_ms += t2.ms();
     WallTimer t3;
@@ -576,8 +635,77 @@ void dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool sa
     s.compact_ms += t3.ms();
     s.hits += nrows;
     dw_deliver(c, orow, odist, nrows, s, cb, user);
-  });
+  }, widen);
   if (st) *st = s;
+}
+
+bool dwithin_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o) {
+  return o.has_window && !o.cancel && !P.points && !R.points && R.has_bvh && P.n && R.n && !P.poly.parent && !R.poly.parent &&
+         probes_batchable(probes_of(P));
+}
+
+void dwithin_batch_run(Context& c, const Layer& PL, const Layer& RL, bool same, bool report, std::vector<BatchDist>& reqs) {
+  const uint32_t k = (uint32_t)reqs.size();
+  GSJ_REQUIRE(k >= 1 && k <= MAX_WINDOWS, GSJ_E_ARG, "batch size out of range");
+  const gsj_join_opts& o = *reqs[0].o;
+  WindowSet ws;
+  DistSet ds{};
+  ws.k = ds.k = k;
+  for (uint32_t r = 0; r < k; r++) {
+    const gsj_join_opts& q = *reqs[r].o;
+    GSJ_REQUIRE(dwithin_batchable(PL, RL, q) && q.upper == o.upper && q.batch_q == o.batch_q && q.max_pairs == o.max_pairs, GSJ_E_ARG,
+                "request cannot join this batch");
+    GSJ_REQUIRE(reqs[r].d >= 0 && reqs[r].d < 1e7, GSJ_E_ARG, "dwithin: distance must be in [0, 1e7) metres");
+    for (int j = 0; j < 4; j++) ws.w[r][j] = q.window[j];
+    ds.d[r] = reqs[r].d;
+  }
+  hipStream_t stream = c.stream;
+  const MembersFn widen = [&](const uint32_t* list, uint32_t n, const uint64_t* mstart) -> const float4* {
+    float4* e = c.s_qbb.as<float4>((uint64_t)n + 16);
+    hipLaunchKernelGGL(k_expand_members, dim3(div_up(n, 256)), dim3(256), 0, stream, PL.poly, list, n, mstart, ds, e);
+    GSJ_LAUNCHED();
+    return e;
+  };
+  gsj_join_stats s{};
+  std::vector<uint64_t> cand(k, 0), hits(k, 0), rlo(k + 1);
+  for_each_candidate_multi(c, probes_of(PL), RL.bvh, o.upper != 0, o, ws, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds& sb) {
+    for (uint32_t r = 0; r < k; r++) cand[r] += sb.h[r + 1] - sb.h[r];
+    WallTimer t2;
+    DwArgs A{PL.poly, RL.poly, pa, pb, 0.0, same ? 1 : 0, report ? 1 : 0, c.s_flag.as<uint8_t>(np + 16), report ? c.s_area.as<double>(np + 16) : nullptr,
+             sb.d, ds};
+    dw_refine(c, A, np, s);
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.refine_ms += t2.ms();
+    WallTimer t3;
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
+    double* odist = report ? c.s_out_area.as<double>(np + 16) : nullptr;
+    const uint64_t* offs = nullptr;
+    const uint64_t nrows = dw_compact(c, nullptr, nullptr, pa, pb, np, A.hit, A.dist, orow, odist, &offs);
+    seg_rows(c, sb, offs, np, nrows, rlo.data());
+    s.compact_ms += t3.ms();
+    if (nrows == 0) return;
+    WallTimer t4;
+    uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nrows);
+    double* hd = odist ? c.h_area.as<double>(nrows) : nullptr;
+    GSJ_HIP(hipMemcpyAsync(hrow, orow, nrows * 8, hipMemcpyDeviceToHost, stream));
+    if (hd) GSJ_HIP(hipMemcpyAsync(hd, odist, nrows * 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.copy_ms += t4.ms();
+    for (uint32_t r = 0; r < k; r++) {
+      const uint64_t a = rlo[r], n = rlo[r + 1] - rlo[r];
+      hits[r] += n;
+      BatchDist& q = reqs[r];
+      if (n == 0 || !q.cb || q.rc != GSJ_OK) continue;
+      if (q.cb(q.user, hrow + 2 * a, hd ? hd + a : nullptr, (size_t)n)) q.rc = GSJ_E_CANCEL;
+    }
+  }, widen);
+  for (uint32_t r = 0; r < k; r++) {
+    if (!reqs[r].st) continue;
+    gsj_join_stats& st = *reqs[r].st;
+    st = s;
+    st.candidates = cand[r];
+    st.hits = hits[r];
+  }
 }
 
 void points_dwithin_run(Context& c, const Layer& PL, const Layer& RL, double d, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,

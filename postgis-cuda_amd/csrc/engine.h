@@ -124,8 +124,9 @@ void tile_boxes(Context& c, const float4* bb, const uint32_t* order, uint32_t n,
 
 // ------------------------------------------------------------------------------------------- traversal
 // count pass (write == false) or write pass over probe boxes qbb[qbase .. qbase+nq)
+// (box_by_pos: the box of position p is qbb[p] instead of the box of its probe)
 void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
-              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b);
+              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b, bool box_by_pos = false);
 
 // The probe side of a candidate search: traversal boxes qbb[0 .. n) (probe index = box index), the probes' own boxes
 // (the window test; differ from qbb when the search box is widened), and the parent structure of a subdivided layer
@@ -146,9 +147,13 @@ Probes probes_of(const Layer& L, const float4* qbb = nullptr);
 // pieces together and sub-batches of at most o.max_pairs candidates (engine/engine.cuh:903-937); with o.has_window
 // only the probes of parents meeting the window. fn(pa, pb, np) runs on every non-empty sub-batch with the pairs on
 // the device and the stream synchronised. Adds traverse_ms, batches and candidates to s; polls o.cancel.
+// on_members(list, n, mstart), when given, runs once before the walk with the probes that will be walked (list ==
+// nullptr: all n probes in id order) and may return the search boxes to use instead of pr.qbb (indexed by probe id;
+// by member position in for_each_candidate_multi, whose mstart gives the windows' member boundaries).
 using PairsFn = std::function<void(uint32_t* pa, uint32_t* pb, uint64_t np)>;
+using MembersFn = std::function<const float4*(const uint32_t* list, uint32_t n, const uint64_t* mstart)>;
 void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
-                              const PairsFn& fn);
+                              const PairsFn& fn, const MembersFn& on_members = nullptr);
 
 // Several windowed requests over the same probes in one candidate search (the batched served path). Window r's
 // members come after window r-1's, each in walk order, so request r's candidates are a contiguous run of every
@@ -167,7 +172,10 @@ struct SegBounds {
 using SegPairsFn = std::function<void(uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds& sb)>;
 bool probes_batchable(const Probes& pr);
 void for_each_candidate_multi(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, const WindowSet& ws,
-                              gsj_join_stats& s, const SegPairsFn& fn);
+                              gsj_join_stats& s, const SegPairsFn& fn, const MembersFn& on_members = nullptr);
+// h_rlo[0 .. sb.k] = row boundaries of the requests of sub-batch sb after an order-preserving compaction whose
+// exclusive offsets are offs[0 .. np) (nrows rows); synchronises
+void seg_rows(Context& c, const SegBounds& sb, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* h_rlo);
 
 // ------------------------------------------------------------------------------------------- refinement
 struct RefineParams {   // engine/engine.cuh:517-518
@@ -217,6 +225,18 @@ void dwithin_run(Context& c, const Layer& P, const Layer& R, double d, bool same
                  gsj_join_stats* st);
 void points_dwithin_run(Context& c, const Layer& pts, const Layer& R, double d, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,
                         gsj_join_stats* st);
+// windowed dwithin requests of the same layers, `same`, `report`, upper / batch_q / max_pairs, each with its own
+// distance, in one pass (as join_batch_run)
+struct BatchDist {
+  const gsj_join_opts* o = nullptr;
+  double d = 0;
+  gsj_dist_cb cb = nullptr;
+  void* user = nullptr;
+  gsj_join_stats* st = nullptr;
+  int rc = GSJ_OK;
+};
+bool dwithin_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o);
+void dwithin_batch_run(Context& c, const Layer& P, const Layer& R, bool same, bool report, std::vector<BatchDist>& reqs);
 // eroded intersects (erode.hip, dwithin.hip)
 void erode_run(Context& c, const Layer& P, const Layer& R, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
                gsj_dist_cb cb, void* user, gsj_join_stats* st);

@@ -112,16 +112,6 @@ __global__ void k_emit_multi(const DPolys P, const DPolys R, const uint64_t* __r
 
 inline int bits_for(uint64_t maxval) { int b = 1; while (b < 64 && (maxval >> b)) b++; return b; }
 
-// row boundaries of the batched requests: rows of candidates [lo[r], lo[r + 1]) are [rlo[r], rlo[r + 1]), from the
-// exclusive scan offs of the kept flags (single-piece rows only)
-__global__ void k_row_bounds(const uint64_t* __restrict__ lo, uint32_t k, const uint64_t* __restrict__ offs, uint64_t np, uint64_t nrows,
-                             uint64_t* __restrict__ rlo) {
-  const uint32_t r = threadIdx.x;
-  if (r > k) return;
-  const uint64_t i = lo[r];
-  rlo[r] = i < np ? offs[i] : nrows;
-}
-
 }  // namespace
 
 // Compact the refined candidates of one sub-batch into parent-pair rows on the device; returns the row count.
@@ -275,12 +265,8 @@ void join_batch_run(Context& c, const Layer& PL, const Layer& RL, std::vector<Ba
     uint32_t* ocode = c.s_out_code.as<uint32_t>(np + 16);
     const uint64_t* offs = nullptr;
     const uint64_t nrows = compact_rows(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode, &offs);
-    uint64_t* drlo = c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1)) + 3 * (MAX_WINDOWS + 1);   // after the windows' bounds
-    hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, stream, sb.d, k, offs, np, nrows, drlo);
-    GSJ_LAUNCHED();
     std::vector<uint64_t> rlo(k + 1);
-    GSJ_HIP(hipMemcpyAsync(rlo.data(), drlo, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, stream));
-    GSJ_HIP(hipStreamSynchronize(stream));
+    seg_rows(c, sb, offs, np, nrows, rlo.data());
     s.compact_ms += t3.ms();
     if (nrows == 0) return;
     WallTimer t4;

@@ -64,30 +64,53 @@ struct gsj_sched {
   void worker(int lane);
   void run(gsj::Context& c, Req& q);
   void run_batch(gsj::Context& c, std::vector<Req*>& group);
-  bool batch_head(const Req& q) const {
-    return max_batch > 1 && q.r.kind == GSJ_REQ_JOIN && gsj::join_batchable(q.r.P->L, q.r.R->L, q.r.opts);
+  static bool batchable(const Req& q) {
+    if (q.r.kind == GSJ_REQ_JOIN) return gsj::join_batchable(q.r.P->L, q.r.R->L, q.r.opts);
+    if (q.r.kind == GSJ_REQ_DWITHIN) return gsj::dwithin_batchable(q.r.P->L, q.r.R->L, q.r.opts);
+    return false;
   }
-  bool batch_with(const Req& head, const Req& x) const {
-    return x.r.kind == GSJ_REQ_JOIN && x.r.P == head.r.P && x.r.R == head.r.R && gsj::join_batchable(x.r.P->L, x.r.R->L, x.r.opts) &&
-           gsj::join_compatible(head.r.opts, x.r.opts);
+  bool batch_head(const Req& q) const { return max_batch > 1 && batchable(q); }
+  static bool batch_with(const Req& head, const Req& x) {
+    if (x.r.kind != head.r.kind || x.r.P != head.r.P || x.r.R != head.r.R || !batchable(x)) return false;
+    const gsj_join_opts &a = head.r.opts, &b = x.r.opts;
+    if (x.r.kind == GSJ_REQ_JOIN) return gsj::join_compatible(a, b);
+    return x.r.same == head.r.same && x.r.report_distance == head.r.report_distance && a.upper == b.upper && a.batch_q == b.batch_q &&
+           a.max_pairs == b.max_pairs;
   }
 };
 
 void gsj_sched::run_batch(gsj::Context& c, std::vector<Req*>& group) {
-  std::vector<gsj::BatchJoin> b(group.size());
-  for (size_t i = 0; i < group.size(); i++) {
-    b[i].o = &group[i]->r.opts;
-    b[i].cb = group[i]->r.rows_cb;
-    b[i].user = group[i]->r.user;
-    b[i].st = &group[i]->stats;
+  const gsj_request& h = group[0]->r;
+  std::vector<int> rcs(group.size(), GSJ_OK);
+  int rc = GSJ_OK;
+  if (h.kind == GSJ_REQ_JOIN) {
+    std::vector<gsj::BatchJoin> b(group.size());
+    for (size_t i = 0; i < group.size(); i++) {
+      b[i].o = &group[i]->r.opts;
+      b[i].cb = group[i]->r.rows_cb;
+      b[i].user = group[i]->r.user;
+      b[i].st = &group[i]->stats;
+    }
+    rc = gsj::guarded_status([&] { gsj::join_batch_run(c, h.P->L, h.R->L, b); });
+    for (size_t i = 0; i < group.size(); i++) rcs[i] = b[i].rc;
+  } else {
+    std::vector<gsj::BatchDist> b(group.size());
+    for (size_t i = 0; i < group.size(); i++) {
+      b[i].o = &group[i]->r.opts;
+      b[i].d = group[i]->r.d;
+      b[i].cb = group[i]->r.dist_cb;
+      b[i].user = group[i]->r.user;
+      b[i].st = &group[i]->stats;
+    }
+    rc = gsj::guarded_status([&] { gsj::dwithin_batch_run(c, h.P->L, h.R->L, h.same != 0, h.report_distance != 0, b); });
+    for (size_t i = 0; i < group.size(); i++) rcs[i] = b[i].rc;
   }
-  const int rc = gsj::guarded_status([&] { gsj::join_batch_run(c, group[0]->r.P->L, group[0]->r.R->L, b); });
   const std::string msg = rc == GSJ_OK ? std::string() : gsj::g_last_error;
   for (size_t i = 0; i < group.size(); i++) {
     Req& q = *group[i];
     q.batch = (int)group.size();
-    q.rc = rc != GSJ_OK ? rc : b[i].rc;
-    q.msg = rc != GSJ_OK ? msg : (b[i].rc != GSJ_OK ? std::string("join cancelled by the row callback") : std::string());
+    q.rc = rc != GSJ_OK ? rc : rcs[i];
+    q.msg = rc != GSJ_OK ? msg : (rcs[i] != GSJ_OK ? std::string("request cancelled by the row callback") : std::string());
   }
 }
 

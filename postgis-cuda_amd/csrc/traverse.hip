@@ -14,11 +14,12 @@ namespace {
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_traverse(const float4* __restrict__ qbb, const uint32_t* __restrict__ qidx, uint32_t qbase, uint32_t nq,
                                                   const BNode* __restrict__ nodes, uint32_t nnodes, int upper, uint32_t* __restrict__ counts,
-                                                  const uint64_t* __restrict__ offs, uint32_t* __restrict__ out_a, uint32_t* __restrict__ out_b) {
+                                                  const uint64_t* __restrict__ offs, uint32_t* __restrict__ out_a, uint32_t* __restrict__ out_b,
+                                                  int box_by_pos) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const uint32_t q = qidx ? qidx[qbase + t] : qbase + t;
-  const float4 b = qbb[q];
+  const float4 b = qbb[box_by_pos ? qbase + t : q];
   uint64_t o = WRITE ? offs[t] : 0;
   uint32_t cnt = 0;
   uint32_t i = 0;
@@ -192,7 +193,7 @@ Probes probes_of(const Layer& L, const float4* qbb) {
 }
 
 void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
-              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b) {
+              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b, bool box_by_pos) {
   if (nq == 0) return;
   if (t.nnodes == 0) {
     if (!offs) GSJ_HIP(hipMemsetAsync(counts, 0, (size_t)nq * 4, c.stream));
@@ -200,10 +201,10 @@ void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbas
   }
   if (!offs)
     hipLaunchKernelGGL(k_traverse<false>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
-                       counts, nullptr, nullptr, nullptr);
+                       counts, nullptr, nullptr, nullptr, box_by_pos ? 1 : 0);
   else
     hipLaunchKernelGGL(k_traverse<true>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
-                       nullptr, offs, out_a, out_b);
+                       nullptr, offs, out_a, out_b, box_by_pos ? 1 : 0);
   GSJ_LAUNCHED();
 }
 
@@ -213,11 +214,11 @@ namespace {
 // mstart (k windows' member boundaries on the device), every sub-batch also carries the windows' candidate
 // boundaries.
 void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s, const uint32_t* qidx,
-                  uint32_t nQ, const std::vector<uint32_t>& hq, const uint64_t* mstart, uint32_t k, const SegPairsFn& fn) {
+                  uint32_t nQ, const std::vector<uint32_t>& hq, const uint64_t* mstart, uint32_t k, const float4* qbb, bool box_by_pos,
+                  const SegPairsFn& fn) {
   hipStream_t stream = c.stream;
   const uint32_t batch_q = o.batch_q ? o.batch_q : (1u << 22);
   const uint64_t max_pairs = o.max_pairs ? o.max_pairs : (128ull << 20);
-  const float4* qbb = pr.qbb;
   static const std::vector<uint32_t> none;
   const std::vector<uint32_t>& hp = pr.h_parent ? *pr.h_parent : none;
   auto pid = [&](uint32_t pos) { return hp[qidx ? hq[pos] : pos]; };   // parent of the probe at a position
@@ -231,7 +232,7 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
     const uint32_t nq = qn - q0;
     WallTimer tt;
     uint32_t* counts = c.s_counts.as<uint32_t>(nq + 16);
-    traverse(c, qbb, qidx, q0, nq, t, upper, counts, nullptr, nullptr, nullptr);
+    traverse(c, qbb, qidx, q0, nq, t, upper, counts, nullptr, nullptr, nullptr, box_by_pos);
     uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);   // probe offsets (s_offs belongs to compaction)
     const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);
     s.traverse_ms += tt.ms();
@@ -262,7 +263,7 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
       GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a sub-batch has >= 2^32 candidates (lower max_pairs)");
       uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
       uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
-      traverse(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb);
+      traverse(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb, box_by_pos);
       SegBounds sb;
       if (mstart) {
         hipLaunchKernelGGL(k_seg_bounds, dim3(1), dim3(64), 0, stream, mstart, k, (uint64_t)(q0 + s0), (uint64_t)(s1 - s0), offs, np, dlo);
@@ -282,7 +283,7 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
 bool probes_batchable(const Probes& pr) { return pr.order && pr.tiles && !pr.h_parent; }
 
 void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, gsj_join_stats& s,
-                              const PairsFn& fn) {
+                              const PairsFn& fn, const MembersFn& on_members) {
   hipStream_t stream = c.stream;
   // probe positions 0 .. nQ: probe ids directly, through the walk order, or through the window's member list (in walk
   // order); a subdivided layer walks in id order (its parents' pieces are contiguous)
@@ -325,11 +326,17 @@ void for_each_candidate_batch(Context& c, const Probes& pr, const DBvh& t, bool 
     qidx = list;
     s.traverse_ms += tw.ms();
   }
-  walk_batches(c, pr, t, upper, o, s, qidx, nQ, hq, nullptr, 0, [&](uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds&) { fn(pa, pb, np); });
+  const float4* qbb = pr.qbb;
+  if (on_members) {
+    const float4* b = on_members(o.has_window ? qidx : nullptr, nQ, nullptr);
+    if (b) qbb = b;
+  }
+  walk_batches(c, pr, t, upper, o, s, qidx, nQ, hq, nullptr, 0, qbb, false,
+               [&](uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds&) { fn(pa, pb, np); });
 }
 
 void for_each_candidate_multi(Context& c, const Probes& pr, const DBvh& t, bool upper, const gsj_join_opts& o, const WindowSet& ws,
-                              gsj_join_stats& s, const SegPairsFn& fn) {
+                              gsj_join_stats& s, const SegPairsFn& fn, const MembersFn& on_members) {
   GSJ_REQUIRE(probes_batchable(pr), GSJ_E_ARG, "batched windows need probes with tile boxes and no parents");
   if (pr.n == 0) return;
   WallTimer tw;
@@ -339,7 +346,28 @@ void for_each_candidate_multi(Context& c, const Probes& pr, const DBvh& t, bool 
   GSJ_HIP(hipStreamSynchronize(c.stream));
   s.traverse_ms += tw.ms();
   static const std::vector<uint32_t> none;
-  walk_batches(c, pr, t, upper, o, s, list, nQ, none, mstart, ws.k, fn);
+  const float4* mbb = on_members && nQ ? on_members(list, nQ, mstart) : nullptr;   // boxes by member position
+  walk_batches(c, pr, t, upper, o, s, list, nQ, none, mstart, ws.k, mbb ? mbb : pr.qbb, mbb != nullptr, fn);
+}
+
+namespace {
+// rows of the batched requests: rows of candidates [lo[r], lo[r + 1]) are [rlo[r], rlo[r + 1]), from the exclusive
+// scan offs of the candidates' kept flags (an order-preserving compaction)
+__global__ void k_row_bounds(const uint64_t* __restrict__ lo, uint32_t k, const uint64_t* __restrict__ offs, uint64_t np, uint64_t nrows,
+                             uint64_t* __restrict__ rlo) {
+  const uint32_t r = threadIdx.x;
+  if (r > k) return;
+  const uint64_t i = lo[r];
+  rlo[r] = i < np ? offs[i] : nrows;
+}
+}  // namespace
+
+void seg_rows(Context& c, const SegBounds& sb, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* h_rlo) {
+  uint64_t* d = c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1)) + 3 * (MAX_WINDOWS + 1);   // after the windows' bounds
+  hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, c.stream, sb.d, sb.k, offs, np, nrows, d);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipMemcpyAsync(h_rlo, d, (size_t)(sb.k + 1) * 8, hipMemcpyDeviceToHost, c.stream));
+  GSJ_HIP(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace gsj

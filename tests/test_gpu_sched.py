@@ -201,3 +201,41 @@ def test_batching_off_and_unbatchable_requests_run_alone(ctx):
             _rows_in_order(g, ctx.join(A, A, **kw))
     with pytest.raises(gsj.GsjError, match="batch"):
         gsj.Scheduler(ctx, lanes=1, batch=33)
+
+
+def test_batched_windowed_dwithin_equal_their_own_calls(ctx):
+    """Windowed distance joins of one layer pair batch across different distances (each member's search box and each
+    candidate's threshold are its own request's); rows, order, distances, candidates and hits equal the direct calls."""
+    M = gsj.generate("parcels_mesh", nx=60, ny=60, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=12)
+    L = ctx.polys(M)
+    big = ctx.polys(gsj.subdivide(_mesh(150, 8), 10))
+    x0, y0 = M.xy[:, 0].min(), M.xy[:, 1].min()
+    W, H = M.xy[:, 0].max() - x0, M.xy[:, 1].max() - y0
+    rng = np.random.default_rng(13)
+    reqs = []
+    for i in range(40):
+        f = rng.uniform(0, 1, 2)
+        s = rng.uniform(0.02, 0.35)
+        w = (x0 + f[0] * W, y0 + f[1] * H, x0 + f[0] * W + s * W, y0 + f[1] * H + s * H)
+        d = 0.0 if i % 10 == 0 else float(rng.uniform(1.0, 40.0))
+        reqs.append(dict(d=d, same=True, upper=True, report_distance=i < 30, window=w))
+    reqs[5]["window"] = (x0 - 1, y0 - 1, x0 + W + 1, y0 + H + 1)
+    reqs[6]["window"] = (x0 + W + 1, y0 + H + 1, x0 + W + 2, y0 + H + 2)
+    s = gsj.Scheduler(ctx, lanes=1, batch=16)
+    try:
+        t0 = s.submit(gsj.REQ_JOIN, big, big, want_relate=True, want_area=True, upper=True)
+        tickets = [s.submit(gsj.REQ_DWITHIN, L, L, **kw) for kw in reqs]
+        s.wait(t0)
+        got = [s.wait(t) for t in tickets]
+    finally:
+        s.close()
+    batched = [g.stats["times"]["batch"] for g in got]
+    assert max(batched) > 1 and sum(b > 1 for b in batched) >= len(got) // 2
+    for kw, g in zip(reqs, got):
+        kw = dict(kw)
+        d = ctx.dwithin(L, L, kw.pop("d"), **kw)
+        assert np.array_equal(g.pairs, d.pairs)
+        if kw["report_distance"] and len(d.pairs):   # an empty result may carry no column
+            assert np.array_equal(g.dist, d.dist)
+        assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
+    assert len(got[5].pairs) > 0 and len(got[6].pairs) == 0
