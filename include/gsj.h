@@ -224,12 +224,14 @@ typedef struct {
 int gsj_sched_create(gsj_ctx* ctx, int lanes, uint64_t scratch_budget_bytes, gsj_sched** out);
 /* waits for every submitted request, then stops the lanes and frees their scratch */
 void gsj_sched_destroy(gsj_sched* s);
-/* Batching of windowed joins (GSJ_REQ_JOIN with has_window, the same P and R, the same want_area / want_relate /
- * upper / snap_half_ulp / no_agg / batch_q / max_pairs, no cancel flag, probe layer with a BVH and neither layer
- * subdivided): a lane admitting one such request also takes up to max_requests - 1 more from the queue and runs them
- * in one pass. Each request's rows (in order), candidates and hits are those it gets alone. max_requests in [1, 32],
- * default 16; 1 turns batching off. No counterpart in the reference, which serves one request at a time
- * (daemon/gpujoind.cu:35). */
+/* Batching of windowed requests: GSJ_REQ_JOIN, GSJ_REQ_DWITHIN and GSJ_REQ_INTERSECTS_BUFFERED with has_window, the
+ * same kind, P and R, no cancel flag, a probe layer with a BVH and neither layer subdivided, and the same options
+ * except the window (joins: want_area / want_relate / upper / snap_half_ulp / no_agg / batch_q / max_pairs; dwithin:
+ * same / report_distance / upper / batch_q / max_pairs, any d; eroded intersects: same / upper / snap_half_ulp /
+ * batch_q / max_pairs, no need_self, any ra / rb). A lane admitting one such request also takes up to
+ * max_requests - 1 more from the queue and runs them in one pass. Each request's rows (in order), candidates and hits
+ * are those it gets alone. max_requests in [1, 32], default 16; 1 turns batching off. No counterpart in the
+ * reference, which serves one request at a time (daemon/gpujoind.cu:35). */
 int gsj_sched_set_batch(gsj_sched* s, int max_requests);
 /* queue a request; *ticket identifies it for gsj_wait / gsj_poll */
 int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket);

@@ -176,6 +176,11 @@ void for_each_candidate_multi(Context& c, const Probes& pr, const DBvh& t, bool 
 // h_rlo[0 .. sb.k] = row boundaries of the requests of sub-batch sb after an order-preserving compaction whose
 // exclusive offsets are offs[0 .. np) (nrows rows); synchronises
 void seg_rows(Context& c, const SegBounds& sb, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* h_rlo);
+// the same on the device: out[r] = boundaries after the compaction of lo[0 .. k] (async)
+void seg_bounds_dev(Context& c, const uint64_t* lo, uint32_t k, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* out);
+// region i of 8 in the small per-context buffer of window and request boundaries (MAX_WINDOWS + 1 entries each):
+// 0 tile starts, 1 sub-batch candidate bounds, 2 member starts, 3 row bounds, 4-7 free for a caller's stages
+uint64_t* wseg(Context& c, int region);
 
 // ------------------------------------------------------------------------------------------- refinement
 struct RefineParams {   // engine/engine.cuh:517-518
@@ -214,6 +219,10 @@ struct BatchJoin {
   int rc = GSJ_OK;   // GSJ_E_CANCEL when its callback asked to stop
 };
 bool join_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o);
+// compaction of refined candidates of layers without parents into rows in candidate order (area / relate as rp asks);
+// *offs = the exclusive offsets of the kept candidates
+uint64_t compact_single_rows(Context& c, const Layer& P, const Layer& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
+                             const RefineOut& ro, uint32_t* orow, double* oarea, const uint64_t** offs);
 bool join_compatible(const gsj_join_opts& a, const gsj_join_opts& b);
 void join_batch_run(Context& c, const Layer& P, const Layer& R, std::vector<BatchJoin>& reqs);
 void points_join_run(Context& c, const Layer& pts, const Layer& R, const gsj_join_opts& o, gsj_points_cb cb, void* user,
@@ -241,5 +250,17 @@ void dwithin_batch_run(Context& c, const Layer& P, const Layer& R, bool same, bo
 void erode_run(Context& c, const Layer& P, const Layer& R, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
                gsj_dist_cb cb, void* user, gsj_join_stats* st);
 void points_erode_run(Context& c, const Layer& pts, const Layer& R, double r, const gsj_join_opts& o, gsj_dist_cb cb, void* user, gsj_join_stats* st);
+// windowed eroded-intersects requests of the same layers, `same`, upper / batch_q / max_pairs / snap_half_ulp, no
+// need_self, each with its own radii, in one pass (as join_batch_run)
+struct BatchErode {
+  const gsj_join_opts* o = nullptr;
+  double ra = 0, rb = 0;
+  gsj_dist_cb cb = nullptr;
+  void* user = nullptr;
+  gsj_join_stats* st = nullptr;
+  int rc = GSJ_OK;
+};
+bool erode_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o, const uint8_t* need_self);
+void erode_batch_run(Context& c, const Layer& P, const Layer& R, bool same, std::vector<BatchErode>& reqs);
 
 }  // namespace gsj

@@ -456,13 +456,29 @@ __device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, co
 
 struct ErCtl { uint32_t n; int needE, needL; uint32_t pad; };
 
+// Radii of the pair (or candidate, or row) at index p: one request's (seg null), or, for batched requests, those of
+// request r with seg[r] <= p < seg[r + 1]
+struct ErRadii {
+  const uint64_t* seg;
+  uint32_t k;
+  double ra[MAX_WINDOWS], rb[MAX_WINDOWS];
+};
+__device__ __forceinline__ uint32_t radii_of(const ErRadii& R, uint64_t p, double& ra, double& rb) {
+  uint32_t r = 0;
+  if (R.seg)
+    while (r + 1 < R.k && R.seg[r + 1] <= p) r++;
+  ra = R.ra[r];
+  rb = R.rb[r];
+  return r;
+}
+
 // LDS passes: a G-lane group per parent pair with CAPE / CAPL lists per side; overflowing pairs are queued with the
 // sizes they need. The pairs are 0 .. np (items null) or items[0 .. items_ctl->n) (the previous pass's queue).
 template <int G, int CAPE, int CAPL>
 __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
                                                               const uint32_t* __restrict__ pfR, const uint32_t* __restrict__ pairs, uint32_t np,
-                                                              const uint32_t* __restrict__ items, const ErCtl* __restrict__ items_ctl, double ra,
-                                                              double rb, uint8_t* __restrict__ flag, uint32_t* __restrict__ heavy, ErCtl* ctl) {
+                                                              const uint32_t* __restrict__ items, const ErCtl* __restrict__ items_ctl, const ErRadii rr,
+                                                              uint8_t* __restrict__ flag, uint32_t* __restrict__ heavy, ErCtl* ctl) {
   constexpr int NG = ER_WAVES * 64 / G;
   __shared__ __align__(16) unsigned char lds[NG][er_bytes(CAPE, CAPL)];
   const int lane = threadIdx.x % G, gib = threadIdx.x / G;
@@ -471,6 +487,8 @@ __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, co
   for (uint32_t k = blockIdx.x * NG + gib; k < n; k += gridDim.x * NG) {
     const uint32_t p = items ? items[k] : k;
     int ne = 0, nl = 0;
+    double ra, rb;
+    radii_of(rr, p, ra, rb);
     const int r = er_pair<G>(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
     if (lane == 0) {
       flag[p] = r > 0 ? 1 : 0;
@@ -490,7 +508,7 @@ __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, co
 constexpr int ER_SLAB_T = 512;
 __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
                                                           const uint32_t* __restrict__ pfR, const uint32_t* __restrict__ pairs,
-                                                          const uint32_t* __restrict__ items, uint32_t ni, double ra, double rb,
+                                                          const uint32_t* __restrict__ items, uint32_t ni, const ErRadii rr,
                                                           uint8_t* __restrict__ flag, unsigned char* __restrict__ slab, int capE, int capL,
                                                           uint32_t* __restrict__ again, ErCtl* ctl) {
   __shared__ ErPair SE;
@@ -502,6 +520,8 @@ __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const 
     if (t < 64) {   // first wave: window, frame and lists
       ErPair E;
       int ne = 0, nl = 0;
+      double ra, rb;
+      radii_of(rr, p, ra, rb);
       const int st = er_setup<64>(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, E, ne, nl);
       if (lane == 0) {
         SE = E;
@@ -548,9 +568,12 @@ __global__ void k_ctl_clear(ErCtl* ctl) { ctl->n = 0; ctl->needE = 0; ctl->needL
 // candidate piece pairs whose eroded polygons can meet: not both whole geometries, or shrunk bboxes that overlap
 // (buffer.cuh:751-776)
 __global__ void k_er_prefilter(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t n,
-                               double ra, double rb, uint32_t* __restrict__ keep) {
+                               const ErRadii rr, uint32_t* __restrict__ keep) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  double ra, rb;
+  radii_of(rr, i, ra, rb);
+  if (ra <= 0 && rb <= 0) { keep[i] = 1u; return; }   // no erosion: every candidate is refined
   const uint32_t a = pa[i], b = pb[i];
   auto whole = [](const DPolys& S, uint32_t g) {
     if (!S.parent) return true;
@@ -584,9 +607,12 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint64_t n, uint32_t*
 // parent rows of pass 1 that go to pass 2: overlap area able to hold a disk of radius rmin (rmin 0: any overlap,
 // < 0: every intersecting pair), self pairs only where needed (buffer.cuh:722-748)
 __global__ void k_er_filter(const uint32_t* __restrict__ rows, const double* __restrict__ area, uint64_t n, const DPolys P,
-                            const uint32_t* __restrict__ pfP, double rmin, int same, const uint8_t* __restrict__ need_self, uint32_t* __restrict__ keep) {
+                            const uint32_t* __restrict__ pfP, const ErRadii rr, int same, const uint8_t* __restrict__ need_self, uint32_t* __restrict__ keep) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  double ra, rb;
+  radii_of(rr, i, ra, rb);
+  const double rmin = (ra > 0 && rb > 0) ? fmin(ra, rb) : (ra > 0 || rb > 0 ? 0.0 : -1.0);
   const uint32_t a = rows[2 * i], b = rows[2 * i + 1];
   bool k = true;
   if (same && a == b && need_self) k = need_self[a] != 0;
@@ -616,6 +642,110 @@ __global__ void k_flag_u32(const uint8_t* __restrict__ f, uint64_t n, uint32_t* 
 
 }  // namespace
 
+namespace {
+
+// Pass 2 over n pass-1 parent rows (rows, overlap areas) on the device: the rows whose eroded polygons meet,
+// compacted into *out in pass-1 order; returns their count. Batched requests: lo_in[0 .. k] are the rows' request
+// boundaries (rr.seg == lo_in), and lo_mid, lo_out receive those of the filtered pairs and of the output.
+uint64_t erode_pass2(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, const uint32_t* rows, const double* area,
+                     uint64_t n, ErRadii rr, bool same, const uint8_t* d_need, ErCtl* ctl, gsj_join_stats& s, uint32_t*& out, uint64_t* lo_mid,
+                     uint64_t* lo_out) {
+  hipStream_t stream = c.stream;
+  uint32_t* keep = c.s_er_keep.as<uint32_t>(n + 16);
+  uint64_t* off = c.s_er_off.as<uint64_t>(n + 16);
+  hipLaunchKernelGGL(k_er_filter, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, area, n, P, pfP, rr, same ? 1 : 0, d_need, keep);
+  GSJ_LAUNCHED();
+  const uint64_t ne = scan_u32_to_u64(c, keep, off, n);
+  if (rr.seg) {
+    seg_bounds_dev(c, rr.seg, rr.k, off, n, ne, lo_mid);
+    rr.seg = lo_mid;
+  }
+  if (ne == 0) {
+    if (lo_out) seg_bounds_dev(c, lo_mid, rr.k, off, 0, 0, lo_out);
+    return 0;
+  }
+  uint32_t* ep = c.s_er_pairs.as<uint32_t>(2 * ne + 16);
+  hipLaunchKernelGGL(k_compact_rows, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, keep, off, n, ep);
+  GSJ_LAUNCHED();
+  uint8_t* flag = c.s_er_flag.as<uint8_t>(ne + 16);
+  uint32_t* lists = c.s_er_list.as<uint32_t>(3 * ne + 48);
+  uint32_t* heavy = lists;
+  uint32_t* again = lists + ne + 16;
+  uint32_t* first = again + ne + 16;   // the 16-lane pass's overflow, for the wave pass
+  ErCtl* ctl16 = ctl + 1;
+  hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
+  hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl16);
+  GSJ_HIP(hipEventRecord(c.ev[0], stream));
+  // 16-lane groups with small lists first (most parent pairs), then a wave per pair for those that overflowed them
+  const uint32_t g16 = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES * 4), (uint64_t)c.cu_count * 32);
+  hipLaunchKernelGGL((k_erode_main<16, ER_CAPE16, ER_CAPL16>), dim3(g16), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, nullptr,
+                     nullptr, rr, flag, first, ctl16);
+  GSJ_LAUNCHED();
+  const uint32_t g = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES), (uint64_t)c.cu_count * 32);
+  hipLaunchKernelGGL((k_erode_main<64, ER_CAPE, ER_CAPL>), dim3(g), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, first, ctl16, rr,
+                     flag, heavy, ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[1], stream));
+  ErCtl* h = c.h_scal.as<ErCtl>(2);
+  GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
+  GSJ_HIP(hipStreamSynchronize(stream));
+  float ms = 0;
+  GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+  s.k_large_ms += ms;
+  s.n_large += ne;
+  s.launches++;
+  uint32_t nh = h->n;
+  int capE = std::max(h->needE, ER_CAPE), capL = std::max(h->needL, 4 * capE);
+  s.heavy += nh;
+  s.n_over += nh;
+  for (int round = 0; nh; round++) {
+    GSJ_REQUIRE(round < 4, GSJ_E_INTERNAL, "erosion: overflowing pairs did not converge");
+    const size_t per = er_bytes(capE, capL);
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(nh, (1ull << 30) / per));   // one list set per block
+    unsigned char* slab = c.s_er_slab.as<unsigned char>(per * blocks + 64);
+    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
+    GSJ_HIP(hipEventRecord(c.ev[2], stream));
+    hipLaunchKernelGGL(k_erode_slab, dim3((uint32_t)blocks), dim3(ER_SLAB_T), 0, stream, P, R, pfP, pfR, ep, heavy, nh, rr, flag, slab, capE, capL, again,
+                       ctl);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[3], stream));
+    GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    float mo = 0;
+    GSJ_HIP(hipEventElapsedTime(&mo, c.ev[2], c.ev[3]));
+    s.k_over_ms += mo;
+    nh = h->n;
+    capE = std::max(capE, h->needE);
+    capL = std::max(capL, h->needL);
+    std::swap(heavy, again);
+  }
+  // rows: flagged parent pairs in pass-1 order
+  uint32_t* f = c.s_er_keep.as<uint32_t>(ne + 16);
+  hipLaunchKernelGGL(k_flag_u32, dim3(div_up(ne, 256)), dim3(256), 0, stream, flag, ne, f);
+  GSJ_LAUNCHED();
+  const uint64_t nhit = scan_u32_to_u64(c, f, off, ne);
+  if (lo_out) seg_bounds_dev(c, lo_mid, rr.k, off, ne, nhit, lo_out);
+  if (nhit == 0) return 0;
+  out = c.s_er_out.as<uint32_t>(2 * nhit + 16);
+  hipLaunchKernelGGL(k_compact_rows, dim3(div_up(ne, 256)), dim3(256), 0, stream, ep, f, off, ne, out);
+  GSJ_LAUNCHED();
+  return nhit;
+}
+
+// need_self flags on the device and the overflow queue's control blocks (one allocation, reused per call)
+ErCtl* er_scratch(Context& c, const DPolys& P, bool same, const uint8_t* need_self, const uint8_t*& d_need) {
+  const size_t need_bytes = ((size_t)P.nparent + 16 + 63) & ~size_t(63);
+  uint8_t* er_misc = c.s_er_need.as<uint8_t>(need_bytes + 64);
+  d_need = nullptr;
+  if (same && need_self) {
+    GSJ_HIP(hipMemcpyAsync(er_misc, need_self, P.nparent, hipMemcpyHostToDevice, c.stream));
+    d_need = er_misc;
+  }
+  return reinterpret_cast<ErCtl*>(er_misc + need_bytes);
+}
+
+}  // namespace
+
 void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
                gsj_dist_cb cb, void* user, gsj_join_stats* st) {
   GSJ_REQUIRE(!PL.points && !RL.points, GSJ_E_ARG, "intersects_buffered: both layers must be polygon layers (points: the points entry point)");
@@ -628,16 +758,12 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
   const DPolys& R = RL.poly;
   const uint32_t* pfP = P.pfirst;   // first piece per parent (subdivided layers), as k_piece_first builds it
   const uint32_t* pfR = R.pfirst;
-  // one allocation for the need_self flags and the overflow queue's control block (allocated once per call)
-  const size_t need_bytes = ((size_t)P.nparent + 16 + 63) & ~size_t(63);
-  uint8_t* er_misc = c.s_er_need.as<uint8_t>(need_bytes + 64);
-  ErCtl* ctl = reinterpret_cast<ErCtl*>(er_misc + need_bytes);
   const uint8_t* d_need = nullptr;
-  if (same && need_self) {
-    GSJ_HIP(hipMemcpyAsync(er_misc, need_self, P.nparent, hipMemcpyHostToDevice, stream));
-    d_need = er_misc;
-  }
-  const double rmin = (ra > 0 && rb > 0) ? std::min(ra, rb) : (ra > 0 || rb > 0 ? 0.0 : -1.0);
+  ErCtl* ctl = er_scratch(c, P, same, need_self, d_need);
+  ErRadii rr{};
+  rr.k = 1;
+  rr.ra[0] = ra;
+  rr.rb[0] = rb;
   gsj_join_opts o1 = o;
   o1.want_area = 1;
   o1.want_relate = 0;
@@ -649,7 +775,7 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
       uint32_t* keep = c.s_er_keep.as<uint32_t>(n + 16);
       uint64_t* off = c.s_er_off.as<uint64_t>(n + 16);
       uint32_t* tmp = c.s_er_pairs.as<uint32_t>(2 * n + 16);
-      hipLaunchKernelGGL(k_er_prefilter, dim3(div_up(n, 256)), dim3(256), 0, stream, P, R, pa, pb, n, ra, rb, keep);
+      hipLaunchKernelGGL(k_er_prefilter, dim3(div_up(n, 256)), dim3(256), 0, stream, P, R, pa, pb, n, rr, keep);
       GSJ_LAUNCHED();
       const uint64_t k = scan_u32_to_u64(c, keep, off, n);
       hipLaunchKernelGGL(k_compact_pairs, dim3(div_up(n, 256)), dim3(256), 0, stream, pa, pb, keep, off, n, tmp, tmp + n);
@@ -666,77 +792,10 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
   double er_ms = 0;
   hooks.on_device = [&](const uint32_t* rows, const double* area, uint64_t n) {
     WallTimer t0;
-    uint32_t* keep = c.s_er_keep.as<uint32_t>(n + 16);
-    uint64_t* off = c.s_er_off.as<uint64_t>(n + 16);
-    hipLaunchKernelGGL(k_er_filter, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, area, n, P, pfP, rmin, same ? 1 : 0, d_need, keep);
-    GSJ_LAUNCHED();
-    const uint64_t ne = scan_u32_to_u64(c, keep, off, n);
-    if (ne == 0) { er_ms += t0.ms(); return; }
-    uint32_t* ep = c.s_er_pairs.as<uint32_t>(2 * ne + 16);
-    hipLaunchKernelGGL(k_compact_rows, dim3(div_up(n, 256)), dim3(256), 0, stream, rows, keep, off, n, ep);
-    GSJ_LAUNCHED();
-    uint8_t* flag = c.s_er_flag.as<uint8_t>(ne + 16);
-    uint32_t* lists = c.s_er_list.as<uint32_t>(3 * ne + 48);
-    uint32_t* heavy = lists;
-    uint32_t* again = lists + ne + 16;
-    uint32_t* first = again + ne + 16;   // the 16-lane pass's overflow, for the wave pass
-    ErCtl* ctl16 = ctl + 1;
-    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
-    hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl16);
-    GSJ_HIP(hipEventRecord(c.ev[0], stream));
-    // 16-lane groups with small lists first (most parent pairs), then a wave per pair for those that overflowed them
-    const uint32_t g16 = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES * 4), (uint64_t)c.cu_count * 32);
-    hipLaunchKernelGGL((k_erode_main<16, ER_CAPE16, ER_CAPL16>), dim3(g16), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, nullptr,
-                       nullptr, ra, rb, flag, first, ctl16);
-    GSJ_LAUNCHED();
-    const uint32_t g = (uint32_t)std::min<uint64_t>(div_up(ne, ER_WAVES), (uint64_t)c.cu_count * 32);
-    hipLaunchKernelGGL((k_erode_main<64, ER_CAPE, ER_CAPL>), dim3(g), dim3(ER_WAVES * 64), 0, stream, P, R, pfP, pfR, ep, (uint32_t)ne, first, ctl16,
-                       ra, rb, flag, heavy, ctl);
-    GSJ_LAUNCHED();
-    GSJ_HIP(hipEventRecord(c.ev[1], stream));
-    ErCtl* h = c.h_scal.as<ErCtl>(2);
-    GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
-    GSJ_HIP(hipStreamSynchronize(stream));
-    float ms = 0;
-    GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
-    s.k_large_ms += ms;
-    s.n_large += ne;
-    s.launches++;
-    uint32_t nh = h->n;
-    int capE = std::max(h->needE, ER_CAPE), capL = std::max(h->needL, 4 * capE);
-    s.heavy += nh;
-    s.n_over += nh;
-    for (int round = 0; nh; round++) {
-      GSJ_REQUIRE(round < 4, GSJ_E_INTERNAL, "erosion: overflowing pairs did not converge");
-      const size_t per = er_bytes(capE, capL);
-      const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(nh, (1ull << 30) / per));   // one list set per block
-      unsigned char* slab = c.s_er_slab.as<unsigned char>(per * blocks + 64);
-      hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
-      GSJ_HIP(hipEventRecord(c.ev[2], stream));
-      hipLaunchKernelGGL(k_erode_slab, dim3((uint32_t)blocks), dim3(ER_SLAB_T), 0, stream, P, R, pfP, pfR, ep, heavy, nh, ra, rb, flag, slab, capE,
-                         capL, again, ctl);
-      GSJ_LAUNCHED();
-      GSJ_HIP(hipEventRecord(c.ev[3], stream));
-      GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
-      GSJ_HIP(hipStreamSynchronize(stream));
-      float mo = 0;
-      GSJ_HIP(hipEventElapsedTime(&mo, c.ev[2], c.ev[3]));
-      s.k_over_ms += mo;
-      nh = h->n;
-      capE = std::max(capE, h->needE);
-      capL = std::max(capL, h->needL);
-      std::swap(heavy, again);
-    }
-    // rows: flagged parent pairs in pass-1 order
-    uint32_t* f = c.s_er_keep.as<uint32_t>(ne + 16);
-    hipLaunchKernelGGL(k_flag_u32, dim3(div_up(ne, 256)), dim3(256), 0, stream, flag, ne, f);
-    GSJ_LAUNCHED();
-    const uint64_t nhit = scan_u32_to_u64(c, f, off, ne);
+    uint32_t* out = nullptr;
+    const uint64_t nhit = erode_pass2(c, P, R, pfP, pfR, rows, area, n, rr, same, d_need, ctl, s, out, nullptr, nullptr);
     er_ms += t0.ms();
     if (nhit == 0) return;
-    uint32_t* out = c.s_er_out.as<uint32_t>(2 * nhit + 16);
-    hipLaunchKernelGGL(k_compact_rows, dim3(div_up(ne, 256)), dim3(256), 0, stream, ep, f, off, ne, out);
-    GSJ_LAUNCHED();
     rows_out += nhit;
     WallTimer t4;
     uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nhit);
@@ -756,6 +815,106 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
   s.n_small = js.n_small + js.n_large;
   s.hits = rows_out;
   if (st) *st = s;
+}
+
+bool erode_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o, const uint8_t* need_self) {
+  return o.has_window && !o.cancel && !need_self && !P.points && !R.points && R.has_bvh && P.n && R.n && !P.poly.parent && !R.poly.parent &&
+         probes_batchable(probes_of(P));
+}
+
+void erode_batch_run(Context& c, const Layer& PL, const Layer& RL, bool same, std::vector<BatchErode>& reqs) {
+  const uint32_t k = (uint32_t)reqs.size();
+  GSJ_REQUIRE(k >= 1 && k <= MAX_WINDOWS, GSJ_E_ARG, "batch size out of range");
+  const gsj_join_opts& o = *reqs[0].o;
+  WindowSet ws;
+  ErRadii rr{};
+  ws.k = rr.k = k;
+  for (uint32_t r = 0; r < k; r++) {
+    const gsj_join_opts& q = *reqs[r].o;
+    GSJ_REQUIRE(erode_batchable(PL, RL, q, nullptr) && q.upper == o.upper && q.batch_q == o.batch_q && q.max_pairs == o.max_pairs &&
+                    q.snap_half_ulp == o.snap_half_ulp,
+                GSJ_E_ARG, "request cannot join this batch");
+    GSJ_REQUIRE(reqs[r].ra >= 0 && reqs[r].rb >= 0 && reqs[r].ra < 1e5 && reqs[r].rb < 1e5, GSJ_E_ARG,
+                "intersects_buffered: radii must be in [0, 1e5) metres");
+    for (int j = 0; j < 4; j++) ws.w[r][j] = q.window[j];
+    rr.ra[r] = reqs[r].ra;
+    rr.rb[r] = reqs[r].rb;
+  }
+  hipStream_t stream = c.stream;
+  const DPolys& P = PL.poly;
+  const DPolys& R = RL.poly;
+  const uint8_t* d_need = nullptr;
+  ErCtl* ctl = er_scratch(c, P, same, nullptr, d_need);
+  RefineParams rp;   // pass 1: intersection areas
+  rp.want_area = 1;
+  rp.want_relate = 0;
+  rp.snap_half_ulp = o.snap_half_ulp ? 1 : 0;
+  gsj_join_stats s{};
+  std::vector<uint64_t> cand(k, 0), hits(k, 0), rlo(k + 1);
+  uint64_t* lo1 = wseg(c, 4);   // kept candidates
+  uint64_t* lo2 = wseg(c, 5);   // pass-1 rows
+  uint64_t* lo3 = wseg(c, 6);   // pass-2 pairs
+  uint64_t* lo4 = wseg(c, 7);   // output rows
+  for_each_candidate_multi(c, probes_of(PL), RL.bvh, o.upper != 0, o, ws, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np, const SegBounds& sb) {
+    for (uint32_t r = 0; r < k; r++) cand[r] += sb.h[r + 1] - sb.h[r];
+    // pass 1: candidates that the erosion can leave intersecting, refined for their overlap areas
+    WallTimer tp;
+    uint32_t* keep = c.s_er_keep.as<uint32_t>(np + 16);
+    uint64_t* off = c.s_er_off.as<uint64_t>(np + 16);
+    uint32_t* tmp = c.s_er_pairs.as<uint32_t>(2 * np + 16);
+    ErRadii r1 = rr;
+    r1.seg = sb.d;
+    hipLaunchKernelGGL(k_er_prefilter, dim3(div_up(np, 256)), dim3(256), 0, stream, P, R, pa, pb, np, r1, keep);
+    GSJ_LAUNCHED();
+    const uint64_t nk = scan_u32_to_u64(c, keep, off, np);
+    seg_bounds_dev(c, sb.d, k, off, np, nk, lo1);
+    hipLaunchKernelGGL(k_compact_pairs, dim3(div_up(np, 256)), dim3(256), 0, stream, pa, pb, keep, off, np, tmp, tmp + np);
+    GSJ_LAUNCHED();
+    s.compact_ms += tp.ms();
+    if (nk == 0) return;
+    hipLaunchKernelGGL(k_copy_u32, dim3(div_up(nk, 256)), dim3(256), 0, stream, tmp, nk, pa);
+    hipLaunchKernelGGL(k_copy_u32, dim3(div_up(nk, 256)), dim3(256), 0, stream, tmp + np, nk, pb);
+    GSJ_LAUNCHED();
+    WallTimer t2;
+    RefineOut ro{};
+    ro.isect = c.s_flag.as<uint8_t>(nk + 16);
+    ro.area = c.s_area.as<double>(nk + 16);
+    s.heavy += refine_pairs(c, P, R, pa, pb, nk, rp, ro, &s);
+    uint32_t* orow = c.s_out_a.as<uint32_t>(2 * nk + 16);
+    double* oarea = c.s_out_area.as<double>(nk + 16);
+    const uint64_t* offs = nullptr;
+    const uint64_t nrows = compact_single_rows(c, PL, RL, pa, pb, nk, rp, ro, orow, oarea, &offs);
+    seg_bounds_dev(c, lo1, k, offs, nk, nrows, lo2);
+    // pass 2
+    ErRadii r2 = rr;
+    r2.seg = lo2;
+    uint32_t* out = nullptr;
+    const uint64_t nhit = nrows ? erode_pass2(c, P, R, nullptr, nullptr, orow, oarea, nrows, r2, same, d_need, ctl, s, out, lo3, lo4) : 0;
+    if (nrows == 0) seg_bounds_dev(c, lo2, k, offs, 0, 0, lo4);
+    GSJ_HIP(hipMemcpyAsync(rlo.data(), lo4, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.refine_ms += t2.ms();
+    if (nhit == 0) return;
+    WallTimer t4;
+    uint32_t* hrow = c.h_pairs.as<uint32_t>(2 * nhit);
+    GSJ_HIP(hipMemcpyAsync(hrow, out, nhit * 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    s.copy_ms += t4.ms();
+    for (uint32_t r = 0; r < k; r++) {
+      const uint64_t a = rlo[r], n = rlo[r + 1] - rlo[r];
+      hits[r] += n;
+      BatchErode& q = reqs[r];
+      if (n == 0 || !q.cb || q.rc != GSJ_OK) continue;
+      if (q.cb(q.user, hrow + 2 * a, nullptr, (size_t)n)) q.rc = GSJ_E_CANCEL;
+    }
+  });
+  for (uint32_t r = 0; r < k; r++) {
+    if (!reqs[r].st) continue;
+    gsj_join_stats& st = *reqs[r].st;
+    st = s;
+    st.candidates = cand[r];
+    st.hits = hits[r];
+  }
 }
 
 }  // namespace gsj

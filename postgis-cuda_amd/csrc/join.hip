@@ -220,6 +220,12 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
   if (st) *st = s;
 }
 
+uint64_t compact_single_rows(Context& c, const Layer& PL, const Layer& RL, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
+                             const RefineOut& ro, uint32_t* orow, double* oarea, const uint64_t** offs) {
+  GSJ_REQUIRE(!PL.poly.pfirst && !RL.poly.pfirst, GSJ_E_INTERNAL, "candidate-ordered rows need layers without parents");
+  return compact_rows(c, PL, RL, pa, pb, np, rp, ro, false, orow, oarea, nullptr, offs);
+}
+
 bool join_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o) {
   // rows map one to one onto candidates (no piece -> parent folding), so a request's rows are a contiguous run
   return o.has_window && !o.cancel && !P.points && !R.points && R.has_bvh && P.n && R.n && probes_batchable(probes_of(P)) &&

@@ -8,9 +8,10 @@
 // up, and their kernels overlap on the device (HIP runs kernels of different streams concurrently). Each lane runs
 // the same code as the direct entry points, so a request's rows are the same alone and in the mix.
 //
-// Windowed joins of the same layers and options are also batched: a lane that admits one takes up to max_batch - 1
-// more such requests from anywhere in the queue and runs them as one candidate search and one refinement pass
-// (join_batch_run), which pays a small request's chain of launches and synchronisations once per batch.
+// Windowed joins, distance joins and eroded intersects of the same layers and options are also batched: a lane that
+// admits one takes up to max_batch - 1 more such requests of its kind from anywhere in the queue and runs them as one
+// candidate search and one refinement pass (join_batch_run, dwithin_batch_run, erode_batch_run), which pays a small
+// request's chain of launches and synchronisations, and its longest pair, once per batch.
 #include "capi_internal.h"
 #include <chrono>
 #include <condition_variable>
@@ -67,6 +68,7 @@ struct gsj_sched {
   static bool batchable(const Req& q) {
     if (q.r.kind == GSJ_REQ_JOIN) return gsj::join_batchable(q.r.P->L, q.r.R->L, q.r.opts);
     if (q.r.kind == GSJ_REQ_DWITHIN) return gsj::dwithin_batchable(q.r.P->L, q.r.R->L, q.r.opts);
+    if (q.r.kind == GSJ_REQ_INTERSECTS_BUFFERED) return gsj::erode_batchable(q.r.P->L, q.r.R->L, q.r.opts, q.r.need_self);
     return false;
   }
   bool batch_head(const Req& q) const { return max_batch > 1 && batchable(q); }
@@ -74,8 +76,9 @@ struct gsj_sched {
     if (x.r.kind != head.r.kind || x.r.P != head.r.P || x.r.R != head.r.R || !batchable(x)) return false;
     const gsj_join_opts &a = head.r.opts, &b = x.r.opts;
     if (x.r.kind == GSJ_REQ_JOIN) return gsj::join_compatible(a, b);
-    return x.r.same == head.r.same && x.r.report_distance == head.r.report_distance && a.upper == b.upper && a.batch_q == b.batch_q &&
-           a.max_pairs == b.max_pairs;
+    const bool common = x.r.same == head.r.same && a.upper == b.upper && a.batch_q == b.batch_q && a.max_pairs == b.max_pairs;
+    if (x.r.kind == GSJ_REQ_DWITHIN) return common && x.r.report_distance == head.r.report_distance;
+    return common && a.snap_half_ulp == b.snap_half_ulp;
   }
 };
 
@@ -92,6 +95,18 @@ void gsj_sched::run_batch(gsj::Context& c, std::vector<Req*>& group) {
       b[i].st = &group[i]->stats;
     }
     rc = gsj::guarded_status([&] { gsj::join_batch_run(c, h.P->L, h.R->L, b); });
+    for (size_t i = 0; i < group.size(); i++) rcs[i] = b[i].rc;
+  } else if (h.kind == GSJ_REQ_INTERSECTS_BUFFERED) {
+    std::vector<gsj::BatchErode> b(group.size());
+    for (size_t i = 0; i < group.size(); i++) {
+      b[i].o = &group[i]->r.opts;
+      b[i].ra = group[i]->r.ra;
+      b[i].rb = group[i]->r.rb;
+      b[i].cb = group[i]->r.dist_cb;
+      b[i].user = group[i]->r.user;
+      b[i].st = &group[i]->stats;
+    }
+    rc = gsj::guarded_status([&] { gsj::erode_batch_run(c, h.P->L, h.R->L, h.same != 0, b); });
     for (size_t i = 0; i < group.size(); i++) rcs[i] = b[i].rc;
   } else {
     std::vector<gsj::BatchDist> b(group.size());

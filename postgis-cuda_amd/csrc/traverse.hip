@@ -143,8 +143,8 @@ uint32_t window_members_tiled(Context& c, const Probes& pr, const WindowSet& ws,
   const uint32_t ne = ws.k * nt;
   uint32_t* tf = c.s_wtile.as<uint32_t>((uint64_t)ne + 16);
   uint64_t* toff = c.s_wtoff.as<uint64_t>((uint64_t)ne + 16);
-  uint64_t* tstart = c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1));
-  mstart = tstart + 2 * (MAX_WINDOWS + 1);
+  uint64_t* tstart = wseg(c, 0);
+  mstart = wseg(c, 2);
   hipLaunchKernelGGL(k_win_tile_flags, dim3(div_up(ne, 256)), dim3(256), 0, stream, pr.tiles, nt, ws, tf);
   GSJ_LAUNCHED();
   const uint64_t nsel = scan_u32_to_u64(c, tf, toff, ne);
@@ -223,7 +223,7 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
   const std::vector<uint32_t>& hp = pr.h_parent ? *pr.h_parent : none;
   auto pid = [&](uint32_t pos) { return hp[qidx ? hq[pos] : pos]; };   // parent of the probe at a position
   std::vector<uint32_t> hcounts;
-  uint64_t* dlo = mstart ? c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1)) + (MAX_WINDOWS + 1) : nullptr;   // beside tstart, mstart
+  uint64_t* dlo = mstart ? wseg(c, 1) : nullptr;
   uint64_t* hlo = mstart ? c.h_seg.as<uint64_t>(MAX_WINDOWS + 1) : nullptr;
   for (uint32_t q0 = 0, qn; q0 < nQ; q0 = qn) {
     qn = (uint32_t)std::min<uint64_t>((uint64_t)q0 + batch_q, nQ);
@@ -362,10 +362,16 @@ __global__ void k_row_bounds(const uint64_t* __restrict__ lo, uint32_t k, const 
 }
 }  // namespace
 
-void seg_rows(Context& c, const SegBounds& sb, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* h_rlo) {
-  uint64_t* d = c.s_wseg.as<uint64_t>(4 * (MAX_WINDOWS + 1)) + 3 * (MAX_WINDOWS + 1);   // after the windows' bounds
-  hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, c.stream, sb.d, sb.k, offs, np, nrows, d);
+uint64_t* wseg(Context& c, int region) { return c.s_wseg.as<uint64_t>(8 * (MAX_WINDOWS + 1)) + (size_t)region * (MAX_WINDOWS + 1); }
+
+void seg_bounds_dev(Context& c, const uint64_t* lo, uint32_t k, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* out) {
+  hipLaunchKernelGGL(k_row_bounds, dim3(1), dim3(64), 0, c.stream, lo, k, offs, np, nrows, out);
   GSJ_LAUNCHED();
+}
+
+void seg_rows(Context& c, const SegBounds& sb, const uint64_t* offs, uint64_t np, uint64_t nrows, uint64_t* h_rlo) {
+  uint64_t* d = wseg(c, 3);
+  seg_bounds_dev(c, sb.d, sb.k, offs, np, nrows, d);
   GSJ_HIP(hipMemcpyAsync(h_rlo, d, (size_t)(sb.k + 1) * 8, hipMemcpyDeviceToHost, c.stream));
   GSJ_HIP(hipStreamSynchronize(c.stream));
 }

@@ -239,3 +239,43 @@ def test_batched_windowed_dwithin_equal_their_own_calls(ctx):
             assert np.array_equal(g.dist, d.dist)
         assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
     assert len(got[5].pairs) > 0 and len(got[6].pairs) == 0
+
+
+def test_batched_windowed_erosion_equal_their_own_calls(ctx):
+    """Windowed eroded-intersects requests batch across different radii (the prefilter, the pass-2 filter and the
+    erosion read each pair's own request's radii); rows, order, candidates and hits equal the direct calls."""
+    M = gsj.generate("parcels_mesh", nx=50, ny=50, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=14)
+    L = ctx.polys(M)
+    big = ctx.polys(gsj.subdivide(_mesh(150, 8), 10))
+    x0, y0 = M.xy[:, 0].min(), M.xy[:, 1].min()
+    W, H = M.xy[:, 0].max() - x0, M.xy[:, 1].max() - y0
+    rng = np.random.default_rng(15)
+    reqs = []
+    for i in range(36):
+        f = rng.uniform(0, 1, 2)
+        s = rng.uniform(0.03, 0.4)
+        w = (x0 + f[0] * W, y0 + f[1] * H, x0 + f[0] * W + s * W, y0 + f[1] * H + s * H)
+        ra, rb = (float(v) for v in rng.uniform(0.5, 3.0, 2))
+        if i % 9 == 0:
+            ra = rb = 0.0        # plain intersects
+        elif i % 9 == 1:
+            rb = 0.0             # one side eroded
+        reqs.append(dict(ra=ra, rb=rb, same=True, upper=True, window=w))
+    reqs[4]["window"] = (x0 - 1, y0 - 1, x0 + W + 1, y0 + H + 1)
+    reqs[7]["window"] = (x0 + W + 1, y0 + H + 1, x0 + W + 2, y0 + H + 2)
+    s = gsj.Scheduler(ctx, lanes=1, batch=16)
+    try:
+        t0 = s.submit(gsj.REQ_JOIN, big, big, want_relate=True, want_area=True, upper=True)
+        tickets = [s.submit(gsj.REQ_INTERSECTS_BUFFERED, L, L, **kw) for kw in reqs]
+        s.wait(t0)
+        got = [s.wait(t) for t in tickets]
+    finally:
+        s.close()
+    batched = [g.stats["times"]["batch"] for g in got]
+    assert max(batched) > 1 and sum(b > 1 for b in batched) >= len(got) // 2
+    for kw, g in zip(reqs, got):
+        kw = dict(kw)
+        d = ctx.intersects_buffered(L, L, kw.pop("ra"), kw.pop("rb"), **kw)
+        assert np.array_equal(g.pairs, d.pairs)
+        assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
+    assert len(got[4].pairs) > 0 and len(got[7].pairs) == 0
