@@ -93,7 +93,7 @@ typedef struct {
   /* Query window (no counterpart in the reference; the windowed serving requests use it): with has_window set,
    * only probe-side parents with a piece whose float box meets the closed rectangle window = {x0, y0, x1, y1}
    * (layer coordinates) are probed, so the rows are exactly the unwindowed rows whose probe parent meets it.
-   * Joins, point joins and the distance / eroded joins honour it; gsj_cluster rejects it. */
+   * Joins, point joins and the distance / eroded joins honour it; gsj_cluster rejects it (gsj_cluster_rows takes it). */
   int has_window;
   double window[4];
 } gsj_join_opts;
@@ -141,6 +141,12 @@ int gsj_points_in_polygons(gsj_ctx* ctx, const gsj_layer* pts, const gsj_layer* 
 /* ST_ClusterIntersecting over the parents of P (replaces cluster_intersecting, engine/engine.cuh:1216-1234):
  * root[g] = smallest parent index of g's connected component, for g in [0, nparent). P needs a BVH. */
 int gsj_cluster(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, uint32_t* root, uint64_t nroot, gsj_join_stats* stats);
+/* The same as rows for the served path: pairs[2i] = parent g, pairs[2i+1] = smallest parent index of g's component,
+ * in ascending g, through cb (area and code NULL). With opts->has_window: cluster_intersecting of the sub-layer of
+ * the parents whose box meets the window (components over the intersecting pairs of two such parents), rows for those
+ * parents only; needs a layer without subdivided parents. No counterpart in the reference (cluster_intersecting,
+ * engine/engine.cuh:1216-1234, has no window; the windowed serving stream asks for one). */
+int gsj_cluster_rows(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, gsj_rows_cb cb, void* user, gsj_join_stats* stats);
 
 /* ------------------------------------------------------------------ distance joins (lon/lat, metres, WGS84)
  * Row callback of the distance joins: pairs[2i], pairs[2i+1] = (a, b) (polygon joins: parent ids of P and R; point
@@ -193,6 +199,7 @@ typedef struct gsj_sched gsj_sched;
 #define GSJ_REQ_POINTS_DWITHIN 5      /* gsj_points_dwithin: d, report_distance, dist_cb */
 #define GSJ_REQ_INTERSECTS_BUFFERED 6 /* gsj_intersects_buffered: ra, rb, same, need_self, dist_cb */
 #define GSJ_REQ_POINTS_INTERSECTS_BUFFERED 7  /* gsj_points_intersects_buffered: rb (the polygon radius), dist_cb */
+#define GSJ_REQ_CLUSTER_ROWS 8        /* gsj_cluster_rows: rows_cb */
 
 typedef struct {
   int kind;                       /* GSJ_REQ_* */

@@ -289,6 +289,16 @@ int gsj_cluster(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, uin
   });
 }
 
+int gsj_cluster_rows(gsj_ctx* ctx, const gsj_layer* P, const gsj_join_opts* opts, gsj_rows_cb cb, void* user, gsj_join_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && P, GSJ_E_ARG, "null context or layer");
+    GSJ_REQUIRE(P->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::cluster_rows_run(ctx->c, P->L, defaults(opts), cb, user, stats);
+  });
+}
+
 void gsj_de9im_str(uint32_t code, char out[9]) {
   for (int k = 0; k < 9; k++) out[k] = "F012"[(code >> (2 * k)) & 3];
 }

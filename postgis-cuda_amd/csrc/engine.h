@@ -206,6 +206,8 @@ struct JoinHooks {
   // candidate filter before refinement (JoinOpts::prefilter, engine/engine.cuh:812, 935): compacts the sub-batch's
   // pairs in place on the device and returns how many are kept
   std::function<uint64_t(uint32_t* pa, uint32_t* pb, uint64_t np)> prefilter;
+  // the probes about to be walked (for_each_candidate_batch's on_members)
+  MembersFn on_members;
 };
 void join_run(Context& c, const Layer& P, const Layer& R, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st,
               const JoinHooks* hooks);
@@ -228,6 +230,7 @@ void join_batch_run(Context& c, const Layer& P, const Layer& R, std::vector<Batc
 void points_join_run(Context& c, const Layer& pts, const Layer& R, const gsj_join_opts& o, gsj_points_cb cb, void* user,
                      gsj_join_stats* st);
 void cluster_run(Context& c, const Layer& P, const gsj_join_opts& o, uint32_t* root, uint64_t nroot, gsj_join_stats* st);
+void cluster_rows_run(Context& c, const Layer& P, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st);
 
 // ------------------------------------------------------------------------------------------- distance joins (dwithin.hip)
 void dwithin_run(Context& c, const Layer& P, const Layer& R, double d, bool same, bool report, const gsj_join_opts& o, gsj_dist_cb cb, void* user,

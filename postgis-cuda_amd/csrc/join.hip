@@ -180,6 +180,7 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
   rp.want_area = o.want_area ? 1 : 0;
   rp.want_relate = o.want_relate ? 1 : 0;
   rp.snap_half_ulp = o.snap_half_ulp ? 1 : 0;
+  const MembersFn none;
   for_each_candidate_batch(c, probes_of(PL), RL.bvh, o.upper != 0, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
     if (hooks && hooks->prefilter) {
       WallTimer tp;
@@ -216,7 +217,7 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
     GSJ_HIP(hipStreamSynchronize(stream));
     s.copy_ms += t4.ms();
     if (cb && cb(user, hrow, harea, hcode, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "join cancelled by the row callback");
-  });
+  }, hooks ? hooks->on_members : none);
   if (st) *st = s;
 }
 

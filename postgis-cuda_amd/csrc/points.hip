@@ -1,6 +1,7 @@
 // Points in polygons (reference: k_pip_pairs / join_points_stream, engine/engine.cuh:1079-1178) and
 // ST_ClusterIntersecting by lock-free union-find over the intersects rows (engine/engine.cuh:1183-1234).
 #include "engine.h"
+#include <algorithm>
 #include <chrono>
 
 namespace gsj {
@@ -122,6 +123,37 @@ __global__ void k_uf_union(const uint32_t* __restrict__ rows, uint64_t n, uint32
   }
 }
 
+// windowed clusters: only the members' entries of parent[] are initialised and ever read
+__global__ void k_uf_init_list(uint32_t* parent, const uint32_t* __restrict__ list, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) parent[list[i]] = list[i];
+}
+// rows whose two parents both meet the window (the probe side always does; the same closed test in double as the
+// membership), hooked as k_uf_union does
+__global__ void k_uf_union_window(const uint32_t* __restrict__ rows, uint64_t n, const float4* __restrict__ bb, double x0, double y0, double x1,
+                                  double y1, uint32_t* parent) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t a = rows[2 * i], b = rows[2 * i + 1];
+  if (a == b) return;
+  const float4 q = bb[b];
+  if (!((double)q.x <= x1 && (double)q.z >= x0 && (double)q.y <= y1 && (double)q.w >= y0)) return;
+  for (;;) {
+    uint32_t ra = uf_find(parent, a), rb = uf_find(parent, b);
+    if (ra == rb) return;
+    if (ra > rb) { const uint32_t t = ra; ra = rb; rb = t; }
+    uint32_t expect = rb;
+    if (__hip_atomic_compare_exchange_strong(parent + rb, &expect, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  }
+}
+__global__ void k_uf_roots_list(uint32_t* parent, const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ rows) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = list[i];
+  rows[2 * i] = g;
+  rows[2 * i + 1] = uf_find(parent, g);
+}
+
 // roots into a separate array (writing them into parent[] would race with other threads' path-halving stores)
 __global__ void k_uf_roots(uint32_t* parent, uint32_t n, uint32_t* root) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -213,6 +245,65 @@ void cluster_run(Context& c, const Layer& PL, const gsj_join_opts& o, uint32_t* 
     GSJ_HIP(hipMemcpyAsync(root, droot, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
   }
   GSJ_HIP(hipStreamSynchronize(stream));
+}
+
+void cluster_rows_run(Context& c, const Layer& PL, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st) {
+  GSJ_REQUIRE(!PL.points, GSJ_E_ARG, "cluster: needs a polygon layer");
+  GSJ_REQUIRE(PL.has_bvh, GSJ_E_ARG, "cluster: layer has no BVH");
+  const uint32_t n = PL.nparent;
+  if (!o.has_window) {   // every parent, in id order
+    std::vector<uint32_t> root(std::max<uint32_t>(n, 1));
+    cluster_run(c, PL, o, root.data(), n, st);
+    const uint32_t chunk = 1u << 20;
+    std::vector<uint32_t> rows(2 * (size_t)std::min(n, chunk));
+    for (uint32_t g0 = 0; g0 < n; g0 += chunk) {
+      const uint32_t m = std::min(chunk, n - g0);
+      for (uint32_t i = 0; i < m; i++) { rows[2 * i] = g0 + i; rows[2 * i + 1] = root[g0 + i]; }
+      if (cb && cb(user, rows.data(), nullptr, nullptr, m)) throw Error(GSJ_E_CANCEL, "cluster cancelled by the row callback");
+    }
+    return;
+  }
+  GSJ_REQUIRE(PL.h_parent.empty(), GSJ_E_ARG, "a windowed cluster needs a layer without subdivided parents");
+  GSJ_REQUIRE(o.window[0] <= o.window[2] && o.window[1] <= o.window[3], GSJ_E_ARG, "window must have x0 <= x1 and y0 <= y1");
+  hipStream_t stream = c.stream;
+  uint32_t* parent = c.s_uf.as<uint32_t>((uint64_t)n + 16);
+  const uint32_t* mlist = nullptr;
+  uint32_t nm = 0;
+  gsj_join_opts jo = o;
+  jo.want_area = 0; jo.want_relate = 0; jo.upper = 1;   // intersects is symmetric: each unordered pair once
+  JoinHooks hooks;
+  hooks.device_only = true;
+  hooks.on_members = [&](const uint32_t* list, uint32_t m, const uint64_t*) -> const float4* {
+    mlist = list;
+    nm = m;
+    if (m) {
+      hipLaunchKernelGGL(k_uf_init_list, dim3(div_up(m, 256)), dim3(256), 0, stream, parent, list, m);
+      GSJ_LAUNCHED();
+    }
+    return nullptr;
+  };
+  hooks.on_device = [&](const uint32_t* rows, const double*, uint64_t nr) {
+    hipLaunchKernelGGL(k_uf_union_window, dim3(div_up(nr, 256)), dim3(256), 0, stream, rows, nr, PL.poly.bb, o.window[0], o.window[1], o.window[2],
+                       o.window[3], parent);
+    GSJ_LAUNCHED();
+  };
+  gsj_join_stats s{};
+  join_run(c, PL, PL, jo, nullptr, nullptr, &s, &hooks);
+  if (nm) {
+    uint32_t* drows = c.s_root.as<uint32_t>(2 * (uint64_t)nm + 16);
+    hipLaunchKernelGGL(k_uf_roots_list, dim3(div_up(nm, 256)), dim3(256), 0, stream, parent, mlist, nm, drows);
+    GSJ_LAUNCHED();
+    std::vector<uint64_t> rows(nm);   // (parent, root) packed, sorted by parent
+    uint32_t* h = c.h_pairs.as<uint32_t>(2 * (uint64_t)nm);
+    GSJ_HIP(hipMemcpyAsync(h, drows, (size_t)nm * 8, hipMemcpyDeviceToHost, stream));
+    GSJ_HIP(hipStreamSynchronize(stream));
+    for (uint32_t i = 0; i < nm; i++) rows[i] = ((uint64_t)h[2 * i] << 32) | h[2 * i + 1];
+    std::sort(rows.begin(), rows.end());
+    for (uint32_t i = 0; i < nm; i++) { h[2 * i] = (uint32_t)(rows[i] >> 32); h[2 * i + 1] = (uint32_t)rows[i]; }
+    if (cb && cb(user, h, nullptr, nullptr, nm)) throw Error(GSJ_E_CANCEL, "cluster cancelled by the row callback");
+  }
+  GSJ_HIP(hipStreamSynchronize(stream));
+  if (st) *st = s;
 }
 
 }  // namespace gsj

@@ -141,6 +141,9 @@ void gsj_sched::run(gsj::Context& c, Req& q) {
     case GSJ_REQ_CLUSTER:
       gsj::cluster_run(c, r.P->L, r.opts, r.root, r.nroot, &q.stats);
       break;
+    case GSJ_REQ_CLUSTER_ROWS:
+      gsj::cluster_rows_run(c, r.P->L, r.opts, r.rows_cb, r.user, &q.stats);
+      break;
     case GSJ_REQ_DWITHIN:
       gsj::dwithin_run(c, r.P->L, r.R->L, r.d, r.same != 0, r.report_distance != 0, r.opts, r.dist_cb, r.user, &q.stats);
       break;
@@ -262,7 +265,7 @@ int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket) {
     GSJ_REQUIRE(s && req && ticket, GSJ_E_ARG, "null scheduler, request or ticket");
     const gsj_request& r = *req;
     GSJ_REQUIRE(r.P, GSJ_E_ARG, "request without a probe layer");
-    GSJ_REQUIRE(r.kind == GSJ_REQ_CLUSTER || r.R, GSJ_E_ARG, "request without a build layer");
+    GSJ_REQUIRE(r.kind == GSJ_REQ_CLUSTER || r.kind == GSJ_REQ_CLUSTER_ROWS || r.R, GSJ_E_ARG, "request without a build layer");
     GSJ_REQUIRE(r.P->L.ctx == &s->owner->c && (!r.R || r.R->L.ctx == &s->owner->c), GSJ_E_ARG,
                 "layer belongs to another context than the scheduler's");
     auto q = std::make_unique<Req>();

@@ -58,7 +58,7 @@ POINTS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.
 DIST_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.c_size_t)
 
 REQ_JOIN, REQ_POINTS_IN_POLYGONS, REQ_CLUSTER, REQ_DWITHIN, REQ_POINTS_DWITHIN = 1, 2, 3, 4, 5
-REQ_INTERSECTS_BUFFERED, REQ_POINTS_INTERSECTS_BUFFERED = 6, 7
+REQ_INTERSECTS_BUFFERED, REQ_POINTS_INTERSECTS_BUFFERED, REQ_CLUSTER_ROWS = 6, 7, 8
 
 
 class Request(C.Structure):
@@ -78,7 +78,7 @@ _lib = None
 EXPORTS = [
     "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
-    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
+    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_cluster_rows", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
     "gsj_points_intersects_buffered",
     "gsj_sched_create", "gsj_sched_destroy", "gsj_sched_set_batch", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
@@ -107,6 +107,7 @@ def load(path: str | None = None):
         "gsj_join": (i32, [vp, vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_in_polygons": (i32, [vp, vp, vp, C.POINTER(JoinOpts), POINTS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_cluster": (i32, [vp, vp, C.POINTER(JoinOpts), u32p, u64, C.POINTER(JoinStats)]),
+        "gsj_cluster_rows": (i32, [vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_dwithin": (i32, [vp, vp, vp, C.c_double, i32, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_dwithin": (i32, [vp, vp, vp, C.c_double, i32, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
         "gsj_intersects_buffered": (i32, [vp, vp, vp, C.c_double, C.c_double, i32, u8p, C.POINTER(JoinOpts), DIST_CB, vp, C.POINTER(JoinStats)]),
@@ -509,6 +510,19 @@ class Context:
         _check(load().gsj_cluster(self.h, P.h, C.byref(o), _ptr(root, C.c_uint32), n, C.byref(st)))
         return root[:n], st.as_dict()
 
+    def cluster_rows(self, P: Layer, **kw) -> Rows:
+        """(parent, component root) rows in ascending parent; with window=, the clusters of the parents meeting it."""
+        o = _opts(**kw)
+        chunks = []
+
+        def cb(user, pairs, area, code, n):
+            chunks.append(np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2))
+            return 0
+
+        st = JoinStats()
+        _check(load().gsj_cluster_rows(self.h, P.h, C.byref(o), ROWS_CB(cb), None, C.byref(st)))
+        return Rows(np.concatenate(chunks) if chunks else np.zeros((0, 2), np.uint32), None, None, None, st.as_dict())
+
     def close(self):
         if self.h:
             for L in list(self._layers):   # layers are device objects of this context: free them first
@@ -562,7 +576,7 @@ class Scheduler:
             n = P.info().nparent
             root = np.zeros(max(n, 1), np.uint32)
             req.root, req.nroot = _ptr(root, C.c_uint32), n
-        elif collect and kind == REQ_JOIN:
+        elif collect and kind in (REQ_JOIN, REQ_CLUSTER_ROWS):
             def cb(user, pairs, area, code, n):
                 p = np.ctypeslib.as_array(pairs, (2 * n,)).copy().reshape(-1, 2)
                 with lock:
@@ -607,6 +621,8 @@ class Scheduler:
         pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
         if kind == REQ_CLUSTER:
             return Rows(np.zeros((0, 2), np.uint32), None, None, None, stats, None, root[:keep[0].nroot])
+        if kind == REQ_CLUSTER_ROWS:
+            return Rows(pairs, None, None, None, stats)
         if kind == REQ_JOIN:
             area = np.concatenate([c[1] for c in chunks]) if chunks and o.want_area else None
             code = np.concatenate([c[2] for c in chunks]) if chunks and o.want_relate else None

@@ -122,3 +122,59 @@ def test_window_through_the_scheduler(ctx):
     finally:
         s.close()
     _same(got, ctx.join(L, L, want_area=True, upper=True), _members(L, M, win), ("area",))
+
+
+def _components(n, edges):
+    """smallest member id of each vertex's connected component (breadth-first search)"""
+    adj = [[] for _ in range(n)]
+    for a, b in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    root = np.full(n, -1, np.int64)
+    for s in range(n):
+        if root[s] >= 0:
+            continue
+        comp, stack = [s], [s]
+        root[s] = s
+        while stack:
+            v = stack.pop()
+            for w in adj[v]:
+                if root[w] < 0:
+                    root[w] = s
+                    comp.append(w)
+                    stack.append(w)
+        root[comp] = min(comp)
+    return root
+
+
+@pytest.mark.parametrize("w", WINDOWS)
+def test_windowed_cluster_rows_are_the_sub_layer_components(ctx, w):
+    """cluster_intersecting of the parcels meeting a window: components over the intersecting pairs of two such
+    parcels (the intersects rows of the full self-join), root = smallest parcel id; rows for those parcels only, in id
+    order. Without a window, the rows are gsj_cluster's roots."""
+    M = _mesh(60, seed=5, median=14.0)
+    L = ctx.polys(M)
+    win = _window(M, *w)
+    keep = _members(L, None, win)
+    full = ctx.join(L, L, upper=True).pairs
+    inside = full[keep[full[:, 0]] & keep[full[:, 1]]]
+    root = _components(M.n, inside)
+    ids = np.flatnonzero(keep)
+    got = ctx.cluster_rows(L, window=win)
+    assert np.array_equal(got.pairs[:, 0], ids)
+    assert np.array_equal(got.pairs[:, 1], root[ids])
+    roots, _ = ctx.cluster(L)
+    allr = ctx.cluster_rows(L)
+    assert np.array_equal(allr.pairs[:, 0], np.arange(M.n)) and np.array_equal(allr.pairs[:, 1], roots)
+    s = gsj.Scheduler(ctx, lanes=2)
+    try:
+        sv = s.wait(s.submit(gsj.REQ_CLUSTER_ROWS, L, window=win))
+    finally:
+        s.close()
+    assert np.array_equal(sv.pairs, got.pairs)
+
+
+def test_windowed_cluster_needs_unsubdivided_layers(ctx):
+    S = ctx.polys(gsj.subdivide(_mesh(20, seed=2, median=14.0), 12))
+    with pytest.raises(gsj.GsjError, match="windowed cluster"):
+        ctx.cluster_rows(S, window=(-1e3, -1e3, 1e3, 1e3))
