@@ -166,9 +166,10 @@ def national_leg(gsj, ctx, args):
     sqrt(scale). Points: 161M at scale 1, half uniform over the extent, half in clusters of sigma 50 m, seed 4.
     On the device: upload, LBVH, self-dwithin at 5 m of the whole parcel layer and points-in-polygons of every point
     (rows counted on the host). Then a request stream over the resident parcels with Poisson arrivals: self joins in
-    query windows of area lognormal in [1e-4, 10] square degrees, mix 40% intersects + area, 20% relate, 20% dwithin
-    d ~ U(1, 50) m, 20% eroded intersects ra, rb ~ U(0.5, 3) m; served (`lanes` lanes) vs serial (one lane) on the
-    same arrival times, at `rate` and at `overload_rate`; latency = arrival -> finished."""
+    query windows of area lognormal in [1e-4, 10] square degrees, mix 35% intersects + area, 20% relate, 20% dwithin
+    d ~ U(1, 50) m, 20% eroded intersects ra, rb ~ U(0.5, 3) m, 5% cluster_intersecting of the window's parcels;
+    served (`lanes` lanes, batching) vs serial (one lane, one request at a time) on the same arrival times, at `rate`
+    and at `overload_rate`; latency = arrival -> finished."""
     f = args.national
     cell = 60.0 / NATIONAL_NX
     nx, ny = max(1, round(NATIONAL_NX * f ** 0.5)), max(1, round(NATIONAL_NY * f ** 0.5))
@@ -222,7 +223,7 @@ def national_leg(gsj, ctx, args):
     area = np.clip(np.exp(rng.normal(mu, sig, nreq)), 1e-4, 10.0)
     side = np.minimum(np.sqrt(area), min(W, H))
     cx, cy = x0 + rng.uniform(0, 1, nreq) * (W - side) + side / 2, y0 + rng.uniform(0, 1, nreq) * (H - side) + side / 2
-    kinds = rng.choice(4, size=nreq, p=[0.4, 0.2, 0.2, 0.2])
+    kinds = rng.choice(5, size=nreq, p=[0.35, 0.2, 0.2, 0.2, 0.05])
     dists, radii = rng.uniform(1.0, 50.0, nreq), rng.uniform(0.5, 3.0, (nreq, 2))
     reqs = []
     for i in range(nreq):
@@ -231,6 +232,8 @@ def national_leg(gsj, ctx, args):
             reqs.append((gsj.REQ_DWITHIN, dict(d=float(dists[i]), same=True, upper=True, window=win)))
         elif kinds[i] == 3:
             reqs.append((gsj.REQ_INTERSECTS_BUFFERED, dict(ra=float(radii[i, 0]), rb=float(radii[i, 1]), same=True, upper=True, window=win)))
+        elif kinds[i] == 4:
+            reqs.append((gsj.REQ_CLUSTER_ROWS, dict(window=win)))
         else:
             reqs.append((gsj.REQ_JOIN, dict(want_area=kinds[i] == 0, want_relate=kinds[i] == 1, upper=True, window=win)))
 
@@ -271,7 +274,8 @@ def national_leg(gsj, ctx, args):
         served, cv = run(args.lanes, arrivals)
         assert cs == cv, "served and serial request results differ"
         streams.append({"requests": nreq, "arrivals": f"Poisson, {rate:g}/s", "window_area_deg2": "lognormal in [1e-4, 10]",
-                        "mix": "40% intersects+area, 20% relate, 20% dwithin d~U(1,50) m, 20% eroded intersects ra,rb~U(0.5,3) m (self joins)",
+                        "mix": "35% intersects+area, 20% relate, 20% dwithin d~U(1,50) m, 20% eroded intersects ra,rb~U(0.5,3) m (self joins), "
+                               "5% cluster_intersecting of the window's parcels",
                         "serial": serial, "served": served,
                         "speedup_pairs_per_s": round(served["refined_pairs_per_s"] / serial["refined_pairs_per_s"], 3),
                         "p99_ratio_served_over_serial": round(served["p99_ms"] / serial["p99_ms"], 3)})

@@ -23,6 +23,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <unordered_map>
 #include <atomic>
 #include <cerrno>
 #include <chrono>
@@ -526,28 +527,65 @@ void op_join(Conn& c, const std::vector<std::string>& a) {
   logf("join %s x %s area=%d: %s%s", a[0].c_str(), a[1].c_str(), (int)want_area, stats.c_str(), c.broken ? " (client went away)" : "");
 }
 
-// CLUSTER (reference op_cluster, daemon/gpujoind.cu:290-322): dense cluster numbers in order of first row
+int cluster_rows_cb(void* u, const uint32_t* pairs, const double*, const uint32_t*, size_t n) {
+  auto* v = static_cast<std::vector<std::pair<uint32_t, uint32_t>>*>(u);
+  for (size_t i = 0; i < n; i++) v->push_back({pairs[2 * i], pairs[2 * i + 1]});
+  return 0;
+}
+
+// CLUSTER (reference op_cluster, daemon/gpujoind.cu:290-322): dense cluster numbers in order of first row. With the
+// trailing window=x0,y0,x1,y1 (an extension): the clusters of the layer's geometries whose box meets the window, rows
+// for their source rows only, numbered the same way.
 void op_cluster(Conn& c, const std::vector<std::string>& a) {
-  if (a.empty()) throw std::runtime_error("CLUSTER needs layer");
+  if (a.empty()) throw std::runtime_error("CLUSTER needs layer[, window=x0,y0,x1,y1]");
+  const Trailing tr = trailing(a, 1, false);
+  if (tr.stats) throw std::runtime_error("CLUSTER takes no stats option");
   auto L = get_layer(a[0]);
   if (!L) throw std::runtime_error("no such layer: " + a[0]);
   if (L->kind != "polygon") throw std::runtime_error("cluster needs a polygon layer");
   auto t0 = Clock::now();
-  std::vector<uint32_t> root(std::max<uint64_t>(L->n_distinct, 1));
   gsj_request r{};
-  r.kind = GSJ_REQ_CLUSTER; r.P = L->gl; r.root = root.data(); r.nroot = L->n_distinct;
-  const gsj_join_stats st = run_request(r, c);
-  std::vector<uint32_t> row_root(L->rows), dense(root.size(), UINT32_MAX), label(L->rows);
-  for (uint32_t gi = 0; gi < L->n_distinct; gi++) L->members(gi, [&](uint32_t rr) { row_root[rr] = root[gi]; });
+  std::vector<uint64_t> srows;     // source rows sent, ascending
+  std::vector<uint32_t> row_root;  // their roots
+  gsj_join_stats st{};
+  if (!tr.has_window) {
+    std::vector<uint32_t> root(std::max<uint64_t>(L->n_distinct, 1));
+    r.kind = GSJ_REQ_CLUSTER; r.P = L->gl; r.root = root.data(); r.nroot = L->n_distinct;
+    st = run_request(r, c);
+    row_root.resize(L->rows);
+    for (uint32_t gi = 0; gi < L->n_distinct; gi++) L->members(gi, [&](uint32_t rr) { row_root[rr] = root[gi]; });
+    srows.resize(L->rows);
+    for (uint64_t rr = 0; rr < L->rows; rr++) srows[rr] = rr;
+  } else {
+    std::vector<std::pair<uint32_t, uint32_t>> gr;   // (distinct geometry, root) of the window's geometries
+    r.kind = GSJ_REQ_CLUSTER_ROWS; r.P = L->gl; r.rows_cb = cluster_rows_cb; r.user = &gr;
+    tr.apply(r.opts);
+    st = run_request(r, c);
+    std::vector<std::pair<uint64_t, uint32_t>> rows;
+    for (const auto& [gi, rt] : gr) L->members(gi, [&](uint32_t rr) { rows.push_back({rr, rt}); });
+    std::sort(rows.begin(), rows.end());
+    for (const auto& [rr, rt] : rows) { srows.push_back(rr); row_root.push_back(rt); }
+  }
+  std::vector<uint32_t> label(srows.size());
   uint32_t k = 0;
-  for (uint64_t rr = 0; rr < L->rows; rr++) {
-    const uint32_t rt = row_root[rr];
-    if (dense[rt] == UINT32_MAX) dense[rt] = k++;
-    label[rr] = dense[rt];
+  if (!tr.has_window) {   // every geometry: a table over the roots
+    std::vector<uint32_t> dense(std::max<uint64_t>(L->n_distinct, 1), UINT32_MAX);
+    for (size_t i = 0; i < srows.size(); i++) {
+      if (dense[row_root[i]] == UINT32_MAX) dense[row_root[i]] = k++;
+      label[i] = dense[row_root[i]];
+    }
+  } else {                // a window's few geometries
+    std::unordered_map<uint32_t, uint32_t> dense;
+    for (size_t i = 0; i < srows.size(); i++) label[i] = dense.emplace(row_root[i], (uint32_t)dense.size()).first->second;
+    k = (uint32_t)dense.size();
   }
   c.u32(GSP_OK);
   c.streaming = true;
-  const uint64_t total = send_parallel(c, L->rows, [&](size_t i, FrameBuf& f) { f.id(L->ids.data(i), L->ids.len(i)); f.u32(label[i]); f.row_done(); });
+  const uint64_t total = send_parallel(c, srows.size(), [&](size_t i, FrameBuf& f) {
+    f.id(L->ids.data(srows[i]), L->ids.len(srows[i]));
+    f.u32(label[i]);
+    f.row_done();
+  });
   const std::string stats = Json().k("kind").v("cluster_intersecting").k("rows").u(total).k("clusters").u(k).k("candidates").u(st.candidates)
       .k("edges").u(st.hits).k("traverse_ms").n(st.traverse_ms).k("refine_ms").n(st.refine_ms).k("total_s").n(secs_since(t0)).done();
   if (!c.broken) end_stream(c, stats);

@@ -123,5 +123,5 @@ class Client:
     def dwithin(self, a, b, mode, dist_a, dist_b, self_pairs="all", extra=None, window=None):
         return self.call(OP_DWITHIN, a, b, mode, dist_a, dist_b, self_pairs, *([extra] if extra else []), *_window_arg(window))
 
-    def cluster(self, layer):
-        return self.call(OP_CLUSTER, layer)
+    def cluster(self, layer, window=None):
+        return self.call(OP_CLUSTER, layer, *_window_arg(window))

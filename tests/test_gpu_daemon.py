@@ -189,6 +189,26 @@ def test_points_and_cluster(ctx, daemon):
     assert rows == exp and st["clusters"] == len(set(dense.values()))
 
 
+def test_windowed_cluster(ctx, daemon):
+    """CLUSTER with the trailing window: the source rows of the geometries meeting the window, numbered densely in row
+    order from the engine's windowed components"""
+    c = daemon["client"]
+    B = Side(ctx, daemon["sets"]["b"], "b", 0)
+    bb = B.layer.export()["bb"].astype(np.float64)
+    x0, y0, x1, y1 = bb[:, 0].min(), bb[:, 1].min(), bb[:, 2].max(), bb[:, 3].max()
+    win = (x0 + 0.3 * (x1 - x0), y0 + 0.2 * (y1 - y0), x0 + 0.7 * (x1 - x0), y0 + 0.6 * (y1 - y0))
+    lab = {}
+    for g, rt in ctx.cluster_rows(B.layer, window=win).pairs.tolist():
+        for m in B.members(g):
+            lab[m] = rt
+    dense, exp = {}, []
+    for rr in sorted(lab):
+        dense.setdefault(lab[rr], len(dense))
+        exp.append((B.ids[rr], dense[lab[rr]]))
+    rows, st, _ = c.cluster("b", window=win)
+    assert rows == exp and 0 < len(rows) < len(B.ids) and st["clusters"] == len(dense)
+
+
 def test_dwithin_modes(ctx, daemon):
     c = daemon["client"]
     A = Side(ctx, daemon["sets"]["a"], "a", 8)
