@@ -121,7 +121,7 @@ def serving_leg(gsj, ctx, args):
         del P
     log(f"serving mix: {n} requests, {int(sizes.sum())} polygons, generated + resident in {time.time() - t0:.1f}s")
 
-    def run(lanes, arrivals):
+    def run(lanes):
         s = gsj.Scheduler(ctx, lanes=lanes, batch=1 if lanes == 1 else 16)   # serial: one request at a time
         try:
             for k, A, B, kw in reqs:   # warm-up: every lane's scratch grows to its working size
