@@ -100,6 +100,16 @@ struct StreamTimer {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// MI355X dispatches the workgroups of a grid round-robin over its 8 XCDs, each with its own L2. This renumbers them
+// (a bijection on [0, gridDim.x)) so that each XCD runs one contiguous range of logical blocks: neighbouring work
+// items, which read the same polygons and tree nodes, then share an L2. For work of even cost per item (the tree
+// walk); the refinement's class lists are sorted by size, where contiguous ranges per XCD would unbalance the XCDs.
+__device__ __forceinline__ uint32_t xcd_block() {
+  constexpr uint32_t NX = 8;
+  const uint32_t b = blockIdx.x, x = b % NX, q = gridDim.x / NX, r = gridDim.x % NX;
+  return x * q + (x < r ? x : r) + b / NX;
+}
+
 // kernel launch check (launch errors surface on the next runtime call otherwise)
 #define GSJ_LAUNCHED() GSJ_HIP(hipGetLastError())
 

@@ -91,10 +91,10 @@ struct Context {
   // scratch, grow-only, reused across calls
   DevBuf s_counts, s_qoffs, s_offs, s_pairs_a, s_pairs_b, s_flag, s_area, s_ev, s_evb, s_tmp, s_tmp2, s_cls, s_out_a, s_out_b, s_out_area,
       s_out_code, s_out_flag, s_keys, s_keys2, s_vals, s_vals2, s_seg, s_heavy, s_heavy_edges, s_scal, s_qbb, s_uf, s_root, s_misc, s_win, s_qidx,
-      s_win_off, s_edgeq, s_qlist, s_wtile, s_wtoff, s_wseg;
+      s_win_off, s_edgeq, s_qlist, s_wtile, s_wtoff, s_wseg, s_slots, s_slots_ctl;
   // erosion pass 2 (it runs inside the join's on-device hook, beside the join's own scratch)
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
-  PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg;
+  PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg, h_slots_ctl;
   int cu_count = 256;
   hipEvent_t ev[8] = {};   // kernel timing events (created with the context)
   ~Context();
@@ -124,9 +124,16 @@ void tile_boxes(Context& c, const float4* bb, const uint32_t* order, uint32_t n,
 
 // ------------------------------------------------------------------------------------------- traversal
 // count pass (write == false) or write pass over probe boxes qbb[qbase .. qbase+nq)
-// (box_by_pos: the box of position p is qbb[p] instead of the box of its probe)
-void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
-              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b, bool box_by_pos = false);
+// Candidates of the probes at positions qbase .. qbase+nq (probe id qidx[pos], or pos): traverse_count walks the tree
+// once per probe, counting into counts[] and keeping the first candidates in slots[] (probes with more are counted in
+// *nover); after the scan of the counts into offs, traverse_write writes every candidate at its offset (SoA), walking
+// again only for the probes whose candidates did not fit (any_over). box_by_pos: the search box of position p is
+// qbb[p] instead of the box of its probe.
+void traverse_count(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
+                    uint32_t* slots, uint32_t* nover, bool box_by_pos);
+void traverse_write(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper,
+                    const uint32_t* counts, const uint64_t* offs, const uint32_t* slots, bool any_over, uint32_t* out_a, uint32_t* out_b,
+                    bool box_by_pos);
 
 // The probe side of a candidate search: traversal boxes qbb[0 .. n) (probe index = box index), the probes' own boxes
 // (the window test; differ from qbb when the search box is widened), and the parent structure of a subdivided layer

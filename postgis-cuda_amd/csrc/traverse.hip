@@ -4,20 +4,27 @@
 // Stackless: nodes are in depth-first preorder, so a lane walks forward through the node array, descending into
 // the next node on overlap and jumping to the node's skip link otherwise. No per-lane stack (which would live in
 // scratch memory on CDNA), one 16-byte + one 8-byte load per visited node, monotonically increasing addresses.
-// Two passes with the same walk: count, then write at the scanned offsets (SoA pair buffers).
+// One walk per probe in the common case: it counts the probe's candidates and keeps the first SLOT_K of them in the
+// probe's slots; after the scan, a gather writes them at their offsets (SoA pair buffers), and only the probes with
+// more candidates than slots walk again, writing at their offsets.
 #include "engine.h"
 
 namespace gsj {
 
 namespace {
 
+constexpr uint32_t SLOT_K = 16;   // candidates kept per probe by the counting walk
+
+// WRITE false: count the probe's candidates into counts[t] and keep the first SLOT_K in slots[t * SLOT_K ..]
+// (probes with more are counted in *nover); WRITE true: the probes with counts[t] > SLOT_K write all theirs at offs[t]
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_traverse(const float4* __restrict__ qbb, const uint32_t* __restrict__ qidx, uint32_t qbase, uint32_t nq,
                                                   const BNode* __restrict__ nodes, uint32_t nnodes, int upper, uint32_t* __restrict__ counts,
                                                   const uint64_t* __restrict__ offs, uint32_t* __restrict__ out_a, uint32_t* __restrict__ out_b,
-                                                  int box_by_pos) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                  int box_by_pos, uint32_t* __restrict__ slots, uint32_t* __restrict__ nover) {
+  const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
+  if (WRITE && counts[t] <= SLOT_K) return;
   const uint32_t q = qidx ? qidx[qbase + t] : qbase + t;
   const float4 b = qbb[box_by_pos ? qbase + t : q];
   uint64_t o = WRITE ? offs[t] : 0;
@@ -36,6 +43,7 @@ __global__ void __launch_bounds__(256) k_traverse(const float4* __restrict__ qbb
       else {
         if (!upper || (uint32_t)prim >= q) {
           if (WRITE) { out_a[o + cnt] = q; out_b[o + cnt] = (uint32_t)prim; }
+          else if (cnt < SLOT_K) slots[(uint64_t)t * SLOT_K + cnt] = (uint32_t)prim;
           cnt++;
         }
         next = skip;
@@ -44,7 +52,31 @@ __global__ void __launch_bounds__(256) k_traverse(const float4* __restrict__ qbb
     if (next <= i) break;
     i = next;
   }
-  if (!WRITE) counts[t] = cnt;
+  if (!WRITE) {
+    counts[t] = cnt;
+    if (cnt > SLOT_K) atomicAdd(nover, 1u);
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// the kept candidates of the probes with at most SLOT_K, at their offsets
+__global__ void k_slot_gather(const uint32_t* __restrict__ qidx, uint32_t qbase, uint32_t nq, const uint32_t* __restrict__ counts,
+                              const uint64_t* __restrict__ offs, const uint32_t* __restrict__ slots, uint32_t* __restrict__ out_a,
+                              uint32_t* __restrict__ out_b) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const uint32_t n = counts[t];
+  if (n == 0 || n > SLOT_K) return;
+  const uint32_t q = qidx ? qidx[qbase + t] : qbase + t;
+  const uint64_t o = offs[t];
+  for (uint32_t j = 0; j < n; j++) {
+    out_a[o + j] = q;
+    out_b[o + j] = slots[(uint64_t)t * SLOT_K + j];
+  }
 }
 
 // window membership of the probe at every walk position: its own (outward float) box meets the closed window,
@@ -192,20 +224,30 @@ Probes probes_of(const Layer& L, const float4* qbb) {
   return p;
 }
 
-void traverse(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
-              const uint64_t* offs, uint32_t* out_a, uint32_t* out_b, bool box_by_pos) {
+void traverse_count(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper, uint32_t* counts,
+                    uint32_t* slots, uint32_t* nover, bool box_by_pos) {
   if (nq == 0) return;
   if (t.nnodes == 0) {
-    if (!offs) GSJ_HIP(hipMemsetAsync(counts, 0, (size_t)nq * 4, c.stream));
+    hipLaunchKernelGGL(k_zero_u32, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, counts, (uint64_t)nq);
+    GSJ_LAUNCHED();
     return;
   }
-  if (!offs)
-    hipLaunchKernelGGL(k_traverse<false>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
-                       counts, nullptr, nullptr, nullptr, box_by_pos ? 1 : 0);
-  else
-    hipLaunchKernelGGL(k_traverse<true>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
-                       nullptr, offs, out_a, out_b, box_by_pos ? 1 : 0);
+  hipLaunchKernelGGL(k_traverse<false>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0, counts,
+                     nullptr, nullptr, nullptr, box_by_pos ? 1 : 0, slots, nover);
   GSJ_LAUNCHED();
+}
+
+void traverse_write(Context& c, const float4* qbb, const uint32_t* qidx, uint32_t qbase, uint32_t nq, const DBvh& t, bool upper,
+                    const uint32_t* counts, const uint64_t* offs, const uint32_t* slots, bool any_over, uint32_t* out_a, uint32_t* out_b,
+                    bool box_by_pos) {
+  if (nq == 0 || t.nnodes == 0) return;
+  hipLaunchKernelGGL(k_slot_gather, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qidx, qbase, nq, counts, offs, slots, out_a, out_b);
+  GSJ_LAUNCHED();
+  if (any_over) {
+    hipLaunchKernelGGL(k_traverse<true>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
+                       const_cast<uint32_t*>(counts), offs, out_a, out_b, box_by_pos ? 1 : 0, nullptr, nullptr);
+    GSJ_LAUNCHED();
+  }
 }
 
 namespace {
@@ -232,9 +274,16 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
     const uint32_t nq = qn - q0;
     WallTimer tt;
     uint32_t* counts = c.s_counts.as<uint32_t>(nq + 16);
-    traverse(c, qbb, qidx, q0, nq, t, upper, counts, nullptr, nullptr, nullptr, box_by_pos);
+    uint32_t* slots = c.s_slots.as<uint32_t>((uint64_t)nq * SLOT_K + 16);
+    uint32_t* nover = c.s_slots_ctl.as<uint32_t>(16);
+    uint32_t* hover = c.h_slots_ctl.as<uint32_t>(4);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, nover, (uint64_t)1);
+    GSJ_LAUNCHED();
+    traverse_count(c, qbb, qidx, q0, nq, t, upper, counts, slots, nover, box_by_pos);
+    GSJ_HIP(hipMemcpyAsync(hover, nover, 4, hipMemcpyDeviceToHost, stream));
     uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);   // probe offsets (s_offs belongs to compaction)
-    const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);
+    const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);   // synchronises: *hover is there too
+    const bool any_over = *hover != 0;
     s.traverse_ms += tt.ms();
     // sub-batches of at most max_pairs candidates (a probe parent's pieces stay together)
     std::vector<std::pair<uint32_t, uint32_t>> subs;
@@ -263,7 +312,7 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
       GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a sub-batch has >= 2^32 candidates (lower max_pairs)");
       uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
       uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
-      traverse(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, nullptr, offs, pa, pb, box_by_pos);
+      traverse_write(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, counts + s0, offs, slots + (uint64_t)s0 * SLOT_K, any_over, pa, pb, box_by_pos);
       SegBounds sb;
       if (mstart) {
         hipLaunchKernelGGL(k_seg_bounds, dim3(1), dim3(64), 0, stream, mstart, k, (uint64_t)(q0 + s0), (uint64_t)(s1 - s0), offs, np, dlo);
