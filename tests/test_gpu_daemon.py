@@ -2,6 +2,7 @@
 protocol. Every response is checked against the library's direct calls on the same distinct, subdivided layer, with
 the reference's row semantics restated here: dedup member expansion, self-join mirroring ("all") or ordering
 ("distinct"), transposed DE-9IM for mirrored rows, point/polygon swap, dense cluster labels."""
+import json
 import math
 import os
 import subprocess
@@ -207,6 +208,40 @@ def test_windowed_cluster(ctx, daemon):
         exp.append((B.ids[rr], dense[lab[rr]]))
     rows, st, _ = c.cluster("b", window=win)
     assert rows == exp and 0 < len(rows) < len(B.ids) and st["clusters"] == len(dense)
+
+
+GSP_CAT = os.path.join(ROOT, "oracle", "_ref", "gsp_cat")
+
+
+@pytest.mark.skipif(not os.path.exists(GSP_CAT), reason="the reference's client is built only where the reference tree is present")
+def test_reference_client_reads_the_streams(daemon):
+    """tools/gsp_cat.c of the reference, compiled unchanged from its source, drains this daemon's JOIN / CLUSTER /
+    DWITHIN streams (frames, u16-length ids, f64 / u32 slots, end marker, stats JSON): its row count equals the rows
+    this repository's client parses from the same request, and its errors come back as the reference's error reply."""
+    c = daemon["client"]
+    env = dict(os.environ, GSP_TOKEN=TOKEN)
+    bb = daemon["sets"]["a"].xy
+    w = (bb[:, 0].min() + 0.002, bb[:, 1].min() + 0.002, bb[:, 0].min() + 0.009, bb[:, 1].min() + 0.008)
+    wa = "window=" + ",".join(repr(float(v)) for v in w)
+    cases = [
+        (G.OP_JOIN, ["a", "a", "1", "all"], lambda: c.join("a", "a", want_area=True)),
+        (G.OP_JOIN, ["a", "b", "0", "distinct"], lambda: c.join("a", "b", self_pairs="distinct")),
+        (G.OP_JOIN, ["p", "b", "0", "all"], lambda: c.join("p", "b")),
+        (G.OP_JOIN, ["a", "a", "1", "all", wa], lambda: c.join("a", "a", want_area=True, window=w)),
+        (G.OP_CLUSTER, ["b"], lambda: c.cluster("b")),
+        (G.OP_CLUSTER, ["b", wa], lambda: c.cluster("b", window=w)),
+        (G.OP_DWITHIN, ["a", "a", "dwithin", "4", "0", "all", "distance"], lambda: c.dwithin("a", "a", "dwithin", 4, 0, "all", extra="distance")),
+        (G.OP_DWITHIN, ["a", "a", "buffer", "-1", "-1.5", "distinct"], lambda: c.dwithin("a", "a", "buffer", -1, -1.5, "distinct")),
+    ]
+    for op, args, ours in cases:
+        r = subprocess.run([GSP_CAT, daemon["sock"], str(op), *args], env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (args, r.stderr)
+        n = int(r.stdout.split("rows=")[1].split()[0])
+        rows, st, _ = ours()
+        assert n == len(rows) and n == st["rows"], (args, n, len(rows))
+        assert json.loads(r.stdout.splitlines()[1])["rows"] == n
+    r = subprocess.run([GSP_CAT, daemon["sock"], str(G.OP_JOIN), "a", "zz", "0", "all"], env=env, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no such layer" in r.stderr
 
 
 def test_dwithin_modes(ctx, daemon):
