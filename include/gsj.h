@@ -278,6 +278,11 @@ int gsj_subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, ui
  * member lists moff[un+1], midx[n] (caller-allocated, n entries suffice for both) */
 int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct, uint64_t* moff,
               uint32_t* midx, uint64_t* n_distinct);
+/* the same dedup on ctx's device (hash, stable radix sorts, group compares and compaction on the GPU; host arrays in
+ * and out): identical distinct set, member lists and order. Replaces dedup_set (engine/dedup.h:51-124) on the load
+ * path, the step the reference runs on host cores. */
+int gsj_dedup_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct,
+                     uint64_t* moff, uint32_t* midx, uint64_t* n_distinct);
 
 /* Deterministic synthetic layers (counter-based RNG, identical on every run):
  *   kind "parcels_grid": config-1 parcels (one 4..12-vertex star polygon per cell of an nx x ny grid)

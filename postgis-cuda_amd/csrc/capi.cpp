@@ -405,6 +405,23 @@ int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64
   });
 }
 
+int gsj_dedup_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct, uint64_t* moff,
+                     uint32_t* midx, uint64_t* n_distinct) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && gro && rvo && distinct && moff && midx && n_distinct, GSJ_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    std::vector<uint64_t> mo;
+    std::vector<uint32_t> mi;
+    auto* h = new gsj_hostset();
+    try { h->s = gsj::dedup_device(ctx->c, gro, rvo, xy, n, mo, mi); } catch (...) { delete h; throw; }
+    std::memcpy(moff, mo.data(), mo.size() * 8);
+    std::memcpy(midx, mi.data(), mi.size() * 4);
+    *n_distinct = h->s.n();
+    *distinct = h;
+  });
+}
+
 int gsj_generate(const char* kind, const char* spec, gsj_hostset** out) {
   return guarded([&] {
     GSJ_REQUIRE(kind && out, GSJ_E_ARG, "null argument");

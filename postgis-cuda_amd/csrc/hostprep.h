@@ -25,6 +25,10 @@ void write_pset(const std::string& path, const double* xy, uint64_t n);
 
 HostSet subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv);
 HostSet dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, std::vector<uint64_t>& moff, std::vector<uint32_t>& midx);
+// the same on the device of c (prep.hip): identical distinct set, member lists and order
+struct Context;
+HostSet dedup_device(Context& c, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, std::vector<uint64_t>& moff,
+                     std::vector<uint32_t>& midx);
 
 HostSet generate(const std::string& kind, const char* spec);
 std::vector<double> generate_points(const char* spec);

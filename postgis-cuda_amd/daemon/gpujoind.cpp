@@ -218,7 +218,7 @@ std::string build_layer(const std::string& name, const std::string& source, int 
       L->moff.assign(n + 1, 0);
       L->midx.assign(n, 0);
       uint64_t nd = 0;
-      check(gsj_dedup(gro, rvo, xy, n, &distinct.h, L->moff.data(), L->midx.data(), &nd), "dedup");
+      check(gsj_dedup_device(g_ctx, gro, rvo, xy, n, &distinct.h, L->moff.data(), L->midx.data(), &nd), "dedup");   // on the GPU
       L->moff.resize(nd + 1);
       L->deduped = true;
       check(gsj_hostset_arrays(distinct.h, &gro, &rvo, &xy, &parent, &syn), "layer arrays");

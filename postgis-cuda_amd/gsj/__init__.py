@@ -82,7 +82,7 @@ EXPORTS = [
     "gsj_points_intersects_buffered",
     "gsj_sched_create", "gsj_sched_destroy", "gsj_sched_set_batch", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
-    "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_generate", "gsj_generate_points",
+    "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_dedup_device", "gsj_generate", "gsj_generate_points",
 ]
 
 
@@ -122,6 +122,7 @@ def load(path: str | None = None):
         "gsj_pset_read": (i32, [C.c_char_p, C.POINTER(dp), u64p]), "gsj_pset_write": (i32, [C.c_char_p, dp, u64]),
         "gsj_free": (None, [vp]), "gsj_subdivide": (i32, [u64p, u64p, dp, u64, i32, C.POINTER(vp)]),
         "gsj_dedup": (i32, [u64p, u64p, dp, u64, C.POINTER(vp), u64p, u32p, u64p]),
+        "gsj_dedup_device": (i32, [vp, u64p, u64p, dp, u64, C.POINTER(vp), u64p, u32p, u64p]),
         "gsj_generate": (i32, [C.c_char_p, C.c_char_p, C.POINTER(vp)]), "gsj_generate_points": (i32, [C.c_char_p, C.POINTER(dp), u64p]),
     }
     for name, (res, args) in sig.items():
@@ -270,16 +271,17 @@ def subdivide(hs: HostSet, maxv: int) -> HostSet:
     return _take_hostset(h)
 
 
-def dedup(hs: HostSet):
-    """(distinct set, moff[un+1], midx[n])"""
+def dedup(hs: HostSet, ctx: "Context" = None):
+    """(distinct set, moff[un+1], midx[n]); with ctx, computed on its device (the same result)"""
     lib = load()
     hs = hs.normalized()
     h = C.c_void_p()
     moff = np.zeros(hs.n + 1, np.uint64)
     midx = np.zeros(max(hs.n, 1), np.uint32)
     un = C.c_uint64()
-    _check(lib.gsj_dedup(_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, C.byref(h),
-                         _ptr(moff, C.c_uint64), _ptr(midx, C.c_uint32), C.byref(un)))
+    args = (_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, C.byref(h), _ptr(moff, C.c_uint64),
+            _ptr(midx, C.c_uint32), C.byref(un))
+    _check(lib.gsj_dedup_device(ctx.h, *args) if ctx is not None else lib.gsj_dedup(*args))
     return _take_hostset(h), moff[: un.value + 1].copy(), midx[: hs.n].copy()
 
 
