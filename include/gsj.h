@@ -278,6 +278,9 @@ int gsj_subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, ui
  * member lists moff[un+1], midx[n] (caller-allocated, n entries suffice for both) */
 int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct, uint64_t* moff,
               uint32_t* midx, uint64_t* n_distinct);
+/* the same subdivision on ctx's device (level by level: a thread per piece decides and a thread per ring and half
+ * clips; host arrays in and out): identical pieces, cut flags, parents and order. */
+int gsj_subdivide_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv, gsj_hostset** out);
 /* the same dedup on ctx's device (hash, stable radix sorts, group compares and compaction on the GPU; host arrays in
  * and out): identical distinct set, member lists and order. Replaces dedup_set (engine/dedup.h:51-124) on the load
  * path, the step the reference runs on host cores. */

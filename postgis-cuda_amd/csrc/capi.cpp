@@ -405,6 +405,17 @@ int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64
   });
 }
 
+int gsj_subdivide_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv, gsj_hostset** out) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && gro && rvo && out, GSJ_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    auto* h = new gsj_hostset();
+    try { h->s = gsj::subdivide_device(ctx->c, gro, rvo, xy, n, maxv); } catch (...) { delete h; throw; }
+    *out = h;
+  });
+}
+
 int gsj_dedup_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct, uint64_t* moff,
                      uint32_t* midx, uint64_t* n_distinct) {
   return guarded([&] {

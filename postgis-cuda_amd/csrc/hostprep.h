@@ -29,6 +29,8 @@ HostSet dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64
 struct Context;
 HostSet dedup_device(Context& c, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, std::vector<uint64_t>& moff,
                      std::vector<uint32_t>& midx);
+// subdivide on the device of c (prep.hip): the same pieces, bit for bit, in the same order
+HostSet subdivide_device(Context& c, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv);
 
 HostSet generate(const std::string& kind, const char* spec);
 std::vector<double> generate_points(const char* spec);

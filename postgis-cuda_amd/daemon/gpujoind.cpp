@@ -226,7 +226,7 @@ std::string build_layer(const std::string& name, const std::string& source, int 
     }
     L->n_distinct = n;
     if (subdiv > 0) {
-      check(gsj_subdivide(gro, rvo, xy, n, subdiv, &pieces.h), "subdivide");
+      check(gsj_subdivide_device(g_ctx, gro, rvo, xy, n, subdiv, &pieces.h), "subdivide");   // on the GPU
       check(gsj_hostset_arrays(pieces.h, &gro, &rvo, &xy, &parent, &syn), "layer arrays");
       check(gsj_hostset_sizes(pieces.h, &n, &nr, &nv), "layer sizes");
     }

@@ -82,7 +82,7 @@ EXPORTS = [
     "gsj_points_intersects_buffered",
     "gsj_sched_create", "gsj_sched_destroy", "gsj_sched_set_batch", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
-    "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_dedup_device", "gsj_generate", "gsj_generate_points",
+    "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_dedup_device", "gsj_subdivide_device", "gsj_generate", "gsj_generate_points",
 ]
 
 
@@ -123,6 +123,7 @@ def load(path: str | None = None):
         "gsj_free": (None, [vp]), "gsj_subdivide": (i32, [u64p, u64p, dp, u64, i32, C.POINTER(vp)]),
         "gsj_dedup": (i32, [u64p, u64p, dp, u64, C.POINTER(vp), u64p, u32p, u64p]),
         "gsj_dedup_device": (i32, [vp, u64p, u64p, dp, u64, C.POINTER(vp), u64p, u32p, u64p]),
+        "gsj_subdivide_device": (i32, [vp, u64p, u64p, dp, u64, i32, C.POINTER(vp)]),
         "gsj_generate": (i32, [C.c_char_p, C.c_char_p, C.POINTER(vp)]), "gsj_generate_points": (i32, [C.c_char_p, C.POINTER(dp), u64p]),
     }
     for name, (res, args) in sig.items():
@@ -263,11 +264,13 @@ def generate_points(**spec) -> np.ndarray:
     return np.asarray(_View(_NativeBuffer(p), p, (2 * n.value,), "<f8")).reshape(-1, 2)
 
 
-def subdivide(hs: HostSet, maxv: int) -> HostSet:
+def subdivide(hs: HostSet, maxv: int, ctx: "Context" = None) -> HostSet:
+    """pieces of at most maxv stored vertices; with ctx, computed on its device (the same result)"""
     lib = load()
     hs = hs.normalized()
     h = C.c_void_p()
-    _check(lib.gsj_subdivide(_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, maxv, C.byref(h)))
+    args = (_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, maxv, C.byref(h))
+    _check(lib.gsj_subdivide_device(ctx.h, *args) if ctx is not None else lib.gsj_subdivide(*args))
     return _take_hostset(h)
 
 
