@@ -219,6 +219,8 @@ def test_batched_windowed_dwithin_equal_their_own_calls(ctx):
         w = (x0 + f[0] * W, y0 + f[1] * H, x0 + f[0] * W + s * W, y0 + f[1] * H + s * H)
         d = 0.0 if i % 10 == 0 else float(rng.uniform(1.0, 40.0))
         reqs.append(dict(d=d, same=True, upper=True, report_distance=i < 30, window=w))
+        if 20 <= i < 30:   # probe batches and candidate sub-batches that cut through the batched requests
+            reqs[-1].update(batch_q=9, max_pairs=50)
     reqs[5]["window"] = (x0 - 1, y0 - 1, x0 + W + 1, y0 + H + 1)
     reqs[6]["window"] = (x0 + W + 1, y0 + H + 1, x0 + W + 2, y0 + H + 2)
     s = gsj.Scheduler(ctx, lanes=1, batch=16)
@@ -261,6 +263,8 @@ def test_batched_windowed_erosion_equal_their_own_calls(ctx):
         elif i % 9 == 1:
             rb = 0.0             # one side eroded
         reqs.append(dict(ra=ra, rb=rb, same=True, upper=True, window=w))
+        if 24 <= i < 34:   # probe batches and candidate sub-batches that cut through the batched requests
+            reqs[-1].update(batch_q=7, max_pairs=40)
     reqs[4]["window"] = (x0 - 1, y0 - 1, x0 + W + 1, y0 + H + 1)
     reqs[7]["window"] = (x0 + W + 1, y0 + H + 1, x0 + W + 2, y0 + H + 2)
     s = gsj.Scheduler(ctx, lanes=1, batch=16)
