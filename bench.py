@@ -295,12 +295,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline and parity legs (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="host work of the cpu_baseline leg (repeats of the join)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-vs-serial leg")
+    ap.add_argument("--no-prep", action="store_true", help="skip the host-vs-device layer preparation timings (profiling runs)")
     ap.add_argument("--requests", type=int, default=256, help="requests in the served-vs-serial mix")
     ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
     ap.add_argument("--lanes", type=int, default=4, help="lanes of the served path")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run 1 GiB device copies (HBM counter calibration)")
     ap.add_argument("--national", type=float, default=0.0,
                     help="run only the resident national-scale leg at this scale (1.0 = 162M parcels + 161M points)")
+    ap.add_argument("--national-scale", type=float, default=1.0, help="scale of the national leg that the default run appends")
+    ap.add_argument("--no-national", action="store_true", help="skip the national leg of the default run")
     ap.add_argument("--stream-requests", type=int, default=1000, help="requests in the national windowed stream")
     ap.add_argument("--rate", type=float, default=500.0, help="Poisson arrival rate (requests/s) of the national stream")
     ap.add_argument("--overload-rate", type=float, default=5000.0, help="second arrival rate of the national stream, above both paths' capacity")
@@ -332,7 +335,7 @@ def main():
         copy_calibration()
     # layer preparation of the same 2.4M parcels: host restatement vs device (identical results, checked)
     prep = {}
-    for name, fn in (("subdivide_host", lambda: gsj.subdivide(M, 256)), ("subdivide_device", lambda: gsj.subdivide(M, 256, ctx=ctx)),
+    for name, fn in () if args.no_prep else (("subdivide_host", lambda: gsj.subdivide(M, 256)), ("subdivide_device", lambda: gsj.subdivide(M, 256, ctx=ctx)),
                      ("dedup_host", lambda: gsj.dedup(M)), ("dedup_device", lambda: gsj.dedup(M, ctx=ctx))):
         fn()
         t0 = time.perf_counter()
@@ -435,6 +438,12 @@ def main():
     serving = None if args.no_serve else serving_leg(gsj, ctx, args)
     if serving:
         log(f"serving: serial {serving['serial']}; served {serving['served']}; speedup {serving['speedup_pairs_per_s']}")
+    national = None
+    if not (args.no_serve or args.no_national):   # the serving configuration: resident national layers, windowed stream
+        L.free()
+        ctx.trim()
+        args.national = args.national_scale
+        national = national_leg(gsj, ctx, args)
     lat = np.array(times) * 1e3
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "refined pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -446,7 +455,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
         "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1), "prep": prep},
-        "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving,
+        "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving, "national": national,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

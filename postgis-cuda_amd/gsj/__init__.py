@@ -414,6 +414,10 @@ class Context:
     def status(self) -> str:
         return load().gsj_ctx_status_json(self.h).decode()
 
+    def trim(self):
+        """free the context's grow-only scratch (it grows again on the next call)"""
+        _check(load().gsj_ctx_trim(self.h))
+
     def polys(self, hs: HostSet, build_bvh=True) -> Layer:
         hs = hs.normalized()
         h = C.c_void_p()
