@@ -283,3 +283,55 @@ def test_batched_windowed_erosion_equal_their_own_calls(ctx):
         assert np.array_equal(g.pairs, d.pairs)
         assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
     assert len(got[4].pairs) > 0 and len(got[7].pairs) == 0
+
+
+def test_windowed_mix_over_lanes_with_batching(ctx):
+    """Windowed joins, relates, distance joins, eroded intersects and clusters of one resident layer, submitted at once
+    to three lanes that batch: every request's rows (in order) and counts equal its own call, whatever lane and batch
+    it ran in."""
+    M = gsj.generate("parcels_mesh", nx=70, ny=70, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=16)
+    L = ctx.polys(M)
+    x0, y0 = M.xy[:, 0].min(), M.xy[:, 1].min()
+    W, H = M.xy[:, 0].max() - x0, M.xy[:, 1].max() - y0
+    rng = np.random.default_rng(17)
+    reqs = []
+    for i in range(60):
+        f = rng.uniform(0, 1, 2)
+        s = rng.uniform(0.02, 0.3)
+        w = (x0 + f[0] * W, y0 + f[1] * H, x0 + f[0] * W + s * W, y0 + f[1] * H + s * H)
+        k = i % 5
+        if k == 0:
+            reqs.append((gsj.REQ_JOIN, dict(want_area=True, upper=True, window=w)))
+        elif k == 1:
+            reqs.append((gsj.REQ_JOIN, dict(want_relate=True, upper=True, window=w)))
+        elif k == 2:
+            reqs.append((gsj.REQ_DWITHIN, dict(d=float(rng.uniform(1, 30)), same=True, upper=True, report_distance=True, window=w)))
+        elif k == 3:
+            reqs.append((gsj.REQ_INTERSECTS_BUFFERED, dict(ra=float(rng.uniform(0.5, 3)), rb=float(rng.uniform(0.5, 3)), same=True, upper=True,
+                                                           window=w)))
+        else:
+            reqs.append((gsj.REQ_CLUSTER_ROWS, dict(window=w)))
+    s = gsj.Scheduler(ctx, lanes=3, batch=8)
+    try:
+        tickets = [s.submit(k, L, L, **kw) for k, kw in reqs]
+        got = [s.wait(t) for t in tickets]
+    finally:
+        s.close()
+    assert sum(g.stats["times"]["batch"] > 1 for g in got) > 0
+    for (k, kw), g in zip(reqs, got):
+        kw = dict(kw)
+        if k == gsj.REQ_JOIN:
+            d = ctx.join(L, L, **kw)
+            _rows_in_order(g, d)
+        elif k == gsj.REQ_DWITHIN:
+            d = ctx.dwithin(L, L, kw.pop("d"), **kw)
+            assert np.array_equal(g.pairs, d.pairs)
+            if len(d.pairs):
+                assert np.array_equal(g.dist, d.dist)
+        elif k == gsj.REQ_INTERSECTS_BUFFERED:
+            d = ctx.intersects_buffered(L, L, kw.pop("ra"), kw.pop("rb"), **kw)
+            assert np.array_equal(g.pairs, d.pairs)
+        else:
+            d = ctx.cluster_rows(L, **kw)
+            assert np.array_equal(g.pairs, d.pairs)
+        assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
