@@ -114,8 +114,9 @@ __global__ void k_refit(int n, const int32_t* left, const int32_t* right, const 
     const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
     const float4 a = box[left[p]], b = box[right[p]];
+    // no fence here: the next level's acquire-release increment publishes this store to the thread that merges
+    // there (an agent-scope fence costs an L2 write-back per XCD; it took the 2.4M-leaf refit from 6 to 11 ms)
     box[p] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
-    __threadfence();
     node = p;
   }
 }
