@@ -833,7 +833,10 @@ __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __r
   const uint32_t cls = (rings && va <= VCAP && vb <= VCAP) ? 0u
                        : (rings && va <= mcap && vb <= mcap) ? 1u
                        : (mixed && min(va, vb) <= (uint32_t)VCAP) ? 2u : 3u;
-  key[i] = (uint64_t)((cls << 6) | min(63u, (va + vb) >> 1));
+  // within a class, pairs of similar size side by side (balanced groups), at a resolution per class: the large class
+  // spans hundreds of vertices, so its buckets are 8 vertices wide (2 for small and mixed, 4 for medium)
+  const uint32_t shift = cls == 3 ? 3u : (cls == 1 ? 2u : 1u);
+  key[i] = (uint64_t)((cls << 6) | min(63u, (va + vb) >> shift));
   val[i] = (uint32_t)i;
 }
 
