@@ -771,8 +771,11 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[7], st));
-  constexpr int CAP_LARGE = 128;
-  {   // large: one wave per pair, 4 pairs per block
+  // large: one wave per pair, 4 pairs per block, 96-edge lists (3 blocks per CU fit in LDS; 128 allowed 2); the pairs
+  // that need more go to the overflow kernel (2.4M relate: large 6.1 -> 3.6 ms, overflow 2.7 -> 4.3 ms; a second LDS
+  // tier of 128-edge lists for those was slower than the overflow kernel)
+  constexpr int CAP_LARGE = 96;
+  {
     const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), max_blocks);
     hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, CAP_LARGE, ovf, ctl);
