@@ -288,6 +288,43 @@ def test_concurrent_clients_get_their_own_rows(daemon):
         _same_rows(rows, alone[i % len(reqs)])
 
 
+def test_concurrent_windowed_requests(daemon):
+    """windowed requests of one unsubdivided layer from many clients at once (the daemon's lanes batch them): each
+    client gets the rows of the same request sent alone"""
+    c = daemon["client"]
+    xy = daemon["sets"]["b"].xy
+    x0, y0, x1, y1 = xy[:, 0].min(), xy[:, 1].min(), xy[:, 0].max(), xy[:, 1].max()
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(24):
+        f = rng.uniform(0, 0.7, 2)
+        w = (x0 + f[0] * (x1 - x0), y0 + f[1] * (y1 - y0), x0 + (f[0] + 0.3) * (x1 - x0), y0 + (f[1] + 0.3) * (y1 - y0))
+        k = i % 4
+        if k == 0:
+            reqs.append(("join", ("b", "b"), dict(want_area=True, self_pairs="all", window=w)))
+        elif k == 1:
+            reqs.append(("relate", ("b", "b", "", "distinct"), dict(window=w)))
+        elif k == 2:
+            reqs.append(("dwithin", ("b", "b", "dwithin", 3, 0, "all"), dict(window=w)))
+        else:
+            reqs.append(("dwithin", ("b", "b", "buffer", -1, -1.5, "distinct"), dict(window=w)))
+    alone = [getattr(c, k)(*a, **kw)[0] for k, a, kw in reqs]
+    out = [None] * len(reqs)
+
+    def run(i):
+        k, a, kw = reqs[i]
+        out[i] = G.Client(daemon["sock"], token=TOKEN).__getattribute__(k)(*a, **kw)[0]
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(reqs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i, rows in enumerate(out):
+        _same_rows(rows, alone[i])
+    assert any(len(r) for r in alone)
+
+
 def test_drop(daemon):
     c = daemon["client"]
     assert c.load("tmp", "gen:parcels_grid:nx=10;ny=10;seed=1")["rows"] == 100
