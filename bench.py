@@ -71,6 +71,36 @@ def hbm_traffic(kernel):
     return None
 
 
+def other_rooflines(n, st, t_bvh):
+    """HBM rooflines of the LBVH build and the traversal, with the compulsory bytes of each: build 72 B per primitive
+    (read its 16 B box; key 8 + value 4, sorted id 4, node boxes 2 x 16, parents 2 x 4, children 8 written once);
+    traversal 16 B per probe box + 8 B per candidate written + 44 B per primitive of tree read once. Times are host wall
+    clock around the synchronised work (the build call; the step's traversal), so the fractions are lower bounds."""
+    build_b = 72.0 * n
+    trav_b = 16.0 * n + 8.0 * st["candidates"] + 44.0 * n
+    out = []
+    for name, b, sec in (("lbvh_build", build_b, t_bvh), ("traversal", trav_b, st["traverse_ms"] * 1e-3)):
+        gbs = b / max(sec, 1e-9) / 1e9
+        out.append({"kernel": name, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 5), "compulsory_bytes": int(b), "ms": round(sec * 1e3, 3)})
+    return out
+
+
+def issued_f64(kernel):
+    """Issued FP64 lane-FLOPs per dispatch of `kernel` (64 x (ADD + MUL + 2 FMA) wave instructions, whatever the exec
+    mask) from the committed counter summary's FP64 instruction table (7 cells a row); None when it lacks the kernel."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+            try:
+                return int(float(cells[6]))
+            except ValueError:
+                return None
+    return None
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """Device-to-device copies of a known size through the HIP runtime (its copy kernel), so that the HBM byte
     counters of a profiling run can be scaled: FETCH_SIZE / WRITE_SIZE of these dispatches should read nbytes each."""
@@ -408,6 +438,10 @@ def main():
                     "flops_counted": {"edge": "edge contributions of the pairs that need them", "screen": "contact and containment",
                                       "all": "the whole refinement"}[kern[4]],
                     "flop_mix": {k: (fl["edge"] if kern[4] == "edge" else fl)[k] for k in ("add", "mul", "fma", "div", "sqrt")}}
+        issued = issued_f64(kern[0])   # lanes the FP64 instructions occupied, masked or not (counter summary)
+        if issued:
+            roofline["issued_f64_lane_flops"] = issued
+            roofline["lane_efficiency"] = round(flops_per_pair * per_launch_pairs / issued, 4)
         # cpu baseline: the oracle's relate self-join of the same 2.4M-parcel layer on the host's share of cores,
         # repeated until about --cpu-seconds of work
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
@@ -455,6 +489,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
         "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1), "prep": prep},
+        "roofline_other": other_rooflines(S.n, st, t_bvh),
         "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving, "national": national,
     }
     print(json.dumps(out), flush=True)
