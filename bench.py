@@ -269,7 +269,7 @@ def national_leg(gsj, ctx, args):
 
     def run(lanes, arrivals):
         # serial: one request at a time, as the reference serves them; served: lanes plus batches of windowed joins
-        s = gsj.Scheduler(ctx, lanes=lanes, batch=1 if lanes == 1 else 16)
+        s = gsj.Scheduler(ctx, lanes=lanes, batch=1 if lanes == 1 else args.batch)
         try:
             for k, kw in reqs[: min(nreq, 16)]:   # warm-up: lanes' scratch
                 s.wait(s.submit(k, L, L, collect=False, **kw))
@@ -329,6 +329,7 @@ def main():
     ap.add_argument("--requests", type=int, default=256, help="requests in the served-vs-serial mix")
     ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
     ap.add_argument("--lanes", type=int, default=4, help="lanes of the served path")
+    ap.add_argument("--batch", type=int, default=16, help="most windowed requests a served lane runs in one pass (national stream)")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run 1 GiB device copies (HBM counter calibration)")
     ap.add_argument("--national", type=float, default=0.0,
                     help="run only the resident national-scale leg at this scale (1.0 = 162M parcels + 161M points)")

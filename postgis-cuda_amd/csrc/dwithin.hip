@@ -69,6 +69,19 @@ __device__ __forceinline__ bool probe_real(const DPolys& S, uint32_t r, double s
   return probe_vertex(S, r, shx, shy, x, y);
 }
 
+// The real edges of L[base .. min(base + 64, n)) whose boxes meet [x0, x1] x [y0, y1], one bit per entry. The pair
+// tests below run over these bits in list order: the box test alone runs once per entry, and the distance work runs
+// together across a wave's lanes, about as many times as the most crowded lane has neighbours.
+__device__ __forceinline__ uint64_t near_mask(const Ed* L, int base, int n, double x0, double x1, double y0, double y1) {
+  const int m = min(64, n - base);
+  uint64_t cand = 0;
+  for (int j = 0; j < m; j++) {
+    const Ed f = L[base + j];
+    if (!(f.syn || fmax(f.x0, f.x1) < x0 || fmin(f.x0, f.x1) > x1 || fmax(f.y0, f.y1) < y0 || fmin(f.y0, f.y1) > y1)) cand |= 1ull << j;
+  }
+  return cand;
+}
+
 // One piece pair on a lane group (buffer.cuh:185-237). Returns 1 / 0, or -1 when a side has more than cap edges in
 // the window (need = edges required). With report and a hit, dist = the pair's distance (see the file comment).
 template <int G>
@@ -98,25 +111,29 @@ __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b,
     if (e.syn) continue;
     const double bx0 = fmin(e.x0, e.x1) - gx, bx1 = fmax(e.x0, e.x1) + gx, by0 = fmin(e.y0, e.y1) - gy, by1 = fmax(e.y0, e.y1) + gy;
     const double ux = (e.x1 - e.x0) * kx, uy = (e.y1 - e.y0) * ky, La2 = ux * ux + uy * uy;
-    for (int j = 0; j < nb; j++) {
-      const Ed f = LB[j];
-      if (f.syn || fmax(f.x0, f.x1) < bx0 || fmin(f.x0, f.x1) > bx1 || fmax(f.y0, f.y1) < by0 || fmin(f.y0, f.y1) > by1) continue;
-      if (seg_isect(e.x0, e.y0, e.x1, e.y1, f.x0, f.y0, f.x1, f.y1)) {
-        hit = cross = true;
-        if (!report) break;
-        continue;
+    for (int base = 0; base < nb && (report || !hit); base += 64) {
+      uint64_t cand = near_mask(LB, base, nb, bx0, bx1, by0, by1);
+      while (cand != 0) {
+        const int j = base + __builtin_ctzll(cand);
+        cand &= cand - 1;
+        const Ed f = LB[j];
+        if (seg_isect(e.x0, e.y0, e.x1, e.y1, f.x0, f.y0, f.x1, f.y1)) {
+          hit = cross = true;
+          if (!report) break;
+          continue;
+        }
+        double px, py, qx, qy;
+        const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
+        const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
+        const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+        const double dlo = d - band, dhi = d + band;
+        if (d2 <= dlo * dlo) hit = true;
+        else if (d2 <= dhi * dhi &&
+                 geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1, false) <= d)
+          hit = true;
+        if (report) m_up = fmin(m_up, sqrt(d2) + band);
+        else if (hit) break;
       }
-      double px, py, qx, qy;
-      const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
-      const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
-      const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
-      const double dlo = d - band, dhi = d + band;
-      if (d2 <= dlo * dlo) hit = true;
-      else if (d2 <= dhi * dhi &&
-               geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1, false) <= d)
-        hit = true;
-      if (report) m_up = fmin(m_up, sqrt(d2) + band);
-      else if (hit) break;
     }
   }
   hit = gany<G>(hit);
@@ -150,16 +167,20 @@ __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b,
         if (e.syn) continue;
         const double bx0 = fmin(e.x0, e.x1) - gx, bx1 = fmax(e.x0, e.x1) + gx, by0 = fmin(e.y0, e.y1) - gy, by1 = fmax(e.y0, e.y1) + gy;
         const double ux = (e.x1 - e.x0) * kx, uy = (e.y1 - e.y0) * ky, La2 = ux * ux + uy * uy;
-        for (int j = 0; j < nb; j++) {
-          const Ed f = LB[j];
-          if (f.syn || fmax(f.x0, f.x1) < bx0 || fmin(f.x0, f.x1) > bx1 || fmax(f.y0, f.y1) < by0 || fmin(f.y0, f.y1) > by1) continue;
-          double px, py, qx, qy;
-          const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
-          const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
-          const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
-          if (sqrt(d2) - band > m) continue;
-          best = fmin(best, geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1,
-                                           false));
+        for (int base = 0; base < nb; base += 64) {
+          uint64_t cand = near_mask(LB, base, nb, bx0, bx1, by0, by1);
+          while (cand != 0) {
+            const int j = base + __builtin_ctzll(cand);
+            cand &= cand - 1;
+            const Ed f = LB[j];
+            double px, py, qx, qy;
+            const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
+            const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
+            const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+            if (sqrt(d2) - band > m) continue;
+            best = fmin(best, geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1,
+                                             false));
+          }
         }
       }
       dist = group_min<G>(best);

@@ -148,19 +148,32 @@ __device__ Contrib edge_contrib_capped(const Ed& E, const Src& L, const EdgeCtx&
   int oidx[MAXO];
   int no = 0;
   ts[ns++] = 0; ts[ns++] = 1;
+  // box test first, then the classification of the marked entries in list order (as in the register path below)
   const int n = L.size();
-  for (int k = 0; k < n; k++) {
-    Ed e;
-    if (!L.get(k, e)) continue;
-    double lo, hi, t;
-    bool same;
-    const int kind = classify_edge(e, px, py, qx, qy, dx, dy, len2, E.cut, c, lo, hi, same, t);
-    if (kind == 1) {
-      if (lo > 0 && ns < MAXS) ts[ns++] = lo;
-      if (hi < 1 && ns < MAXS) ts[ns++] = hi;
-      if (no < MAXO) { ol0[no] = lo; ol1[no] = hi; osame[no] = same; oreal[no] = !e.syn; oidx[no] = k; no++; }
-    } else if (kind == 2 && ns < MAXS) {
-      ts[ns++] = t;
+  const double tx0 = fmin(px, qx) - c.tol, tx1 = fmax(px, qx) + c.tol, ty0 = fmin(py, qy) - c.tol, ty1 = fmax(py, qy) + c.tol;
+  for (int base = 0; base < n; base += 64) {
+    const int m = min(64, n - base);
+    uint64_t cand = 0;
+    for (int j = 0; j < m; j++) {
+      Ed e;
+      if (!L.get(base + j, e)) continue;
+      if (!(fmax(e.x0, e.x1) < tx0 || fmin(e.x0, e.x1) > tx1 || fmax(e.y0, e.y1) < ty0 || fmin(e.y0, e.y1) > ty1)) cand |= 1ull << j;
+    }
+    while (cand != 0) {
+      const int k = base + __builtin_ctzll(cand);
+      cand &= cand - 1;
+      Ed e;
+      L.get(k, e);
+      double lo, hi, t;
+      bool same;
+      const int kind = classify_edge(e, px, py, qx, qy, dx, dy, len2, E.cut, c, lo, hi, same, t);
+      if (kind == 1) {
+        if (lo > 0 && ns < MAXS) ts[ns++] = lo;
+        if (hi < 1 && ns < MAXS) ts[ns++] = hi;
+        if (no < MAXO) { ol0[no] = lo; ol1[no] = hi; osame[no] = same; oreal[no] = !e.syn; oidx[no] = k; no++; }
+      } else if (kind == 2 && ns < MAXS) {
+        ts[ns++] = t;
+      }
     }
   }
   for (int a = 1; a < ns; a++) {
@@ -211,22 +224,38 @@ __device__ Contrib edge_contrib(const Ed& E, const Src& L, const EdgeCtx& c) {
     s0 = n0; s1 = n1; s2 = n2; s3 = n3; s4 = n4; s5 = n5; s6 = n6; s7 = n7;
     ns++;
   };
+  // Two passes per run of 64 list entries: the box test alone marks the entries whose box meets this edge's box grown
+  // by tol (the first test of classify_edge, same expressions), then the classification runs for the marked entries
+  // in list order. Every lane of a wave takes the second loop's iterations together, so the classification runs
+  // about as many times as the most crowded lane has neighbours, not once per list entry whenever any lane has one.
   const int n = L.size();
-  for (int k = 0; k < n && !spill; k++) {
-    Ed e;
-    if (!L.get(k, e)) continue;
-    double lo, hi, t;
-    bool same;
-    const int kind = classify_edge(e, px, py, qx, qy, dx, dy, len2, E.cut, c, lo, hi, same, t);
-    if (kind == 1) {
-      if (lo > 0) insert(lo);
-      if (hi < 1) insert(hi);
-      if (no == 0) { o0lo = lo; o0hi = hi; o0same = same; o0real = !e.syn; o0idx = k; }
-      else if (no == 1) { o1lo = lo; o1hi = hi; o1same = same; o1real = !e.syn; o1idx = k; }
-      else spill = true;
-      no++;
-    } else if (kind == 2) {
-      insert(t);
+  const double tx0 = fmin(px, qx) - c.tol, tx1 = fmax(px, qx) + c.tol, ty0 = fmin(py, qy) - c.tol, ty1 = fmax(py, qy) + c.tol;
+  for (int base = 0; base < n && !spill; base += 64) {
+    const int m = min(64, n - base);
+    uint64_t cand = 0;
+    for (int j = 0; j < m; j++) {
+      Ed e;
+      if (!L.get(base + j, e)) continue;
+      if (!(fmax(e.x0, e.x1) < tx0 || fmin(e.x0, e.x1) > tx1 || fmax(e.y0, e.y1) < ty0 || fmin(e.y0, e.y1) > ty1)) cand |= 1ull << j;
+    }
+    while (cand != 0 && !spill) {
+      const int k = base + __builtin_ctzll(cand);
+      cand &= cand - 1;
+      Ed e;
+      L.get(k, e);
+      double lo, hi, t;
+      bool same;
+      const int kind = classify_edge(e, px, py, qx, qy, dx, dy, len2, E.cut, c, lo, hi, same, t);
+      if (kind == 1) {
+        if (lo > 0) insert(lo);
+        if (hi < 1) insert(hi);
+        if (no == 0) { o0lo = lo; o0hi = hi; o0same = same; o0real = !e.syn; o0idx = k; }
+        else if (no == 1) { o1lo = lo; o1hi = hi; o1same = same; o1real = !e.syn; o1idx = k; }
+        else spill = true;
+        no++;
+      } else if (kind == 2) {
+        insert(t);
+      }
     }
   }
   if (spill) { out.bits = SPILL; return out; }
@@ -522,10 +551,12 @@ __global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DP
     cb.in_c = (st & ST_INA) != 0;
     uint32_t bits = 0;
     for (int s = gl; s < na + nb; s += G) {
+      // one call with the side chosen per lane: a wave holds A- and B-edge lanes in the same round, and two calls
+      // would run one after the other
       const bool sideA = s < na;
       Ed e;
       (sideA ? LA : LB).get(sideA ? s : s - na, e);
-      const Contrib c = sideA ? edge_contrib<VertSrc, false>(e, LB, ca) : edge_contrib<VertSrc, false>(e, LA, cb);
+      const Contrib c = edge_contrib<VertSrc, false>(e, sideA ? LB : LA, sideA ? ca : cb);
       acc[s] = c.acc;
       if (rel) { lin[s] = c.lin; lon[s] = c.lon; }
       bits |= c.bits;
@@ -624,7 +655,8 @@ __device__ bool refine_lists(const DPolys& P, const DPolys& R, uint32_t a, uint3
     cb.in_c = inA_c;
     for (int s = gl; s < na + nb; s += G) {
       const bool sideA = s < na;
-      const Contrib c = sideA ? edge_contrib<ListSrc, CAPPED>(S.LA[s], LB, ca) : edge_contrib<ListSrc, CAPPED>(S.LB[s - na], LA, cb);
+      const Ed e = (sideA ? S.LA : S.LB)[sideA ? s : s - na];
+      const Contrib c = edge_contrib<ListSrc, CAPPED>(e, ListSrc{sideA ? S.LB : S.LA, sideA ? nb : na}, sideA ? ca : cb);
       S.acc[s] = c.acc;
       if (rel) { S.lin[s] = c.lin; S.lon[s] = c.lon; }
       bits |= c.bits;
