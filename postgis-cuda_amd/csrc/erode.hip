@@ -47,37 +47,47 @@ __device__ __forceinline__ ErScratch er_carve(unsigned char* base, int capE, int
 __device__ bool in_excl(const BEd* E, int n, double r, double x, double y, double eps) {
   if (r <= 0) return false;
   const double ri = r - eps;
-  for (int k = 0; k < n; k++) {
-    const BEd& e = E[k];
-    if (x < fmin(e.x0, e.x1) - r || x > fmax(e.x0, e.x1) + r || y < fmin(e.y0, e.y1) - r || y > fmax(e.y0, e.y1) + r) continue;
-    const double dx = x - e.x0, dy = y - e.y0;
-    const double t = dx * e.ux + dy * e.uy, s = dy * e.ux - dx * e.uy;   // s > 0: interior (left) side
-    if (fabs(s) < ri && t > -eps && t < e.L + eps) return true;         // two-sided strip along the edge
-    if (e.vt == 0) continue;
-    const double vx = x - e.x1, vy = y - e.y1, d2 = vx * vx + vy * vy;
-    if (d2 >= r * r) continue;
-    const BEd& f = E[e.succ];
-    if (e.vt == 1) {   // convex vertex: disk outside the interior wedge
-      const double s2 = vy * f.ux - vx * f.uy;
-      if (d2 < ri * ri && !(s > eps && s2 > eps)) return true;
-      continue;
+  // the answer is an OR over the edges: the box test marks 64 edges at a time, then the region tests run over the
+  // marked ones (a wave's lanes take those iterations together)
+  for (int base = 0; base < n; base += 64) {
+    uint64_t cand = 0;
+    for (int j = 0, m = min(64, n - base); j < m; j++) {
+      const BEd& e = E[base + j];
+      if (!(x < fmin(e.x0, e.x1) - r || x > fmax(e.x0, e.x1) + r || y < fmin(e.y0, e.y1) - r || y > fmax(e.y0, e.y1) + r)) cand |= 1ull << j;
     }
-    // reflex vertex: fillet polygon (v, P0 .. Pm) from n_k clockwise to n_s
-    const double nkx = -e.uy, nky = e.ux, nsx = -f.uy, nsy = f.ux;
-    if (nkx * vy - nky * vx > eps) continue;
-    if (nsx * vy - nsy * vx < -eps) continue;
-    const double chh = sqrt(0.5 * (1.0 + e.ci)), shh = sqrt(fmax(0.0, 0.5 * (1.0 - e.ci)));   // cos, sin of half the increment
-    const double ch = r * chh - eps;                                                         // chord distance from v
-    if (d2 < ch * ch) return true;
-    double mx = nkx * chh + nky * shh, my = -nkx * shh + nky * chh;   // chord normals
-    bool in = true;
-    for (int i = 0, m = e.vt - 1; i < m && in; i++) {
-      if (mx * vx + my * vy >= ch) in = false;
-      const double t2 = mx * e.ci + my * e.si;
-      my = -mx * e.si + my * e.ci;
-      mx = t2;
+    while (cand != 0) {
+      const int k = base + __builtin_ctzll(cand);
+      cand &= cand - 1;
+      const BEd& e = E[k];
+      const double dx = x - e.x0, dy = y - e.y0;
+      const double t = dx * e.ux + dy * e.uy, s = dy * e.ux - dx * e.uy;   // s > 0: interior (left) side
+      if (fabs(s) < ri && t > -eps && t < e.L + eps) return true;         // two-sided strip along the edge
+      if (e.vt == 0) continue;
+      const double vx = x - e.x1, vy = y - e.y1, d2 = vx * vx + vy * vy;
+      if (d2 >= r * r) continue;
+      const BEd& f = E[e.succ];
+      if (e.vt == 1) {   // convex vertex: disk outside the interior wedge
+        const double s2 = vy * f.ux - vx * f.uy;
+        if (d2 < ri * ri && !(s > eps && s2 > eps)) return true;
+        continue;
+      }
+      // reflex vertex: fillet polygon (v, P0 .. Pm) from n_k clockwise to n_s
+      const double nkx = -e.uy, nky = e.ux, nsx = -f.uy, nsy = f.ux;
+      if (nkx * vy - nky * vx > eps) continue;
+      if (nsx * vy - nsy * vx < -eps) continue;
+      const double chh = sqrt(0.5 * (1.0 + e.ci)), shh = sqrt(fmax(0.0, 0.5 * (1.0 - e.ci)));   // cos, sin of half the increment
+      const double ch = r * chh - eps;                                                         // chord distance from v
+      if (d2 < ch * ch) return true;
+      double mx = nkx * chh + nky * shh, my = -nkx * shh + nky * chh;   // chord normals
+      bool in = true;
+      for (int i = 0, m = e.vt - 1; i < m && in; i++) {
+        if (mx * vx + my * vy >= ch) in = false;
+        const double t2 = mx * e.ci + my * e.si;
+        my = -mx * e.si + my * e.ci;
+        mx = t2;
+      }
+      if (in) return true;
     }
-    if (in) return true;
   }
   return false;
 }
@@ -352,6 +362,17 @@ __device__ int er_setup(const DPolys& P, const DPolys& R, const uint32_t* pfP, c
   return 1;
 }
 
+// elements L[base .. min(base + 64, n)) whose boxes meet [x0, x1] x [y0, y1], one bit each: the crossing tests below
+// are ORs over element pairs, so they run over these bits, the lanes of a wave together
+__device__ __forceinline__ uint64_t box_mask(const BEl* L, int base, int n, double x0, double x1, double y0, double y1) {
+  uint64_t cand = 0;
+  for (int j = 0, m = min(64, n - base); j < m; j++) {
+    const BEl q = L[base + j];
+    if (!(fmax(q.x0, q.x1) < x0 || fmin(q.x0, q.x1) > x1 || fmax(q.y0, q.y1) < y0 || fmin(q.y0, q.y1) > y1)) cand |= 1ull << j;
+  }
+  return cand;
+}
+
 // (a) the eroded boundaries cross outside both exclusion zones: this thread's share (element i of A, i = t, t + NT, ...)
 __device__ bool er_cross(const ErScratch& sc, const ErPair& E, int t, int NT, const volatile int* stop) {
   auto inW = [&](double x, double y) { return x >= E.Wx0 - E.eps && x <= E.Wx1 + E.eps && y >= E.Wy0 - E.eps && y <= E.Wy1 + E.eps; };
@@ -359,14 +380,17 @@ __device__ bool er_cross(const ErScratch& sc, const ErPair& E, int t, int NT, co
     if (stop && *stop) return false;
     const BEl p = sc.LA[i];
     const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
-    for (int j = 0; j < E.lb; j++) {
-      const BEl q = sc.LB[j];
-      if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
-      if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
-      double xs[3], ys[3];
-      const int k = seg_points(p, q, xs, ys);
-      for (int u = 0; u < k; u++)
-        if (inW(xs[u], ys[u]) && !in_excl(sc.EA, E.na, E.A.r, xs[u], ys[u], E.eps) && !in_excl(sc.EB, E.nb, E.B.r, xs[u], ys[u], E.eps)) return true;
+    for (int base = 0; base < E.lb; base += 64) {
+      uint64_t cand = box_mask(sc.LB, base, E.lb, px0, px1, py0, py1);
+      while (cand != 0) {
+        const BEl q = sc.LB[base + __builtin_ctzll(cand)];
+        cand &= cand - 1;
+        if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
+        double xs[3], ys[3];
+        const int k = seg_points(p, q, xs, ys);
+        for (int u = 0; u < k; u++)
+          if (inW(xs[u], ys[u]) && !in_excl(sc.EA, E.na, E.A.r, xs[u], ys[u], E.eps) && !in_excl(sc.EB, E.nb, E.B.r, xs[u], ys[u], E.eps)) return true;
+      }
     }
   }
   return false;
@@ -389,14 +413,17 @@ __device__ bool er_inside(const ErScratch& sc, const ErPair& E, bool inA_c, bool
     const BEl p = LS[i];
     if (vertex_in_other(p.x0, p.y0) || vertex_in_other(p.x1, p.y1)) return true;
     const double px0 = fmin(p.x0, p.x1), px1 = fmax(p.x0, p.x1), py0 = fmin(p.y0, p.y1), py1 = fmax(p.y0, p.y1);
-    for (int j = i + 1; j < ls; j++) {
-      const BEl q = LS[j];
-      if (fmax(q.x0, q.x1) < px0 || fmin(q.x0, q.x1) > px1 || fmax(q.y0, q.y1) < py0 || fmin(q.y0, q.y1) > py1) continue;
-      if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
-      double xs[3], ys[3];
-      const int k = seg_points(p, q, xs, ys);
-      for (int u = 0; u < k; u++)
-        if (vertex_in_other(xs[u], ys[u])) return true;
+    for (int base = i + 1; base < ls; base += 64) {
+      uint64_t cand = box_mask(LS, base, ls, px0, px1, py0, py1);
+      while (cand != 0) {
+        const BEl q = LS[base + __builtin_ctzll(cand)];
+        cand &= cand - 1;
+        if (!seg_isect(p.x0, p.y0, p.x1, p.y1, q.x0, q.y0, q.x1, q.y1)) continue;
+        double xs[3], ys[3];
+        const int k = seg_points(p, q, xs, ys);
+        for (int u = 0; u < k; u++)
+          if (vertex_in_other(xs[u], ys[u])) return true;
+      }
     }
   }
   return false;
