@@ -529,11 +529,13 @@ __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, co
   }
 }
 
-// pairs that overflowed: a block per pair (ER_SLAB_T threads) with lists in global memory of the size the queue
-// reported; its first wave builds the lists, then the whole block runs the crossing and containment tests; pairs that
-// overflow the element lists again are re-queued with their exact element count
-constexpr int ER_SLAB_T = 512;
-__global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
+// pairs that overflowed: a block of T threads per pair with lists in global memory of the size the queue reported;
+// its first wave builds the lists, then the whole block runs the crossing and containment tests; pairs that overflow
+// the element lists again are re-queued with their exact element count. T is chosen per launch from the number of
+// pairs (slab_threads): a few pairs take 512 threads each, many pairs one wave each, so that the other waves of a
+// block do not wait through the list building while the queue still holds pairs for them.
+template <int T>
+__global__ void __launch_bounds__(T) k_erode_slab(const DPolys P, const DPolys R, const uint32_t* __restrict__ pfP,
                                                           const uint32_t* __restrict__ pfR, const uint32_t* __restrict__ pairs,
                                                           const uint32_t* __restrict__ items, uint32_t ni, const ErRadii rr,
                                                           uint8_t* __restrict__ flag, unsigned char* __restrict__ slab, int capE, int capL,
@@ -574,13 +576,13 @@ __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const 
         if (lane == 0) { s_inA = a; s_inB = b; }
       }
       __syncthreads();
-      if (self && er_vertices(sc, E, s_inA != 0, s_inB != 0, self_side(E), t, ER_SLAB_T, &s_hit)) s_hit = 1;
+      if (self && er_vertices(sc, E, s_inA != 0, s_inB != 0, self_side(E), t, T, &s_hit)) s_hit = 1;
       __syncthreads();
-      if (!s_hit && er_cross(sc, E, t, ER_SLAB_T, &s_hit)) s_hit = 1;
+      if (!s_hit && er_cross(sc, E, t, T, &s_hit)) s_hit = 1;
       __syncthreads();
       if (!s_hit) {
         for (int side = 0; side < 2; side++) {
-          if (!s_hit && er_inside(sc, E, s_inA != 0, s_inB != 0, side, t, ER_SLAB_T, &s_hit)) s_hit = 1;
+          if (!s_hit && er_inside(sc, E, s_inA != 0, s_inB != 0, side, t, T, &s_hit)) s_hit = 1;
           __syncthreads();
         }
       }
@@ -588,6 +590,14 @@ __global__ void __launch_bounds__(ER_SLAB_T) k_erode_slab(const DPolys P, const 
     }
     __syncthreads();   // the lists and SE are rewritten for the next pair
   }
+}
+
+// threads per overflowing pair: one wave each once the pairs fill every CU's waves (8 per CU at these kernels'
+// register counts), two below that, a block of 512 for a few pairs
+int slab_threads(uint64_t nh, int cu) {
+  if (nh >= 8ull * cu) return 64;
+  if (nh >= 2ull * cu) return 128;
+  return 512;
 }
 
 __global__ void k_ctl_clear(ErCtl* ctl) { ctl->n = 0; ctl->needE = 0; ctl->needL = 0; }
@@ -732,8 +742,15 @@ uint64_t erode_pass2(Context& c, const DPolys& P, const DPolys& R, const uint32_
     unsigned char* slab = c.s_er_slab.as<unsigned char>(per * blocks + 64);
     hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(1), 0, stream, ctl);
     GSJ_HIP(hipEventRecord(c.ev[2], stream));
-    hipLaunchKernelGGL(k_erode_slab, dim3((uint32_t)blocks), dim3(ER_SLAB_T), 0, stream, P, R, pfP, pfR, ep, heavy, nh, rr, flag, slab, capE, capL, again,
-                       ctl);
+    const int T = slab_threads(nh, c.cu_count);
+    if (T == 64)
+      hipLaunchKernelGGL(k_erode_slab<64>, dim3((uint32_t)blocks), dim3(64), 0, stream, P, R, pfP, pfR, ep, heavy, nh, rr, flag, slab, capE, capL, again, ctl);
+    else if (T == 128)
+      hipLaunchKernelGGL(k_erode_slab<128>, dim3((uint32_t)blocks), dim3(128), 0, stream, P, R, pfP, pfR, ep, heavy, nh, rr, flag, slab, capE, capL, again,
+                         ctl);
+    else
+      hipLaunchKernelGGL(k_erode_slab<512>, dim3((uint32_t)blocks), dim3(512), 0, stream, P, R, pfP, pfR, ep, heavy, nh, rr, flag, slab, capE, capL, again,
+                         ctl);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[3], stream));
     GSJ_HIP(hipMemcpyAsync(h, ctl, sizeof(ErCtl), hipMemcpyDeviceToHost, stream));
