@@ -23,6 +23,7 @@ struct DPolys {
   const uint32_t* ginfo = nullptr;   // [n] stored vertices (low 16 bits, saturated) | rings << 16, for work classes
   const uint32_t* parent = nullptr;  // [n] piece -> parent, nullptr if not subdivided
   const uint32_t* synb = nullptr;    // [(nv+31)/32] bit v: edge v -> v+1 was made by subdivision; nullptr if none
+  const uint8_t* vflag = nullptr;    // [nv] VF_* bits of vertex v (present exactly when synb is)
   const double* blen = nullptr;      // [nparent] real boundary length per parent
   const uint32_t* pfirst = nullptr;  // [nparent+1] first piece of each parent (subdivided layers only)
   uint32_t nparent = 0;
@@ -33,6 +34,10 @@ struct DPolys {
   const float4* cbb = nullptr;       // [nchunks]
 };
 constexpr int CHUNK = 64;
+// per-vertex flags of a subdivided layer (ring-aware, so the pair kernels need no neighbour lookups): VF_SYN edge
+// v -> v+1 is synthetic; VF_CUT0 / VF_CUT1 the ring's previous / next edge is (wrapping around the closed ring);
+// all 0 at a ring's last vertex
+constexpr uint8_t VF_SYN = 2, VF_CUT0 = 4, VF_CUT1 = 8;
 constexpr uint32_t NO_CHUNKS = 0xFFFFFFFFu;
 
 constexpr uint32_t PROBE_TILE = 1024;   // probes per tile box (window membership)

@@ -59,6 +59,24 @@ __global__ void k_synb(uint64_t nv, const uint8_t* syn, uint32_t* synb) {
   synb[w] = bits;
 }
 
+// VF_* flags of every vertex from the synthetic-edge bits, a thread per ring (the same neighbour rule as the pair
+// kernels' edge collection, engine/engine.cuh:358-377)
+__global__ void k_vflag(uint32_t nr, const uint64_t* rvo, const uint32_t* synb, uint8_t* vf) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const uint64_t s = rvo[r], e = rvo[r + 1];
+  auto bit = [&](uint64_t i) { return (synb[i >> 5] >> (i & 31)) & 1u; };
+  for (uint64_t i = s; i < e; i++) {
+    uint8_t f = 0;
+    if (i + 1 < e) {
+      if (bit(i)) f |= VF_SYN;
+      if (bit(i > s ? i - 1 : e - 2)) f |= VF_CUT0;
+      if (bit(i + 2 < e ? i + 1 : s)) f |= VF_CUT1;
+    }
+    vf[i] = f;
+  }
+}
+
 // first piece of each parent (parents sorted; parents without pieces get an empty range)
 __global__ void k_pfirst(uint32_t n, uint32_t nparent, const uint32_t* parent, uint32_t* pf) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,6 +197,10 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
     if (syn && nv) {
       hipLaunchKernelGGL(k_synb, dim3(div_up((nv + 31) / 32, 256)), dim3(256), 0, st, nv, syn8, const_cast<uint32_t*>(d.synb));
       GSJ_LAUNCHED();
+      // the uploaded 0/1 bytes are read by k_poly_prep and k_synb above (same stream); then they become the flags
+      hipLaunchKernelGGL(k_vflag, dim3(div_up(nr, 256)), dim3(256), 0, st, (uint32_t)nr, d.rvo, d.synb, syn8);
+      GSJ_LAUNCHED();
+      d.vflag = syn8;
     }
     if (parent) {
       hipLaunchKernelGGL(k_pfirst, dim3(div_up(n, 256)), dim3(256), 0, st, (uint32_t)n, nparent, d.parent, const_cast<uint32_t*>(d.pfirst));

@@ -101,7 +101,7 @@ __device__ __forceinline__ void walk_edge_runs(const DPolys& S, uint32_t g, cons
 template <int G>
 __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, Ed* out, int cap, int gl) {
   int cnt = 0;
-  walk_edge_runs(S, g, [&](const float4 c) { return chunk_misses(c, shx, shy, ov); }, [&](uint64_t s, uint64_t e, uint64_t i0, uint64_t i1) {
+  walk_edge_runs(S, g, [&](const float4 c) { return chunk_misses(c, shx, shy, ov); }, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
     for (uint64_t base = i0; base < i1; base += G) {
       const uint64_t i = base + gl;
       bool v = false;
@@ -109,8 +109,9 @@ __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, cons
       if (i < i1) {
         const double2 p = S.xy[i], q = S.xy[i + 1];
         ed.x0 = shx + p.x; ed.y0 = shy + p.y; ed.x1 = shx + q.x; ed.y1 = shy + q.y;
-        ed.syn = syn_bit(S.synb, i);
-        ed.cut = S.synb ? ((syn_bit(S.synb, i > s ? i - 1 : e - 2) ? 1 : 0) | (syn_bit(S.synb, i + 2 < e ? i + 1 : s) ? 2 : 0)) : 0;
+        const uint8_t f = S.vflag ? S.vflag[i] : 0;
+        ed.syn = (f & VF_SYN) ? 1 : 0;
+        ed.cut = (f >> 2) & 3;   // bit 0 VF_CUT0, bit 1 VF_CUT1
         v = !edge_outside(ed.x0, ed.y0, ed.x1, ed.y1, ov);
       }
       const uint64_t m = gballot<G>(v);

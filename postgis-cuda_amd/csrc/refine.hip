@@ -28,7 +28,9 @@ namespace {
 
 // ------------------------------------------------------------------------------------------- small-class source
 // The small class reads a polygon's edges from its LDS vertex list (the list classes use ListSrc).
-constexpr uint8_t F_VALID = 1, F_SYN = 2, F_CUT0 = 4, F_CUT1 = 8, F_INOV = 16;
+// per-vertex flags in LDS: the layer's VF_SYN / VF_CUT0 / VF_CUT1 bits plus these two
+constexpr uint8_t F_VALID = 1, F_SYN = VF_SYN, F_INOV = 16;
+static_assert(VF_CUT0 == 4 && VF_CUT1 == 8, "Ed::cut = (flags >> 2) & 3");
 
 struct VertSrc {   // the edges of a small polygon meeting the overlap box: list entry j is edge (V[I[j]], V[I[j] + 1])
   const double2* V;
@@ -361,16 +363,11 @@ __device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, 
   for (int k = gl; k < nv; k += G) {
     int r = 0;
     while (r + 1 < (int)nrings && sp.rs[r + 1] <= k) r++;
-    const int s = sp.rs[r], e = sp.rs[r + 1];
+    const int e = sp.rs[r + 1];
     uint8_t f = 0;
     if (k + 1 < e) {
-      const uint64_t i = v0 + k, si = v0 + s, ei = v0 + e;
       f = F_VALID;
-      if (syn_bit(S.synb, i)) f |= F_SYN;
-      if (S.synb) {
-        if (syn_bit(S.synb, i > si ? i - 1 : ei - 2)) f |= F_CUT0;
-        if (syn_bit(S.synb, i + 2 < ei ? i + 1 : si)) f |= F_CUT1;
-      }
+      if (S.vflag) f |= S.vflag[v0 + k];   // VF_SYN, VF_CUT0, VF_CUT1
       const double2 a = sp.V[k], b = sp.V[k + 1];
       if (!edge_outside(a.x, a.y, b.x, b.y, ov)) f |= F_INOV;
     }
