@@ -443,8 +443,33 @@ __device__ __forceinline__ void load_pair(const DPolys& P, const DPolys& R, uint
   load_small<G, CAP>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
 }
 
+// A group's walk over a pair list (positions first, first + stride, ...). List entry -> pair -> its two geometries are
+// dependent loads; the next pair's list entry is loaded two iterations ahead and its geometries one iteration ahead,
+// so they arrive during the previous pair's work (the loads of a group's current pair complete in issue order
+// after them, so waiting for the current pair never waits on the prefetch).
+struct PairStream {
+  const uint32_t* cl;
+  const uint32_t* pa;
+  const uint32_t* pb;
+  uint32_t n, stride, k, p, a, b, pn, an, bn, pnn;
+  __device__ __forceinline__ PairStream(const uint32_t* cl_, const uint32_t* pa_, const uint32_t* pb_, uint32_t n_, uint32_t first, uint32_t stride_)
+      : cl(cl_), pa(pa_), pb(pb_), n(n_), stride(stride_), k(first), p(0), a(0), b(0), pn(0), an(0), bn(0), pnn(0) {
+    if (k < n) { p = cl[k]; a = pa[p]; b = pb[p]; }
+    if ((uint64_t)k + stride < n) pn = cl[k + stride];
+  }
+  // the loads for the next two iterations: call once at the top of each iteration (k < n)
+  __device__ __forceinline__ void prefetch() {
+    if ((uint64_t)k + stride < n) { an = pa[pn]; bn = pb[pn]; }
+    if ((uint64_t)k + 2ull * stride < n) pnn = cl[k + 2 * stride];
+  }
+  __device__ __forceinline__ void advance() {
+    k += stride;
+    p = pn; a = an; b = bn; pn = pnn;
+  }
+};
+
 template <int G, int CAP, int WHICH>
-__global__ void __launch_bounds__(256) k_screen_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+__global__ void __launch_bounds__(256, 5) k_screen_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                       const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
                                                       RefineOut out, EdgeQueue q) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -457,9 +482,10 @@ __global__ void __launch_bounds__(256) k_screen_small(const DPolys P, const DPol
   const uint32_t nlist = counts[WHICH];
   const uint32_t base = WHICH == 0 ? 0u : counts[0];
   const uint32_t* cl = list + base;
-  for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
-    const uint32_t p = cl[k];
-    const uint32_t a = pa[p], b = pb[p];
+  PairStream ps(cl, pa, pb, nlist, blockIdx.x * gpb + gib, gridDim.x * gpb);
+  for (; ps.k < nlist; ps.advance()) {
+    ps.prefetch();
+    const uint32_t k = ps.k, p = ps.p, a = ps.a, b = ps.b;
     const PairSetup f = setup_pair(P, R, a, b, rp);
     load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
     const int na = SA.ni, nb = SB.ni;   // edges meeting the overlap box
@@ -536,9 +562,10 @@ __global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DP
   double* lon = lin + 2 * CAP;
   const uint32_t nlist = q.count[WHICH];
   const uint32_t* cl = q.list + q.first[WHICH];
-  for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
-    const uint32_t p = cl[k];
-    const uint32_t a = pa[p], b = pb[p];
+  PairStream ps(cl, pa, pb, nlist, blockIdx.x * gpb + gib, gridDim.x * gpb);
+  for (; ps.k < nlist; ps.advance()) {
+    ps.prefetch();
+    const uint32_t p = ps.p, a = ps.a, b = ps.b;
     const uint8_t st = out.isect[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
     load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
