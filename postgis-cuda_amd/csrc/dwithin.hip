@@ -229,9 +229,10 @@ __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t
   uint32_t first = 0;
   for (int w = 0; w < which; w++) first += counts[w];
   const uint32_t* L = list + first;
-  for (uint32_t k = blockIdx.x * gpb + gib; k < n; k += gridDim.x * gpb) {
-    const uint32_t p = L[k];
-    const uint32_t a = A.pa[p], b = A.pb[p];
+  PairStream ps(L, A.pa, A.pb, n, blockIdx.x * gpb + gib, gridDim.x * gpb);
+  for (; ps.k < n; ps.advance()) {
+    ps.prefetch();
+    const uint32_t p = ps.p, a = ps.a, b = ps.b;
     if (self_piece_pair(A, a, b)) {
       if (gl == 0) { A.hit[p] = 1; if (A.dist) A.dist[p] = 0.0; }
       continue;

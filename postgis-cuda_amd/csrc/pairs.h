@@ -154,6 +154,31 @@ __device__ __forceinline__ bool probe_vertex(const DPolys& S, uint32_t r, double
   return true;
 }
 
+// A group's walk over a pair list (positions first, first + stride, ...). List entry -> pair -> its two geometries are
+// dependent loads; the next pair's list entry is loaded two iterations ahead and its geometries one iteration ahead,
+// so they arrive during the previous pair's work (the loads of a group's current pair complete in issue order
+// after them, so waiting for the current pair never waits on the prefetch).
+struct PairStream {
+  const uint32_t* cl;
+  const uint32_t* pa;
+  const uint32_t* pb;
+  uint32_t n, stride, k, p, a, b, pn, an, bn, pnn;
+  __device__ __forceinline__ PairStream(const uint32_t* cl_, const uint32_t* pa_, const uint32_t* pb_, uint32_t n_, uint32_t first, uint32_t stride_)
+      : cl(cl_), pa(pa_), pb(pb_), n(n_), stride(stride_), k(first), p(0), a(0), b(0), pn(0), an(0), bn(0), pnn(0) {
+    if (k < n) { p = cl[k]; a = pa[p]; b = pb[p]; }
+    if ((uint64_t)k + stride < n) pn = cl[k + stride];
+  }
+  // the loads for the next two iterations: call once at the top of each iteration (k < n)
+  __device__ __forceinline__ void prefetch() {
+    if ((uint64_t)k + stride < n) { an = pa[pn]; bn = pb[pn]; }
+    if ((uint64_t)k + 2ull * stride < n) pnn = cl[k + 2 * stride];
+  }
+  __device__ __forceinline__ void advance() {
+    k += stride;
+    p = pn; a = an; b = bn; pn = pnn;
+  }
+};
+
 struct OverflowCtl { uint32_t count; uint32_t need; uint32_t pad0, pad1; };
 
 // hand pair p to the overflow kernel (lists of up to `need` edges per side); called by one lane
