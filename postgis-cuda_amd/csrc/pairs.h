@@ -154,10 +154,10 @@ __device__ __forceinline__ bool probe_vertex(const DPolys& S, uint32_t r, double
   return true;
 }
 
-// A group's walk over a pair list (positions first, first + stride, ...). List entry -> pair -> its two geometries are
-// dependent loads; the next pair's list entry is loaded two iterations ahead and its geometries one iteration ahead,
-// so they arrive during the previous pair's work (the loads of a group's current pair complete in issue order
-// after them, so waiting for the current pair never waits on the prefetch).
+// A group's walk over a pair list (positions first, first + stride, ...). List position -> pair index -> the pair's two
+// geometry ids are dependent loads; a pair's index is loaded two iterations ahead and its geometry ids one iteration
+// ahead, so both arrive while the group works on earlier pairs. The prefetch loads are issued before the current
+// pair's geometry loads, so waiting for those never stalls on a prefetch that was not already due.
 struct PairStream {
   const uint32_t* cl;
   const uint32_t* pa;
