@@ -96,6 +96,35 @@ def test_mesh_subdivided_relate(ctx, maxv):
     _check_rows(got, exp, area=True, code=True)
 
 
+def _holed_and_crowded():
+    """a square with a square hole (120 vertices per side), and a ring whose vertices crowd a point"""
+    t = np.linspace(0, 1, 121)[:-1]
+    outer = np.concatenate([np.c_[t, 0 * t], np.c_[1 + 0 * t, t], np.c_[1 - t, 1 + 0 * t], np.c_[0 * t, 1 - t]])
+    inner = 0.3 + 0.4 * outer[::-1]
+    a = np.linspace(0, 2 * np.pi, 400, endpoint=False)
+    crowd = np.c_[0.3 + 1e-9 * np.cos(a), 0.2 + 1e-9 * np.sin(a)]
+    spiky = np.concatenate([[[0.0, 0.0], [1.0, 0.0], [1.0, 1.0]], crowd])
+    rings = [np.vstack([outer, outer[:1]]), np.vstack([inner, inner[:1]]), np.vstack([spiky, spiky[:1]])]
+    xy = np.concatenate(rings) + np.array([-95.0, 38.0])
+    rvo = np.concatenate([[0], np.cumsum([len(r) for r in rings])])
+    return gsj.HostSet(np.array([0, 2, 3]), rvo, xy)
+
+
+@pytest.mark.parametrize("maxv", [12, 32])
+def test_subdivided_rings_relate(ctx, maxv):
+    # pieces with synthetic edges and cut points on outer and inner rings, so the per-vertex synthetic / cut flags
+    # wrap around every ring, joined with a parcel mesh over the same square and with themselves
+    H = gsj.subdivide(_holed_and_crowded(), maxv)
+    M = gsj.generate("parcels_mesh", nx=24, ny=24, x0=-95.0, y0=38.0, w=1 / 24, h=1 / 24, seed=3)
+    dH, dM = ctx.polys(H), ctx.polys(M)
+    oH, oM = O.Layer.polys(H), O.Layer.polys(M)
+    for dP, dR, oP, oR, kw in ((dM, dH, oM, oH, {}), (dH, dH, oH, oH, {"upper": True})):
+        got = ctx.join(dP, dR, want_area=True, want_relate=True, **kw)
+        exp = O.join(oP, oR, want_area=True, want_relate=True, **kw)
+        assert got.stats["candidates"] == exp["candidates"] > 0
+        _check_rows(got, exp, area=True, code=True)
+
+
 def test_intersects_only_and_regions_relate(ctx):
     P, R = _cfg1(60, segs=300, k=3)
     R = gsj.subdivide(R, 64)
