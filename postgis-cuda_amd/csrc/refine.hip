@@ -728,24 +728,104 @@ __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const 
   }
 }
 
-// overflow: one wave per pair, lists and partial sums in a per-wave slab of global memory, array path for every edge
-__global__ void __launch_bounds__(256, 2) k_refine_overflow(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
-                                                         const uint32_t* __restrict__ list, uint32_t nlist, RefineParams rp, RefineOut out, int cap,
-                                                         unsigned char* __restrict__ slab) {
-  const int wpb = blockDim.x / 64;
-  const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
+// overflow: a block of T threads per pair (T = 64: one wave), lists and partial sums in a per-block slab of global
+// memory, array path for every edge. The first wave collects the lists and runs the steps linear in the list sizes
+// (containment of c, the ring probes) and the ordered sums; the whole block runs the contact grid and the edge
+// contributions, whose work grows with the product of the list sizes. T is chosen per launch from the number of pairs: four
+// waves per pair when the pairs are too few to fill the GPU (a windowed request's few largest pairs), one wave per pair
+// otherwise. Every barrier sits in block-uniform control flow (the loop bounds and branches around them depend only on
+// the pair and on values the block has read back from shared memory after a barrier).
+template <int T>
+__global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                           const uint32_t* __restrict__ list, uint32_t nlist, RefineParams rp, RefineOut out, int cap,
+                                                           unsigned char* __restrict__ slab) {
+  __shared__ int s_na, s_nb, s_hit, s_hit_any, s_found, s_contain, s_inA, s_inB;   // s_hit: contact; s_found: contact or containment
+  __shared__ uint32_t s_bits;
+  const int t = threadIdx.x, gl = t & 63, w = t >> 6;
   const bool rel = rp.want_relate != 0;
-  const uint64_t wave = (uint64_t)blockIdx.x * wpb + wib;
-  const Scratch S = carve(slab + wave * list_bytes(cap, rel), cap, rel);
-  for (uint32_t k = (uint32_t)wave; k < nlist; k += gridDim.x * wpb) {
+  const Scratch S = carve(slab + (uint64_t)blockIdx.x * list_bytes(cap, rel), cap, rel);
+  for (uint32_t k = blockIdx.x; k < nlist; k += gridDim.x) {
     const uint32_t p = list[k];
     const uint32_t a = pa[p], b = pb[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    const int na = collect<64>(P, a, f.ashx, f.ashy, f.ov, S.LA, cap, gl);
-    const int nb = collect<64>(R, b, f.bshx, f.bshy, f.ov, S.LB, cap, gl);
-    wave_lds_sync();
-    refine_lists<64, true>(P, R, a, b, p, rp, f, S, min(na, cap), min(nb, cap), gl, out);
-    wave_lds_sync();
+    if (w == 0) {
+      const int na = collect<64>(P, a, f.ashx, f.ashy, f.ov, S.LA, cap, gl);
+      const int nb = collect<64>(R, b, f.bshx, f.bshy, f.ov, S.LB, cap, gl);
+      if (gl == 0) { s_na = min(na, cap); s_nb = min(nb, cap); s_hit = 0; s_hit_any = 0; s_contain = 0; s_bits = 0; }
+    }
+    __syncthreads();
+    const int na = s_na, nb = s_nb;
+    // boundary contact over the A x B edge grid (engine/engine.cuh:378-381): a real contact anywhere, else any contact
+    {
+      bool hit = false, hit_any = false;
+      for (int i = t; i < na && !hit; i += T) {
+        const Ed ea = S.LA[i];
+        const double ax0 = fmin(ea.x0, ea.x1), ax1 = fmax(ea.x0, ea.x1), ay0 = fmin(ea.y0, ea.y1), ay1 = fmax(ea.y0, ea.y1);
+        for (int j = 0; j < nb; j++) {
+          const Ed eb = S.LB[j];
+          if (fmax(eb.x0, eb.x1) < ax0 || fmin(eb.x0, eb.x1) > ax1 || fmax(eb.y0, eb.y1) < ay0 || fmin(eb.y0, eb.y1) > ay1) continue;
+          if (seg_isect(ea.x0, ea.y0, ea.x1, ea.y1, eb.x0, eb.y0, eb.x1, eb.y1)) {
+            hit_any = true;
+            if (!(ea.syn | eb.syn)) { hit = true; break; }
+          }
+        }
+      }
+      if (hit) atomicOr(&s_hit, 1);
+      if (hit_any) atomicOr(&s_hit_any, 1);
+    }
+    __syncthreads();
+    const bool rtouch = s_hit != 0;
+    // containment of c and the ring probes: the first wave
+    if (w == 0) {
+      bool hit = rtouch;
+      bool inB_c = false, inA_c = false;
+      if (!hit || rp.want_area || rel) {
+        inB_c = pip_full<64>(R, b, f.bshx, f.bshy, f.cx, f.cy, gl);
+        inA_c = pip_full<64>(P, a, f.ashx, f.ashy, f.cx, f.cy, gl);
+      }
+      bool contain_any = false;
+      if (!hit) {
+        const ListSrc LA{S.LA, na}, LB{S.LB, nb};
+        bool h = false, ha = false;
+        for (uint32_t r = P.gro[a] + gl; r < P.gro[a + 1] && !h; r += 64) {
+          if (P.rvo[r] == P.rvo[r + 1]) continue;
+          double x, y;
+          const bool real = probe_vertex(P, r, f.ashx, f.ashy, x, y);
+          if (!real) { const double2 v = P.xy[P.rvo[r]]; x = f.ashx + v.x; y = f.ashy + v.y; }
+          if (x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 && inside_from_c(LB, f.cx, f.cy, inB_c, x, y)) { ha = true; if (real) h = true; }
+        }
+        for (uint32_t r = R.gro[b] + gl; r < R.gro[b + 1] && !h; r += 64) {
+          if (R.rvo[r] == R.rvo[r + 1]) continue;
+          double x, y;
+          const bool real = probe_vertex(R, r, f.bshx, f.bshy, x, y);
+          if (!real) { const double2 v = R.xy[R.rvo[r]]; x = f.bshx + v.x; y = f.bshy + v.y; }
+          if (x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 && inside_from_c(LA, f.cx, f.cy, inA_c, x, y)) { ha = true; if (real) h = true; }
+        }
+        hit = gany<64>(h);
+        contain_any = gany<64>(ha);
+      }
+      if (gl == 0) { s_found = hit ? 1 : 0; s_contain = contain_any ? 1 : 0; s_inA = inA_c ? 1 : 0; s_inB = inB_c ? 1 : 0; }
+    }
+    __syncthreads();
+    const bool hit = s_found != 0;
+    const bool computed = (hit || s_hit_any != 0 || s_contain != 0) && (rp.want_area || rel);
+    if (computed) {
+      EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, s_inB != 0, rel}, cb = ca;
+      cb.in_c = s_inA != 0;
+      uint32_t bits = 0;
+      for (int s = t; s < na + nb; s += T) {
+        const bool sideA = s < na;
+        const Ed e = (sideA ? S.LA : S.LB)[sideA ? s : s - na];
+        const Contrib c = edge_contrib<ListSrc, true>(e, ListSrc{sideA ? S.LB : S.LA, sideA ? nb : na}, sideA ? ca : cb);
+        S.acc[s] = c.acc;
+        if (rel) { S.lin[s] = c.lin; S.lon[s] = c.lon; }
+        bits |= c.bits;
+      }
+      if (bits) atomicOr(&s_bits, bits);
+    }
+    __syncthreads();
+    if (w == 0) finish_pair<64>(gl, p, hit, rtouch, computed, rel, na + nb, na, S.acc, S.lin, S.lon, s_bits, rp, out);
+    __syncthreads();   // the lists and the shared votes are rewritten for the next pair
   }
 }
 
@@ -835,11 +915,13 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   if (nover) {
     const size_t per = list_bytes((int)need, rel);
     const uint64_t budget = 1ull << 30;
-    uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(nover, budget / std::max<size_t>(per, 1)));
-    waves = (waves + 3) / 4 * 4;
-    unsigned char* slab = c.s_heavy_edges.as<unsigned char>(per * waves + 64);
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(nover, budget / std::max<size_t>(per, 1)));   // one slab per block
+    unsigned char* slab = c.s_heavy_edges.as<unsigned char>(per * blocks + 64);
     GSJ_HIP(hipEventRecord(c.ev[3], st));
-    hipLaunchKernelGGL(k_refine_overflow, dim3((uint32_t)(waves / 4)), dim3(T), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
+    if (nover <= 2ull * c.cu_count)   // a few pairs: four waves each (2.4M relate: one wave each, 2.5 vs 4.5 ms)
+      hipLaunchKernelGGL(k_refine_overflow<256>, dim3((uint32_t)blocks), dim3(256), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
+    else
+      hipLaunchKernelGGL(k_refine_overflow<64>, dim3((uint32_t)blocks), dim3(64), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[4], st));
     if (stats) {
