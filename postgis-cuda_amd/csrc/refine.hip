@@ -10,8 +10,8 @@
 //        meets the overlap box), so the full-ring parity scans read LDS, not HBM; the contact test covers the whole
 //        A x B edge grid across the group; A's and B's edge contributions run in one combined pass;
 //      large: a wave64 per pair, compacted lists of the edges meeting the overlap box in LDS;
-//      overflow (lists beyond the LDS capacity): a wave64 per pair with lists in global memory sized by the largest
-//        overflow.
+//      overflow (lists beyond the LDS capacity): a wave64 per pair (four waves per pair when a launch holds few
+//        pairs) with lists in global memory sized by the largest overflow.
 //  * split points of an edge live in eight named registers kept sorted by a branch-free min/max insertion, with the
 //    overlaps in two register slots. A pair with an edge of more than six interior splits or two overlaps is handed to
 //    the overflow kernel, which runs the array path with the reference's caps (24 / 8); the two paths give the same
