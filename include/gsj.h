@@ -26,12 +26,12 @@ extern "C" {
 
 /* ------------------------------------------------------------------ status */
 #define GSJ_OK 0
-#define GSJ_E_ARG -1      /* invalid argument (null, size out of range, wrong layer kind) */
+#define GSJ_E_ARG -1      /* invalid argument (null, size out of range, wrong layer kind, malformed offsets or file contents) */
 #define GSJ_E_HIP -2      /* HIP runtime error (message carries hipGetErrorString) */
 #define GSJ_E_NOMEM -3    /* device or host allocation failed */
 #define GSJ_E_LIMIT -4    /* a documented limit was exceeded (geometries >= 2^31, rings >= 2^32, ...) */
 #define GSJ_E_CANCEL -5   /* the cancel flag was raised; partial results were delivered */
-#define GSJ_E_IO -6       /* file could not be read or written, or has a bad header */
+#define GSJ_E_IO -6       /* file could not be opened, read or written */
 #define GSJ_E_INTERNAL -7
 
 /* message of the last failing call on this thread ("" if none) */
@@ -272,6 +272,9 @@ int gsj_pset_read(const char* path, double** xy, uint64_t* n);   /* *xy freed wi
 int gsj_pset_write(const char* path, const double* xy, uint64_t n);
 void gsj_free(void* p);
 
+/* Input sets of the four preparation calls below: gro[n+1], rvo[gro[n]+1], xy[2 rvo[gro[n]]] (the lengths follow from
+ * the offsets, as in a .gset file). gro and rvo are checked first (start at 0, never decrease: GSJ_E_ARG otherwise);
+ * xy must hold the vertex count the offsets name. */
 /* ST_Subdivide analogue (engine/subdivide.h:71-129): pieces of at most maxv stored vertices (depth <= 24) */
 int gsj_subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv, gsj_hostset** out);
 /* exact dedup of identical geometries (engine/dedup.h:51-124, first-occurrence order): distinct set plus

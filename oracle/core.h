@@ -16,6 +16,17 @@
 #include <omp.h>
 #endif
 
+// Planted mistakes (tests only): a build with -DORC_MUTANT=k changes one rule of the restatement, so that a test can
+// show that its checks fail on that mistake. 0 (every shipped build) changes nothing.
+//   1  relate evidence: a real edge running along the other piece's cut edge counts its full length inside (not half)
+//   2  area: a sub-segment on a coincident boundary with interiors on one side weighs 1 (not 1/2)
+//   3  cut flags: the first edge of a ring takes its start-vertex cut flag from the closing vertex (off by one)
+//   4  piece -> parent fold: area of area-only piece pairs (flag 2) left out of the parent's sum
+//   5  coincidence: a cut endpoint gets no tolerance (rounded cut points must lie exactly on the other line)
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 namespace orc {
 
 // ------------------------------------------------------------------------------------------- arithmetic type
@@ -278,7 +289,7 @@ static void collect(const View& A, const Box& ov, std::vector<Edge>& out) {
       Edge ed;
       A.v(i, ed.x0, ed.y0); A.v(i + 1, ed.x1, ed.y1);
       ed.syn = A.syn(i);
-      ed.cut = A.L->has_syn() ? ((A.syn(i > s ? i - 1 : e - 2) ? 1 : 0) | (A.syn(i + 2 < e ? i + 1 : s) ? 2 : 0)) : 0;
+      ed.cut = A.L->has_syn() ? ((A.syn(i > s ? i - 1 : e - (ORC_MUTANT == 3 ? 1 : 2)) ? 1 : 0) | (A.syn(i + 2 < e ? i + 1 : s) ? 2 : 0)) : 0;
       if (!edge_outside(ed.x0, ed.y0, ed.x1, ed.y1, ov)) out.push_back(ed);
     }
   }

@@ -47,11 +47,13 @@ static Contrib edge_contrib(Real px, Real py, Real qx, Real qy, const std::vecto
     Real r = dx * fy - dy * fx;
     Real flen2 = fx * fx + fy * fy;
     Real c0 = orient(px, py, qx, qy, e.x0, e.y0), c1 = orient(px, py, qx, qy, e.x1, e.y1);
-    bool col = (c0 * c0 <= snap2 * len2 || ((e.cut & 1) && c0 * c0 <= tol2len2)) &&
-               (c1 * c1 <= snap2 * len2 || ((e.cut & 2) && c1 * c1 <= tol2len2));
+    const bool cut_tol = ORC_MUTANT != 5;
+    bool col = (c0 * c0 <= snap2 * len2 || (cut_tol && (e.cut & 1) && c0 * c0 <= tol2len2)) &&
+               (c1 * c1 <= snap2 * len2 || (cut_tol && (e.cut & 2) && c1 * c1 <= tol2len2));
     if (!col && flen2 > 0) {
       Real g0 = orient(e.x0, e.y0, e.x1, e.y1, px, py), g1 = orient(e.x0, e.y0, e.x1, e.y1, qx, qy), tf = tol * tol * flen2;
-      col = (g0 * g0 <= snap2 * flen2 || ((cut & 1) && g0 * g0 <= tf)) && (g1 * g1 <= snap2 * flen2 || ((cut & 2) && g1 * g1 <= tf));
+      col = (g0 * g0 <= snap2 * flen2 || (cut_tol && (cut & 1) && g0 * g0 <= tf)) &&
+            (g1 * g1 <= snap2 * flen2 || (cut_tol && (cut & 2) && g1 * g1 <= tf));
     }
     if (col) {
       Real tc = ((e.x0 - px) * dx + (e.y0 - py) * dy) / len2, te = ((e.x1 - px) * dx + (e.y1 - py) * dy) / len2;
@@ -89,12 +91,12 @@ static Contrib edge_contrib(Real px, Real py, Real qx, Real qy, const std::vecto
     if (rel && real) {
       Real seg = r_sqrt(len2) * (t1 - t0);
       if (net_real != 0) { out.lon += seg; out.bits |= (net_real > 0 ? REL_COIN_SAME : REL_COIN_OPP) | REL_TOUCH; }
-      else if (n_real == 0 && (net == 1 || net == -1)) out.lin += 0.5 * seg;
+      else if (n_real == 0 && (net == 1 || net == -1)) out.lin += (ORC_MUTANT == 1 ? 1.0 : 0.5) * seg;
       else if (inside_m()) out.lin += seg;
     }
     Real w;
     if (first < 0) w = inside_m() ? 1.0 : 0.0;
-    else if (net == 1 || net == -1) w = 0.5;
+    else if (net == 1 || net == -1) w = ORC_MUTANT == 2 ? 1.0 : 0.5;
     else if (net == 0) w = inside_m() ? 1.0 : 0.0;
     else {
       const Edge& eb = L[oidx[first]];
@@ -264,7 +266,8 @@ void join(const Layer& P, const Layer& R, const Params& rp, bool upper, bool no_
       double ar = 0, ev[4] = {0, 0, 0, 0}; uint32_t bits = 0; uint8_t f = 255;
       while (e < keyed.size() && keyed[e].first == keyed[s].first) {
         const PairOut& o = ref[keyed[e].second];
-        ar += o.area; for (int k = 0; k < 4; k++) ev[k] += o.ev[k];
+        if (ORC_MUTANT != 4 || o.isect == 1) ar += o.area;
+        for (int k = 0; k < 4; k++) ev[k] += o.ev[k];
         bits |= o.bits; f = std::min(f, o.isect);
         e++;
       }

@@ -9,7 +9,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# GSJ_ORACLE_LIB names another build of the same sources (the planted-mistake builds of the invariance tests)
+LIB_PATH = os.environ.get("GSJ_ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 FLOPS_LIB_PATH = os.path.join(_HERE, "build", "liboracle_flops.so")
 _lib = None
 _flib = None

@@ -384,6 +384,8 @@ int gsj_subdivide(const uint64_t* gro, const uint64_t* rvo, const double* xy, ui
   return guarded([&] {
     GSJ_REQUIRE(gro && rvo && out, GSJ_E_ARG, "null argument");
     GSJ_REQUIRE(maxv >= 4, GSJ_E_ARG, "subdivide: maxv must be >= 4");
+    gsj::check_offsets(gro, rvo, n);
+    GSJ_REQUIRE(xy || rvo[gro[n]] == 0, GSJ_E_ARG, "null xy");
     auto* h = new gsj_hostset();
     h->s = gsj::subdivide(gro, rvo, xy, n, maxv);
     *out = h;
@@ -394,6 +396,8 @@ int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64
               uint64_t* n_distinct) {
   return guarded([&] {
     GSJ_REQUIRE(gro && rvo && distinct && moff && midx && n_distinct, GSJ_E_ARG, "null argument");
+    gsj::check_offsets(gro, rvo, n);
+    GSJ_REQUIRE(xy || rvo[gro[n]] == 0, GSJ_E_ARG, "null xy");
     std::vector<uint64_t> mo;
     std::vector<uint32_t> mi;
     auto* h = new gsj_hostset();
@@ -408,6 +412,9 @@ int gsj_dedup(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64
 int gsj_subdivide_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, int maxv, gsj_hostset** out) {
   return guarded([&] {
     GSJ_REQUIRE(ctx && gro && rvo && out, GSJ_E_ARG, "null argument");
+    GSJ_REQUIRE(maxv >= 4, GSJ_E_ARG, "subdivide: maxv must be >= 4");
+    gsj::check_offsets(gro, rvo, n);
+    GSJ_REQUIRE(xy || rvo[gro[n]] == 0, GSJ_E_ARG, "null xy");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     set_device(ctx->c);
     auto* h = new gsj_hostset();
@@ -420,6 +427,8 @@ int gsj_dedup_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, con
                      uint32_t* midx, uint64_t* n_distinct) {
   return guarded([&] {
     GSJ_REQUIRE(ctx && gro && rvo && distinct && moff && midx && n_distinct, GSJ_E_ARG, "null argument");
+    gsj::check_offsets(gro, rvo, n);
+    GSJ_REQUIRE(xy || rvo[gro[n]] == 0, GSJ_E_ARG, "null xy");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     set_device(ctx->c);
     std::vector<uint64_t> mo;

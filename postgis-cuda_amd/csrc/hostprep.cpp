@@ -14,24 +14,68 @@
 namespace gsj {
 
 // ------------------------------------------------------------------------------------------- files
+// Malformed contents (header, sizes, offsets) throw std::invalid_argument (GSJ_E_ARG at the C ABI); failing file
+// operations throw std::runtime_error (GSJ_E_IO).
 // .gset: u32 'GSET', u32 version 1, u64 n, nr, nv, u64 gro[n+1], u64 rvo[nr+1], f64 xy[2 nv] (little endian)
 // (reference format: engine/gset.h:13-31, scripts/dump_gset.py:5-12)
+namespace {
+// size of an open file in bytes
+uint64_t file_size(FILE* f, const std::string& path) {
+  if (std::fseek(f, 0, SEEK_END) != 0) throw std::runtime_error("cannot seek " + path);
+  const long sz = std::ftell(f);
+  if (sz < 0 || std::fseek(f, 0, SEEK_SET) != 0) throw std::runtime_error("cannot seek " + path);
+  return (uint64_t)sz;
+}
+}  // namespace
+
+void check_offsets(const uint64_t* gro, const uint64_t* rvo, uint64_t n, uint64_t nr_max, uint64_t nv_max) {
+  if (!gro || !rvo) throw std::invalid_argument("null gro or rvo");
+  if (gro[0] != 0) throw std::invalid_argument("gro[0] must be 0");
+  for (uint64_t g = 0; g < n; g++)
+    if (gro[g + 1] < gro[g]) throw std::invalid_argument("gro must be non-decreasing (geometry " + std::to_string(g) + ")");
+  const uint64_t nr = gro[n];
+  if (nr > nr_max) throw std::invalid_argument("gro[n] exceeds the ring count");
+  if (rvo[0] != 0) throw std::invalid_argument("rvo[0] must be 0");
+  for (uint64_t r = 0; r < nr; r++)
+    if (rvo[r + 1] < rvo[r]) throw std::invalid_argument("rvo must be non-decreasing (ring " + std::to_string(r) + ")");
+  if (rvo[nr] > nv_max) throw std::invalid_argument("rvo[nr] exceeds the vertex count");
+}
+
 HostSet read_gset(const std::string& path) {
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) throw std::runtime_error("cannot open " + path);
   HostSet s;
-  char magic[4];
-  uint32_t ver = 0;
-  uint64_t hdr[3];
-  bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "GSET", 4) == 0 && std::fread(&ver, 4, 1, f) == 1 && std::fread(hdr, 8, 3, f) == 3;
-  if (!ok) { std::fclose(f); throw std::runtime_error("bad gset header in " + path); }
-  const uint64_t n = hdr[0], nr = hdr[1], nv = hdr[2];
-  s.gro.resize(n + 1); s.rvo.resize(nr + 1); s.xy.resize(2 * nv);
-  ok = std::fread(s.gro.data(), 8, n + 1, f) == n + 1 && std::fread(s.rvo.data(), 8, nr + 1, f) == nr + 1 &&
-       std::fread(s.xy.data(), 8, 2 * nv, f) == 2 * nv;
-  std::fclose(f);
-  if (!ok) throw std::runtime_error("short read in " + path);
-  if (s.gro[0] != 0 || s.gro[n] != nr || s.rvo[0] != 0 || s.rvo[nr] != nv) throw std::runtime_error("inconsistent offsets in " + path);
+  try {
+    const uint64_t size = file_size(f, path);
+    char magic[4];
+    uint32_t ver = 0;
+    uint64_t hdr[3];
+    const bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "GSET", 4) == 0 && std::fread(&ver, 4, 1, f) == 1 &&
+                    std::fread(hdr, 8, 3, f) == 3;
+    if (!ok) throw std::invalid_argument("bad gset header in " + path);
+    const uint64_t n = hdr[0], nr = hdr[1], nv = hdr[2];
+    // the three arrays must fill the rest of the file exactly: bound each count by the file size before any
+    // arithmetic on it, so that n + 1, nr + 1 and 2 nv cannot overflow and no allocation exceeds the file
+    const uint64_t body = size - 32;
+    if (size < 32 || n >= body / 8 || nr >= body / 8 || nv > body / 16 || (n + 1) * 8 + (nr + 1) * 8 + nv * 16 != body)
+      throw std::invalid_argument("gset header (n " + std::to_string(n) + ", nr " + std::to_string(nr) + ", nv " + std::to_string(nv) +
+                               ") does not match the file size " + std::to_string(size) + " of " + path);
+    s.gro.resize(n + 1); s.rvo.resize(nr + 1); s.xy.resize(2 * nv);
+    if (std::fread(s.gro.data(), 8, n + 1, f) != n + 1 || std::fread(s.rvo.data(), 8, nr + 1, f) != nr + 1 ||
+        std::fread(s.xy.data(), 8, 2 * nv, f) != 2 * nv)
+      throw std::runtime_error("short read in " + path);
+    std::fclose(f);
+    f = nullptr;
+    if (s.gro[n] != nr || s.rvo[nr] != nv) throw std::invalid_argument("inconsistent offsets in " + path);
+    try {
+      check_offsets(s.gro.data(), s.rvo.data(), n, nr, nv);
+    } catch (const std::invalid_argument& e) {
+      throw std::invalid_argument(std::string(e.what()) + " in " + path);
+    }
+  } catch (...) {
+    if (f) std::fclose(f);
+    throw;
+  }
   return s;
 }
 
@@ -50,15 +94,24 @@ void write_gset(const std::string& path, const uint64_t* gro, const uint64_t* rv
 std::vector<double> read_pset(const std::string& path) {
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) throw std::runtime_error("cannot open " + path);
-  char magic[4];
-  uint32_t ver;
-  uint64_t n = 0;
-  bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "PSET", 4) == 0 && std::fread(&ver, 4, 1, f) == 1 && std::fread(&n, 8, 1, f) == 1;
-  if (!ok) { std::fclose(f); throw std::runtime_error("bad pset header in " + path); }
-  std::vector<double> xy(2 * n);
-  ok = std::fread(xy.data(), 8, 2 * n, f) == 2 * n;
+  std::vector<double> xy;
+  try {
+    const uint64_t size = file_size(f, path);
+    char magic[4];
+    uint32_t ver;
+    uint64_t n = 0;
+    const bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "PSET", 4) == 0 && std::fread(&ver, 4, 1, f) == 1 &&
+                    std::fread(&n, 8, 1, f) == 1;
+    if (!ok) throw std::invalid_argument("bad pset header in " + path);
+    if (size < 16 || n > (size - 16) / 16 || n * 16 != size - 16)
+      throw std::invalid_argument("pset header (n " + std::to_string(n) + ") does not match the file size " + std::to_string(size) + " of " + path);
+    xy.resize(2 * n);
+    if (std::fread(xy.data(), 8, 2 * n, f) != 2 * n) throw std::runtime_error("short read in " + path);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
   std::fclose(f);
-  if (!ok) throw std::runtime_error("short read in " + path);
   return xy;
 }
 

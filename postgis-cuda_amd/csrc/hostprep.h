@@ -18,6 +18,11 @@ struct HostSet {
   uint64_t nv() const { return xy.size() / 2; }
 };
 
+// Offsets of a flat set: gro[0] = 0, gro non-decreasing over n + 1 entries, gro[n] <= nr_max, rvo[0] = 0, rvo
+// non-decreasing over gro[n] + 1 entries, rvo[gro[n]] <= nv_max. Throws std::invalid_argument otherwise. Every entry
+// point that takes caller arrays (files, subdivide, dedup, their device versions) checks them before reading further.
+void check_offsets(const uint64_t* gro, const uint64_t* rvo, uint64_t n, uint64_t nr_max = UINT64_MAX, uint64_t nv_max = UINT64_MAX);
+
 HostSet read_gset(const std::string& path);
 void write_gset(const std::string& path, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, uint64_t nr, uint64_t nv);
 std::vector<double> read_pset(const std::string& path);

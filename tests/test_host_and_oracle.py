@@ -194,6 +194,67 @@ def test_gset_pset_layout_and_round_trip():
             gsj.read_gset(os.path.join(d, "bad.gset"))
 
 
+def _gset_bytes(gro, rvo, xy, n=None, nr=None, nv=None):
+    gro, rvo, xy = np.asarray(gro, "<u8"), np.asarray(rvo, "<u8"), np.asarray(xy, "<f8")
+    hdr = struct.pack("<4sIQQQ", b"GSET", 1, len(gro) - 1 if n is None else n, len(rvo) - 1 if nr is None else nr,
+                      len(xy) // 2 if nv is None else nv)
+    return hdr + gro.tobytes() + rvo.tobytes() + xy.tobytes()
+
+
+def test_malformed_files_are_rejected_before_use():
+    """Header counts bounded by the file size (no overflow, no allocation beyond the file), offsets checked for
+    monotonicity and range: a corrupt or crafted file is an argument error, never an out-of-range read."""
+    sq = [[0, 0], [1, 0], [1, 1], [0, 1], [0, 0]]
+    good = _gset_bytes([0, 1], [0, 5], np.ravel(sq))
+    bad = {
+        "truncated": good[:-8],
+        "trailing bytes": good + b"\0" * 8,
+        "huge n": _gset_bytes([0, 1], [0, 5], np.ravel(sq), n=2**64 - 1),
+        "n past the file": _gset_bytes([0, 1], [0, 5], np.ravel(sq), n=1000),
+        "huge nv": _gset_bytes([0, 1], [0, 5], np.ravel(sq), nv=2**63),
+        "gro decreasing": _gset_bytes([0, 2, 1], [0, 5], np.ravel(sq + sq)[:10]),
+        "rvo decreasing": _gset_bytes([0, 2], [0, 4, 3], np.ravel(sq[:3])),
+        "rvo past nv": _gset_bytes([0, 1], [0, 7], np.ravel(sq)),
+        "gro[0] != 0": _gset_bytes([1, 1], [0, 5], np.ravel(sq)),
+        "header only": good[:20],
+    }
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "ok.gset")
+        open(p, "wb").write(good)
+        assert gsj.read_gset(p).n == 1
+        for name, raw in bad.items():
+            p = os.path.join(d, "bad.gset")
+            open(p, "wb").write(raw)
+            with pytest.raises(gsj.GsjError) as e:
+                gsj.read_gset(p)
+            assert e.value.code == -1, (name, str(e.value))
+        for name, raw in {"truncated": struct.pack("<4sIQ", b"PSET", 1, 3) + b"\0" * 40,
+                          "huge n": struct.pack("<4sIQ", b"PSET", 1, 2**62) + b"\0" * 16}.items():
+            p = os.path.join(d, "bad.pset")
+            open(p, "wb").write(raw)
+            with pytest.raises(gsj.GsjError) as e:
+                gsj.read_pset(p)
+            assert e.value.code == -1, (name, str(e.value))
+        with pytest.raises(gsj.GsjError) as e:
+            gsj.read_gset(os.path.join(d, "missing.gset"))
+        assert e.value.code == -6
+
+
+def test_prep_entry_points_check_offsets():
+    """gro / rvo must start at 0 and never decrease (the arrays keep their lengths, so every read stays in them)"""
+    M = mesh(6, seed=2)
+    rvo_swapped = M.rvo.copy()
+    rvo_swapped[[2, 3]] = rvo_swapped[[3, 2]]
+    gro_dip = M.gro.copy()
+    gro_dip[5] = gro_dip[4] - 1
+    for gro, rvo in ((M.gro[::-1].copy(), M.rvo), (gro_dip, M.rvo), (M.gro, rvo_swapped), (M.gro + 1, M.rvo)):
+        H = gsj.HostSet(gro, rvo, M.xy)
+        for f in (lambda: gsj.subdivide(H, 8), lambda: gsj.dedup(H)):
+            with pytest.raises(gsj.GsjError) as e:
+                f()
+            assert e.value.code == -1
+
+
 # ------------------------------------------------------------------------------------------- generators
 def test_generators_are_deterministic_and_match_their_spec():
     a, b = mesh(50, seed=9), mesh(50, seed=9)
