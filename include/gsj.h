@@ -131,6 +131,15 @@ typedef int (*gsj_rows_cb)(void* user, const uint32_t* pairs, const double* area
  * Replaces join_stream (engine/engine.cuh:895-1061). Rows are distinct parent pairs. */
 int gsj_join(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, const gsj_join_opts* opts, gsj_rows_cb cb, void* user,
              gsj_join_stats* stats);
+/* gsj_join without any allocation (no counterpart in the reference): for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it). The same kernels and rows as gsj_join; every scratch
+ * buffer is a slice of a fixed 128 MiB device array per GPU (held by each GPU the library runs on until the process
+ * ends) and a 16 MiB host staging region per GPU set up by gsj_ctx_create. Limits: max_pairs <= 131072 and
+ * batch_q <= 32768 (0 selects these; larger values fail with GSJ_E_LIMIT), no query window (GSJ_E_ARG), and a request
+ * whose sub-batch needs more scratch than the array holds fails with GSJ_E_LIMIT. Calls on one GPU share the array and
+ * are serialised by a per-GPU lock. */
+int gsj_join_noalloc(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R, const gsj_join_opts* opts, gsj_rows_cb cb, void* user,
+                     gsj_join_stats* stats);
 
 /* Points in polygons, boundary counts (replaces join_points_stream, engine/engine.cuh:1112-1178). Callback rows:
  * pairs (point index, parent polygon), where[i] = 1 interior, 2 boundary. */

@@ -37,6 +37,7 @@ int guarded_status(const std::function<void()>& f) {
 }
 
 Context::~Context() {
+  if (borrowed) return;
   for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -51,6 +52,7 @@ void context_init(Context& c, int device) {
   if (hipGetDeviceProperties(&p, device) == hipSuccess) c.cu_count = p.multiProcessorCount;
   GSJ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (hipEvent_t& e : c.ev) GSJ_HIP(hipEventCreate(&e));
+  noalloc_setup(device);
 }
 
 void layer_release(gsj_layer* layer) {
@@ -120,13 +122,7 @@ int gsj_ctx_trim(gsj_ctx* ctx) {
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     set_device(ctx->c);
     GSJ_HIP(hipStreamSynchronize(ctx->c.stream));
-    gsj::Context& c = ctx->c;
-    for (gsj::DevBuf* b : {&c.s_counts, &c.s_qoffs, &c.s_offs, &c.s_pairs_a, &c.s_pairs_b, &c.s_flag, &c.s_area, &c.s_ev, &c.s_evb, &c.s_tmp,
-                           &c.s_tmp2, &c.s_cls, &c.s_out_a, &c.s_out_b, &c.s_out_area, &c.s_out_code, &c.s_out_flag, &c.s_keys, &c.s_keys2,
-                           &c.s_vals, &c.s_vals2, &c.s_seg, &c.s_heavy, &c.s_heavy_edges, &c.s_scal, &c.s_qbb, &c.s_uf, &c.s_root, &c.s_misc,
-                           &c.s_win, &c.s_qidx, &c.s_win_off, &c.s_edgeq, &c.s_qlist, &c.s_wtile, &c.s_wtoff, &c.s_wseg, &c.s_slots, &c.s_slots_ctl,
-                           &c.s_er_keep, &c.s_er_off, &c.s_er_pairs, &c.s_er_flag, &c.s_er_list, &c.s_er_slab, &c.s_er_out, &c.s_er_need, &c.s_er_pf})
-      b->release();
+    ctx->c.for_each_devbuf([](gsj::DevBuf& b) { b.release(); });
   });
 }
 

@@ -21,4 +21,6 @@ int guarded_status(const std::function<void()>& f);
 void context_init(Context& c, int device);
 // drop one reference; the last one frees the layer (after its owner context's stream drains)
 void layer_release(gsj_layer* layer);
+// once per device, with it current: the address of the no-allocation entry point's scratch on it (noalloc.hip)
+void noalloc_setup(int device);
 }  // namespace gsj

@@ -1,5 +1,6 @@
 // Engine-internal plumbing: status codes, HIP error checks, grow-only device scratch, stream timers.
 #pragma once
+#include <algorithm>
 #include <chrono>
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -31,13 +32,38 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::gsj::Error(code, msg); \
   } while (0)
 
+// Bump allocator over memory owned elsewhere (the no-allocation entry point's scratch). A buffer bound to an arena
+// takes its storage from it and never allocates or frees; reset() hands the whole region out again.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t cap = 0, used = 0;
+  const char* name = "scratch";
+  size_t left() const { const size_t off = (used + 255) & ~size_t(255); return off >= cap ? 0 : cap - off; }
+  void* take(size_t n) {
+    const size_t off = (used + 255) & ~size_t(255);
+    if (off > cap || n > cap - off)
+      throw Error(GSJ_E_LIMIT, std::string("no-allocation ") + name + " of " + std::to_string(cap >> 20) + " MiB exhausted (" +
+                                   std::to_string((off + n) >> 20) + " MiB needed): lower batch_q / max_pairs or use the allocating entry point");
+    used = off + n;
+    return base + off;
+  }
+  void reset() { used = 0; }
+};
+
 // grow-only device buffer; contents are not preserved on growth
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  Arena* arena = nullptr;   // set: storage comes from the arena (no hipMalloc / hipFree)
   void* get(size_t n) {
     if (n == 0) n = 1;
     if (n > cap) {
+      if (arena) {   // the old region stays taken until the arena is reset; double to bound that waste
+        const size_t want = std::max(n, 2 * cap) <= arena->left() ? std::max(n, 2 * cap) : n;
+        p = arena->take(want);
+        cap = want;
+        return p;
+      }
       if (p) GSJ_HIP(hipFree(p));
       p = nullptr; cap = 0;
       size_t want = n + n / 8;   // headroom so slowly growing requests do not reallocate every call
@@ -47,20 +73,28 @@ struct DevBuf {
     return p;
   }
   template <class T> T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  void release() { if (p && !arena) (void)hipFree(p); p = nullptr; cap = 0; }
   ~DevBuf() { release(); }
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
-// grow-only pinned host buffer
+// grow-only pinned host buffer (bound to an arena: plain host memory from it; copies into it are then staged by the
+// runtime, and every use of these buffers synchronises the stream before reading them)
 struct PinBuf {
   void* p = nullptr;
   size_t cap = 0;
+  Arena* arena = nullptr;
   void* get(size_t n) {
     if (n == 0) n = 1;
     if (n > cap) {
+      if (arena) {
+        const size_t want = std::max(n, 2 * cap) <= arena->left() ? std::max(n, 2 * cap) : n;
+        p = arena->take(want);
+        cap = want;
+        return p;
+      }
       if (p) GSJ_HIP(hipHostFree(p));
       p = nullptr; cap = 0;
       size_t want = n + n / 8;
@@ -70,7 +104,7 @@ struct PinBuf {
     return p;
   }
   template <class T> T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
-  void release() { if (p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+  void release() { if (p && !arena) (void)hipHostFree(p); p = nullptr; cap = 0; }
   ~PinBuf() { release(); }
   PinBuf() = default;
   PinBuf(const PinBuf&) = delete;

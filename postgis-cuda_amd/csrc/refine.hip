@@ -914,7 +914,9 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   }
   if (nover) {
     const size_t per = list_bytes((int)need, rel);
-    const uint64_t budget = 1ull << 30;
+    // slabs of at most 1 GiB (half of what is left when the scratch is a fixed arena: compaction still needs room)
+    const uint64_t budget = c.dev_arena ? std::min<uint64_t>(1ull << 30, c.dev_arena->left() / 2) : (1ull << 30);
+    GSJ_REQUIRE(!c.dev_arena || per <= budget, GSJ_E_LIMIT, "overflow pair lists exceed the no-allocation scratch left for them");
     const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(nover, budget / std::max<size_t>(per, 1)));   // one slab per block
     unsigned char* slab = c.s_heavy_edges.as<unsigned char>(per * blocks + 64);
     GSJ_HIP(hipEventRecord(c.ev[3], st));

@@ -78,7 +78,7 @@ _lib = None
 EXPORTS = [
     "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
-    "gsj_join", "gsj_points_in_polygons", "gsj_cluster", "gsj_cluster_rows", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
+    "gsj_join", "gsj_join_noalloc", "gsj_points_in_polygons", "gsj_cluster", "gsj_cluster_rows", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
     "gsj_points_intersects_buffered",
     "gsj_sched_create", "gsj_sched_destroy", "gsj_sched_set_batch", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
@@ -105,6 +105,7 @@ def load(path: str | None = None):
         "gsj_layer_create_points": (i32, [vp, dp, u64, C.POINTER(vp)]), "gsj_layer_free": (None, [vp]), "gsj_bvh_build": (i32, [vp]),
         "gsj_layer_info_get": (i32, [vp, C.POINTER(LayerInfo)]), "gsj_layer_export": (i32, [vp, dp, dp, fp, dp, dp]),
         "gsj_join": (i32, [vp, vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
+        "gsj_join_noalloc": (i32, [vp, vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_points_in_polygons": (i32, [vp, vp, vp, C.POINTER(JoinOpts), POINTS_CB, vp, C.POINTER(JoinStats)]),
         "gsj_cluster": (i32, [vp, vp, C.POINTER(JoinOpts), u32p, u64, C.POINTER(JoinStats)]),
         "gsj_cluster_rows": (i32, [vp, vp, C.POINTER(JoinOpts), ROWS_CB, vp, C.POINTER(JoinStats)]),
@@ -435,8 +436,9 @@ class Context:
         self._layers.add(L)
         return L.build_bvh() if build_bvh else L
 
-    def join(self, P: Layer, R: Layer, collect=True, **kw) -> Rows:
-        """Rows of the join; collect=False delivers them to a no-op callback (counts and stats only)."""
+    def join(self, P: Layer, R: Layer, collect=True, noalloc=False, **kw) -> Rows:
+        """Rows of the join; collect=False delivers them to a no-op callback (counts and stats only); noalloc=True runs
+        gsj_join_noalloc (fixed scratch, no allocation; same rows)."""
         o = _opts(**kw)
         chunks = []
 
@@ -450,7 +452,8 @@ class Context:
             return 0
 
         st = JoinStats()
-        _check(load().gsj_join(self.h, P.h, R.h, C.byref(o), ROWS_CB(cb), None, C.byref(st)))
+        entry = load().gsj_join_noalloc if noalloc else load().gsj_join
+        _check(entry(self.h, P.h, R.h, C.byref(o), ROWS_CB(cb), None, C.byref(st)))
         pairs = np.concatenate([c[0] for c in chunks]) if chunks else np.zeros((0, 2), np.uint32)
         area = np.concatenate([c[1] for c in chunks]) if chunks and o.want_area else (np.zeros(0) if o.want_area else None)
         code = np.concatenate([c[2] for c in chunks]) if chunks and o.want_relate else (np.zeros(0, np.uint32) if o.want_relate else None)
