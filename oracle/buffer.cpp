@@ -44,11 +44,8 @@ static DwOut dw_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t b, dou
   std::vector<Edge> LA, LB;
   collect(A, ov, LA);
   collect(B, ov, LB);
-  const double phi0 = oay + 0.5 * (val(ov.y0) + val(ov.y1));
-  double kx, ky;
-  geo::m_per_deg(phi0, kx, ky);
-  const double tl = std::fabs(gsj::gm::tan(phi0 * geo::DEG)) + 0.01;
-  const double delta = 2.0 * 0.5 * (val(ov.y1) - val(ov.y0)) * geo::DEG * (tl + 0.04) + 1e-9;
+  const geo::Frame F = geo::frame_of(oay + val(ov.y0), oay + val(ov.y1));   // metric frame of the pair's window
+  const double kx = F.kx, ky = F.ky;
   bool hit = false, cross = false;
   double m_up = INFINITY;
   // per real edge pair near each other: crossing, frame distance and band (the reference's hit loop, run to the end)
@@ -65,10 +62,9 @@ static DwOut dw_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t b, dou
       const double fx0 = val(f.x0), fy0 = val(f.y0), fx1 = val(f.x1), fy1 = val(f.y1);
       if (f.syn || std::fmax(fx0, fx1) < bx0 || std::fmin(fx0, fx1) > bx1 || std::fmax(fy0, fy1) < by0 || std::fmin(fy0, fy1) > by1) continue;
       if (seg_isect(ex0, ey0, ex1, ey1, fx0, fy0, fx1, fy1)) { hit = cross = true; continue; }
-      double px, py, qx, qy;
-      const double d2 = geo::seg_seg_d2(ex0 * kx, ey0 * ky, ex1 * kx, ey1 * ky, fx0 * kx, fy0 * ky, fx1 * kx, fy1 * ky, px, py, qx, qy);
+      const double d2 = geo::seg_seg_d2(ex0 * kx, ey0 * ky, ex1 * kx, ey1 * ky, fx0 * kx, fy0 * ky, fx1 * kx, fy1 * ky);
       const double vx = (fx1 - fx0) * kx, vy = (fy1 - fy0) * ky;
-      const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+      const double band = geo::geog_band(d, F, La2, vx * vx + vy * vy);
       const double dlo = d - band, dhi = d + band;
       if (d2 <= dlo * dlo) hit = true;
       else if (d2 <= dhi * dhi &&
@@ -176,11 +172,9 @@ static DwOut pt_dwithin(const Layer& pts, const Layer& R, uint32_t pi, uint32_t 
   const double ptx = pts.xw[2 * pi], pty = pts.xw[2 * pi + 1], ox_ = R.ox[g], oy_ = R.oy[g];
   View B{&R, g, ox_ - ox_, oy_ - oy_};
   const double px = ptx - ox_, py = pty - oy_;
-  double kx, ky;
-  geo::m_per_deg(pty, kx, ky);
-  const double gy = d / ky;
-  const double tl = std::fabs(gsj::gm::tan(pty * geo::DEG)) + 0.01;
-  const double delta = 2.0 * gy * geo::DEG * (tl + 0.04) + 1e-9;
+  const double reach = d / geo::KY_MIN;   // latitude the closest boundary point can be from the point
+  const geo::Frame F = geo::frame_of(pty - reach, pty + reach);
+  const double kx = F.kx, ky = F.ky;
   const double ox = val(B.shx) - 1.0;
   bool par = false, h = false;
   double m_up = INFINITY;
@@ -194,10 +188,9 @@ static DwOut pt_dwithin(const Layer& pts, const Layer& R, uint32_t pi, uint32_t 
       Real rx1, ry1; B.v(j, rx1, ry1);
       const double x1 = val(rx1), y1 = val(ry1);
       if (!B.syn(j - 1)) {
-        double qx, qy;
-        const double d2 = geo::pt_seg_d2(px * kx, py * ky, x0 * kx, y0 * ky, x1 * kx, y1 * ky, qx, qy);
+        const double d2 = geo::pt_seg_d2(px * kx, py * ky, x0 * kx, y0 * ky, x1 * kx, y1 * ky);
         const double ex = (x1 - x0) * kx, ey = (y1 - y0) * ky;
-        const double band = geo::geog_band(d, delta, ex * ex + ey * ey, 0.0, tl), dlo = d - band, dhi = d + band;
+        const double band = geo::geog_band(d, F, ex * ex + ey * ey, 0.0), dlo = d - band, dhi = d + band;
         if (d2 <= dlo * dlo || (d2 <= dhi * dhi && geo::geog_dist(ox_ + x0, oy_ + y0, ox_ + x1, oy_ + y1, ptx, pty, ptx, pty, true) <= d)) h = true;
         if (report) {
           const double fr = std::sqrt(d2);
@@ -755,6 +748,15 @@ void orc_geo_m_per_deg(double lat, double* kx, double* ky) { geo::m_per_deg(lat,
 void orc_geo_deg_cover(double y0, double y1, double d, int larger, double* dx, double* dy) { geo::deg_cover(y0, y1, d, larger != 0, *dx, *dy); }
 double orc_geo_utm_k(double glon, double glat, double lon, double lat) { return geo::utm_k(glon, glat, lon, lat); }
 double orc_geo_vincenty(double lon1, double lat1, double lon2, double lat2) { return geo::vincenty_m(lon1, lat1, lon2, lat2); }
+// frame of a window's latitude range: out = {kx, ky, rel, curv}; band of a distance d for two edges in that frame
+void orc_geo_frame(double lat_lo, double lat_hi, double* out) {
+  const geo::Frame F = geo::frame_of(lat_lo, lat_hi);
+  out[0] = F.kx; out[1] = F.ky; out[2] = F.rel; out[3] = F.curv;
+}
+double orc_geo_band(double lat_lo, double lat_hi, double d, double La2, double Lb2) { return geo::geog_band(d, geo::frame_of(lat_lo, lat_hi), La2, Lb2); }
+double orc_geo_seg_seg_d2(double a0x, double a0y, double a1x, double a1y, double b0x, double b0y, double b1x, double b1y) {
+  return geo::seg_seg_d2(a0x, a0y, a1x, a1y, b0x, b0y, b1x, b1y);
+}
 double orc_geo_geog_dist(double a0x, double a0y, double a1x, double a1y, double b0x, double b0y, double b1x, double b1y, int b_is_point) {
   return geo::geog_dist(a0x, a0y, a1x, a1y, b0x, b0y, b1x, b1y, b_is_point != 0);
 }

@@ -50,6 +50,7 @@ def load(flops=False):
         "orc_gm_sin": (d, [d]), "orc_gm_cos": (d, [d]), "orc_gm_tan": (d, [d]), "orc_gm_atan": (d, [d]), "orc_gm_atan2": (d, [d, d]),
         "orc_geo_m_per_deg": (None, [d, dp, dp]), "orc_geo_deg_cover": (None, [d, d, d, i32, dp, dp]),
         "orc_geo_utm_k": (d, [d] * 4), "orc_geo_vincenty": (d, [d] * 4), "orc_geo_geog_dist": (d, [d] * 8 + [i32]),
+        "orc_geo_frame": (None, [d, d, dp]), "orc_geo_band": (d, [d] * 5), "orc_geo_seg_seg_d2": (d, [d] * 8),
         "orc_dw_pair": (i32, [vp, vp, u32, u32, d, i32, dp]),
         "orc_dw_new": (vp, []), "orc_dw_free": (None, [vp]), "orc_dw_rows": (u64, [vp]), "orc_dw_candidates": (u64, [vp]),
         "orc_dw_copy": (None, [vp, u32p, dp]), "orc_dwithin": (None, [vp, vp, d, i32, i32, i32, vp]),
@@ -320,6 +321,21 @@ def utm_k(glon, glat, lon, lat):
 
 def vincenty(lon1, lat1, lon2, lat2):
     return load().orc_geo_vincenty(lon1, lat1, lon2, lat2)
+
+
+def geo_frame(lat_lo, lat_hi):
+    """metric frame of a window's latitude range: (kx, ky, rel, curv)"""
+    out = np.zeros(4)
+    load().orc_geo_frame(lat_lo, lat_hi, _p(out, C.c_double))
+    return tuple(out.tolist())
+
+
+def geog_band(lat_lo, lat_hi, d, La2, Lb2=0.0):
+    return float(load().orc_geo_band(lat_lo, lat_hi, d, La2, Lb2))
+
+
+def seg_seg_d2(a0, a1, b0, b1):
+    return float(load().orc_geo_seg_seg_d2(*a0, *a1, *b0, *b1))
 
 
 def geog_dist(a0, a1, b0, b1=None):

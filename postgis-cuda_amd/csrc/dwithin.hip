@@ -98,12 +98,8 @@ __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b,
   const int nb = collect<G>(R, b, sx, sy, ov, LB, cap, gl);
   if (na > cap || nb > cap) { need = max(na, nb); return -1; }
   wave_lds_sync();
-  const double phi0 = oa.y + 0.5 * (ov.y0 + ov.y1);
-  double kx, ky;
-  geo::m_per_deg(phi0, kx, ky);
-  const double tl = fabs(gm::tan(phi0 * geo::DEG)) + 0.01;
-  // relative error of the frame metric over the window (scale varies with latitude), with a 2x margin
-  const double delta = 2.0 * 0.5 * (ov.y1 - ov.y0) * geo::DEG * (tl + 0.04) + 1e-9;
+  const geo::Frame F = geo::frame_of(oa.y + ov.y0, oa.y + ov.y1);   // metric frame of the pair's window
+  const double kx = F.kx, ky = F.ky;
   bool hit = false, cross = false;
   double m_up = INFINITY;
   for (int i = gl; i < na && (report || !hit); i += G) {
@@ -122,10 +118,9 @@ __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b,
           if (!report) break;
           continue;
         }
-        double px, py, qx, qy;
-        const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
+        const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky);
         const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
-        const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+        const double band = geo::geog_band(d, F, La2, vx * vx + vy * vy);
         const double dlo = d - band, dhi = d + band;
         if (d2 <= dlo * dlo) hit = true;
         else if (d2 <= dhi * dhi &&
@@ -173,10 +168,9 @@ __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b,
             const int j = base + __builtin_ctzll(cand);
             cand &= cand - 1;
             const Ed f = LB[j];
-            double px, py, qx, qy;
-            const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky, px, py, qx, qy);
+            const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky);
             const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
-            const double band = geo::geog_band(d, delta, La2, vx * vx + vy * vy, tl);
+            const double band = geo::geog_band(d, F, La2, vx * vx + vy * vy);
             if (sqrt(d2) - band > m) continue;
             best = fmin(best, geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1,
                                              false));
@@ -281,11 +275,9 @@ __global__ void __launch_bounds__(256) k_pt_dwithin(const DPoints pts, const DPo
   const uint32_t g = pb[i];
   const double2 pt = pts.xy[pa[i]], o = R.origin[g];
   const double px = pt.x - o.x, py = pt.y - o.y;
-  double kx, ky;
-  geo::m_per_deg(pt.y, kx, ky);
-  const double gy = d / ky;
-  const double tl = fabs(gm::tan(pt.y * geo::DEG)) + 0.01;
-  const double delta = 2.0 * gy * geo::DEG * (tl + 0.04) + 1e-9;
+  const double reach = d / geo::KY_MIN;   // latitude the closest boundary point can be from the point
+  const geo::Frame F = geo::frame_of(pt.y - reach, pt.y + reach);
+  const double kx = F.kx, ky = F.ky;
   const double ox = (o.x - o.x) - 1.0;
   bool par = false, h = false;
   double m_up = INFINITY;
@@ -297,10 +289,9 @@ __global__ void __launch_bounds__(256) k_pt_dwithin(const DPoints pts, const DPo
       const double2 v1 = R.xy[j];
       const double x0 = v0.x, y0 = v0.y, x1 = v1.x, y1 = v1.y;
       if (!syn_bit(R.synb, j - 1)) {
-        double qx, qy;
-        const double d2 = geo::pt_seg_d2(px * kx, py * ky, x0 * kx, y0 * ky, x1 * kx, y1 * ky, qx, qy);
+        const double d2 = geo::pt_seg_d2(px * kx, py * ky, x0 * kx, y0 * ky, x1 * kx, y1 * ky);
         const double ex = (x1 - x0) * kx, ey = (y1 - y0) * ky;
-        const double band = geo::geog_band(d, delta, ex * ex + ey * ey, 0.0, tl), dlo = d - band, dhi = d + band;
+        const double band = geo::geog_band(d, F, ex * ex + ey * ey, 0.0), dlo = d - band, dhi = d + band;
         if (d2 <= dlo * dlo || (d2 <= dhi * dhi && geo::geog_dist(o.x + x0, o.y + y0, o.x + x1, o.y + y1, pt.x, pt.y, pt.x, pt.y, true) <= d)) h = true;
         if (report) m_up = fmin(m_up, sqrt(d2) + band);
       }
@@ -325,10 +316,9 @@ __global__ void __launch_bounds__(256) k_pt_dwithin(const DPoints pts, const DPo
     for (uint64_t j = s0 + 1; j < e; j++) {
       if (syn_bit(R.synb, j - 1)) continue;
       const double2 v0 = R.xy[j - 1], v1 = R.xy[j];
-      double qx, qy;
-      const double d2 = geo::pt_seg_d2(px * kx, py * ky, v0.x * kx, v0.y * ky, v1.x * kx, v1.y * ky, qx, qy);
+      const double d2 = geo::pt_seg_d2(px * kx, py * ky, v0.x * kx, v0.y * ky, v1.x * kx, v1.y * ky);
       const double ex = (v1.x - v0.x) * kx, ey = (v1.y - v0.y) * ky;
-      const double band = geo::geog_band(d, delta, ex * ex + ey * ey, 0.0, tl);
+      const double band = geo::geog_band(d, F, ex * ex + ey * ey, 0.0);
       if (sqrt(d2) - band > m_up) continue;
       best = fmin(best, geo::geog_dist(o.x + v0.x, o.y + v0.y, o.x + v1.x, o.y + v1.y, pt.x, pt.y, pt.x, pt.y, true));
     }

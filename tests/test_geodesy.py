@@ -87,13 +87,17 @@ def test_metres_per_degree_from_the_radii_of_curvature():
 
 
 def test_degree_cover_covers_and_is_covered():
+    """The widened cover holds d metres at every latitude up to d metres beyond the range (d over the smallest metres
+    per degree of latitude, a (1 - e^2) pi / 180 at the equator); the shrunk cover is held by d at every latitude of
+    the range; and a geodesic of length d from any point of the range ends inside the widened box."""
     rng = random.Random(3)
+    ky_min = A * (1 - F * (2 - F)) * math.pi / 180
     for _ in range(200):
         y0 = rng.uniform(-80, 80)
         y1 = y0 + rng.uniform(0, 0.5)
         d = 10 ** rng.uniform(-1, 4)
         dx, dy = O.deg_cover(y0, y1, d, True)
-        pad = d / 110000.0
+        pad = d / ky_min
         for lat in np.linspace(y0 - pad, y1 + pad, 41):
             kx, ky = O.m_per_deg(float(lat))
             assert dx * kx >= d and dy * ky >= d
@@ -102,6 +106,63 @@ def test_degree_cover_covers_and_is_covered():
             kx, ky = O.m_per_deg(float(lat))
             assert sx * kx <= d and sy * ky <= d
     assert O.deg_cover(10.0, 11.0, 0.0, True) == (0.0, 0.0)
+    assert O.deg_cover(89.995, 89.999, 5.0, True)[0] == 360.0   # a range at a pole: every longitude
+    # end points of geodesics of length just under d leave the start's latitude range by less than the cover
+    for _ in range(300):
+        lat, lon = rng.uniform(-75, 75), rng.uniform(-170, 170)
+        d = 10 ** rng.uniform(0, 4)
+        dx, dy = O.deg_cover(lat, lat, d, True)
+        ang = rng.uniform(0, 2 * math.pi)
+        # walk outwards along a bearing until Vincenty says d: the end lies within (dx, dy) of the start
+        lo_, hi_ = 0.0, 3 * d / ky_min
+        for _ in range(60):
+            t = 0.5 * (lo_ + hi_)
+            q = (lon + t * math.cos(ang) / max(math.cos(math.radians(lat)), 1e-3), lat + t * math.sin(ang))
+            if O.vincenty(lon, lat, q[0], q[1]) < d:
+                lo_ = t
+            else:
+                hi_ = t
+        q = (lon + lo_ * math.cos(ang) / max(math.cos(math.radians(lat)), 1e-3), lat + lo_ * math.sin(ang))
+        assert abs(q[0] - lon) <= dx and abs(q[1] - lat) <= dy
+
+
+def test_frame_band_holds_the_geography_distance():
+    """Near the threshold the frame distance of two straight lon/lat edges (metres per degree at the window's middle
+    latitude) is within the band of the geography distance of the same edges (great-circle arcs, Vincenty between the
+    closest points): random edge pairs of 1-300 m at latitudes up to 70 degrees, windows as the distance join builds
+    them. The band is also not wasteful: within 1e-3 of d plus the edges' curvature term."""
+    rng = random.Random(17)
+    worst = 0.0
+    for _ in range(3000):
+        lat, lon = rng.uniform(-70, 70), rng.uniform(-170, 170)
+        kx, ky = O.m_per_deg(lat)
+        d = 10 ** rng.uniform(0, 2.7)
+
+        def edge(cx, cy):
+            L, ang = 10 ** rng.uniform(0, 2.5), rng.uniform(0, math.pi)
+            return ((cx - 0.5 * L * math.cos(ang) / kx, cy - 0.5 * L * math.sin(ang) / ky),
+                    (cx + 0.5 * L * math.cos(ang) / kx, cy + 0.5 * L * math.sin(ang) / ky))
+
+        a0, a1 = edge(lon, lat)
+        g = d * rng.uniform(0.8, 1.2) + 0.6 * 10 ** rng.uniform(0, 2.5)
+        ang = rng.uniform(0, 2 * math.pi)
+        b0, b1 = edge(lon + g * math.cos(ang) / kx, lat + g * math.sin(ang) / ky)
+        if O.seg_isect(*a0, *a1, *b0, *b1):
+            continue
+        ys = [a0[1], a1[1], b0[1], b1[1]]
+        lo_, hi_ = min(ys), max(ys)
+        fkx, fky, rel, curv = O.geo_frame(lo_, hi_)
+        ox, oy = a0   # the join measures in the frame of the first geometry's origin
+        fr = math.sqrt(O.seg_seg_d2(((a0[0] - ox) * fkx, (a0[1] - oy) * fky), ((a1[0] - ox) * fkx, (a1[1] - oy) * fky),
+                                    ((b0[0] - ox) * fkx, (b0[1] - oy) * fky), ((b1[0] - ox) * fkx, (b1[1] - oy) * fky)))
+        La2 = ((a1[0] - a0[0]) * fkx) ** 2 + ((a1[1] - a0[1]) * fky) ** 2
+        Lb2 = ((b1[0] - b0[0]) * fkx) ** 2 + ((b1[1] - b0[1]) * fky) ** 2
+        band = O.geog_band(lo_, hi_, fr, La2, Lb2)
+        geo = O.geog_dist(a0, a1, b0, b1)
+        worst = max(worst, abs(fr - geo) / band)
+        assert abs(fr - geo) <= band, (lat, d, fr, geo, band)
+        assert band <= 1e-3 * fr + 2 * (La2 + Lb2) * curv + 1e-5
+    assert worst > 0.01   # the pins are not vacuous: the two distances do differ at the band's scale
 
 
 def _vincenty_mp(lon1, lat1, lon2, lat2):
