@@ -1,14 +1,16 @@
 // gpujoind: the spatial join daemon on libgsj (reference: daemon/gpujoind.cu). Keeps named layers resident in HBM
 // and serves the gpujoind wire protocol (protocol.h) over a unix socket and optionally TCP.
 //
-//   gpujoind [--socket PATH] [--tcp HOST:PORT] [--token-file FILE] [--lanes N] [--threads N] [--data-dir DIR]
-//            [--device D] [--scratch-mb MB]
+//   gpujoind [--socket PATH] [--socket-mode OCTAL] [--tcp HOST:PORT] [--token-file FILE] [--lanes N] [--threads N]
+//            [--data-dir DIR] [--device D] [--scratch-mb MB] [--max-rows N] [--batch N]
 //
 // Differences from the reference: requests do not serialise on a global GPU mutex; each connection's request goes to
 // libgsj's scheduler (gsj_submit), whose lanes admit requests as they arrive and run them concurrently on the device
-// (continuous batching). Rows are encoded and sent from the lane's callback, in parallel (OpenMP), as frames of up
-// to ~1 MiB. A client that goes away cancels its request at the next sub-batch. LOAD reads layer files or
-// generators (there is no Postgres here; see protocol.h).
+// (continuous batching). Rows are encoded from the lane's callback on several threads (OpenMP) and leave as frames of
+// at most 1 MiB of rows, written straight from the encoded chunks. A client that goes away cancels its request at the
+// next sub-batch; --max-rows ends a stream that would exceed it with a GSP_FRAME_ERR frame. LOAD reads layer files
+// under --data-dir (paths that resolve outside it are refused) or generators (there is no Postgres here; see
+// protocol.h).
 #include "../../include/gsj.h"
 #include "protocol.h"
 
@@ -19,6 +21,7 @@
 #include <poll.h>
 #include <sys/socket.h>
 #include <sys/stat.h>
+#include <sys/uio.h>
 #include <sys/un.h>
 #include <unistd.h>
 
@@ -26,6 +29,8 @@
 #include <unordered_map>
 #include <atomic>
 #include <cerrno>
+#include <climits>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <csignal>
@@ -50,7 +55,9 @@ double secs_since(Clock::time_point t) { return std::chrono::duration<double>(Cl
 
 std::string g_token;
 std::string g_data_dir = ".";
+std::string g_data_real;   // its real path (set at start)
 int g_threads = 16;
+uint64_t g_max_rows = 0;   // 0: no limit on the rows of one stream
 gsj_ctx* g_ctx = nullptr;
 gsj_sched* g_sched = nullptr;
 
@@ -137,7 +144,16 @@ std::shared_ptr<Layer> get_layer(const std::string& n) {
   return it == g_layers.end() ? nullptr : it->second;
 }
 
-std::string resolve(const std::string& p) { return (!p.empty() && p[0] == '/') ? p : g_data_dir + "/" + p; }
+// a layer file named by a LOAD: relative to --data-dir, and after resolving links and "..", still inside it
+std::string resolve(const std::string& p) {
+  const std::string full = (!p.empty() && p[0] == '/') ? p : g_data_dir + "/" + p;
+  char real[PATH_MAX];
+  if (!::realpath(full.c_str(), real)) throw std::runtime_error("cannot open " + p + ": " + strerror(errno));
+  const std::string r = real;
+  if (r.size() <= g_data_real.size() || r.compare(0, g_data_real.size(), g_data_real) != 0 || r[g_data_real.size()] != '/')
+    throw std::runtime_error("layer file " + p + " is outside the data directory");
+  return r;
+}
 
 // ids from PATH.ids (one per line, as the reference's loader reads them), then row numbers for any rows left
 void read_ids(const std::string& path, uint64_t n, IdStore& ids) {
@@ -253,82 +269,202 @@ std::string build_layer(const std::string& name, const std::string& source, int 
   return js;
 }
 
-// ------------------------------------------------------------------------------------------------- io
-struct Conn {
-  int fd;
-  volatile int broken = 0;      // set when a send fails; also the request's cancel flag (polled between sub-batches)
-  bool streaming = false;       // frames started: errors go out as GSP_FRAME_ERR
-  std::mutex wmu;
-  bool read_full(void* p, size_t n) {
-    char* c = (char*)p;
-    while (n) {
-      const ssize_t r = ::read(fd, c, n);
-      if (r <= 0) { if (r < 0 && errno == EINTR) continue; return false; }
-      c += r; n -= (size_t)r;
+
+// ------------------------------------------------------------------------------------------------- connection
+// One accepted connection. Reads wait at most IDLE_MS for each piece of the request, so a client that stalls
+// mid-request releases its thread; `gone` is set when the peer stops reading or writing and doubles as the running
+// request's cancel flag (the engine polls it between sub-batches).
+constexpr int IDLE_MS = 120000;
+
+class Peer {
+ public:
+  explicit Peer(int fd) : fd_(fd) {}
+  ~Peer() { ::close(fd_); }
+  Peer(const Peer&) = delete;
+  Peer& operator=(const Peer&) = delete;
+  volatile int gone = 0;
+
+  bool read_exact(void* dst, size_t n) {
+    auto* p = static_cast<char*>(dst);
+    while (n > 0) {
+      pollfd w{fd_, POLLIN, 0};
+      const int pr = ::poll(&w, 1, IDLE_MS);
+      if (pr < 0 && errno == EINTR) continue;
+      if (pr <= 0) return false;   // idle too long, or poll failed
+      const ssize_t got = ::recv(fd_, p, n, 0);
+      if (got < 0 && errno == EINTR) continue;
+      if (got <= 0) return false;
+      p += got;
+      n -= (size_t)got;
     }
     return true;
   }
-  bool write_full(const void* p, size_t n) {
-    const char* c = (const char*)p;
-    while (n && !broken) {
-      const ssize_t r = ::send(fd, c, n, MSG_NOSIGNAL);
-      if (r <= 0) { if (r < 0 && errno == EINTR) continue; broken = 1; return false; }
-      c += r; n -= (size_t)r;
+  // every byte of the vectors, in order; on failure the peer is marked gone and later writes are dropped
+  bool write_all(iovec* v, int cnt) {
+    std::lock_guard<std::mutex> lk(wmu_);
+    while (cnt > 0 && !gone) {
+      msghdr m{};
+      m.msg_iov = v;
+      m.msg_iovlen = (size_t)cnt;
+      const ssize_t put = ::sendmsg(fd_, &m, MSG_NOSIGNAL);
+      if (put < 0 && errno == EINTR) continue;
+      if (put <= 0) { gone = 1; break; }
+      size_t left = (size_t)put;
+      while (cnt > 0 && left >= v->iov_len) { left -= v->iov_len; v++; cnt--; }
+      if (cnt > 0) { v->iov_base = static_cast<char*>(v->iov_base) + left; v->iov_len -= left; }
     }
-    return !broken;
+    return !gone;
   }
-  bool u32(uint32_t v) { return write_full(&v, 4); }
-  bool str(const std::string& s) { return u32((uint32_t)s.size()) && write_full(s.data(), s.size()); }
-  void ok_json(const std::string& j) { u32(GSP_OK); str(j); }
-  void error(const std::string& m) { u32(GSP_ERR); str(m); }
+  bool write_u32s(std::initializer_list<uint32_t> xs) {
+    uint32_t buf[4];
+    size_t k = 0;
+    for (uint32_t x : xs) buf[k++] = x;
+    iovec v{buf, k * 4};
+    return write_all(&v, 1);
+  }
+  bool write_text(uint32_t lead, const std::string& s) {   // u32 lead, u32 length, bytes
+    uint32_t hdr[2] = {lead, (uint32_t)s.size()};
+    iovec v[2] = {{hdr, 8}, {const_cast<char*>(s.data()), s.size()}};
+    return write_all(v, 2);
+  }
+
+ private:
+  int fd_;
+  std::mutex wmu_;
 };
 
-// frames of complete rows, closed at ~1 MiB (reference FrameBuf, daemon/gpujoind.cu:177-187)
-struct FrameBuf {
-  std::string out;
-  size_t fstart = 0;
-  uint32_t cnt = 0;
-  uint64_t rows = 0;
-  void begin() { fstart = out.size(); out.append(4, '\0'); cnt = 0; }
-  void id(const char* p, size_t l) { const uint16_t L = (uint16_t)std::min<size_t>(l, 65535); out.append((const char*)&L, 2); out.append(p, L); }
-  void f64(double x) { out.append((const char*)&x, 8); }
-  void u32(uint32_t x) { out.append((const char*)&x, 4); }
-  void row_done() { cnt++; rows++; if (out.size() - fstart >= (1u << 20)) { close(); begin(); } }
-  void close() { memcpy(&out[fstart], &cnt, 4); }
-  void finish() { if (cnt) close(); else out.resize(fstart); }
+// ------------------------------------------------------------------------------------------------- request
+constexpr uint32_t MAX_ARGS = 64;                 // protocol limits
+constexpr uint32_t MAX_ARG_BYTES = 64u << 20;
+constexpr uint64_t MAX_REQUEST_BYTES = 256ull << 20;
+
+struct WireRequest {
+  uint32_t op = 0;
+  std::string token;
+  std::vector<std::string> args;   // the op's arguments (the token removed)
 };
 
-// encode items [0, n) in parallel slices (enc(i, frame)), send the slices in order; returns rows sent
-template <class Enc>
-uint64_t send_parallel(Conn& c, size_t n, Enc&& enc) {
-  if (!n || c.broken) return 0;
-  const int T = std::max(1, g_threads);
-  const size_t nch = std::min<size_t>(n, (size_t)T * 4);
-  std::vector<FrameBuf> fb(nch);
-#pragma omp parallel for schedule(dynamic, 1) num_threads(T)
-  for (int64_t k = 0; k < (int64_t)nch; k++) {
-    const size_t i0 = n * k / nch, i1 = n * (k + 1) / nch;
-    fb[k].out.reserve((i1 - i0) * 64);
-    fb[k].begin();
-    for (size_t i = i0; i < i1; i++) enc(i, fb[k]);
-    fb[k].finish();
+// false for anything but a complete, well-formed request within the limits
+bool read_request(Peer& p, WireRequest& q) {
+  uint32_t head[3];
+  if (!p.read_exact(head, sizeof head) || head[0] != GSP_MAGIC || head[2] == 0 || head[2] > MAX_ARGS) return false;
+  q.op = head[1];
+  std::vector<std::string> all(head[2]);
+  uint64_t total = 0;
+  for (std::string& a : all) {
+    uint32_t len = 0;
+    if (!p.read_exact(&len, 4) || len > MAX_ARG_BYTES || (total += len) > MAX_REQUEST_BYTES) return false;
+    a.resize(len);
+    if (len && !p.read_exact(&a[0], len)) return false;
   }
-  uint64_t rows = 0;
-  std::lock_guard<std::mutex> g(c.wmu);
-  for (auto& f : fb) { if (!c.write_full(f.out.data(), f.out.size())) break; rows += f.rows; }
-  return rows;
+  q.token = std::move(all[0]);
+  q.args.assign(std::make_move_iterator(all.begin() + 1), std::make_move_iterator(all.end()));
+  return true;
 }
-void end_stream(Conn& c, const std::string& stats) { c.u32(0); c.str(stats); }
 
-// run one request on the scheduler and wait for it
-gsj_join_stats run_request(gsj_request& r, Conn& c, double* run_s = nullptr) {
-  r.opts.cancel = &c.broken;
+// whether the client's token is the daemon's: the time taken does not depend on where the two first differ (all
+// positions of the longer one are folded in)
+bool token_matches(const std::string& got) {
+  const size_t n = std::max(got.size(), g_token.size());
+  uint8_t diff = got.size() == g_token.size() ? 0 : 1;
+  for (size_t i = 0; i < n; i++) {
+    const uint8_t a = i < got.size() ? (uint8_t)got[i] : 0, b = i < g_token.size() ? (uint8_t)g_token[i] : 0;
+    diff = (uint8_t)(diff | (a ^ b));
+  }
+  return diff == 0;
+}
+
+// ------------------------------------------------------------------------------------------------- rows
+// Rows of one callback batch, encoded on several threads into chunks: each chunk keeps its rows' bytes back to back
+// and the end offset of every row, and the reply cuts frames from it at row boundaries (at most FRAME_BYTES of rows
+// a frame) and hands each frame to the socket as two pieces, its count and a slice of the chunk.
+constexpr size_t FRAME_BYTES = 1u << 20;
+
+struct RowChunk {
+  std::string bytes;
+  std::vector<uint32_t> ends;
+  void id(const char* p, size_t len) {
+    const uint16_t l = (uint16_t)std::min<size_t>(len, 65535);   // ids longer than the u16 field are cut
+    bytes.append(reinterpret_cast<const char*>(&l), 2);
+    bytes.append(p, l);
+  }
+  void f64(double x) { bytes.append(reinterpret_cast<const char*>(&x), 8); }
+  void u32(uint32_t x) { bytes.append(reinterpret_cast<const char*>(&x), 4); }
+  void raw(const char* p, size_t len) { bytes.append(p, len); }
+  void end_row() { ends.push_back((uint32_t)bytes.size()); }
+};
+
+// The response side of one request. JSON ops answer with json(); streaming ops open the stream, send row batches and
+// close it with the stats; fail() answers GSP_ERR before the stream opened and a GSP_FRAME_ERR frame after. A row
+// limit (--max-rows) stops a stream that would exceed it: the batch is not sent, the engine is asked to stop, and the
+// op ends the stream with GSP_FRAME_ERR.
+class Reply {
+ public:
+  explicit Reply(Peer& p) : peer_(p) {}
+  Peer& peer() { return peer_; }
+  bool streaming() const { return streaming_; }
+  bool over_limit() const { return over_limit_; }
+  // nonzero: the engine should stop delivering rows (client gone, or row limit reached)
+  int stop() const { return (peer_.gone || over_limit_) ? 1 : 0; }
+
+  void json(const std::string& j) { peer_.write_text(GSP_OK, j); }
+  void fail(const std::string& msg) { peer_.write_text(streaming_ ? GSP_FRAME_ERR : GSP_ERR, msg); }
+  void open_stream() {
+    peer_.write_u32s({GSP_OK});
+    streaming_ = true;
+  }
+  void close_stream(const std::string& stats) {
+    if (over_limit_) throw std::runtime_error("row limit of " + std::to_string(g_max_rows) + " rows exceeded");
+    if (!peer_.gone) peer_.write_text(0, stats);
+  }
+
+  // encode items [0, n) with enc(i, chunk) on the encoding threads and send their rows in item order; returns rows sent
+  template <class Enc>
+  uint64_t send_rows(size_t n, Enc&& enc) {
+    if (n == 0 || stop()) return 0;
+    const int T = std::max(1, g_threads);
+    const size_t nchunk = std::min<size_t>(n, (size_t)T * 4);
+    std::vector<RowChunk> ch(nchunk);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(T)
+    for (int64_t k = 0; k < (int64_t)nchunk; k++) {
+      const size_t lo = n * (size_t)k / nchunk, hi = n * (size_t)(k + 1) / nchunk;
+      for (size_t i = lo; i < hi; i++) enc(i, ch[k]);
+    }
+    uint64_t rows = 0;
+    for (const RowChunk& c : ch) rows += c.ends.size();
+    if (g_max_rows && sent_ + rows > g_max_rows) { over_limit_ = true; return 0; }
+    for (RowChunk& c : ch) {
+      size_t first = 0;   // first row of the next frame
+      while (first < c.ends.size() && !peer_.gone) {
+        const uint32_t base = first ? c.ends[first - 1] : 0;
+        size_t last = first + 1;   // one past the frame's last row: as many rows as fit FRAME_BYTES (at least one)
+        while (last < c.ends.size() && c.ends[last] - base <= FRAME_BYTES) last++;
+        uint32_t count = (uint32_t)(last - first);
+        iovec v[2] = {{&count, 4}, {&c.bytes[base], c.ends[last - 1] - base}};
+        if (!peer_.write_all(v, 2)) break;
+        sent_ += count;
+        first = last;
+      }
+    }
+    return rows;
+  }
+
+ private:
+  Peer& peer_;
+  bool streaming_ = false, over_limit_ = false;
+  uint64_t sent_ = 0;
+};
+
+// run one request on the scheduler, wait for it, and return its stats; a stop asked for by the reply (client gone,
+// row limit) is not an error here
+gsj_join_stats run_request(gsj_request& r, Reply& rep, double* run_s = nullptr) {
+  r.opts.cancel = &rep.peer().gone;
   uint64_t t = 0;
   check(gsj_submit(g_sched, &r, &t), "submit");
   gsj_join_stats st{};
   gsj_request_times tm{};
   const int rc = gsj_wait(g_sched, t, &st, &tm);
-  if (rc == GSJ_E_CANCEL && c.broken) return st;
+  if (rc == GSJ_E_CANCEL && rep.stop()) return st;
   check(rc, "request");
   if (run_s) *run_s = tm.run_ms / 1e3;
   return st;
@@ -370,7 +506,7 @@ struct Expand {
 
 // ------------------------------------------------------------------------------------------------- ops
 struct JoinSink {
-  Conn* c;
+  Reply* c;
   Expand ex;
   bool stats_only, want_area, points, swapped;
   std::atomic<uint64_t> total{0};
@@ -391,15 +527,15 @@ int join_rows_cb(void* u, const uint32_t* pairs, const double* area, const uint3
     s.total += t;
     s.area_sum += as;
   } else {
-    s.total += send_parallel(*s.c, n, [&](size_t i, FrameBuf& f) {
+    s.total += s.c->send_rows(n, [&](size_t i, RowChunk& f) {
       const double ar = area ? area[i] : NaN;
       s.ex.rows(pairs[2 * i], pairs[2 * i + 1], [&](uint32_t ra, uint32_t rb, bool) {
-        f.id(s.ex.A->ids.data(ra), s.ex.A->ids.len(ra)); f.id(s.ex.B->ids.data(rb), s.ex.B->ids.len(rb)); f.f64(ar); f.row_done();
+        f.id(s.ex.A->ids.data(ra), s.ex.A->ids.len(ra)); f.id(s.ex.B->ids.data(rb), s.ex.B->ids.len(rb)); f.f64(ar); f.end_row();
       });
     });
   }
   s.host_s += secs_since(th);
-  return s.c->broken ? 1 : 0;
+  return s.c->stop();
 }
 
 int join_points_cb(void* u, const uint32_t* pairs, const uint8_t*, size_t n) {
@@ -413,18 +549,18 @@ int join_points_cb(void* u, const uint32_t* pairs, const uint8_t*, size_t n) {
     for (size_t i = 0; i < n; i++) t += G->msize(pairs[2 * i + 1]);
     s.total += t;
   } else {
-    s.total += send_parallel(*s.c, n, [&](size_t i, FrameBuf& f) {
+    s.total += s.c->send_rows(n, [&](size_t i, RowChunk& f) {
       G->members(pairs[2 * i + 1], [&](uint32_t rg) {
         const uint32_t rp = pairs[2 * i];
         if (s.swapped) { f.id(G->ids.data(rg), G->ids.len(rg)); f.id(P->ids.data(rp), P->ids.len(rp)); }
         else { f.id(P->ids.data(rp), P->ids.len(rp)); f.id(G->ids.data(rg), G->ids.len(rg)); }
         f.f64(s.want_area ? 0.0 : NaN);
-        f.row_done();
+        f.end_row();
       });
     });
   }
   s.host_s += secs_since(th);
-  return s.c->broken ? 1 : 0;
+  return s.c->stop();
 }
 
 struct LayerPair {
@@ -488,15 +624,14 @@ Trailing trailing(const std::vector<std::string>& a, size_t from, bool allow_dis
 }
 
 // JOIN (reference op_join, daemon/gpujoind.cu:210-307)
-void op_join(Conn& c, const std::vector<std::string>& a) {
+void op_join(Reply& c, const std::vector<std::string>& a) {
   if (a.size() < 4) throw std::runtime_error("JOIN needs layer_a, layer_b, want_area, self_pairs[, stats][, window=x0,y0,x1,y1]");
   const Trailing tr = trailing(a, 4, false);
   const bool stats_only = tr.stats;
   LayerPair lp = layer_pair(a[0], a[1], "joins");
   const bool want_area = a[2] == "1" || a[2] == "t" || a[2] == "true";
   const bool distinct = self_pairs_distinct(a[3]);
-  c.u32(GSP_OK);
-  c.streaming = true;
+  c.open_stream();
   auto t0 = Clock::now();
   JoinSink s{&c, Expand{lp.A.get(), lp.B.get(), lp.same, distinct}, stats_only, want_area, false, lp.swapped};
   gsj_request r{};
@@ -523,8 +658,8 @@ void op_join(Conn& c, const std::vector<std::string>& a) {
     if (stats_only && want_area) j.k("area_sum").n(s.area_sum);
     stats = j.done();
   }
-  if (!c.broken) end_stream(c, stats);
-  logf("join %s x %s area=%d: %s%s", a[0].c_str(), a[1].c_str(), (int)want_area, stats.c_str(), c.broken ? " (client went away)" : "");
+  c.close_stream(stats);
+  logf("join %s x %s area=%d: %s%s", a[0].c_str(), a[1].c_str(), (int)want_area, stats.c_str(), c.peer().gone ? " (client went away)" : "");
 }
 
 int cluster_rows_cb(void* u, const uint32_t* pairs, const double*, const uint32_t*, size_t n) {
@@ -536,7 +671,7 @@ int cluster_rows_cb(void* u, const uint32_t* pairs, const double*, const uint32_
 // CLUSTER (reference op_cluster, daemon/gpujoind.cu:290-322): dense cluster numbers in order of first row. With the
 // trailing window=x0,y0,x1,y1 (an extension): the clusters of the layer's geometries whose box meets the window, rows
 // for their source rows only, numbered the same way.
-void op_cluster(Conn& c, const std::vector<std::string>& a) {
+void op_cluster(Reply& c, const std::vector<std::string>& a) {
   if (a.empty()) throw std::runtime_error("CLUSTER needs layer[, window=x0,y0,x1,y1]");
   const Trailing tr = trailing(a, 1, false);
   if (tr.stats) throw std::runtime_error("CLUSTER takes no stats option");
@@ -579,50 +714,49 @@ void op_cluster(Conn& c, const std::vector<std::string>& a) {
     for (size_t i = 0; i < srows.size(); i++) label[i] = dense.emplace(row_root[i], (uint32_t)dense.size()).first->second;
     k = (uint32_t)dense.size();
   }
-  c.u32(GSP_OK);
-  c.streaming = true;
-  const uint64_t total = send_parallel(c, srows.size(), [&](size_t i, FrameBuf& f) {
+  c.open_stream();
+  const uint64_t total = c.send_rows(srows.size(), [&](size_t i, RowChunk& f) {
     f.id(L->ids.data(srows[i]), L->ids.len(srows[i]));
     f.u32(label[i]);
-    f.row_done();
+    f.end_row();
   });
   const std::string stats = Json().k("kind").v("cluster_intersecting").k("rows").u(total).k("clusters").u(k).k("candidates").u(st.candidates)
       .k("edges").u(st.hits).k("traverse_ms").n(st.traverse_ms).k("refine_ms").n(st.refine_ms).k("total_s").n(secs_since(t0)).done();
-  if (!c.broken) end_stream(c, stats);
+  c.close_stream(stats);
   logf("cluster %s: %s", a[0].c_str(), stats.c_str());
 }
 
 // RELATE (reference op_relate, daemon/gpujoind.cu:325-389)
 struct RelateSink {
-  Conn* c;
+  Reply* c;
   Expand ex;
   std::string pat;
   bool swapped;
   std::atomic<uint64_t> total{0};
   double host_s = 0;
-  void put(FrameBuf& f, const Layer& LA, uint32_t ra, const Layer& LB, uint32_t rb, uint32_t code) const {
+  void put(RowChunk& f, const Layer& LA, uint32_t ra, const Layer& LB, uint32_t rb, uint32_t code) const {
     if (!pat.empty() && !gsj_de9im_match(code, pat.c_str())) return;
     char m[9];
     gsj_de9im_str(code, m);
-    f.id(LA.ids.data(ra), LA.ids.len(ra)); f.id(LB.ids.data(rb), LB.ids.len(rb)); f.out.append(m, 9); f.row_done();
+    f.id(LA.ids.data(ra), LA.ids.len(ra)); f.id(LB.ids.data(rb), LB.ids.len(rb)); f.raw(m, 9); f.end_row();
   }
 };
 int relate_rows_cb(void* u, const uint32_t* pairs, const double*, const uint32_t* code, size_t n) {
   RelateSink& s = *(RelateSink*)u;
   auto th = Clock::now();
-  s.total += send_parallel(*s.c, n, [&](size_t i, FrameBuf& f) {
+  s.total += s.c->send_rows(n, [&](size_t i, RowChunk& f) {
     const uint32_t cd = code[i], ct = gsj_de9im_transpose(cd);
     s.ex.rows(pairs[2 * i], pairs[2 * i + 1], [&](uint32_t ra, uint32_t rb, bool mirrored) { s.put(f, *s.ex.A, ra, *s.ex.B, rb, mirrored ? ct : cd); });
   });
   s.host_s += secs_since(th);
-  return s.c->broken ? 1 : 0;
+  return s.c->stop();
 }
 int relate_points_cb(void* u, const uint32_t* pairs, const uint8_t* where, size_t n) {
   RelateSink& s = *(RelateSink*)u;
   auto th = Clock::now();
   const Layer* P = s.swapped ? s.ex.B : s.ex.A;
   const Layer* G = s.swapped ? s.ex.A : s.ex.B;
-  s.total += send_parallel(*s.c, n, [&](size_t i, FrameBuf& f) {
+  s.total += s.c->send_rows(n, [&](size_t i, RowChunk& f) {
     const uint32_t code = where[i] == 2 ? GSJ_DE9IM_PT_BOUNDARY : GSJ_DE9IM_PT_INTERIOR;   // relate(point, polygon)
     G->members(pairs[2 * i + 1], [&](uint32_t rg) {
       if (s.swapped) s.put(f, *G, rg, *P, pairs[2 * i], gsj_de9im_transpose(code));
@@ -630,9 +764,9 @@ int relate_points_cb(void* u, const uint32_t* pairs, const uint8_t* where, size_
     });
   });
   s.host_s += secs_since(th);
-  return s.c->broken ? 1 : 0;
+  return s.c->stop();
 }
-void op_relate(Conn& c, const std::vector<std::string>& a) {
+void op_relate(Reply& c, const std::vector<std::string>& a) {
   if (a.size() < 4) throw std::runtime_error("RELATE needs layer_a, layer_b, pattern, self_pairs[, window=x0,y0,x1,y1]");
   const Trailing tr = trailing(a, 4, false);
   if (tr.stats) throw std::runtime_error("RELATE has no stats mode");
@@ -644,8 +778,7 @@ void op_relate(Conn& c, const std::vector<std::string>& a) {
       if (!strchr("TtFf*012", ch)) throw std::runtime_error("bad DE-9IM pattern character '" + std::string(1, ch) + "'");
   }
   const bool distinct = self_pairs_distinct(a[3]);
-  c.u32(GSP_OK);
-  c.streaming = true;
+  c.open_stream();
   auto t0 = Clock::now();
   RelateSink s{&c, Expand{lp.A.get(), lp.B.get(), lp.same, distinct}, pat, lp.swapped};
   gsj_request r{};
@@ -663,13 +796,13 @@ void op_relate(Conn& c, const std::vector<std::string>& a) {
   }
   const std::string stats = Json().k("kind").v("relate").k("pattern").v(pat).k("candidates").u(st.candidates).k("gpu_hits").u(st.hits)
       .k("rows").u(s.total).k("host_encode_send_s").n(s.host_s).k("total_s").n(secs_since(t0)).done();
-  if (!c.broken) end_stream(c, stats);
-  logf("relate %s x %s: %s%s", a[0].c_str(), a[1].c_str(), stats.c_str(), c.broken ? " (client went away)" : "");
+  c.close_stream(stats);
+  logf("relate %s x %s: %s%s", a[0].c_str(), a[1].c_str(), stats.c_str(), c.peer().gone ? " (client went away)" : "");
 }
 
 // DWITHIN (reference op_dwithin, daemon/gpujoind.cu:396-477)
 struct DistSink {
-  Conn* c;
+  Reply* c;
   Expand ex;
   bool stats_only, points, swapped, with_distance;
   std::atomic<uint64_t> total{0};
@@ -686,14 +819,14 @@ int dist_rows_cb(void* u, const uint32_t* pairs, const double* dist, size_t n) {
       for (size_t i = 0; i < n; i++) t += G->msize(pairs[2 * i + 1]);
       s.total += t;
     } else {
-      s.total += send_parallel(*s.c, n, [&](size_t i, FrameBuf& f) {
+      s.total += s.c->send_rows(n, [&](size_t i, RowChunk& f) {
         const double v = (s.with_distance && dist) ? dist[i] : NaN;
         G->members(pairs[2 * i + 1], [&](uint32_t rg) {
           const uint32_t rp = pairs[2 * i];
           if (s.swapped) { f.id(G->ids.data(rg), G->ids.len(rg)); f.id(P->ids.data(rp), P->ids.len(rp)); }
           else { f.id(P->ids.data(rp), P->ids.len(rp)); f.id(G->ids.data(rg), G->ids.len(rg)); }
           f.f64(v);
-          f.row_done();
+          f.end_row();
         });
       });
     }
@@ -702,17 +835,17 @@ int dist_rows_cb(void* u, const uint32_t* pairs, const double* dist, size_t n) {
     for (size_t i = 0; i < n; i++) t += s.ex.count(pairs[2 * i], pairs[2 * i + 1]);
     s.total += t;
   } else {
-    s.total += send_parallel(*s.c, n, [&](size_t i, FrameBuf& f) {
+    s.total += s.c->send_rows(n, [&](size_t i, RowChunk& f) {
       const double v = (s.with_distance && dist) ? dist[i] : NaN;
       s.ex.rows(pairs[2 * i], pairs[2 * i + 1], [&](uint32_t ra, uint32_t rb, bool) {
-        f.id(s.ex.A->ids.data(ra), s.ex.A->ids.len(ra)); f.id(s.ex.B->ids.data(rb), s.ex.B->ids.len(rb)); f.f64(v); f.row_done();
+        f.id(s.ex.A->ids.data(ra), s.ex.A->ids.len(ra)); f.id(s.ex.B->ids.data(rb), s.ex.B->ids.len(rb)); f.f64(v); f.end_row();
       });
     });
   }
   s.host_s += secs_since(th);
-  return s.c->broken ? 1 : 0;
+  return s.c->stop();
 }
-void op_dwithin(Conn& c, const std::vector<std::string>& a) {
+void op_dwithin(Reply& c, const std::vector<std::string>& a) {
   if (a.size() < 6) throw std::runtime_error("DWITHIN needs layer_a, layer_b, mode, dist_a, dist_b, self_pairs[, stats|distance][, window=x0,y0,x1,y1]");
   const Trailing tr = trailing(a, 6, true);
   const bool stats_only = tr.stats, with_distance = tr.distance;
@@ -735,8 +868,7 @@ void op_dwithin(Conn& c, const std::vector<std::string>& a) {
   if (lp.swapped) std::swap(ra, rb);
   const bool points = lp.A->kind == "point" || lp.swapped;
   if (points && erode && ra > 0) throw std::runtime_error("a point layer cannot have a negative buffer");
-  c.u32(GSP_OK);
-  c.streaming = true;
+  c.open_stream();
   auto t0 = Clock::now();
   DistSink s{&c, Expand{lp.A.get(), lp.B.get(), lp.same, distinct}, stats_only, points, lp.swapped, with_distance && !erode};
   gsj_request r{};
@@ -770,9 +902,9 @@ void op_dwithin(Conn& c, const std::vector<std::string>& a) {
       .k("rows").u(s.total).k("heavy").u(st.heavy).k("traverse_ms").n(st.traverse_ms).k("refine_ms").n(st.refine_ms)
       .k("compact_ms").n(st.compact_ms).k("d2h_ms").n(st.copy_ms).k("host_encode_send_s").n(s.host_s).k("total_s").n(secs_since(t0))
       .k("stats_only").b(stats_only).k("distance").b(s.with_distance).done();
-  if (!c.broken) end_stream(c, stats);
+  c.close_stream(stats);
   logf("dwithin %s x %s %s %s %s: %s%s", a[0].c_str(), a[1].c_str(), mode.c_str(), a[3].c_str(), a[4].c_str(), stats.c_str(),
-       c.broken ? " (client went away)" : "");
+       c.peer().gone ? " (client went away)" : "");
 }
 
 std::string op_status() {
@@ -806,137 +938,190 @@ int to_int(const std::string& s, int dflt) {
 }
 bool to_bool(const std::string& s, bool dflt) { return s.empty() ? dflt : (s == "1" || s == "t" || s == "true" || s == "on"); }
 
-// one connection, one request (reference handle, daemon/gpujoind.cu:516-566)
-void handle(int fd) {
-  Conn c{fd};
-  try {
-    uint32_t hdr[3];
-    if (!c.read_full(hdr, 12) || hdr[0] != GSP_MAGIC || hdr[2] > 64) { close(fd); return; }
-    std::vector<std::string> args(hdr[2]);
-    for (auto& s : args) {
-      uint32_t l;
-      if (!c.read_full(&l, 4) || l > (64u << 20)) { close(fd); return; }
-      s.resize(l);
-      if (l && !c.read_full(&s[0], l)) { close(fd); return; }
-    }
-    {   // args[0] is the token: constant-time compare, then drop it
-      const std::string tok = args.empty() ? std::string() : args[0];
-      unsigned diff = tok.size() != g_token.size();
-      for (size_t i = 0; i < g_token.size(); i++) diff |= (unsigned)(uint8_t)g_token[i] ^ (unsigned)(uint8_t)(i < tok.size() ? tok[i] : 0);
-      if (diff) { logf("rejected request: bad token"); c.error("authentication failed"); close(fd); return; }
-      if (!args.empty()) args.erase(args.begin());
-    }
-    switch (hdr[1]) {
-      case GSP_OP_LOAD:
-        if (args.size() < 7) throw std::runtime_error("LOAD needs 7 args");
-        c.ok_json(build_layer(args[0], args[1], to_int(args[5], 0), to_bool(args[6], true)));
-        break;
-      case GSP_OP_LOAD_QUERY:
-        if (args.size() < 4) throw std::runtime_error("LOAD_QUERY needs 4 args");
-        c.ok_json(build_layer(args[0], args[1], to_int(args[2], 0), to_bool(args[3], true)));
-        break;
-      case GSP_OP_DROP: {
-        if (args.empty()) throw std::runtime_error("DROP needs layer");
-        std::shared_ptr<Layer> old;
-        {
-          std::lock_guard<std::mutex> g(g_reg);
-          auto it = g_layers.find(args[0]);
-          if (it != g_layers.end()) { old = it->second; g_layers.erase(it); }
-        }
-        c.ok_json(old ? "true" : "false");
-        break;
-      }
-      case GSP_OP_LIST: c.ok_json(op_list()); break;
-      case GSP_OP_STATUS: c.ok_json(op_status()); break;
-      case GSP_OP_JOIN: op_join(c, args); break;
-      case GSP_OP_CLUSTER: op_cluster(c, args); break;
-      case GSP_OP_RELATE: op_relate(c, args); break;
-      case GSP_OP_DWITHIN: op_dwithin(c, args); break;
-      default: c.error("unknown op");
-    }
-  } catch (const std::exception& e) {
-    logf("error: %s", e.what());
-    if (c.streaming) { c.u32(GSP_FRAME_ERR); c.str(e.what()); }
-    else c.error(e.what());
+// ------------------------------------------------------------------------------------------------- dispatch
+using OpFn = void (*)(Reply&, const std::vector<std::string>&);
+
+void op_load(Reply& r, const std::vector<std::string>& a) {
+  if (a.size() < 7) throw std::runtime_error("LOAD needs layer, source, id_col, geom_col, filter, subdivide, dedup");
+  r.json(build_layer(a[0], a[1], to_int(a[5], 0), to_bool(a[6], true)));
+}
+void op_load_query(Reply& r, const std::vector<std::string>& a) {
+  if (a.size() < 4) throw std::runtime_error("LOAD_QUERY needs layer, source, subdivide, dedup");
+  r.json(build_layer(a[0], a[1], to_int(a[2], 0), to_bool(a[3], true)));
+}
+void op_drop(Reply& r, const std::vector<std::string>& a) {
+  if (a.empty()) throw std::runtime_error("DROP needs layer");
+  std::shared_ptr<Layer> old;   // released after the registry lock; running requests keep the device layer
+  {
+    std::lock_guard<std::mutex> g(g_reg);
+    auto it = g_layers.find(a[0]);
+    if (it != g_layers.end()) { old = it->second; g_layers.erase(it); }
   }
-  close(fd);
+  r.json(old ? "true" : "false");
 }
 
-int listen_unix(const std::string& path) {
-  const int fd = socket(AF_UNIX, SOCK_STREAM, 0);
-  sockaddr_un a{};
-  a.sun_family = AF_UNIX;
-  strncpy(a.sun_path, path.c_str(), sizeof(a.sun_path) - 1);
-  unlink(path.c_str());
-  if (fd < 0 || bind(fd, (sockaddr*)&a, sizeof a) || listen(fd, 64)) throw std::runtime_error("unix socket " + path + ": " + strerror(errno));
-  chmod(path.c_str(), 0666);
-  return fd;
+const std::unordered_map<uint32_t, OpFn>& op_table() {
+  static const std::unordered_map<uint32_t, OpFn> table = {
+      {GSP_OP_LOAD, op_load},
+      {GSP_OP_LOAD_QUERY, op_load_query},
+      {GSP_OP_DROP, op_drop},
+      {GSP_OP_LIST, [](Reply& r, const std::vector<std::string>&) { r.json(op_list()); }},
+      {GSP_OP_STATUS, [](Reply& r, const std::vector<std::string>&) { r.json(op_status()); }},
+      {GSP_OP_JOIN, op_join},
+      {GSP_OP_CLUSTER, op_cluster},
+      {GSP_OP_RELATE, op_relate},
+      {GSP_OP_DWITHIN, op_dwithin},
+  };
+  return table;
 }
-int listen_tcp(const std::string& hp) {
-  const auto pos = hp.rfind(':');
-  if (pos == std::string::npos) throw std::runtime_error("tcp address must be HOST:PORT");
-  const std::string host = hp.substr(0, pos);
-  const int port = atoi(hp.c_str() + pos + 1);
-  const int fd = socket(AF_INET, SOCK_STREAM, 0), one = 1;
-  setsockopt(fd, SOL_SOCKET, SO_REUSEADDR, &one, sizeof one);
-  sockaddr_in a{};
-  a.sin_family = AF_INET;
-  a.sin_port = htons((uint16_t)port);
-  if (inet_pton(AF_INET, host.c_str(), &a.sin_addr) != 1) throw std::runtime_error("bad tcp address " + hp);
-  if (fd < 0 || bind(fd, (sockaddr*)&a, sizeof a) || listen(fd, 64)) throw std::runtime_error("tcp " + hp + ": " + strerror(errno));
-  return fd;
+
+std::atomic<int> g_connections{0};
+constexpr int MAX_CONNECTIONS = 2048;
+
+// one connection: one request, one response (the extension opens a connection per call)
+void serve_connection(int fd) {
+  Peer peer(fd);
+  struct Count {   // connections in flight, for the cap in accept_loop
+    ~Count() { g_connections--; }
+  } count;
+  WireRequest q;
+  if (!read_request(peer, q)) return;   // malformed, oversized or truncated: dropped without a reply
+  Reply rep(peer);
+  if (!token_matches(q.token)) {
+    logf("rejected request: bad token");
+    rep.fail("authentication failed");
+    return;
+  }
+  try {
+    const auto& ops = op_table();
+    const auto it = ops.find(q.op);
+    if (it == ops.end()) throw std::runtime_error("unknown op " + std::to_string(q.op));
+    it->second(rep, q.args);
+  } catch (const std::exception& e) {
+    logf("error: %s", e.what());
+    rep.fail(e.what());
+  }
 }
+
+// ------------------------------------------------------------------------------------------------- listening
+class Listener {
+ public:
+  void unix_socket(const std::string& path, mode_t mode) {
+    sockaddr_un addr{};
+    addr.sun_family = AF_UNIX;
+    if (path.size() >= sizeof addr.sun_path) throw std::runtime_error("unix socket path too long: " + path);
+    memcpy(addr.sun_path, path.c_str(), path.size() + 1);
+    ::unlink(path.c_str());
+    add(AF_UNIX, reinterpret_cast<sockaddr*>(&addr), sizeof addr, "unix:" + path);
+    if (::chmod(path.c_str(), mode) != 0) throw std::runtime_error("chmod " + path + ": " + strerror(errno));
+  }
+  void tcp(const std::string& host_port) {
+    const size_t colon = host_port.rfind(':');
+    if (colon == std::string::npos) throw std::runtime_error("tcp address must be HOST:PORT");
+    sockaddr_in addr{};
+    addr.sin_family = AF_INET;
+    addr.sin_port = htons((uint16_t)std::stoi(host_port.substr(colon + 1)));
+    if (inet_pton(AF_INET, host_port.substr(0, colon).c_str(), &addr.sin_addr) != 1) throw std::runtime_error("bad tcp address " + host_port);
+    add(AF_INET, reinterpret_cast<sockaddr*>(&addr), sizeof addr, "tcp:" + host_port);
+  }
+  // accept forever; every connection gets a thread of its own, up to MAX_CONNECTIONS at once
+  [[noreturn]] void accept_loop() {
+    for (;;) {
+      if (::poll(fds_.data(), fds_.size(), -1) < 0) {
+        if (errno == EINTR) continue;
+        throw std::runtime_error(std::string("poll: ") + strerror(errno));
+      }
+      for (pollfd& l : fds_) {
+        if (!(l.revents & POLLIN)) continue;
+        const int fd = ::accept(l.fd, nullptr, nullptr);
+        if (fd < 0) continue;
+        const int on = 1;
+        setsockopt(fd, IPPROTO_TCP, TCP_NODELAY, &on, sizeof on);   // no effect on unix sockets
+        if (g_connections.fetch_add(1) >= MAX_CONNECTIONS) {
+          g_connections--;
+          Peer busy(fd);
+          busy.write_text(GSP_ERR, "daemon busy: too many connections");
+          continue;
+        }
+        std::thread(serve_connection, fd).detach();
+      }
+    }
+  }
+
+ private:
+  std::vector<pollfd> fds_;
+  void add(int family, const sockaddr* addr, socklen_t len, const std::string& what) {
+    const int fd = ::socket(family, SOCK_STREAM, 0);
+    if (fd < 0) throw std::runtime_error(what + ": " + strerror(errno));
+    const int on = 1;
+    if (family == AF_INET) setsockopt(fd, SOL_SOCKET, SO_REUSEADDR, &on, sizeof on);
+    if (::bind(fd, addr, len) != 0 || ::listen(fd, 128) != 0) {
+      const std::string err = strerror(errno);
+      ::close(fd);
+      throw std::runtime_error(what + ": " + err);
+    }
+    fds_.push_back({fd, POLLIN, 0});
+    logf("listening on %s", what.c_str());
+  }
+};
 
 }  // namespace
 
+// ------------------------------------------------------------------------------------------------- main
 int main(int argc, char** argv) {
-  std::string sock = "/tmp/gpujoind.sock", tcp;
-  int lanes = 4, device = 0;
-  uint64_t scratch_mb = 0;
+  struct Options {
+    std::string socket = "/tmp/gpujoind.sock", tcp, token_file;
+    int lanes = 4, device = 0, batch = 16;
+    uint64_t scratch_mb = 0;
+    mode_t socket_mode = 0660;
+  } o;
+  const std::vector<std::pair<std::string, std::function<void(const char*)>>> flags = {
+      {"--socket", [&](const char* v) { o.socket = v; }},
+      {"--socket-mode", [&](const char* v) { o.socket_mode = (mode_t)std::stoul(v, nullptr, 8); }},
+      {"--tcp", [&](const char* v) { o.tcp = v; }},
+      {"--token-file", [&](const char* v) { o.token_file = v; }},
+      {"--lanes", [&](const char* v) { o.lanes = std::stoi(v); }},
+      {"--threads", [&](const char* v) { g_threads = std::stoi(v); }},
+      {"--data-dir", [&](const char* v) { g_data_dir = v; }},
+      {"--device", [&](const char* v) { o.device = std::stoi(v); }},
+      {"--scratch-mb", [&](const char* v) { o.scratch_mb = std::stoull(v); }},
+      {"--max-rows", [&](const char* v) { g_max_rows = std::stoull(v); }},
+      {"--batch", [&](const char* v) { o.batch = std::stoi(v); }},
+  };
+  auto usage = [&] {
+    std::string u = "usage: gpujoind";
+    for (const auto& f : flags) u += " [" + f.first + " V]";
+    fprintf(stderr, "%s\n", u.c_str());
+    return 1;
+  };
   for (int i = 1; i < argc; i++) {
-    const std::string a = argv[i];
-    if (a == "--socket" && i + 1 < argc) sock = argv[++i];
-    else if (a == "--tcp" && i + 1 < argc) tcp = argv[++i];
-    else if (a == "--threads" && i + 1 < argc) g_threads = atoi(argv[++i]);
-    else if (a == "--lanes" && i + 1 < argc) lanes = atoi(argv[++i]);
-    else if (a == "--device" && i + 1 < argc) device = atoi(argv[++i]);
-    else if (a == "--scratch-mb" && i + 1 < argc) scratch_mb = strtoull(argv[++i], nullptr, 10);
-    else if (a == "--data-dir" && i + 1 < argc) g_data_dir = argv[++i];
-    else if (a == "--token-file" && i + 1 < argc) {
-      std::ifstream tf(argv[++i]);
-      std::getline(tf, g_token);
-      if (g_token.empty()) { fprintf(stderr, "empty token file\n"); return 1; }
-    } else {
-      fprintf(stderr, "usage: gpujoind [--socket PATH] [--tcp HOST:PORT] [--token-file FILE] [--lanes N] [--threads N] [--data-dir DIR] "
-                      "[--device D] [--scratch-mb MB]\n");
-      return 1;
+    const auto f = std::find_if(flags.begin(), flags.end(), [&](const auto& x) { return x.first == argv[i]; });
+    if (f == flags.end() || i + 1 >= argc) return usage();
+    try {
+      f->second(argv[++i]);
+    } catch (const std::exception&) {
+      return usage();
     }
   }
-  if (!tcp.empty() && g_token.empty()) { fprintf(stderr, "--tcp requires --token-file\n"); return 1; }
+  if (!o.token_file.empty()) {
+    std::ifstream tf(o.token_file);
+    std::getline(tf, g_token);
+    if (g_token.empty()) { fprintf(stderr, "gpujoind: empty token file %s\n", o.token_file.c_str()); return 1; }
+  }
+  if (!o.tcp.empty() && g_token.empty()) { fprintf(stderr, "gpujoind: --tcp requires --token-file\n"); return 1; }
+  char real[PATH_MAX];
+  if (!::realpath(g_data_dir.c_str(), real)) { fprintf(stderr, "gpujoind: data dir %s: %s\n", g_data_dir.c_str(), strerror(errno)); return 1; }
+  g_data_real = real;
   signal(SIGPIPE, SIG_IGN);
   try {
-    check(gsj_ctx_create(device, &g_ctx), "device");
-    check(gsj_sched_create(g_ctx, lanes, scratch_mb << 20, &g_sched), "scheduler");
-    logf("gpujoind on %s, %d lanes", gsj_ctx_status_json(g_ctx), lanes);
-    std::vector<pollfd> fds;
-    fds.push_back({listen_unix(sock), POLLIN, 0});
-    logf("listening on unix:%s", sock.c_str());
-    if (!tcp.empty()) { fds.push_back({listen_tcp(tcp), POLLIN, 0}); logf("listening on tcp:%s", tcp.c_str()); }
-    for (;;) {
-      if (poll(fds.data(), fds.size(), -1) < 0) {
-        if (errno == EINTR) continue;
-        perror("poll");
-        return 1;
-      }
-      for (auto& f : fds)
-        if (f.revents & POLLIN) {
-          const int cfd = accept(f.fd, nullptr, nullptr);
-          if (cfd < 0) continue;
-          const int one = 1;
-          setsockopt(cfd, IPPROTO_TCP, TCP_NODELAY, &one, sizeof one);
-          std::thread(handle, cfd).detach();
-        }
-    }
+    check(gsj_ctx_create(o.device, &g_ctx), "device");
+    check(gsj_sched_create(g_ctx, o.lanes, o.scratch_mb << 20, &g_sched), "scheduler");
+    check(gsj_sched_set_batch(g_sched, o.batch), "batching");
+    logf("gpujoind on %s, %d lanes, data in %s", gsj_ctx_status_json(g_ctx), o.lanes, g_data_real.c_str());
+    Listener l;
+    l.unix_socket(o.socket, o.socket_mode);
+    if (!o.tcp.empty()) l.tcp(o.tcp);
+    l.accept_loop();
   } catch (const std::exception& e) {
     fprintf(stderr, "gpujoind: %s\n", e.what());
     return 1;

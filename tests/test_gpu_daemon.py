@@ -329,3 +329,73 @@ def test_drop(daemon):
     c = daemon["client"]
     assert c.load("tmp", "gen:parcels_grid:nx=10;ny=10;seed=1")["rows"] == 100
     assert c.call(G.OP_DROP, "tmp") is True and c.call(G.OP_DROP, "tmp") is False
+
+
+def test_row_limit_ends_the_stream_with_a_frame_error(daemon):
+    """A second daemon with --max-rows: a join whose rows would exceed the limit opens its stream, then ends it with a
+    GSP_FRAME_ERR frame (daemon/protocol.h's mid-stream error), read as an error by this repository's client and by
+    the reference's own client; a request under the limit still completes on the same daemon."""
+    d = daemon["dir"]
+    sock = os.path.join(d, "limited.sock")
+    log = open(os.path.join(d, "limited.log"), "w")
+    proc = subprocess.Popen([DAEMON, "--socket", sock, "--data-dir", d, "--token-file", os.path.join(d, "token"), "--max-rows", "200",
+                             "--lanes", "2", "--threads", "2"], stdout=log, stderr=log)
+    try:
+        for _ in range(600):
+            if os.path.exists(sock) or proc.poll() is not None:
+                break
+            time.sleep(0.1)
+        assert proc.poll() is None
+        c = G.Client(sock, token=TOKEN)
+        c.load("a", "gset:a.gset", subdivide=8)
+        c.load("tiny", "gen:parcels_grid:nx=3;ny=3;seed=1")
+        rows, st, _ = c.join("tiny", "tiny", want_area=True)
+        assert len(rows) == st["rows"] == 9
+        with pytest.raises(G.DaemonError, match="row limit of 200"):
+            c.join("a", "a", want_area=True)
+        if os.path.exists(GSP_CAT):
+            r = subprocess.run([GSP_CAT, sock, str(G.OP_JOIN), "a", "a", "1", "all"], env=dict(os.environ, GSP_TOKEN=TOKEN),
+                               capture_output=True, text=True, timeout=60)
+            assert r.returncode == 1 and "stream error: row limit of 200" in r.stderr, r.stderr
+    finally:
+        proc.terminate()
+        proc.wait(timeout=60)
+        log.close()
+
+
+def test_layer_files_stay_inside_the_data_directory(daemon):
+    c = daemon["client"]
+    outside = tempfile.mkdtemp()
+    A = daemon["sets"]["a"]
+    gsj.write_gset(os.path.join(outside, "x.gset"), A)
+    os.symlink(os.path.join(outside, "x.gset"), os.path.join(daemon["dir"], "link.gset"))
+    for src in (f"gset:{outside}/x.gset", "gset:../" + os.path.basename(outside) + "/x.gset", "gset:link.gset", "gset:/etc/passwd"):
+        with pytest.raises(G.DaemonError, match="outside the data directory|cannot open"):
+            c.load("evil", src)
+    with pytest.raises(G.DaemonError, match="cannot open"):
+        c.load("missing", "gset:nope.gset")
+    assert c.load("abs", "gset:" + os.path.join(daemon["dir"], "a.gset"))["rows"] == A.n   # absolute, but inside
+
+
+GSJLOAD = os.path.join(ROOT, "postgis-cuda_amd", "bin", "gsjload")
+
+
+def test_load_client_drains_streams_and_times_them(daemon):
+    """gsjload (the open-loop client of the serving benchmark) sends each request at its arrival time on a connection
+    of its own and drains every row: its row counts equal what this repository's client parses from the same
+    requests, errors are reported per request, and every latency is positive."""
+    c = daemon["client"]
+    xy = daemon["sets"]["b"].xy
+    w = "window=" + ",".join(repr(float(v)) for v in (xy[:, 0].min(), xy[:, 1].min(), xy[:, 0].mean(), xy[:, 1].mean()))
+    reqs = [(G.OP_JOIN, ["a", "a", "1", "all"]), (G.OP_RELATE, ["a", "b", "", "distinct"]), (G.OP_CLUSTER, ["b", w]),
+            (G.OP_DWITHIN, ["b", "b", "dwithin", "3", "0", "distinct", w]), (G.OP_JOIN, ["p", "b", "0", "all"]), (G.OP_JOIN, ["a", "zz", "0", "all"])]
+    text = "".join("\t".join([f"{0.01 * i:.3f}", str(op)] + args) + "\n" for i, (op, args) in enumerate(reqs))
+    r = subprocess.run([GSJLOAD, "--socket", daemon["sock"], "--token", TOKEN], input=text, capture_output=True, text=True, timeout=120)
+    recs = [json.loads(x) for x in r.stdout.splitlines()]
+    per, summary = recs[:len(reqs)], recs[-1]
+    assert len(recs) == len(reqs) + 1 and summary["summary"] and summary["failed"] == 1 and r.returncode == 2
+    for (op, args), rec in zip(reqs[:-1], per[:-1]):
+        rows, st, _ = c.call(op, *args)
+        assert rec["ok"] and rec["rows"] == len(rows) == rec["stats"]["rows"] and rec["latency_ms"] > 0, (args, rec)
+    assert not per[-1]["ok"] and "no such layer" in per[-1]["error"]   # the unknown layer
+    assert summary["rows"] == sum(x["rows"] for x in per) and summary["p99_ms"] >= summary["p50_ms"] > 0
