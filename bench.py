@@ -52,6 +52,162 @@ def pair_class(hs, pairs):
     return np.where(small, 0, np.where(med, 1, np.where(mixed, 2, 3)))
 
 
+def pair_class2(hp, hr, pairs):
+    """pair_class for a join of two sets (probe hp, build hr)"""
+    def shape(hs):
+        return np.diff(hs.gro.astype(np.int64)), hs.rvo[hs.gro[1:]].astype(np.int64) - hs.rvo[hs.gro[:-1]].astype(np.int64)
+    (ra, va), (rb, vb) = shape(hp), shape(hr)
+    va, vb = va[pairs[:, 0]], vb[pairs[:, 1]]
+    rok = (ra[pairs[:, 0]] <= 8) & (rb[pairs[:, 1]] <= 8)
+    small = rok & (va <= 32) & (vb <= 32)
+    med = ~small & rok & (va <= 64) & (vb <= 64)
+    mixed = ~small & ~med & (np.minimum(va, vb) <= 32)
+    return np.where(small, 0, np.where(med, 1, np.where(mixed, 2, 3)))
+
+
+def time_cpu(fn, seconds):
+    """run fn() until about `seconds` of host time; returns (total seconds, repeats, last result)"""
+    reps, t, r = 0, 0.0, None
+    while reps == 0 or t < seconds:
+        t1 = time.perf_counter()
+        r = fn()
+        t += time.perf_counter() - t1
+        reps += 1
+    return t, reps, r
+
+
+def host_threads():
+    return min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
+
+
+def config1_leg(gsj, ctx, args):
+    """Parcels x regions intersects + intersection area: 100 x 100 parcels of 4-12 vertices under 16 regions of 2000
+    vertices, seed 0. One step = one request (layers resident). The dominant kernel is the mixed class (a parcel
+    against a region: 16-lane groups over the region's edges near the parcel); its FLOPs are the counting oracle's
+    for every candidate pair of the request."""
+    from oracle import oracle as O
+
+    P = gsj.generate("parcels_grid", nx=100, ny=100, seed=0)
+    R = gsj.generate("regions_grid", nx=100, ny=100, kx=4, ky=4, edge_segs=500, seed=0)
+    LP, LR = ctx.polys(P, build_bvh=False), ctx.polys(R)
+    for _ in range(3):
+        ctx.join(LP, LR, want_area=True, collect=False)
+    steps = max(10, 20 * args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = ctx.join(LP, LR, want_area=True, collect=False)
+    wall = time.perf_counter() - t0
+    st = r.stats
+    kname, kfield, kcount = max((("k_refine_mixed", "k_mixed_ms", "n_mixed"), ("k_refine_large", "k_large_ms", "n_large"),
+                                 ("k_screen_small", "k_screen_ms", "n_small")), key=lambda k: st[k[1]])
+    OP, OR = O.Layer.polys(P), O.Layer.polys(R)
+    cand = O.candidates(OP, OR)
+    cls = pair_class2(P, R, cand)
+    pick = cand[cls == {"k_refine_mixed": 2, "k_refine_large": 3, "k_screen_small": 0}[kname]]
+    fl = O.refine_flops(P, R, pick, want_area=True)
+    per_launch_ms = st[kfield] / max(st["launches"], 1)
+    achieved = fl["flops"] * (st[kcount] / max(len(pick), 1)) / max(st["launches"], 1) / (per_launch_ms * 1e-3) / 1e12
+    O.set_threads(host_threads())
+    t_cpu, reps, exp = time_cpu(lambda: O.join(OP, OR, want_area=True), args.cpu_seconds / 2)
+    got = ctx.join(LP, LR, want_area=True)
+    g, e = np.lexsort((got.pairs[:, 1], got.pairs[:, 0])), np.lexsort((exp["pairs"][:, 1], exp["pairs"][:, 0]))
+    parity = {"rows_gpu": len(got.pairs), "rows_oracle": len(exp["pairs"]), "pairs_equal": bool(np.array_equal(got.pairs[g], exp["pairs"][e])),
+              "areas_bit_identical": bool(np.array_equal(got.area[g], exp["area"][e])), "candidates_equal": got.stats["candidates"] == exp["candidates"]}
+    out = {"workload": "config 1: 10000 parcels (4-12 vertices) x 16 regions (2000 vertices), intersects + area, seed 0",
+           "value": round(st["candidates"] * steps / wall, 1), "unit": "refined pairs/s", "ms_per_request": round(1e3 * wall / steps, 4),
+           "candidates": st["candidates"], "rows": st["hits"],
+           "roofline": {"kernel": kname, "bound": "valu", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None, "flops_per_pair": round(fl["flops"] / max(len(pick), 1), 1),
+                        "pairs_per_launch": int(st[kcount] / max(st["launches"], 1)), "launch_ms": round(per_launch_ms, 4),
+                        "flops_counted": "the whole refinement of the class's pairs"},
+           "cpu_baseline": {"value": round(exp["candidates"] * reps / t_cpu, 1), "unit": "refined pairs/s", "cores": host_threads(), "kind": "port",
+                            "sample": f"the whole request ({exp['candidates']} candidates), run {reps} times in {t_cpu:.1f} s"},
+           "parity": parity}
+    LP.free()
+    LR.free()
+    return out
+
+
+def config3_leg(gsj, ctx, args, L, S):
+    """Points in polygons and geodesic dwithin. PIP: 5M points (half uniform over the 1 x 1 degree extent, half in 1000
+    clusters of sigma 50 m, seed 2) against the resident 2.4M-parcel layer L (subdivided pieces S; rows are parents).
+    dwithin: self-join of a 1.2M-parcel layer of the same generator over 0.6 x 0.6 degrees from (-95.2, 38.6), d = 5 m
+    on WGS84, with the distance of every row. Rooflines: HBM, algorithmic bytes = what each candidate's test reads
+    once (point 16 B + pair ids 8 B + the piece's per-geometry records 40 B + 16 B per stored vertex; dwithin: pair ids
+    8 B + both pieces' records and vertices) and writes (1 B flag, 8 B distance), over the kernel's HIP-event time."""
+    from oracle import oracle as O
+
+    out = {}
+    xy = gsj.generate_points(n=5_000_000, x0=-95.5, y0=38.5, w=1.0, h=1.0, clusters=1000, sigma_m=50.0, seed=2)
+    pts = ctx.points(xy)
+    for _ in range(2):
+        ctx.points_in_polygons(pts, L, collect=False)
+    steps = max(3, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = ctx.points_in_polygons(pts, L, collect=False)
+    wall = time.perf_counter() - t0
+    st = r.stats
+    nv_piece = S.nv / S.n
+    bytes_per = 16 + 8 + 40 + 16 * nv_piece + 1
+    k_ms = st["k_small_ms"] / max(st["launches"], 1)
+    gbs = bytes_per * st["candidates"] / max(st["launches"], 1) / (k_ms * 1e-3) / 1e9
+    nsamp = 500_000
+    O.set_threads(host_threads())
+    OS = O.Layer.polys(S)
+    Osamp = O.Layer.points_layer(xy[:nsamp])
+    t_cpu, reps, exp = time_cpu(lambda: O.points_join(Osamp, OS), args.cpu_seconds / 2)
+    got = ctx.points_in_polygons(ctx.points(xy[:nsamp]), L)
+    g, e = np.lexsort((got.pairs[:, 1], got.pairs[:, 0])), np.lexsort((exp["pairs"][:, 1], exp["pairs"][:, 0]))
+    out["points_in_polygons"] = {
+        "workload": "config 3: 5M points (50% uniform, 50% in 1000 clusters of sigma 50 m, seed 2) in the 2.4M-parcel layer (subdivide 256)",
+        "value": round(st["candidates"] * steps / wall, 1), "unit": "refined pairs/s", "ms_per_request": round(1e3 * wall / steps, 3),
+        "candidates": st["candidates"], "rows": st["hits"],
+        "roofline": {"kernel": "k_pip", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None, "bytes_per_pair": round(bytes_per, 1), "launch_ms": round(k_ms, 4)},
+        "cpu_baseline": {"value": round(exp["candidates"] * reps / t_cpu, 1), "unit": "refined pairs/s", "cores": host_threads(), "kind": "port",
+                         "sample": f"the first {nsamp} points ({exp['candidates']} candidates), run {reps} times in {t_cpu:.1f} s"},
+        "parity": {"rows_gpu": len(got.pairs), "rows_oracle": len(exp["pairs"]),
+                   "rows_equal": bool(np.array_equal(got.pairs[g], exp["pairs"][e]) and np.array_equal(got.where[g], exp["where"][e]))}}
+    pts.free()
+    del Osamp, xy
+    n12 = 1095
+    M12 = gsj.generate("parcels_mesh", nx=n12, ny=n12, x0=-95.2, y0=38.6, w=0.6 / n12, h=0.6 / n12, seed=2)
+    L12 = ctx.polys(M12)
+    kw = dict(same=True, upper=True, report_distance=True, collect=False)
+    ctx.dwithin(L12, L12, 5.0, **kw)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = ctx.dwithin(L12, L12, 5.0, **kw)
+    wall = time.perf_counter() - t0
+    st = r.stats
+    kms = {k: st[f"k_{k}_ms"] for k in ("small", "medium", "mixed", "large", "over")}
+    kbig = max(kms, key=kms.get)
+    nv_geom = M12.nv / M12.n
+    bytes_dw = 8 + 2 * (40 + 16 * nv_geom) + 1 + 8
+    n_k = {"small": "n_small", "medium": "n_medium", "mixed": "n_mixed", "large": "n_large", "over": "n_over"}[kbig]
+    k_ms = kms[kbig] / max(st["launches"], 1)
+    gbs = bytes_dw * st[n_k] / max(st["launches"], 1) / (k_ms * 1e-3) / 1e9
+    OM = O.Layer.polys(M12)
+    t_cpu, reps, exp = time_cpu(lambda: O.dwithin(OM, OM, 5.0, same=True, upper=True, report=True), args.cpu_seconds / 2)
+    got = ctx.dwithin(L12, L12, 5.0, same=True, upper=True, report_distance=True)
+    g, e = np.lexsort((got.pairs[:, 1], got.pairs[:, 0])), np.lexsort((exp["pairs"][:, 1], exp["pairs"][:, 0]))
+    same_rows = bool(np.array_equal(got.pairs[g], exp["pairs"][e]))
+    out["self_dwithin_5m"] = {
+        "workload": f"config 3: self dwithin 5 m with distances, {M12.n} parcels over 0.6 x 0.6 degrees from (-95.2, 38.6), seed 2",
+        "value": round(st["candidates"] * steps / wall, 1), "unit": "refined pairs/s", "ms_per_request": round(1e3 * wall / steps, 3),
+        "candidates": st["candidates"], "rows": st["hits"], "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
+        "roofline": {"kernel": f"k_dw_lists ({kbig} class)" if kbig != "over" else "k_dw_overflow", "bound": "hbm", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "bytes_per_pair": round(bytes_dw, 1),
+                     "launch_ms": round(k_ms, 4)},
+        "cpu_baseline": {"value": round(exp["candidates"] * reps / t_cpu, 1), "unit": "refined pairs/s", "cores": host_threads(), "kind": "port",
+                         "sample": f"the whole self-join ({exp['candidates']} candidates), run {reps} times in {t_cpu:.1f} s"},
+        "parity": {"rows_gpu": len(got.pairs), "rows_oracle": len(exp["pairs"]), "rows_equal": same_rows,
+                   "max_distance_diff_m": float(np.max(np.abs(got.dist[g] - exp["dist"][e]))) if same_rows and len(g) else None}}
+    L12.free()
+    return out
+
+
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "refine_relate_2.4M.md")
 
 
@@ -118,37 +274,84 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     log(f"copy calibration: {reps} device copies of {nbytes} bytes")
 
 
+def config4_mix(n, smax, seed=3):
+    """The batched-requests configuration's requests: sizes lognormal in [1e3, smax] polygons, kinds 40% intersects +
+    area, 20% relate, 20% dwithin (d ~ U(1, 50) m), 20% eroded intersects (ra, rb ~ U(0.5, 3) m). One dict per request:
+    kind ("area", "relate", "dwithin", "erode"), side (cells per side of its config-1 layers), layer seed, parameters."""
+    rng = np.random.default_rng(seed)
+    mu, sig = np.log(np.sqrt(1e3 * smax)), np.log(smax / 1e3) / 6
+    sizes = np.clip(np.exp(rng.normal(mu, sig, n)), 1e3, smax).astype(np.int64)
+    kinds = rng.choice(4, size=n, p=[0.4, 0.2, 0.2, 0.2])
+    dists = rng.uniform(1.0, 50.0, n)
+    radii = rng.uniform(0.5, 3.0, (n, 2))
+    return [{"kind": ("area", "relate", "dwithin", "erode")[kinds[i]], "side": max(1, int(round(np.sqrt(sizes[i])))), "seed": 1000 * seed + i,
+             "d": float(dists[i]), "ra": float(radii[i, 0]), "rb": float(radii[i, 1])} for i in range(n)]
+
+
+def config4_layers(gsj, ctx, q):
+    """device layers of one config-4 request: parcels (and regions unless it is a dwithin self-join)"""
+    P = gsj.generate("parcels_grid", nx=q["side"], ny=q["side"], seed=q["seed"])
+    if q["kind"] == "dwithin":
+        return P, None, ctx.polys(P), None
+    R = gsj.generate("regions_grid", nx=q["side"], ny=q["side"], kx=4, ky=4, edge_segs=500, seed=q["seed"])
+    return P, R, ctx.polys(P, build_bvh=False), ctx.polys(R)
+
+
+def config4_submit_args(gsj, q, LP, LR):
+    """(kind, P, R, keyword arguments) of a config-4 request for gsj.Scheduler.submit"""
+    if q["kind"] == "dwithin":
+        return gsj.REQ_DWITHIN, LP, LP, dict(d=q["d"], same=True, upper=True)
+    if q["kind"] == "erode":
+        return gsj.REQ_INTERSECTS_BUFFERED, LP, LR, dict(ra=q["ra"], rb=q["rb"])
+    return gsj.REQ_JOIN, LP, LR, dict(want_area=q["kind"] == "area", want_relate=q["kind"] == "relate")
+
+
+def window_stream(nreq, x0, y0, W, H, seed=4):
+    """The serving configuration's windowed requests over one resident parcel layer: windows of area lognormal in
+    [1e-4, 10] square degrees (clipped to the extent), kinds 35% intersects + area, 20% relate, 20% dwithin
+    (d ~ U(1, 50) m), 20% eroded intersects (ra, rb ~ U(0.5, 3) m) as self joins, 5% cluster_intersecting of the
+    window's parcels. One (kind name, keyword arguments) per request."""
+    rng = np.random.default_rng(seed)
+    mu, sig = np.log(np.sqrt(1e-4 * 10.0)), np.log(10.0 / 1e-4) / 6
+    area = np.clip(np.exp(rng.normal(mu, sig, nreq)), 1e-4, 10.0)
+    side = np.minimum(np.sqrt(area), min(W, H))
+    cx, cy = x0 + rng.uniform(0, 1, nreq) * (W - side) + side / 2, y0 + rng.uniform(0, 1, nreq) * (H - side) + side / 2
+    kinds = rng.choice(5, size=nreq, p=[0.35, 0.2, 0.2, 0.2, 0.05])
+    dists, radii = rng.uniform(1.0, 50.0, nreq), rng.uniform(0.5, 3.0, (nreq, 2))
+    out = []
+    for i in range(nreq):
+        win = (cx[i] - side[i] / 2, cy[i] - side[i] / 2, cx[i] + side[i] / 2, cy[i] + side[i] / 2)
+        if kinds[i] == 2:
+            out.append(("dwithin", dict(d=float(dists[i]), same=True, upper=True, window=win)))
+        elif kinds[i] == 3:
+            out.append(("erode", dict(ra=float(radii[i, 0]), rb=float(radii[i, 1]), same=True, upper=True, window=win)))
+        elif kinds[i] == 4:
+            out.append(("cluster", dict(window=win)))
+        else:
+            out.append(("area" if kinds[i] == 0 else "relate", dict(want_area=kinds[i] == 0, want_relate=kinds[i] == 1, upper=True, window=win)))
+    return out
+
+
+def window_submit_kind(gsj, name):
+    return {"dwithin": gsj.REQ_DWITHIN, "erode": gsj.REQ_INTERSECTS_BUFFERED, "cluster": gsj.REQ_CLUSTER_ROWS}.get(name, gsj.REQ_JOIN)
+
+
 def serving_leg(gsj, ctx, args):
     """Served vs serial on a mixed stream of uneven requests (the batched-requests configuration: sizes lognormal in
     [1k, 2M] polygons of the config-1 generator on sqrt(s) x sqrt(s) cells, seed 3; 40% intersects + area and 20%
     relate of parcels x regions, 20% dwithin self-joins with d ~ U(1, 50) m, 20% eroded intersects of parcels x
     regions with ra, rb ~ U(0.5, 3) m). Every request arrives at t = 0; serial = one lane (one request at a time),
     served = `lanes` lanes with continuous admission. Latency = submit -> finished."""
-    rng = np.random.default_rng(3)
     n = args.requests
-    mu, sig = np.log(np.sqrt(1e3 * args.smax)), np.log(args.smax / 1e3) / 6
-    sizes = np.clip(np.exp(rng.normal(mu, sig, n)), 1e3, args.smax).astype(np.int64)
-    kinds = rng.choice(4, size=n, p=[0.4, 0.2, 0.2, 0.2])
-    dists = rng.uniform(1.0, 50.0, n)
-    radii = rng.uniform(0.5, 3.0, (n, 2))
+    mix = config4_mix(n, args.smax)
     t0 = time.time()
     reqs, layers = [], []
-    for i in range(n):
-        side = max(1, int(round(np.sqrt(sizes[i]))))
-        P = gsj.generate("parcels_grid", nx=side, ny=side, seed=3000 + i)
-        if kinds[i] == 2:
-            LP = ctx.polys(P)
-            reqs.append((gsj.REQ_DWITHIN, LP, LP, dict(d=float(dists[i]), same=True, upper=True)))
-            layers.append(LP)
-        else:
-            R = gsj.generate("regions_grid", nx=side, ny=side, kx=4, ky=4, edge_segs=500, seed=3000 + i)
-            LP, LR = ctx.polys(P, build_bvh=False), ctx.polys(R)
-            if kinds[i] == 3:
-                reqs.append((gsj.REQ_INTERSECTS_BUFFERED, LP, LR, dict(ra=float(radii[i, 0]), rb=float(radii[i, 1]))))
-            else:
-                reqs.append((gsj.REQ_JOIN, LP, LR, dict(want_area=kinds[i] != 1, want_relate=kinds[i] == 1)))
-            layers += [LP, LR]
-        del P
+    for q in mix:
+        P, R, LP, LR = config4_layers(gsj, ctx, q)
+        reqs.append(config4_submit_args(gsj, q, LP, LR))
+        layers += [L for L in (LP, LR) if L is not None]
+        del P, R
+    sizes = np.array([q["side"] ** 2 for q in mix])
     log(f"serving mix: {n} requests, {int(sizes.sum())} polygons, generated + resident in {time.time() - t0:.1f}s")
 
     def run(lanes):
@@ -247,25 +450,9 @@ def national_leg(gsj, ctx, args):
     P.free()
 
     # windowed request stream, Poisson arrivals
-    rng = np.random.default_rng(4)
     nreq = args.stream_requests
-    mu, sig = np.log(np.sqrt(1e-4 * 10.0)), np.log(10.0 / 1e-4) / 6
-    area = np.clip(np.exp(rng.normal(mu, sig, nreq)), 1e-4, 10.0)
-    side = np.minimum(np.sqrt(area), min(W, H))
-    cx, cy = x0 + rng.uniform(0, 1, nreq) * (W - side) + side / 2, y0 + rng.uniform(0, 1, nreq) * (H - side) + side / 2
-    kinds = rng.choice(5, size=nreq, p=[0.35, 0.2, 0.2, 0.2, 0.05])
-    dists, radii = rng.uniform(1.0, 50.0, nreq), rng.uniform(0.5, 3.0, (nreq, 2))
-    reqs = []
-    for i in range(nreq):
-        win = (cx[i] - side[i] / 2, cy[i] - side[i] / 2, cx[i] + side[i] / 2, cy[i] + side[i] / 2)
-        if kinds[i] == 2:
-            reqs.append((gsj.REQ_DWITHIN, dict(d=float(dists[i]), same=True, upper=True, window=win)))
-        elif kinds[i] == 3:
-            reqs.append((gsj.REQ_INTERSECTS_BUFFERED, dict(ra=float(radii[i, 0]), rb=float(radii[i, 1]), same=True, upper=True, window=win)))
-        elif kinds[i] == 4:
-            reqs.append((gsj.REQ_CLUSTER_ROWS, dict(window=win)))
-        else:
-            reqs.append((gsj.REQ_JOIN, dict(want_area=kinds[i] == 0, want_relate=kinds[i] == 1, upper=True, window=win)))
+    stream = window_stream(nreq, x0, y0, W, H)
+    reqs = [(window_submit_kind(gsj, name), kw) for name, kw in stream]
 
     def run(lanes, arrivals):
         # serial: one request at a time, as the reference serves them; served: lanes plus batches of windowed joins
@@ -308,10 +495,84 @@ def national_leg(gsj, ctx, args):
                                "5% cluster_intersecting of the window's parcels",
                         "serial": serial, "served": served,
                         "speedup_pairs_per_s": round(served["refined_pairs_per_s"] / serial["refined_pairs_per_s"], 3),
-                        "p99_ratio_served_over_serial": round(served["p99_ms"] / serial["p99_ms"], 3)})
+                        "p99_ratio_served_over_serial": round(served["p99_ms"] / serial["p99_ms"], 3),
+                        "reading": "capacity: both paths saturated, refined pairs/s is each path's throughput" if rate > args.rate else
+                                   "the configuration's top rate: both paths keep up with the arrivals, so their refined pairs/s follow the "
+                                   "arrivals; latency is the difference"})
         log(f"national stream {rate:g}/s: serial {serial}; served {served}")
     out["stream"], out["stream_overload"] = streams
     L.free()
+    if not args.no_wire:
+        ctx.trim()
+        spec = (f"gen:parcels_mesh:nx={nx};ny={ny};x0={x0!r};y0={y0!r};w={cell!r};h={cell!r};median=10;kmax=256;seed=4")
+        out["wire"] = wire_streams(args, spec, stream, (args.rate, args.overload_rate))
+    return out
+
+
+def wire_line(name, kw, layer="n"):
+    """one request of the windowed stream as gpujoind arguments (self joins list each unordered pair once)"""
+    from gsj import client as G
+
+    w = "window=" + ",".join(repr(float(v)) for v in kw["window"])
+    if name in ("area", "relate"):
+        return (G.OP_JOIN, [layer, layer, "1", "distinct", w]) if name == "area" else (G.OP_RELATE, [layer, layer, "", "distinct", w])
+    if name == "dwithin":
+        return G.OP_DWITHIN, [layer, layer, "dwithin", repr(kw["d"]), "0", "distinct", w]
+    if name == "erode":
+        return G.OP_DWITHIN, [layer, layer, "buffer", repr(-kw["ra"]), repr(-kw["rb"]), "distinct", w]
+    return G.OP_CLUSTER, [layer, w]
+
+
+def wire_streams(args, spec, stream, rates):
+    """The windowed stream through the daemon over its unix socket, as the extension's clients would send it: gpujoind
+    with one lane and no batching (serial) and with `lanes` lanes and batching (served), each holding the national
+    parcel layer (LOAD of the generator spec); gsjload sends every request on its own connection at its Poisson arrival
+    time and measures its latency at the client, from the last request byte to the end-of-stream stats frame, with
+    every row drained. Refined pairs/s = the daemon's reported candidates over the client's wall time."""
+    import subprocess
+    import tempfile
+    from gsj import client as G
+
+    daemon = os.path.join(ROOT, "postgis-cuda_amd", "bin", "gpujoind")
+    load = os.path.join(ROOT, "postgis-cuda_amd", "bin", "gsjload")
+    out = {"measured": "client side over the daemon's unix socket: last request byte -> end-of-stream stats frame, all rows drained",
+           "self_pairs": "distinct (each unordered pair once)"}
+    lines = [wire_line(name, kw) for name, kw in stream]
+    for mode, lanes, batch in (("serial", 1, 1), ("served", args.lanes, args.batch)):
+        d = tempfile.mkdtemp()
+        sock = os.path.join(d, "gpujoind.sock")
+        with open(os.path.join(d, "daemon.log"), "w") as logf:
+            proc = subprocess.Popen([daemon, "--socket", sock, "--data-dir", d, "--lanes", str(lanes), "--batch", str(batch), "--threads", "8"],
+                                    stdout=logf, stderr=logf)
+            try:
+                for _ in range(600):
+                    if os.path.exists(sock) or proc.poll() is not None:
+                        break
+                    time.sleep(0.1)
+                assert proc.poll() is None, open(os.path.join(d, "daemon.log")).read()
+                t0 = time.time()
+                info = G.Client(sock).call(G.OP_LOAD, "n", spec, "id", "geom", "", 0, "false")
+                log(f"wire {mode}: layer loaded in {time.time() - t0:.1f}s ({info['rows']} rows)")
+                for rate in rates:
+                    arrivals = np.cumsum(np.random.default_rng(40).exponential(1.0 / rate, len(lines)))
+                    arrivals -= arrivals[0]
+                    text = "\n".join("\t".join([f"{t:.6f}", str(op)] + a) for t, (op, a) in zip(arrivals, lines)) + "\n"
+                    r = subprocess.run([load, "--socket", sock], input=text, capture_output=True, text=True, timeout=900)
+                    recs = [json.loads(x) for x in r.stdout.splitlines() if x.strip()]
+                    summ = recs[-1]
+                    cand = sum(int(x["stats"].get("candidates", 0)) for x in recs[:-1] if x.get("stats"))
+                    res = {"lanes": lanes, "batch": batch, "wall_s": summ["wall_s"], "refined_pairs_per_s": round(cand / summ["wall_s"], 1),
+                           "rows_per_s": round(summ["rows"] / summ["wall_s"], 1), "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"],
+                           "candidates": cand, "rows": summ["rows"], "failed": summ["failed"]}
+                    out.setdefault(f"{rate:g}/s", {})[mode] = res
+                    log(f"wire {mode} {rate:g}/s: {res}")
+            finally:
+                proc.terminate()
+                proc.wait(timeout=120)
+    for key, v in out.items():
+        if isinstance(v, dict) and "serial" in v and "served" in v:
+            v["speedup_pairs_per_s"] = round(v["served"]["refined_pairs_per_s"] / max(v["serial"]["refined_pairs_per_s"], 1e-9), 3)
+            v["p99_ratio_served_over_serial"] = round(v["served"]["p99_ms"] / max(v["serial"]["p99_ms"], 1e-9), 3)
     return out
 
 
@@ -325,6 +586,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline and parity legs (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="host work of the cpu_baseline leg (repeats of the join)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-vs-serial leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the parcels x regions and points / dwithin legs")
+    ap.add_argument("--no-wire", action="store_true", help="skip the national stream through the daemon's socket")
     ap.add_argument("--no-prep", action="store_true", help="skip the host-vs-device layer preparation timings (profiling runs)")
     ap.add_argument("--requests", type=int, default=256, help="requests in the served-vs-serial mix")
     ap.add_argument("--smax", type=float, default=2e6, help="largest request (polygons) in the mix")
@@ -470,6 +733,12 @@ def main():
                   "candidates_equal": got.stats["candidates"] == exp["candidates"]}
         log(f"cpu baseline {cpu['value']:.0f} pairs/s on {threads} threads; parity {parity}; roofline {roofline}")
 
+    configs = {}
+    if not args.no_configs:   # the parcels x regions and the points / distance configurations, each with its own roofline
+        configs["config1"] = config1_leg(gsj, ctx, args)
+        log(f"config 1: {configs['config1']}")
+        configs["config3"] = config3_leg(gsj, ctx, args, L, S)
+        log(f"config 3: {configs['config3']}")
     serving = None if args.no_serve else serving_leg(gsj, ctx, args)
     if serving:
         log(f"serving: serial {serving['serial']}; served {serving['served']}; speedup {serving['speedup_pairs_per_s']}")
@@ -491,7 +760,7 @@ def main():
         "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1), "prep": prep},
         "roofline_other": other_rooflines(S.n, st, t_bvh),
-        "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, "serving": serving, "national": national,
+        "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, **configs, "serving": serving, "national": national,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
