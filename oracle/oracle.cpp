@@ -193,10 +193,19 @@ static PairOut refine_pair(const Layer& P, const Layer& R, uint32_t a, uint32_t 
       Contrib c = edge_contrib(e.x0, e.y0, e.x1, e.y1, LB, cx, cy, inB_c, ov, tol, rp.eps_par, !e.syn, rel, e.cut, snap);
       s2 += c.acc; ai += c.lin; ao += c.lon; bits |= c.bits;
     }
+#ifdef ORC_COUNT_FLOPS
+    // a polygon with itself: side B repeats side A's computation exactly, and the engine copies A's results into B's
+    // slots instead, so B's operations are not counted
+    const FlopCount before_b = g_fc;
+    const bool self_pair = &P == &R && a == b;
+#endif
     for (const Edge& e : LB) {
       Contrib c = edge_contrib(e.x0, e.y0, e.x1, e.y1, LA, cx, cy, inA_c, ov, tol, rp.eps_par, !e.syn, rel, e.cut, snap);
       s2 += c.acc; bi += c.lin; bo += c.lon; bits |= c.bits;
     }
+#ifdef ORC_COUNT_FLOPS
+    if (self_pair) g_fc = before_b;
+#endif
     ar = 0.5 * s2;
     e4[0] = ai; e4[1] = ao; e4[2] = bi; e4[3] = bo;
     eb4 = bits;

@@ -292,6 +292,10 @@ __device__ __forceinline__ double pair_snap(double2 o, double2 e) {   // engine/
   return 0.5 * (nextafter(m, INFINITY) - m);
 }
 
+// a polygon paired with itself (a self-join's diagonal): both sides see the same edges in the same frame, so every
+// per-edge result of side B equals side A's and is computed once
+__device__ __forceinline__ bool is_self(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b) { return a == b && P.xy == R.xy; }
+
 struct PairSetup {
   Box ov;
   double tol, snap, cx, cy, bshx, bshy, ashx, ashy;
@@ -462,9 +466,12 @@ __global__ void __launch_bounds__(256, 5) k_screen_small(const DPolys P, const D
     ps.prefetch();
     const uint32_t k = ps.k, p = ps.p, a = ps.a, b = ps.b;
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
-    const int na = SA.ni, nb = SB.ni;   // edges meeting the overlap box
-    const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SB.V, SB.F, SB.I, nb};
+    const bool self = is_self(P, R, a, b);
+    if (self) load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
+    else load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
+    const SP& SBx = self ? SA : SB;
+    const int na = SA.ni, nb = SBx.ni;   // edges meeting the overlap box
+    const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SBx.V, SBx.F, SBx.I, nb};
     // boundary contact over the A x B edge grid
     bool hit = false, hit_any = false;
     const int grid = na * nb;
@@ -488,17 +495,17 @@ __global__ void __launch_bounds__(256, 5) k_screen_small(const DPolys P, const D
     const bool rtouch = hit;
     bool inB_c = false, inA_c = false;
     if (!hit || rp.want_area || rel) {
-      inB_c = pip_small<G, CAP>(SB, f.bshx, f.cx, f.cy, gl);
       inA_c = pip_small<G, CAP>(SA, f.ashx, f.cx, f.cy, gl);
+      inB_c = self ? inA_c : pip_small<G, CAP>(SB, f.bshx, f.cx, f.cy, gl);
     }
     bool contain_any = false;
     if (!hit) {
       bool h = false, ha = false;
-      const int nra = SA.nr, nrb = SB.nr;
+      const int nra = SA.nr, nrb = SBx.nr;
       for (int r = gl; r < nra + nrb; r += G) {
         const bool sideA = r < nra;
         double x, y;
-        const int pr = probe_small(sideA ? SA : SB, sideA ? r : r - nra, (sideA ? P.synb : R.synb) != nullptr, x, y);
+        const int pr = probe_small(sideA ? SA : SBx, sideA ? r : r - nra, (sideA ? P.synb : R.synb) != nullptr, x, y);
         const bool real = pr == 2;
         if (pr && x >= f.ov.x0 && x <= f.ov.x1 && y >= f.ov.y0 && y <= f.ov.y1 &&
             inside_from_c(sideA ? LB : LA, f.cx, f.cy, sideA ? inB_c : inA_c, x, y)) {
@@ -543,21 +550,29 @@ __global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DP
     const uint32_t p = ps.p, a = ps.a, b = ps.b;
     const uint8_t st = out.isect[p];
     const PairSetup f = setup_pair(P, R, a, b, rp);
-    load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
-    const int na = SA.ni, nb = SB.ni;
-    const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SB.V, SB.F, SB.I, nb};
+    const bool self = is_self(P, R, a, b);
+    if (self) load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
+    else load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
+    const SP& SBx = self ? SA : SB;
+    const int na = SA.ni, nb = SBx.ni;
+    const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SBx.V, SBx.F, SBx.I, nb};
     EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, (st & ST_INB) != 0, rel}, cb = ca;
     cb.in_c = (st & ST_INA) != 0;
     uint32_t bits = 0;
-    for (int s = gl; s < na + nb; s += G) {
+    for (int s = gl; s < (self ? na : na + nb); s += G) {
       // one call with the side chosen per lane: a wave holds A- and B-edge lanes in the same round, and two calls
-      // would run one after the other
+      // would run one after the other. A polygon with itself: B's edge s computes exactly A's (same edge, same list,
+      // same containment of c), so A's result fills both slots.
       const bool sideA = s < na;
       Ed e;
       (sideA ? LA : LB).get(sideA ? s : s - na, e);
       const Contrib c = edge_contrib<VertSrc, false>(e, sideA ? LB : LA, sideA ? ca : cb);
       acc[s] = c.acc;
       if (rel) { lin[s] = c.lin; lon[s] = c.lon; }
+      if (self) {
+        acc[na + s] = c.acc;
+        if (rel) { lin[na + s] = c.lin; lon[na + s] = c.lon; }
+      }
       bits |= c.bits;
     }
     wave_lds_sync();
@@ -622,10 +637,11 @@ __device__ bool refine_lists(const DPolys& P, const DPolys& R, uint32_t a, uint3
   hit_any = gany<G>(hit_any || hit);
   const bool rtouch = hit;
   const bool rel = rp.want_relate != 0;
+  const bool self = is_self(P, R, a, b);
   bool inB_c = false, inA_c = false;
   if (!hit || rp.want_area || rel) {
-    inB_c = pip_full<G>(R, b, f.bshx, f.bshy, f.cx, f.cy, gl);
     inA_c = pip_full<G>(P, a, f.ashx, f.ashy, f.cx, f.cy, gl);
+    inB_c = self ? inA_c : pip_full<G>(R, b, f.bshx, f.bshy, f.cx, f.cy, gl);
   }
   bool contain_any = false;
   if (!hit) {
@@ -652,12 +668,16 @@ __device__ bool refine_lists(const DPolys& P, const DPolys& R, uint32_t a, uint3
   if (computed) {
     EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, inB_c, rel}, cb = ca;
     cb.in_c = inA_c;
-    for (int s = gl; s < na + nb; s += G) {
+    for (int s = gl; s < (self ? na : na + nb); s += G) {   // self: A's results fill B's slots too
       const bool sideA = s < na;
       const Ed e = (sideA ? S.LA : S.LB)[sideA ? s : s - na];
       const Contrib c = edge_contrib<ListSrc, CAPPED>(e, ListSrc{sideA ? S.LB : S.LA, sideA ? nb : na}, sideA ? ca : cb);
       S.acc[s] = c.acc;
       if (rel) { S.lin[s] = c.lin; S.lon[s] = c.lon; }
+      if (self) {
+        S.acc[na + s] = c.acc;
+        if (rel) { S.lin[na + s] = c.lin; S.lon[na + s] = c.lon; }
+      }
       bits |= c.bits;
     }
     wave_lds_sync();
@@ -780,8 +800,8 @@ __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPo
       bool hit = rtouch;
       bool inB_c = false, inA_c = false;
       if (!hit || rp.want_area || rel) {
-        inB_c = pip_full<64>(R, b, f.bshx, f.bshy, f.cx, f.cy, gl);
         inA_c = pip_full<64>(P, a, f.ashx, f.ashy, f.cx, f.cy, gl);
+        inB_c = is_self(P, R, a, b) ? inA_c : pip_full<64>(R, b, f.bshx, f.bshy, f.cx, f.cy, gl);
       }
       bool contain_any = false;
       if (!hit) {
@@ -813,12 +833,17 @@ __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPo
       EdgeCtx ca{f.cx, f.cy, f.tol, rp.eps_par, f.snap, f.ov, s_inB != 0, rel}, cb = ca;
       cb.in_c = s_inA != 0;
       uint32_t bits = 0;
-      for (int s = t; s < na + nb; s += T) {
+      const bool self = is_self(P, R, a, b);
+      for (int s = t; s < (self ? na : na + nb); s += T) {   // self: A's results fill B's slots too
         const bool sideA = s < na;
         const Ed e = (sideA ? S.LA : S.LB)[sideA ? s : s - na];
         const Contrib c = edge_contrib<ListSrc, true>(e, ListSrc{sideA ? S.LB : S.LA, sideA ? nb : na}, sideA ? ca : cb);
         S.acc[s] = c.acc;
         if (rel) { S.lin[s] = c.lin; S.lon[s] = c.lon; }
+        if (self) {
+          S.acc[na + s] = c.acc;
+          if (rel) { S.lin[na + s] = c.lin; S.lon[na + s] = c.lon; }
+        }
         bits |= c.bits;
       }
       if (bits) atomicOr(&s_bits, bits);
