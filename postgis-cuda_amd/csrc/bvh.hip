@@ -256,6 +256,9 @@ void bvh_build(Layer& L) {
   L.bvh.nodes = nodes;
   L.bvh.order = order;
   L.bvh.tiles = tiles;
+  L.h_tiles.resize((n + PROBE_TILE - 1) / PROBE_TILE);
+  GSJ_HIP(hipMemcpyAsync(L.h_tiles.data(), tiles, L.h_tiles.size() * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
+  GSJ_HIP(hipStreamSynchronize(st));
   L.has_bvh = true;
 }
 

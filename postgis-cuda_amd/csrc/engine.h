@@ -85,6 +85,7 @@ struct Layer {
   DBvh bvh;
   bool has_bvh = false;
   std::vector<uint32_t> h_parent;   // host copy: keeps a parent's pieces in one probe batch
+  std::vector<float4> h_tiles;      // host copy of the probe tile boxes (scratch estimates of windowed requests)
   ~Layer();
 };
 

@@ -237,6 +237,11 @@ void layer_upload_points(Layer& L, const double* xy, uint64_t n) {
   }
   GSJ_HIP(hipStreamSynchronize(c.stream));
   L.pts.n = (uint32_t)n; L.pts.xy = dxy; L.pts.bb = dbb; L.pts.order = n ? order : nullptr; L.pts.tiles = n ? tiles : nullptr;
+  L.h_tiles.resize(n ? nt : 0);
+  if (n) {
+    GSJ_HIP(hipMemcpyAsync(L.h_tiles.data(), tiles, nt * sizeof(float4), hipMemcpyDeviceToHost, c.stream));
+    GSJ_HIP(hipStreamSynchronize(c.stream));
+  }
 }
 
 }  // namespace gsj

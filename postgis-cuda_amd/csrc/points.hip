@@ -171,9 +171,16 @@ void points_join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_joi
   for_each_candidate_batch(c, probes_of(PL), RL.bvh, false, o, s, [&](uint32_t* pa, uint32_t* pb, uint64_t np) {
       WallTimer t2;
       uint8_t* hit = c.s_flag.as<uint8_t>(np + 16);
+      GSJ_HIP(hipEventRecord(c.ev[0], stream));
       hipLaunchKernelGGL(k_pip, dim3(div_up(np, 256)), dim3(256), 0, stream, PL.pts, RL.poly, pa, pb, np, hit);
       GSJ_LAUNCHED();
+      GSJ_HIP(hipEventRecord(c.ev[1], stream));
       GSJ_HIP(hipStreamSynchronize(stream));
+      float ms = 0;
+      GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+      s.k_small_ms += ms;
+      s.n_small += np;
+      s.launches++;
       s.refine_ms += t2.ms();
       WallTimer t3;
       uint32_t* f = c.s_tmp.as<uint32_t>(np + 16);

@@ -38,9 +38,17 @@ struct Req {
 };
 
 // Scratch estimate of a request: candidate pairs bounded by the sub-batch limit, ~120 B of pair state each
-// (pairs, flags, areas, evidence, partition lists, compaction keys), plus per-probe traversal counters.
+// (pairs, flags, areas, evidence, partition lists, compaction keys), plus per-probe traversal counters. The probes are
+// the probe layer's, or for a window over a layer with tile boxes, the probes of the tiles meeting the window.
 uint64_t estimate_scratch(const gsj_request& r) {
-  const uint64_t nq = r.P ? r.P->L.n : 0;
+  uint64_t nq = r.P ? r.P->L.n : 0;
+  const std::vector<float4>& tiles = r.P ? r.P->L.h_tiles : std::vector<float4>();
+  if (r.opts.has_window && !tiles.empty()) {
+    const double* w = r.opts.window;
+    uint64_t meet = 0;
+    for (const float4& b : tiles) meet += ((double)b.x <= w[2] && (double)b.z >= w[0] && (double)b.y <= w[3] && (double)b.w >= w[1]) ? 1 : 0;
+    nq = std::min<uint64_t>(nq, meet * gsj::PROBE_TILE);
+  }
   const uint64_t max_pairs = r.opts.max_pairs ? r.opts.max_pairs : (128ull << 20);
   const uint64_t pairs = std::min<uint64_t>(max_pairs, 8 * nq + 1024);
   return pairs * 120 + nq * 24;
@@ -270,7 +278,7 @@ int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket) {
                 "layer belongs to another context than the scheduler's");
     auto q = std::make_unique<Req>();
     q->r = r;
-    q->est = estimate_scratch(r);
+    q->est = s->budget ? estimate_scratch(r) : 0;   // admission by estimate only under a budget
     q->t_submit = Clock::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);

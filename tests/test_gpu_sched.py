@@ -335,3 +335,35 @@ def test_windowed_mix_over_lanes_with_batching(ctx):
             d = ctx.cluster_rows(L, **kw)
             assert np.array_equal(g.pairs, d.pairs)
         assert g.stats["candidates"] == d.stats["candidates"] and g.stats["hits"] == d.stats["hits"]
+
+
+def test_budget_admits_windowed_requests_by_their_windows(ctx):
+    """Under a scratch budget a windowed request is estimated from the probes of the tiles its window meets, not from
+    the whole layer: two small windows fit the budget together and run side by side; two windows covering the layer
+    do not, so the second waits until the first has finished (each still runs: a request alone is always admitted)."""
+    import time
+
+    hs = _mesh(150, 4)
+    L = ctx.polys(hs)
+    x0, y0 = hs.xy[:, 0].min(), hs.xy[:, 1].min()
+    x1, y1 = hs.xy[:, 0].max(), hs.xy[:, 1].max()
+    small = [(x0, y0, x0 + 0.2 * (x1 - x0), y0 + 0.2 * (y1 - y0)), (x1 - 0.2 * (x1 - x0), y1 - 0.2 * (y1 - y0), x1, y1)]
+    full = [(x0, y0, x1, y1)] * 2
+    s = gsj.Scheduler(ctx, lanes=2, scratch_budget=16 << 20, batch=1)   # small windows ~5 MB each, full ~22 MB
+    try:
+        spans = {}
+        for name, wins in (("small", small), ("full", full)):
+            t_sub, tickets = [], []
+            for w in wins:
+                t_sub.append(time.perf_counter())
+                tickets.append(s.submit(gsj.REQ_JOIN, L, L, want_relate=True, upper=True, window=w))
+            res = [s.wait(t) for t in tickets]
+            spans[name] = [(t + r.stats["times"]["queue_ms"] / 1e3, t + (r.stats["times"]["queue_ms"] + r.stats["times"]["run_ms"]) / 1e3)
+                           for t, r in zip(t_sub, res)]
+            assert all(len(r.pairs) > 0 for r in res)
+        (a0, a1), (b0, b1) = spans["small"]
+        assert b0 < a1, spans   # side by side
+        (a0, a1), (b0, b1) = spans["full"]
+        assert b0 >= a1 - 2e-4, spans   # one after the other
+    finally:
+        s.close()
