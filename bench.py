@@ -40,16 +40,10 @@ def mesh(gsj, n, seed=1, subdiv=256):
 
 
 def pair_class(hs, pairs):
-    """Refinement work class of each candidate pair of a self-join of hs: 0 small (both pieces <= 32 stored vertices),
-    1 medium (both <= 64), both with <= 8 rings; 2 mixed (one piece <= 32 vertices); 3 large."""
-    rings = np.diff(hs.gro.astype(np.int64))
-    verts = hs.rvo[hs.gro[1:]].astype(np.int64) - hs.rvo[hs.gro[:-1]].astype(np.int64)
-    va, vb = verts[pairs[:, 0]], verts[pairs[:, 1]]
-    rok = (rings[pairs[:, 0]] <= 8) & (rings[pairs[:, 1]] <= 8)
-    small = rok & (va <= 32) & (vb <= 32)
-    med = ~small & rok & (va <= 64) & (vb <= 64)
-    mixed = ~small & ~med & (np.minimum(va, vb) <= 32)
-    return np.where(small, 0, np.where(med, 1, np.where(mixed, 2, 3)))
+    """Refinement work class of each candidate pair of a self-join of hs (the engine's partition): 4 tiny (both pieces
+    <= 16 stored vertices), 0 small (both <= 32), 1 medium (both <= 64), all three with <= 8 rings; 2 mixed (one piece
+    <= 32 vertices); 3 large."""
+    return pair_class2(hs, hs, pairs)
 
 
 def pair_class2(hp, hr, pairs):
@@ -59,10 +53,11 @@ def pair_class2(hp, hr, pairs):
     (ra, va), (rb, vb) = shape(hp), shape(hr)
     va, vb = va[pairs[:, 0]], vb[pairs[:, 1]]
     rok = (ra[pairs[:, 0]] <= 8) & (rb[pairs[:, 1]] <= 8)
-    small = rok & (va <= 32) & (vb <= 32)
-    med = ~small & rok & (va <= 64) & (vb <= 64)
-    mixed = ~small & ~med & (np.minimum(va, vb) <= 32)
-    return np.where(small, 0, np.where(med, 1, np.where(mixed, 2, 3)))
+    tiny = rok & (va <= 16) & (vb <= 16)
+    small = ~tiny & rok & (va <= 32) & (vb <= 32)
+    med = ~tiny & ~small & rok & (va <= 64) & (vb <= 64)
+    mixed = ~tiny & ~small & ~med & (np.minimum(va, vb) <= 32)
+    return np.where(tiny, 4, np.where(small, 0, np.where(med, 1, np.where(mixed, 2, 3))))
 
 
 def time_cpu(fn, seconds):
@@ -181,11 +176,11 @@ def config3_leg(gsj, ctx, args, L, S):
         r = ctx.dwithin(L12, L12, 5.0, **kw)
     wall = time.perf_counter() - t0
     st = r.stats
-    kms = {k: st[f"k_{k}_ms"] for k in ("small", "medium", "mixed", "large", "over")}
+    kms = {k: st[f"k_{k}_ms"] for k in ("tiny", "small", "medium", "mixed", "large", "over")}
     kbig = max(kms, key=kms.get)
     nv_geom = M12.nv / M12.n
     bytes_dw = 8 + 2 * (40 + 16 * nv_geom) + 1 + 8
-    n_k = {"small": "n_small", "medium": "n_medium", "mixed": "n_mixed", "large": "n_large", "over": "n_over"}[kbig]
+    n_k = {"tiny": "n_tiny", "small": "n_small", "medium": "n_medium", "mixed": "n_mixed", "large": "n_large", "over": "n_over"}[kbig]
     k_ms = kms[kbig] / max(st["launches"], 1)
     gbs = bytes_dw * st[n_k] / max(st["launches"], 1) / (k_ms * 1e-3) / 1e9
     OM = O.Layer.polys(M12)
@@ -662,15 +657,17 @@ def main():
     ms_step = 1e3 * t_all / args.steps
     value = cand * args.steps / t_all
     log(f"step {ms_step:.2f} ms: {cand} candidates, {rows} rows; traverse {st['traverse_ms']:.2f} refine {st['refine_ms']:.2f} "
-        f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels screen {st['k_screen_ms']:.2f} small {st['k_small_ms']:.2f} "
-        f"medium {st['k_medium_ms']:.2f} mixed {st['k_mixed_ms']:.2f} large {st['k_large_ms']:.2f} over {st['k_over_ms']:.2f} ms (pairs "
-        f"{st['n_small']}/{st['n_medium']}/{st['n_mixed']}/{st['n_large']}/{st['n_over']}, edge work {st['n_small_edge']}/{st['n_medium_edge']})")
+        f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels screen {st['k_screen_ms']:.2f} tiny {st['k_tiny_ms']:.2f} "
+        f"small {st['k_small_ms']:.2f} medium {st['k_medium_ms']:.2f} mixed {st['k_mixed_ms']:.2f} large {st['k_large_ms']:.2f} "
+        f"over {st['k_over_ms']:.2f} ms (pairs {st['n_tiny']}/{st['n_small']}/{st['n_medium']}/{st['n_mixed']}/{st['n_large']}/{st['n_over']}, "
+        f"edge work {st['n_tiny_edge']}/{st['n_small_edge']}/{st['n_medium_edge']})")
 
     # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator):
     # (name, duration field, pair class of the sample, pairs field, FLOPs counted: edge phase / screen part / all)
-    kernels = (("k_screen_small", "k_screen_ms", (0, 1), ("n_small", "n_medium"), "screen"),
-               ("k_edges_small<16, 32, 0>", "k_small_ms", (0,), ("n_small_edge",), "edge"),
-               ("k_edges_small<32, 64, 1>", "k_medium_ms", (1,), ("n_medium_edge",), "edge"),
+    kernels = (("k_screen_small", "k_screen_ms", (4, 0, 1), ("n_tiny", "n_small", "n_medium"), "screen"),
+               ("k_edges_small<8, 16, 0>", "k_tiny_ms", (4,), ("n_tiny_edge",), "edge"),
+               ("k_edges_small<16, 32, 1>", "k_small_ms", (0,), ("n_small_edge",), "edge"),
+               ("k_edges_small<32, 64, 2>", "k_medium_ms", (1,), ("n_medium_edge",), "edge"),
                ("k_refine_mixed", "k_mixed_ms", (2,), ("n_mixed",), "all"),
                ("k_refine_large", "k_large_ms", (3,), ("n_large",), "all"))
     kern = max(kernels, key=lambda k: st[k[1]])
@@ -757,7 +754,8 @@ def main():
                    "pieces": S.n, "vertices": S.nv, "candidates_per_step": cand, "rows_per_step": rows, "parallelism": "1 GPU"},
         "rows_per_s": round(rows * args.steps / t_all, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
-        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_small_ms", "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
+        "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_tiny_ms", "k_small_ms",
+                                                      "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
         "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1), "prep": prep},
         "roofline_other": other_rooflines(S.n, st, t_bvh),
         "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, **configs, "serving": serving, "national": national,

@@ -43,7 +43,9 @@ const char* gsj_version(void);
 typedef struct gsj_ctx gsj_ctx;
 typedef struct gsj_layer gsj_layer;
 
-/* Create a context on HIP device `device` (its own stream, default priority). */
+/* Create a context on HIP device `device` (its own stream, default priority). Its launch-shape table comes from the
+ * file named by the environment variable GSJ_TUNING, else from tuning/<arch>.txt beside the library's directory
+ * (postgis-cuda_amd/tuning/gfx950.txt in-tree), else the built-in shapes; a malformed file fails with GSJ_E_ARG. */
 int gsj_ctx_create(int device, gsj_ctx** out);
 void gsj_ctx_destroy(gsj_ctx* ctx);
 /* JSON description of the device and memory (reference op_status, daemon/gpujoind.cu:496-504); the returned
@@ -51,6 +53,15 @@ void gsj_ctx_destroy(gsj_ctx* ctx);
 const char* gsj_ctx_status_json(gsj_ctx* ctx);
 /* Release the context's cached scratch buffers (they are grow-only and reused across calls). */
 int gsj_ctx_trim(gsj_ctx* ctx);
+/* Launch shapes of the pair kernels per launch size, as text: '#' comments and one line per size bucket,
+ * "bucket np>=N key=value ..." with keys tcap (0..16), mcap (0 or 33..64), screen_occ (4, 5), screen_bpc, edge_bpc,
+ * mixed_bpc, large_bpc, dw_bpc (blocks per CU, 1..64) and trav_block (64, 128, 256); buckets ascend from np>=0.
+ * Replaces the reference's fixed launch shapes (engine/engine.cuh:945-951, engine/buffer.cuh:946-947); every shape
+ * gives the same rows. set: GSJ_E_ARG on malformed text (the table is unchanged); applies to later calls and to
+ * schedulers created afterwards. get: the table in effect; the string is owned by the context, valid until the next
+ * call of gsj_ctx_tuning on it. */
+int gsj_ctx_set_tuning(gsj_ctx* ctx, const char* text);
+const char* gsj_ctx_tuning(gsj_ctx* ctx);
 
 /* ------------------------------------------------------------------ layers
  * Polygon layer from the flat .gset arrays (engine/gset.h:5-11): gro[n+1] geometry -> first ring, rvo[nr+1] ring
@@ -110,7 +121,7 @@ typedef struct {
    * k_screen_ms, every small and medium pair) and k_small_ms / k_medium_ms time their edge-contribution kernels over
    * the n_small_edge / n_medium_edge pairs that need one; large = a wave per pair; over = the global-memory overflow
    * pass; mixed = 16-lane groups with edge lists for pairs of a small polygon (<= 32 vertices) and a larger one.
-   * Distance joins: small, medium, mixed and large list classes. Point joins: the per-pair kernel as small. */
+   * Distance joins: tiny, small, medium, mixed and large list classes. Point joins: the per-pair kernel as small. */
   double k_small_ms, k_large_ms, k_over_ms;
   uint64_t n_small, n_large, n_over;
   uint64_t launches;       /* refinement launches (sub-batches) */
@@ -120,6 +131,10 @@ typedef struct {
   uint64_t n_small_edge, n_medium_edge;
   double k_mixed_ms;
   uint64_t n_mixed;
+  /* tiny = 8-lane class (both polygons <= 16 stored vertices and <= 8 rings), screened with the small and medium
+   * classes; k_tiny_ms times its edge-contribution kernel over n_tiny_edge pairs (distance joins: its list kernel) */
+  double k_tiny_ms;
+  uint64_t n_tiny, n_tiny_edge;
 } gsj_join_stats;
 
 /* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;

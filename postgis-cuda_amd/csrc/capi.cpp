@@ -49,7 +49,13 @@ void context_init(Context& c, int device) {
   GSJ_HIP(hipSetDevice(device));
   c.device = device;
   hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, device) == hipSuccess) c.cu_count = p.multiProcessorCount;
+  std::string arch;
+  if (hipGetDeviceProperties(&p, device) == hipSuccess) {
+    c.cu_count = p.multiProcessorCount;
+    arch = p.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));   // "gfx950:sramecc+:xnack-" -> "gfx950"
+  }
+  c.tune = load_tuning(arch);
   GSJ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (hipEvent_t& e : c.ev) GSJ_HIP(hipEventCreate(&e));
   noalloc_setup(device);
@@ -114,6 +120,22 @@ const char* gsj_ctx_status_json(gsj_ctx* ctx) {
   });
   if (rc != GSJ_OK) ctx->c.status = "{}";
   return ctx->c.status.c_str();
+}
+
+int gsj_ctx_set_tuning(gsj_ctx* ctx, const char* text) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && text, GSJ_E_ARG, "null context or text");
+    gsj::Tuning t = gsj::Tuning::parse(text, "gsj_ctx_set_tuning");
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    ctx->c.tune = std::move(t);
+  });
+}
+
+const char* gsj_ctx_tuning(gsj_ctx* ctx) {
+  if (!ctx) return "";
+  std::lock_guard<std::mutex> lk(ctx->c.mu);
+  ctx->c.tune_text = ctx->c.tune.text();
+  return ctx->c.tune_text.c_str();
 }
 
 int gsj_ctx_trim(gsj_ctx* ctx) {

@@ -209,8 +209,8 @@ __device__ __forceinline__ bool self_piece_pair(const DwArgs& A, uint32_t a, uin
   return qa == qb;
 }
 
-// list classes: a G-lane group per pair, CAP-edge LDS lists per side; class `which` (0 small, 1 medium, 2 mixed,
-// 3 large) of the partition; pairs whose lists overflow are queued
+// list classes: a G-lane group per pair, CAP-edge LDS lists per side; class `which` (PairClass) of the partition;
+// pairs whose lists overflow are queued
 template <int G, int CAP>
 __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, int which,
                                                   uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
@@ -220,9 +220,7 @@ __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t
   Ed* LA = lds[gib];
   Ed* LB = LA + CAP;
   const uint32_t n = counts[which];
-  uint32_t first = 0;
-  for (int w = 0; w < which; w++) first += counts[w];
-  const uint32_t* L = list + first;
+  const uint32_t* L = list + class_first(counts, which);
   PairStream ps(L, A.pa, A.pb, n, blockIdx.x * gpb + gib, gridDim.x * gpb);
   for (; ps.k < n; ps.advance()) {
     ps.prefetch();
@@ -562,38 +560,45 @@ void dw_deliver(Context& c, const uint32_t* orow, const double* odist, uint64_t 
 // refine one sub-batch's candidates into A.hit / A.dist (list classes, then the overflow queue)
 void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
   hipStream_t stream = c.stream;
-  const PairClasses pc = partition_pairs(c, A.P, A.R, A.pa, A.pb, np, MCAP, true);
+  const LaunchShape& sh = c.shape(np);
+  const PairClasses pc = partition_pairs(c, A.P, A.R, A.pa, A.pb, np, sh.mcap, true, sh.tcap);
   uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
-  const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
+  const uint32_t max_blocks = (uint32_t)c.cu_count * sh.dw_bpc;
   GSJ_HIP(hipEventRecord(c.ev[0], stream));
-  hipLaunchKernelGGL((k_dw_lists<16, 32>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 0, ovf,
-                     pc.ctl);
+  hipLaunchKernelGGL((k_dw_lists<8, 16>), dim3(std::min<uint32_t>(div_up(np, 32), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_TINY,
+                     ovf, pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[8], stream));
+  hipLaunchKernelGGL((k_dw_lists<16, 32>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_SMALL,
+                     ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[1], stream));
-  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 1, ovf,
-                     pc.ctl);
+  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_MEDIUM,
+                     ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[5], stream));
-  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 2, ovf,
-                     pc.ctl);
+  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_MIXED,
+                     ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[7], stream));
-  hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, 3, ovf,
-                     pc.ctl);
+  hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_LARGE,
+                     ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[2], stream));
-  struct Host { uint32_t counts[4]; OverflowCtl ctl; };
+  struct Host { uint32_t counts[8]; OverflowCtl ctl; };
   Host* h = c.h_scal.as<Host>(2);
-  GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 16, hipMemcpyDeviceToHost, stream));
+  GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 32, hipMemcpyDeviceToHost, stream));
   GSJ_HIP(hipMemcpyAsync(&h->ctl, pc.ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, stream));
   GSJ_HIP(hipStreamSynchronize(stream));
-  float ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
-  GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[0], c.ev[1]));
+  float ms_t = 0, ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
+  GSJ_HIP(hipEventElapsedTime(&ms_t, c.ev[0], c.ev[8]));
+  GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[8], c.ev[1]));
   GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
   GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
   GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
-  s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_mixed_ms += ms_x; s.k_large_ms += ms_l;
-  s.n_small += h->counts[0]; s.n_medium += h->counts[1]; s.n_mixed += h->counts[2]; s.n_large += h->counts[3];
+  s.k_tiny_ms += ms_t; s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_mixed_ms += ms_x; s.k_large_ms += ms_l;
+  s.n_tiny += h->counts[CL_TINY]; s.n_small += h->counts[CL_SMALL]; s.n_medium += h->counts[CL_MEDIUM]; s.n_mixed += h->counts[CL_MIXED];
+  s.n_large += h->counts[CL_LARGE];
   s.launches++;
   const uint32_t nover = h->ctl.count;
   if (nover) {

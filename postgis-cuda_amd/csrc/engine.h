@@ -2,6 +2,7 @@
 #pragma once
 #include "common.h"
 #include "geom.h"
+#include "tune.h"
 #include <functional>
 #include <memory>
 #include <string>
@@ -102,9 +103,13 @@ struct Context {
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg, h_slots_ctl;
   int cu_count = 256;
-  hipEvent_t ev[8] = {};   // kernel timing events (created with the context)
+  hipEvent_t ev[10] = {};   // kernel timing events (created with the context)
   bool borrowed = false;   // stream and events belong to another context (not destroyed with this one)
   Arena* dev_arena = nullptr;   // set: every scratch buffer comes from this arena (the no-allocation entry point)
+  Tuning tune;                  // launch shapes of the pair kernels per launch size
+  const Tuning* tune_from = nullptr;   // set: launch with another context's shapes instead
+  const LaunchShape& shape(uint64_t np) const { return (tune_from ? *tune_from : tune).pick(np); }
+  std::string tune_text;        // last text handed out by gsj_ctx_tuning
   template <class F> void for_each_devbuf(F f) {
     for (DevBuf* b : {&s_counts, &s_qoffs, &s_offs, &s_pairs_a, &s_pairs_b, &s_flag, &s_area, &s_ev, &s_evb, &s_tmp, &s_tmp2, &s_cls, &s_out_a,
                       &s_out_b, &s_out_area, &s_out_code, &s_out_flag, &s_keys, &s_keys2, &s_vals, &s_vals2, &s_seg, &s_heavy, &s_heavy_edges,

@@ -190,13 +190,17 @@ __device__ __forceinline__ void queue_overflow(uint32_t p, int need, uint32_t* o
 
 
 // ------------------------------------------------------------------------------------------- work classes
+constexpr int TCAP = 16;             // tiny class: stored vertices per polygon (rings within RCAP)
 constexpr int VCAP = 32, RCAP = 8;   // small class: stored vertices and rings per polygon
 constexpr int MCAP = 64;             // medium class: stored vertices per polygon (rings within RCAP)
+// the classes, in list order
+enum PairClass : int { CL_TINY = 0, CL_SMALL = 1, CL_MEDIUM = 2, CL_MIXED = 3, CL_LARGE = 4, CL_COUNT = 5 };
 
-// Candidate pairs split by work class, on the device: list[0 .. counts[0]) the small pairs (both polygons within
-// VCAP / RCAP), then counts[1] medium pairs (not small, both within mcap / RCAP; none when mcap is 0), counts[2]
-// mixed pairs (with `mixed`: one polygon within VCAP vertices), then the counts[3] others; within a class by mean
-// vertex count, then candidate order; ctl cleared for the overflow queue.
+// Candidate pairs split by work class, on the device: list[0 .. counts[0]) the tiny pairs (both polygons within tcap
+// / RCAP; none when tcap is 0), then counts[1] small pairs (both within VCAP / RCAP), counts[2] medium pairs (both
+// within mcap / RCAP; none when mcap is 0), counts[3] mixed pairs (with `mixed`: one polygon within VCAP vertices),
+// then the counts[4] others; within a class by mean vertex count, then candidate order; ctl cleared for the overflow
+// queue. counts[8 ..) is left zeroed for the callers' queues.
 // Uses the context's s_keys, s_keys2, s_vals, s_vals2, s_tmp2 and s_misc scratch (the list lives in one of the
 // s_vals buffers until their next use).
 struct PairClasses {
@@ -205,7 +209,13 @@ struct PairClasses {
   OverflowCtl* ctl;
 };
 PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap,
-                            bool mixed);
+                            bool mixed, uint32_t tcap);
+// first list position of class cl
+__host__ __device__ __forceinline__ uint32_t class_first(const uint32_t* counts, int cl) {
+  uint32_t f = 0;
+  for (int w = 0; w < cl; w++) f += counts[w];
+  return f;
+}
 
 // group-wide minimum (order-independent, so deterministic)
 template <int G>

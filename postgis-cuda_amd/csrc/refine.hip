@@ -436,8 +436,8 @@ struct EdgeQueue {   // pairs with edge work: flag[k] per position k of the clas
   uint32_t* flag;    // flagged pairs in class-list order: class c's at list[first[c] .. first[c] + count[c])
   uint64_t* off;
   uint32_t* list;
-  uint32_t* count;   // [2]
-  uint32_t* first;   // [2]
+  uint32_t* count;   // [3]: tiny, small, medium
+  uint32_t* first;   // [3]
 };
 
 template <int G, int CAP>
@@ -447,8 +447,8 @@ __device__ __forceinline__ void load_pair(const DPolys& P, const DPolys& R, uint
   load_small<G, CAP>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
 }
 
-template <int G, int CAP, int WHICH>
-__global__ void __launch_bounds__(256, 5) k_screen_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+template <int G, int CAP, int WHICH, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_screen_small(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                       const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
                                                       RefineOut out, EdgeQueue q) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(256, 5) k_screen_small(const DPolys P, const D
   SP& SA = *reinterpret_cast<SP*>(smem + (size_t)gib * 2 * sizeof(SP));
   SP& SB = *(&SA + 1);
   const uint32_t nlist = counts[WHICH];
-  const uint32_t base = WHICH == 0 ? 0u : counts[0];
+  const uint32_t base = class_first(counts, WHICH);
   const uint32_t* cl = list + base;
   PairStream ps(cl, pa, pb, nlist, blockIdx.x * gpb + gib, gridDim.x * gpb);
   for (; ps.k < nlist; ps.advance()) {
@@ -585,13 +585,19 @@ __global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DP
   }
 }
 
-// the flagged pairs of the vertex-list classes in class-list order (stable compaction of the screen's flags)
+// the flagged pairs of the vertex-list classes (tiny, small, medium) in class-list order (stable compaction of the
+// screens' flags)
 __global__ void k_queue_scatter(const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, EdgeQueue q) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n = (uint64_t)counts[0] + counts[1];
+  const uint64_t n = (uint64_t)counts[CL_TINY] + counts[CL_SMALL] + counts[CL_MEDIUM];
   if (k == 0) {
-    const uint32_t n0 = (uint32_t)q.off[counts[0]], n1 = (uint32_t)(q.off[n] - q.off[counts[0]]);
-    q.first[0] = 0; q.count[0] = n0; q.first[1] = n0; q.count[1] = n1;
+    uint64_t lo = 0;
+    for (int cl = CL_TINY; cl <= CL_MEDIUM; cl++) {
+      const uint64_t hi = lo + counts[cl];
+      q.first[cl] = (uint32_t)q.off[lo];
+      q.count[cl] = (uint32_t)(q.off[hi] - q.off[lo]);
+      lo = hi;
+    }
   }
   if (k < n && q.flag[k]) q.list[q.off[k]] = list[k];
 }
@@ -698,8 +704,8 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
   const int wib = threadIdx.x / 64, gl = threadIdx.x % 64;
   const bool rel = rp.want_relate != 0;
   const Scratch S = carve(smem + (size_t)wib * list_bytes(cap, rel), cap, rel);
-  const uint32_t nclass = counts[3], nlist = nclass + promo_ctl->count;
-  const uint32_t* large = list + counts[0] + counts[1] + counts[2];   // the large class comes last
+  const uint32_t nclass = counts[CL_LARGE], nlist = nclass + promo_ctl->count;
+  const uint32_t* large = list + class_first(counts, CL_LARGE);   // the large class comes last
   for (uint32_t k = blockIdx.x * wpb + wib; k < nlist; k += gridDim.x * wpb) {
     const uint32_t p = k < nclass ? large[k] : promo[k - nclass];
     const uint32_t a = pa[p], b = pb[p];
@@ -730,8 +736,8 @@ __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const 
   const int gib = threadIdx.x / G, gl = threadIdx.x % G;
   const bool rel = rp.want_relate != 0;
   const Scratch S = carve(smem + (size_t)gib * list_bytes(CAP_MIXED, rel), CAP_MIXED, rel);
-  const uint32_t nlist = counts[2];
-  const uint32_t* cl = list + counts[0] + counts[1];
+  const uint32_t nlist = counts[CL_MIXED];
+  const uint32_t* cl = list + class_first(counts, CL_MIXED);
   for (uint32_t k = blockIdx.x * gpb + gib; k < nlist; k += gridDim.x * gpb) {
     const uint32_t p = cl[k];
     const uint32_t a = pa[p], b = pb[p];
@@ -856,13 +862,32 @@ __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPo
 
 }  // namespace
 
+namespace {
+// the three screens at one register budget (OCC waves per SIMD)
+template <int OCC>
+void launch_screens(uint32_t gt, uint32_t gs, uint32_t gm, hipStream_t st, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb,
+                    const uint32_t* list, const uint32_t* counts, const RefineParams& rp, const RefineOut& out, const EdgeQueue& eq) {
+  constexpr int T = 256;
+  hipLaunchKernelGGL((k_screen_small<8, TCAP, CL_TINY, OCC>), dim3(gt), dim3(T), (T / 8) * 2 * sizeof(SmallPoly<TCAP>), st, P, R, pa, pb, list, counts,
+                     rp, out, eq);
+  GSJ_LAUNCHED();
+  hipLaunchKernelGGL((k_screen_small<16, VCAP, CL_SMALL, OCC>), dim3(gs), dim3(T), (T / 16) * 2 * sizeof(SmallPoly<VCAP>), st, P, R, pa, pb, list,
+                     counts, rp, out, eq);
+  GSJ_LAUNCHED();
+  hipLaunchKernelGGL((k_screen_small<32, MCAP, CL_MEDIUM, OCC>), dim3(gm), dim3(T), (T / 32) * 2 * sizeof(SmallPoly<MCAP>), st, P, R, pa, pb, list,
+                     counts, rp, out, eq);
+  GSJ_LAUNCHED();
+}
+}  // namespace
+
 uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, const RefineParams& rp,
                       const RefineOut& out, gsj_join_stats* stats) {
   if (np == 0) return 0;
   GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "refine: more than 2^32 pairs in one launch");
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
-  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, MCAP, true);
+  const LaunchShape& sh = c.shape(np);
+  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, sh.mcap, true, sh.tcap);
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
@@ -870,38 +895,43 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   uint32_t* promo = ovf + np + 64;   // mixed pairs promoted to the large class
   OverflowCtl* promo_ctl = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);
   const int T = 256;
-  const uint32_t max_blocks = (uint32_t)c.cu_count * 16;
+  const uint32_t cu = (uint32_t)c.cu_count;
   GSJ_HIP(hipEventRecord(c.ev[0], st));
   EdgeQueue eq;
   eq.flag = c.s_edgeq.as<uint32_t>(2 * np + 32);
   eq.list = eq.flag + np + 16;
   eq.off = c.s_tmp2.as<uint64_t>(np + 17);
-  eq.count = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(counts) + 32);
-  eq.first = eq.count + 2;
-  const uint32_t gs = std::min<uint32_t>(div_up(np, T / 16), max_blocks), gm = std::min<uint32_t>(div_up(np, T / 32), max_blocks);
-  // screens (every small and medium pair), then the edge work of the flagged ones in class-list order
-  hipLaunchKernelGGL((k_screen_small<16, VCAP, 0>), dim3(gs), dim3(T), (T / 16) * 2 * sizeof(SmallPoly<VCAP>), st, P, R, pa, pb, list, counts, rp,
-                     out, eq);
-  GSJ_LAUNCHED();
-  hipLaunchKernelGGL((k_screen_small<32, MCAP, 1>), dim3(gm), dim3(T), (T / 32) * 2 * sizeof(SmallPoly<MCAP>), st, P, R, pa, pb, list, counts, rp,
-                     out, eq);
-  GSJ_LAUNCHED();
+  eq.count = counts + 8;
+  eq.first = eq.count + 3;
+  // grid-stride kernels: grids of one group per pair up to the shape's cap in blocks per CU
+  auto grid = [&](uint32_t lanes, uint32_t bpc) { return std::min<uint32_t>(div_up(np, T / lanes), cu * bpc); };
+  // screens (every tiny, small and medium pair), then the edge work of the flagged ones in class-list order
+  if (sh.screen_occ == 4) {
+    launch_screens<4>(grid(8, sh.screen_bpc), grid(16, sh.screen_bpc), grid(32, sh.screen_bpc), st, P, R, pa, pb, list, counts, rp, out, eq);
+  } else {
+    launch_screens<5>(grid(8, sh.screen_bpc), grid(16, sh.screen_bpc), grid(32, sh.screen_bpc), st, P, R, pa, pb, list, counts, rp, out, eq);
+  }
   // over np + 1 positions so that off[n] exists for n = np; flags past the two classes are stale and only reach
   // offsets that are never read
   scan_u32_to_u64_async(c, eq.flag, eq.off, np + 1, nullptr);
   hipLaunchKernelGGL(k_queue_scatter, dim3(div_up(np, 256)), dim3(256), 0, st, list, counts, eq);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[6], st));
-  hipLaunchKernelGGL((k_edges_small<16, VCAP, 0>), dim3(gs), dim3(T), (T / 16) * small_bytes<VCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
+  const uint32_t gt = grid(8, sh.edge_bpc), gs = grid(16, sh.edge_bpc), gm = grid(32, sh.edge_bpc);
+  hipLaunchKernelGGL((k_edges_small<8, TCAP, CL_TINY>), dim3(gt), dim3(T), (T / 8) * small_bytes<TCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
+                     ovf, ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[8], st));
+  hipLaunchKernelGGL((k_edges_small<16, VCAP, CL_SMALL>), dim3(gs), dim3(T), (T / 16) * small_bytes<VCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
                      ovf, ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[1], st));
-  hipLaunchKernelGGL((k_edges_small<32, MCAP, 1>), dim3(gm), dim3(T), (T / 32) * small_bytes<MCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
+  hipLaunchKernelGGL((k_edges_small<32, MCAP, CL_MEDIUM>), dim3(gm), dim3(T), (T / 32) * small_bytes<MCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
                      ovf, ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[5], st));
   {   // mixed: 16-lane groups with edge lists, 8 pairs per block
-    const uint32_t g = std::min<uint32_t>(div_up(np, MIXED_T / 16), max_blocks * 2);
+    const uint32_t g = std::min<uint32_t>(div_up(np, MIXED_T / 16), cu * sh.mixed_bpc);
     hipLaunchKernelGGL(k_refine_mixed, dim3(g), dim3(MIXED_T), (MIXED_T / 16) * list_bytes(CAP_MIXED, rel), st, P, R, pa, pb, list, counts, rp, out,
                        promo, promo_ctl, ovf, ctl);
     GSJ_LAUNCHED();
@@ -913,28 +943,31 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   constexpr int CAP_LARGE = 96;
   {
     const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
-    const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), max_blocks);
+    const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), cu * sh.large_bpc);
     hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, CAP_LARGE, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], st));
-  struct Host { uint32_t counts[12]; OverflowCtl ctl; };   // class sizes [0, 3), edge queue sizes [8, 10)
+  struct Host { uint32_t counts[16]; OverflowCtl ctl; };   // class sizes [0, 5), edge queue sizes [8, 11)
   Host* h = c.h_scal.as<Host>(2);
-  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 48, hipMemcpyDeviceToHost, st));
+  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 64, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipMemcpyAsync(&h->ctl, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
   const uint32_t nover = h->ctl.count, need = h->ctl.need;
   if (stats) {
-    float ms_q = 0, ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
+    float ms_q = 0, ms_t = 0, ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
     GSJ_HIP(hipEventElapsedTime(&ms_q, c.ev[0], c.ev[6]));
-    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[6], c.ev[1]));
+    GSJ_HIP(hipEventElapsedTime(&ms_t, c.ev[6], c.ev[8]));
+    GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[8], c.ev[1]));
     GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
     GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
     GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
-    stats->k_screen_ms += ms_q; stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_mixed_ms += ms_x; stats->k_large_ms += ms_l;
-    stats->n_small += h->counts[0]; stats->n_medium += h->counts[1]; stats->n_mixed += h->counts[2]; stats->n_large += h->counts[3];
+    stats->k_screen_ms += ms_q; stats->k_tiny_ms += ms_t; stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_mixed_ms += ms_x;
+    stats->k_large_ms += ms_l;
+    stats->n_tiny += h->counts[CL_TINY]; stats->n_small += h->counts[CL_SMALL]; stats->n_medium += h->counts[CL_MEDIUM];
+    stats->n_mixed += h->counts[CL_MIXED]; stats->n_large += h->counts[CL_LARGE];
     stats->n_over += nover;
-    stats->n_small_edge += h->counts[8]; stats->n_medium_edge += h->counts[9];
+    stats->n_tiny_edge += h->counts[8 + CL_TINY]; stats->n_small_edge += h->counts[8 + CL_SMALL]; stats->n_medium_edge += h->counts[8 + CL_MEDIUM];
     stats->launches++;
   }
   if (nover) {
@@ -962,32 +995,34 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
 }
 
 namespace {
-// sort key per candidate: class in bits 6-7 — 0 small (both polygons within VCAP stored vertices), 1 medium (not
-// small, both within mcap; none when mcap is 0), both with rings within RCAP; 2 mixed (when enabled: neither, one
-// polygon within VCAP stored vertices), 3 large — and the mean stored vertex count (saturated at 63) below, so that
-// the groups of a wave get pairs of similar work
+// sort key per candidate: class (PairClass, bits 6-8) — tiny (both polygons within tcap stored vertices; none when
+// tcap is 0), small (both within VCAP), medium (both within mcap; none when mcap is 0), all three with rings within
+// RCAP; mixed (when enabled: none of those, one polygon within VCAP stored vertices); large — and the mean stored vertex
+// count (saturated at 63) below, so that the groups of a wave get pairs of similar work
 __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                             uint32_t mcap, int mixed, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+                             uint32_t mcap, int mixed, uint32_t tcap, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
   const uint32_t va = ia & 0xFFFFu, vb = ib & 0xFFFFu;
   const bool rings = (ia >> 16) <= RCAP && (ib >> 16) <= RCAP;
-  const uint32_t cls = (rings && va <= VCAP && vb <= VCAP) ? 0u
-                       : (rings && va <= mcap && vb <= mcap) ? 1u
-                       : (mixed && min(va, vb) <= (uint32_t)VCAP) ? 2u : 3u;
+  const uint32_t cls = (rings && va <= tcap && vb <= tcap) ? CL_TINY
+                       : (rings && va <= VCAP && vb <= VCAP) ? CL_SMALL
+                       : (rings && va <= mcap && vb <= mcap) ? CL_MEDIUM
+                       : (mixed && min(va, vb) <= (uint32_t)VCAP) ? CL_MIXED : CL_LARGE;
   // within a class, pairs of similar size side by side (balanced groups), at a resolution per class: the large class
-  // spans hundreds of vertices, so its buckets are 8 vertices wide (2 for small and mixed, 4 for medium)
-  const uint32_t shift = cls == 3 ? 3u : (cls == 1 ? 2u : 1u);
+  // spans hundreds of vertices, so its buckets are 8 vertices wide (1 for tiny, 2 for small and mixed, 4 for medium)
+  const uint32_t shift = cls == CL_LARGE ? 3u : (cls == CL_MEDIUM ? 2u : (cls == CL_TINY ? 0u : 1u));
   key[i] = (uint64_t)((cls << 6) | min(63u, (va + vb) >> shift));
   val[i] = (uint32_t)i;
 }
 
-// class sizes from the sorted keys: first index of classes 1, 2 and 3 by binary search
+// class sizes from the sorted keys: first index of every class after the first by binary search
 __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, uint32_t* __restrict__ counts) {
-  uint64_t first[4];
+  uint64_t first[CL_COUNT + 1];
   first[0] = 0;
-  for (int c = 1; c <= 3; c++) {
+  first[CL_COUNT] = np;
+  for (int c = 1; c < CL_COUNT; c++) {
     uint64_t lo = 0, hi = np;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) / 2;
@@ -996,20 +1031,19 @@ __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, ui
     }
     first[c] = lo;
   }
-  for (int c = 0; c < 3; c++) counts[c] = (uint32_t)(first[c + 1] - first[c]);
-  counts[3] = (uint32_t)(np - first[3]);
+  for (int c = 0; c < CL_COUNT; c++) counts[c] = (uint32_t)(first[c + 1] - first[c]);
 }
 
 __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
   ctl->count = 0; ctl->need = 0;
   OverflowCtl* promo = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);   // the promotion queue's
   promo->count = 0; promo->need = 0;
-  for (int k = 0; k < 16; k++) counts[k] = 0;   // class sizes [0, 3) and the edge queues' counts [8, 10)
+  for (int k = 0; k < 16; k++) counts[k] = 0;   // class sizes [0, 5) and the edge queues' counts and firsts [8, 14)
 }
 }  // namespace
 
 PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap,
-                            bool mixed) {
+                            bool mixed, uint32_t tcap) {
   hipStream_t st = c.stream;
   char* misc = c.s_misc.as<char>(256);   // [0, 32) belongs to the scan / sort primitives
   PairClasses pc;
@@ -1023,9 +1057,9 @@ PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const 
   uint64_t* keys2 = c.s_keys2.as<uint64_t>(np + 16);
   uint32_t* vals = c.s_vals.as<uint32_t>(np + 16);
   uint32_t* vals2 = c.s_vals2.as<uint32_t>(np + 16);
-  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, mixed ? 1 : 0, keys, vals);
+  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, mixed ? 1 : 0, tcap, keys, vals);
   GSJ_LAUNCHED();
-  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 8);   // stable: candidate order within a key
+  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 9);   // stable: candidate order within a key
   hipLaunchKernelGGL(k_class_counts, dim3(1), dim3(1), 0, st, keys, np, pc.counts);
   GSJ_LAUNCHED();
   pc.list = vals;

@@ -238,6 +238,7 @@ int gsj_sched_create(gsj_ctx* ctx, int nlanes, uint64_t scratch_budget_bytes, gs
     for (int i = 0; i < nlanes; i++) {
       s->lanes.push_back(std::make_unique<gsj::Context>());
       gsj::context_init(*s->lanes.back(), ctx->c.device);
+      s->lanes.back()->tune = ctx->c.tune;   // lanes launch with their parent context's shapes
     }
     for (int i = 0; i < nlanes; i++) s->workers.emplace_back([p = s.get(), i] { p->worker(i); });
     *out = s.release();

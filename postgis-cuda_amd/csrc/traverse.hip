@@ -232,7 +232,8 @@ void traverse_count(Context& c, const float4* qbb, const uint32_t* qidx, uint32_
     GSJ_LAUNCHED();
     return;
   }
-  hipLaunchKernelGGL(k_traverse<false>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0, counts,
+  const uint32_t tb = c.shape(nq).trav_block;   // one thread per probe; the bucket is chosen by the probe count
+  hipLaunchKernelGGL(k_traverse<false>, dim3(div_up(nq, tb)), dim3(tb), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0, counts,
                      nullptr, nullptr, nullptr, box_by_pos ? 1 : 0, slots, nover);
   GSJ_LAUNCHED();
 }
@@ -244,7 +245,8 @@ void traverse_write(Context& c, const float4* qbb, const uint32_t* qidx, uint32_
   hipLaunchKernelGGL(k_slot_gather, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qidx, qbase, nq, counts, offs, slots, out_a, out_b);
   GSJ_LAUNCHED();
   if (any_over) {
-    hipLaunchKernelGGL(k_traverse<true>, dim3(div_up(nq, 256)), dim3(256), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
+    const uint32_t tb = c.shape(nq).trav_block;
+    hipLaunchKernelGGL(k_traverse<true>, dim3(div_up(nq, tb)), dim3(tb), 0, c.stream, qbb, qidx, qbase, nq, t.nodes, t.nnodes, upper ? 1 : 0,
                        const_cast<uint32_t*>(counts), offs, out_a, out_b, box_by_pos ? 1 : 0, nullptr, nullptr);
     GSJ_LAUNCHED();
   }

@@ -42,7 +42,8 @@ class JoinStats(C.Structure):
                 ("k_small_ms", C.c_double), ("k_large_ms", C.c_double), ("k_over_ms", C.c_double),
                 ("n_small", C.c_uint64), ("n_large", C.c_uint64), ("n_over", C.c_uint64), ("launches", C.c_uint64),
                 ("k_medium_ms", C.c_double), ("n_medium", C.c_uint64), ("k_screen_ms", C.c_double), ("n_small_edge", C.c_uint64),
-                ("n_medium_edge", C.c_uint64), ("k_mixed_ms", C.c_double), ("n_mixed", C.c_uint64)]
+                ("n_medium_edge", C.c_uint64), ("k_mixed_ms", C.c_double), ("n_mixed", C.c_uint64),
+                ("k_tiny_ms", C.c_double), ("n_tiny", C.c_uint64), ("n_tiny_edge", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -76,7 +77,7 @@ _lib = None
 
 # every symbol include/gsj.h declares (tests check the library exports all of them)
 EXPORTS = [
-    "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim",
+    "gsj_last_error", "gsj_version", "gsj_ctx_create", "gsj_ctx_destroy", "gsj_ctx_status_json", "gsj_ctx_trim", "gsj_ctx_set_tuning", "gsj_ctx_tuning",
     "gsj_layer_create_polys", "gsj_layer_create_points", "gsj_layer_free", "gsj_bvh_build", "gsj_layer_info_get", "gsj_layer_export",
     "gsj_join", "gsj_join_noalloc", "gsj_points_in_polygons", "gsj_cluster", "gsj_cluster_rows", "gsj_dwithin", "gsj_points_dwithin", "gsj_intersects_buffered",
     "gsj_points_intersects_buffered",
@@ -100,7 +101,7 @@ def load(path: str | None = None):
     sig = {
         "gsj_last_error": (C.c_char_p, []), "gsj_version": (C.c_char_p, []),
         "gsj_ctx_create": (i32, [i32, C.POINTER(vp)]), "gsj_ctx_destroy": (None, [vp]), "gsj_ctx_status_json": (C.c_char_p, [vp]),
-        "gsj_ctx_trim": (i32, [vp]),
+        "gsj_ctx_trim": (i32, [vp]), "gsj_ctx_set_tuning": (i32, [vp, C.c_char_p]), "gsj_ctx_tuning": (C.c_char_p, [vp]),
         "gsj_layer_create_polys": (i32, [vp, u64p, u64p, dp, u64, u64, u64, u32p, u8p, C.POINTER(vp)]),
         "gsj_layer_create_points": (i32, [vp, dp, u64, C.POINTER(vp)]), "gsj_layer_free": (None, [vp]), "gsj_bvh_build": (i32, [vp]),
         "gsj_layer_info_get": (i32, [vp, C.POINTER(LayerInfo)]), "gsj_layer_export": (i32, [vp, dp, dp, fp, dp, dp]),
@@ -418,6 +419,15 @@ class Context:
     def trim(self):
         """free the context's grow-only scratch (it grows again on the next call)"""
         _check(load().gsj_ctx_trim(self.h))
+
+    @property
+    def tuning(self) -> str:
+        """the launch-shape table in effect (text form of gsj_ctx_set_tuning)"""
+        return load().gsj_ctx_tuning(self.h).decode()
+
+    def set_tuning(self, text: str):
+        """replace the launch-shape table; the rows of every call are the same under any table"""
+        _check(load().gsj_ctx_set_tuning(self.h, text.encode()))
 
     def polys(self, hs: HostSet, build_bvh=True) -> Layer:
         hs = hs.normalized()

@@ -27,5 +27,8 @@ def ctx():
     import gsj
 
     c = gsj.Context(0)
+    # the built-in launch shapes, whatever table ships beside the library: small test joins then use every work class
+    # (rows are the same under every table; tests/test_gpu_tuning.py checks the shipped one)
+    c.set_tuning("bucket np>=0\n")
     yield c
     c.close()

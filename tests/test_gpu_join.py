@@ -73,11 +73,11 @@ def test_mesh_self_relate(ctx, upper):
 
 @pytest.mark.parametrize("want_area,want_relate", [(True, True), (True, False), (False, True)])
 def test_every_work_class_matches_oracle(ctx, want_area, want_relate):
-    # median 24 stored vertices: pairs in the 16-lane (<= 32), 32-lane (<= 64) and wave (larger) classes
+    # median 24 stored vertices: pairs in the 8-lane (<= 16), 16-lane (<= 32), 32-lane (<= 64) and wave (larger) classes
     M = _mesh(70, seed=8, median=24.0)
     L = ctx.polys(M)
     got = ctx.join(L, L, want_area=want_area, want_relate=want_relate, upper=True)
-    assert min(got.stats["n_small"], got.stats["n_medium"], got.stats["n_large"]) > 1000
+    assert min(got.stats["n_small"], got.stats["n_medium"], got.stats["n_large"]) > 1000 and got.stats["n_tiny"] > 100, got.stats
     OM = O.Layer.polys(M)
     exp = O.join(OM, OM, want_area=want_area, want_relate=want_relate, upper=True)
     assert got.stats["candidates"] == exp["candidates"]
