@@ -206,20 +206,33 @@ def config3_leg(gsj, ctx, args, L, S):
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "refine_relate_2.4M.md")
 
 
+def summary_column(kernel, ncells, col):
+    """Column `col` of the committed counter summary's table of `ncells` cells for `kernel`; a name without template
+    arguments sums its instantiations (k_screen_small: the tiny, small and medium screens, timed together as the
+    engine's k_screen_ms). None when the summary lacks the kernel."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    total, found = 0, False
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) != ncells:
+            continue
+        name = cells[0].strip("`")
+        if name == kernel or ("<" not in kernel and name.startswith(kernel + "<")):
+            try:
+                total += int(float(cells[col]))
+                found = True
+            except ValueError:
+                return None
+    return total if found else None
+
+
 def hbm_traffic(kernel):
     """HBM bytes per dispatch of `kernel` on this workload, from the committed rocprofv3 counter summary (FETCH_SIZE and
     WRITE_SIZE passes, each scaled by the copy calibration recorded there); None when the summary lacks the kernel.
-    Rows of the summary's traffic table read: | `kernel` | fetch KiB | write KiB | HBM bytes |."""
-    if not os.path.exists(TRAFFIC_SUMMARY):
-        return None
-    for line in open(TRAFFIC_SUMMARY):
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 4 and cells[0] == f"`{kernel}`":   # the traffic table (the trace table has 5 columns)
-            try:
-                return int(float(cells[3]))
-            except ValueError:
-                return None
-    return None
+    Rows of the summary's traffic table read: | `kernel` | fetch KiB | write KiB | HBM bytes | (the trace table has 5
+    columns)."""
+    return summary_column(kernel, 4, 3)
 
 
 def other_rooflines(n, st, t_bvh):
@@ -240,16 +253,7 @@ def other_rooflines(n, st, t_bvh):
 def issued_f64(kernel):
     """Issued FP64 lane-FLOPs per dispatch of `kernel` (64 x (ADD + MUL + 2 FMA) wave instructions, whatever the exec
     mask) from the committed counter summary's FP64 instruction table (7 cells a row); None when it lacks the kernel."""
-    if not os.path.exists(TRAFFIC_SUMMARY):
-        return None
-    for line in open(TRAFFIC_SUMMARY):
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0] == f"`{kernel}`":
-            try:
-                return int(float(cells[6]))
-            except ValueError:
-                return None
-    return None
+    return summary_column(kernel, 7, 6)
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
