@@ -314,6 +314,35 @@ int gsj_subdivide_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo,
 int gsj_dedup_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, gsj_hostset** distinct,
                      uint64_t* moff, uint32_t* midx, uint64_t* n_distinct);
 
+/* The layer loader's parse of a binary COPY stream (daemon/loader.h:125-170: COPY (SELECT id::text, geom ...) TO
+ * STDOUT (FORMAT binary), two fields per row, geometry as EWKB), with the EWKB work on ctx's device: the host frames
+ * the rows, a thread per row walks each value's headers and a thread per ring decodes its points and orients it.
+ * Rows: a polygonal value (POLYGON, MULTIPOLYGON, or a GEOMETRYCOLLECTION of them; either byte order, SRID, Z, M,
+ * ISO type codes) becomes one geometry holding the rings of all its polygons with 4 or more points, each polygon's
+ * first ring counter-clockwise and the others clockwise by the sign of a compensated area (daemon/loader.h:56-70,
+ * 100-102); a POINT a point. The layer's kind is that of its first accepted row, or `kind` when an earlier piece of
+ * the same stream decided it (1 polygons, 2 points); rows of the other kind, unsupported types, points inside
+ * collections and malformed values are skipped_type, values without rings or POINT EMPTY skipped_empty, NULL ids or
+ * geometries skipped_null (the reference's counters). Collections nest at most 32 deep (deeper: skipped_type).
+ * header: buf starts with the PGCOPY signature (GSJ_E_ARG otherwise); rows are read up to the trailer or the first
+ * incomplete row (stats->consumed bytes: a caller reading a stream in pieces keeps the rest for the next call, with
+ * header 0 and the kind returned). Outputs: *polys (gsj_hostset_free) for a polygon layer, else *pts[2 *npts] for a
+ * point layer; ids[id_off[k] .. id_off[k+1]) per accepted row in stream order; *pts, *ids, *id_off freed with gsj_free. */
+typedef struct {
+  uint64_t rows, skipped_null, skipped_type, skipped_empty, polys, points;
+  uint64_t consumed;   /* bytes of buf read */
+  int done;            /* the trailer was read */
+  int kind;            /* the layer's kind after this piece: 0 none yet, 1 polygons, 2 points */
+} gsj_copy_stats;
+int gsj_copy_parse_device(gsj_ctx* ctx, const uint8_t* buf, uint64_t len, int header, int kind, gsj_hostset** polys, double** pts,
+                          uint64_t* npts, char** ids, uint64_t** id_off, gsj_copy_stats* stats);
+/* A polygon set as such a stream (little-endian EWKB; the SRID flag with srid when srid > 0): a POLYGON per geometry,
+ * or a MULTIPOLYGON whose polygons start at each counter-clockwise ring; ids/id_off as above, or NULL for the row
+ * numbers "0", "1", ...; *out freed with gsj_free. No counterpart in the reference (its input comes from Postgres);
+ * it feeds the parse from the synthetic layers. */
+int gsj_copy_write(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, const char* ids, const uint64_t* id_off, int srid,
+                   uint8_t** out, uint64_t* len);
+
 /* Deterministic synthetic layers (counter-based RNG, identical on every run):
  *   kind "parcels_grid": config-1 parcels (one 4..12-vertex star polygon per cell of an nx x ny grid)
  *   kind "regions_grid": config-1 regions (kx x ky tiles, `edge_verts` vertices per side, shared jittered borders)

@@ -57,6 +57,9 @@ def load(flops=False):
         "orc_points_dwithin": (None, [vp, vp, d, i32, vp]),
         "orc_er_pair": (i32, [vp, vp, u32, u32, d, d]), "orc_erode": (None, [vp, vp, d, d, i32, i32, u8p, vp]),
         "orc_points_erode": (None, [vp, vp, d, vp]),
+        # binary COPY / EWKB parse (ewkb.cpp)
+        "orc_copy_parse": (vp, [u8p, u64, i32, i32]), "orc_copy_error": (C.c_char_p, [vp]), "orc_copy_sizes": (None, [vp, u64p]),
+        "orc_copy_arrays": (None, [vp] + [C.c_void_p] * 7), "orc_copy_free": (None, [vp]), "orc_ring_twice_area": (d, [dp, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -401,3 +404,44 @@ def points_erode(pts: Layer, R: Layer, r: float):
     del out["dist"]
     return out
 
+
+
+# ---------------------------------------------------------------------------------------------- binary COPY / EWKB parse
+def copy_parse(buf: bytes, header=True, kind=0):
+    """The layer loader's parse of a binary COPY stream of (id, geom) rows (oracle/ewkb.cpp): dict of the counters
+    (rows, nulls, bad, empty, polys, points), consumed / done, the polygon arrays gro, rvo, xy (n, 2), the points pxy
+    (n, 2), ids (str per accepted row) and row (stream row of each accepted row). kind: the layer kind an earlier piece
+    of the stream decided (1 polygons, 2 points)."""
+    lib = load()
+    raw = C.create_string_buffer(bytes(buf), max(len(buf), 1))
+    h = lib.orc_copy_parse(C.cast(raw, C.POINTER(C.c_uint8)), len(buf), int(header), int(kind))
+    try:
+        err = lib.orc_copy_error(h).decode()
+        if err:
+            raise ValueError(err)
+        sz = np.zeros(12, np.uint64)
+        lib.orc_copy_sizes(h, _p(sz, C.c_uint64))
+        rows, nulls, bad, empty, polys, points, consumed, done, n, nr, nv, npts = (int(v) for v in sz)
+        gro, rvo, xy, pxy, ido, ids, row = (C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint64)(), C.POINTER(C.c_double)(), C.POINTER(C.c_double)(),
+                                            C.POINTER(C.c_uint64)(), C.c_char_p(), C.POINTER(C.c_uint64)())
+        lib.orc_copy_arrays(h, C.byref(gro), C.byref(rvo), C.byref(xy), C.byref(pxy), C.byref(ido), C.byref(ids), C.byref(row))
+        nacc = polys + points
+
+        def arr(p, cnt, dt):
+            return np.ctypeslib.as_array(p, (cnt,)).copy() if cnt else np.zeros(0, dt)
+
+        id_off = arr(ido, nacc + 1, np.uint64)
+        blob = C.string_at(ids, int(id_off[-1])) if nacc else b""
+        return {"rows": rows, "nulls": nulls, "bad": bad, "empty": empty, "polys": polys, "points": points, "consumed": consumed,
+                "done": bool(done), "gro": arr(gro, n + 1, np.uint64), "rvo": arr(rvo, nr + 1, np.uint64),
+                "xy": arr(xy, 2 * nv, np.float64).reshape(-1, 2), "pxy": arr(pxy, 2 * npts, np.float64).reshape(-1, 2),
+                "ids": [blob[int(id_off[k]):int(id_off[k + 1])].decode() for k in range(nacc)], "row": arr(row, nacc, np.uint64),
+                "stats": {"rows": rows, "nulls": nulls, "bad": bad, "empty": empty, "polys": polys, "points": points}}
+    finally:
+        lib.orc_copy_free(h)
+
+
+def ring_twice_area(xy):
+    """twice the signed area of a closed ring (n, 2) as the loader orients rings (oracle/ewkb.cpp twice_area)"""
+    a = np.ascontiguousarray(xy, np.float64).reshape(-1)
+    return load().orc_ring_twice_area(_p(a, C.c_double), len(a) // 2)

@@ -460,6 +460,65 @@ int gsj_dedup_device(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rvo, con
   });
 }
 
+int gsj_copy_parse_device(gsj_ctx* ctx, const uint8_t* buf, uint64_t len, int header, int kind, gsj_hostset** polys, double** pts,
+                          uint64_t* npts, char** ids, uint64_t** id_off, gsj_copy_stats* stats) {
+  return guarded([&] {
+    GSJ_REQUIRE(ctx && (buf || len == 0) && polys && pts && npts && ids && id_off && stats, GSJ_E_ARG, "null argument");
+    *polys = nullptr;
+    *pts = nullptr;
+    *npts = 0;
+    *ids = nullptr;
+    *id_off = nullptr;
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    set_device(ctx->c);
+    gsj::CopyLayer L = gsj::copy_parse_device(ctx->c, buf, len, header != 0, kind);
+    const uint64_t nacc = L.points_layer ? L.npoints : L.npolys;
+    GSJ_REQUIRE(L.id_off.size() == nacc + 1, GSJ_E_INTERNAL, "copy parse: id offsets do not match the accepted rows");
+    auto take = [](size_t bytes) {
+      void* p = std::malloc(std::max<size_t>(bytes, 1));
+      if (!p) throw std::bad_alloc();
+      return p;
+    };
+    std::unique_ptr<void, decltype(&std::free)> b_ids(take(L.ids.size()), &std::free), b_off(take((nacc + 1) * 8), &std::free),
+        b_pts(nullptr, &std::free);
+    std::memcpy(b_ids.get(), L.ids.data(), L.ids.size());
+    std::memcpy(b_off.get(), L.id_off.data(), (nacc + 1) * 8);
+    std::unique_ptr<gsj_hostset> set;
+    if (L.points_layer) {
+      b_pts.reset(take(L.pxy.size() * 8));
+      std::memcpy(b_pts.get(), L.pxy.data(), L.pxy.size() * 8);
+    } else {
+      set.reset(new gsj_hostset());
+      set->s = std::move(L.polys);
+    }
+    *stats = gsj_copy_stats{L.rows, L.nulls, L.bad_type, L.empty, L.npolys, L.npoints, L.consumed, L.done ? 1 : 0,
+                            kind ? kind : (L.npolys ? 1 : (L.npoints ? 2 : 0))};
+    *ids = static_cast<char*>(b_ids.release());
+    *id_off = static_cast<uint64_t*>(b_off.release());
+    if (L.points_layer) {
+      *pts = static_cast<double*>(b_pts.release());
+      *npts = L.npoints;
+    } else {
+      *polys = set.release();
+    }
+  });
+}
+
+int gsj_copy_write(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, const char* ids, const uint64_t* id_off, int srid,
+                   uint8_t** out, uint64_t* len) {
+  return guarded([&] {
+    GSJ_REQUIRE(gro && rvo && out && len && (!ids || id_off), GSJ_E_ARG, "null argument");
+    gsj::check_offsets(gro, rvo, n);
+    GSJ_REQUIRE(xy || rvo[gro[n]] == 0, GSJ_E_ARG, "null xy");
+    const std::string s = gsj::copy_write(gro, rvo, xy, n, ids, id_off, srid);
+    void* p = std::malloc(s.size());
+    if (!p) throw std::bad_alloc();
+    std::memcpy(p, s.data(), s.size());
+    *out = static_cast<uint8_t*>(p);
+    *len = s.size();
+  });
+}
+
 int gsj_generate(const char* kind, const char* spec, gsj_hostset** out) {
   return guarded([&] {
     GSJ_REQUIRE(kind && out, GSJ_E_ARG, "null argument");

@@ -638,4 +638,82 @@ std::vector<double> generate_points(const char* spec) {
   return xy;
 }
 
+// ------------------------------------------------------------------------------------------- binary COPY writer
+namespace {
+void put_be16(std::string& s, int16_t v) {
+  const uint16_t u = (uint16_t)v;
+  s.push_back((char)(u >> 8));
+  s.push_back((char)(u & 0xFF));
+}
+void put_be32(std::string& s, int32_t v) {
+  const uint32_t u = (uint32_t)v;
+  for (int sh = 24; sh >= 0; sh -= 8) s.push_back((char)((u >> sh) & 0xFF));
+}
+void put_le32(std::string& s, uint32_t v) { s.append(reinterpret_cast<const char*>(&v), 4); }
+void put_f64(std::string& s, double v) { s.append(reinterpret_cast<const char*>(&v), 8); }
+
+// sign of a ring's area (shoelace relative to its first vertex): true for counter-clockwise
+bool ring_ccw(const double* xy, uint64_t n) {
+  double s = 0.0;
+  for (uint64_t i = 1; i + 1 < n; i++) s += (xy[2 * i] - xy[0]) * (xy[2 * i + 3] - xy[1]) - (xy[2 * i + 1] - xy[1]) * (xy[2 * i + 2] - xy[0]);
+  return s > 0;
+}
+
+// one geometry as little-endian EWKB: a POLYGON when its first ring is its only counter-clockwise one, else a
+// MULTIPOLYGON whose polygons start at every counter-clockwise ring (clockwise rings are holes of the polygon before)
+void put_geometry(std::string& s, const uint64_t* rvo, const double* xy, uint64_t r0, uint64_t r1, int srid) {
+  std::vector<uint64_t> starts;
+  for (uint64_t r = r0; r < r1; r++)
+    if (r == r0 || ring_ccw(xy + 2 * rvo[r], rvo[r + 1] - rvo[r])) starts.push_back(r);
+  starts.push_back(r1);
+  auto head = [&](uint32_t type, bool with_srid) {
+    s.push_back(1);
+    put_le32(s, type | (with_srid ? 0x20000000u : 0u));
+    if (with_srid) put_le32(s, (uint32_t)srid);
+  };
+  auto polygon = [&](uint64_t a, uint64_t b, bool with_srid) {
+    head(3, with_srid);
+    put_le32(s, (uint32_t)(b - a));
+    for (uint64_t r = a; r < b; r++) {
+      put_le32(s, (uint32_t)(rvo[r + 1] - rvo[r]));
+      for (uint64_t v = rvo[r]; v < rvo[r + 1]; v++) {
+        put_f64(s, xy[2 * v]);
+        put_f64(s, xy[2 * v + 1]);
+      }
+    }
+  };
+  const bool with_srid = srid > 0;
+  if (starts.size() <= 2) {
+    polygon(r0, r1, with_srid);
+    return;
+  }
+  head(6, with_srid);
+  put_le32(s, (uint32_t)(starts.size() - 1));
+  for (size_t k = 0; k + 1 < starts.size(); k++) polygon(starts[k], starts[k + 1], false);
+}
+}  // namespace
+
+std::string copy_write(const uint64_t* gro, const uint64_t* rvo, const double* xy, uint64_t n, const char* ids, const uint64_t* id_off, int srid) {
+  std::string s("PGCOPY\n\xff\r\n\0", 11);
+  const uint64_t nr = n ? gro[n] : 0, nv = n ? rvo[nr] : 0;
+  s.reserve(21 + n * 48 + nr * 8 + nv * 16);   // framing and headers per row, a count per ring, 16 bytes per point
+  put_be32(s, 0);   // flags
+  put_be32(s, 0);   // header extension length
+  for (uint64_t g = 0; g < n; g++) {
+    put_be16(s, 2);
+    std::string id = ids ? std::string(ids + id_off[g], ids + id_off[g + 1]) : std::to_string(g);
+    if (id.size() >= (1ull << 31)) throw std::invalid_argument("copy_write: id of 2 GiB or more");
+    put_be32(s, (int32_t)id.size());
+    s += id;
+    const size_t at = s.size();
+    put_be32(s, 0);   // geometry length, patched below
+    put_geometry(s, rvo, xy, gro[g], gro[g + 1], srid);
+    const uint64_t len = s.size() - at - 4;
+    if (len >= (1ull << 31)) throw std::invalid_argument("copy_write: geometry value of 2 GiB or more");
+    for (int k = 0; k < 4; k++) s[at + k] = (char)((len >> (24 - 8 * k)) & 0xFF);
+  }
+  put_be16(s, -1);
+  return s;
+}
+
 }  // namespace gsj

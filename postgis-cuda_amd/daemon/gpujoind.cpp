@@ -120,6 +120,7 @@ struct Layer {
   std::vector<uint64_t> moff;   // distinct geometry -> members [moff[g], moff[g+1]) of midx
   std::vector<uint32_t> midx;
   uint64_t n_distinct = 0, gpu_geoms = 0, verts = 0, gpu_bytes = 0;
+  uint64_t copy_rows = 0, skipped_null = 0, skipped_type = 0, skipped_empty = 0;   // copy: sources (the loader's counters)
   double prep_s = 0, upload_s = 0;
   gsj_layer* gl = nullptr;
   template <class F> void members(uint32_t g, F&& f) const {
@@ -131,7 +132,9 @@ struct Layer {
   std::string json() const {
     return Json().k("layer").v(name).k("kind").v(kind).k("source").v(source).k("rows").u(rows).k("distinct").u(n_distinct)
         .k("gpu_geoms").u(gpu_geoms).k("vertices").u(verts).k("subdivide").u(subdiv).k("dedup").b(deduped)
-        .k("gpu_mb").n(gpu_bytes / 1048576.0).k("prepare_s").n(prep_s).k("upload_s").n(upload_s).done();
+        .k("gpu_mb").n(gpu_bytes / 1048576.0).k("prepare_s").n(prep_s).k("upload_s").n(upload_s)
+        .k("stream_rows").u(copy_rows).k("skipped_null").u(skipped_null).k("skipped_type").u(skipped_type).k("skipped_empty").u(skipped_empty)
+        .done();
   }
 };
 
@@ -170,6 +173,70 @@ struct HostsetGuard {
   ~HostsetGuard() { if (h) gsj_hostset_free(h); }
 };
 
+// a layer read from a binary COPY file (the stream the reference's loader reads from Postgres, daemon/loader.h:176-279)
+struct CopyRows {
+  std::vector<uint64_t> gro{0}, rvo{0};
+  std::vector<double> xy, pts;
+  uint64_t rows = 0, nulls = 0, bad_type = 0, empty = 0;
+  int kind = 0;   // 1 polygons, 2 points
+};
+
+// the file in pieces of 256 MiB, each parsed on the device; the rest of a piece (an incomplete row) and the layer
+// kind carry over to the next, so the result is that of one parse of the whole stream
+void read_copy_file(const std::string& path, CopyRows& out, IdStore& ids) {
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path.c_str(), "rb"), &std::fclose);
+  if (!f) throw std::runtime_error("cannot open " + path + ": " + strerror(errno));
+  constexpr size_t PIECE = 256u << 20;
+  std::vector<uint8_t> buf;
+  size_t have = 0;
+  bool header = true, eof = false;
+  while (!eof) {
+    buf.resize(have + PIECE);
+    const size_t got = std::fread(buf.data() + have, 1, PIECE, f.get());
+    have += got;
+    eof = got < PIECE;
+    HostsetGuard ph;
+    double* pp = nullptr;
+    char* pid = nullptr;
+    uint64_t* poff = nullptr;
+    uint64_t np = 0;
+    gsj_copy_stats st{};
+    check(gsj_copy_parse_device(g_ctx, buf.data(), have, header ? 1 : 0, out.kind, &ph.h, &pp, &np, &pid, &poff, &st), "parsing the COPY stream");
+    std::unique_ptr<double, void (*)(void*)> g_pts(pp, &gsj_free);
+    std::unique_ptr<char, void (*)(void*)> g_ids(pid, &gsj_free);
+    std::unique_ptr<uint64_t, void (*)(void*)> g_off(poff, &gsj_free);
+    if (header && st.consumed == 0) {
+      if (eof) throw std::runtime_error("COPY stream " + path + " ends inside its header");
+      continue;   // the header is not complete yet
+    }
+    header = false;
+    out.kind = st.kind;
+    out.rows += st.rows;
+    out.nulls += st.skipped_null;
+    out.bad_type += st.skipped_type;
+    out.empty += st.skipped_empty;
+    const uint64_t nacc = st.polys + st.points;
+    for (uint64_t k = 0; k < nacc; k++) ids.add(pid + poff[k], poff[k + 1] - poff[k]);
+    if (pp) out.pts.insert(out.pts.end(), pp, pp + 2 * np);
+    if (ph.h && st.polys) {
+      const uint64_t *gro, *rvo;
+      const double* xy;
+      const uint32_t* parent;
+      const uint8_t* syn;
+      uint64_t n, nr, nv;
+      check(gsj_hostset_arrays(ph.h, &gro, &rvo, &xy, &parent, &syn), "layer arrays");
+      check(gsj_hostset_sizes(ph.h, &n, &nr, &nv), "layer sizes");
+      const uint64_t r0 = out.gro.back(), v0 = out.rvo.back();
+      for (uint64_t g = 1; g <= n; g++) out.gro.push_back(r0 + gro[g]);
+      for (uint64_t r = 1; r <= nr; r++) out.rvo.push_back(v0 + rvo[r]);
+      out.xy.insert(out.xy.end(), xy, xy + 2 * nv);
+    }
+    std::memmove(buf.data(), buf.data() + st.consumed, have - st.consumed);
+    have -= st.consumed;
+    if (st.done) break;
+  }
+}
+
 // LOAD / LOAD_QUERY: source -> host set -> dedup -> subdivide -> device layer + BVH (reference build_layer,
 // daemon/gpujoind.cu:100-149)
 std::string build_layer(const std::string& name, const std::string& source, int subdiv, bool dedup) {
@@ -179,7 +246,19 @@ std::string build_layer(const std::string& name, const std::string& source, int 
   HostsetGuard hs;
   std::vector<double> pts;
   bool is_point = false;
-  if (source.rfind("gset:", 0) == 0) {
+  CopyRows cr;
+  const bool from_copy = source.rfind("copy:", 0) == 0;
+  if (from_copy) {
+    read_copy_file(resolve(source.substr(5)), cr, L->ids);
+    L->skipped_null = cr.nulls;
+    L->skipped_type = cr.bad_type;
+    L->skipped_empty = cr.empty;
+    L->copy_rows = cr.rows;
+    if (cr.kind == 2) {
+      pts.swap(cr.pts);
+      is_point = true;
+    }
+  } else if (source.rfind("gset:", 0) == 0) {
     const std::string path = resolve(source.substr(5));
     check(gsj_gset_read(path.c_str(), &hs.h), "reading layer file");
     uint64_t n, nr, nv;
@@ -210,7 +289,7 @@ std::string build_layer(const std::string& name, const std::string& source, int 
     is_point = true;
     read_ids("", n, L->ids);
   } else {
-    throw std::runtime_error("unknown layer source '" + source + "' (gset:PATH, pset:PATH, gen:KIND:SPEC, genpts:SPEC)");
+    throw std::runtime_error("unknown layer source '" + source + "' (gset:PATH, pset:PATH, copy:PATH, gen:KIND:SPEC, genpts:SPEC)");
   }
   L->rows = L->ids.size();
   if (is_point) {
@@ -227,8 +306,19 @@ std::string build_layer(const std::string& name, const std::string& source, int 
     const uint32_t* parent;
     const uint8_t* syn;
     uint64_t n, nr, nv;
-    check(gsj_hostset_arrays(hs.h, &gro, &rvo, &xy, &parent, &syn), "layer arrays");
-    check(gsj_hostset_sizes(hs.h, &n, &nr, &nv), "layer sizes");
+    if (from_copy) {
+      gro = cr.gro.data();
+      rvo = cr.rvo.data();
+      xy = cr.xy.data();
+      parent = nullptr;
+      syn = nullptr;
+      n = cr.gro.size() - 1;
+      nr = cr.rvo.size() - 1;
+      nv = cr.xy.size() / 2;
+    } else {
+      check(gsj_hostset_arrays(hs.h, &gro, &rvo, &xy, &parent, &syn), "layer arrays");
+      check(gsj_hostset_sizes(hs.h, &n, &nr, &nv), "layer sizes");
+    }
     HostsetGuard distinct, pieces;
     if (dedup) {
       L->moff.assign(n + 1, 0);

@@ -84,6 +84,7 @@ EXPORTS = [
     "gsj_sched_create", "gsj_sched_destroy", "gsj_sched_set_batch", "gsj_submit", "gsj_wait", "gsj_poll", "gsj_de9im_str", "gsj_de9im_transpose", "gsj_de9im_match",
     "gsj_hostset_free", "gsj_hostset_sizes", "gsj_hostset_arrays", "gsj_gset_read", "gsj_gset_write", "gsj_pset_read", "gsj_pset_write",
     "gsj_free", "gsj_subdivide", "gsj_dedup", "gsj_dedup_device", "gsj_subdivide_device", "gsj_generate", "gsj_generate_points",
+    "gsj_copy_parse_device", "gsj_copy_write",
 ]
 
 
@@ -127,6 +128,9 @@ def load(path: str | None = None):
         "gsj_dedup_device": (i32, [vp, u64p, u64p, dp, u64, C.POINTER(vp), u64p, u32p, u64p]),
         "gsj_subdivide_device": (i32, [vp, u64p, u64p, dp, u64, i32, C.POINTER(vp)]),
         "gsj_generate": (i32, [C.c_char_p, C.c_char_p, C.POINTER(vp)]), "gsj_generate_points": (i32, [C.c_char_p, C.POINTER(dp), u64p]),
+        "gsj_copy_parse_device": (i32, [vp, C.c_void_p, u64, i32, i32, C.POINTER(vp), C.POINTER(dp), u64p, C.POINTER(C.c_void_p),
+                                        C.POINTER(u64p), C.POINTER(CopyStats)]),
+        "gsj_copy_write": (i32, [u64p, u64p, dp, u64, C.c_char_p, u64p, i32, C.POINTER(C.c_void_p), u64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -288,6 +292,69 @@ def dedup(hs: HostSet, ctx: "Context" = None):
             _ptr(midx, C.c_uint32), C.byref(un))
     _check(lib.gsj_dedup_device(ctx.h, *args) if ctx is not None else lib.gsj_dedup(*args))
     return _take_hostset(h), moff[: un.value + 1].copy(), midx[: hs.n].copy()
+
+
+class CopyStats(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("skipped_null", C.c_uint64), ("skipped_type", C.c_uint64), ("skipped_empty", C.c_uint64),
+                ("polys", C.c_uint64), ("points", C.c_uint64), ("consumed", C.c_uint64), ("done", C.c_int), ("kind", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+@dataclass
+class CopyLayer:
+    """a parsed binary COPY stream: polys (polygon layer) or pts (n, 2) (point layer), ids per accepted row"""
+    polys: HostSet | None
+    pts: np.ndarray | None
+    ids: list
+    stats: dict
+
+
+def copy_parse(ctx: "Context", buf: bytes, header=True, kind=0) -> CopyLayer:
+    """gsj_copy_parse_device: the rows of a binary COPY stream of (id text, geom) parsed on ctx's device; stats carries
+    the loader's counters, consumed / done, and the layer kind for the next piece of the same stream"""
+    lib = load()
+    raw = C.create_string_buffer(bytes(buf), max(len(buf), 1))
+    h, p, n = C.c_void_p(), C.POINTER(C.c_double)(), C.c_uint64()
+    ids, off, st = C.c_void_p(), C.POINTER(C.c_uint64)(), CopyStats()
+    _check(lib.gsj_copy_parse_device(ctx.h, C.cast(raw, C.c_void_p), len(buf), int(header), int(kind), C.byref(h), C.byref(p), C.byref(n),
+                                     C.byref(ids), C.byref(off), C.byref(st)))
+    try:
+        nacc = st.polys + st.points
+        o = np.ctypeslib.as_array(off, (nacc + 1,)).copy()
+        blob = C.string_at(ids.value, int(o[-1])) if nacc and o[-1] else b""
+        idl = [blob[int(o[k]):int(o[k + 1])].decode() for k in range(nacc)]
+        pts = np.ctypeslib.as_array(p, (2 * n.value,)).copy().reshape(-1, 2) if n.value else (np.zeros((0, 2)) if p else None)
+    finally:
+        lib.gsj_free(ids)
+        lib.gsj_free(C.cast(off, C.c_void_p))
+        if p:
+            lib.gsj_free(C.cast(p, C.c_void_p))
+    polys = _take_hostset(h) if h.value else None
+    return CopyLayer(polys, pts, idl, st.as_dict())
+
+
+def copy_write(hs: HostSet, ids=None, srid=0) -> bytes:
+    """gsj_copy_write: the set as a binary COPY stream of (id text, geom EWKB) rows (ids: str per geometry, default
+    the row numbers)"""
+    lib = load()
+    hs = hs.normalized()
+    out, ln = C.c_void_p(), C.c_uint64()
+    if ids is not None:
+        enc = [str(i).encode() for i in ids]
+        off = np.zeros(len(enc) + 1, np.uint64)
+        off[1:] = np.cumsum([len(e) for e in enc])
+        blob = b"".join(enc)
+        idp, offp = C.c_char_p(blob), _ptr(off, C.c_uint64)
+    else:
+        idp, offp = None, None
+    _check(lib.gsj_copy_write(_ptr(hs.gro, C.c_uint64), _ptr(hs.rvo, C.c_uint64), _ptr(hs.xy, C.c_double), hs.n, idp, offp, int(srid),
+                              C.byref(out), C.byref(ln)))
+    try:
+        return C.string_at(out.value, ln.value)
+    finally:
+        lib.gsj_free(out)
 
 
 def read_gset(path: str) -> HostSet:

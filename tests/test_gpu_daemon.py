@@ -325,6 +325,32 @@ def test_concurrent_windowed_requests(daemon):
     assert any(len(r) for r in alone)
 
 
+def test_load_from_a_binary_copy_stream(daemon):
+    """LOAD copy:FILE parses a binary COPY dump on the device (rows of a, its ids, plus NULL, empty and wrong-kind rows
+    the loader skips, and a few rows with the other byte order and Z): the layer answers joins with the rows of the
+    gset-loaded a"""
+    import ewkb_rows as E
+
+    c, A = daemon["client"], daemon["sets"]["a"]
+    rows = []
+    for g in range(A.n):
+        rings = [[tuple(p) for p in A.xy[int(A.rvo[r]):int(A.rvo[r + 1])]] for r in range(int(A.gro[g]), int(A.gro[g + 1]))]
+        rows.append((f"a{g}", ("polygon", rings)))
+        if g % 97 == 5:
+            rows += [(f"null{g}", None), (f"empty{g}", ("polygon", [])), (f"pt{g}", ("point", 1.0, 2.0))]
+    opts = {i: dict(le=False, z=True) for i in range(0, len(rows), 11)}
+    with open(os.path.join(daemon["dir"], "a.copy"), "wb") as f:
+        f.write(E.stream(rows, opts))
+    info = c.load("acopy", "copy:a.copy", subdivide=8)
+    extra = sum(1 for g in range(A.n) if g % 97 == 5)
+    assert info["rows"] == A.n and info["stream_rows"] == A.n + 3 * extra
+    assert (info["skipped_null"], info["skipped_empty"], info["skipped_type"]) == (extra, extra, extra)
+    assert info["distinct"] == daemon["info"]["a"]["distinct"] and info["vertices"] == daemon["info"]["a"]["vertices"]
+    for op in (lambda x: c.join(x, x, True, "distinct"), lambda x: c.relate(x, "b", "", "all"), lambda x: c.dwithin(x, x, "dwithin", 3, 0, "distinct")):
+        _same_rows(op("acopy")[0], op("a")[0])
+    assert c.call(G.OP_DROP, "acopy") is True
+
+
 def test_drop(daemon):
     c = daemon["client"]
     assert c.load("tmp", "gen:parcels_grid:nx=10;ny=10;seed=1")["rows"] == 100
