@@ -75,6 +75,45 @@ def host_threads():
     return min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
 
 
+def parse_leg(gsj, ctx, M, args):
+    """Layer load from a binary COPY stream, the loader's input: the config-2 parcels as (id text, EWKB with SRID) rows,
+    parsed on the device (gsj_copy_parse_device: host framing, upload, row walk, ring records and oriented
+    coordinates, download, ids). Roofline of the device passes: compulsory bytes = the rows read once + 16 B per vertex
+    + 8 B per ring and per geometry written + 8 B per row of the row map, over the passes' HIP-event time. Beside it
+    (cpu_baseline leg) the oracle's restatement of the loader's parse on one host thread, and parity with it."""
+    buf = gsj.copy_write(M, srid=4326)
+    d = gsj.copy_parse(ctx, buf)   # warm-up: the parse's buffers reach their size (and the ids, for parity)
+    best = float("inf")
+    for _ in range(3):   # timed: the C ABI call (framing, upload, passes, download, ids as one buffer)
+        t0 = time.perf_counter()
+        gsj.copy_parse(ctx, buf, want_ids=False)
+        best = min(best, time.perf_counter() - t0)
+    nr, nv = int(M.gro[-1]), len(M.xy)
+    kms = d.stats["scan_ms"] + d.stats["emit_ms"]
+    bytes_c = len(buf) + 16 * nv + 8 * (nr + 1) + 8 * (M.n + 1) + 8 * M.n
+    gbs = bytes_c / (kms * 1e-3) / 1e9
+    out = {"workload": f"config-2 parcels as a binary COPY stream: {M.n} rows, {nv} vertices, {len(buf) / 1e6:.1f} MB (EWKB, SRID 4326)",
+           "value": round(M.n / best, 1), "unit": "rows/s", "ms": round(best * 1e3, 2),
+           "kernel_ms": {"scan": round(d.stats["scan_ms"], 3), "emit": round(d.stats["emit_ms"], 3)},
+           "roofline": {"kernel": "k_ewkb_scan + k_ewkb_rings + k_ewkb_coords", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "compulsory_bytes": bytes_c, "launch_ms": round(kms, 3)},
+           "same_as_input": bool(np.array_equal(d.polys.xy.view(np.uint64), M.xy.view(np.uint64)) and np.array_equal(d.polys.rvo, M.rvo))}
+    log(f"parse: {M.n} rows in {best * 1e3:.1f} ms end to end, device passes {kms:.2f} ms ({gbs:.0f} GB/s compulsory)")
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        o = O.copy_parse(buf)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(M.n / t_cpu, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole stream once ({t_cpu:.2f} s)"}
+        out["parity"] = {"rows_equal": o["polys"] == d.stats["polys"] and o["ids"] == d.ids,
+                         "arrays_bit_identical": bool(np.array_equal(o["xy"].view(np.uint64), d.polys.xy.view(np.uint64))
+                                                      and np.array_equal(o["gro"], d.polys.gro) and np.array_equal(o["rvo"], d.polys.rvo))}
+        log(f"parse cpu: {t_cpu:.2f} s; parity {out['parity']}")
+    return out
+
+
 def config1_leg(gsj, ctx, args):
     """Parcels x regions intersects + intersection area: 100 x 100 parcels of 4-12 vertices under 16 regions of 2000
     vertices, seed 0. One step = one request (layers resident). The dominant kernel is the mixed class (a parcel
@@ -637,6 +676,7 @@ def main():
         if name == "subdivide_device":
             assert np.array_equal(r.xy.view(np.uint64), S.xy.view(np.uint64)) and np.array_equal(r.syn, S.syn), "device subdivision differs"
     log(f"prep (ms, host arrays in and out): {prep}")
+    parse = None if args.no_prep else parse_leg(gsj, ctx, M, args)
     t0 = time.time()
     L = ctx.polys(S, build_bvh=False)
     t_up = time.time() - t0
@@ -760,7 +800,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3), "path": "serial"},
         "breakdown_ms": {k: round(st[k], 3) for k in ("traverse_ms", "refine_ms", "compact_ms", "copy_ms", "k_screen_ms", "k_tiny_ms", "k_small_ms",
                                                       "k_medium_ms", "k_mixed_ms", "k_large_ms", "k_over_ms")},
-        "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1), "prep": prep},
+        "load_ms": {"upload": round(t_up * 1e3, 1), "bvh_build": round(t_bvh * 1e3, 1), "prep": prep}, "layer_parse": parse,
         "roofline_other": other_rooflines(S.n, st, t_bvh),
         "roofline": roofline, "cpu_baseline": cpu, "sample_parity": parity, **configs, "serving": serving, "national": national,
     }

@@ -333,6 +333,8 @@ typedef struct {
   uint64_t consumed;   /* bytes of buf read */
   int done;            /* the trailer was read */
   int kind;            /* the layer's kind after this piece: 0 none yet, 1 polygons, 2 points */
+  double scan_ms;      /* device time of the row walk (outcomes, ring and point counts) */
+  double emit_ms;      /* device time of the ring records and the oriented coordinates (or the points) */
 } gsj_copy_stats;
 int gsj_copy_parse_device(gsj_ctx* ctx, const uint8_t* buf, uint64_t len, int header, int kind, gsj_hostset** polys, double** pts,
                           uint64_t* npts, char** ids, uint64_t** id_off, gsj_copy_stats* stats);

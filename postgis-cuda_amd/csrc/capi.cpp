@@ -492,7 +492,7 @@ int gsj_copy_parse_device(gsj_ctx* ctx, const uint8_t* buf, uint64_t len, int he
       set->s = std::move(L.polys);
     }
     *stats = gsj_copy_stats{L.rows, L.nulls, L.bad_type, L.empty, L.npolys, L.npoints, L.consumed, L.done ? 1 : 0,
-                            kind ? kind : (L.npolys ? 1 : (L.npoints ? 2 : 0))};
+                            kind ? kind : (L.npolys ? 1 : (L.npoints ? 2 : 0)), L.scan_ms, L.emit_ms};
     *ids = static_cast<char*>(b_ids.release());
     *id_off = static_cast<uint64_t*>(b_off.release());
     if (L.points_layer) {

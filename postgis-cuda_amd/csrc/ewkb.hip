@@ -325,6 +325,7 @@ CopyLayer copy_parse_device(Context& c, const uint8_t* buf, uint64_t len, bool h
   const uint32_t blocks = (uint32_t)div_up(n, 256);
   hipLaunchKernelGGL(k_ewkb_init, dim3(1), dim3(1), 0, st, first, hist);
   GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[0], st));
   hipLaunchKernelGGL(k_ewkb_scan, dim3(blocks), dim3(256), 0, st, dbuf, at, ln, n, cls, nr, nv, first);
   GSJ_LAUNCHED();
   uint8_t* kind = reinterpret_cast<uint8_t*>(misc + 8);
@@ -332,9 +333,13 @@ CopyLayer copy_parse_device(Context& c, const uint8_t* buf, uint64_t len, bool h
   GSJ_LAUNCHED();
   hipLaunchKernelGGL(k_ewkb_keep, dim3(blocks), dim3(256), 0, st, cls, nr, nv, keep, n, first, given, kind, hist);
   GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[1], st));
   uint32_t* hh = c.h_scal.as<uint32_t>(8);
   GSJ_HIP(hipMemcpyAsync(hh, hist, 32, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  GSJ_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+  out.scan_ms = ms;
   uint32_t h_hist[8];
   std::memcpy(h_hist, hh, sizeof h_hist);
   out.nulls = h_hist[R_NULL];
@@ -349,9 +354,11 @@ CopyLayer copy_parse_device(Context& c, const uint8_t* buf, uint64_t len, bool h
   if (out.points_layer) {
     out.npoints = ng;
     double2* pxy = b_xy.as<double2>(ng);
+    GSJ_HIP(hipEventRecord(c.ev[2], st));
     hipLaunchKernelGGL(k_ewkb_points, dim3(blocks), dim3(256), 0, st, dbuf, at, ln, n, cls, gix, pxy, row);
     GSJ_LAUNCHED();
     out.pxy.resize(2 * ng);
+    GSJ_HIP(hipEventRecord(c.ev[3], st));
     GSJ_HIP(hipMemcpyAsync(out.pxy.data(), pxy, ng * 16, hipMemcpyDeviceToHost, st));
   } else {
     out.npolys = ng;
@@ -363,10 +370,12 @@ CopyLayer copy_parse_device(Context& c, const uint8_t* buf, uint64_t len, bool h
     uint64_t* rvo = b_rvo.as<uint64_t>(nrt + 1);
     RingRec* rings = b_ring.as<RingRec>(nrt);
     double2* xy = b_xy.as<double2>(nvt);
+    GSJ_HIP(hipEventRecord(c.ev[2], st));
     hipLaunchKernelGGL(k_ewkb_rings, dim3(blocks), dim3(256), 0, st, dbuf, at, ln, n, cls, gix, roff, voff, gro, rvo, rings, row);
     GSJ_LAUNCHED();
     hipLaunchKernelGGL(k_ewkb_coords, dim3((uint32_t)div_up(nrt, 256)), dim3(256), 0, st, dbuf, rings, nrt, xy);
     GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[3], st));
     out.polys.gro.resize(ng + 1);
     out.polys.rvo.resize(nrt + 1);
     out.polys.xy.resize(2 * nvt);
@@ -378,6 +387,8 @@ CopyLayer copy_parse_device(Context& c, const uint8_t* buf, uint64_t len, bool h
   }
   GSJ_HIP(hipMemcpyAsync(out.row.data(), row, ng * 8, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
+  GSJ_HIP(hipEventElapsedTime(&ms, c.ev[2], c.ev[3]));
+  out.emit_ms = ms;
   // the accepted rows' ids, in row order
   out.id_off.reserve(ng + 1);
   for (uint64_t g = 0; g < ng; g++) {

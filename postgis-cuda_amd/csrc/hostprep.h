@@ -58,6 +58,7 @@ struct CopyLayer {
   std::string ids;
   uint64_t rows = 0, nulls = 0, bad_type = 0, empty = 0, npolys = 0, npoints = 0, consumed = 0;
   bool done = false;
+  double scan_ms = 0, emit_ms = 0;   // device time of the row walk (pass 1) and of the ring records + coordinates
 };
 // kind: the layer kind an earlier piece of the same stream decided (0 none yet, 1 polygons, 2 points)
 CopyLayer copy_parse_device(Context& c, const uint8_t* buf, uint64_t len, bool header, int kind);

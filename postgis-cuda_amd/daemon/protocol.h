@@ -22,8 +22,9 @@
  *   DWITHIN    layer_a, layer_b, mode ("dwithin" | "buffer"), dist_a, dist_b, self_pairs[, "stats" | "distance"]
  *
  * Differences, all additive: there is no Postgres behind this daemon, so LOAD's `source` (LOAD_QUERY's sql) names a
- * layer file or a generator instead of a table or query ("gset:PATH", "pset:PATH", "gen:KIND:SPEC",
- * "genpts:SPEC"; id_col, geom_col and filter are accepted and ignored); DWITHIN's optional 7th argument may be
+ * layer file or a generator instead of a table or query ("gset:PATH", "pset:PATH", "copy:PATH" (a binary COPY dump of
+ * (id text, geom) rows, the stream the reference's loader reads from Postgres), "gen:KIND:SPEC", "genpts:SPEC";
+ * id_col, geom_col and filter are accepted and ignored); DWITHIN's optional 7th argument may be
  * "distance", which fills the f64 slot with the geography distance (dwithin mode only). */
 #ifndef GSJ_DAEMON_PROTOCOL_H
 #define GSJ_DAEMON_PROTOCOL_H

@@ -296,7 +296,8 @@ def dedup(hs: HostSet, ctx: "Context" = None):
 
 class CopyStats(C.Structure):
     _fields_ = [("rows", C.c_uint64), ("skipped_null", C.c_uint64), ("skipped_type", C.c_uint64), ("skipped_empty", C.c_uint64),
-                ("polys", C.c_uint64), ("points", C.c_uint64), ("consumed", C.c_uint64), ("done", C.c_int), ("kind", C.c_int)]
+                ("polys", C.c_uint64), ("points", C.c_uint64), ("consumed", C.c_uint64), ("done", C.c_int), ("kind", C.c_int),
+                ("scan_ms", C.c_double), ("emit_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -311,9 +312,10 @@ class CopyLayer:
     stats: dict
 
 
-def copy_parse(ctx: "Context", buf: bytes, header=True, kind=0) -> CopyLayer:
+def copy_parse(ctx: "Context", buf: bytes, header=True, kind=0, want_ids=True) -> CopyLayer:
     """gsj_copy_parse_device: the rows of a binary COPY stream of (id text, geom) parsed on ctx's device; stats carries
-    the loader's counters, consumed / done, and the layer kind for the next piece of the same stream"""
+    the loader's counters, consumed / done, and the layer kind for the next piece of the same stream. want_ids=False
+    leaves ids as None (no Python string per row)."""
     lib = load()
     raw = C.create_string_buffer(bytes(buf), max(len(buf), 1))
     h, p, n = C.c_void_p(), C.POINTER(C.c_double)(), C.c_uint64()
@@ -324,7 +326,7 @@ def copy_parse(ctx: "Context", buf: bytes, header=True, kind=0) -> CopyLayer:
         nacc = st.polys + st.points
         o = np.ctypeslib.as_array(off, (nacc + 1,)).copy()
         blob = C.string_at(ids.value, int(o[-1])) if nacc and o[-1] else b""
-        idl = [blob[int(o[k]):int(o[k + 1])].decode() for k in range(nacc)]
+        idl = [blob[int(o[k]):int(o[k + 1])].decode() for k in range(nacc)] if want_ids else None
         pts = np.ctypeslib.as_array(p, (2 * n.value,)).copy().reshape(-1, 2) if n.value else (np.zeros((0, 2)) if p else None)
     finally:
         lib.gsj_free(ids)

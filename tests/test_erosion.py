@@ -86,3 +86,36 @@ def test_erosion_does_not_depend_on_subdivision():
         b = O.erode(O.Layer.polys(S), O.Layer.polys(S), ra, rb, same=True, upper=True)
         assert {tuple(p) for p in a["pairs"].tolist()} == {tuple(p) for p in b["pairs"].tolist()}
         assert 0 < len(a["pairs"]) < a["candidates"]
+
+
+POLAR = 75.0
+
+
+def _rects_at(lat, lon0, o_m, w_m=200.0, h_m=20.0):
+    """two w x h metre rectangles at (lon0, lat) overlapping by o_m metres east-west (metres of the local degree
+    lengths at lat)"""
+    kx, ky = O.m_per_deg(lat)
+
+    def rect(x0):
+        w, h = w_m / kx, h_m / ky
+        return [(x0, lat), (x0 + w, lat), (x0 + w, lat + h), (x0, lat + h), (x0, lat)]
+
+    x0 = lon0 - w_m / kx / 2
+    return _set([rect(x0), rect(x0 + (w_m - o_m) / kx)])
+
+
+@pytest.mark.parametrize("ra,rb", [(2.0, 1.0), (0.0, 2.5), (1.5, 1.5)])
+def test_polar_erosion_takes_scale_one(ra, rb):
+    """Above 70 degrees PostGIS's _ST_BestSRID picks a Lambert azimuthal equal-area projection, whose scale the
+    reference takes as 1 (docs/technical-details.md:468-473, engine/buffer.cuh:55-57), and so does this build: at
+    75 degrees eroded rectangles meet iff their overlap exceeds ra + rb true metres, while on a UTM zone's central
+    meridian the threshold is (ra + rb) / 0.9996. An overlap of 1.0002 (ra + rb) therefore meets at 75 degrees and
+    does not at 34 degrees on the meridian (-117)."""
+    t = ra + rb
+    for f, expect in ((0.999, False), (1.0002, True), (1.001, True)):
+        L = O.Layer.polys(_rects_at(POLAR, 20.0, t * f))
+        assert O.er_pair(L, L, 0, 1, ra, rb) == expect, (ra, rb, f)
+    L = O.Layer.polys(_rects_at(LAT, -117.0, t * 1.0002))
+    assert not O.er_pair(L, L, 0, 1, ra, rb)
+    L = O.Layer.polys(_rects_at(LAT, -117.0, t / K0 * 1.0002))
+    assert O.er_pair(L, L, 0, 1, ra, rb)

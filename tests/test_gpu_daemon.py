@@ -351,6 +351,79 @@ def test_load_from_a_binary_copy_stream(daemon):
     assert c.call(G.OP_DROP, "acopy") is True
 
 
+@pytest.fixture(scope="module")
+def replay(tmp_path_factory):
+    """tests/srf_replay.c: the extension's way of reading streams (1 MiB buffer, u16 ids, row layouts per SRF)"""
+    exe = tmp_path_factory.mktemp("srf") / "srf_replay"
+    subprocess.run(["gcc", "-O2", "-o", str(exe), os.path.join(ROOT, "tests", "srf_replay.c"), "-lm"], check=True)
+    return str(exe)
+
+
+def _replay(exe, sock, kind, op, *args):
+    r = subprocess.run([exe, sock, TOKEN, kind, str(op), *map(str, args)], capture_output=True, timeout=300)
+    lines = r.stdout.decode().splitlines()
+    stats = json.loads(lines[-1][len("#stats "):]) if r.returncode == 0 else None
+    return r.returncode, (lines[:-1] if r.returncode == 0 else lines), stats
+
+
+def _as_lines(kind, rows):
+    if kind == "join":
+        return [f"{a}\t{b}\t" + ("NULL" if math.isnan(v) else "%.17g" % v) for a, b, v in rows]
+    if kind == "relate":
+        return [f"{a}\t{b}\t{code}" for a, b, code in rows]
+    if kind == "cluster":
+        return [f"{a}\t{k}" for a, k in rows]
+    return [f"{a}\t{b}" for a, b, _ in rows]
+
+
+def test_the_extension_reader_reads_every_stream(daemon, replay):
+    """every streaming op as the extension's SRFs request and read it (gpu_join, gpu_relate, gpu_cluster_intersecting,
+    gpu_dwithin / gpu_intersects_buffered pair rows, the stats modes): the rows this daemon streams are the rows its
+    Python client reads, each row once, and the stats JSON counts them"""
+    c, sock = daemon["client"], daemon["sock"]
+    cases = [("join", G.OP_JOIN, ("a", "a", "1", "distinct"), lambda: c.join("a", "a", True, "distinct")),
+             ("join", G.OP_JOIN, ("a", "b", "0", "all"), lambda: c.join("a", "b", False, "all")),
+             ("relate", G.OP_RELATE, ("a", "b", "", "all"), lambda: c.relate("a", "b", "", "all")),
+             ("relate", G.OP_RELATE, ("a", "a", "T*T***T**", "distinct"), lambda: c.relate("a", "a", "T*T***T**", "distinct")),
+             ("cluster", G.OP_CLUSTER, ("a",), lambda: c.cluster("a")),
+             ("pair", G.OP_DWITHIN, ("a", "a", "dwithin", "3", "0", "distinct"), lambda: c.dwithin("a", "a", "dwithin", 3, 0, "distinct")),
+             ("pair", G.OP_DWITHIN, ("a", "b", "buffer", "-1", "-0.5", "all"), lambda: c.dwithin("a", "b", "buffer", -1, -0.5, "all")),
+             ("pair", G.OP_DWITHIN, ("p", "b", "dwithin", "10", "0", "all"), lambda: c.dwithin("p", "b", "dwithin", 10, 0, "all"))]
+    for kind, op, args, ref in cases:
+        rc, lines, stats = _replay(replay, sock, kind, op, *args)
+        assert rc == 0, (args, lines[-3:])
+        rows, _, _ = ref()
+        assert sorted(lines) == sorted(_as_lines(kind, rows)), args
+        assert len(lines) == len(set(lines)) or kind == "cluster"
+        assert stats["rows"] == len(lines), (args, stats)
+    for op, args in ((G.OP_JOIN, ("a", "a", "1", "all", "stats")), (G.OP_DWITHIN, ("a", "a", "dwithin", "3", "0", "all", "stats"))):
+        rc, lines, stats = _replay(replay, sock, "stats", op, *args)
+        assert rc == 0 and lines == [] and stats["rows"] > 0, (args, lines)
+    rc, lines, _ = _replay(replay, sock, "join", G.OP_JOIN, "nope", "a", "1", "all")
+    assert rc == 1 and lines[0].startswith("ERROR: gpujoind: ") and "nope" in lines[0]
+    rc, lines, _ = _replay(replay, sock, "join", G.OP_JOIN, "a", "a", "1", "some")
+    assert rc == 1 and "self_pairs" in lines[0]
+
+
+def test_ids_at_the_u16_limit(daemon, replay):
+    """ids travel as u16 length + bytes: one of 65535 bytes arrives whole, a longer one is cut to 65535 bytes (as the
+    reference daemon cuts it); rows of two such ids (131 KB each) pass the extension's 1 MiB buffer"""
+    d = daemon["dir"]
+    sq = [(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0), (0.0, 0.0)]
+    sq2 = [(0.5, 0.5), (1.5, 0.5), (1.5, 1.5), (0.5, 1.5), (0.5, 0.5)]
+    hs = gsj.HostSet(np.array([0, 1, 2], np.uint64), np.array([0, 5, 10], np.uint64), np.array(sq + sq2, float))
+    gsj.write_gset(os.path.join(d, "long.gset"), hs)
+    with open(os.path.join(d, "long.gset.ids"), "w") as f:
+        f.write("x" * 65535 + "\n" + "y" * 70000 + "\n")
+    c = daemon["client"]
+    assert c.load("long", "gset:long.gset")["rows"] == 2
+    rc, lines, stats = _replay(replay, daemon["sock"], "join", G.OP_JOIN, "long", "long", "1", "all")
+    assert rc == 0 and stats["rows"] == 4
+    ids = {tuple(l.split("\t")[:2]) for l in lines}
+    assert ("x" * 65535, "y" * 65535) in ids and ("y" * 65535, "y" * 65535) in ids
+    assert c.call(G.OP_DROP, "long") is True
+
+
 def test_drop(daemon):
     c = daemon["client"]
     assert c.load("tmp", "gen:parcels_grid:nx=10;ny=10;seed=1")["rows"] == 100
@@ -379,6 +452,10 @@ def test_row_limit_ends_the_stream_with_a_frame_error(daemon):
         assert len(rows) == st["rows"] == 9
         with pytest.raises(G.DaemonError, match="row limit of 200"):
             c.join("a", "a", want_area=True)
+        exe = os.path.join(tempfile.mkdtemp(), "srf_replay")   # the extension's reader raises it as its ERROR
+        subprocess.run(["gcc", "-O2", "-o", exe, os.path.join(ROOT, "tests", "srf_replay.c"), "-lm"], check=True)
+        rc, lines, _ = _replay(exe, sock, "join", G.OP_JOIN, "a", "a", "1", "all")
+        assert rc == 1 and lines[-1].startswith("ERROR: gpujoind: row limit of 200"), lines[-3:]
         if os.path.exists(GSP_CAT):
             r = subprocess.run([GSP_CAT, sock, str(G.OP_JOIN), "a", "a", "1", "all"], env=dict(os.environ, GSP_TOKEN=TOKEN),
                                capture_output=True, text=True, timeout=60)

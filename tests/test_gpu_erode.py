@@ -84,9 +84,29 @@ def test_points_in_eroded_polygons(ctx, subdiv):
 
 
 def test_erosion_errors(ctx):
+    """erosion radii are >= 0: a negative one (a grown side, i.e. buffers of opposite sign) is refused, as the
+    reference refuses opposite-sign buffers (daemon/gpujoind.cu:406-411); two grown sides are a distance join"""
     L = ctx.polys(_layer(5))
-    with pytest.raises(gsj.GsjError):
-        ctx.intersects_buffered(L, L, -1.0, 1.0)
+    for ra, rb in ((-1.0, 1.0), (1.0, -0.5), (1e5, 1.0)):
+        with pytest.raises(gsj.GsjError, match=r"radii must be in \[0, 1e5\) metres"):
+            ctx.intersects_buffered(L, L, ra, rb)
     pts = ctx.points(np.array([[-95.19, 38.61]]))
     with pytest.raises(gsj.GsjError):
         ctx.intersects_buffered(pts, L, 1.0, 1.0)
+
+
+@pytest.mark.parametrize("lat,lon,expect", [(75.0, 20.0, True), (34.0, -117.0, False), (-78.0, 160.0, True)])
+def test_polar_erosion_takes_scale_one(ctx, lat, lon, expect):
+    """the engine's eroded intersects at polar latitudes (scale 1, as the reference) and on a UTM central meridian
+    (scale 0.9996): rectangles overlapping by 1.0002 (ra + rb) metres meet only where the scale is 1; rows equal the
+    oracle's (tests/test_erosion.py pins the oracle by the closed form)"""
+    from test_erosion import _rects_at
+
+    ra, rb = 2.0, 1.0
+    hs = _rects_at(lat, lon, (ra + rb) * 1.0002)
+    L = ctx.polys(hs)
+    got = ctx.intersects_buffered(L, L, ra, rb, same=True, upper=True)
+    OL = O.Layer.polys(hs)
+    exp = O.erode(OL, OL, ra, rb, same=True, upper=True)
+    assert _rows(got.pairs) == _rows(exp["pairs"])
+    assert ((0, 1) in _rows(got.pairs)) == expect
