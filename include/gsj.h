@@ -55,7 +55,9 @@ const char* gsj_ctx_status_json(gsj_ctx* ctx);
 int gsj_ctx_trim(gsj_ctx* ctx);
 /* Launch shapes of the pair kernels per launch size, as text: '#' comments and one line per size bucket,
  * "bucket np>=N key=value ..." with keys tcap (0..16), mcap (0 or 33..64), screen_occ (4, 5), screen_bpc, edge_bpc,
- * mixed_bpc, large_bpc, dw_bpc (blocks per CU, 1..64) and trav_block (64, 128, 256); buckets ascend from np>=0.
+ * mixed_bpc, large_bpc, dw_bpc (blocks per CU, 1..64), trav_block (64, 128, 256) and ovf_lds (1: overflow pairs
+ * with lists of at most 448 edges per side keep them in LDS; 0: in global memory) and fork (1: the mixed and large
+ * classes run on a second stream of the context beside the other classes; 0: in line); buckets ascend from np>=0.
  * Replaces the reference's fixed launch shapes (engine/engine.cuh:945-951, engine/buffer.cuh:946-947); every shape
  * gives the same rows. set: GSJ_E_ARG on malformed text (the table is unchanged); applies to later calls and to
  * schedulers created afterwards. get: the table in effect; the string is owned by the context, valid until the next

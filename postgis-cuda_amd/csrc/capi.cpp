@@ -39,6 +39,7 @@ int guarded_status(const std::function<void()>& f) {
 Context::~Context() {
   if (borrowed) return;
   for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -57,6 +58,7 @@ void context_init(Context& c, int device) {
   }
   c.tune = load_tuning(arch);
   GSJ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  GSJ_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
   for (hipEvent_t& e : c.ev) GSJ_HIP(hipEventCreate(&e));
   noalloc_setup(device);
 }

@@ -93,6 +93,7 @@ struct Layer {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;     // second stream (default priority): list classes beside the vertex-list classes
   std::mutex mu;                  // one call at a time per context
   std::string status;             // last status JSON
   // scratch, grow-only, reused across calls
@@ -103,7 +104,7 @@ struct Context {
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg, h_slots_ctl;
   int cu_count = 256;
-  hipEvent_t ev[10] = {};   // kernel timing events (created with the context)
+  hipEvent_t ev[12] = {};   // kernel timing events (created with the context); ev[10], ev[11]: fork / join of `side`
   bool borrowed = false;   // stream and events belong to another context (not destroyed with this one)
   Arena* dev_arena = nullptr;   // set: every scratch buffer comes from this arena (the no-allocation entry point)
   Tuning tune;                  // launch shapes of the pair kernels per launch size

@@ -79,6 +79,7 @@ extern "C" int gsj_join_noalloc(gsj_ctx* ctx, const gsj_layer* P, const gsj_laye
     gsj::Context c;
     c.device = owner.device;
     c.stream = owner.stream;
+    c.side = owner.side;
     for (size_t i = 0; i < sizeof c.ev / sizeof c.ev[0]; i++) c.ev[i] = owner.ev[i];
     c.cu_count = owner.cu_count;
     c.tune_from = &owner.tune;

@@ -726,6 +726,7 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
 // side in LDS (the large polygon's edges meeting the overlap box, found through its edge-chunk boxes); pairs whose
 // lists overflow are queued
 constexpr int CAP_MIXED = 32, MIXED_T = 128;
+constexpr uint32_t OVF_LDS_CAP = 448;   // overflow lists up to this many edges per side go to LDS (448 x 128 B = 56 KiB)
 __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                           const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
                                                           RefineOut out, uint32_t* __restrict__ promo, OverflowCtl* promo_ctl,
@@ -754,22 +755,24 @@ __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const 
   }
 }
 
-// overflow: a block of T threads per pair (T = 64: one wave), lists and partial sums in a per-block slab of global
+// overflow: a block of T threads per pair (T = 64: one wave), lists and partial sums in LDS when the launch's largest
+// lists fit (IN_LDS; the edge loops then read LDS instead of global memory), else in a per-block slab of global
 // memory, array path for every edge. The first wave collects the lists and runs the steps linear in the list sizes
 // (containment of c, the ring probes) and the ordered sums; the whole block runs the contact grid and the edge
 // contributions, whose work grows with the product of the list sizes. T is chosen per launch from the number of pairs: four
 // waves per pair when the pairs are too few to fill the GPU (a windowed request's few largest pairs), one wave per pair
 // otherwise. Every barrier sits in block-uniform control flow (the loop bounds and branches around them depend only on
 // the pair and on values the block has read back from shared memory after a barrier).
-template <int T>
+template <int T, bool IN_LDS>
 __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                            const uint32_t* __restrict__ list, uint32_t nlist, RefineParams rp, RefineOut out, int cap,
                                                            unsigned char* __restrict__ slab) {
+  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_na, s_nb, s_hit, s_hit_any, s_found, s_contain, s_inA, s_inB;   // s_hit: contact; s_found: contact or containment
   __shared__ uint32_t s_bits;
   const int t = threadIdx.x, gl = t & 63, w = t >> 6;
   const bool rel = rp.want_relate != 0;
-  const Scratch S = carve(slab + (uint64_t)blockIdx.x * list_bytes(cap, rel), cap, rel);
+  const Scratch S = carve(IN_LDS ? smem : slab + (uint64_t)blockIdx.x * list_bytes(cap, rel), cap, rel);
   for (uint32_t k = blockIdx.x; k < nlist; k += gridDim.x) {
     const uint32_t p = list[k];
     const uint32_t a = pa[p], b = pb[p];
@@ -781,13 +784,18 @@ __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPo
     }
     __syncthreads();
     const int na = s_na, nb = s_nb;
-    // boundary contact over the A x B edge grid (engine/engine.cuh:378-381): a real contact anywhere, else any contact
+    // boundary contact over the A x B edge grid (engine/engine.cuh:378-381): a real contact anywhere, else any contact.
+    // The grid is cut into T strips of whole rows when A has T edges or more, else each row into T / na pieces, so
+    // that every thread of the block gets a share however few edges A has.
     {
       bool hit = false, hit_any = false;
-      for (int i = t; i < na && !hit; i += T) {
+      const int parts = na >= T || na == 0 ? 1 : T / na;   // pieces per row
+      const int chunk = (nb + parts - 1) / parts;
+      for (int r = t; r < na * parts && !hit; r += T) {
+        const int i = r / parts, j0 = (r % parts) * chunk, j1 = min(nb, j0 + chunk);
         const Ed ea = S.LA[i];
         const double ax0 = fmin(ea.x0, ea.x1), ax1 = fmax(ea.x0, ea.x1), ay0 = fmin(ea.y0, ea.y1), ay1 = fmax(ea.y0, ea.y1);
-        for (int j = 0; j < nb; j++) {
+        for (int j = j0; j < j1; j++) {
           const Ed eb = S.LB[j];
           if (fmax(eb.x0, eb.x1) < ax0 || fmin(eb.x0, eb.x1) > ax1 || fmax(eb.y0, eb.y1) < ay0 || fmin(eb.y0, eb.y1) > ay1) continue;
           if (seg_isect(ea.x0, ea.y0, ea.x1, ea.y1, eb.x0, eb.y0, eb.x1, eb.y1)) {
@@ -896,13 +904,42 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   OverflowCtl* promo_ctl = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);
   const int T = 256;
   const uint32_t cu = (uint32_t)c.cu_count;
-  GSJ_HIP(hipEventRecord(c.ev[0], st));
-  EdgeQueue eq;
+  // The list classes (mixed, then large: the large kernel takes the pairs mixed promotes) depend only on the
+  // partition. With sh.fork they run on the side stream while the vertex-list classes run here, so that a small
+  // launch pays the longer of the two chains instead of their sum; both chains append to the overflow queue with
+  // atomics, and every pair's outputs are written by the one kernel of its class.
+  EdgeQueue eq;   // allocated before the fork: a growing buffer would synchronise the device mid-chain
   eq.flag = c.s_edgeq.as<uint32_t>(2 * np + 32);
   eq.list = eq.flag + np + 16;
   eq.off = c.s_tmp2.as<uint64_t>(np + 17);
   eq.count = counts + 8;
   eq.first = eq.count + 3;
+  hipStream_t sl = st;
+  if (sh.fork) {
+    sl = c.side;
+    GSJ_HIP(hipEventRecord(c.ev[10], st));
+    GSJ_HIP(hipStreamWaitEvent(sl, c.ev[10], 0));
+  }
+  GSJ_HIP(hipEventRecord(c.ev[5], sl));
+  {   // mixed: 16-lane groups with edge lists, 8 pairs per block
+    const uint32_t g = std::min<uint32_t>(div_up(np, MIXED_T / 16), cu * sh.mixed_bpc);
+    hipLaunchKernelGGL(k_refine_mixed, dim3(g), dim3(MIXED_T), (MIXED_T / 16) * list_bytes(CAP_MIXED, rel), sl, P, R, pa, pb, list, counts, rp, out,
+                       promo, promo_ctl, ovf, ctl);
+    GSJ_LAUNCHED();
+  }
+  GSJ_HIP(hipEventRecord(c.ev[7], sl));
+  // large: one wave per pair, 4 pairs per block, 96-edge lists (3 blocks per CU fit in LDS; 128 allowed 2); the pairs
+  // that need more go to the overflow kernel (2.4M relate: large 6.1 -> 3.6 ms, overflow 2.7 -> 4.3 ms; a second LDS
+  // tier of 128-edge lists for those was slower than the overflow kernel)
+  constexpr int CAP_LARGE = 96;
+  {
+    const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
+    const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), cu * sh.large_bpc);
+    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, sl, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, CAP_LARGE, ovf, ctl);
+    GSJ_LAUNCHED();
+  }
+  GSJ_HIP(hipEventRecord(c.ev[2], sl));
+  GSJ_HIP(hipEventRecord(c.ev[0], st));
   // grid-stride kernels: grids of one group per pair up to the shape's cap in blocks per CU
   auto grid = [&](uint32_t lanes, uint32_t bpc) { return std::min<uint32_t>(div_up(np, T / lanes), cu * bpc); };
   // screens (every tiny, small and medium pair), then the edge work of the flagged ones in class-list order
@@ -929,25 +966,11 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   hipLaunchKernelGGL((k_edges_small<32, MCAP, CL_MEDIUM>), dim3(gm), dim3(T), (T / 32) * small_bytes<MCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
                      ovf, ctl);
   GSJ_LAUNCHED();
-  GSJ_HIP(hipEventRecord(c.ev[5], st));
-  {   // mixed: 16-lane groups with edge lists, 8 pairs per block
-    const uint32_t g = std::min<uint32_t>(div_up(np, MIXED_T / 16), cu * sh.mixed_bpc);
-    hipLaunchKernelGGL(k_refine_mixed, dim3(g), dim3(MIXED_T), (MIXED_T / 16) * list_bytes(CAP_MIXED, rel), st, P, R, pa, pb, list, counts, rp, out,
-                       promo, promo_ctl, ovf, ctl);
-    GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[9], st));
+  if (sh.fork) {   // the overflow queue is complete once both chains are done
+    GSJ_HIP(hipEventRecord(c.ev[11], sl));
+    GSJ_HIP(hipStreamWaitEvent(st, c.ev[11], 0));
   }
-  GSJ_HIP(hipEventRecord(c.ev[7], st));
-  // large: one wave per pair, 4 pairs per block, 96-edge lists (3 blocks per CU fit in LDS; 128 allowed 2); the pairs
-  // that need more go to the overflow kernel (2.4M relate: large 6.1 -> 3.6 ms, overflow 2.7 -> 4.3 ms; a second LDS
-  // tier of 128-edge lists for those was slower than the overflow kernel)
-  constexpr int CAP_LARGE = 96;
-  {
-    const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
-    const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), cu * sh.large_bpc);
-    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, st, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, CAP_LARGE, ovf, ctl);
-    GSJ_LAUNCHED();
-  }
-  GSJ_HIP(hipEventRecord(c.ev[2], st));
   struct Host { uint32_t counts[16]; OverflowCtl ctl; };   // class sizes [0, 5), edge queue sizes [8, 11)
   Host* h = c.h_scal.as<Host>(2);
   GSJ_HIP(hipMemcpyAsync(h->counts, counts, 64, hipMemcpyDeviceToHost, st));
@@ -959,7 +982,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     GSJ_HIP(hipEventElapsedTime(&ms_q, c.ev[0], c.ev[6]));
     GSJ_HIP(hipEventElapsedTime(&ms_t, c.ev[6], c.ev[8]));
     GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[8], c.ev[1]));
-    GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
+    GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[9]));
     GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
     GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
     stats->k_screen_ms += ms_q; stats->k_tiny_ms += ms_t; stats->k_small_ms += ms_s; stats->k_medium_ms += ms_m; stats->k_mixed_ms += ms_x;
@@ -970,7 +993,26 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     stats->n_tiny_edge += h->counts[8 + CL_TINY]; stats->n_small_edge += h->counts[8 + CL_SMALL]; stats->n_medium_edge += h->counts[8 + CL_MEDIUM];
     stats->launches++;
   }
-  if (nover) {
+  if (nover && sh.ovf_lds && need <= OVF_LDS_CAP) {   // lists in LDS: no slab
+    const size_t lds = list_bytes((int)need, rel);
+    GSJ_HIP(hipEventRecord(c.ev[3], st));
+    // pairs too few to fill the GPU (a windowed request's few largest): eight waves each, so that the edge
+    // contributions of both lists (up to 2 x 448 edges) take one or two rounds
+    const bool few = nover <= (uint64_t)c.cu_count;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(nover, (uint64_t)c.cu_count * (few ? 1 : 8));
+    if (few)
+      hipLaunchKernelGGL((k_refine_overflow<512, true>), dim3(blocks), dim3(512), lds, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, nullptr);
+    else
+      hipLaunchKernelGGL((k_refine_overflow<64, true>), dim3(blocks), dim3(64), lds, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, nullptr);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[4], st));
+    if (stats) {
+      GSJ_HIP(hipEventSynchronize(c.ev[4]));
+      float ms_o = 0;
+      GSJ_HIP(hipEventElapsedTime(&ms_o, c.ev[3], c.ev[4]));
+      stats->k_over_ms += ms_o;
+    }
+  } else if (nover) {
     const size_t per = list_bytes((int)need, rel);
     // slabs of at most 1 GiB (half of what is left when the scratch is a fixed arena: compaction still needs room)
     const uint64_t budget = c.dev_arena ? std::min<uint64_t>(1ull << 30, c.dev_arena->left() / 2) : (1ull << 30);
@@ -979,9 +1021,9 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     unsigned char* slab = c.s_heavy_edges.as<unsigned char>(per * blocks + 64);
     GSJ_HIP(hipEventRecord(c.ev[3], st));
     if (nover <= 2ull * c.cu_count)   // a few pairs: four waves each (2.4M relate: one wave each, 2.5 vs 4.5 ms)
-      hipLaunchKernelGGL(k_refine_overflow<256>, dim3((uint32_t)blocks), dim3(256), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
+      hipLaunchKernelGGL((k_refine_overflow<256, false>), dim3((uint32_t)blocks), dim3(256), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
     else
-      hipLaunchKernelGGL(k_refine_overflow<64>, dim3((uint32_t)blocks), dim3(64), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
+      hipLaunchKernelGGL((k_refine_overflow<64, false>), dim3((uint32_t)blocks), dim3(64), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[4], st));
     if (stats) {

@@ -38,17 +38,20 @@ def test_engine_is_invariant_under_subdivision(ctx, cases, ci, name, maxv):
         assert I.failures(rep["self"]) == 0, rep
 
 
-@pytest.mark.parametrize("n,few", [(12, True), (40, False)])
-def test_join_overflow_block_shapes(ctx, n, few):
+@pytest.mark.parametrize("n,few,maxv", [(8, True, 0), (40, False, 0), (8, True, 400), (40, False, 400)])
+def test_join_overflow_block_shapes(ctx, n, few, maxv):
     """Parcels of median 150 vertices: their self pairs (and some neighbours) collect more than 96 edges per side and
-    go to the overflow kernel, four waves per pair when a launch holds at most two pairs per CU (n = 12), one wave
-    per pair otherwise (n = 40)."""
+    go to the overflow kernel, eight waves per pair when a launch holds at most one pair per CU (n = 8), one wave
+    per pair otherwise (n = 40). Unsubdivided, the largest lists exceed 448 edges and the launch keeps the lists in
+    global memory; subdivided at 400 vertices they fit and sit in LDS."""
     M = gsj.generate("parcels_mesh", nx=n, ny=n, w=1 / 1549, h=1 / 1549, seed=9, median=150.0)
+    if maxv:
+        M = gsj.subdivide(M, maxv)
     L = ctx.polys(M)
     cu = json.loads(ctx.status())["cu"]
     got = ctx.join(L, L, want_area=True, want_relate=True, upper=True)
     assert got.stats["n_over"] > 0
-    assert (got.stats["n_over"] <= 2 * cu) == few, got.stats["n_over"]
+    assert (got.stats["n_over"] <= cu) == few, got.stats["n_over"]
     OM = O.Layer.polys(M)
     exp = O.join(OM, OM, want_area=True, want_relate=True, upper=True)
     assert got.stats["candidates"] == exp["candidates"]
