@@ -78,7 +78,7 @@ int gsj_layer_create_polys(gsj_ctx* ctx, const uint64_t* gro, const uint64_t* rv
 int gsj_layer_create_points(gsj_ctx* ctx, const double* xy, uint64_t n, gsj_layer** out);
 void gsj_layer_free(gsj_layer* layer);
 /* Build (or rebuild) the layer's LBVH (engine/engine.cuh:197-228, build_bvh): 64-bit Morton keys, on-device
- * radix sort, Karras hierarchy, atomic bottom-up refit, nodes stored in depth-first order with skip links. */
+ * radix sort, Karras hierarchy, internal boxes from leaf ranges, nodes stored in depth-first order with skip links. */
 int gsj_bvh_build(gsj_layer* layer);
 
 typedef struct {
@@ -87,6 +87,7 @@ typedef struct {
   int is_points;           /* 1 for a point layer */
   int has_bvh;
   uint64_t device_bytes;   /* layer + BVH */
+  double bvh_ms;           /* device time of the last BVH build (HIP events around its kernels) */
 } gsj_layer_info;
 int gsj_layer_info_get(const gsj_layer* layer, gsj_layer_info* info);
 /* Copy the prepared device arrays back (for verification): origin[2n], ext[2n], bb[4n] (float), q[2nv]
@@ -137,6 +138,7 @@ typedef struct {
    * classes; k_tiny_ms times its edge-contribution kernel over n_tiny_edge pairs (distance joins: its list kernel) */
   double k_tiny_ms;
   uint64_t n_tiny, n_tiny_edge;
+  double k_traverse_ms;    /* the candidate search's walks (count and write passes), HIP-event time */
 } gsj_join_stats;
 
 /* Row callback: n rows; pairs[2i], pairs[2i+1] = (a, b) parent ids (probe side, build side); area[i] or NULL;

@@ -3,7 +3,9 @@
 // MI355X design:
 //  * 64-bit Morton keys (32 bits per axis, quantised in FP64) instead of 32-bit, sorted by the engine's own LSD radix
 //    sort (prims.hip);
-//  * Karras (2012) hierarchy emission with ties broken by index, atomic bottom-up refit with acquire/release RMWs;
+//  * Karras (2012) hierarchy emission with ties broken by index; internal boxes from each node's leaf range over a
+//    pyramid of aligned block unions (no bottom-up walk: its device-scope acquire/release counters made the L2 of
+//    every XCD write back, 6.6 ms at 2.4M leaves);
 //  * the finished tree is re-emitted in depth-first preorder: the left child of an internal node is the next node
 //    and each node carries the index of the first node after its subtree. Traversal is then a stackless forward
 //    scan (no per-lane stack in scratch memory) that touches nodes in increasing address order. The preorder index
@@ -103,22 +105,42 @@ __global__ void k_leaf_boxes(const float4* bb, const uint32_t* order, uint32_t n
   if (i < n) box[n - 1 + i] = bb[order[i]];
 }
 
-// bottom-up union of child boxes: the second child to arrive at a node merges (acquire/release on the counter)
-__global__ void k_refit(int n, const int32_t* left, const int32_t* right, const int32_t* parent, float4* box, uint32_t* flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int node = n - 1 + i;
-  for (int guard = 0; guard < 256; guard++) {
-    const int p = parent[node];
-    if (p < 0) return;
-    const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    const float4 a = box[left[p]], b = box[right[p]];
-    // no fence here: the next level's acquire-release increment publishes this store to the thread that merges
-    // there (an agent-scope fence costs an L2 write-back per XCD; it took the 2.4M-leaf refit from 6 to 11 ms)
-    box[p] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
-    node = p;
+// Internal node boxes without a bottom-up walk: node x covers the leaves lo[x] .. hi[x] (Karras), so its box is the
+// union of their boxes, read from a pyramid of aligned blocks (level k entry j: the union of leaves [j 2^k, (j+1) 2^k))
+// as at most two blocks per level. The same leaves give the same min / max, so the boxes equal those of a refit
+// through the children; every node is independent (no arrival counters, no device-scope atomics, no cross-XCD
+// coherence traffic).
+constexpr int PYR_LEVELS = 34;
+struct Pyramid {
+  const float4* lev[PYR_LEVELS];
+  uint64_t size[PYR_LEVELS];
+  int n;
+};
+
+__device__ __forceinline__ float4 box_union(float4 a, float4 b) {
+  return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+}
+
+// one pyramid level from the one below (an odd last entry is carried up alone)
+__global__ void k_box_level(const float4* __restrict__ below, uint64_t nb, float4* __restrict__ above, uint64_t na) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= na) return;
+  const float4 a = below[2 * j];
+  above[j] = 2 * j + 1 < nb ? box_union(a, below[2 * j + 1]) : a;
+}
+
+__global__ void k_range_boxes(int n, const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, Pyramid P, float4* __restrict__ box) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n - 1) return;
+  uint64_t l = lo[x], r = (uint64_t)hi[x] + 1;   // leaves [l, r) at level 0
+  float4 u = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+  for (int k = 0; l < r && k < P.n; k++) {
+    if (l & 1) u = box_union(u, P.lev[k][l++]);
+    if (r & 1) u = box_union(u, P.lev[k][--r]);
+    l >>= 1;
+    r >>= 1;
   }
+  box[x] = u;
 }
 
 // preorder position and skip link of every node; scatter into the final node array
@@ -214,7 +236,7 @@ void bvh_build(Layer& L) {
   BNode* nodes = static_cast<BNode*>(L.bvh_mem);
   uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(L.bvh_mem) + (((size_t)nn * sizeof(BNode) + 255) & ~size_t(255)));
   // temporaries
-  DevBuf t_keys, t_keys2, t_vals, t_vals2, t_tree, t_box, t_misc;
+  DevBuf t_keys, t_keys2, t_vals, t_vals2, t_tree, t_box, t_misc, t_pyr;
   uint64_t* keys = t_keys.as<uint64_t>(n);
   uint64_t* keys2 = t_keys2.as<uint64_t>(n);
   uint32_t* vals = t_vals.as<uint32_t>(n);
@@ -225,10 +247,10 @@ void bvh_build(Layer& L) {
   int32_t* parent = right + n;
   uint32_t* lo = reinterpret_cast<uint32_t*>(parent + nn);
   uint32_t* hi = lo + n;
-  uint32_t* flags = hi + n;
   float4* box = t_box.as<float4>(nn);
   int32_t* bounds = t_misc.as<int32_t>(16);
   uint32_t* err = reinterpret_cast<uint32_t*>(bounds + 8);
+  GSJ_HIP(hipEventRecord(c.ev[12], st));
   hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, st, bounds);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, err, (uint64_t)1, 0u);
   hipLaunchKernelGGL(k_centre_bounds, dim3(std::min<uint32_t>(div_up(n, 256), 2048)), dim3(256), 0, st, bb, n, bounds);
@@ -237,21 +259,42 @@ void bvh_build(Layer& L) {
   radix_sort_pairs(c, keys, keys2, vals, vals2, n, 0, 64);
   GSJ_HIP(hipMemcpyAsync(order, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_fill_i32, dim3(div_up(nn, 256)), dim3(256), 0, st, parent, (uint64_t)nn, -1);
-  hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, 256)), dim3(256), 0, st, flags, (uint64_t)n, 0u);
   hipLaunchKernelGGL(k_leaf_boxes, dim3(div_up(n, 256)), dim3(256), 0, st, bb, order, n, box);
   if (n > 1) {
     hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, st, keys, (int)n, left, right, parent, lo, hi);
-    hipLaunchKernelGGL(k_refit, dim3(div_up(n, 256)), dim3(256), 0, st, (int)n, left, right, parent, box, flags);
+    // the block pyramid over the leaf boxes (level 0 is box[n - 1 ..], leaves in Morton order), then every internal
+    // node from its leaf range
+    Pyramid pyr{};
+    pyr.lev[0] = box + (n - 1);
+    pyr.size[0] = n;
+    uint64_t total = 0;
+    for (uint64_t s = n; s > 1; s = (s + 1) / 2) total += (s + 1) / 2;
+    float4* up = t_pyr.as<float4>(total + 1);
+    int k = 0;
+    for (uint64_t s = n; s > 1 && k + 1 < PYR_LEVELS; k++) {
+      const uint64_t na = (s + 1) / 2;
+      hipLaunchKernelGGL(k_box_level, dim3(div_up(na, 256)), dim3(256), 0, st, pyr.lev[k], s, up, na);
+      pyr.lev[k + 1] = up;
+      pyr.size[k + 1] = na;
+      up += na;
+      s = na;
+    }
+    pyr.n = k + 1;
+    GSJ_LAUNCHED();
+    hipLaunchKernelGGL(k_range_boxes, dim3(div_up(n - 1, 256)), dim3(256), 0, st, (int)n, lo, hi, pyr, box);
   }
   hipLaunchKernelGGL(k_preorder, dim3(div_up(nn, 256)), dim3(256), 0, st, (int)n, left, parent, lo, hi, box, order, nodes, err);
   GSJ_LAUNCHED();
+  float4* tiles = reinterpret_cast<float4*>(static_cast<char*>(L.bvh_mem) + tiles_off);
+  tile_boxes(c, bb, order, n, tiles);
+  GSJ_HIP(hipEventRecord(c.ev[13], st));
   uint32_t* herr = c.h_scal.as<uint32_t>(4);
   GSJ_HIP(hipMemcpyAsync(herr, err, 4, hipMemcpyDeviceToHost, st));
   GSJ_HIP(hipStreamSynchronize(st));
   GSJ_REQUIRE(herr[0] == 0, GSJ_E_INTERNAL, "bvh build: tree deeper than expected");
-  float4* tiles = reinterpret_cast<float4*>(static_cast<char*>(L.bvh_mem) + tiles_off);
-  tile_boxes(c, bb, order, n, tiles);
-  GSJ_HIP(hipStreamSynchronize(st));
+  float ms = 0;
+  GSJ_HIP(hipEventElapsedTime(&ms, c.ev[12], c.ev[13]));
+  L.build_ms = ms;
   L.bvh.nnodes = nn;
   L.bvh.nodes = nodes;
   L.bvh.order = order;

@@ -205,6 +205,7 @@ int gsj_layer_info_get(const gsj_layer* layer, gsj_layer_info* info) {
     info->n = L.n; info->nr = L.nr; info->nv = L.nv; info->nparent = L.nparent;
     info->is_points = L.points ? 1 : 0; info->has_bvh = L.has_bvh ? 1 : 0;
     info->device_bytes = L.mem_bytes + L.bvh_bytes;
+    info->bvh_ms = L.build_ms;
   });
 }
 

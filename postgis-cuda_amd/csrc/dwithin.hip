@@ -564,6 +564,23 @@ void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
   const PairClasses pc = partition_pairs(c, A.P, A.R, A.pa, A.pb, np, sh.mcap, true, sh.tcap);
   uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
   const uint32_t max_blocks = (uint32_t)c.cu_count * sh.dw_bpc;
+  // the classes are independent (each appends its overflowing pairs to the queue with atomics): with sh.fork the
+  // mixed and large classes run on the side stream beside the others
+  hipStream_t sl = stream;
+  if (sh.fork) {
+    sl = c.side;
+    GSJ_HIP(hipEventRecord(c.ev[10], stream));
+    GSJ_HIP(hipStreamWaitEvent(sl, c.ev[10], 0));
+  }
+  GSJ_HIP(hipEventRecord(c.ev[5], sl));
+  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, sl, A, pc.list, pc.counts, (int)CL_MIXED,
+                     ovf, pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[7], sl));
+  hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, sl, A, pc.list, pc.counts, (int)CL_LARGE,
+                     ovf, pc.ctl);
+  GSJ_LAUNCHED();
+  GSJ_HIP(hipEventRecord(c.ev[2], sl));
   GSJ_HIP(hipEventRecord(c.ev[0], stream));
   hipLaunchKernelGGL((k_dw_lists<8, 16>), dim3(std::min<uint32_t>(div_up(np, 32), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_TINY,
                      ovf, pc.ctl);
@@ -576,15 +593,11 @@ void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
   hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_MEDIUM,
                      ovf, pc.ctl);
   GSJ_LAUNCHED();
-  GSJ_HIP(hipEventRecord(c.ev[5], stream));
-  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_MIXED,
-                     ovf, pc.ctl);
-  GSJ_LAUNCHED();
-  GSJ_HIP(hipEventRecord(c.ev[7], stream));
-  hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_LARGE,
-                     ovf, pc.ctl);
-  GSJ_LAUNCHED();
-  GSJ_HIP(hipEventRecord(c.ev[2], stream));
+  GSJ_HIP(hipEventRecord(c.ev[9], stream));
+  if (sh.fork) {
+    GSJ_HIP(hipEventRecord(c.ev[11], sl));
+    GSJ_HIP(hipStreamWaitEvent(stream, c.ev[11], 0));
+  }
   struct Host { uint32_t counts[8]; OverflowCtl ctl; };
   Host* h = c.h_scal.as<Host>(2);
   GSJ_HIP(hipMemcpyAsync(h->counts, pc.counts, 32, hipMemcpyDeviceToHost, stream));
@@ -593,7 +606,7 @@ void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
   float ms_t = 0, ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
   GSJ_HIP(hipEventElapsedTime(&ms_t, c.ev[0], c.ev[8]));
   GSJ_HIP(hipEventElapsedTime(&ms_s, c.ev[8], c.ev[1]));
-  GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[5]));
+  GSJ_HIP(hipEventElapsedTime(&ms_m, c.ev[1], c.ev[9]));
   GSJ_HIP(hipEventElapsedTime(&ms_x, c.ev[5], c.ev[7]));
   GSJ_HIP(hipEventElapsedTime(&ms_l, c.ev[7], c.ev[2]));
   s.k_tiny_ms += ms_t; s.k_small_ms += ms_s; s.k_medium_ms += ms_m; s.k_mixed_ms += ms_x; s.k_large_ms += ms_l;

@@ -87,6 +87,7 @@ struct Layer {
   bool has_bvh = false;
   std::vector<uint32_t> h_parent;   // host copy: keeps a parent's pieces in one probe batch
   std::vector<float4> h_tiles;      // host copy of the probe tile boxes (scratch estimates of windowed requests)
+  double build_ms = 0;              // device time of the last BVH build (HIP events around its kernels)
   ~Layer();
 };
 
@@ -104,7 +105,8 @@ struct Context {
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg, h_slots_ctl;
   int cu_count = 256;
-  hipEvent_t ev[12] = {};   // kernel timing events (created with the context); ev[10], ev[11]: fork / join of `side`
+  hipEvent_t ev[16] = {};   // kernel timing events (created with the context); ev[10], ev[11]: fork / join of `side`;
+                            // ev[12 .. 15]: the traversal passes and the BVH build
   bool borrowed = false;   // stream and events belong to another context (not destroyed with this one)
   Arena* dev_arena = nullptr;   // set: every scratch buffer comes from this arena (the no-allocation entry point)
   Tuning tune;                  // launch shapes of the pair kernels per launch size

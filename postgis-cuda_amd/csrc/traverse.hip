@@ -281,11 +281,18 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
     uint32_t* hover = c.h_slots_ctl.as<uint32_t>(4);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, nover, (uint64_t)1);
     GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[12], stream));
     traverse_count(c, qbb, qidx, q0, nq, t, upper, counts, slots, nover, box_by_pos);
+    GSJ_HIP(hipEventRecord(c.ev[13], stream));
     GSJ_HIP(hipMemcpyAsync(hover, nover, 4, hipMemcpyDeviceToHost, stream));
     uint64_t* offs = c.s_qoffs.as<uint64_t>((uint64_t)nq + 16);   // probe offsets (s_offs belongs to compaction)
     const uint64_t total = scan_u32_to_u64(c, counts, offs, nq);   // synchronises: *hover is there too
     const bool any_over = *hover != 0;
+    {
+      float ms = 0;
+      GSJ_HIP(hipEventElapsedTime(&ms, c.ev[12], c.ev[13]));
+      s.k_traverse_ms += ms;
+    }
     s.traverse_ms += tt.ms();
     // sub-batches of at most max_pairs candidates (a probe parent's pieces stay together)
     std::vector<std::pair<uint32_t, uint32_t>> subs;
@@ -314,7 +321,9 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
       GSJ_REQUIRE(np < (1ull << 32), GSJ_E_LIMIT, "a sub-batch has >= 2^32 candidates (lower max_pairs)");
       uint32_t* pa = c.s_pairs_a.as<uint32_t>(np + 16);
       uint32_t* pb = c.s_pairs_b.as<uint32_t>(np + 16);
+      GSJ_HIP(hipEventRecord(c.ev[14], stream));
       traverse_write(c, qbb, qidx, q0 + s0, s1 - s0, t, upper, counts + s0, offs, slots + (uint64_t)s0 * SLOT_K, any_over, pa, pb, box_by_pos);
+      GSJ_HIP(hipEventRecord(c.ev[15], stream));
       SegBounds sb;
       if (mstart) {
         hipLaunchKernelGGL(k_seg_bounds, dim3(1), dim3(64), 0, stream, mstart, k, (uint64_t)(q0 + s0), (uint64_t)(s1 - s0), offs, np, dlo);
@@ -324,6 +333,11 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
       }
       GSJ_HIP(hipStreamSynchronize(stream));
       s.traverse_ms += t1.ms();
+      {
+        float ms = 0;
+        GSJ_HIP(hipEventElapsedTime(&ms, c.ev[14], c.ev[15]));
+        s.k_traverse_ms += ms;
+      }
       fn(pa, pb, np, sb);
     }
   }

@@ -43,7 +43,7 @@ class JoinStats(C.Structure):
                 ("n_small", C.c_uint64), ("n_large", C.c_uint64), ("n_over", C.c_uint64), ("launches", C.c_uint64),
                 ("k_medium_ms", C.c_double), ("n_medium", C.c_uint64), ("k_screen_ms", C.c_double), ("n_small_edge", C.c_uint64),
                 ("n_medium_edge", C.c_uint64), ("k_mixed_ms", C.c_double), ("n_mixed", C.c_uint64),
-                ("k_tiny_ms", C.c_double), ("n_tiny", C.c_uint64), ("n_tiny_edge", C.c_uint64)]
+                ("k_tiny_ms", C.c_double), ("n_tiny", C.c_uint64), ("n_tiny_edge", C.c_uint64), ("k_traverse_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -51,7 +51,7 @@ class JoinStats(C.Structure):
 
 class LayerInfo(C.Structure):
     _fields_ = [("n", C.c_uint64), ("nr", C.c_uint64), ("nv", C.c_uint64), ("nparent", C.c_uint32), ("is_points", C.c_int),
-                ("has_bvh", C.c_int), ("device_bytes", C.c_uint64)]
+                ("has_bvh", C.c_int), ("device_bytes", C.c_uint64), ("bvh_ms", C.c_double)]
 
 
 ROWS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t)
