@@ -33,6 +33,12 @@ struct DPolys {
   // vertices (origin-relative, rounded outward to float). cfirst[g] == NO_CHUNKS for smaller geometries.
   const uint32_t* cfirst = nullptr;  // [n]
   const float4* cbb = nullptr;       // [nchunks]
+  // packed records of the pieces the vertex-list classes can take (at most PACK_VCAP stored vertices and PACK_RCAP
+  // rings), in 16-byte units: [origin][extent][header: stored vertices, rings, ring starts][vertices as in xy][one
+  // VF_* byte per vertex, padded], so that a pair kernel reads one contiguous record per piece instead of four
+  // dependent arrays (geometry -> rings -> vertices, flags)
+  const uint4* pk = nullptr;         // [records]
+  const uint32_t* pk_off = nullptr;  // [n] first unit of the piece's record, NO_PACK for the others
 };
 constexpr int CHUNK = 64;
 // per-vertex flags of a subdivided layer (ring-aware, so the pair kernels need no neighbour lookups): VF_SYN edge
@@ -40,6 +46,9 @@ constexpr int CHUNK = 64;
 // all 0 at a ring's last vertex
 constexpr uint8_t VF_SYN = 2, VF_CUT0 = 4, VF_CUT1 = 8;
 constexpr uint32_t NO_CHUNKS = 0xFFFFFFFFu;
+constexpr uint32_t PACK_VCAP = 64, PACK_RCAP = 8, NO_PACK = 0xFFFFFFFFu;
+// 16-byte units of a packed record: origin, extent, header, the vertices, their flag bytes
+__host__ __device__ constexpr uint64_t pack_units(uint64_t nv) { return 3 + nv + (nv + 15) / 16; }
 
 constexpr uint32_t PROBE_TILE = 1024;   // probes per tile box (window membership)
 

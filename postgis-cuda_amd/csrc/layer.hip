@@ -51,6 +51,32 @@ __global__ void k_poly_prep(uint32_t n, const uint32_t* gro, const uint64_t* rvo
   for (uint64_t v = v0; v < v1; v++) { double2 p = xy[v]; xy[v] = make_double2(p.x - x0, p.y - y0); }
 }
 
+// packed record of every piece with a unit offset (after k_poly_prep and the flags): origin, extent, header (stored
+// vertices, rings, ring starts relative to the first vertex), the origin-relative vertices, their VF_* flag bytes
+__global__ void k_pack(uint32_t n, const uint32_t* __restrict__ gro, const uint64_t* __restrict__ rvo, const double2* __restrict__ xy,
+                       const uint8_t* __restrict__ vflag, const double2* __restrict__ origin, const double2* __restrict__ ext,
+                       const uint32_t* __restrict__ pk_off, uint4* __restrict__ pk) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n || pk_off[g] == NO_PACK) return;
+  uint4* rec = pk + pk_off[g];
+  const uint32_t r0 = gro[g], nr = gro[g + 1] - r0;
+  const uint64_t v0 = rvo[r0];
+  const uint32_t nv = (uint32_t)(rvo[r0 + nr] - v0);
+  reinterpret_cast<double2*>(rec)[0] = origin[g];
+  reinterpret_cast<double2*>(rec)[1] = ext[g];
+  uint8_t h[16] = {};
+  h[0] = (uint8_t)nv;
+  h[1] = (uint8_t)nr;
+  for (uint32_t r = 0; r <= nr; r++) h[2 + r] = (uint8_t)(rvo[r0 + r] - v0);
+  uint4 hw;
+  __builtin_memcpy(&hw, h, 16);
+  rec[2] = hw;
+  double2* V = reinterpret_cast<double2*>(rec + 3);
+  for (uint32_t k = 0; k < nv; k++) V[k] = xy[v0 + k];
+  uint8_t* F = reinterpret_cast<uint8_t*>(rec + 3 + nv);
+  for (uint32_t k = 0; k < (nv + 15) / 16 * 16; k++) F[k] = (k < nv && vflag) ? vflag[v0 + k] : 0;
+}
+
 __global__ void k_synb(uint64_t nv, const uint8_t* syn, uint32_t* synb) {
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= (nv + 31) / 32) return;
@@ -155,6 +181,19 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
     }
   }
   GSJ_REQUIRE(nchunks < NO_CHUNKS, GSJ_E_LIMIT, "layer too large: edge chunks >= 2^32");
+  // packed records (see DPolys::pk): offsets in 16-byte units
+  std::vector<uint32_t> hpk(n);
+  uint64_t npk = 0;
+  for (uint64_t g = 0; g < n; g++) {
+    const uint64_t nrg = gro[g + 1] - gro[g], nvg = rvo[gro[g + 1]] - rvo[gro[g]];
+    if (nvg <= PACK_VCAP && nrg <= PACK_RCAP) {
+      hpk[g] = (uint32_t)npk;
+      npk += pack_units(nvg);
+      GSJ_REQUIRE(npk < NO_PACK, GSJ_E_LIMIT, "layer too large: packed records >= 2^32 units of 16 bytes");
+    } else {
+      hpk[g] = NO_PACK;
+    }
+  }
   // sizes (pass 1) and assignment (pass 2)
   auto carve = [&](Slicer& s, DPolys& d, uint32_t*& gro32, uint8_t*& syn8, double*& glen) {
     d.origin = s.take<double2>(n); d.ext = s.take<double2>(n); gro32 = s.take<uint32_t>(n + 1); d.rvo = s.take<uint64_t>(nr + 1);
@@ -167,6 +206,8 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
     glen = parent ? s.take<double>(n) : nullptr;
     d.cfirst = nchunks ? s.take<uint32_t>(n) : nullptr;
     d.cbb = nchunks ? s.take<float4>(nchunks) : nullptr;
+    d.pk_off = s.take<uint32_t>(n);
+    d.pk = s.take<uint4>(npk + 1);
   };
   Slicer s1; DPolys d; uint32_t* gro32; uint8_t* syn8; double* glen;
   carve(s1, d, gro32, syn8, glen);
@@ -183,6 +224,7 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
   if (parent) GSJ_HIP(hipMemcpyAsync((void*)d.parent, parent, n * 4, hipMemcpyHostToDevice, st));
   if (syn) GSJ_HIP(hipMemcpyAsync(syn8, syn, nv, hipMemcpyHostToDevice, st));
   if (nchunks) GSJ_HIP(hipMemcpyAsync((void*)d.cfirst, hcf.data(), n * 4, hipMemcpyHostToDevice, st));
+  if (n) GSJ_HIP(hipMemcpyAsync((void*)d.pk_off, hpk.data(), n * 4, hipMemcpyHostToDevice, st));
   if (n) {
     // without parents the boundary length of a geometry is its parent's
     double* glen_out = parent ? glen : const_cast<double*>(d.blen);
@@ -202,6 +244,9 @@ void layer_upload_polys(Layer& L, const uint64_t* gro, const uint64_t* rvo, cons
       GSJ_LAUNCHED();
       d.vflag = syn8;
     }
+    hipLaunchKernelGGL(k_pack, dim3(div_up(n, 256)), dim3(256), 0, st, (uint32_t)n, gro32, d.rvo, d.xy, d.vflag, d.origin, d.ext, d.pk_off,
+                       const_cast<uint4*>(d.pk));
+    GSJ_LAUNCHED();
     if (parent) {
       hipLaunchKernelGGL(k_pfirst, dim3(div_up(n, 256)), dim3(256), 0, st, (uint32_t)n, nparent, d.parent, const_cast<uint32_t*>(d.pfirst));
       GSJ_LAUNCHED();

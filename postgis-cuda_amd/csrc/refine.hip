@@ -24,6 +24,8 @@
 
 namespace gsj {
 
+static_assert(MCAP <= (int)PACK_VCAP && RCAP <= (int)PACK_RCAP, "every piece of the vertex-list classes has a packed record");
+
 namespace {
 
 // ------------------------------------------------------------------------------------------- small-class source
@@ -301,9 +303,8 @@ struct PairSetup {
   double tol, snap, cx, cy, bshx, bshy, ashx, ashy;
 };
 
-__device__ __forceinline__ PairSetup setup_pair(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, const RefineParams& rp) {
+__device__ __forceinline__ PairSetup setup_pair_of(const double2 oa, const double2 ea, const double2 ob, const double2 eb, const RefineParams& rp) {
   PairSetup s;
-  const double2 oa = P.origin[a], ob = R.origin[b], ea = P.ext[a], eb = R.ext[b];
   const double sx = ob.x - oa.x, sy = ob.y - oa.y;
   const double pad = 4.0 * fmax(rp.tol_abs, rp.tol_rel * fmax(fmax(ea.x, ea.y), fmax(eb.x, eb.y)));
   s.ov = Box{fmax(0.0, sx) - pad, fmax(0.0, sy) - pad, fmin(ea.x, sx + eb.x) + pad, fmin(ea.y, sy + eb.y) + pad};
@@ -314,6 +315,15 @@ __device__ __forceinline__ PairSetup setup_pair(const DPolys& P, const DPolys& R
   s.cx = s.ov.x0 + 0.381966 * (s.ov.x1 - s.ov.x0);
   s.cy = s.ov.y0 + 0.414213 * (s.ov.y1 - s.ov.y0);
   return s;
+}
+__device__ __forceinline__ PairSetup setup_pair(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, const RefineParams& rp) {
+  return setup_pair_of(P.origin[a], P.ext[a], R.origin[b], R.ext[b], rp);
+}
+// the vertex-list classes: origin and extent from the pieces' packed records
+__device__ __forceinline__ PairSetup setup_pair_rec(const uint4* ra, const uint4* rb, const RefineParams& rp) {
+  const double2* a = reinterpret_cast<const double2*>(ra);
+  const double2* b = reinterpret_cast<const double2*>(rb);
+  return setup_pair_of(a[0], a[1], b[0], b[1], rp);
 }
 
 // group epilogue: deterministic sums of the per-edge partials (stored at acc/lin/lon[0 .. n)), lane 0 writes
@@ -354,14 +364,17 @@ struct SmallPoly {        // a polygon of at most CAP stored vertices (<= 255) a
 template <int CAP>
 constexpr size_t small_bytes(bool rel) { return ((2 * sizeof(SmallPoly<CAP>) + 15) & ~size_t(15)) + (size_t)2 * CAP * 8 * (rel ? 3 : 1); }
 
-// all vertices of geometry g into LDS with per-vertex flags (reference edge collection, engine/engine.cuh:358-377)
+// all vertices of a piece into LDS with per-vertex flags (reference edge collection, engine/engine.cuh:358-377), read
+// from its packed record (DPolys::pk): the header's counts and ring starts, the vertices, their flag bytes
 template <int G, int CAP>
-__device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, SmallPoly<CAP>& sp, int gl) {
-  const uint32_t r0 = S.gro[g], nrings = S.gro[g + 1] - r0;
-  const uint64_t v0 = S.rvo[r0];
-  const int nv = (int)(S.rvo[r0 + nrings] - v0);
-  for (int k = gl; k < nv; k += G) { const double2 p = S.xy[v0 + k]; sp.V[k] = make_double2(shx + p.x, shy + p.y); }
-  for (int r = gl; r <= (int)nrings; r += G) sp.rs[r] = (uint8_t)(S.rvo[r0 + r] - v0);
+__device__ void load_small(const DPolys& S, uint32_t off, double shx, double shy, const Box& ov, SmallPoly<CAP>& sp, int gl) {
+  const uint4* rec = S.pk + off;
+  const uint8_t* hb = reinterpret_cast<const uint8_t*>(rec + 2);
+  const int nv = hb[0], nrings = hb[1];
+  const double2* RV = reinterpret_cast<const double2*>(rec + 3);
+  const uint8_t* RF = reinterpret_cast<const uint8_t*>(rec + 3 + nv);
+  for (int k = gl; k < nv; k += G) { const double2 p = RV[k]; sp.V[k] = make_double2(shx + p.x, shy + p.y); }
+  for (int r = gl; r <= nrings; r += G) sp.rs[r] = hb[2 + r];
   if (gl == 0) { sp.nv = (uint8_t)nv; sp.nr = (uint8_t)nrings; }
   wave_lds_sync();
   for (int k = gl; k < nv; k += G) {
@@ -370,8 +383,7 @@ __device__ void load_small(const DPolys& S, uint32_t g, double shx, double shy, 
     const int e = sp.rs[r + 1];
     uint8_t f = 0;
     if (k + 1 < e) {
-      f = F_VALID;
-      if (S.vflag) f |= S.vflag[v0 + k];   // VF_SYN, VF_CUT0, VF_CUT1
+      f = F_VALID | RF[k];   // VF_SYN, VF_CUT0, VF_CUT1 (0 without subdivision)
       const double2 a = sp.V[k], b = sp.V[k + 1];
       if (!edge_outside(a.x, a.y, b.x, b.y, ov)) f |= F_INOV;
     }
@@ -441,10 +453,10 @@ struct EdgeQueue {   // pairs with edge work: flag[k] per position k of the clas
 };
 
 template <int G, int CAP>
-__device__ __forceinline__ void load_pair(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, const PairSetup& f, SmallPoly<CAP>& SA,
+__device__ __forceinline__ void load_pair(const DPolys& P, const DPolys& R, uint32_t ra, uint32_t rb, const PairSetup& f, SmallPoly<CAP>& SA,
                                           SmallPoly<CAP>& SB, int gl) {
-  load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
-  load_small<G, CAP>(R, b, f.bshx, f.bshy, f.ov, SB, gl);
+  load_small<G, CAP>(P, ra, f.ashx, f.ashy, f.ov, SA, gl);
+  load_small<G, CAP>(R, rb, f.bshx, f.bshy, f.ov, SB, gl);
 }
 
 template <int G, int CAP, int WHICH, int OCC>
@@ -465,10 +477,11 @@ __global__ void __launch_bounds__(256, OCC) k_screen_small(const DPolys P, const
   for (; ps.k < nlist; ps.advance()) {
     ps.prefetch();
     const uint32_t k = ps.k, p = ps.p, a = ps.a, b = ps.b;
-    const PairSetup f = setup_pair(P, R, a, b, rp);
+    const uint32_t ra = P.pk_off[a], rb = R.pk_off[b];   // record offsets (32-bit: fewer registers than pointers)
+    const PairSetup f = setup_pair_rec(P.pk + ra, R.pk + rb, rp);
     const bool self = is_self(P, R, a, b);
-    if (self) load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
-    else load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
+    if (self) load_small<G, CAP>(P, ra, f.ashx, f.ashy, f.ov, SA, gl);
+    else load_pair<G, CAP>(P, R, ra, rb, f, SA, SB, gl);
     const SP& SBx = self ? SA : SB;
     const int na = SA.ni, nb = SBx.ni;   // edges meeting the overlap box
     const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SBx.V, SBx.F, SBx.I, nb};
@@ -549,10 +562,11 @@ __global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DP
     ps.prefetch();
     const uint32_t p = ps.p, a = ps.a, b = ps.b;
     const uint8_t st = out.isect[p];
-    const PairSetup f = setup_pair(P, R, a, b, rp);
+    const uint32_t ra = P.pk_off[a], rb = R.pk_off[b];   // record offsets (32-bit: fewer registers than pointers)
+    const PairSetup f = setup_pair_rec(P.pk + ra, R.pk + rb, rp);
     const bool self = is_self(P, R, a, b);
-    if (self) load_small<G, CAP>(P, a, f.ashx, f.ashy, f.ov, SA, gl);
-    else load_pair<G, CAP>(P, R, a, b, f, SA, SB, gl);
+    if (self) load_small<G, CAP>(P, ra, f.ashx, f.ashy, f.ov, SA, gl);
+    else load_pair<G, CAP>(P, R, ra, rb, f, SA, SB, gl);
     const SP& SBx = self ? SA : SB;
     const int na = SA.ni, nb = SBx.ni;
     const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SBx.V, SBx.F, SBx.I, nb};
