@@ -132,8 +132,9 @@ def config1_leg(gsj, ctx, args):
         r = ctx.join(LP, LR, want_area=True, collect=False)
     wall = time.perf_counter() - t0
     st = r.stats
-    kname, kfield, kcount = max((("k_refine_mixed", "k_mixed_ms", "n_mixed"), ("k_refine_large", "k_large_ms", "n_large"),
-                                 ("k_screen_small", "k_screen_ms", "n_small")), key=lambda k: st[k[1]])
+    # the longest kernel among the classes that had pairs (an empty class's launches still take a few microseconds)
+    kname, kfield, kcount = max(((k, f, n) for k, f, n in (("k_refine_mixed", "k_mixed_ms", "n_mixed"), ("k_refine_large", "k_large_ms", "n_large"),
+                                                           ("k_screen_small", "k_screen_ms", "n_small")) if st[n] > 0), key=lambda k: st[k[1]])
     OP, OR = O.Layer.polys(P), O.Layer.polys(R)
     cand = O.candidates(OP, OR)
     cls = pair_class2(P, R, cand)
