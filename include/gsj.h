@@ -57,7 +57,9 @@ int gsj_ctx_trim(gsj_ctx* ctx);
  * "bucket np>=N key=value ..." with keys tcap (0..16), mcap (0 or 33..64), screen_occ (4, 5), screen_bpc, edge_bpc,
  * mixed_bpc, large_bpc, dw_bpc (blocks per CU, 1..64), trav_block (64, 128, 256) and ovf_lds (1: overflow pairs
  * with lists of at most 448 edges per side keep them in LDS; 0: in global memory) and fork (1: the mixed and large
- * classes run on a second stream of the context beside the other classes; 0: in line); buckets ascend from np>=0.
+ * classes run on a second stream of the context beside the other classes; 0: in line), mixed (0: no mixed class)
+ * and large_cap (0..96: the large class's list capacity; 0 sends every large pair to the overflow kernel); buckets
+ * ascend from np>=0.
  * Replaces the reference's fixed launch shapes (engine/engine.cuh:945-951, engine/buffer.cuh:946-947); every shape
  * gives the same rows. set: GSJ_E_ARG on malformed text (the table is unchanged); applies to later calls and to
  * schedulers created afterwards. get: the table in effect; the string is owned by the context, valid until the next

@@ -909,7 +909,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
   const LaunchShape& sh = c.shape(np);
-  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, sh.mcap, true, sh.tcap);
+  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, sh.mcap, sh.mixed != 0, sh.tcap);
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
@@ -945,11 +945,13 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   // large: one wave per pair, 4 pairs per block, 96-edge lists (3 blocks per CU fit in LDS; 128 allowed 2); the pairs
   // that need more go to the overflow kernel (2.4M relate: large 6.1 -> 3.6 ms, overflow 2.7 -> 4.3 ms; a second LDS
   // tier of 128-edge lists for those was slower than the overflow kernel)
-  constexpr int CAP_LARGE = 96;
+  // large_cap 0 (launches of a few pairs): every large pair goes to the overflow kernel, whose blocks of eight waves
+  // per pair finish a few big pairs sooner than one wave each
+  const int cap_large = (int)sh.large_cap;
   {
-    const size_t sm = (size_t)(T / 64) * list_bytes(CAP_LARGE, rel);
+    const size_t sm = (size_t)(T / 64) * list_bytes(std::max(cap_large, 1), rel);
     const uint32_t g = std::min<uint32_t>(div_up(np, T / 64), cu * sh.large_bpc);
-    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, sl, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, CAP_LARGE, ovf, ctl);
+    hipLaunchKernelGGL(k_refine_large, dim3(g), dim3(T), sm, sl, P, R, pa, pb, list, counts, promo, promo_ctl, rp, out, cap_large, ovf, ctl);
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], sl));
