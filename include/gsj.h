@@ -58,8 +58,9 @@ int gsj_ctx_trim(gsj_ctx* ctx);
  * mixed_bpc, large_bpc, dw_bpc (blocks per CU, 1..64), trav_block (64, 128, 256) and ovf_lds (1: overflow pairs
  * with lists of at most 448 edges per side keep them in LDS; 0: in global memory) and fork (1: the mixed and large
  * classes run on a second stream of the context beside the other classes; 0: in line), mixed (0: no mixed class)
- * and large_cap (0..96: the large class's list capacity; 0 sends every large pair to the overflow kernel); buckets
- * ascend from np>=0.
+ * large_cap (0..96: the large class's list capacity; 0 sends every large pair to the overflow kernel) and size_sort
+ * and dw_size_sort (join and dwithin, 0..4: v > 0 puts pairs of similar size side by side within a class, in size
+ * buckets 2^(v-1) times the class's base width; 0 keeps candidate order); buckets ascend from np>=0.
  * Replaces the reference's fixed launch shapes (engine/engine.cuh:945-951, engine/buffer.cuh:946-947); every shape
  * gives the same rows. set: GSJ_E_ARG on malformed text (the table is unchanged); applies to later calls and to
  * schedulers created afterwards. get: the table in effect; the string is owned by the context, valid until the next

@@ -561,7 +561,7 @@ void dw_deliver(Context& c, const uint32_t* orow, const double* odist, uint64_t 
 void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
   hipStream_t stream = c.stream;
   const LaunchShape& sh = c.shape(np);
-  const PairClasses pc = partition_pairs(c, A.P, A.R, A.pa, A.pb, np, sh.mcap, true, sh.tcap);
+  const PairClasses pc = partition_pairs(c, A.P, A.R, A.pa, A.pb, np, sh.mcap, true, sh.tcap, sh.dw_size_sort);
   uint32_t* ovf = c.s_heavy.as<uint32_t>(np + 64);
   const uint32_t max_blocks = (uint32_t)c.cu_count * sh.dw_bpc;
   // the classes are independent (each appends its overflowing pairs to the queue with atomics): with sh.fork the

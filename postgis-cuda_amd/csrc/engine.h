@@ -142,9 +142,10 @@ uint64_t scan_u32_to_u64(Context& c, const uint32_t* in, uint64_t* out, uint64_t
 // exclusive prefix sum without host sync: total written to *d_total
 void scan_u32_to_u64_async(Context& c, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* d_total);
 // stable LSD radix sort of (key, value) by key; only the digits in [bit0, bit1) are sorted; returns pointers to the
-// sorted arrays (either the inputs or the alternates)
+// sorted arrays (either the inputs or the alternates). probe: first read back which bits vary (a host synchronisation)
+// and skip the digits that do not; without it every digit in the range is sorted and nothing synchronises
 void radix_sort_pairs(Context& c, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint64_t n, int bit0,
-                      int bit1);
+                      int bit1, bool probe = true);
 // OR and AND of all keys (for choosing digits); synchronises
 void key_bits(Context& c, const uint64_t* keys, uint64_t n, uint64_t& or_all, uint64_t& and_all);
 

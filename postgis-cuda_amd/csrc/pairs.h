@@ -208,8 +208,10 @@ struct PairClasses {
   uint32_t* counts;
   OverflowCtl* ctl;
 };
+// size_sort: within a class, pairs of similar size side by side, in buckets 2^(size_sort-1) times the class's base
+// width (0: candidate order, a probe's pairs together)
 PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap,
-                            bool mixed, uint32_t tcap);
+                            bool mixed, uint32_t tcap, uint32_t size_sort = 1);
 // first list position of class cl
 __host__ __device__ __forceinline__ uint32_t class_first(const uint32_t* counts, int cl) {
   uint32_t f = 0;

@@ -221,12 +221,16 @@ void key_bits(Context& c, const uint64_t* keys, uint64_t n, uint64_t& or_all, ui
   or_all = h[0]; and_all = n ? h[1] : 0;
 }
 
-void radix_sort_pairs(Context& c, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint64_t n, int bit0, int bit1) {
+void radix_sort_pairs(Context& c, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint64_t n, int bit0, int bit1,
+                      bool probe) {
   if (n <= 1) return;
   GSJ_REQUIRE(n < (1ull << 32), GSJ_E_LIMIT, "radix sort: more than 2^32 elements");
-  uint64_t o, a;
-  key_bits(c, keys, n, o, a);
-  const uint64_t varying = o ^ a;
+  uint64_t varying = ~0ull;
+  if (probe) {
+    uint64_t o, a;
+    key_bits(c, keys, n, o, a);
+    varying = o ^ a;
+  }
   const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
   uint32_t* ghist = c.s_tmp2.as<uint32_t>((size_t)256 * ntiles + 8);
   for (int shift = bit0 & ~7; shift < bit1; shift += 8) {

@@ -909,7 +909,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   hipStream_t st = c.stream;
   const bool rel = rp.want_relate != 0;
   const LaunchShape& sh = c.shape(np);
-  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, sh.mcap, sh.mixed != 0, sh.tcap);
+  const PairClasses pc = partition_pairs(c, P, R, pa, pb, np, sh.mcap, sh.mixed != 0, sh.tcap, sh.size_sort);
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
@@ -1058,7 +1058,7 @@ namespace {
 // RCAP; mixed (when enabled: none of those, one polygon within VCAP stored vertices); large — and the mean stored vertex
 // count (saturated at 63) below, so that the groups of a wave get pairs of similar work
 __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
-                             uint32_t mcap, int mixed, uint32_t tcap, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+                             uint32_t mcap, int mixed, uint32_t tcap, uint32_t size_sort, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t ia = P.ginfo[pa[i]], ib = R.ginfo[pb[i]];
@@ -1068,10 +1068,11 @@ __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __r
                        : (rings && va <= VCAP && vb <= VCAP) ? CL_SMALL
                        : (rings && va <= mcap && vb <= mcap) ? CL_MEDIUM
                        : (mixed && min(va, vb) <= (uint32_t)VCAP) ? CL_MIXED : CL_LARGE;
-  // within a class, pairs of similar size side by side (balanced groups), at a resolution per class: the large class
-  // spans hundreds of vertices, so its buckets are 8 vertices wide (1 for tiny, 2 for small and mixed, 4 for medium)
-  const uint32_t shift = cls == CL_LARGE ? 3u : (cls == CL_MEDIUM ? 2u : (cls == CL_TINY ? 0u : 1u));
-  key[i] = (uint64_t)((cls << 6) | min(63u, (va + vb) >> shift));
+  // within a class, pairs of similar size side by side (balanced groups), at a base resolution per class: the large
+  // class spans hundreds of vertices, so its buckets are 8 vertices wide (1 for tiny, 2 for small and mixed, 4 for
+  // medium); size_sort > 1 widens them by powers of two (fewer, longer runs of candidate order)
+  const uint32_t shift = (cls == CL_LARGE ? 3u : (cls == CL_MEDIUM ? 2u : (cls == CL_TINY ? 0u : 1u))) + size_sort - 1;
+  key[i] = (uint64_t)((cls << 6) | (size_sort ? min(63u, (va + vb) >> shift) : 0u));
   val[i] = (uint32_t)i;
 }
 
@@ -1101,7 +1102,7 @@ __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
 }  // namespace
 
 PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pa, const uint32_t* pb, uint64_t np, uint32_t mcap,
-                            bool mixed, uint32_t tcap) {
+                            bool mixed, uint32_t tcap, uint32_t size_sort) {
   hipStream_t st = c.stream;
   char* misc = c.s_misc.as<char>(256);   // [0, 32) belongs to the scan / sort primitives
   PairClasses pc;
@@ -1115,9 +1116,10 @@ PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const 
   uint64_t* keys2 = c.s_keys2.as<uint64_t>(np + 16);
   uint32_t* vals = c.s_vals.as<uint32_t>(np + 16);
   uint32_t* vals2 = c.s_vals2.as<uint32_t>(np + 16);
-  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, mixed ? 1 : 0, tcap, keys, vals);
+  hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, mixed ? 1 : 0, tcap, size_sort, keys, vals);
   GSJ_LAUNCHED();
-  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 9);   // stable: candidate order within a key
+  // stable (candidate order within a key); both digits sorted without probing which vary: no host synchronisation
+  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 9, false);
   hipLaunchKernelGGL(k_class_counts, dim3(1), dim3(1), 0, st, keys, np, pc.counts);
   GSJ_LAUNCHED();
   pc.list = vals;

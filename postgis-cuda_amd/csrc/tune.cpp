@@ -25,7 +25,8 @@ const Key kKeys[] = {
     {"large_bpc", &LaunchShape::large_bpc, 1, 64}, {"dw_bpc", &LaunchShape::dw_bpc, 1, 64},
     {"trav_block", &LaunchShape::trav_block, 64, 256}, {"ovf_lds", &LaunchShape::ovf_lds, 0, 1},
     {"fork", &LaunchShape::fork, 0, 1},           {"mixed", &LaunchShape::mixed, 0, 1},
-    {"large_cap", &LaunchShape::large_cap, 0, 96},
+    {"large_cap", &LaunchShape::large_cap, 0, 96},  {"size_sort", &LaunchShape::size_sort, 0, 4},
+    {"dw_size_sort", &LaunchShape::dw_size_sort, 0, 4},
 };
 
 [[noreturn]] void bad(const std::string& source, int line, const std::string& what) {

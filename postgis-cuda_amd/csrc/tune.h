@@ -24,6 +24,8 @@ struct LaunchShape {
   uint32_t fork = 1;          // mixed and large classes on a second stream beside the vertex-list classes (0: in line)
   uint32_t mixed = 1;         // mixed class (0: its pairs are large-class pairs)
   uint32_t large_cap = 96;    // large-class LDS list capacity (0: every large pair goes to the overflow kernel)
+  uint32_t size_sort = 1;     // join: within a class, pairs ordered by size in buckets 2^(v-1) times the base width (0: candidate order)
+  uint32_t dw_size_sort = 0;  // the same for dwithin
 };
 
 struct Tuning {

@@ -12,7 +12,7 @@ from conftest import sorted_rows
 
 pytestmark = pytest.mark.gpu
 
-EXTREME = ("bucket np>=0 tcap=0 mcap=0 screen_occ=4 screen_bpc=1 edge_bpc=1 mixed_bpc=1 large_bpc=1 dw_bpc=1 trav_block=64 ovf_lds=0 fork=0 mixed=0 large_cap=0\n")
+EXTREME = ("bucket np>=0 tcap=0 mcap=0 screen_occ=4 screen_bpc=1 edge_bpc=1 mixed_bpc=1 large_bpc=1 dw_bpc=1 trav_block=64 ovf_lds=0 fork=0 mixed=0 large_cap=0 size_sort=0 dw_size_sort=1\n")
 SWITCHING = ("bucket np>=0 tcap=8 mcap=48 screen_occ=4 screen_bpc=2 edge_bpc=64 trav_block=128\n"
              "bucket np>=2000 tcap=12 mcap=0 screen_occ=5 screen_bpc=64 edge_bpc=3 mixed_bpc=5 large_bpc=7 dw_bpc=2 large_cap=64 mixed=0\n"
              "bucket np>=20000 tcap=16 mcap=64\n")

@@ -61,9 +61,9 @@ bucket np>=32768 mcap=0 screen_occ=4
     lines = out.splitlines()
     assert lines[0] == "# source: stdin"
     assert lines[1] == ("bucket np>=0 tcap=0 mcap=64 screen_occ=5 screen_bpc=16 edge_bpc=4 mixed_bpc=32 large_bpc=16 dw_bpc=16 "
-                        "trav_block=256 ovf_lds=1 fork=1 mixed=1 large_cap=96")
+                        "trav_block=256 ovf_lds=1 fork=1 mixed=1 large_cap=96 size_sort=1 dw_size_sort=0")
     assert lines[2].startswith("bucket np>=32768 tcap=16 mcap=0 screen_occ=4 ")
-    assert lines[3].endswith("edge_bpc=32 mixed_bpc=32 large_bpc=16 dw_bpc=16 trav_block=128 ovf_lds=1 fork=1 mixed=1 large_cap=96")
+    assert lines[3].endswith("edge_bpc=32 mixed_bpc=32 large_bpc=16 dw_bpc=16 trav_block=128 ovf_lds=1 fork=1 mixed=1 large_cap=96 size_sort=1 dw_size_sort=0")
     picks = [l for l in lines if l.startswith("pick")]
     assert picks == ["pick 0 np>=0 tcap=0 edge_bpc=4", "pick 32767 np>=0 tcap=0 edge_bpc=4", "pick 32768 np>=32768 tcap=16 edge_bpc=16",
                      "pick 2097151 np>=32768 tcap=16 edge_bpc=16", "pick 2097152 np>=2097152 tcap=16 edge_bpc=32",
@@ -75,7 +75,7 @@ bucket np>=32768 mcap=0 screen_occ=4
     # the built-in table: one bucket of the shapes the kernels were written for
     assert [l for l in lines if l.startswith("default")] == ["default # source: built-in"]
     assert lines[-1] == ("bucket np>=0 tcap=16 mcap=64 screen_occ=5 screen_bpc=16 edge_bpc=16 mixed_bpc=32 large_bpc=16 dw_bpc=16 trav_block=256 "
-                         "ovf_lds=1 fork=1 mixed=1 large_cap=96")
+                         "ovf_lds=1 fork=1 mixed=1 large_cap=96 size_sort=1 dw_size_sort=0")
 
 
 @pytest.mark.parametrize("text,msg", [
