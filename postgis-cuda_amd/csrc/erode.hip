@@ -128,7 +128,7 @@ __device__ bool pip_parent(const DPolys& S, const ESide& s, const EFrame& F, dou
     bool par = false;
     // runs whose chunk lies strictly above or below the row y = cy in the frame cross nothing
     auto skip = [&](const float4 c) { return (shy + (double)c.y) * F.ky > cy || (shy + (double)c.w) * F.ky < cy; };
-    walk_edge_runs(S, g, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+    walk_edge_runs<G>(S, g, lane, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
       for (uint64_t i = i0 + lane; i < i1; i += G) {
         double x0, y0, x1, y1;
         fv(S, shx, shy, i, F, x0, y0);
@@ -158,7 +158,7 @@ __device__ int collect_side(const DPolys& S, const ESide& s, const EFrame& F, do
     auto skip = [&](const float4 c) {
       return (shx + (double)c.z) * F.kx < cx0 || (shx + (double)c.x) * F.kx > cx1 || (shy + (double)c.w) * F.ky < cy0 || (shy + (double)c.y) * F.ky > cy1;
     };
-    walk_edge_runs(S, g, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+    walk_edge_runs<G>(S, g, lane, skip, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
       for (uint64_t base = i0; base < i1; base += G) {
         const uint64_t i = base + lane;
         bool v = false;

@@ -74,12 +74,20 @@ __device__ __forceinline__ bool chunk_misses(const float4 c, double shx, double 
 __device__ __forceinline__ bool chunk_off_row(const float4 c, double shy, double my) { return shy + (double)c.y > my || shy + (double)c.w < my; }
 
 // calls run(i0, i1) for the runs of edges i in [i0, i1) of geometry g, in ring and edge order; skip(chunk box) decides
-// whether a chunked run can be skipped (group-uniform arguments, so the walk is group-uniform)
-template <class Skip, class Run>
-__device__ __forceinline__ void walk_edge_runs(const DPolys& S, uint32_t g, const Skip& skip, const Run& run) {
+// whether a chunked run can be skipped. A G-lane group walks together (group-uniform arguments): the skip decisions
+// of 64 chunks at a time are taken across its lanes (lane l: chunks w + l, w + l + G, ...; their box loads in flight
+// together) and read back from the ballot as the walk reaches them, so a polygon of thousands of edges costs a few
+// rounds of box loads instead of one load latency per chunk. The runs and their order are those of a chunk-by-chunk
+// walk.
+template <int G, class Skip, class Run>
+__device__ __forceinline__ void walk_edge_runs(const DPolys& S, uint32_t g, int gl, const Skip& skip, const Run& run) {
   const uint32_t r0 = S.gro[g], r1 = S.gro[g + 1];
   const uint64_t v0 = S.rvo[r0];
   const uint32_t cf = S.cfirst ? S.cfirst[g] : NO_CHUNKS;
+  // chunks of the geometry (as the layer builds them: chunk k exists while v0 + CHUNK k + 1 < the geometry's end)
+  const uint64_t nch = cf != NO_CHUNKS ? (S.rvo[r1] - v0 - 1 + CHUNK - 1) / CHUNK : 0;
+  uint64_t wb = 0, keep = 0;   // decisions of chunks [wb, wb + 64): bit j set = chunk wb + j is walked
+  bool have = false;
   for (uint32_t r = r0; r < r1; r++) {
     const uint64_t s = S.rvo[r], e = S.rvo[r + 1];
     for (uint64_t i0 = s; i0 + 1 < e;) {
@@ -87,7 +95,26 @@ __device__ __forceinline__ void walk_edge_runs(const DPolys& S, uint32_t g, cons
       if (cf != NO_CHUNKS) {
         const uint64_t k = (i0 - v0) / CHUNK;
         i1 = min(i1, v0 + (k + 1) * CHUNK);
-        if (skip(S.cbb[cf + k])) { i0 = i1; continue; }
+        if (!have || k >= wb + 64) {   // the walk's chunk index only grows
+          wb = k;
+          have = true;
+          keep = 0;
+#pragma unroll
+          for (int j = 0; j < 64 / G; j++) {
+            const uint64_t c = wb + (uint64_t)(j * G + gl);
+            const bool walk = c < nch && !skip(S.cbb[cf + c]);
+            keep |= gballot<G>(walk) << (j * G);
+          }
+        }
+        if (!((keep >> (k - wb)) & 1)) {
+          // straight to the next walked chunk of the window (or the window's end), or the ring's end: the chunks in
+          // between are skipped ones
+          const uint32_t o = (uint32_t)(k - wb) + 1;
+          const uint64_t rest = o < 64 ? keep >> o : 0;
+          const uint64_t kn = rest ? k + 1 + (uint64_t)__builtin_ctzll(rest) : wb + 64;
+          i0 = min(e - 1, v0 + kn * CHUNK);
+          continue;
+        }
       }
       run(s, e, i0, i1);
       i0 = i1;
@@ -101,7 +128,7 @@ __device__ __forceinline__ void walk_edge_runs(const DPolys& S, uint32_t g, cons
 template <int G>
 __device__ int collect(const DPolys& S, uint32_t g, double shx, double shy, const Box& ov, Ed* out, int cap, int gl) {
   int cnt = 0;
-  walk_edge_runs(S, g, [&](const float4 c) { return chunk_misses(c, shx, shy, ov); }, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+  walk_edge_runs<G>(S, g, gl, [&](const float4 c) { return chunk_misses(c, shx, shy, ov); }, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
     for (uint64_t base = i0; base < i1; base += G) {
       const uint64_t i = base + gl;
       bool v = false;
@@ -128,7 +155,7 @@ template <int G>
 __device__ bool pip_full(const DPolys& S, uint32_t g, double shx, double shy, double mx, double my, int gl) {
   const double ox = shx - 1.0, oy = my;
   bool par = false;
-  walk_edge_runs(S, g, [&](const float4 c) { return chunk_off_row(c, shy, my); }, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
+  walk_edge_runs<G>(S, g, gl, [&](const float4 c) { return chunk_off_row(c, shy, my); }, [&](uint64_t, uint64_t, uint64_t i0, uint64_t i1) {
     for (uint64_t i = i0 + gl; i < i1; i += G) {
       const double2 p = S.xy[i], q = S.xy[i + 1];
       const double x0 = shx + p.x, y0 = shy + p.y, x1 = shx + q.x, y1 = shy + q.y;
