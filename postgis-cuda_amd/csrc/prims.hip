@@ -1,6 +1,7 @@
 // Device primitives written for CDNA (wave64): exclusive scan and a stable LSD radix sort of (u64 key, u32 value).
 //
-// Scan: reduce-then-scan over 2048-element tiles (256 threads x 8 items, one u64 running sum per thread).
+// Scan: reduce-then-scan over 2048-element tiles (256 threads x 8 items, one u64 running sum per thread); inputs of
+// up to 16384 elements are scanned by one block in one launch (a request's small scans cost one launch, not three).
 // Radix sort: 8-bit digits, 4096-element tiles (256 threads x 16 items). Per pass one histogram kernel, one scan of
 // the digit-major histogram, one scatter kernel. Ranks inside a tile come from a wave-level "match" (8 ballots give
 // the mask of lanes holding the same digit) plus per-wave digit counts in LDS, so each element's destination is
@@ -92,11 +93,51 @@ __global__ void __launch_bounds__(SCAN_T) k_tile_scan(const TI* in, TO* out, uin
   }
 }
 
+// one block: exclusive scan of n <= SS_MAX elements (thread t holds elements [SS_ITEMS t, SS_ITEMS (t + 1))); every
+// element is read before the barrier and written after it, so in and out may be the same array
+constexpr int SS_T = 1024, SS_ITEMS = 16, SS_MAX = SS_T * SS_ITEMS;
+template <class TI, class TO>
+__global__ void __launch_bounds__(SS_T) k_scan_small(const TI* in, TO* out, uint64_t n, uint64_t* total) {
+  __shared__ uint64_t wsum[SS_T / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)threadIdx.x * SS_ITEMS;
+  uint64_t v[SS_ITEMS];
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SS_ITEMS; k++) {
+    v[k] = base + k < n ? (uint64_t)in[base + k] : 0;
+    s += v[k];
+  }
+  const uint64_t inc = wave_incl_scan(s);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint64_t ex = inc - s, tot = 0;
+#pragma unroll
+  for (int k = 0; k < SS_T / 64; k++) {
+    if (k < w) ex += wsum[k];
+    tot += wsum[k];
+  }
+#pragma unroll
+  for (int k = 0; k < SS_ITEMS; k++) {
+    if (base + k < n) out[base + k] = (TO)ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
 // tile sums for the contiguous-per-thread layout used by k_tile_scan (same tile boundaries, any order inside)
 template <class TI>
 void scan_impl(Context& c, const TI* in, void* out, bool out64, uint64_t n, uint64_t* d_total) {
   if (n == 0) {
     if (d_total) GSJ_HIP(hipMemsetAsync(d_total, 0, 8, c.stream));
+    return;
+  }
+  if (n <= (uint64_t)SS_MAX) {
+    if (out64)
+      hipLaunchKernelGGL((k_scan_small<TI, uint64_t>), dim3(1), dim3(SS_T), 0, c.stream, in, (uint64_t*)out, n, d_total);
+    else
+      hipLaunchKernelGGL((k_scan_small<TI, uint32_t>), dim3(1), dim3(SS_T), 0, c.stream, in, (uint32_t*)out, n, d_total);
+    GSJ_LAUNCHED();
     return;
   }
   const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;

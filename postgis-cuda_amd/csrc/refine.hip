@@ -1053,10 +1053,12 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
 }
 
 namespace {
-// sort key per candidate: class (PairClass, bits 6-8) — tiny (both polygons within tcap stored vertices; none when
+constexpr int CLASS_BIT = 5;   // partition key: class above, size bucket below
+
+// sort key per candidate, 8 bits (one radix pass): class (PairClass, bits 5-7) — tiny (both polygons within tcap stored vertices; none when
 // tcap is 0), small (both within VCAP), medium (both within mcap; none when mcap is 0), all three with rings within
-// RCAP; mixed (when enabled: none of those, one polygon within VCAP stored vertices); large — and the mean stored vertex
-// count (saturated at 63) below, so that the groups of a wave get pairs of similar work
+// RCAP; mixed (when enabled: none of those, one polygon within VCAP stored vertices); large — and the pair's stored vertex
+// count in buckets (saturated at 31) below, so that the groups of a wave get pairs of similar work
 __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb, uint64_t np,
                              uint32_t mcap, int mixed, uint32_t tcap, uint32_t size_sort, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1072,7 +1074,7 @@ __global__ void k_class_keys(const DPolys P, const DPolys R, const uint32_t* __r
   // class spans hundreds of vertices, so its buckets are 8 vertices wide (1 for tiny, 2 for small and mixed, 4 for
   // medium); size_sort > 1 widens them by powers of two (fewer, longer runs of candidate order)
   const uint32_t shift = (cls == CL_LARGE ? 3u : (cls == CL_MEDIUM ? 2u : (cls == CL_TINY ? 0u : 1u))) + size_sort - 1;
-  key[i] = (uint64_t)((cls << 6) | (size_sort ? min(63u, (va + vb) >> shift) : 0u));
+  key[i] = (uint64_t)((cls << CLASS_BIT) | (size_sort ? min(31u, (va + vb) >> shift) : 0u));
   val[i] = (uint32_t)i;
 }
 
@@ -1085,7 +1087,7 @@ __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, ui
     uint64_t lo = 0, hi = np;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) / 2;
-      if ((key[mid] >> 6) < (uint64_t)c) lo = mid + 1;
+      if ((key[mid] >> CLASS_BIT) < (uint64_t)c) lo = mid + 1;
       else hi = mid;
     }
     first[c] = lo;
@@ -1118,8 +1120,9 @@ PairClasses partition_pairs(Context& c, const DPolys& P, const DPolys& R, const 
   uint32_t* vals2 = c.s_vals2.as<uint32_t>(np + 16);
   hipLaunchKernelGGL(k_class_keys, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, np, mcap, mixed ? 1 : 0, tcap, size_sort, keys, vals);
   GSJ_LAUNCHED();
-  // stable (candidate order within a key); both digits sorted without probing which vary: no host synchronisation
-  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 9, false);
+  // stable (candidate order within a key); one 8-bit digit, sorted without probing whether it varies: no host
+  // synchronisation
+  radix_sort_pairs(c, keys, keys2, vals, vals2, np, 0, 8, false);
   hipLaunchKernelGGL(k_class_counts, dim3(1), dim3(1), 0, st, keys, np, pc.counts);
   GSJ_LAUNCHED();
   pc.list = vals;
