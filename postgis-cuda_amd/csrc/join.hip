@@ -194,15 +194,15 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
     ro.area = rp.want_area ? c.s_area.as<double>(np + 16) : nullptr;
     ro.ev = rp.want_relate ? c.s_ev.as<double4>(np + 16) : nullptr;
     ro.evb = rp.want_relate ? c.s_evb.as<uint32_t>(np + 16) : nullptr;
+    // no host waits between refinement, compaction and the row copy beyond the ones that read sizes back: the
+    // stream orders the work (refine_ms and compact_ms are host times up to those reads)
     s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro, &s);
-    GSJ_HIP(hipStreamSynchronize(stream));
     s.refine_ms += t2.ms();
     WallTimer t3;
     uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
     double* oarea = c.s_out_area.as<double>(np + 16);
     uint32_t* ocode = c.s_out_code.as<uint32_t>(np + 16);
     const uint64_t nrows = compact_rows(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode);
-    GSJ_HIP(hipStreamSynchronize(stream));
     s.compact_ms += t3.ms();
     s.hits += nrows;
     if (hooks && hooks->on_device && nrows) hooks->on_device(orow, rp.want_area ? oarea : nullptr, nrows);
@@ -264,7 +264,6 @@ void join_batch_run(Context& c, const Layer& PL, const Layer& RL, std::vector<Ba
     ro.ev = rp.want_relate ? c.s_ev.as<double4>(np + 16) : nullptr;
     ro.evb = rp.want_relate ? c.s_evb.as<uint32_t>(np + 16) : nullptr;
     s.heavy += refine_pairs(c, PL.poly, RL.poly, pa, pb, np, rp, ro, &s);
-    GSJ_HIP(hipStreamSynchronize(stream));
     s.refine_ms += t2.ms();
     WallTimer t3;
     uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);

@@ -330,15 +330,17 @@ void walk_batches(Context& c, const Probes& pr, const DBvh& t, bool upper, const
         GSJ_LAUNCHED();
         GSJ_HIP(hipMemcpyAsync(hlo, dlo, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, stream));
         sb.k = k; sb.h = hlo; sb.d = dlo;
+        GSJ_HIP(hipStreamSynchronize(stream));   // the bounds are read on the host
       }
-      GSJ_HIP(hipStreamSynchronize(stream));
+      // otherwise no wait here: the pair work follows in stream order
       s.traverse_ms += t1.ms();
+      fn(pa, pb, np, sb);
       {
         float ms = 0;
+        GSJ_HIP(hipEventSynchronize(c.ev[15]));   // done already when the pair work read anything back
         GSJ_HIP(hipEventElapsedTime(&ms, c.ev[14], c.ev[15]));
         s.k_traverse_ms += ms;
       }
-      fn(pa, pb, np, sb);
     }
   }
 }
