@@ -247,6 +247,14 @@ typedef struct {
   void* user;
   uint32_t* root;                 /* GSJ_REQ_CLUSTER output, nroot >= parents of P */
   uint64_t nroot;
+  /* buffered requests, in place of need_self (which must then be NULL) when need_listed is set: a parent's pair with
+     itself is tested only for the parents in need_list[0, n_need), ascending (host, must outlive the request; none
+     when n_need is 0). A list costs 4 bytes per listed parent instead of one byte per parent of P, and requests with
+     an empty list (or with neither) can share a batch. No counterpart in the reference: it rebuilds its flags for
+     every request (daemon/gpujoind.cu:438-443). */
+  int need_listed;
+  const uint32_t* need_list;
+  uint64_t n_need;
 } gsj_request;
 
 typedef struct {

@@ -285,7 +285,9 @@ int gsj_intersects_buffered(gsj_ctx* ctx, const gsj_layer* P, const gsj_layer* R
     GSJ_REQUIRE(P->L.ctx == &ctx->c && R->L.ctx == &ctx->c, GSJ_E_ARG, "layer belongs to another context");
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     set_device(ctx->c);
-    gsj::erode_run(ctx->c, P->L, R->L, ra, rb, same != 0, need_self, defaults(opts), cb, user, stats);
+    gsj::SelfNeedHost need;
+    need.flags = need_self;
+    gsj::erode_run(ctx->c, P->L, R->L, ra, rb, same != 0, need, defaults(opts), cb, user, stats);
   });
 }
 

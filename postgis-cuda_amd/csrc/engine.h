@@ -294,7 +294,22 @@ struct BatchDist {
 bool dwithin_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o);
 void dwithin_batch_run(Context& c, const Layer& P, const Layer& R, bool same, bool report, std::vector<BatchDist>& reqs);
 // eroded intersects (erode.hip, dwithin.hip)
-void erode_run(Context& c, const Layer& P, const Layer& R, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
+// Which pairs of a parent with itself (same layers) the eroded intersects tests: with `listed`, the parents in the
+// ascending list[0, n) (none when n is 0); otherwise those whose flag is set (flags: one per parent of P), or every
+// one when flags is null. SelfNeedHost is the caller's (host memory), SelfNeed its copy on the device.
+struct SelfNeedHost {
+  const uint8_t* flags = nullptr;
+  const uint32_t* list = nullptr;
+  uint64_t n = 0;
+  bool listed = false;
+};
+struct SelfNeed {
+  const uint8_t* flags;
+  const uint32_t* list;
+  uint32_t n;
+  int listed;
+};
+void erode_run(Context& c, const Layer& P, const Layer& R, double ra, double rb, bool same, const SelfNeedHost& need, const gsj_join_opts& o,
                gsj_dist_cb cb, void* user, gsj_join_stats* st);
 void points_erode_run(Context& c, const Layer& pts, const Layer& R, double r, const gsj_join_opts& o, gsj_dist_cb cb, void* user, gsj_join_stats* st);
 // windowed eroded-intersects requests of the same layers, `same`, upper / batch_q / max_pairs / snap_half_ulp, no
@@ -307,7 +322,8 @@ struct BatchErode {
   gsj_join_stats* st = nullptr;
   int rc = GSJ_OK;
 };
-bool erode_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o, const uint8_t* need_self);
-void erode_batch_run(Context& c, const Layer& P, const Layer& R, bool same, std::vector<BatchErode>& reqs);
+bool erode_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o, const SelfNeedHost& need);
+// no_self: no parent's pair with itself is tested (the requests' empty lists); otherwise every one
+void erode_batch_run(Context& c, const Layer& P, const Layer& R, bool same, bool no_self, std::vector<BatchErode>& reqs);
 
 }  // namespace gsj

@@ -641,10 +641,21 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint64_t n, uint32_t*
   if (i < n) b[i] = a[i];
 }
 
+// whether parent a is in the ascending list
+__device__ __forceinline__ bool in_sorted(const uint32_t* __restrict__ list, uint32_t n, uint32_t a) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (list[mid] < a) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < n && list[lo] == a;
+}
+
 // parent rows of pass 1 that go to pass 2: overlap area able to hold a disk of radius rmin (rmin 0: any overlap,
-// < 0: every intersecting pair), self pairs only where needed (buffer.cuh:722-748)
+// < 0: every intersecting pair), self pairs only where needed (buffer.cuh:722-748): by the flags, or by the list
 __global__ void k_er_filter(const uint32_t* __restrict__ rows, const double* __restrict__ area, uint64_t n, const DPolys P,
-                            const uint32_t* __restrict__ pfP, const ErRadii rr, int same, const uint8_t* __restrict__ need_self, uint32_t* __restrict__ keep) {
+                            const uint32_t* __restrict__ pfP, const ErRadii rr, int same, const SelfNeed need, uint32_t* __restrict__ keep) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double ra, rb;
@@ -652,7 +663,10 @@ __global__ void k_er_filter(const uint32_t* __restrict__ rows, const double* __r
   const double rmin = (ra > 0 && rb > 0) ? fmin(ra, rb) : (ra > 0 || rb > 0 ? 0.0 : -1.0);
   const uint32_t a = rows[2 * i], b = rows[2 * i + 1];
   bool k = true;
-  if (same && a == b && need_self) k = need_self[a] != 0;
+  if (same && a == b) {
+    if (need.listed) k = in_sorted(need.list, need.n, a);
+    else if (need.flags) k = need.flags[a] != 0;
+  }
   if (k && rmin >= 0) {
     const uint32_t g = pfP ? pfP[a] : a;
     const double y0 = P.origin[g].y, y1 = y0 + P.ext[g].y;
@@ -685,7 +699,7 @@ namespace {
 // compacted into *out in pass-1 order; returns their count. Batched requests: lo_in[0 .. k] are the rows' request
 // boundaries (rr.seg == lo_in), and lo_mid, lo_out receive those of the filtered pairs and of the output.
 uint64_t erode_pass2(Context& c, const DPolys& P, const DPolys& R, const uint32_t* pfP, const uint32_t* pfR, const uint32_t* rows, const double* area,
-                     uint64_t n, ErRadii rr, bool same, const uint8_t* d_need, ErCtl* ctl, gsj_join_stats& s, uint32_t*& out, uint64_t* lo_mid,
+                     uint64_t n, ErRadii rr, bool same, const SelfNeed& d_need, ErCtl* ctl, gsj_join_stats& s, uint32_t*& out, uint64_t* lo_mid,
                      uint64_t* lo_out) {
   hipStream_t stream = c.stream;
   uint32_t* keep = c.s_er_keep.as<uint32_t>(n + 16);
@@ -776,21 +790,28 @@ uint64_t erode_pass2(Context& c, const DPolys& P, const DPolys& R, const uint32_
   return nhit;
 }
 
-// need_self flags on the device and the overflow queue's control blocks (one allocation, reused per call)
-ErCtl* er_scratch(Context& c, const DPolys& P, bool same, const uint8_t* need_self, const uint8_t*& d_need) {
-  const size_t need_bytes = ((size_t)P.nparent + 16 + 63) & ~size_t(63);
+// the self-pair rule on the device (the flags, one per parent, or the list) and the overflow queue's control blocks
+// (one allocation, reused per call)
+ErCtl* er_scratch(Context& c, const DPolys& P, bool same, const SelfNeedHost& need, SelfNeed& d_need) {
+  const bool flags = same && !need.listed && need.flags, list = same && need.listed && need.n;
+  const size_t need_bytes = flags ? (((size_t)P.nparent + 16 + 63) & ~size_t(63)) : list ? ((need.n * 4 + 63) & ~uint64_t(63)) : 0;
   uint8_t* er_misc = c.s_er_need.as<uint8_t>(need_bytes + 64);
-  d_need = nullptr;
-  if (same && need_self) {
-    GSJ_HIP(hipMemcpyAsync(er_misc, need_self, P.nparent, hipMemcpyHostToDevice, c.stream));
-    d_need = er_misc;
+  d_need = SelfNeed{};
+  d_need.listed = same && need.listed ? 1 : 0;
+  if (flags) {
+    GSJ_HIP(hipMemcpyAsync(er_misc, need.flags, P.nparent, hipMemcpyHostToDevice, c.stream));
+    d_need.flags = er_misc;
+  } else if (list) {
+    GSJ_HIP(hipMemcpyAsync(er_misc, need.list, need.n * 4, hipMemcpyHostToDevice, c.stream));
+    d_need.list = reinterpret_cast<const uint32_t*>(er_misc);
+    d_need.n = (uint32_t)need.n;
   }
   return reinterpret_cast<ErCtl*>(er_misc + need_bytes);
 }
 
 }  // namespace
 
-void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double rb, bool same, const uint8_t* need_self, const gsj_join_opts& o,
+void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double rb, bool same, const SelfNeedHost& need, const gsj_join_opts& o,
                gsj_dist_cb cb, void* user, gsj_join_stats* st) {
   GSJ_REQUIRE(!PL.points && !RL.points, GSJ_E_ARG, "intersects_buffered: both layers must be polygon layers (points: the points entry point)");
   GSJ_REQUIRE(RL.has_bvh, GSJ_E_ARG, "intersects_buffered: build layer has no BVH");
@@ -802,8 +823,9 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
   const DPolys& R = RL.poly;
   const uint32_t* pfP = P.pfirst;   // first piece per parent (subdivided layers), as k_piece_first builds it
   const uint32_t* pfR = R.pfirst;
-  const uint8_t* d_need = nullptr;
-  ErCtl* ctl = er_scratch(c, P, same, need_self, d_need);
+  GSJ_REQUIRE(!need.listed || need.n < (1ull << 32), GSJ_E_ARG, "intersects_buffered: self-pair list too long");
+  SelfNeed d_need;
+  ErCtl* ctl = er_scratch(c, P, same, need, d_need);
   ErRadii rr{};
   rr.k = 1;
   rr.ra[0] = ra;
@@ -861,12 +883,13 @@ void erode_run(Context& c, const Layer& PL, const Layer& RL, double ra, double r
   if (st) *st = s;
 }
 
-bool erode_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o, const uint8_t* need_self) {
-  return o.has_window && !o.cancel && !need_self && !P.points && !R.points && R.has_bvh && P.n && R.n && !P.poly.parent && !R.poly.parent &&
+bool erode_batchable(const Layer& P, const Layer& R, const gsj_join_opts& o, const SelfNeedHost& need) {
+  // one self-pair rule for the whole batch: every self pair, or (an empty list) none
+  return o.has_window && !o.cancel && !need.flags && !(need.listed && need.n) && !P.points && !R.points && R.has_bvh && P.n && R.n && !P.poly.parent && !R.poly.parent &&
          probes_batchable(probes_of(P));
 }
 
-void erode_batch_run(Context& c, const Layer& PL, const Layer& RL, bool same, std::vector<BatchErode>& reqs) {
+void erode_batch_run(Context& c, const Layer& PL, const Layer& RL, bool same, bool no_self, std::vector<BatchErode>& reqs) {
   const uint32_t k = (uint32_t)reqs.size();
   GSJ_REQUIRE(k >= 1 && k <= MAX_WINDOWS, GSJ_E_ARG, "batch size out of range");
   const gsj_join_opts& o = *reqs[0].o;
@@ -875,7 +898,7 @@ void erode_batch_run(Context& c, const Layer& PL, const Layer& RL, bool same, st
   ws.k = rr.k = k;
   for (uint32_t r = 0; r < k; r++) {
     const gsj_join_opts& q = *reqs[r].o;
-    GSJ_REQUIRE(erode_batchable(PL, RL, q, nullptr) && q.upper == o.upper && q.batch_q == o.batch_q && q.max_pairs == o.max_pairs &&
+    GSJ_REQUIRE(erode_batchable(PL, RL, q, SelfNeedHost{}) && q.upper == o.upper && q.batch_q == o.batch_q && q.max_pairs == o.max_pairs &&
                     q.snap_half_ulp == o.snap_half_ulp,
                 GSJ_E_ARG, "request cannot join this batch");
     GSJ_REQUIRE(reqs[r].ra >= 0 && reqs[r].rb >= 0 && reqs[r].ra < 1e5 && reqs[r].rb < 1e5, GSJ_E_ARG,
@@ -887,8 +910,10 @@ void erode_batch_run(Context& c, const Layer& PL, const Layer& RL, bool same, st
   hipStream_t stream = c.stream;
   const DPolys& P = PL.poly;
   const DPolys& R = RL.poly;
-  const uint8_t* d_need = nullptr;
-  ErCtl* ctl = er_scratch(c, P, same, nullptr, d_need);
+  SelfNeedHost need;
+  need.listed = no_self;   // an empty list: no parent's pair with itself
+  SelfNeed d_need;
+  ErCtl* ctl = er_scratch(c, P, same, need, d_need);
   RefineParams rp;   // pass 1: intersection areas
   rp.want_area = 1;
   rp.want_relate = 0;

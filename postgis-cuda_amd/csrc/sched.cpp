@@ -76,7 +76,7 @@ struct gsj_sched {
   static bool batchable(const Req& q) {
     if (q.r.kind == GSJ_REQ_JOIN) return gsj::join_batchable(q.r.P->L, q.r.R->L, q.r.opts);
     if (q.r.kind == GSJ_REQ_DWITHIN) return gsj::dwithin_batchable(q.r.P->L, q.r.R->L, q.r.opts);
-    if (q.r.kind == GSJ_REQ_INTERSECTS_BUFFERED) return gsj::erode_batchable(q.r.P->L, q.r.R->L, q.r.opts, q.r.need_self);
+    if (q.r.kind == GSJ_REQ_INTERSECTS_BUFFERED) return gsj::erode_batchable(q.r.P->L, q.r.R->L, q.r.opts, self_need(q.r));
     return false;
   }
   bool batch_head(const Req& q) const { return max_batch > 1 && batchable(q); }
@@ -86,7 +86,15 @@ struct gsj_sched {
     if (x.r.kind == GSJ_REQ_JOIN) return gsj::join_compatible(a, b);
     const bool common = x.r.same == head.r.same && a.upper == b.upper && a.batch_q == b.batch_q && a.max_pairs == b.max_pairs;
     if (x.r.kind == GSJ_REQ_DWITHIN) return common && x.r.report_distance == head.r.report_distance;
-    return common && a.snap_half_ulp == b.snap_half_ulp;
+    return common && a.snap_half_ulp == b.snap_half_ulp && (x.r.need_listed != 0) == (head.r.need_listed != 0);
+  }
+  static gsj::SelfNeedHost self_need(const gsj_request& r) {
+    gsj::SelfNeedHost s;
+    s.flags = r.need_self;
+    s.list = r.need_list;
+    s.n = r.n_need;
+    s.listed = r.need_listed != 0;
+    return s;
   }
 };
 
@@ -114,7 +122,7 @@ void gsj_sched::run_batch(gsj::Context& c, std::vector<Req*>& group) {
       b[i].user = group[i]->r.user;
       b[i].st = &group[i]->stats;
     }
-    rc = gsj::guarded_status([&] { gsj::erode_batch_run(c, h.P->L, h.R->L, h.same != 0, b); });
+    rc = gsj::guarded_status([&] { gsj::erode_batch_run(c, h.P->L, h.R->L, h.same != 0, h.need_listed != 0, b); });
     for (size_t i = 0; i < group.size(); i++) rcs[i] = b[i].rc;
   } else {
     std::vector<gsj::BatchDist> b(group.size());
@@ -159,7 +167,7 @@ void gsj_sched::run(gsj::Context& c, Req& q) {
       gsj::points_dwithin_run(c, r.P->L, r.R->L, r.d, r.report_distance != 0, r.opts, r.dist_cb, r.user, &q.stats);
       break;
     case GSJ_REQ_INTERSECTS_BUFFERED:
-      gsj::erode_run(c, r.P->L, r.R->L, r.ra, r.rb, r.same != 0, r.need_self, r.opts, r.dist_cb, r.user, &q.stats);
+      gsj::erode_run(c, r.P->L, r.R->L, r.ra, r.rb, r.same != 0, self_need(r), r.opts, r.dist_cb, r.user, &q.stats);
       break;
     case GSJ_REQ_POINTS_INTERSECTS_BUFFERED:
       gsj::points_erode_run(c, r.P->L, r.R->L, r.rb, r.opts, r.dist_cb, r.user, &q.stats);
@@ -277,6 +285,8 @@ int gsj_submit(gsj_sched* s, const gsj_request* req, uint64_t* ticket) {
     GSJ_REQUIRE(r.kind == GSJ_REQ_CLUSTER || r.kind == GSJ_REQ_CLUSTER_ROWS || r.R, GSJ_E_ARG, "request without a build layer");
     GSJ_REQUIRE(r.P->L.ctx == &s->owner->c && (!r.R || r.R->L.ctx == &s->owner->c), GSJ_E_ARG,
                 "layer belongs to another context than the scheduler's");
+    GSJ_REQUIRE(!r.need_listed || (!r.need_self && (r.need_list || r.n_need == 0)), GSJ_E_ARG,
+                "a listed self-pair rule takes need_list (or an empty list) and no need_self");
     auto q = std::make_unique<Req>();
     q->r = r;
     q->est = s->budget ? estimate_scratch(r) : 0;   // admission by estimate only under a budget

@@ -119,6 +119,7 @@ struct Layer {
   bool deduped = false;
   std::vector<uint64_t> moff;   // distinct geometry -> members [moff[g], moff[g+1]) of midx
   std::vector<uint32_t> midx;
+  std::vector<uint32_t> self_needed;   // distinct geometries of more than one row, ascending (built once, at LOAD)
   uint64_t n_distinct = 0, gpu_geoms = 0, verts = 0, gpu_bytes = 0;
   uint64_t copy_rows = 0, skipped_null = 0, skipped_type = 0, skipped_empty = 0;   // copy: sources (the loader's counters)
   double prep_s = 0, upload_s = 0;
@@ -327,6 +328,8 @@ std::string build_layer(const std::string& name, const std::string& source, int 
       check(gsj_dedup_device(g_ctx, gro, rvo, xy, n, &distinct.h, L->moff.data(), L->midx.data(), &nd), "dedup");   // on the GPU
       L->moff.resize(nd + 1);
       L->deduped = true;
+      for (uint64_t gi = 0; gi < nd; gi++)
+        if (L->moff[gi + 1] - L->moff[gi] > 1) L->self_needed.push_back((uint32_t)gi);
       check(gsj_hostset_arrays(distinct.h, &gro, &rvo, &xy, &parent, &syn), "layer arrays");
       check(gsj_hostset_sizes(distinct.h, &n, &nr, &nv), "layer sizes");
     }
@@ -964,7 +967,6 @@ void op_dwithin(Reply& c, const std::vector<std::string>& a) {
   gsj_request r{};
   r.dist_cb = dist_rows_cb;
   r.user = &s;
-  std::vector<uint8_t> need;
   if (points) {
     r.P = (lp.swapped ? lp.B : lp.A)->gl;
     r.R = (lp.swapped ? lp.A : lp.B)->gl;
@@ -976,10 +978,9 @@ void op_dwithin(Reply& c, const std::vector<std::string>& a) {
     if (erode) {
       r.kind = GSJ_REQ_INTERSECTS_BUFFERED; r.ra = ra; r.rb = rb;
       if (lp.same && distinct) {   // a geometry's pair with itself only matters for groups of identical rows
-        need.assign(std::max<uint64_t>(lp.A->n_distinct, 1), 0);
-        if (lp.A->deduped)
-          for (uint64_t gi = 0; gi < lp.A->n_distinct; gi++) need[gi] = lp.A->moff[gi + 1] - lp.A->moff[gi] > 1;
-        r.need_self = need.data();
+        r.need_listed = 1;
+        r.need_list = lp.A->self_needed.data();
+        r.n_need = lp.A->self_needed.size();
       }
     } else {
       r.kind = GSJ_REQ_DWITHIN; r.d = d; r.report_distance = s.with_distance;

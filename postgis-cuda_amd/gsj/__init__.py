@@ -66,7 +66,8 @@ class Request(C.Structure):
     _fields_ = [("kind", C.c_int), ("P", C.c_void_p), ("R", C.c_void_p), ("opts", JoinOpts), ("d", C.c_double), ("same", C.c_int),
                 ("report_distance", C.c_int), ("ra", C.c_double), ("rb", C.c_double), ("need_self", C.POINTER(C.c_uint8)),
                 ("rows_cb", ROWS_CB), ("points_cb", POINTS_CB), ("dist_cb", DIST_CB), ("user", C.c_void_p),
-                ("root", C.POINTER(C.c_uint32)), ("nroot", C.c_uint64)]
+                ("root", C.POINTER(C.c_uint32)), ("nroot", C.c_uint64), ("need_listed", C.c_int),
+                ("need_list", C.POINTER(C.c_uint32)), ("n_need", C.c_uint64)]
 
 
 class RequestTimes(C.Structure):
@@ -650,7 +651,9 @@ class Scheduler:
         self._pending = {}
 
     def submit(self, kind: int, P: "Layer", R: "Layer" = None, collect=True, d=0.0, same=False, report_distance=False, ra=0.0, rb=0.0,
-               need_self=None, **kw) -> int:
+               need_self=None, need_list=None, **kw) -> int:
+        """need_list (buffered requests, instead of need_self): the parents whose pair with itself is tested, ascending
+        (an empty list: none)."""
         import threading
 
         req = Request()
@@ -662,6 +665,10 @@ class Scheduler:
             ns = np.ascontiguousarray(need_self, np.uint8)
             req.need_self = _ptr(ns, C.c_uint8)
             keep.append(ns)
+        if need_list is not None:
+            nl = np.ascontiguousarray(need_list, np.uint32)
+            req.need_listed, req.need_list, req.n_need = 1, _ptr(nl, C.c_uint32), len(nl)
+            keep.append(nl)
         root = None
         if kind == REQ_CLUSTER:
             n = P.info().nparent

@@ -285,6 +285,45 @@ def test_batched_windowed_erosion_equal_their_own_calls(ctx):
     assert len(got[4].pairs) > 0 and len(got[7].pairs) == 0
 
 
+def test_listed_self_pairs_equal_the_flags_and_batch(ctx):
+    """Eroded self-intersects with the self-pair rule as a list of parents (need_list) give the rows of the same rule
+    as one flag per parent (need_self), alone and, with an empty list (no parent's pair with itself), in batches of
+    windowed requests; a request naming both forms is refused."""
+    M = gsj.generate("parcels_mesh", nx=40, ny=40, x0=-95.2, y0=38.6, w=CELL, h=CELL, seed=21)
+    L = ctx.polys(M)
+    B = ctx.polys(gsj.subdivide(_mesh(150, 8), 10))
+    flags = (np.arange(M.n) % 5 == 0).astype(np.uint8)
+    listed = np.flatnonzero(flags).astype(np.uint32)
+    x0, y0 = M.xy[:, 0].min(), M.xy[:, 1].min()
+    W, H = M.xy[:, 0].max() - x0, M.xy[:, 1].max() - y0
+    rng = np.random.default_rng(22)
+    wins = []
+    for _ in range(12):
+        f, sz = rng.uniform(0, 0.7, 2), rng.uniform(0.1, 0.3)
+        wins.append((x0 + f[0] * W, y0 + f[1] * H, x0 + (f[0] + sz) * W, y0 + (f[1] + sz) * H))
+    s = gsj.Scheduler(ctx, lanes=1, batch=16)
+    try:
+        with pytest.raises(gsj.GsjError):
+            s.wait(s.submit(gsj.REQ_INTERSECTS_BUFFERED, L, L, ra=1.0, rb=1.0, same=True, upper=True, need_self=flags, need_list=listed))
+        whole = s.wait(s.submit(gsj.REQ_INTERSECTS_BUFFERED, L, L, ra=1.0, rb=1.5, same=True, upper=True, need_list=listed))
+        big = s.submit(gsj.REQ_JOIN, B, B, want_relate=True, want_area=True, upper=True)   # keeps the lane busy: the rest queue up
+        tickets = [s.submit(gsj.REQ_INTERSECTS_BUFFERED, L, L, ra=1.0, rb=1.0, same=True, upper=True, window=w, need_list=[]) for w in wins]
+        s.wait(big)
+        got = [s.wait(t) for t in tickets]
+    finally:
+        s.close()
+    exp = ctx.intersects_buffered(L, L, 1.0, 1.5, same=True, need_self=flags, upper=True)
+    assert np.array_equal(whole.pairs, exp.pairs) and len(whole.pairs) > 0
+    selfs = [a for a, b in whole.pairs.tolist() if a == b]
+    assert selfs and all(flags[a] for a in selfs)
+    assert max(g.stats["times"]["batch"] for g in got) > 1
+    none = np.zeros(M.n, np.uint8)
+    for w, g in zip(wins, got):
+        d = ctx.intersects_buffered(L, L, 1.0, 1.0, same=True, need_self=none, upper=True, window=w)
+        assert np.array_equal(g.pairs, d.pairs)
+        assert not any(a == b for a, b in g.pairs.tolist())
+
+
 def test_windowed_mix_over_lanes_with_batching(ctx):
     """Windowed joins, relates, distance joins, eroded intersects and clusters of one resident layer, submitted at once
     to three lanes that batch: every request's rows (in order) and counts equal its own call, whatever lane and batch
