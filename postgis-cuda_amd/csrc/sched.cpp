@@ -32,6 +32,7 @@ struct Req {
   int lane = -1;
   int batch = 1;   // requests run together in its pass
   bool done = false;
+  std::condition_variable cv_done;   // its waiter's: a finished request wakes only the thread waiting for it
   int rc = GSJ_OK;
   std::string msg;
   gsj_join_stats stats{};
@@ -61,7 +62,7 @@ struct gsj_sched {
   std::vector<std::unique_ptr<gsj::Context>> lanes;
   std::vector<std::thread> workers;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work;
   std::deque<Req*> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
   uint64_t next_ticket = 1;
@@ -226,9 +227,9 @@ void gsj_sched::worker(int lane) {
         running--;
         x->t_end = now;
         x->done = true;
+        x->cv_done.notify_all();   // under the lock: its waiter frees it only after taking the lock back
       }
     }
-    cv_done.notify_all();
     cv_work.notify_all();
   }
 }
@@ -313,7 +314,7 @@ int gsj_wait(gsj_sched* s, uint64_t ticket, gsj_join_stats* stats, gsj_request_t
     auto it = s->reqs.find(ticket);
     GSJ_REQUIRE(it != s->reqs.end(), GSJ_E_ARG, "unknown ticket " + std::to_string(ticket));
     Req* p = it->second.get();
-    s->cv_done.wait(lk, [&] { return p->done; });
+    p->cv_done.wait(lk, [&] { return p->done; });
     q = std::move(it->second);
     s->reqs.erase(it);
   });
