@@ -364,45 +364,6 @@ struct SmallPoly {        // a polygon of at most CAP stored vertices (<= 255) a
 template <int CAP>
 constexpr size_t small_bytes(bool rel) { return ((2 * sizeof(SmallPoly<CAP>) + 15) & ~size_t(15)) + (size_t)2 * CAP * 8 * (rel ? 3 : 1); }
 
-// all vertices of a piece into LDS with per-vertex flags (reference edge collection, engine/engine.cuh:358-377), read
-// from its packed record (DPolys::pk): the header's counts and ring starts, the vertices, their flag bytes
-template <int G, int CAP>
-__device__ void load_small(const DPolys& S, uint32_t off, double shx, double shy, const Box& ov, SmallPoly<CAP>& sp, int gl) {
-  const uint4* rec = S.pk + off;
-  const uint8_t* hb = reinterpret_cast<const uint8_t*>(rec + 2);
-  const int nv = hb[0], nrings = hb[1];
-  const double2* RV = reinterpret_cast<const double2*>(rec + 3);
-  const uint8_t* RF = reinterpret_cast<const uint8_t*>(rec + 3 + nv);
-  for (int k = gl; k < nv; k += G) { const double2 p = RV[k]; sp.V[k] = make_double2(shx + p.x, shy + p.y); }
-  for (int r = gl; r <= nrings; r += G) sp.rs[r] = hb[2 + r];
-  if (gl == 0) { sp.nv = (uint8_t)nv; sp.nr = (uint8_t)nrings; }
-  wave_lds_sync();
-  for (int k = gl; k < nv; k += G) {
-    int r = 0;
-    while (r + 1 < (int)nrings && sp.rs[r + 1] <= k) r++;
-    const int e = sp.rs[r + 1];
-    uint8_t f = 0;
-    if (k + 1 < e) {
-      f = F_VALID | RF[k];   // VF_SYN, VF_CUT0, VF_CUT1 (0 without subdivision)
-      const double2 a = sp.V[k], b = sp.V[k + 1];
-      if (!edge_outside(a.x, a.y, b.x, b.y, ov)) f |= F_INOV;
-    }
-    sp.F[k] = f;
-  }
-  wave_lds_sync();
-  // compacted list of the edges meeting the overlap box, in vertex order (group ballot)
-  int cnt = 0;
-  for (int base = 0; base < nv; base += G) {
-    const int k = base + gl;
-    const bool in = k < nv && (sp.F[k] & (F_VALID | F_INOV)) == (F_VALID | F_INOV);
-    const uint64_t m = gballot<G>(in);
-    if (in) sp.I[cnt + __popcll(m & ((1ull << gl) - 1))] = (uint8_t)k;
-    cnt += __popcll(m);
-  }
-  if (gl == 0) sp.ni = (uint8_t)cnt;
-  wave_lds_sync();
-}
-
 // parity of the segment from a point left of the geometry's bbox to m over every edge (engine/engine.cuh:382-397)
 template <int G, int CAP>
 __device__ bool pip_small(const SmallPoly<CAP>& sp, double shx, double mx, double my, int gl) {
@@ -452,11 +413,82 @@ struct EdgeQueue {   // pairs with edge work: flag[k] per position k of the clas
   uint32_t* first;   // [3]
 };
 
+// both polygons of a pair (B only when it is not A itself) into LDS with per-vertex flags (reference edge collection,
+// engine/engine.cuh:358-377), read from their packed records (DPolys::pk): the headers' counts and ring starts, the
+// vertices, their flag bytes. Every global load of both records is issued into registers before the first LDS write,
+// so the pair pays one memory latency for its vertices instead of four (A's vertices, A's flags, B's vertices, B's
+// flags, one polygon after the other), and three wave barriers instead of six.
 template <int G, int CAP>
 __device__ __forceinline__ void load_pair(const DPolys& P, const DPolys& R, uint32_t ra, uint32_t rb, const PairSetup& f, SmallPoly<CAP>& SA,
-                                          SmallPoly<CAP>& SB, int gl) {
-  load_small<G, CAP>(P, ra, f.ashx, f.ashy, f.ov, SA, gl);
-  load_small<G, CAP>(R, rb, f.bshx, f.bshy, f.ov, SB, gl);
+                                          SmallPoly<CAP>& SB, int gl, bool self) {
+  constexpr int PER = (CAP + G - 1) / G;   // vertices per lane and polygon
+  const uint4* reca = P.pk + ra;
+  const uint4* recb = R.pk + rb;
+  const uint8_t* ha = reinterpret_cast<const uint8_t*>(reca + 2);
+  const uint8_t* hb = reinterpret_cast<const uint8_t*>(recb + 2);
+  const int nva = ha[0], nra = ha[1];
+  const int nvb = self ? 0 : hb[0], nrb = self ? 0 : hb[1];
+  const double2* VA = reinterpret_cast<const double2*>(reca + 3);
+  const double2* VB = reinterpret_cast<const double2*>(recb + 3);
+  const uint8_t* FA = reinterpret_cast<const uint8_t*>(reca + 3 + nva);
+  const uint8_t* FB = reinterpret_cast<const uint8_t*>(recb + 3 + nvb);
+  double2 va[PER], vb[PER];
+  uint8_t fa[PER], fb[PER];
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int k = gl + j * G;
+    if (k < nva) { va[j] = VA[k]; fa[j] = FA[k]; }
+    if (k < nvb) { vb[j] = VB[k]; fb[j] = FB[k]; }
+  }
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int k = gl + j * G;
+    if (k < nva) SA.V[k] = make_double2(f.ashx + va[j].x, f.ashy + va[j].y);
+    if (k < nvb) SB.V[k] = make_double2(f.bshx + vb[j].x, f.bshy + vb[j].y);
+  }
+  for (int r = gl; r <= nra; r += G) SA.rs[r] = ha[2 + r];
+  if (!self)
+    for (int r = gl; r <= nrb; r += G) SB.rs[r] = hb[2 + r];
+  if (gl == 0) {
+    SA.nv = (uint8_t)nva; SA.nr = (uint8_t)nra;
+    if (!self) { SB.nv = (uint8_t)nvb; SB.nr = (uint8_t)nrb; }
+  }
+  wave_lds_sync();
+  // per-vertex flags: valid (not the ring's last vertex), the record's flag bits, meets the overlap box
+  auto flag = [&](const SmallPoly<CAP>& sp, int k, int nrings, uint8_t rf) -> uint8_t {
+    int r = 0;
+    while (r + 1 < nrings && sp.rs[r + 1] <= k) r++;
+    if (k + 1 >= sp.rs[r + 1]) return 0;
+    uint8_t fl = F_VALID | rf;
+    const double2 a = sp.V[k], b = sp.V[k + 1];
+    if (!edge_outside(a.x, a.y, b.x, b.y, f.ov)) fl |= F_INOV;
+    return fl;
+  };
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int k = gl + j * G;
+    if (k < nva) SA.F[k] = flag(SA, k, nra, fa[j]);
+    if (k < nvb) SB.F[k] = flag(SB, k, nrb, fb[j]);
+  }
+  wave_lds_sync();
+  // compacted lists of the edges meeting the overlap box, in vertex order (group ballot)
+  int cnta = 0, cntb = 0;
+  for (int base = 0; base < max(nva, nvb); base += G) {
+    const int k = base + gl;
+    const bool ina = k < nva && (SA.F[k] & (F_VALID | F_INOV)) == (F_VALID | F_INOV);
+    const bool inb = k < nvb && (SB.F[k] & (F_VALID | F_INOV)) == (F_VALID | F_INOV);
+    const uint64_t ma = gballot<G>(ina), mb = gballot<G>(inb);
+    const uint64_t below = (1ull << gl) - 1;
+    if (ina) SA.I[cnta + __popcll(ma & below)] = (uint8_t)k;
+    if (inb) SB.I[cntb + __popcll(mb & below)] = (uint8_t)k;
+    cnta += __popcll(ma);
+    cntb += __popcll(mb);
+  }
+  if (gl == 0) {
+    SA.ni = (uint8_t)cnta;
+    if (!self) SB.ni = (uint8_t)cntb;
+  }
+  wave_lds_sync();
 }
 
 template <int G, int CAP, int WHICH, int OCC>
@@ -480,8 +512,7 @@ __global__ void __launch_bounds__(256, OCC) k_screen_small(const DPolys P, const
     const uint32_t ra = P.pk_off[a], rb = R.pk_off[b];   // record offsets (32-bit: fewer registers than pointers)
     const PairSetup f = setup_pair_rec(P.pk + ra, R.pk + rb, rp);
     const bool self = is_self(P, R, a, b);
-    if (self) load_small<G, CAP>(P, ra, f.ashx, f.ashy, f.ov, SA, gl);
-    else load_pair<G, CAP>(P, R, ra, rb, f, SA, SB, gl);
+    load_pair<G, CAP>(P, R, ra, rb, f, SA, SB, gl, self);
     const SP& SBx = self ? SA : SB;
     const int na = SA.ni, nb = SBx.ni;   // edges meeting the overlap box
     const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SBx.V, SBx.F, SBx.I, nb};
@@ -565,8 +596,7 @@ __global__ void __launch_bounds__(256, 3) k_edges_small(const DPolys P, const DP
     const uint32_t ra = P.pk_off[a], rb = R.pk_off[b];   // record offsets (32-bit: fewer registers than pointers)
     const PairSetup f = setup_pair_rec(P.pk + ra, R.pk + rb, rp);
     const bool self = is_self(P, R, a, b);
-    if (self) load_small<G, CAP>(P, ra, f.ashx, f.ashy, f.ov, SA, gl);
-    else load_pair<G, CAP>(P, R, ra, rb, f, SA, SB, gl);
+    load_pair<G, CAP>(P, R, ra, rb, f, SA, SB, gl, self);
     const SP& SBx = self ? SA : SB;
     const int na = SA.ni, nb = SBx.ni;
     const VertSrc LA{SA.V, SA.F, SA.I, na}, LB{SBx.V, SBx.F, SBx.I, nb};
