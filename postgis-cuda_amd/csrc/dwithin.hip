@@ -69,6 +69,112 @@ __device__ __forceinline__ bool probe_real(const DPolys& S, uint32_t r, double s
   return probe_vertex(S, r, shx, shy, x, y);
 }
 
+// A piece read through its packed record (DPolys::pk; the vertex-list classes' pieces, at most PACK_VCAP stored
+// vertices and PACK_RCAP rings): vertices, flag bytes (VF_*) and ring starts of one contiguous record. The functions
+// below give exactly what collect / pip_full / probe_real give from the layer's arrays (same edges in the same order,
+// same arithmetic), from loads of one record instead of the geometry -> ring offsets -> vertices chain.
+struct RecView {
+  const double2* V;
+  const uint8_t* F;
+  const uint8_t* rs;   // ring starts, rs[nr] = nv
+  double2 o, e;        // origin and extent (the layer's, copied into the record)
+  int nv, nr;
+};
+__device__ __forceinline__ RecView rec_view(const DPolys& S, uint32_t g) {
+  const uint4* rec = S.pk + S.pk_off[g];
+  const uint8_t* h = reinterpret_cast<const uint8_t*>(rec + 2);
+  RecView v;
+  v.o = reinterpret_cast<const double2*>(rec)[0];
+  v.e = reinterpret_cast<const double2*>(rec)[1];
+  v.nv = h[0];
+  v.nr = h[1];
+  v.rs = h + 2;
+  v.V = reinterpret_cast<const double2*>(rec + 3);
+  v.F = reinterpret_cast<const uint8_t*>(rec + 3 + v.nv);
+  return v;
+}
+// whether vertex k starts an edge (k + 1 within its ring)
+__device__ __forceinline__ bool rec_edge(const RecView& v, int k) {
+  int r = 0;
+  while (r + 1 < v.nr && v.rs[r + 1] <= k) r++;
+  return k + 1 < v.rs[r + 1];
+}
+// collect() of both pieces at once: their edges meeting ov, in ring order, ballot-compacted into LA / LB; every load
+// of both records (PER vertices, their successors and flag bytes per lane) is issued before the first list write.
+// Returns the full counts (more than cap: the list is incomplete and the pair overflows).
+template <int G, int PER>
+__device__ void collect_rec2(const RecView& A, double ash, double ashy, const RecView& B, double bsh, double bshy, const Box& ov, Ed* LA, Ed* LB,
+                             int cap, int gl, int& na, int& nb) {
+  double2 pa[PER], qa[PER], pb[PER], qb[PER];
+  uint8_t fa[PER], fb[PER];
+  bool ea[PER], eb[PER];
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int k = j * G + gl;
+    ea[j] = k < A.nv && rec_edge(A, k);
+    eb[j] = k < B.nv && rec_edge(B, k);
+    if (ea[j]) { pa[j] = A.V[k]; qa[j] = A.V[k + 1]; fa[j] = A.F[k]; }
+    if (eb[j]) { pb[j] = B.V[k]; qb[j] = B.V[k + 1]; fb[j] = B.F[k]; }
+  }
+  int ca = 0, cb = 0;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    Ed x, y;
+    bool va = false, vb = false;
+    if (ea[j]) {
+      x.x0 = ash + pa[j].x; x.y0 = ashy + pa[j].y; x.x1 = ash + qa[j].x; x.y1 = ashy + qa[j].y;
+      x.syn = (fa[j] & VF_SYN) ? 1 : 0;
+      x.cut = (fa[j] >> 2) & 3;
+      va = !edge_outside(x.x0, x.y0, x.x1, x.y1, ov);
+    }
+    if (eb[j]) {
+      y.x0 = bsh + pb[j].x; y.y0 = bshy + pb[j].y; y.x1 = bsh + qb[j].x; y.y1 = bshy + qb[j].y;
+      y.syn = (fb[j] & VF_SYN) ? 1 : 0;
+      y.cut = (fb[j] >> 2) & 3;
+      vb = !edge_outside(y.x0, y.y0, y.x1, y.y1, ov);
+    }
+    const uint64_t ma = gballot<G>(va), mb = gballot<G>(vb), below = (1ull << gl) - 1;
+    const int posa = ca + __popcll(ma & below), posb = cb + __popcll(mb & below);
+    if (va && posa < cap) LA[posa] = x;
+    if (vb && posb < cap) LB[posb] = y;
+    ca += __popcll(ma);
+    cb += __popcll(mb);
+  }
+  na = ca;
+  nb = cb;
+}
+// pip_full() over the record's edges
+template <int G>
+__device__ bool pip_rec(const RecView& S, double shx, double shy, double mx, double my, int gl) {
+  const double ox = shx - 1.0, oy = my;
+  bool par = false;
+  for (int k = gl; k < S.nv; k += G) {
+    if (!rec_edge(S, k)) continue;
+    const double2 p = S.V[k], q = S.V[k + 1];
+    const double x0 = shx + p.x, y0 = shy + p.y, x1 = shx + q.x, y1 = shy + q.y;
+    if ((y0 > my) == (y1 > my) && y0 != my && y1 != my) continue;
+    const bool so = left_of(x0, y0, x1, y1, ox, oy), sc = left_of(x0, y0, x1, y1, mx, my);
+    if (so == sc) continue;
+    const bool su = orient(ox, oy, mx, my, x0, y0) > 0, sv = orient(ox, oy, mx, my, x1, y1) > 0;
+    if (su != sv) par = !par;
+  }
+  return (__popcll(gballot<G>(par)) & 1) != 0;
+}
+// probe_real() of ring r of the record (has_syn: the layer has synthetic edges)
+__device__ __forceinline__ bool probe_rec(const RecView& S, int r, bool has_syn, double shx, double shy, double& x, double& y) {
+  const int s = S.rs[r], e = S.rs[r + 1];
+  if (s == e) return false;
+  int v = s;
+  if (has_syn) {
+    while (v + 1 < e && (S.F[v] & VF_SYN)) v++;
+    if (v + 1 >= e) return false;
+  }
+  const double2 p = S.V[v];
+  x = shx + p.x;
+  y = shy + p.y;
+  return true;
+}
+
 // The real edges of L[base .. min(base + 64, n)) whose boxes meet [x0, x1] x [y0, y1], one bit per entry. The pair
 // tests below run over these bits in list order: the box test alone runs once per entry, and the distance work runs
 // together across a wave's lanes, about as many times as the most crowded lane has neighbours.
@@ -84,52 +190,62 @@ __device__ __forceinline__ uint64_t near_mask(const Ed* L, int base, int n, doub
 
 // One piece pair on a lane group (buffer.cuh:185-237). Returns 1 / 0, or -1 when a side has more than cap edges in
 // the window (need = edges required). With report and a hit, dist = the pair's distance (see the file comment).
-template <int G>
+// REC: both pieces have packed records with at most PER * G stored vertices, read instead of the layer's arrays.
+template <int G, bool REC, int PER = 1>
 __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b, double d, bool report, Ed* LA, Ed* LB, int cap, int gl,
                        double& dist, int& need) {
-  const double2 oa = P.origin[a], ob = R.origin[b], ea = P.ext[a], eb = R.ext[b];
+  RecView VA{}, VB{};
+  if constexpr (REC) { VA = rec_view(P, a); VB = rec_view(R, b); }
+  const double2 oa = REC ? VA.o : P.origin[a], ob = REC ? VB.o : R.origin[b], ea = REC ? VA.e : P.ext[a], eb = REC ? VB.e : R.ext[b];
   const double sx = ob.x - oa.x, sy = ob.y - oa.y;
   double gx, gy;
   geo::deg_cover(fmin(oa.y, ob.y), fmax(oa.y + ea.y, ob.y + eb.y), d, true, gx, gy);
   const Box ov{fmax(-gx, sx - gx), fmax(-gy, sy - gy), fmin(ea.x + gx, sx + eb.x + gx), fmin(ea.y + gy, sy + eb.y + gy)};
   if (ov.x0 > ov.x1 || ov.y0 > ov.y1) return 0;
   const double ash = oa.x - oa.x, ashy = oa.y - oa.y;   // A in its own frame
-  const int na = collect<G>(P, a, ash, ashy, ov, LA, cap, gl);
-  const int nb = collect<G>(R, b, sx, sy, ov, LB, cap, gl);
+  int na, nb;
+  if constexpr (REC) collect_rec2<G, PER>(VA, ash, ashy, VB, sx, sy, ov, LA, LB, cap, gl, na, nb);
+  else { na = collect<G>(P, a, ash, ashy, ov, LA, cap, gl); nb = collect<G>(R, b, sx, sy, ov, LB, cap, gl); }
   if (na > cap || nb > cap) { need = max(na, nb); return -1; }
   wave_lds_sync();
   const geo::Frame F = geo::frame_of(oa.y + ov.y0, oa.y + ov.y1);   // metric frame of the pair's window
   const double kx = F.kx, ky = F.ky;
   bool hit = false, cross = false;
   double m_up = INFINITY;
-  for (int i = gl; i < na && (report || !hit); i += G) {
-    const Ed e = LA[i];
-    if (e.syn) continue;
-    const double bx0 = fmin(e.x0, e.x1) - gx, bx1 = fmax(e.x0, e.x1) + gx, by0 = fmin(e.y0, e.y1) - gy, by1 = fmax(e.y0, e.y1) + gy;
-    const double ux = (e.x1 - e.x0) * kx, uy = (e.y1 - e.y0) * ky, La2 = ux * ux + uy * uy;
-    for (int base = 0; base < nb && (report || !hit); base += 64) {
-      uint64_t cand = near_mask(LB, base, nb, bx0, bx1, by0, by1);
-      while (cand != 0) {
-        const int j = base + __builtin_ctzll(cand);
-        cand &= cand - 1;
-        const Ed f = LB[j];
-        if (seg_isect(e.x0, e.y0, e.x1, e.y1, f.x0, f.y0, f.x1, f.y1)) {
-          hit = cross = true;
-          if (!report) break;
-          continue;
+  // A's edges G at a time, the group stopping together once the pair is settled: a hit anywhere settles "within" (and,
+  // without report, the row); with report, a crossing anywhere settles the distance (0), so the remaining edge pairs
+  // cannot change the row either
+  for (int i0 = 0; i0 < na; i0 += G) {
+    const int i = i0 + gl;
+    const bool go = i < na && (report || !hit) && !LA[i].syn;
+    if (go) {
+      const Ed e = LA[i];
+      const double bx0 = fmin(e.x0, e.x1) - gx, bx1 = fmax(e.x0, e.x1) + gx, by0 = fmin(e.y0, e.y1) - gy, by1 = fmax(e.y0, e.y1) + gy;
+      const double ux = (e.x1 - e.x0) * kx, uy = (e.y1 - e.y0) * ky, La2 = ux * ux + uy * uy;
+      for (int base = 0; base < nb && !cross && (report || !hit); base += 64) {
+        uint64_t cand = near_mask(LB, base, nb, bx0, bx1, by0, by1);
+        while (cand != 0) {
+          const int j = base + __builtin_ctzll(cand);
+          cand &= cand - 1;
+          const Ed f = LB[j];
+          if (seg_isect(e.x0, e.y0, e.x1, e.y1, f.x0, f.y0, f.x1, f.y1)) {   // settles the row, distance 0 included
+            hit = cross = true;
+            break;
+          }
+          const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky);
+          const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
+          const double band = geo::geog_band(d, F, La2, vx * vx + vy * vy);
+          const double dlo = d - band, dhi = d + band;
+          if (d2 <= dlo * dlo) hit = true;
+          else if (!hit && d2 <= dhi * dhi &&   // (with report, a lane already within d needs no geography distance here)
+                   geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1, false) <= d)
+            hit = true;
+          if (report) m_up = fmin(m_up, sqrt(d2) + band);
+          else if (hit) break;
         }
-        const double d2 = geo::seg_seg_d2(e.x0 * kx, e.y0 * ky, e.x1 * kx, e.y1 * ky, f.x0 * kx, f.y0 * ky, f.x1 * kx, f.y1 * ky);
-        const double vx = (f.x1 - f.x0) * kx, vy = (f.y1 - f.y0) * ky;
-        const double band = geo::geog_band(d, F, La2, vx * vx + vy * vy);
-        const double dlo = d - band, dhi = d + band;
-        if (d2 <= dlo * dlo) hit = true;
-        else if (d2 <= dhi * dhi &&
-                 geo::geog_dist(oa.x + e.x0, oa.y + e.y0, oa.x + e.x1, oa.y + e.y1, oa.x + f.x0, oa.y + f.y0, oa.x + f.x1, oa.y + f.y1, false) <= d)
-          hit = true;
-        if (report) m_up = fmin(m_up, sqrt(d2) + band);
-        else if (hit) break;
       }
     }
+    if (gany<G>(report ? cross : hit)) break;
   }
   hit = gany<G>(hit);
   cross = gany<G>(cross);
@@ -137,16 +253,33 @@ __device__ int dw_eval(const DPolys& P, const DPolys& R, uint32_t a, uint32_t b,
   if (!hit || report) {
     // no boundary within d: within d iff one contains the other (a real vertex of one inside the other piece)
     const double cx = ov.x0 + 0.381966 * (ov.x1 - ov.x0), cy = ov.y0 + 0.414213 * (ov.y1 - ov.y0);
-    const bool inB_c = pip_full<G>(R, b, sx, sy, cx, cy, gl), inA_c = pip_full<G>(P, a, ash, ashy, cx, cy, gl);
+    bool inB_c, inA_c;
+    if constexpr (REC) { inB_c = pip_rec<G>(VB, sx, sy, cx, cy, gl); inA_c = pip_rec<G>(VA, ash, ashy, cx, cy, gl); }
+    else { inB_c = pip_full<G>(R, b, sx, sy, cx, cy, gl); inA_c = pip_full<G>(P, a, ash, ashy, cx, cy, gl); }
     const ListSrc SA{LA, na}, SB{LB, nb};
     bool h = false;
-    for (uint32_t r = P.gro[a] + gl; r < P.gro[a + 1] && !h; r += G) {
-      double x, y;
-      if (probe_real(P, r, ash, ashy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(SB, cx, cy, inB_c, x, y)) h = true;
-    }
-    for (uint32_t r = R.gro[b] + gl; r < R.gro[b + 1] && !h; r += G) {
-      double x, y;
-      if (probe_real(R, r, sx, sy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(SA, cx, cy, inA_c, x, y)) h = true;
+    if constexpr (REC) {
+      for (int r = gl; r < VA.nr && !h; r += G) {
+        double x, y;
+        if (probe_rec(VA, r, P.synb != nullptr, ash, ashy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 &&
+            inside_from_c(SB, cx, cy, inB_c, x, y))
+          h = true;
+      }
+      for (int r = gl; r < VB.nr && !h; r += G) {
+        double x, y;
+        if (probe_rec(VB, r, R.synb != nullptr, sx, sy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 &&
+            inside_from_c(SA, cx, cy, inA_c, x, y))
+          h = true;
+      }
+    } else {
+      for (uint32_t r = P.gro[a] + gl; r < P.gro[a + 1] && !h; r += G) {
+        double x, y;
+        if (probe_real(P, r, ash, ashy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(SB, cx, cy, inB_c, x, y)) h = true;
+      }
+      for (uint32_t r = R.gro[b] + gl; r < R.gro[b + 1] && !h; r += G) {
+        double x, y;
+        if (probe_real(R, r, sx, sy, x, y) && x >= ov.x0 && x <= ov.x1 && y >= ov.y0 && y <= ov.y1 && inside_from_c(SA, cx, cy, inA_c, x, y)) h = true;
+      }
     }
     contained = gany<G>(h);
   }
@@ -210,8 +343,9 @@ __device__ __forceinline__ bool self_piece_pair(const DwArgs& A, uint32_t a, uin
 }
 
 // list classes: a G-lane group per pair, CAP-edge LDS lists per side; class `which` (PairClass) of the partition;
-// pairs whose lists overflow are queued
-template <int G, int CAP>
+// pairs whose lists overflow are queued. REC: the class's pieces have packed records (tiny, small, medium: at most CAP
+// stored vertices each), read instead of the layer's arrays.
+template <int G, int CAP, bool REC>
 __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, int which,
                                                   uint32_t* __restrict__ ovf_list, OverflowCtl* ctl) {
   __shared__ Ed lds[256 / G][2 * CAP];
@@ -231,7 +365,7 @@ __global__ void __launch_bounds__(256) k_dw_lists(const DwArgs A, const uint32_t
     }
     double dist = NAN;
     int need = 0;
-    const int r = dw_eval<G>(A.P, A.R, a, b, pair_distance(A, p), A.report != 0, LA, LB, CAP, gl, dist, need);
+    const int r = dw_eval<G, REC, CAP / G>(A.P, A.R, a, b, pair_distance(A, p), A.report != 0, LA, LB, CAP, gl, dist, need);
     if (gl == 0) {
       if (r < 0) queue_overflow(p, need, ovf_list, ctl);
       else {
@@ -254,7 +388,7 @@ __global__ void __launch_bounds__(256) k_dw_overflow(const DwArgs A, const uint3
     const uint32_t p = list[k];
     double dist = NAN;
     int need = 0;
-    const int r = dw_eval<64>(A.P, A.R, A.pa[p], A.pb[p], pair_distance(A, p), A.report != 0, LA, LB, cap, gl, dist, need);
+    const int r = dw_eval<64, false>(A.P, A.R, A.pa[p], A.pb[p], pair_distance(A, p), A.report != 0, LA, LB, cap, gl, dist, need);
     if (gl == 0) {
       A.hit[p] = r > 0 ? 1 : 0;
       if (A.dist) A.dist[p] = r > 0 ? dist : NAN;
@@ -573,24 +707,24 @@ void dw_refine(Context& c, const DwArgs& A, uint64_t np, gsj_join_stats& s) {
     GSJ_HIP(hipStreamWaitEvent(sl, c.ev[10], 0));
   }
   GSJ_HIP(hipEventRecord(c.ev[5], sl));
-  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, sl, A, pc.list, pc.counts, (int)CL_MIXED,
+  hipLaunchKernelGGL((k_dw_lists<32, 64, false>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, sl, A, pc.list, pc.counts, (int)CL_MIXED,
                      ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[7], sl));
-  hipLaunchKernelGGL((k_dw_lists<64, 128>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, sl, A, pc.list, pc.counts, (int)CL_LARGE,
+  hipLaunchKernelGGL((k_dw_lists<64, 128, false>), dim3(std::min<uint32_t>(div_up(np, 4), max_blocks)), dim3(256), 0, sl, A, pc.list, pc.counts, (int)CL_LARGE,
                      ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[2], sl));
   GSJ_HIP(hipEventRecord(c.ev[0], stream));
-  hipLaunchKernelGGL((k_dw_lists<8, 16>), dim3(std::min<uint32_t>(div_up(np, 32), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_TINY,
+  hipLaunchKernelGGL((k_dw_lists<8, 16, true>), dim3(std::min<uint32_t>(div_up(np, 32), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_TINY,
                      ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[8], stream));
-  hipLaunchKernelGGL((k_dw_lists<16, 32>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_SMALL,
+  hipLaunchKernelGGL((k_dw_lists<16, 32, true>), dim3(std::min<uint32_t>(div_up(np, 16), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_SMALL,
                      ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[1], stream));
-  hipLaunchKernelGGL((k_dw_lists<32, 64>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_MEDIUM,
+  hipLaunchKernelGGL((k_dw_lists<32, 64, true>), dim3(std::min<uint32_t>(div_up(np, 8), max_blocks)), dim3(256), 0, stream, A, pc.list, pc.counts, (int)CL_MEDIUM,
                      ovf, pc.ctl);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[9], stream));
