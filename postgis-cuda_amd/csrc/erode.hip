@@ -43,12 +43,47 @@ __device__ __forceinline__ ErScratch er_carve(unsigned char* base, int capE, int
   return s;
 }
 
-// strictly inside (by more than eps) the region within r of the boundary (buffer.cuh:285-320)
+// the point is strictly inside (by more than eps) the part of the exclusion region that edge E[k] contributes: its
+// two-sided strip, or the disk (convex) or fillet polygon (reflex) at its end vertex; the box test against the edge's
+// box widened by r goes first (buffer.cuh:285-320)
+__device__ __forceinline__ bool in_zone(const BEd* E, int k, double r, double ri, double x, double y, double eps) {
+  const BEd& e = E[k];
+  if (x < fmin(e.x0, e.x1) - r || x > fmax(e.x0, e.x1) + r || y < fmin(e.y0, e.y1) - r || y > fmax(e.y0, e.y1) + r) return false;
+  const double dx = x - e.x0, dy = y - e.y0;
+  const double t = dx * e.ux + dy * e.uy, s = dy * e.ux - dx * e.uy;   // s > 0: interior (left) side
+  if (fabs(s) < ri && t > -eps && t < e.L + eps) return true;         // two-sided strip along the edge
+  if (e.vt == 0) return false;
+  const double vx = x - e.x1, vy = y - e.y1, d2 = vx * vx + vy * vy;
+  if (d2 >= r * r) return false;
+  const BEd& f = E[e.succ];
+  if (e.vt == 1) {   // convex vertex: disk outside the interior wedge
+    const double s2 = vy * f.ux - vx * f.uy;
+    return d2 < ri * ri && !(s > eps && s2 > eps);
+  }
+  // reflex vertex: fillet polygon (v, P0 .. Pm) from n_k clockwise to n_s
+  const double nkx = -e.uy, nky = e.ux, nsx = -f.uy, nsy = f.ux;
+  if (nkx * vy - nky * vx > eps) return false;
+  if (nsx * vy - nsy * vx < -eps) return false;
+  const double chh = sqrt(0.5 * (1.0 + e.ci)), shh = sqrt(fmax(0.0, 0.5 * (1.0 - e.ci)));   // cos, sin of half the increment
+  const double ch = r * chh - eps;                                                         // chord distance from v
+  if (d2 < ch * ch) return true;
+  double mx = nkx * chh + nky * shh, my = -nkx * shh + nky * chh;   // chord normals
+  bool in = true;
+  for (int i = 0, m = e.vt - 1; i < m && in; i++) {
+    if (mx * vx + my * vy >= ch) in = false;
+    const double t2 = mx * e.ci + my * e.si;
+    my = -mx * e.si + my * e.ci;
+    mx = t2;
+  }
+  return in;
+}
+
+// strictly inside (by more than eps) the region within r of the boundary: an OR of in_zone over the edges
 __device__ bool in_excl(const BEd* E, int n, double r, double x, double y, double eps) {
   if (r <= 0) return false;
   const double ri = r - eps;
-  // the answer is an OR over the edges: the box test marks 64 edges at a time, then the region tests run over the
-  // marked ones (a wave's lanes take those iterations together)
+  // the box test marks 64 edges at a time, then the region tests run over the marked ones (a wave's lanes take those
+  // iterations together)
   for (int base = 0; base < n; base += 64) {
     uint64_t cand = 0;
     for (int j = 0, m = min(64, n - base); j < m; j++) {
@@ -58,53 +93,49 @@ __device__ bool in_excl(const BEd* E, int n, double r, double x, double y, doubl
     while (cand != 0) {
       const int k = base + __builtin_ctzll(cand);
       cand &= cand - 1;
-      const BEd& e = E[k];
-      const double dx = x - e.x0, dy = y - e.y0;
-      const double t = dx * e.ux + dy * e.uy, s = dy * e.ux - dx * e.uy;   // s > 0: interior (left) side
-      if (fabs(s) < ri && t > -eps && t < e.L + eps) return true;         // two-sided strip along the edge
-      if (e.vt == 0) continue;
-      const double vx = x - e.x1, vy = y - e.y1, d2 = vx * vx + vy * vy;
-      if (d2 >= r * r) continue;
-      const BEd& f = E[e.succ];
-      if (e.vt == 1) {   // convex vertex: disk outside the interior wedge
-        const double s2 = vy * f.ux - vx * f.uy;
-        if (d2 < ri * ri && !(s > eps && s2 > eps)) return true;
-        continue;
-      }
-      // reflex vertex: fillet polygon (v, P0 .. Pm) from n_k clockwise to n_s
-      const double nkx = -e.uy, nky = e.ux, nsx = -f.uy, nsy = f.ux;
-      if (nkx * vy - nky * vx > eps) continue;
-      if (nsx * vy - nsy * vx < -eps) continue;
-      const double chh = sqrt(0.5 * (1.0 + e.ci)), shh = sqrt(fmax(0.0, 0.5 * (1.0 - e.ci)));   // cos, sin of half the increment
-      const double ch = r * chh - eps;                                                         // chord distance from v
-      if (d2 < ch * ch) return true;
-      double mx = nkx * chh + nky * shh, my = -nkx * shh + nky * chh;   // chord normals
-      bool in = true;
-      for (int i = 0, m = e.vt - 1; i < m && in; i++) {
-        if (mx * vx + my * vy >= ch) in = false;
-        const double t2 = mx * e.ci + my * e.si;
-        my = -mx * e.si + my * e.ci;
-        mx = t2;
-      }
-      if (in) return true;
+      if (in_zone(E, k, r, ri, x, y, eps)) return true;
     }
   }
   return false;
+}
+
+// whether edge E[k] crosses the segment c -> x (parity step of inside_m)
+__device__ __forceinline__ bool crosses_cx(const BEd* E, int k, double cx, double cy, double x, double y, double sx0, double sx1, double sy0,
+                                           double sy1) {
+  const BEd& e = E[k];
+  if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) return false;
+  const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
+  if (sc == sx) return false;
+  const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
+  return su != sv;
 }
 
 // inside(x) = inside(c) xor parity of the real edges crossing c -> x (buffer.cuh:323-335)
 __device__ bool inside_m(const BEd* E, int n, double cx, double cy, bool in_c, double x, double y) {
   bool in = in_c;
   const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
-  for (int k = 0; k < n; k++) {
-    const BEd& e = E[k];
-    if (fmax(e.x0, e.x1) < sx0 || fmin(e.x0, e.x1) > sx1 || fmax(e.y0, e.y1) < sy0 || fmin(e.y0, e.y1) > sy1) continue;
-    const bool sc = left_of(e.x0, e.y0, e.x1, e.y1, cx, cy), sx = left_of(e.x0, e.y0, e.x1, e.y1, x, y);
-    if (sc == sx) continue;
-    const bool su = orient(cx, cy, x, y, e.x0, e.y0) > 0, sv = orient(cx, cy, x, y, e.x1, e.y1) > 0;
-    if (su != sv) in = !in;
-  }
+  for (int k = 0; k < n; k++)
+    if (crosses_cx(E, k, cx, cy, x, y, sx0, sx1, sy0, sy1)) in = !in;
   return in;
+}
+
+// in_excl and inside_m of one point on a G-lane group, each lane a share of the edges (k = lane, lane + G, ...),
+// combined by ballot: the same OR and the same parity as the one-lane loops, in 1/G of their latency
+template <int G>
+__device__ __forceinline__ bool in_excl_g(const BEd* E, int n, double r, double x, double y, double eps, int lane) {
+  if (r <= 0) return false;
+  const double ri = r - eps;
+  bool hit = false;
+  for (int k = lane; k < n && !hit; k += G) hit = in_zone(E, k, r, ri, x, y, eps);
+  return gany<G>(hit);
+}
+template <int G>
+__device__ __forceinline__ bool inside_m_g(const BEd* E, int n, double cx, double cy, bool in_c, double x, double y, int lane) {
+  const double sx0 = fmin(cx, x), sx1 = fmax(cx, x), sy0 = fmin(cy, y), sy1 = fmax(cy, y);
+  bool par = false;
+  for (int k = lane; k < n; k += G)
+    if (crosses_cx(E, k, cx, cy, x, y, sx0, sx1, sy0, sy1)) par = !par;
+  return in_c != ((__popcll(gballot<G>(par)) & 1) != 0);
 }
 
 struct ESide { uint32_t g0, g1; double r, bx0, by0, bx1, by1; };
@@ -430,22 +461,30 @@ __device__ bool er_inside(const ErScratch& sc, const ErPair& E, bool inA_c, bool
 }
 
 // The vertex part of (b) / (c) for one side: an element end of that side's eroded boundary inside the other eroded
-// polygon. A subset of er_inside's tests, so a hit here is a hit of the pair.
-__device__ bool er_vertices(const ErScratch& sc, const ErPair& E, bool inA_c, bool inB_c, int side, int t, int NT, const volatile int* stop) {
-  auto inW = [&](double x, double y) { return x >= E.Wx0 - E.eps && x <= E.Wx1 + E.eps && y >= E.Wy0 - E.eps && y <= E.Wy1 + E.eps; };
+// polygon (a subset of er_inside's tests, so a hit here is a hit of the pair). Each element end is checked by a whole
+// G-lane group, the lanes sharing its edges (in_excl_g, inside_m_g): group gi of ng takes elements gi, gi + ng, ... in
+// order. A parent's self pair is settled by its first qualifying end, so the group's latency per end, not one lane's
+// walk over all edges per end, sets its time (windowed erosion on the national layer: 648 -> 362 ms for 209
+// requests). stop (another group's success) is polled between elements, group-uniformly.
+template <int G>
+__device__ bool er_vertices_g(const ErScratch& sc, const ErPair& E, bool inA_c, bool inB_c, int side, int gi, int ng, int lane,
+                              const volatile int* stop) {
   const BEd* ES = side ? sc.EB : sc.EA;
   const BEd* EO = side ? sc.EA : sc.EB;
   const BEl* LS = side ? sc.LB : sc.LA;
   const int ns = side ? E.nb : E.na, no = side ? E.na : E.nb, ls = side ? E.lb : E.la;
   const double rs = side ? E.B.r : E.A.r, ro = side ? E.A.r : E.B.r;
   const bool inO_c = side ? inA_c : inB_c;
-  auto vertex_in_other = [&](double x, double y) {
-    return inW(x, y) && !in_excl(ES, ns, rs, x, y, E.eps) && inside_m(EO, no, E.cx, E.cy, inO_c, x, y) && !in_excl(EO, no, ro, x, y, E.eps);
-  };
-  for (int i = t; i < ls; i += NT) {
-    if (stop && *stop) return false;
+  for (int i = gi; i < ls; i += ng) {
+    if (stop && gany<G>(*stop != 0)) return false;
     const BEl p = LS[i];
-    if (vertex_in_other(p.x0, p.y0) || vertex_in_other(p.x1, p.y1)) return true;
+    for (int end = 0; end < 2; end++) {
+      const double x = end ? p.x1 : p.x0, y = end ? p.y1 : p.y0;
+      if (!(x >= E.Wx0 - E.eps && x <= E.Wx1 + E.eps && y >= E.Wy0 - E.eps && y <= E.Wy1 + E.eps)) continue;
+      if (in_excl_g<G>(ES, ns, rs, x, y, E.eps, lane)) continue;
+      if (!inside_m_g<G>(EO, no, E.cx, E.cy, inO_c, x, y, lane)) continue;
+      if (!in_excl_g<G>(EO, no, ro, x, y, E.eps, lane)) return true;
+    }
   }
   return false;
 }
@@ -469,7 +508,7 @@ __device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, co
     inA_c = pip_parent<G>(P, E.A, E.F, E.cx, E.cy, lane);
     inB_c = pip_parent<G>(R, E.B, E.F, E.cx, E.cy, lane);
     have_c = true;
-    if (gany<G>(er_vertices(sc, E, inA_c, inB_c, self_side(E), lane, G, nullptr))) return 1;
+    if (er_vertices_g<G>(sc, E, inA_c, inB_c, self_side(E), 0, 1, lane, nullptr)) return 1;
   }
   if (gany<G>(er_cross(sc, E, lane, G, nullptr))) return 1;
   if (!have_c) {
@@ -576,7 +615,7 @@ __global__ void __launch_bounds__(T) k_erode_slab(const DPolys P, const DPolys R
         if (lane == 0) { s_inA = a; s_inB = b; }
       }
       __syncthreads();
-      if (self && er_vertices(sc, E, s_inA != 0, s_inB != 0, self_side(E), t, T, &s_hit)) s_hit = 1;
+      if (self && er_vertices_g<64>(sc, E, s_inA != 0, s_inB != 0, self_side(E), t / 64, T / 64, lane, &s_hit)) s_hit = 1;
       __syncthreads();
       if (!s_hit && er_cross(sc, E, t, T, &s_hit)) s_hit = 1;
       __syncthreads();
