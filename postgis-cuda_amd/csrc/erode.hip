@@ -504,9 +504,9 @@ __device__ int er_pair(const DPolys& P, const DPolys& R, const uint32_t* pfP, co
   if (st <= 0) return st;
   const bool self = pa == pb && P.xy == R.xy;
   bool inA_c = false, inB_c = false, have_c = false;
-  if (self) {
+  if (self) {   // one parent, one point c: both containments are the same walk
     inA_c = pip_parent<G>(P, E.A, E.F, E.cx, E.cy, lane);
-    inB_c = pip_parent<G>(R, E.B, E.F, E.cx, E.cy, lane);
+    inB_c = inA_c;
     have_c = true;
     if (er_vertices_g<G>(sc, E, inA_c, inB_c, self_side(E), 0, 1, lane, nullptr)) return 1;
   }
@@ -555,7 +555,15 @@ __global__ void __launch_bounds__(ER_WAVES * 64) k_erode_main(const DPolys P, co
     int ne = 0, nl = 0;
     double ra, rb;
     radii_of(rr, p, ra, rb);
-    const int r = er_pair<G>(P, R, pfP, pfR, pairs[2 * p], pairs[2 * p + 1], ra, rb, sc, lane, ne, nl);
+    const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+    // a parent of an unsubdivided layer paired with itself: its window holds nearly all its edges, so a piece of more
+    // edges than CAPE overflows this pass's lists; it goes on to the next pass untried (every pass decides a pair the
+    // same way; needE = its edge count bounds what it collects)
+    int r;
+    const uint32_t gi = !pfP && a == b && P.xy == R.xy ? P.ginfo[a] : 0u;
+    const int edges = (int)(gi & 0xFFFFu) - (int)(gi >> 16);
+    if (edges > CAPE) { r = -1; ne = edges; nl = 0; }
+    else r = er_pair<G>(P, R, pfP, pfR, a, b, ra, rb, sc, lane, ne, nl);
     if (lane == 0) {
       flag[p] = r > 0 ? 1 : 0;
       if (r < 0) {
@@ -611,7 +619,7 @@ __global__ void __launch_bounds__(T) k_erode_slab(const DPolys P, const DPolys R
       const bool self = pairs[2 * p] == pairs[2 * p + 1] && P.xy == R.xy;
       if (t < 64) {
         const bool a = pip_parent<64>(P, E.A, E.F, E.cx, E.cy, lane);
-        const bool b = pip_parent<64>(R, E.B, E.F, E.cx, E.cy, lane);
+        const bool b = self ? a : pip_parent<64>(R, E.B, E.F, E.cx, E.cy, lane);   // one parent, one point c
         if (lane == 0) { s_inA = a; s_inB = b; }
       }
       __syncthreads();
