@@ -169,6 +169,41 @@ static uint64_t compact_rows(Context& c, const Layer& PL, const Layer& RL, const
   return nsingle + nkeep;
 }
 
+// Sub-batches of at most this many candidates whose rows need no piece -> parent folding compact and copy their rows
+// with one host wait (compact_copy_small)
+constexpr uint64_t SMALL_COPY_PAIRS = 1u << 20;
+
+// compact_rows and the row copy of such a sub-batch: the copy takes np rows (the candidate count, known on the host, is
+// an upper bound of the row count) together with the device's row count, so the count is not read back first: one
+// host wait instead of two. Returns the row count; the rows are in the context's pinned buffers.
+static uint64_t compact_copy_small(Context& c, const Layer& PL, const Layer& RL, const uint32_t* pa, const uint32_t* pb, uint64_t np,
+                                   const RefineParams& rp, const RefineOut& ro, bool no_agg, uint32_t* orow, double* oarea, uint32_t* ocode,
+                                   uint32_t*& hrow, double*& harea, uint32_t*& hcode) {
+  hipStream_t st = c.stream;
+  const DPolys& P = PL.poly;
+  const DPolys& R = RL.poly;
+  uint32_t* fs = c.s_tmp.as<uint32_t>(2 * np + 16);
+  uint32_t* fm = fs + np;
+  uint64_t* offs = c.s_offs.as<uint64_t>(np + 16);
+  uint64_t* d_total = c.s_misc.as<uint64_t>(4);   // the scan primitives' scalar
+  hipLaunchKernelGGL(k_row_flags, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, ro.isect, np, no_agg ? 1 : 0, fs, fm);
+  GSJ_LAUNCHED();
+  scan_u32_to_u64_async(c, fs, offs, np, d_total);
+  hipLaunchKernelGGL(k_emit_single, dim3(div_up(np, 256)), dim3(256), 0, st, P, R, pa, pb, fs, offs, np, no_agg ? 1 : 0, ro.area, ro.ev, ro.evb,
+                     orow, rp.want_area ? oarea : nullptr, rp.want_relate ? ocode : nullptr);
+  GSJ_LAUNCHED();
+  uint64_t* h_total = c.h_scal.as<uint64_t>(4);
+  hrow = c.h_pairs.as<uint32_t>(2 * np);
+  harea = rp.want_area ? c.h_area.as<double>(np) : nullptr;
+  hcode = rp.want_relate ? c.h_code.as<uint32_t>(np) : nullptr;
+  GSJ_HIP(hipMemcpyAsync(h_total, d_total, 8, hipMemcpyDeviceToHost, st));
+  GSJ_HIP(hipMemcpyAsync(hrow, orow, np * 8, hipMemcpyDeviceToHost, st));
+  if (harea) GSJ_HIP(hipMemcpyAsync(harea, oarea, np * 8, hipMemcpyDeviceToHost, st));
+  if (hcode) GSJ_HIP(hipMemcpyAsync(hcode, ocode, np * 4, hipMemcpyDeviceToHost, st));
+  GSJ_HIP(hipStreamSynchronize(st));
+  return h_total[0];
+}
+
 void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts& o, gsj_rows_cb cb, void* user, gsj_join_stats* st,
               const JoinHooks* hooks) {
   GSJ_REQUIRE(!PL.points && !RL.points, GSJ_E_ARG, "join: both layers must be polygon layers");
@@ -202,6 +237,16 @@ void join_run(Context& c, const Layer& PL, const Layer& RL, const gsj_join_opts&
     uint32_t* orow = c.s_out_a.as<uint32_t>(2 * np + 16);
     double* oarea = c.s_out_area.as<double>(np + 16);
     uint32_t* ocode = c.s_out_code.as<uint32_t>(np + 16);
+    if (!hooks && np <= SMALL_COPY_PAIRS && (o.no_agg || (!PL.poly.pfirst && !RL.poly.pfirst))) {
+      uint32_t* hrow = nullptr;
+      double* harea = nullptr;
+      uint32_t* hcode = nullptr;
+      const uint64_t nrows = compact_copy_small(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode, hrow, harea, hcode);
+      s.compact_ms += t3.ms();
+      s.hits += nrows;
+      if (nrows && cb && cb(user, hrow, harea, hcode, (size_t)nrows)) throw Error(GSJ_E_CANCEL, "join cancelled by the row callback");
+      return;
+    }
     const uint64_t nrows = compact_rows(c, PL, RL, pa, pb, np, rp, ro, o.no_agg != 0, orow, oarea, ocode);
     s.compact_ms += t3.ms();
     s.hits += nrows;

@@ -346,9 +346,12 @@ __device__ __forceinline__ void finish_pair(int gl, uint64_t p, bool hit, bool r
   }
   if (gl == 0) {
     const bool ev_any = ai != 0 || ao != 0 || bi != 0 || bo != 0;
-    out.isect[p] = hit ? 1 : ((ar != 0.0 || ev_any) ? 2 : 0);
-    if (out.area) out.area[p] = ar;
-    if (out.ev) { out.ev[p] = make_double4(ai, ao, bi, bo); out.evb[p] = eb4; }
+    const uint8_t v = hit ? 1 : ((ar != 0.0 || ev_any) ? 2 : 0);
+    out.isect[p] = v;
+    if (v) {   // compaction reads a pair's area and evidence only where its flag is set
+      if (out.area) out.area[p] = ar;
+      if (out.ev) { out.ev[p] = make_double4(ai, ao, bi, bo); out.evb[p] = eb4; }
+    }
   }
 }
 
