@@ -143,6 +143,22 @@ def test_batching_is_invisible(ctx):
     _check_rows(small, {"pairs": ref.pairs, "area": ref.area, "code": ref.code}, area=True, code=True)
 
 
+@pytest.mark.parametrize("subdiv", [0, 40])
+def test_row_copy_paths_agree(ctx, subdiv):
+    """Sub-batches of at most 1M candidates copy their rows in one host wait (the copy sized by the candidate count);
+    larger ones read the row count first. A layer of more than 1M candidates run as one sub-batch and as sub-batches
+    of 2^19 gives the same rows, areas and codes (subdivided: both fold pieces into parents, at both sizes)."""
+    M = _mesh(560, seed=11)
+    if subdiv:
+        M = gsj.subdivide(M, subdiv)
+    L = ctx.polys(M)
+    whole = ctx.join(L, L, want_area=True, want_relate=True, upper=True)
+    assert whole.stats["candidates"] > (1 << 20) and whole.stats["batches"] == 1
+    parts = ctx.join(L, L, want_area=True, want_relate=True, upper=True, max_pairs=1 << 19)
+    assert parts.stats["batches"] > 2
+    _check_rows(parts, {"pairs": whole.pairs, "area": whole.area, "code": whole.code}, area=True, code=True)
+
+
 def test_points_in_polygons(ctx):
     M = _mesh(80, seed=3)
     x0, y0 = -95.5, 38.5
