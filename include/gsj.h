@@ -56,7 +56,9 @@ int gsj_ctx_trim(gsj_ctx* ctx);
 /* Launch shapes of the pair kernels per launch size, as text: '#' comments and one line per size bucket,
  * "bucket np>=N key=value ..." with keys tcap (0..16), mcap (0 or 33..64), screen_occ (4, 5), screen_bpc, edge_bpc,
  * mixed_bpc, large_bpc, dw_bpc (blocks per CU, 1..64), trav_block (64, 128, 256) and ovf_lds (1: overflow pairs
- * with lists of at most 448 edges per side keep them in LDS; 0: in global memory) and fork (1: the mixed and large
+ * with lists of at most 448 edges per side keep them in LDS; 0: in global memory; 2: as 1, and the mixed and large
+ * classes' overflow pairs with lists within 256 edges are refined right behind them, on their stream, before the
+ * host reads the queue) and fork (1: the mixed and large
  * classes run on a second stream of the context beside the other classes; 0: in line), mixed (0: no mixed class)
  * large_cap (0..96: the large class's list capacity; 0 sends every large pair to the overflow kernel) and size_sort
  * and dw_size_sort (join and dwithin, 0..4: v > 0 puts pairs of similar size side by side within a class, in size

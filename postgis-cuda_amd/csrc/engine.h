@@ -114,8 +114,8 @@ struct Context {
   DevBuf s_er_keep, s_er_off, s_er_pairs, s_er_flag, s_er_list, s_er_slab, s_er_out, s_er_need, s_er_pf;
   PinBuf h_pairs, h_area, h_code, h_where, h_scal, h_counts, h_seg, h_slots_ctl;
   int cu_count = 256;
-  hipEvent_t ev[16] = {};   // kernel timing events (created with the context); ev[10], ev[11]: fork / join of `side`;
-                            // ev[12 .. 15]: the traversal passes and the BVH build
+  hipEvent_t ev[17] = {};   // kernel timing events (created with the context); ev[10], ev[11]: fork / join of `side`;
+                            // ev[12 .. 15]: the traversal passes and the BVH build; ev[16]: the list classes' overflow
   bool borrowed = false;   // stream and events belong to another context (not destroyed with this one)
   Arena* dev_arena = nullptr;   // set: every scratch buffer comes from this arena (the no-allocation entry point)
   Tuning tune;                  // launch shapes of the pair kernels per launch size

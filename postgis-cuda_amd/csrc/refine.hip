@@ -774,6 +774,7 @@ __global__ void __launch_bounds__(256, 3) k_refine_large(const DPolys P, const D
 // lists overflow are queued
 constexpr int CAP_MIXED = 32, MIXED_T = 128;
 constexpr uint32_t OVF_LDS_CAP = 448;   // overflow lists up to this many edges per side go to LDS (448 x 128 B = 56 KiB)
+constexpr uint32_t OVF_SIDE_CAP = 256;  // the list classes' own overflow launch (ovf_lds 2): LDS lists of up to 256 edges
 __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                           const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, RefineParams rp,
                                                           RefineOut out, uint32_t* __restrict__ promo, OverflowCtl* promo_ctl,
@@ -810,16 +811,22 @@ __global__ void __launch_bounds__(MIXED_T) k_refine_mixed(const DPolys P, const 
 // waves per pair when the pairs are too few to fill the GPU (a windowed request's few largest pairs), one wave per pair
 // otherwise. Every barrier sits in block-uniform control flow (the loop bounds and branches around them depend only on
 // the pair and on values the block has read back from shared memory after a barrier).
+// With `dev_n` the queue length is read on the device (a launch enqueued behind the kernels that fill the queue, with
+// no host read in between), and with `defer` a pair whose lists exceed `cap` is handed on to that queue (its count
+// of such pairs kept in dev_n->pad0) instead of being refined here.
 template <int T, bool IN_LDS>
 __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPolys R, const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                            const uint32_t* __restrict__ list, uint32_t nlist, RefineParams rp, RefineOut out, int cap,
-                                                           unsigned char* __restrict__ slab) {
+                                                           unsigned char* __restrict__ slab, OverflowCtl* dev_n, uint32_t* __restrict__ defer_list,
+                                                           OverflowCtl* defer) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_na, s_nb, s_hit, s_hit_any, s_found, s_contain, s_inA, s_inB;   // s_hit: contact; s_found: contact or containment
+  __shared__ int s_defer;
   __shared__ uint32_t s_bits;
   const int t = threadIdx.x, gl = t & 63, w = t >> 6;
   const bool rel = rp.want_relate != 0;
   const Scratch S = carve(IN_LDS ? smem : slab + (uint64_t)blockIdx.x * list_bytes(cap, rel), cap, rel);
+  if (dev_n) nlist = dev_n->count;
   for (uint32_t k = blockIdx.x; k < nlist; k += gridDim.x) {
     const uint32_t p = list[k];
     const uint32_t a = pa[p], b = pb[p];
@@ -827,10 +834,15 @@ __global__ void __launch_bounds__(T) k_refine_overflow(const DPolys P, const DPo
     if (w == 0) {
       const int na = collect<64>(P, a, f.ashx, f.ashy, f.ov, S.LA, cap, gl);
       const int nb = collect<64>(R, b, f.bshx, f.bshy, f.ov, S.LB, cap, gl);
-      if (gl == 0) { s_na = min(na, cap); s_nb = min(nb, cap); s_hit = 0; s_hit_any = 0; s_contain = 0; s_bits = 0; }
+      if (gl == 0) {
+        s_na = min(na, cap); s_nb = min(nb, cap); s_hit = 0; s_hit_any = 0; s_contain = 0; s_bits = 0;
+        s_defer = defer && (na > cap || nb > cap);
+        if (s_defer) { queue_overflow(p, max(na, nb), defer_list, defer); atomicAdd(&dev_n->pad0, 1u); }
+      }
     }
     __syncthreads();
     const int na = s_na, nb = s_nb;
+    if (s_defer) { __syncthreads(); continue; }   // every wave has read s_defer before the next pair rewrites it
     // boundary contact over the A x B edge grid (engine/engine.cuh:378-381): a real contact anywhere, else any contact.
     // The grid is cut into T strips of whole rows when A has T edges or more, else each row into T / na pieces, so
     // that every thread of the block gets a share however few edges A has.
@@ -946,9 +958,15 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   uint32_t* list = pc.list;
   uint32_t* counts = pc.counts;
   OverflowCtl* ctl = pc.ctl;
-  uint32_t* ovf = c.s_heavy.as<uint32_t>(2 * np + 128);
+  uint32_t* ovf = c.s_heavy.as<uint32_t>(3 * np + 192);
   uint32_t* promo = ovf + np + 64;   // mixed pairs promoted to the large class
   OverflowCtl* promo_ctl = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);
+  // ovf_lds 2: the list classes' overflow pairs are refined on their own chain, right behind the large kernel, by a
+  // launch that reads the queue length on the device; the vertex-list classes' spills and the pairs whose lists
+  // exceed OVF_SIDE_CAP go to a second queue (ovf_main) that the host reads once both chains are done
+  const bool side_ovf = sh.ovf_lds == 2;
+  uint32_t* ovf_main = side_ovf ? promo + np + 64 : ovf;
+  OverflowCtl* ctl_main = side_ovf ? reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 16) : ctl;
   const int T = 256;
   const uint32_t cu = (uint32_t)c.cu_count;
   // The list classes (mixed, then large: the large kernel takes the pairs mixed promotes) depend only on the
@@ -988,6 +1006,13 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     GSJ_LAUNCHED();
   }
   GSJ_HIP(hipEventRecord(c.ev[2], sl));
+  if (side_ovf) {   // one wave per pair over a grid sized for the largest queue the launch can have, idle blocks leave at once
+    const uint32_t g = std::min<uint32_t>((uint32_t)std::min<uint64_t>(np, 1u << 30), cu * 8);
+    hipLaunchKernelGGL((k_refine_overflow<64, true>), dim3(g), dim3(64), list_bytes(OVF_SIDE_CAP, rel), sl, P, R, pa, pb, ovf, 0u, rp, out,
+                       (int)OVF_SIDE_CAP, nullptr, ctl, ovf_main, ctl_main);
+    GSJ_LAUNCHED();
+    GSJ_HIP(hipEventRecord(c.ev[16], sl));
+  }
   GSJ_HIP(hipEventRecord(c.ev[0], st));
   // grid-stride kernels: grids of one group per pair up to the shape's cap in blocks per CU
   auto grid = [&](uint32_t lanes, uint32_t bpc) { return std::min<uint32_t>(div_up(np, T / lanes), cu * bpc); };
@@ -1005,27 +1030,32 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
   GSJ_HIP(hipEventRecord(c.ev[6], st));
   const uint32_t gt = grid(8, sh.edge_bpc), gs = grid(16, sh.edge_bpc), gm = grid(32, sh.edge_bpc);
   hipLaunchKernelGGL((k_edges_small<8, TCAP, CL_TINY>), dim3(gt), dim3(T), (T / 8) * small_bytes<TCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
-                     ovf, ctl);
+                     ovf_main, ctl_main);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[8], st));
   hipLaunchKernelGGL((k_edges_small<16, VCAP, CL_SMALL>), dim3(gs), dim3(T), (T / 16) * small_bytes<VCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
-                     ovf, ctl);
+                     ovf_main, ctl_main);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[1], st));
   hipLaunchKernelGGL((k_edges_small<32, MCAP, CL_MEDIUM>), dim3(gm), dim3(T), (T / 32) * small_bytes<MCAP>(rel), st, P, R, pa, pb, counts, rp, out, eq,
-                     ovf, ctl);
+                     ovf_main, ctl_main);
   GSJ_LAUNCHED();
   GSJ_HIP(hipEventRecord(c.ev[9], st));
   if (sh.fork) {   // the overflow queue is complete once both chains are done
     GSJ_HIP(hipEventRecord(c.ev[11], sl));
     GSJ_HIP(hipStreamWaitEvent(st, c.ev[11], 0));
   }
-  struct Host { uint32_t counts[16]; OverflowCtl ctl; };   // class sizes [0, 5), edge queue sizes [8, 11)
+  // class sizes [0, 5), edge queue sizes [8, 11); the overflow queues (the list classes', the host-read one)
+  struct Host { uint32_t counts[16]; OverflowCtl ctl[2]; };
   Host* h = c.h_scal.as<Host>(2);
-  GSJ_HIP(hipMemcpyAsync(h->counts, counts, 64, hipMemcpyDeviceToHost, st));
-  GSJ_HIP(hipMemcpyAsync(&h->ctl, ctl, sizeof(OverflowCtl), hipMemcpyDeviceToHost, st));
+  static_assert(sizeof(Host) == 96, "counts and the two queues' controls are copied as one block");
+  GSJ_HIP(hipMemcpyAsync(h, counts, sizeof(Host), hipMemcpyDeviceToHost, st));   // the counts, then both controls
   GSJ_HIP(hipStreamSynchronize(st));
-  const uint32_t nover = h->ctl.count, need = h->ctl.need;
+  const OverflowCtl& qm = h->ctl[side_ovf ? 1 : 0];
+  const uint32_t nover = qm.count, need = qm.need;
+  // pairs refined by an overflow kernel, each once: the list classes' queue, plus the host-read queue less the pairs
+  // the first handed on to it
+  const uint32_t n_over = side_ovf ? h->ctl[0].count - h->ctl[0].pad0 + nover : nover;
   if (stats) {
     float ms_q = 0, ms_t = 0, ms_s = 0, ms_m = 0, ms_x = 0, ms_l = 0;
     GSJ_HIP(hipEventElapsedTime(&ms_q, c.ev[0], c.ev[6]));
@@ -1038,7 +1068,12 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     stats->k_large_ms += ms_l;
     stats->n_tiny += h->counts[CL_TINY]; stats->n_small += h->counts[CL_SMALL]; stats->n_medium += h->counts[CL_MEDIUM];
     stats->n_mixed += h->counts[CL_MIXED]; stats->n_large += h->counts[CL_LARGE];
-    stats->n_over += nover;
+    stats->n_over += n_over;
+    if (side_ovf) {
+      float ms_v = 0;
+      GSJ_HIP(hipEventElapsedTime(&ms_v, c.ev[2], c.ev[16]));
+      stats->k_over_ms += ms_v;
+    }
     stats->n_tiny_edge += h->counts[8 + CL_TINY]; stats->n_small_edge += h->counts[8 + CL_SMALL]; stats->n_medium_edge += h->counts[8 + CL_MEDIUM];
     stats->launches++;
   }
@@ -1050,9 +1085,9 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     const bool few = nover <= (uint64_t)c.cu_count;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(nover, (uint64_t)c.cu_count * (few ? 1 : 8));
     if (few)
-      hipLaunchKernelGGL((k_refine_overflow<512, true>), dim3(blocks), dim3(512), lds, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, nullptr);
+      hipLaunchKernelGGL((k_refine_overflow<512, true>), dim3(blocks), dim3(512), lds, st, P, R, pa, pb, ovf_main, nover, rp, out, (int)need, nullptr, nullptr, nullptr, nullptr);
     else
-      hipLaunchKernelGGL((k_refine_overflow<64, true>), dim3(blocks), dim3(64), lds, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, nullptr);
+      hipLaunchKernelGGL((k_refine_overflow<64, true>), dim3(blocks), dim3(64), lds, st, P, R, pa, pb, ovf_main, nover, rp, out, (int)need, nullptr, nullptr, nullptr, nullptr);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[4], st));
     if (stats) {
@@ -1070,9 +1105,9 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
     unsigned char* slab = c.s_heavy_edges.as<unsigned char>(per * blocks + 64);
     GSJ_HIP(hipEventRecord(c.ev[3], st));
     if (nover <= 2ull * c.cu_count)   // a few pairs: four waves each (2.4M relate: one wave each, 2.5 vs 4.5 ms)
-      hipLaunchKernelGGL((k_refine_overflow<256, false>), dim3((uint32_t)blocks), dim3(256), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
+      hipLaunchKernelGGL((k_refine_overflow<256, false>), dim3((uint32_t)blocks), dim3(256), 0, st, P, R, pa, pb, ovf_main, nover, rp, out, (int)need, slab, nullptr, nullptr, nullptr);
     else
-      hipLaunchKernelGGL((k_refine_overflow<64, false>), dim3((uint32_t)blocks), dim3(64), 0, st, P, R, pa, pb, ovf, nover, rp, out, (int)need, slab);
+      hipLaunchKernelGGL((k_refine_overflow<64, false>), dim3((uint32_t)blocks), dim3(64), 0, st, P, R, pa, pb, ovf_main, nover, rp, out, (int)need, slab, nullptr, nullptr, nullptr);
     GSJ_LAUNCHED();
     GSJ_HIP(hipEventRecord(c.ev[4], st));
     if (stats) {
@@ -1082,7 +1117,7 @@ uint64_t refine_pairs(Context& c, const DPolys& P, const DPolys& R, const uint32
       stats->k_over_ms += ms_o;
     }
   }
-  return nover;
+  return n_over;
 }
 
 namespace {
@@ -1128,10 +1163,11 @@ __global__ void k_class_counts(const uint64_t* __restrict__ key, uint64_t np, ui
   for (int c = 0; c < CL_COUNT; c++) counts[c] = (uint32_t)(first[c + 1] - first[c]);
 }
 
+// the overflow queue's control at ctl, the host-read queue's (ovf_lds 2) right after it, the promotion queue's 64 bytes on
 __global__ void k_ctl_reset(OverflowCtl* ctl, uint32_t* counts) {
-  ctl->count = 0; ctl->need = 0;
-  OverflowCtl* promo = reinterpret_cast<OverflowCtl*>(reinterpret_cast<char*>(ctl) + 64);   // the promotion queue's
-  promo->count = 0; promo->need = 0;
+  ctl[0] = OverflowCtl{0, 0, 0, 0};
+  ctl[1] = OverflowCtl{0, 0, 0, 0};
+  ctl[4] = OverflowCtl{0, 0, 0, 0};
   for (int k = 0; k < 16; k++) counts[k] = 0;   // class sizes [0, 5) and the edge queues' counts and firsts [8, 14)
 }
 }  // namespace

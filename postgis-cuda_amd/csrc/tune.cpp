@@ -23,7 +23,7 @@ const Key kKeys[] = {
     {"screen_occ", &LaunchShape::screen_occ, 4, 5}, {"screen_bpc", &LaunchShape::screen_bpc, 1, 64},
     {"edge_bpc", &LaunchShape::edge_bpc, 1, 64},  {"mixed_bpc", &LaunchShape::mixed_bpc, 1, 64},
     {"large_bpc", &LaunchShape::large_bpc, 1, 64}, {"dw_bpc", &LaunchShape::dw_bpc, 1, 64},
-    {"trav_block", &LaunchShape::trav_block, 64, 256}, {"ovf_lds", &LaunchShape::ovf_lds, 0, 1},
+    {"trav_block", &LaunchShape::trav_block, 64, 256}, {"ovf_lds", &LaunchShape::ovf_lds, 0, 2},
     {"fork", &LaunchShape::fork, 0, 1},           {"mixed", &LaunchShape::mixed, 0, 1},
     {"large_cap", &LaunchShape::large_cap, 0, 96},  {"size_sort", &LaunchShape::size_sort, 0, 4},
     {"dw_size_sort", &LaunchShape::dw_size_sort, 0, 4},

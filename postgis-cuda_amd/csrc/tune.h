@@ -20,7 +20,8 @@ struct LaunchShape {
   uint32_t large_bpc = 16;
   uint32_t dw_bpc = 16;       // distance-join list kernels
   uint32_t trav_block = 256;  // traversal block size (64, 128 or 256 threads)
-  uint32_t ovf_lds = 1;       // overflow pairs with lists of at most 448 edges keep them in LDS (0: global slabs)
+  uint32_t ovf_lds = 1;       // overflow pairs with lists of at most 448 edges keep them in LDS (0: global slabs; 2: as 1, and the
+                              // list classes' overflow pairs within 256 edges are refined on their own chain, no host read first)
   uint32_t fork = 1;          // mixed and large classes on a second stream beside the vertex-list classes (0: in line)
   uint32_t mixed = 1;         // mixed class (0: its pairs are large-class pairs)
   uint32_t large_cap = 96;    // large-class LDS list capacity (0: every large pair goes to the overflow kernel)

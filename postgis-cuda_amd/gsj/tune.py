@@ -28,7 +28,7 @@ BUCKETS = (("small", 0), ("medium", 1 << 15), ("large", 1 << 21))
 # the values tried per key; the first is the built-in shape's
 KEYS = (("tcap", (16, 0, 8, 12)), ("mcap", (64, 0, 48)), ("screen_occ", (5, 4)), ("screen_bpc", (16, 4, 8, 32, 64)),
         ("edge_bpc", (16, 4, 8, 32, 64)), ("mixed_bpc", (32, 8, 16, 64)), ("large_bpc", (16, 4, 8, 32)), ("trav_block", (256, 64, 128)),
-        ("dw_bpc", (16, 4, 8, 32, 64)), ("ovf_lds", (1, 0)), ("fork", (1, 0)), ("mixed", (1, 0)),
+        ("dw_bpc", (16, 4, 8, 32, 64)), ("ovf_lds", (1, 0, 2)), ("fork", (1, 0)), ("mixed", (1, 0)),
         ("large_cap", (96, 0, 64)), ("size_sort", (1, 0, 2, 3)), ("dw_size_sort", (0, 1, 3)))
 DEFAULT = {k: v[0] for k, v in KEYS}
 

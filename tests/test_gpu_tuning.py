@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 EXTREME = ("bucket np>=0 tcap=0 mcap=0 screen_occ=4 screen_bpc=1 edge_bpc=1 mixed_bpc=1 large_bpc=1 dw_bpc=1 trav_block=64 ovf_lds=0 fork=0 mixed=0 large_cap=0 size_sort=0 dw_size_sort=1\n")
 SWITCHING = ("bucket np>=0 tcap=8 mcap=48 screen_occ=4 screen_bpc=2 edge_bpc=64 trav_block=128\n"
-             "bucket np>=2000 tcap=12 mcap=0 screen_occ=5 screen_bpc=64 edge_bpc=3 mixed_bpc=5 large_bpc=7 dw_bpc=2 large_cap=64 mixed=0\n"
+             "bucket np>=2000 tcap=12 mcap=0 screen_occ=5 screen_bpc=64 edge_bpc=3 mixed_bpc=5 large_bpc=7 dw_bpc=2 large_cap=64 mixed=0 ovf_lds=2\n"
              "bucket np>=20000 tcap=16 mcap=64\n")
 
 
@@ -80,6 +80,50 @@ def test_extreme_table_matches_the_oracle(ctx, layer):
         ctx.set_tuning("\n".join(l for l in before.splitlines() if not l.startswith("#")) + "\n")
     OM = O.Layer.polys(M)
     exp = O.join(OM, OM, want_area=True, want_relate=True, upper=True)
+    g = sorted_rows(got.pairs, got.area, got.code)
+    e = sorted_rows(exp["pairs"], exp["area"], exp["code"])
+    assert np.array_equal(g[0], e[0]) and np.array_equal(g[2], e[2])
+    assert np.array_equal(g[1].view(np.uint64), e[1].view(np.uint64)), "areas not bit-identical"
+
+
+def _circles(k=6, seed=5):
+    """k x k overlapping circles of 120 to 600 vertices: large-class pairs whose lists in the overlap box run from a
+    few dozen to several hundred edges (past the 96 of the large class, the 256 of the list classes' own overflow
+    launch and the 448 of the LDS overflow path)"""
+    rng = np.random.default_rng(seed)
+    rings = []
+    for i in range(k * k):
+        n = (120, 200, 300, 420, 600)[i % 5]
+        t = rng.uniform(0, 2 * np.pi) + np.arange(n) * (2 * np.pi / n)
+        r = 1e-3 * rng.uniform(0.9, 1.1)
+        ring = np.c_[-95 + 1.3e-3 * (i % k) + r * np.cos(t), 38 + 1.3e-3 * (i // k) + r * np.sin(t)]
+        rings.append(np.vstack([ring, ring[:1]]))
+    xy = np.concatenate(rings)
+    rvo = np.concatenate([[0], np.cumsum([len(r) for r in rings])])
+    return gsj.HostSet(np.arange(k * k + 1), rvo, xy)
+
+
+@pytest.mark.parametrize("fork", [1, 0])
+def test_list_class_overflow_on_its_own_chain(ctx, fork):
+    """ovf_lds=2: the list classes' overflow pairs are refined right behind the large kernel (queue length read on the
+    device), the pairs whose lists pass 256 edges handed on to the host-read queue. Rows equal the oracle's and the
+    built-in shapes', and every overflow pair is counted once."""
+    M = _circles()
+    L = ctx.polys(M)
+    before = ctx.tuning
+    try:
+        ctx.set_tuning("bucket np>=0\n")
+        base = ctx.join(L, L, want_area=True, want_relate=True)
+        for cap in (96, 0):
+            ctx.set_tuning(f"bucket np>=0 ovf_lds=2 fork={fork} large_cap={cap}\n")
+            got = ctx.join(L, L, want_area=True, want_relate=True)
+            assert same_rows(got, base), cap
+            n = got.stats["n_over"]
+            assert (n == base.stats["n_over"] if cap else n <= got.stats["n_large"]) and n > 0, (cap, got.stats)
+    finally:
+        ctx.set_tuning("\n".join(l for l in before.splitlines() if not l.startswith("#")) + "\n")
+    OM = O.Layer.polys(M)
+    exp = O.join(OM, OM, want_area=True, want_relate=True)
     g = sorted_rows(got.pairs, got.area, got.code)
     e = sorted_rows(exp["pairs"], exp["area"], exp["code"])
     assert np.array_equal(g[0], e[0]) and np.array_equal(g[2], e[2])

@@ -90,6 +90,7 @@ bucket np>=32768 mcap=0 screen_occ=4
     ("bucket np>=0 mcap=65\n", "mcap outside"),
     ("bucket np>=0 screen_occ=3\n", "screen_occ outside [4, 5]"),
     ("bucket np>=0 edge_bpc=0\n", "edge_bpc outside [1, 64]"),
+    ("bucket np>=0 ovf_lds=3\n", "ovf_lds outside [0, 2]"),
     ("bucket np>=0 trav_block=96\n", "trav_block must be 64, 128 or 256"),
     ("bucket np>=0 warp=32\n", "unknown key 'warp'"),
     ("bucket np>=0 tcap=-1\n", "bad value"),
