@@ -677,6 +677,7 @@ def main():
         prep[name] = round((time.perf_counter() - t0) * 1e3, 1)
         if name == "subdivide_device":
             assert np.array_equal(r.xy.view(np.uint64), S.xy.view(np.uint64)) and np.array_equal(r.syn, S.syn), "device subdivision differs"
+        del r   # a prep result is 2.4M parcels of host arrays: freed here, not when a later `r` is bound inside a timed step
     log(f"prep (ms, host arrays in and out): {prep}")
     parse = None if args.no_prep else parse_leg(gsj, ctx, M, args)
     t0 = time.time()
@@ -707,7 +708,7 @@ def main():
         f"compact {st['compact_ms']:.2f} copy {st['copy_ms']:.2f} ms; kernels screen {st['k_screen_ms']:.2f} tiny {st['k_tiny_ms']:.2f} "
         f"small {st['k_small_ms']:.2f} medium {st['k_medium_ms']:.2f} mixed {st['k_mixed_ms']:.2f} large {st['k_large_ms']:.2f} "
         f"over {st['k_over_ms']:.2f} ms (pairs {st['n_tiny']}/{st['n_small']}/{st['n_medium']}/{st['n_mixed']}/{st['n_large']}/{st['n_over']}, "
-        f"edge work {st['n_tiny_edge']}/{st['n_small_edge']}/{st['n_medium_edge']})")
+        f"edge work {st['n_tiny_edge']}/{st['n_small_edge']}/{st['n_medium_edge']}); steps (ms) {[round(t * 1e3, 2) for t in times]}")
 
     # dominant refinement kernel and its algorithmic FLOPs (counting oracle on a sample of the same generator):
     # (name, duration field, pair class of the sample, pairs field, FLOPs counted: edge phase / screen part / all)
