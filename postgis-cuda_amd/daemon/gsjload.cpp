@@ -72,6 +72,22 @@ class Stream {   // buffered reads from a socket
     return true;
   }
   bool u32(uint32_t& v) { return read(&v, 4); }
+  // a whole row already in the buffer ((u16 len, id) x ids, then `tail` bytes): step over it in place; false (nothing
+  // consumed) when it runs past the buffered bytes
+  bool skip_row_buffered(int ids, size_t tail) {
+    size_t p = pos_;
+    for (int k = 0; k < ids; k++) {
+      if (len_ - p < 2) return false;
+      uint16_t l;
+      memcpy(&l, buf_.data() + p, 2);
+      p += 2;
+      if (len_ - p < l) return false;
+      p += l;
+    }
+    if (len_ - p < tail) return false;
+    pos_ = p + tail;
+    return true;
+  }
   bool text(std::string& s) {
     uint32_t l;
     if (!u32(l)) return false;
@@ -89,6 +105,7 @@ class Stream {   // buffered reads from a socket
 // rows of one frame: (u16 len, id) pairs and the op's fixed tail
 bool skip_row(Stream& s, uint32_t op) {
   const int ids = op == GSP_OP_CLUSTER ? 1 : 2;
+  if (s.skip_row_buffered(ids, op == GSP_OP_CLUSTER ? 4 : (op == GSP_OP_RELATE ? 9 : 8))) return true;
   for (int k = 0; k < ids; k++) {
     uint16_t l;
     if (!s.read(&l, 2) || !s.skip(l)) return false;
