@@ -601,9 +601,13 @@ def wire_streams(args, spec, stream, rates):
                     recs = [json.loads(x) for x in r.stdout.splitlines() if x.strip()]
                     summ = recs[-1]
                     cand = sum(int(x["stats"].get("candidates", 0)) for x in recs[:-1] if x.get("stats"))
+                    # the daemon's own account, summed over requests: its time per request (receipt to stats frame) and
+                    # the part of it spent encoding and sending rows
+                    dsum = lambda k: round(sum(float(x["stats"].get(k, 0.0)) for x in recs[:-1] if x.get("stats")), 3)
                     res = {"lanes": lanes, "batch": batch, "wall_s": summ["wall_s"], "refined_pairs_per_s": round(cand / summ["wall_s"], 1),
                            "rows_per_s": round(summ["rows"] / summ["wall_s"], 1), "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"],
-                           "candidates": cand, "rows": summ["rows"], "failed": summ["failed"]}
+                           "candidates": cand, "rows": summ["rows"], "failed": summ["failed"],
+                           "daemon_s": {"total": dsum("total_s"), "encode_send": dsum("host_encode_send_s")}}
                     out.setdefault(f"{rate:g}/s", {})[mode] = res
                     log(f"wire {mode} {rate:g}/s: {res}")
             finally:

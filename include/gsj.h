@@ -257,6 +257,12 @@ typedef struct {
   int need_listed;
   const uint32_t* need_list;
   uint64_t n_need;
+  /* called once on the lane's worker thread when the request has finished (its last callback has returned; before
+     gsj_wait can return for it), whatever its status; may be NULL. Lets a caller that sends a request's rows on from
+     another thread wait for rows and for the end of the request together. No counterpart in the reference, whose
+     daemon writes rows from the thread that runs the request (daemon/gpujoind.cu:189-206). */
+  void (*done_cb)(void* done_user);
+  void* done_user;
 } gsj_request;
 
 typedef struct {

@@ -218,6 +218,7 @@ void gsj_sched::worker(int lane) {
     for (Req* x : group) {
       gsj::layer_release(const_cast<gsj_layer*>(x->r.P));
       gsj::layer_release(const_cast<gsj_layer*>(x->r.R));
+      if (x->r.done_cb) x->r.done_cb(x->r.done_user);   // before `done`: its waiter cannot free it meanwhile
     }
     {
       std::lock_guard<std::mutex> lk(mu);

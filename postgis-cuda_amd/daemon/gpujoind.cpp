@@ -30,6 +30,8 @@
 #include <atomic>
 #include <cerrno>
 #include <climits>
+#include <condition_variable>
+#include <deque>
 #include <functional>
 #include <chrono>
 #include <cmath>
@@ -491,6 +493,9 @@ struct RowChunk {
 // close it with the stats; fail() answers GSP_ERR before the stream opened and a GSP_FRAME_ERR frame after. A row
 // limit (--max-rows) stops a stream that would exceed it: the batch is not sent, the engine is asked to stop, and the
 // op ends the stream with GSP_FRAME_ERR.
+// While a request runs on the scheduler (between begin_async and pump), row batches encoded on the lane's thread are
+// queued and the connection's own thread writes them (pump), so a lane never waits for a client to read: it goes on
+// to its next request while the rows of this one drain.
 class Reply {
  public:
   explicit Reply(Peer& p) : peer_(p) {}
@@ -511,7 +516,8 @@ class Reply {
     if (!peer_.gone) peer_.write_text(0, stats);
   }
 
-  // encode items [0, n) with enc(i, chunk) on the encoding threads and send their rows in item order; returns rows sent
+  // encode items [0, n) with enc(i, chunk) on the encoding threads and send their rows in item order (queued for pump
+  // while a request runs); returns the rows taken
   template <class Enc>
   uint64_t send_rows(size_t n, Enc&& enc) {
     if (n == 0 || stop()) return 0;
@@ -525,7 +531,50 @@ class Reply {
     }
     uint64_t rows = 0;
     for (const RowChunk& c : ch) rows += c.ends.size();
-    if (g_max_rows && sent_ + rows > g_max_rows) { over_limit_ = true; return 0; }
+    if (g_max_rows && taken_ + rows > g_max_rows) { over_limit_ = true; return 0; }
+    taken_ += rows;
+    if (async_) {
+      std::lock_guard<std::mutex> lk(qmu_);
+      queue_.push_back(std::move(ch));
+      qcv_.notify_one();
+    } else {
+      write_chunks(ch);
+    }
+    return rows;
+  }
+
+  // the request about to be submitted streams its rows through the queue until pump returns
+  void begin_async() {
+    async_ = true;
+    done_ = false;
+  }
+  // gsj_request::done_cb: the request has finished, no row batch follows (notified under the lock: pump may return
+  // and the reply go away as soon as it is released)
+  static void request_done(void* self) {
+    Reply& r = *static_cast<Reply*>(self);
+    std::lock_guard<std::mutex> lk(r.qmu_);
+    r.done_ = true;
+    r.qcv_.notify_one();
+  }
+  // on the connection's thread: write queued batches as they come until the request has finished and all are out
+  void pump() {
+    for (;;) {
+      std::vector<RowChunk> ch;
+      {
+        std::unique_lock<std::mutex> lk(qmu_);
+        qcv_.wait(lk, [&] { return !queue_.empty() || done_; });
+        if (queue_.empty()) break;   // done, nothing left
+        ch = std::move(queue_.front());
+        queue_.pop_front();
+      }
+      write_chunks(ch);   // after the client went away: dropped
+    }
+    async_ = false;
+  }
+
+ private:
+  // frames of at most FRAME_BYTES of rows cut from each chunk at row boundaries, each written as its count and a slice
+  void write_chunks(std::vector<RowChunk>& ch) {
     for (RowChunk& c : ch) {
       size_t first = 0;   // first row of the next frame
       while (first < c.ends.size() && !peer_.gone) {
@@ -535,25 +584,34 @@ class Reply {
         uint32_t count = (uint32_t)(last - first);
         iovec v[2] = {{&count, 4}, {&c.bytes[base], c.ends[last - 1] - base}};
         if (!peer_.write_all(v, 2)) break;
-        sent_ += count;
         first = last;
       }
     }
-    return rows;
   }
 
- private:
   Peer& peer_;
   bool streaming_ = false, over_limit_ = false;
-  uint64_t sent_ = 0;
+  uint64_t taken_ = 0;   // rows taken for this stream (the row limit's count), on the thread that calls send_rows
+  bool async_ = false;   // set before the submit and cleared by pump on the connection's thread: the lane reads it
+                         // only while the request runs, between the two
+  std::mutex qmu_;
+  std::condition_variable qcv_;
+  std::deque<std::vector<RowChunk>> queue_;
+  bool done_ = false;
 };
 
 // run one request on the scheduler, wait for it, and return its stats; a stop asked for by the reply (client gone,
 // row limit) is not an error here
 gsj_join_stats run_request(gsj_request& r, Reply& rep, double* run_s = nullptr) {
   r.opts.cancel = &rep.peer().gone;
+  r.done_cb = &Reply::request_done;
+  r.done_user = &rep;
   uint64_t t = 0;
-  check(gsj_submit(g_sched, &r, &t), "submit");
+  rep.begin_async();
+  const int src = gsj_submit(g_sched, &r, &t);
+  if (src != GSJ_OK) Reply::request_done(&rep);   // nothing will run: pump returns at once
+  rep.pump();
+  check(src, "submit");
   gsj_join_stats st{};
   gsj_request_times tm{};
   const int rc = gsj_wait(g_sched, t, &st, &tm);

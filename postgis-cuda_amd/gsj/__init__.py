@@ -67,7 +67,7 @@ class Request(C.Structure):
                 ("report_distance", C.c_int), ("ra", C.c_double), ("rb", C.c_double), ("need_self", C.POINTER(C.c_uint8)),
                 ("rows_cb", ROWS_CB), ("points_cb", POINTS_CB), ("dist_cb", DIST_CB), ("user", C.c_void_p),
                 ("root", C.POINTER(C.c_uint32)), ("nroot", C.c_uint64), ("need_listed", C.c_int),
-                ("need_list", C.POINTER(C.c_uint32)), ("n_need", C.c_uint64)]
+                ("need_list", C.POINTER(C.c_uint32)), ("n_need", C.c_uint64), ("done_cb", C.c_void_p), ("done_user", C.c_void_p)]
 
 
 class RequestTimes(C.Structure):
