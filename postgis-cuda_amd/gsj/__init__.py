@@ -57,6 +57,7 @@ class LayerInfo(C.Structure):
 ROWS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t)
 POINTS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8), C.c_size_t)
 DIST_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.c_size_t)
+DONE_CB = C.CFUNCTYPE(None, C.c_void_p)
 
 REQ_JOIN, REQ_POINTS_IN_POLYGONS, REQ_CLUSTER, REQ_DWITHIN, REQ_POINTS_DWITHIN = 1, 2, 3, 4, 5
 REQ_INTERSECTS_BUFFERED, REQ_POINTS_INTERSECTS_BUFFERED, REQ_CLUSTER_ROWS = 6, 7, 8
@@ -67,7 +68,7 @@ class Request(C.Structure):
                 ("report_distance", C.c_int), ("ra", C.c_double), ("rb", C.c_double), ("need_self", C.POINTER(C.c_uint8)),
                 ("rows_cb", ROWS_CB), ("points_cb", POINTS_CB), ("dist_cb", DIST_CB), ("user", C.c_void_p),
                 ("root", C.POINTER(C.c_uint32)), ("nroot", C.c_uint64), ("need_listed", C.c_int),
-                ("need_list", C.POINTER(C.c_uint32)), ("n_need", C.c_uint64), ("done_cb", C.c_void_p), ("done_user", C.c_void_p)]
+                ("need_list", C.POINTER(C.c_uint32)), ("n_need", C.c_uint64), ("done_cb", DONE_CB), ("done_user", C.c_void_p)]
 
 
 class RequestTimes(C.Structure):
@@ -651,9 +652,10 @@ class Scheduler:
         self._pending = {}
 
     def submit(self, kind: int, P: "Layer", R: "Layer" = None, collect=True, d=0.0, same=False, report_distance=False, ra=0.0, rb=0.0,
-               need_self=None, need_list=None, **kw) -> int:
+               need_self=None, need_list=None, on_done=None, **kw) -> int:
         """need_list (buffered requests, instead of need_self): the parents whose pair with itself is tested, ascending
-        (an empty list: none)."""
+        (an empty list: none). on_done(): called once on the lane's thread when the request has finished, before wait
+        returns for it (gsj_request.done_cb)."""
         import threading
 
         req = Request()
@@ -698,6 +700,9 @@ class Scheduler:
                 return 0
             req.dist_cb = DIST_CB(cb)
             keep.append(req.dist_cb)
+        if on_done is not None:
+            req.done_cb = DONE_CB(lambda user: on_done())
+            keep.append(req.done_cb)
         t = C.c_uint64()
         _check(load().gsj_submit(self.h, C.byref(req), C.byref(t)))
         self._pending[t.value] = (kind, chunks, keep, root, req.opts)

@@ -125,6 +125,34 @@ def test_layers_outlive_their_owner_handle_and_errors_stay_with_the_request(ctx)
         s.close()
 
 
+def test_done_callback_once_per_request_before_wait(ctx):
+    """gsj_request.done_cb: called once per request, on a lane's thread, before wait returns for it; a request that
+    fails on its lane gets it too"""
+    import threading
+    M = ctx.polys(_mesh(30, 7))
+    N = ctx.polys(_mesh(10, 6), build_bvh=False)
+    calls, lock, main = {}, threading.Lock(), threading.get_ident()
+
+    def mark(i):
+        def f():
+            with lock:
+                calls.setdefault(i, []).append(threading.get_ident())
+        return f
+
+    s = gsj.Scheduler(ctx, lanes=2)
+    try:
+        good = [s.submit(gsj.REQ_JOIN, M, M, upper=True, on_done=mark(i)) for i in range(6)]
+        bad = s.submit(gsj.REQ_JOIN, N, N, on_done=mark("bad"))
+        for i, t in enumerate(good):
+            r = s.wait(t)
+            assert len(calls.get(i, [])) == 1 and calls[i][0] != main and len(r.pairs) > 0, (i, calls.get(i))
+        with pytest.raises(gsj.GsjError):
+            s.wait(bad)
+        assert len(calls["bad"]) == 1
+    finally:
+        s.close()
+
+
 def _rows_in_order(a, b):
     """identical rows in identical order (batching keeps every request's own row order)"""
     assert np.array_equal(a.pairs, b.pairs)
